@@ -1,0 +1,633 @@
+// Fused MUSCL-Hancock + HLLC + conservative-update sweep kernels, ghost
+// fill and CFL reduction for all boxes of a level.
+//
+// Reference path replaced (per box, per direction d):
+//   compute_MUSCL_hancock_flux<d>  Source/Fluxes/MUSCL_Hancock.cpp:15-52
+//   conservative update            Source/AmrLevelAdv_advance.cpp:330-341
+//   fill_consv_boundary            Source/BoundaryConditions/BCs.cpp:429-442
+//   max_wave_speed / estTimeStep   Source/System/Euler/Euler.cpp:46-74,
+//                                  Source/AmrLevelAdv.cpp:395-477, 537-619
+//
+// Each sweep is ONE persistent launch over a device-resident box list.  The
+// level's work is cut into items; every wave owns a contiguous range of
+// items (a range stays inside one XCD's L2 because consecutive ranges are
+// handed to blocks of the same XCD).
+//
+//  * x sweep (unit stride): an item is 64 consecutive positions of one
+//    box plane, flattened over (valid rows) x (row incl. ghosts).  Lane t
+//    holds one position; the neighbour values the stencil needs move between
+//    lanes with __shfl_up/__shfl_down.  60 of 64 lanes produce an update.
+//  * y/z sweeps (strided): an item is 64 (transverse cell, segment) pairs;
+//    lane t walks its column along the sweep direction through K cells,
+//    keeping the 3-cell window, the previous half-stepped state and the
+//    previous face flux in registers (K+2 reconstructions per K updates).
+// No temporaries leave registers: per cell the sweep reads U once and
+// writes U once (plus halo re-reads that hit L2).
+#include <hip/hip_runtime.h>
+
+#include "bx_internal.h"
+#include "bx_physics.h"
+#include "bx_kernels.h"
+
+namespace bx {
+
+__device__ __forceinline__ int find_box(const long long *__restrict__ begin,
+                                        int nbox, long long item)
+{
+    int lo = 0, hi = nbox; // invariant: begin[lo] <= item < begin[hi]
+    while (hi - lo > 1)
+    {
+        const int mid = (lo + hi) >> 1;
+        if (begin[mid] <= item)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ unsigned long long umax64(unsigned long long a,
+                                                     unsigned long long b)
+{
+    return a > b ? a : b;
+}
+
+// Contiguous item range of this wave, XCD-aware: blocks are dealt to the 8
+// XCDs round-robin, so block b runs on XCD b%8; renumber so that XCD x owns
+// a contiguous slice of the item space.
+__device__ __forceinline__ void wave_item_range(long long nitems,
+                                                long long &first,
+                                                long long &last)
+{
+    const int  wpb    = blockDim.x >> 6;
+    const int  nblk   = gridDim.x;
+    int        blk    = blockIdx.x;
+    if ((nblk & 7) == 0)
+        blk = (blk & 7) * (nblk >> 3) + (blk >> 3);
+    const long long nw   = (long long)nblk * wpb;
+    const long long w    = (long long)blk * wpb + (threadIdx.x >> 6);
+    const long long per  = (nitems + nw - 1) / nw;
+    first                = w * per;
+    last                 = first + per;
+    if (last > nitems)
+        last = nitems;
+}
+
+__device__ __forceinline__ void wave_max_to_global(unsigned long long v,
+                                                   unsigned long long *dst)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const unsigned long long o = __shfl_xor(v, off);
+        v                          = umax64(v, o);
+    }
+    if ((threadIdx.x & 63) == 0 && v != 0ULL)
+        atomicMax(dst, v);
+}
+
+// ---------------------------------------------------------------- x sweep
+template <int D, int LIM, int LV, bool FIRST_ORDER>
+__global__ __launch_bounds__(256) void k_sweep_x(
+    const double *__restrict__ Uin, double *__restrict__ Uout,
+    const DevBox *__restrict__ boxes, const long long *__restrict__ ibeg,
+    int nbox, long long nitems, int ng, const double *__restrict__ d_dt,
+    double dx, EulerParm pm, LimTable lt, unsigned long long *d_maxbits)
+{
+    constexpr int NC   = D + 2;
+    constexpr int STEP = 60; // updated positions per 64-lane item
+    const int     lane = threadIdx.x & 63;
+    const double  dt   = *d_dt;
+    const double  hdtdx = 0.5 * dt / dx;
+    const double  dtdx  = dt / dx;
+    const int     ngy   = (D >= 2) ? ng : 0;
+    const int     ngz   = (D >= 3) ? ng : 0;
+
+    long long first, last;
+    wave_item_range(nitems, first, last);
+    unsigned long long vmax = 0ULL;
+    int                b    = -1;
+    long long          bend = -1;
+    DevBox             bx;
+    int nx = 0, ny = 0, nxd = 0, cpp = 0;
+    long long P = 0;
+    for (long long item = first; item < last; ++item)
+    {
+        if (item >= bend)
+        {
+            b    = find_box(ibeg, nbox, item);
+            bend = ibeg[b + 1];
+            bx   = boxes[b];
+            nx   = bx.hi[0] - bx.lo[0] + 1;
+            ny   = (D >= 2) ? bx.hi[1] - bx.lo[1] + 1 : 1;
+            nxd  = nx + 2 * ng;
+            P    = (long long)ny * nxd;
+            cpp  = (int)((P + STEP - 1) / STEP);
+        }
+        const unsigned local = (unsigned)(item - ibeg[b]);
+        const int      kpl   = (int)(local / (unsigned)cpp);
+        const int      chunk = (int)(local - (unsigned)kpl * (unsigned)cpp);
+        const long long pbase = bx.base + (long long)(kpl + ngz) * bx.kstride
+                                + (long long)ngy * bx.jstride;
+        const long long p  = (long long)chunk * STEP - 2 + lane;
+        const long long pc = p < 0 ? 0 : (p >= P ? P - 1 : p);
+
+        double U[NC];
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            U[n] = Uin[pbase + pc + n * bx.nstride];
+        double Um[NC], F[NC];
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            Um[n] = __shfl_up(U[n], 1);
+        if (FIRST_ORDER)
+        {
+            hllc_flux<D, 0>(Um, U, F, pm);
+        }
+        else
+        {
+            double Up[NC], hL[NC], hR[NC], hRm[NC];
+#pragma unroll
+            for (int n = 0; n < NC; ++n)
+                Up[n] = __shfl_down(U[n], 1);
+            reconstruct_half_step<D, 0, LIM, LV>(Um, U, Up, hL, hR, hdtdx, pm,
+                                                 lt);
+#pragma unroll
+            for (int n = 0; n < NC; ++n)
+                hRm[n] = __shfl_up(hR[n], 1);
+            hllc_flux<D, 0>(hRm, hL, F, pm);
+        }
+        double Fp[NC];
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            Fp[n] = __shfl_down(F[n], 1);
+
+        const int  col = (int)((unsigned)pc % (unsigned)nxd);
+        const bool upd = lane >= 2 && lane < 62 && p < P && col >= ng
+                         && col < ng + nx;
+        if (upd)
+        {
+            double Un[NC];
+#pragma unroll
+            for (int n = 0; n < NC; ++n)
+            {
+                Un[n] = U[n] - dtdx * (Fp[n] - F[n]);
+                Uout[pbase + p + n * bx.nstride] = Un[n];
+            }
+            if (d_maxbits)
+            {
+                const double s = cell_wave_speed<D>(Un, pm);
+                vmax = umax64(vmax, __double_as_longlong(s));
+            }
+        }
+    }
+    if (d_maxbits)
+        wave_max_to_global(vmax, d_maxbits);
+}
+
+// -------------------------------------------------------------- y/z sweep
+template <int D, int DIR, int LIM, int LV, bool FIRST_ORDER>
+__global__ __launch_bounds__(256) void k_sweep_t(
+    const double *__restrict__ Uin, double *__restrict__ Uout,
+    const DevBox *__restrict__ boxes, const long long *__restrict__ ibeg,
+    int nbox, long long nitems, int ng, int K, const double *__restrict__ d_dt,
+    double dx, EulerParm pm, LimTable lt, unsigned long long *d_maxbits)
+{
+    constexpr int NC    = D + 2;
+    const int     lane  = threadIdx.x & 63;
+    const double  dt    = *d_dt;
+    const double  hdtdx = 0.5 * dt / dx;
+    const double  dtdx  = dt / dx;
+    const int     ngy   = (D >= 2) ? ng : 0;
+    const int     ngz   = (D >= 3) ? ng : 0;
+
+    long long first, last;
+    wave_item_range(nitems, first, last);
+    unsigned long long vmax = 0ULL;
+    int                b    = -1;
+    long long          bend = -1;
+    DevBox             bx;
+    int na = 0, nb = 0, L = 0, nseg = 0;
+    long long Q = 0, sd = 0;
+    for (long long item = first; item < last; ++item)
+    {
+        if (item >= bend)
+        {
+            b    = find_box(ibeg, nbox, item);
+            bend = ibeg[b + 1];
+            bx   = boxes[b];
+            na   = bx.hi[0] - bx.lo[0] + 1;
+            if (DIR == 1)
+            {
+                nb = (D >= 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
+                L  = bx.hi[1] - bx.lo[1] + 1;
+                sd = bx.jstride;
+            }
+            else
+            {
+                nb = bx.hi[1] - bx.lo[1] + 1;
+                L  = bx.hi[2] - bx.lo[2] + 1;
+                sd = bx.kstride;
+            }
+            nseg = (L + K - 1) / K;
+            Q    = (long long)na * nb * nseg;
+        }
+        const long long q = (item - ibeg[b]) * 64 + lane;
+        if (q >= Q)
+            continue;
+        const unsigned  qu  = (unsigned)q;
+        const int       a   = (int)(qu % (unsigned)na);
+        const unsigned  q2  = qu / (unsigned)na;
+        const int       bb  = (int)(q2 % (unsigned)nb);
+        const int       seg = (int)(q2 / (unsigned)nb);
+        long long       col;
+        if (DIR == 1)
+            col = bx.base + (a + ng) + (long long)(bb + ngz) * bx.kstride
+                  + (long long)ngy * bx.jstride;
+        else
+            col = bx.base + (a + ng) + (long long)(bb + ngy) * bx.jstride
+                  + (long long)ngz * bx.kstride;
+        const int s0 = seg * K;
+        const int s1 = (s0 + K < L) ? s0 + K : L; // cells [s0, s1)
+
+        auto load = [&](int s, double (&V)[NC]) {
+            const long long o = col + (long long)s * sd;
+#pragma unroll
+            for (int n = 0; n < NC; ++n)
+                V[n] = Uin[o + n * bx.nstride];
+        };
+        double Um1[NC], U0[NC], hRprev[NC], Fprev[NC];
+        load(s0 - 1, Um1);
+        load(s0, U0);
+        if (!FIRST_ORDER)
+        {
+            double Um2[NC], hLd[NC];
+            load(s0 - 2, Um2);
+            reconstruct_half_step<D, DIR, LIM, LV>(Um2, Um1, U0, hLd, hRprev,
+                                                   hdtdx, pm, lt);
+        }
+        for (int s = s0; s <= s1; ++s)
+        {
+            double F[NC];
+            double Up[NC];
+            if (FIRST_ORDER)
+            {
+                hllc_flux<D, DIR>(Um1, U0, F, pm);
+            }
+            else
+            {
+                double hL0[NC], hR0[NC];
+                load(s + 1, Up);
+                reconstruct_half_step<D, DIR, LIM, LV>(Um1, U0, Up, hL0, hR0,
+                                                       hdtdx, pm, lt);
+                hllc_flux<D, DIR>(hRprev, hL0, F, pm);
+#pragma unroll
+                for (int n = 0; n < NC; ++n)
+                    hRprev[n] = hR0[n];
+            }
+            if (s > s0)
+            {
+                // update cell s-1 from faces s-1 (Fprev) and s (F)
+                const long long o = col + (long long)(s - 1) * sd;
+                double          Un[NC];
+#pragma unroll
+                for (int n = 0; n < NC; ++n)
+                {
+                    Un[n] = Um1[n] - dtdx * (F[n] - Fprev[n]);
+                    Uout[o + n * bx.nstride] = Un[n];
+                }
+                if (d_maxbits)
+                {
+                    const double sp = cell_wave_speed<D>(Un, pm);
+                    vmax = umax64(vmax, __double_as_longlong(sp));
+                }
+            }
+#pragma unroll
+            for (int n = 0; n < NC; ++n)
+            {
+                Fprev[n] = F[n];
+                Um1[n]   = U0[n];
+            }
+            if (s < s1)
+            {
+                if (FIRST_ORDER)
+                    load(s + 1, U0);
+                else
+#pragma unroll
+                    for (int n = 0; n < NC; ++n)
+                        U0[n] = Up[n];
+            }
+        }
+    }
+    if (d_maxbits)
+        wave_max_to_global(vmax, d_maxbits);
+}
+
+// ------------------------------------------------------------- ghost fill
+__global__ __launch_bounds__(256) void k_fill_ghosts(
+    double *__restrict__ U, const DevBox *__restrict__ boxes,
+    const GhostSeg *__restrict__ segs, int nseg, long long ncells, int ng,
+    int D, int NC)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const int       ngy    = (D >= 2) ? ng : 0;
+    const int       ngz    = (D >= 3) ? ng : 0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         c < ncells; c += stride)
+    {
+        int lo = 0, hi = nseg;
+        while (hi - lo > 1)
+        {
+            const int mid = (lo + hi) >> 1;
+            if (segs[mid].cell_begin <= c)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const GhostSeg &g     = segs[lo];
+        long long       local = c - g.cell_begin;
+        const int       ex    = g.hi[0] - g.lo[0] + 1;
+        const int       ey    = g.hi[1] - g.lo[1] + 1;
+        const int       x     = g.lo[0] + (int)(local % ex);
+        local /= ex;
+        const int y = g.lo[1] + (int)(local % ey);
+        const int z = g.lo[2] + (int)(local / ey);
+        const int sx = g.a[0] * x + g.b[0];
+        const int sy = g.a[1] * y + g.b[1];
+        const int sz = g.a[2] * z + g.b[2];
+        const DevBox &db = boxes[g.dst];
+        const DevBox &sb = boxes[g.src];
+        const long long doff = db.base + (x - db.lo[0] + ng)
+                               + (long long)(y - db.lo[1] + ngy) * db.jstride
+                               + (long long)(z - db.lo[2] + ngz) * db.kstride;
+        const long long soff = sb.base + (sx - sb.lo[0] + ng)
+                               + (long long)(sy - sb.lo[1] + ngy) * sb.jstride
+                               + (long long)(sz - sb.lo[2] + ngz) * sb.kstride;
+        for (int n = 0; n < NC; ++n)
+        {
+            double v = U[soff + n * sb.nstride];
+            if ((g.flip_mask >> n) & 1)
+                v = -v;
+            U[doff + n * db.nstride] = v;
+        }
+    }
+}
+
+// ------------------------------------------------------ max wave speed
+template <int D>
+__global__ __launch_bounds__(256) void k_max_wave_speed(
+    const double *__restrict__ U, const DevBox *__restrict__ boxes,
+    const long long *__restrict__ cbeg, int nbox, long long ncells, int ng,
+    EulerParm pm, unsigned long long *d_maxbits)
+{
+    constexpr int      NC     = D + 2;
+    const long long    stride = (long long)gridDim.x * blockDim.x;
+    const int          ngy    = (D >= 2) ? ng : 0;
+    const int          ngz    = (D >= 3) ? ng : 0;
+    unsigned long long vmax   = 0ULL;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+         c < ncells; c += stride)
+    {
+        const int     b     = find_box(cbeg, nbox, c);
+        const DevBox &bx    = boxes[b];
+        long long     local = c - cbeg[b];
+        const int     nx    = bx.hi[0] - bx.lo[0] + 1;
+        const int     ny    = (D >= 2) ? bx.hi[1] - bx.lo[1] + 1 : 1;
+        const int     i     = (int)(local % nx);
+        local /= nx;
+        const int       j = (int)(local % ny);
+        const int       k = (int)(local / ny);
+        const long long o = bx.base + (i + ng) + (long long)(j + ngy) * bx.jstride
+                            + (long long)(k + ngz) * bx.kstride;
+        double V[NC];
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            V[n] = U[o + n * bx.nstride];
+        vmax = umax64(vmax, __double_as_longlong(cell_wave_speed<D>(V, pm)));
+    }
+    wave_max_to_global(vmax, d_maxbits);
+}
+
+// ----------------------------------------------------------- dt control
+// estTimeStep (AmrLevelAdv.cpp:395-477) for the explicit methods, followed by
+// the stop-time clamp of computeNewDt (AmrLevelAdv.cpp:589-618).  The
+// accumulator is cleared for the next fused reduction.
+__global__ void k_compute_dt(unsigned long long *d_maxbits, double *d_dt,
+                             const double *d_time, int D, double dx0,
+                             double dx1, double dx2, double cfl,
+                             double stop_time, double fixed_dt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    double dt_0;
+    if (fixed_dt > 0)
+    {
+        dt_0 = fixed_dt;
+    }
+    else
+    {
+        const double max_speed = __longlong_as_double(*d_maxbits);
+        const double dxs[3]    = { dx0, dx1, dx2 };
+        double       dt_est    = 1.0e20;
+        for (int d = 0; d < D; ++d)
+        {
+            const double c = dxs[d] / max_speed;
+            dt_est         = (c < dt_est) ? c : dt_est;
+        }
+        dt_est *= cfl;
+        dt_0 = 1.0e+100;
+        dt_0 = (dt_est < dt_0) ? dt_est : dt_0;
+    }
+    const double eps      = 0.001 * dt_0;
+    const double cur_time = *d_time;
+    if (stop_time >= 0.0)
+    {
+        if ((cur_time + dt_0) > (stop_time - eps))
+            dt_0 = stop_time - cur_time;
+    }
+    *d_dt      = dt_0;
+    *d_maxbits = 0ULL;
+}
+
+__global__ void k_advance_time(double *d_time, const double *d_dt,
+                               long long *d_step)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    *d_time = *d_time + *d_dt;
+    *d_step = *d_step + 1;
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *p = v;
+}
+
+// ------------------------------------------------------------ launchers
+template <int D, int LIM, int LV, bool FO>
+static hipError_t launch_x(const SweepLaunch &L)
+{
+    hipLaunchKernelGGL((k_sweep_x<D, LIM, LV, FO>), dim3(L.grid),
+                       dim3(L.block), 0, L.stream, L.Uin, L.Uout, L.boxes,
+                       L.item_begin, L.nbox, L.nitems, L.ng, L.d_dt, L.dx,
+                       L.pm, L.lt, L.d_maxbits);
+    return hipGetLastError();
+}
+
+template <int D, int DIR, int LIM, int LV, bool FO>
+static hipError_t launch_t(const SweepLaunch &L)
+{
+    hipLaunchKernelGGL((k_sweep_t<D, DIR, LIM, LV, FO>), dim3(L.grid),
+                       dim3(L.block), 0, L.stream, L.Uin, L.Uout, L.boxes,
+                       L.item_begin, L.nbox, L.nitems, L.ng, L.K, L.d_dt,
+                       L.dx, L.pm, L.lt, L.d_maxbits);
+    return hipGetLastError();
+}
+
+template <int D, int LIM, int LV, bool FO>
+static hipError_t dispatch_dir(int dir, const SweepLaunch &L)
+{
+    if (dir == 0)
+        return launch_x<D, LIM, LV, FO>(L);
+    if constexpr (D >= 2)
+        if (dir == 1)
+            return launch_t<D, 1, LIM, LV, FO>(L);
+    if constexpr (D >= 3)
+        if (dir == 2)
+            return launch_t<D, 2, LIM, LV, FO>(L);
+    return hipErrorInvalidValue;
+}
+
+template <int D, int LIM, int LV>
+static hipError_t dispatch_fo(int dir, bool first_order, const SweepLaunch &L)
+{
+    if (first_order)
+        return dispatch_dir<D, LIM, LV, true>(dir, L);
+    return dispatch_dir<D, LIM, LV, false>(dir, L);
+}
+
+template <int D, int LIM>
+static hipError_t dispatch_lv(int dir, bool fo, int lv, const SweepLaunch &L)
+{
+    switch (lv)
+    {
+    case LV_SHARED: return dispatch_fo<D, LIM, LV_SHARED>(dir, fo, L);
+    case LV_OWN: return dispatch_fo<D, LIM, LV_OWN>(dir, fo, L);
+    default: return dispatch_fo<D, LIM, LV_TABLE>(dir, fo, L);
+    }
+}
+
+template <int D>
+static hipError_t dispatch_lim(int dir, bool fo, int lim, int lv,
+                               const SweepLaunch &L)
+{
+    if (fo) // first order: no reconstruction, limiter irrelevant
+        return dispatch_fo<D, LIM_MINBEE, LV_SHARED>(dir, true, L);
+    switch (lim)
+    {
+    case LIM_MINBEE: return dispatch_lv<D, LIM_MINBEE>(dir, fo, lv, L);
+    case LIM_VAN_LEER: return dispatch_lv<D, LIM_VAN_LEER>(dir, fo, lv, L);
+    case LIM_VAN_ALBADA: return dispatch_lv<D, LIM_VAN_ALBADA>(dir, fo, lv, L);
+    case LIM_SUPERBEE: return dispatch_lv<D, LIM_SUPERBEE>(dir, fo, lv, L);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_sweep(int D, int dir, bool first_order, int lim, int lv,
+                        const SweepLaunch &L)
+{
+    switch (D)
+    {
+    case 1: return dispatch_lim<1>(dir, first_order, lim, lv, L);
+    case 2: return dispatch_lim<2>(dir, first_order, lim, lv, L);
+    case 3: return dispatch_lim<3>(dir, first_order, lim, lv, L);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+const void *sweep_kernel_ptr(int D, int dir)
+{
+    // representative instantiation for occupancy queries
+    if (dir == 0)
+    {
+        if (D == 1)
+            return (const void *)&k_sweep_x<1, LIM_MINBEE, LV_SHARED, false>;
+        if (D == 2)
+            return (const void *)&k_sweep_x<2, LIM_MINBEE, LV_SHARED, false>;
+        return (const void *)&k_sweep_x<3, LIM_MINBEE, LV_SHARED, false>;
+    }
+    if (D == 2)
+        return (const void *)&k_sweep_t<2, 1, LIM_MINBEE, LV_SHARED, false>;
+    if (dir == 1)
+        return (const void *)&k_sweep_t<3, 1, LIM_MINBEE, LV_SHARED, false>;
+    return (const void *)&k_sweep_t<3, 2, LIM_MINBEE, LV_SHARED, false>;
+}
+
+hipError_t launch_fill_ghosts(double *U, const DevBox *boxes,
+                              const GhostSeg *segs, int nseg, long long ncells,
+                              int ng, int D, int NC, int grid,
+                              hipStream_t stream)
+{
+    if (ncells == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fill_ghosts, dim3(grid), dim3(256), 0, stream, U,
+                       boxes, segs, nseg, ncells, ng, D, NC);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_wave_speed(int D, const double *U, const DevBox *boxes,
+                                 const long long *cbeg, int nbox,
+                                 long long ncells, int ng, EulerParm pm,
+                                 unsigned long long *d_maxbits, int grid,
+                                 hipStream_t stream)
+{
+    switch (D)
+    {
+    case 1:
+        hipLaunchKernelGGL(k_max_wave_speed<1>, dim3(grid), dim3(256), 0,
+                           stream, U, boxes, cbeg, nbox, ncells, ng, pm,
+                           d_maxbits);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_max_wave_speed<2>, dim3(grid), dim3(256), 0,
+                           stream, U, boxes, cbeg, nbox, ncells, ng, pm,
+                           d_maxbits);
+        break;
+    case 3:
+        hipLaunchKernelGGL(k_max_wave_speed<3>, dim3(grid), dim3(256), 0,
+                           stream, U, boxes, cbeg, nbox, ncells, ng, pm,
+                           d_maxbits);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_compute_dt(unsigned long long *d_maxbits, double *d_dt,
+                             const double *d_time, int D, const double dx[3],
+                             double cfl, double stop_time, double fixed_dt,
+                             hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_compute_dt, dim3(1), dim3(64), 0, stream, d_maxbits,
+                       d_dt, d_time, D, dx[0], dx[1], dx[2], cfl, stop_time,
+                       fixed_dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance_time(double *d_time, const double *d_dt,
+                               long long *d_step, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, stream, d_time,
+                       d_dt, d_step);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_u64(unsigned long long *p, unsigned long long v,
+                          hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, stream, p, v);
+    return hipGetLastError();
+}
+
+} // namespace bx

@@ -1,0 +1,56 @@
+// Host-side launch interface of the explicit-path kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bx_internal.h"
+#include "bx_physics.h"
+
+namespace bx {
+
+struct SweepLaunch
+{
+    const double       *Uin;
+    double             *Uout;
+    const DevBox       *boxes;
+    const long long    *item_begin;
+    int                 nbox;
+    long long           nitems;
+    int                 ng;
+    int                 K; // cells per lane segment (y/z sweeps)
+    const double       *d_dt;
+    double              dx;
+    EulerParm           pm;
+    LimTable            lt;
+    unsigned long long *d_maxbits; // nullable: fused CFL reduction target
+    int                 grid;
+    int                 block;
+    hipStream_t         stream;
+};
+
+hipError_t  launch_sweep(int D, int dir, bool first_order, int lim, int lv,
+                         const SweepLaunch &L);
+const void *sweep_kernel_ptr(int D, int dir);
+
+hipError_t launch_fill_ghosts(double *U, const DevBox *boxes,
+                              const GhostSeg *segs, int nseg, long long ncells,
+                              int ng, int D, int NC, int grid,
+                              hipStream_t stream);
+
+hipError_t launch_max_wave_speed(int D, const double *U, const DevBox *boxes,
+                                 const long long *cbeg, int nbox,
+                                 long long ncells, int ng, EulerParm pm,
+                                 unsigned long long *d_maxbits, int grid,
+                                 hipStream_t stream);
+
+hipError_t launch_compute_dt(unsigned long long *d_maxbits, double *d_dt,
+                             const double *d_time, int D, const double dx[3],
+                             double cfl, double stop_time, double fixed_dt,
+                             hipStream_t stream);
+
+hipError_t launch_advance_time(double *d_time, const double *d_dt,
+                               long long *d_step, hipStream_t stream);
+
+hipError_t launch_set_u64(unsigned long long *p, unsigned long long v,
+                          hipStream_t stream);
+
+} // namespace bx

@@ -1,0 +1,331 @@
+// Per-cell Euler physics for the fused MUSCL-Hancock / HLLC sweep kernels.
+//
+// Every expression keeps the operand order of the reference so that, with
+// floating-point contraction disabled (-ffp-contract=off), each result is
+// bit-identical to the CPU evaluation:
+//   primv_from_consv   Source/System/Euler/Euler.H:167-198
+//   euler_flux         Source/System/Euler/Euler.H:211-244 (= Euler.cpp:18-43)
+//   limiters           Source/Fluxes/Limiters.H:42-73
+//   reconstruction     Source/Fluxes/Flux_Helpers.cpp:92-148
+//   Toro wave speeds   Source/Fluxes/HLLC.H:213-309
+//   HLLC star state    Source/Fluxes/HLLC.H:311-337
+//   HLLC flux          Source/Fluxes/HLLC.cpp:90-172
+//   max wave speed     Source/System/Euler/Euler.cpp:53-68
+// State vectors are conserved variables U = (rho, m_0..m_{D-1}, E), NC = D+2.
+#pragma once
+#include "bx_pow.h"
+
+#define BX_DEV __device__ __forceinline__
+
+namespace bx {
+
+enum LimiterKind
+{
+    LIM_MINBEE     = 0,
+    LIM_VAN_LEER   = 1,
+    LIM_VAN_ALBADA = 2,
+    LIM_SUPERBEE   = 3
+};
+
+struct EulerParm
+{
+    double gamma;
+    double eps;
+};
+
+// Q = (rho, u_0..u_{D-1}, p)
+template <int D>
+BX_DEV void primv_from_consv(const double (&U)[D + 2], double (&Q)[D + 2],
+                             const EulerParm &pm)
+{
+    Q[0] = U[0];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        Q[1 + d] = U[1 + d] / U[0];
+    double u_sq = Q[1] * Q[1];
+#pragma unroll
+    for (int d = 1; d < D; ++d)
+        u_sq = u_sq + Q[1 + d] * Q[1 + d];
+    Q[1 + D] = (pm.gamma - 1) * (U[1 + D] - 0.5 * pm.eps * U[0] * u_sq);
+}
+
+template <int D, int dir>
+BX_DEV void euler_flux(const double (&U)[D + 2], const double (&Q)[D + 2],
+                       double (&F)[D + 2], const EulerParm &pm)
+{
+    F[0] = U[1 + dir];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        F[1 + d] = U[1 + dir] * Q[1 + d];
+    F[1 + dir] = F[1 + dir] + Q[1 + D] / pm.eps;
+    F[1 + D]   = (U[1 + D] + Q[1 + D]) * Q[1 + dir];
+}
+
+template <int LIM>
+BX_DEV double slope_limit(double r)
+{
+    if (LIM == LIM_MINBEE)
+        return r <= 0 ? 0. : (r > 1 ? fmin(2. / (1 + r), 1.) : r);
+    else if (LIM == LIM_VAN_LEER)
+        return r <= 0 ? 0. : fmin(2. * r / (1 + r), 2. / (1 + r));
+    else if (LIM == LIM_VAN_ALBADA)
+        return r <= 0 ? 0. : fmin(r * (1 + r) / (1 + r * r), 2. / (1 + r));
+    else
+        return (r <= 0) ? 0.
+                        : ((0 < r && r <= 0.5)
+                               ? 2 * r
+                               : ((0.5 < r && r <= 1)
+                                      ? 1.
+                                      : fmin(r, fmin(2. / (1 + r), 2.))));
+}
+
+template <int LIM>
+BX_DEV double limiter_value(double vl, double vc, double vr)
+{
+    const double numerator = vc - vl;
+    const double denom     = vr - vc;
+    if (numerator == 0. && denom == 0.)
+        return slope_limit<LIM>(1.);
+    else if (denom == 0)
+        return 0.;
+    return slope_limit<LIM>(numerator / denom);
+}
+
+// Limiting-variable selection (CalcLimitingVal.H:9-22).  The host folds the
+// runtime (limiter_indices, limit_on_primv) table into one of three modes:
+//   LV_SHARED: every component limited on the same variable (default: energy)
+//   LV_OWN:    component n limited on conserved variable n
+//   LV_TABLE:  arbitrary table (evaluated with selects)
+enum LimVarMode
+{
+    LV_SHARED = 0,
+    LV_OWN    = 1,
+    LV_TABLE  = 2
+};
+
+struct LimTable
+{
+    int idx[5];
+    int prim[5];
+};
+
+template <int D>
+BX_DEV double pick_var(const double (&U)[D + 2], const double (&Q)[D + 2],
+                       int idx, int prim)
+{
+    double v = 0.;
+#pragma unroll
+    for (int m = 0; m < D + 2; ++m)
+        if (idx == m)
+            v = prim ? Q[m] : U[m];
+    return v;
+}
+
+// Reconstruct + half time step at one cell (Flux_Helpers.cpp:92-148).
+// Um/U0/Up: conserved states at c-1, c, c+1 along dir.  Outputs the
+// half-stepped low-side (hL) and high-side (hR) states of cell c.
+template <int D, int dir, int LIM, int LV>
+BX_DEV void reconstruct_half_step(const double (&Um)[D + 2],
+                                  const double (&U0)[D + 2],
+                                  const double (&Up)[D + 2],
+                                  double (&hL)[D + 2], double (&hR)[D + 2],
+                                  double hdtdx, const EulerParm &pm,
+                                  const LimTable &lt)
+{
+    constexpr int NC = D + 2;
+    double        phi[NC];
+    if (LV == LV_SHARED)
+    {
+        double vl, vc, vr;
+        if (lt.prim[0])
+        {
+            double Qm[NC], Q0[NC], Qp[NC];
+            primv_from_consv<D>(Um, Qm, pm);
+            primv_from_consv<D>(U0, Q0, pm);
+            primv_from_consv<D>(Up, Qp, pm);
+            vl = pick_var<D>(Um, Qm, lt.idx[0], 1);
+            vc = pick_var<D>(U0, Q0, lt.idx[0], 1);
+            vr = pick_var<D>(Up, Qp, lt.idx[0], 1);
+        }
+        else
+        {
+            vl = pick_var<D>(Um, Um, lt.idx[0], 0);
+            vc = pick_var<D>(U0, U0, lt.idx[0], 0);
+            vr = pick_var<D>(Up, Up, lt.idx[0], 0);
+        }
+        const double p = limiter_value<LIM>(vl, vc, vr);
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            phi[n] = p;
+    }
+    else if (LV == LV_OWN)
+    {
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            phi[n] = limiter_value<LIM>(Um[n], U0[n], Up[n]);
+    }
+    else
+    {
+        double Qm[NC], Q0[NC], Qp[NC];
+        primv_from_consv<D>(Um, Qm, pm);
+        primv_from_consv<D>(U0, Q0, pm);
+        primv_from_consv<D>(Up, Qp, pm);
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            phi[n] = limiter_value<LIM>(pick_var<D>(Um, Qm, lt.idx[n], lt.prim[n]),
+                                        pick_var<D>(U0, Q0, lt.idx[n], lt.prim[n]),
+                                        pick_var<D>(Up, Qp, lt.idx[n], lt.prim[n]));
+    }
+
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+    {
+        const double delta_left  = U0[n] - Um[n];
+        const double delta_right = Up[n] - U0[n];
+        const double delta       = 0.5 * delta_left + 0.5 * delta_right;
+        hL[n]                    = U0[n] - 0.5 * phi[n] * delta;
+        hR[n]                    = U0[n] + 0.5 * phi[n] * delta;
+    }
+    double QL[NC], QR[NC], FL[NC], FR[NC];
+    primv_from_consv<D>(hL, QL, pm);
+    primv_from_consv<D>(hR, QR, pm);
+    euler_flux<D, dir>(hL, QL, FL, pm);
+    euler_flux<D, dir>(hR, QR, FR, pm);
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+    {
+        hL[n] = hL[n] - hdtdx * (FR[n] - FL[n]);
+        hR[n] = hR[n] - hdtdx * (FR[n] - FL[n]);
+    }
+}
+
+BX_DEV double amrex_max(double a, double b) { return (a < b) ? b : a; }
+
+// HLLC flux at the face between left state UL and right state UR
+// (HLLC.cpp:90-172 with estimate_wave_speed_HLLC_Toro, HLLC.H:213-309).
+template <int D, int dir>
+BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
+                      double (&F)[D + 2], const EulerParm &pm)
+{
+    constexpr int NC = D + 2;
+    double        QL[NC], QR[NC];
+    primv_from_consv<D>(UL, QL, pm);
+    primv_from_consv<D>(UR, QR, pm);
+
+    const double adiabatic = pm.gamma;
+    const double epsilon   = pm.eps;
+    constexpr double min_rho = 1e-14;
+    constexpr double min_p   = 1e-14;
+    const double density_L = amrex_max(QL[0], min_rho);
+    const double density_R = amrex_max(QR[0], min_rho);
+    const double p_L       = amrex_max(QL[1 + D], min_p);
+    const double p_R       = amrex_max(QR[1 + D], min_p);
+    const double u_L       = QL[1 + dir];
+    const double u_R       = QR[1 + dir];
+
+    const double c_L = sqrt(adiabatic * p_L / (density_L * epsilon));
+    const double c_R = sqrt(adiabatic * p_R / (density_R * epsilon));
+
+    const double z = (adiabatic - 1) / (2 * adiabatic);
+    const double p_rr
+        = bx_pow((c_L + c_R - (adiabatic - 1) / 2 * (u_R - u_L))
+                     / (c_L * bx_pow(p_L, -z) + c_R * bx_pow(p_R, -z)),
+                 1 / z);
+    const double q_L
+        = (p_rr > p_L)
+              ? sqrt(1 + (adiabatic + 1) / (2 * adiabatic) * (p_rr / p_L - 1))
+              : 1.;
+    const double q_R
+        = (p_rr > p_R)
+              ? sqrt(1 + (adiabatic + 1) / (2 * adiabatic) * (p_rr / p_R - 1))
+              : 1.;
+    const double speed_L = u_L - c_L * q_L;
+    const double speed_R = u_R + c_R * q_R;
+    const double denom
+        = (density_L * (speed_L - u_L) - density_R * (speed_R - u_R));
+    const double speed_interm
+        = (denom != 0) ? ((p_R - p_L) / epsilon
+                          + density_L * u_L * (speed_L - u_L)
+                          - density_R * u_R * (speed_R - u_R))
+                             / denom
+                       : 0.;
+
+    // Branch selection as in HLLC.cpp:101-167; sides share the star-state
+    // arithmetic of HLLC_interm_state_replace on un-floored primitives.
+    const bool use_L     = (0 <= speed_L);
+    const bool use_starL = !use_L && (speed_L <= 0 && 0 <= speed_interm);
+    const bool use_starR
+        = !use_L && !use_starL && (speed_interm <= 0 && 0 <= speed_R);
+    const bool use_R = !use_L && !use_starL && !use_starR && (0 >= speed_R);
+    const bool left  = use_L || use_starL;
+
+    double Us[NC], Qs[NC];
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+    {
+        Us[n] = left ? UL[n] : UR[n];
+        Qs[n] = left ? QL[n] : QR[n];
+    }
+    double Fs[NC];
+    euler_flux<D, dir>(Us, Qs, Fs, pm);
+
+    if (use_L || use_R)
+    {
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            F[n] = Fs[n];
+    }
+    else if (use_starL || use_starR)
+    {
+        const double speed_K = use_starL ? speed_L : speed_R;
+        // HLLC_interm_state_replace (HLLC.H:311-337)
+        const double density  = Qs[0];
+        const double u_K      = Qs[1 + dir];
+        const double pressure = Qs[1 + D];
+        const double energy   = Us[1 + D];
+        double       st[NC];
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            st[n] = Qs[n];
+        st[0]       = 1;
+        st[1 + dir] = speed_interm;
+        const double scale_factor
+            = density * (speed_K - u_K) / (speed_K - speed_interm);
+        st[1 + D] = energy / density
+                    + (speed_interm - u_K)
+                          * (speed_interm * epsilon
+                             + pressure / (density * (speed_K - u_K)));
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            st[n] = st[n] * scale_factor;
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            F[n] = Fs[n] + speed_K * (st[n] - Us[n]);
+    }
+    else
+    {
+        // NaN wave speeds: the reference leaves the face unwritten
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            F[n] = __builtin_nan("");
+    }
+}
+
+// First-order HLLC (num_method = hllc) uses the same face flux on the cell
+// states themselves (Fluxes.H:192-204).
+
+// max over a cell of sqrt(gamma p / (rho eps)) + |u| (Euler.cpp:59-67)
+template <int D>
+BX_DEV double cell_wave_speed(const double (&U)[D + 2], const EulerParm &pm)
+{
+    double Q[D + 2];
+    primv_from_consv<D>(U, Q, pm);
+    double vsq = Q[1] * Q[1];
+#pragma unroll
+    for (int d = 1; d < D; ++d)
+        vsq = vsq + Q[1 + d] * Q[1 + d];
+    const double velocity = sqrt(vsq);
+    return sqrt(pm.gamma * Q[1 + D] / (Q[0] * pm.eps)) + velocity;
+}
+
+} // namespace bx

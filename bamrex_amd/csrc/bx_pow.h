@@ -1,0 +1,212 @@
+// Portable binary64 pow(x, y) built only from IEEE-754 operations that give
+// the same bits on the CPU and on CDNA4 (+, -, *, /, fma, rint, integer bit
+// manipulation), so the HIP kernels and the CPU oracle can share one
+// definition and agree bit for bit.
+//
+// The reference computes the HLLC wave-speed estimate with std::pow
+// (Source/Fluxes/HLLC.H:270-273): pow(p_K, -z) and pow(., 1/z) with
+// z = (gamma-1)/(2 gamma).  glibc's pow and OCML's pow differ from each
+// other in the last bit for a fraction of inputs, so neither can be used on
+// both sides.  This one follows the textbook table-driven method
+// (Tang, "Table-driven implementation of the logarithm/exponential function",
+// ACM TOMS 1989/1990; Muller, "Handbook of Floating-Point Arithmetic", ch. 11):
+//
+//   log: x = 2^k m, m in [1,2); j = top 7 bits of m; c_j = 1 + (j+1/2)/128;
+//        m*invc_j = 1 + r computed as an exact double-double (fma), |r| < 2^-8;
+//        log x = k ln2 + (-log invc_j) + log1p(r), kept as a double-double.
+//   exp: t = y log x (double-double); t = (64 k + j) ln2/64 + r, |r| <= ln2/128;
+//        e^t = 2^k * 2^(j/64) * e^r, with e^r - 1 a degree-7 polynomial.
+//
+// Internal error is below 2^-62 relative before the final rounding, so the
+// result is within 0.51 ulp of the exact value (tested against 60-digit
+// decimal arithmetic and against glibc in the test suite).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define BX_POW_TABLE static __device__ __constant__ const
+#define BX_POW_FN __device__ __forceinline__
+#else
+#define BX_POW_TABLE static const
+#define BX_POW_FN static inline
+#endif
+
+#include "bx_pow_tables.h"
+
+namespace bxpow {
+
+BX_POW_FN uint64_t d2u(double x) { return __builtin_bit_cast(uint64_t, x); }
+BX_POW_FN double u2d(uint64_t u) { return __builtin_bit_cast(double, u); }
+
+// Exact sum s + e = a + b (Knuth TwoSum).
+BX_POW_FN void two_sum(double a, double b, double &s, double &e)
+{
+    s = a + b;
+    double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// Exact sum s + e = a + b, valid when |a| >= |b| or a == 0.
+BX_POW_FN void fast_two_sum(double a, double b, double &s, double &e)
+{
+    s = a + b;
+    e = b - (s - a);
+}
+
+// log(x) as a double-double, x positive, finite, normal.
+BX_POW_FN void log_dd(double x, double &lhi, double &llo)
+{
+    uint64_t ix = d2u(x);
+    int      k  = (int)((ix >> 52) & 0x7ff) - 1023;
+    double   m  = u2d((ix & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+    int      j  = (int)((ix >> 45) & 127);
+
+    const double invc = bx_log_invc[j];
+    // m * invc = p_hi + p_lo exactly; r = p - 1
+    const double p_hi = m * invc;
+    const double p_lo = __builtin_fma(m, invc, -p_hi);
+    const double r0   = p_hi - 1.0; // exact: p_hi in [1 - 2^-7, 1 + 2^-7]
+    double       r_hi, r_lo;
+    two_sum(r0, p_lo, r_hi, r_lo);
+
+    // log1p(r) = r - r^2/2 + r^3 (1/3 - r/4 + r^2/5 - r^3/6 + r^4/7 - r^5/8)
+    const double sq_hi = r_hi * r_hi;
+    const double sq_lo = __builtin_fma(r_hi, r_hi, -sq_hi);
+    const double r3    = sq_hi * r_hi;
+    double       poly  = -0.125;
+    poly = __builtin_fma(poly, r_hi, 0x1.2492492492492p-3);  // 1/7
+    poly = __builtin_fma(poly, r_hi, -0x1.5555555555555p-3); // -1/6
+    poly = __builtin_fma(poly, r_hi, 0x1.999999999999ap-3);  // 1/5
+    poly = __builtin_fma(poly, r_hi, -0.25);
+    poly = __builtin_fma(poly, r_hi, 0x1.5555555555555p-2);  // 1/3
+
+    // k ln2 + logc: both multiples of 2^-42, the sum is exact
+    const double kd = (double)k;
+    const double a  = kd * BX_LN2_HI + bx_log_logc_hi[j];
+
+    double b_hi, b_lo;
+    two_sum(a, r_hi, b_hi, b_lo);
+    double c_hi, c_lo;
+    two_sum(b_hi, -0.5 * sq_hi, c_hi, c_lo);
+
+    const double small = kd * BX_LN2_LO + bx_log_logc_lo[j] + r_lo
+                         - 0.5 * sq_lo - r_hi * r_lo + r3 * poly;
+    fast_two_sum(c_hi, c_lo + b_lo + small, lhi, llo);
+}
+
+// e^(t_hi + t_lo) for |t_hi| well inside the finite range.
+BX_POW_FN double exp_dd(double t_hi, double t_lo)
+{
+    const double kk   = __builtin_rint(t_hi * BX_INV_LN2_64);
+    const int    ki   = (int)kk;
+    const int    j    = ki & 63;
+    const int    k    = (ki - j) >> 6;
+    const double r_hi = __builtin_fma(-kk, BX_LN2_64_HI, t_hi); // exact
+    const double r_lo = t_lo - kk * BX_LN2_64_LO;
+    double       r, rl;
+    fast_two_sum(r_hi, r_lo, r, rl);
+
+    // e^r - 1 = r + [rl + r^2/2 + r^3 (1/6 + r/24 + r^2/120 + r^3/720 + r^4/5040)]
+    double poly = 0x1.a01a01a01a01ap-13;                 // 1/5040
+    poly        = __builtin_fma(poly, r, 0x1.6c16c16c16c17p-10); // 1/720
+    poly        = __builtin_fma(poly, r, 0x1.1111111111111p-7);  // 1/120
+    poly        = __builtin_fma(poly, r, 0x1.5555555555555p-5);  // 1/24
+    poly        = __builtin_fma(poly, r, 0x1.5555555555555p-3);  // 1/6
+    const double r2  = r * r;
+    const double elo = rl + 0.5 * r2 + r * rl + (r2 * r) * poly;
+
+    const double th  = bx_exp_t_hi[j];
+    const double tl  = bx_exp_t_lo[j];
+    const double phi = th * r;
+    const double plo = __builtin_fma(th, r, -phi);
+    const double sml = plo + th * elo + tl + tl * (r + elo);
+    double       s_hi, s_lo;
+    two_sum(th, phi, s_hi, s_lo);
+    double res = s_hi + (s_lo + sml);
+
+    // scale by 2^k without overflow in the scale factor itself
+    if (k > -1022 && k < 1024)
+        return res * u2d((uint64_t)(k + 1023) << 52);
+    const int k1 = k / 2;
+    const int k2 = k - k1;
+    res *= u2d((uint64_t)(k1 + 1023) << 52);
+    return res * u2d((uint64_t)(k2 + 1023) << 52);
+}
+
+BX_POW_FN bool is_integer(double y)
+{
+    return __builtin_rint(y) == y;
+}
+
+BX_POW_FN bool is_odd_integer(double y)
+{
+    if (!is_integer(y) || __builtin_fabs(y) >= 0x1p53)
+        return false;
+    const double h = 0.5 * y;
+    return __builtin_rint(h) != h;
+}
+
+} // namespace bxpow
+
+// pow(x, y) with C99 Annex F special cases for the finite/NaN inputs this
+// code meets; inf inputs follow Annex F too.
+BX_POW_FN double bx_pow(double x, double y)
+{
+    using namespace bxpow;
+    if (y == 0.0)
+        return 1.0;
+    if (x != x || y != y)
+        return x == 1.0 ? 1.0 : x + y;
+    double sign = 1.0;
+    if (!(x > 0.0))
+    {
+        if (x == 0.0)
+        {
+            const bool odd = is_odd_integer(y);
+            if (y > 0)
+                return odd ? x : 0.0;
+            return odd ? 1.0 / x : __builtin_inf();
+        }
+        // x < 0
+        if (!is_integer(y))
+            return __builtin_nan("");
+        if (is_odd_integer(y))
+            sign = -1.0;
+        x = -x;
+    }
+    if (x == __builtin_inf())
+        return sign * (y > 0 ? __builtin_inf() : 0.0);
+    if (__builtin_fabs(y) == __builtin_inf())
+    {
+        if (x == 1.0)
+            return 1.0;
+        return ((x > 1.0) == (y > 0)) ? __builtin_inf() : 0.0;
+    }
+    // subnormal x: rescale into the normal range
+    int    kadj = 0;
+    double xs   = x;
+    if (xs < 0x1p-1022)
+    {
+        xs *= 0x1p54;
+        kadj = 54;
+    }
+    double lhi, llo;
+    bxpow::log_dd(xs, lhi, llo);
+    if (kadj)
+    {
+        // log x = log xs - 54 ln2
+        double s, e;
+        bxpow::two_sum(lhi, -54.0 * BX_LN2_HI, s, e);
+        llo = llo + e - 54.0 * BX_LN2_LO;
+        bxpow::fast_two_sum(s, llo, lhi, llo);
+    }
+    const double t_hi = y * lhi;
+    const double t_lo = __builtin_fma(y, lhi, -t_hi) + y * llo;
+    double       th, tl;
+    bxpow::fast_two_sum(t_hi, t_lo, th, tl);
+    if (th > 709.8)
+        return sign * __builtin_inf();
+    if (th < -745.2)
+        return sign * 0.0;
+    return sign * bxpow::exp_dd(th, tl);
+}

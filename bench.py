@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit hot path (one JSON line on stdout).
+
+Workload (the configuration the headline metric is quoted on): 2D
+four-quadrant Riemann problem, one uniform level of 4096^2 cells,
+max_grid_size 128 (1024 boxes), gamma 1.4, MUSCL-Hancock + HLLC with the
+minbee limiter, CFL 0.8, FP64.  A step is one full split step of the level:
+device-side CFL dt, x sweep, ghost fill, y sweep (with the fused CFL
+reduction), ghost fill -- replayed as a captured hipGraph.
+
+value      = cell-updates/s over K timed steps (state resident in HBM).
+roofline   = the slowest sweep kernel: algorithmic bytes 2*NC*8 = 64 B per
+             valid cell per sweep / its average duration (HIP events around
+             the kernel on its stream), against 8 TB/s.
+cpu_baseline = the CPU oracle (same arithmetic) on a bounded sample of the
+             same workload, timed on this host.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MHM-HLLC FP64 cell-updates/s; IMEX solve s/step learned-precond vs plain MLMG"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--ncell", type=int, default=4096)
+    ap.add_argument("--mgs", type=int, default=128)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (each GPU box has one "
+              "MI355X); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from bamrex_amd import _lib, problems
+    from bamrex_amd.level import Level, make_params
+
+    lib = _lib.lib()
+    name = ctypes.create_string_buffer(256)
+    _lib.check(lib.bx_device_name(name, 256), "bx_device_name")
+    log("device: " + name.value.decode())
+
+    log("building the %d^2 quadrant level (mgs %d)" % (args.ncell, args.mgs))
+    s = problems.quadrant_setup(ncell=args.ncell, max_grid_size=args.mgs)
+    lay = s["layout"]
+    ncells = lay.ncells()
+    gp = make_params(2, s["adiabatic"], s["epsilon"], limiter=s["limiter"])
+    stream = ctypes.c_void_p()
+    _lib.check(lib.bx_stream_create(ctypes.byref(stream)), "stream")
+    lev = Level(gp, lay, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], stream=stream)
+    lev.set_state(s["U"])
+    lev.fill_boundary()
+    dt0 = lev.estTimeStep(s["dx"], s["cfl"])
+    tuning = None
+    if not args.no_autotune:
+        log("autotuning launch shapes")
+        lev.autotune(s["dx"], dt0)
+        tuning = lev.get_tuning()
+        log("tuning: %s" % tuning)
+        lev.set_state(s["U"])
+
+    # ------------------------------------------------- per-kernel timing
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(lib.bx_event_create(ctypes.byref(ev0)), "event")
+    _lib.check(lib.bx_event_create(ctypes.byref(ev1)), "event")
+    lev.fill_boundary()
+    lev.set_tmp(s["U"])
+    kern_ms = []
+    for d in range(2):
+        lev.sweep(d, dt0, s["dx"])  # warm
+        _lib.check(lib.bx_stream_sync(stream), "sync")
+        _lib.check(lib.bx_event_record(ev0, stream), "record")
+        for _ in range(args.kernel_reps):
+            lev.sweep(d, dt0, s["dx"])
+        _lib.check(lib.bx_event_record(ev1, stream), "record")
+        ms = ctypes.c_float()
+        _lib.check(lib.bx_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)), "elapsed")
+        kern_ms.append(ms.value / args.kernel_reps)
+    log("sweep kernel avg ms: x %.3f  y %.3f" % tuple(kern_ms))
+
+    # ----------------------------------------------------- timed steps
+    lev.set_state(s["U"])
+    if args.warmup > 0:
+        lev.run(s["dx"], s["cfl"], max_step=args.warmup, use_graph=True)
+    lev.set_state(s["U"])
+    _lib.check(lib.bx_stream_sync(stream), "sync")
+    _lib.check(lib.bx_event_record(ev0, stream), "record")
+    t0 = time.perf_counter()
+    t_end, nsteps = lev.run(s["dx"], s["cfl"], max_step=args.steps, use_graph=True)
+    _lib.check(lib.bx_event_record(ev1, stream), "record")
+    _lib.check(lib.bx_stream_sync(stream), "sync")
+    wall = time.perf_counter() - t0
+    ms = ctypes.c_float()
+    _lib.check(lib.bx_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)), "elapsed")
+    elapsed = max(wall, ms.value / 1000.0)
+    value = nsteps * ncells / elapsed
+    log("%d steps in %.3f ms (t=%.6g): %.3e cell-updates/s" % (nsteps, elapsed * 1e3, t_end, value))
+
+    dom = max(range(2), key=lambda d: kern_ms[d])
+    algo_bytes = ncells * 2 * 4 * 8  # read + write NC doubles per valid cell
+    achieved = algo_bytes / (kern_ms[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_sweep_%s" % ("x" if dom == 0 else "t (y)"),
+                "kernel_ms": round(kern_ms[dom], 4),
+                "sweep_ms": [round(x, 4) for x in kern_ms],
+                "algo_bytes_per_launch": algo_bytes}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_seconds)
+
+    lev.close()
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": nsteps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / nsteps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "quadrant-2d-%dx%d-mgs%d-mhm-hllc-minbee-cfl0.8" % (
+            args.ncell, args.ncell, args.mgs), "cells": ncells, "boxes": len(lay.boxes),
+            "ghost_cells": lay.ng, "graph": True, "tuning": tuning},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(budget_s):
+    """The oracle on a bounded sample of the same workload (a 1024^2 quadrant
+    level with the same boxes size and limiter), all host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from bamrex_amd import problems
+    ncell = 1024
+    s = problems.quadrant_setup(ncell=ncell, max_grid_size=128)
+    p = pyoracle.params(2, s["adiabatic"], s["epsilon"], limiter=s["limiter"])
+    o = pyoracle.Oracle(bxpow=False)
+    # one step to size the sample
+    t0 = time.perf_counter()
+    U = s["U"].copy()
+    o.run(p, s["layout"], U, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], s["dx"], s["cfl"],
+          max_step=1)
+    one = time.perf_counter() - t0
+    nsteps = max(1, min(200, int(budget_s / max(one, 1e-3))))
+    U = s["U"].copy()
+    t0 = time.perf_counter()
+    o.run(p, s["layout"], U, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], s["dx"], s["cfl"],
+          max_step=nsteps)
+    el = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log("cpu oracle: %d steps of %d^2 in %.2f s" % (nsteps, ncell, el))
+    return {"value": nsteps * ncell * ncell / el, "unit": "cell-updates/s", "cores": cores,
+            "kind": "port", "sample": "%d steps of the quadrant problem at %d^2, mgs 128" % (
+                nsteps, ncell)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,212 @@
+/*
+ * bamrex_hip.h -- C ABI of the MI355X-native BAMReX explicit hot path.
+ *
+ * Plain C types only (pointers, sizes, ints, doubles).  Device buffers are
+ * owned by the caller; the library owns only its context and the device
+ * copies of the box list / copy plans it builds.  Every call that takes a
+ * `stream` (a hipStream_t, NULL = default stream) is stream-ordered and
+ * asynchronous unless its comment says otherwise.  A context is not
+ * thread-safe: use one context per stream.  Status: 0 = BX_OK, otherwise a
+ * BX_ERR_* code and bx_last_error() describes it.  No exception crosses the ABI.
+ *
+ * Data layout (the amrex::Array4 / FArrayBox layout): the FAB of a box with
+ * valid region [lo, hi] and ng ghost cells covers grow([lo,hi], ng) in the
+ * first `dim` directions; element (i,j,k,n) sits at
+ *   offset + (i-lo0+ng) + (j-lo1+ng)*jstride + (k-lo2+ng)*kstride + n*nstride
+ * with jstride = nx+2ng, kstride = jstride*(ny+2ng), nstride = FAB volume,
+ * counted in doubles from the level base pointer passed to each call.
+ * Components are conserved variables (rho, mom_x..mom_{dim-1}, E), dim+2 of them.
+ */
+#ifndef BAMREX_HIP_H
+#define BAMREX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BX_OK 0
+#define BX_ERR_INVALID 1     /* bad argument (amrex::Abort / AMREX_ASSERT in the reference) */
+#define BX_ERR_HIP 2         /* HIP runtime error */
+#define BX_ERR_UNSUPPORTED 3 /* valid input the reference also rejects (RuntimeError) */
+#define BX_ERR_STATE 4       /* call order violated (e.g. no level set) */
+
+/* adv.num_method (Source/NumericalMethods.H:10-44); the explicit methods
+ * implemented here. */
+#define BX_METHOD_MUSCL_HANCOCK 0
+#define BX_METHOD_HLLC 1
+
+/* limiting.limiter (Source/Fluxes/Limiters.H:13) */
+#define BX_LIMITER_MINBEE 0
+#define BX_LIMITER_VAN_LEER 1
+#define BX_LIMITER_VAN_ALBADA 2
+#define BX_LIMITER_SUPERBEE 3
+
+/* bc.{x,y,z} (Source/BoundaryConditions/BCTypes.H; mapping BCs.cpp:215-397) */
+#define BX_BC_TRANSMISSIVE 0 /* amrex foextrap */
+#define BX_BC_PERIODIC 1     /* amrex int_dir */
+#define BX_BC_REFLECTIVE 2   /* reflect_even, normal momentum reflect_odd */
+
+typedef struct bx_ctx bx_ctx;
+
+/* struct Parm (Source/System/Parm.H:14-61) restricted to the hot path.
+ * limiter_indices[n] / limit_on_primv[n]: variable that limits component n
+ * (FluidSystem.cpp:27-71; default: conserved energy, index dim+1, for all n). */
+typedef struct
+{
+    int    dim;
+    int    num_method;
+    double adiabatic;
+    double epsilon;
+    int    limiter;
+    int    limiter_indices[5];
+    int    limit_on_primv[5];
+} bx_params;
+
+typedef struct
+{
+    int     lo[3]; /* valid region, inclusive, global cell index (unused dims 0) */
+    int     hi[3];
+    int64_t offset; /* doubles from the level base pointer to the FAB start */
+} bx_box;
+
+typedef struct
+{
+    int domain_lo[3];
+    int domain_hi[3];
+    int bc_lo[3]; /* BX_BC_* per direction */
+    int bc_hi[3];
+    int ng;       /* ghost cells of every FAB, >= 2 */
+} bx_geometry;
+
+/* ------------------------------------------------------------------ setup */
+
+/* Parm set-up: AmrLevelAdv::read_params + FluidSystem::ReadParameters
+ * (Source/AmrLevelAdv.cpp:837-909, Source/System/FluidSystem.cpp:10-71). */
+int  bx_create(const bx_params *params, bx_ctx **out);
+void bx_destroy(bx_ctx *ctx);
+
+/* Define the level: box list (BoxArray + FAB offsets), domain and boundary
+ * conditions.  Builds the device box list, the persistent-kernel work tables
+ * and the ghost-copy plan (replaces MFIter set-up and the FillBoundary /
+ * PhysBCFunct copy schedule, Source/BoundaryConditions/BCs.cpp:429-442). */
+int bx_set_level(bx_ctx *ctx, const bx_geometry *geom, const bx_box *boxes,
+                 int nboxes);
+
+/* Packed FAB offsets for a box list (each FAB 256-byte aligned);
+ * total_doubles receives the level buffer size.  Host only. */
+int bx_pack_offsets(int dim, int ng, const bx_box *boxes, int nboxes,
+                    int64_t *offsets_out, int64_t *total_doubles);
+
+/* ------------------------------------------------------- hot-path calls */
+
+/* One directional sweep over every box of the level: MUSCL-Hancock
+ * reconstruction + half step + HLLC flux + conservative update for
+ * num_method muscl_hancock (compute_MUSCL_hancock_flux, Fluxes.H:237-257 /
+ * MUSCL_Hancock.cpp:15-52, then AmrLevelAdv_advance.cpp:330-341), or the
+ * first-order HLLC flux + update for num_method hllc (Fluxes.H:206-227).
+ * Reads U_in (valid + 2 ghost layers along dir), writes the valid cells of
+ * U_out. */
+int bx_sweep(bx_ctx *ctx, int dir, double dt, const double dx[3],
+             const double *U_in, double *U_out, void *stream);
+
+/* Per-box drop-in of the reference flux function:
+ * compute_MUSCL_hancock_flux(dir, time, bx, flux, consv_values, dx, dt, ...)
+ * (Fluxes.H:237-257) / compute_HLLC_flux (Fluxes.H:206-227) for box ibox of
+ * the level.  flux_base + flux_offset addresses an Array4 on
+ * surroundingNodes(bx, dir) (no ghosts, nodal in dir) with dim+2 components. */
+int bx_flux_box(bx_ctx *ctx, int ibox, int dir, double dt, const double dx[3],
+                const double *U, double *flux_base, int64_t flux_offset,
+                void *stream);
+
+/* Split conservative update of box ibox from the fluxes bx_flux_box wrote
+ * (AmrLevelAdv_advance.cpp:330-341). */
+int bx_update_box(bx_ctx *ctx, int ibox, int dir, double dt, const double dx[3],
+                  const double *U_in, const double *flux_base,
+                  int64_t flux_offset, double *U_out, void *stream);
+
+/* bamrexBCData::fill_consv_boundary (BCs.cpp:429-442): all ghost cells of U
+ * from the valid cells of the level (periodic images included) and the
+ * physical boundary conditions. */
+int bx_fill_boundary(bx_ctx *ctx, double *U, void *stream);
+
+/* max_wave_speed (Source/System/Euler/Euler.cpp:46-74): max over valid cells
+ * of sqrt(gamma p / (rho eps)) + |u|.  Synchronises the stream. */
+int bx_max_wave_speed(bx_ctx *ctx, const double *U, double *out,
+                      void *stream);
+
+/* estTimeStep for the explicit methods (AmrLevelAdv.cpp:395-477, 468) plus the
+ * stop-time clamp of computeNewDt (AmrLevelAdv.cpp:600-611); stop_time < 0
+ * disables the clamp.  Synchronises the stream. */
+int bx_est_dt(bx_ctx *ctx, const double *U, const double dx[3], double cfl,
+              double time, double stop_time, double *dt_out, void *stream);
+
+/* AmrLevelAdv::advance / advance_explicit for the explicit methods
+ * (AmrLevelAdv_advance.cpp:21-259, 261-439) on a single level: sweep x, y, z
+ * in that order with the full dt, refilling ghosts after each sweep.  U holds
+ * the state with filled ghosts on entry and the new state (ghosts filled) on
+ * return; U_tmp is a second level buffer of the same layout. */
+int bx_advance(bx_ctx *ctx, double dt, const double dx[3], double *U,
+               double *U_tmp, void *stream);
+
+/* The main.cpp time loop (Source/main.cpp:76-94) for a single level:
+ *   while (step < max_step && time < stop_time) { dt = clamp(cfl * est);
+ *   advance; time += dt }  (max_step < 0 or stop_time < 0 disables a bound).
+ * U holds the initial state (ghosts need not be filled).  The dt reduction is
+ * fused into the last sweep of each step and the dt control runs on the
+ * device.  With stop_time < 0 and use_graph != 0 each step is captured once as
+ * a hipGraph and replayed, with no host synchronisation inside the loop.
+ * On return *result_in_tmp tells whether the final state is in U_tmp (odd
+ * number of sweeps in graph mode) and time / steps are filled.  Synchronous. */
+int bx_run(bx_ctx *ctx, const double dx[3], double cfl, double start_time,
+           double stop_time, int max_step, double *U, double *U_tmp,
+           int use_graph, void *stream, double *time_out, int *steps_out,
+           int *result_in_tmp);
+
+/* ------------------------------------------------------------- tuning */
+
+typedef struct
+{
+    int block_x;      /* threads per block of the x sweep (64..256, multiple of 64) */
+    int block_t;      /* threads per block of the y/z sweeps */
+    int seg_k;        /* cells per lane along y/z */
+    int grid_cap;     /* max resident blocks per CU used for persistent grids */
+} bx_tuning;
+
+int bx_get_tuning(const bx_ctx *ctx, bx_tuning *out);
+int bx_set_tuning(bx_ctx *ctx, const bx_tuning *t);
+
+/* Times every candidate launch shape of every sweep direction on (U, U_tmp)
+ * and keeps the fastest (block size x segment length x grid cap).  The state
+ * buffers are overwritten.  Synchronous; writes the timings (ms per sweep)
+ * of the winning shapes to best_ms[dir] when best_ms is not NULL. */
+int bx_autotune(bx_ctx *ctx, const double dx[3], double dt, double *U,
+                double *U_tmp, void *stream, double *best_ms);
+
+/* -------------------------------------------- device memory / timing helpers
+ * Thin wrappers so that hosts without a HIP toolchain (ctypes, other FFIs)
+ * can drive the library. */
+int         bx_device_count(int *n);
+int         bx_malloc(void **ptr, size_t bytes);
+int         bx_free(void *ptr);
+int         bx_memcpy_htod(void *dst, const void *src, size_t bytes);
+int         bx_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int         bx_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int         bx_memset(void *dst, int value, size_t bytes);
+int         bx_stream_create(void **stream);
+int         bx_stream_destroy(void *stream);
+int         bx_stream_sync(void *stream);
+int         bx_event_create(void **ev);
+int         bx_event_destroy(void *ev);
+int         bx_event_record(void *ev, void *stream);
+int         bx_event_elapsed_ms(void *start, void *stop, float *ms);
+int         bx_device_name(char *buf, int len);
+const char *bx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BAMREX_HIP_H */

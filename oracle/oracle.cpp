@@ -1,0 +1,933 @@
+// CPU ORACLE -- test infrastructure only.
+//
+// A plain C++ restatement of the reference's explicit hot path, used by the
+// tests (as the checker), by __graft_entry__.smoke() and by bench.py's
+// cpu_baseline leg.  It is never part of the product path.
+//
+// It follows the reference operation by operation, per box, with the same
+// temporaries the reference allocates:
+//   Euler.H:138-254        consv_from_primv / primv_from_consv / euler_flux
+//   Euler.cpp:10-74        compute_flux_function, max_wave_speed
+//   Limiters.H:29-81       slope limiters, minmod
+//   CalcLimitingVal.H:9-22 limiting variable
+//   Flux_Helpers.cpp:52-149 reconstruct_and_half_time_step (temporaries on
+//                          grow(bx,1), Q on grow(gbx,dir,1))
+//   HLLC.H:213-337         estimate_wave_speed_HLLC_Toro, HLLC_interm_state_replace
+//   HLLC.cpp:64-173        compute_HLLC_flux_LR_euler
+//   MUSCL_Hancock.cpp:15-52 compute_MUSCL_hancock_flux (swapped L/R arrays)
+//   Fluxes.H:192-227       compute_HLLC_flux (first order)
+//   AmrLevelAdv_advance.cpp:261-360 advance_explicit (x,y,z order, full dt,
+//                          update, fill_consv_boundary after every sweep)
+//   BCs.cpp:215-442        bc types -> FillBoundary + PhysBCFunct
+//   AmrLevelAdv.cpp:395-619 estTimeStep, computeInitialDt / computeNewDt
+//   main.cpp:76-94         time loop
+// Compiled with -ffp-contract=off and no fast-math.  ORACLE_BXPOW selects the
+// portable pow of the product (bamrex_amd/csrc/bx_pow.h) instead of
+// std::pow, to separate the one libm dependency from everything else.
+//
+// Parity status of each part (which check pins it) is listed in the header of
+// tests/test_oracle.py.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#ifdef ORACLE_BXPOW
+#include "../bamrex_amd/csrc/bx_pow.h"
+static inline double or_pow(double x, double y) { return bx_pow(x, y); }
+#else
+static inline double or_pow(double x, double y) { return std::pow(x, y); }
+#endif
+
+namespace oracle {
+
+typedef std::array<double, 5> Vec;
+
+struct Parm
+{
+    int    D;
+    double adiabatic;
+    double epsilon;
+    int    limiter; // 0 minbee, 1 van_leer, 2 van_albada, 3 superbee
+    int    limiter_indices[5];
+    bool   limit_on_primv[5];
+    bool   need_primitive_for_limiting;
+};
+
+struct Box
+{
+    int lo[3];
+    int hi[3];
+};
+
+static Box grow(const Box &b, int D, int n)
+{
+    Box r = b;
+    for (int d = 0; d < D; ++d)
+    {
+        r.lo[d] -= n;
+        r.hi[d] += n;
+    }
+    return r;
+}
+
+static Box grow_dir(const Box &b, int dir, int n)
+{
+    Box r = b;
+    r.lo[dir] -= n;
+    r.hi[dir] += n;
+    return r;
+}
+
+static Box surrounding_nodes(const Box &b, int dir)
+{
+    Box r = b;
+    r.hi[dir] += 1;
+    return r;
+}
+
+// amrex::FArrayBox / Array4 with (i, j, k, n) indexing over box bx
+struct Fab
+{
+    Box                 bx;
+    int                 nc = 0;
+    long long           jst = 0, kst = 0, nst = 0;
+    std::vector<double> a;
+    void define(const Box &b, int ncomp)
+    {
+        bx  = b;
+        nc  = ncomp;
+        jst = bx.hi[0] - bx.lo[0] + 1;
+        kst = jst * (bx.hi[1] - bx.lo[1] + 1);
+        nst = kst * (bx.hi[2] - bx.lo[2] + 1);
+        a.assign(nst * nc, 0.0);
+    }
+    double &operator()(int i, int j, int k, int n)
+    {
+        return a[(i - bx.lo[0]) + (j - bx.lo[1]) * jst + (k - bx.lo[2]) * kst
+                 + n * nst];
+    }
+    double operator()(int i, int j, int k, int n) const
+    {
+        return a[(i - bx.lo[0]) + (j - bx.lo[1]) * jst + (k - bx.lo[2]) * kst
+                 + n * nst];
+    }
+};
+
+template <class F>
+static void for_box(const Box &b, F f)
+{
+    for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+        for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+            for (int i = b.lo[0]; i <= b.hi[0]; ++i)
+                f(i, j, k);
+}
+
+// ------------------------------------------------------------ Euler.H
+// consv_from_primv  Euler.H:138-165
+static Vec consv_from_primv(const Vec &q, double adiabatic, double epsilon,
+                            int D)
+{
+    Vec c{};
+    c[0] = q[0];
+    for (int d = 0; d < D; ++d)
+        c[1 + d] = q[0] * q[1 + d];
+    const double pressure            = q[1 + D];
+    const double vol_internal_energy = pressure / (adiabatic - 1);
+    double       u_sq                = q[1] * q[1];
+    for (int d = 1; d < D; ++d)
+        u_sq = u_sq + q[1 + d] * q[1 + d];
+    c[1 + D] = vol_internal_energy + 0.5 * epsilon * q[0] * u_sq;
+    return c;
+}
+
+// primv_from_consv  Euler.H:167-198
+static Vec primv_from_consv(const Fab &U, double adiabatic, double epsilon,
+                            int i, int j, int k, int D)
+{
+    Vec q{};
+    q[0] = U(i, j, k, 0);
+    for (int d = 0; d < D; ++d)
+        q[1 + d] = U(i, j, k, 1 + d) / U(i, j, k, 0);
+    double u_sq = q[1] * q[1];
+    for (int d = 1; d < D; ++d)
+        u_sq = u_sq + q[1 + d] * q[1 + d];
+    q[1 + D] = (adiabatic - 1)
+               * (U(i, j, k, 1 + D) - 0.5 * epsilon * U(i, j, k, 0) * u_sq);
+    return q;
+}
+
+// euler_flux<dir>  Euler.H:211-244
+static Vec euler_flux(const Fab &U, double adiabatic, double epsilon, int i,
+                      int j, int k, int dir, int D)
+{
+    const Vec    q        = primv_from_consv(U, adiabatic, epsilon, i, j, k, D);
+    const double pressure = q[1 + D];
+    const double energy   = U(i, j, k, 1 + D);
+    Vec          f{};
+    f[0] = U(i, j, k, 1 + dir);
+    for (int d = 0; d < D; ++d)
+        f[1 + d] = U(i, j, k, 1 + dir) * q[1 + d];
+    f[1 + dir] += pressure / epsilon;
+    f[1 + D] = (energy + pressure) * q[1 + dir];
+    return f;
+}
+
+// compute_flux_function<dir>  Euler.cpp:10-44
+static void compute_flux_function(const Box &bx, Fab &flux_func, const Fab &U,
+                                  double adiabatic, double epsilon, int dir,
+                                  int D)
+{
+    for_box(bx, [&](int i, int j, int k) {
+        const Vec    q        = primv_from_consv(U, adiabatic, epsilon, i, j, k, D);
+        const double pressure = q[1 + D];
+        const double energy   = U(i, j, k, 1 + D);
+        flux_func(i, j, k, 0) = U(i, j, k, 1 + dir);
+        for (int d = 0; d < D; ++d)
+            flux_func(i, j, k, 1 + d) = U(i, j, k, 1 + dir) * q[1 + d];
+        flux_func(i, j, k, 1 + dir) += pressure / epsilon;
+        flux_func(i, j, k, 1 + D) = (energy + pressure) * q[1 + dir];
+    });
+}
+
+// ---------------------------------------------------------- Limiters.H
+static double minbee(double r)
+{
+    return r <= 0 ? 0 : (r > 1 ? std::fmin(2. / (1 + r), 1.) : r);
+}
+static double van_leer(double r)
+{
+    return r <= 0 ? 0 : std::fmin(2. * r / (1 + r), 2. / (1 + r));
+}
+static double van_albada(double r)
+{
+    return r <= 0 ? 0 : std::fmin(r * (1 + r) / (1 + r * r), 2. / (1 + r));
+}
+static double superbee(double r)
+{
+    return (r <= 0) ? 0
+                    : ((0 < r && r <= 0.5)
+                           ? 2 * r
+                           : ((0.5 < r && r <= 1)
+                                  ? 1
+                                  : std::fmin(r, std::fmin(2. / (1 + r), 2.))));
+}
+static double slope_limit(int limiter, double r)
+{
+    switch (limiter)
+    {
+    case 0: return minbee(r);
+    case 1: return van_leer(r);
+    case 2: return van_albada(r);
+    default: return superbee(r);
+    }
+}
+static double minmod(double delta_L, double delta_R)
+{
+    return (delta_L > 0 && delta_R > 0)
+               ? std::fmin(delta_L, delta_R)
+               : ((delta_L < 0 && delta_R < 0) ? std::fmax(delta_L, delta_R)
+                                               : 0);
+}
+
+// calc_limiting_var  CalcLimitingVal.H:9-22
+static double calc_limiting_var(const Fab &U, const Fab &Q, const Parm &parm,
+                                int i, int j, int k, int n)
+{
+    if (parm.limit_on_primv[n])
+        return Q(i, j, k, parm.limiter_indices[n]);
+    return U(i, j, k, parm.limiter_indices[n]);
+}
+
+// UtoQ  FluidSystem.cpp:161-178
+static void UtoQ(const Box &bx, Fab &Q, const Fab &U, const Parm &parm)
+{
+    for_box(bx, [&](int i, int j, int k) {
+        const Vec q = primv_from_consv(U, parm.adiabatic, parm.epsilon, i, j, k,
+                                       parm.D);
+        for (int n = 0; n < parm.D + 2; ++n)
+            Q(i, j, k, n) = q[n];
+    });
+}
+
+// reconstruct_and_half_time_step<limiter,dir>  Flux_Helpers.cpp:52-149
+static void reconstruct_and_half_time_step(const Box &gbx, Fab &half_L,
+                                           Fab &half_R, const Fab &consv,
+                                           const double dx_arr[3], double dt,
+                                           const Parm &parm, int dir)
+{
+    const int    D      = parm.D;
+    const int    NSTATE = D + 2;
+    const double dx     = dx_arr[dir];
+    const Box    gbx2   = grow_dir(gbx, dir, 1);
+    Fab          flux_func_L, flux_func_R, Q;
+    flux_func_L.define(gbx, NSTATE);
+    flux_func_R.define(gbx, NSTATE);
+    if (parm.need_primitive_for_limiting)
+    {
+        Q.define(gbx2, NSTATE);
+        UtoQ(gbx2, Q, consv, parm);
+    }
+    const int i_off = (dir == 0) ? 1 : 0;
+    const int j_off = (dir == 1) ? 1 : 0;
+    const int k_off = (dir == 2) ? 1 : 0;
+
+    for (int n = 0; n < NSTATE; ++n)
+        for_box(gbx, [&](int i, int j, int k) {
+            const double val_C = calc_limiting_var(consv, Q, parm, i, j, k, n);
+            const double val_L = calc_limiting_var(consv, Q, parm, i - i_off,
+                                                   j - j_off, k - k_off, n);
+            const double val_R = calc_limiting_var(consv, Q, parm, i + i_off,
+                                                   j + j_off, k + k_off, n);
+            const double numerator = val_C - val_L;
+            const double denom     = val_R - val_C;
+            double       limiter_val;
+            if (numerator == 0. && denom == 0.)
+                limiter_val = slope_limit(parm.limiter, 1);
+            else if (denom == 0)
+                limiter_val = 0;
+            else
+                limiter_val = slope_limit(parm.limiter, numerator / denom);
+
+            const double omega      = 0;
+            const double delta_left = consv(i, j, k, n)
+                                      - consv(i - i_off, j - j_off, k - k_off, n);
+            const double delta_right
+                = consv(i + i_off, j + j_off, k + k_off, n) - consv(i, j, k, n);
+            const double delta = 0.5 * (1 + omega) * delta_left
+                                 + 0.5 * (1 - omega) * delta_right;
+            half_L(i, j, k, n) = consv(i, j, k, n) - 0.5 * limiter_val * delta;
+            half_R(i, j, k, n) = consv(i, j, k, n) + 0.5 * limiter_val * delta;
+        });
+
+    compute_flux_function(gbx, flux_func_L, half_L, parm.adiabatic,
+                          parm.epsilon, dir, D);
+    compute_flux_function(gbx, flux_func_R, half_R, parm.adiabatic,
+                          parm.epsilon, dir, D);
+
+    const double hdtdx = 0.5 * dt / dx;
+    for (int n = 0; n < NSTATE; ++n)
+        for_box(gbx, [&](int i, int j, int k) {
+            half_L(i, j, k, n)
+                -= hdtdx * (flux_func_R(i, j, k, n) - flux_func_L(i, j, k, n));
+            half_R(i, j, k, n)
+                -= hdtdx * (flux_func_R(i, j, k, n) - flux_func_L(i, j, k, n));
+        });
+}
+
+// estimate_wave_speed_HLLC_Toro<dir>  HLLC.H:213-309
+static void estimate_wave_speed_HLLC_Toro(const Fab &cL, const Fab &cR, int i,
+                                          int j, int k, double adiabatic_L,
+                                          double adiabatic_R, double epsilon,
+                                          int dir, int D, double &speed_L,
+                                          double &speed_R, double &speed_interm)
+{
+    const int i_off = (dir == 0) ? 1 : 0;
+    const int j_off = (dir == 1) ? 1 : 0;
+    const int k_off = (dir == 2) ? 1 : 0;
+    Vec primv_L = primv_from_consv(cL, adiabatic_L, epsilon, i - i_off,
+                                   j - j_off, k - k_off, D);
+    Vec primv_R = primv_from_consv(cR, adiabatic_R, epsilon, i, j, k, D);
+
+    double       density_L = primv_L[0];
+    double       density_R = primv_R[0];
+    double       p_L       = primv_L[1 + D];
+    double       p_R       = primv_R[1 + D];
+    const double u_L       = primv_L[1 + dir];
+    const double u_R       = primv_R[1 + dir];
+
+    constexpr double min_rho = 1e-14;
+    constexpr double min_p   = 1e-14;
+    density_L                = std::max(density_L, min_rho);
+    density_R                = std::max(density_R, min_rho);
+    p_L                      = std::max(p_L, min_p);
+    p_R                      = std::max(p_R, min_p);
+
+    const double c_L = std::sqrt(adiabatic_L * p_L / (density_L * epsilon));
+    const double c_R = std::sqrt(adiabatic_R * p_R / (density_R * epsilon));
+    const double adiabatic = adiabatic_L;
+
+    const double z = (adiabatic - 1) / (2 * adiabatic);
+    const double p_rr
+        = or_pow((c_L + c_R - (adiabatic - 1) / 2 * (u_R - u_L))
+                     / (c_L * or_pow(p_L, -z) + c_R * or_pow(p_R, -z)),
+                 1 / z);
+    const double q_L
+        = (p_rr > p_L) ? std::sqrt(1 + (adiabatic + 1) / (2 * adiabatic)
+                                           * (p_rr / p_L - 1))
+                       : 1;
+    const double q_R
+        = (p_rr > p_R) ? std::sqrt(1 + (adiabatic + 1) / (2 * adiabatic)
+                                           * (p_rr / p_R - 1))
+                       : 1;
+    speed_L = u_L - c_L * q_L;
+    speed_R = u_R + c_R * q_R;
+    const double denom
+        = (density_L * (speed_L - u_L) - density_R * (speed_R - u_R));
+    speed_interm = (denom != 0) ? ((p_R - p_L) / epsilon
+                                   + density_L * u_L * (speed_L - u_L)
+                                   - density_R * u_R * (speed_R - u_R))
+                                      / denom
+                                : 0;
+}
+
+// HLLC_interm_state_replace<dir>  HLLC.H:311-337
+static void HLLC_interm_state_replace(Vec &primv, double energy, double speed_K,
+                                      double speed_interm, double epsilon,
+                                      int dir, int D)
+{
+    const double density  = primv[0];
+    const double u_K      = primv[1 + dir];
+    const double pressure = primv[1 + D];
+    primv[0]              = 1;
+    primv[1 + dir]        = speed_interm;
+    const double scale_factor
+        = density * (speed_K - u_K) / (speed_K - speed_interm);
+    primv[1 + D] = energy / density
+                   + (speed_interm - u_K)
+                         * (speed_interm * epsilon
+                            + pressure / (density * (speed_K - u_K)));
+    for (int i = 0; i < D + 2; ++i)
+        primv[i] *= scale_factor;
+}
+
+// compute_HLLC_flux_LR_euler<dir>  HLLC.cpp:64-173
+static void compute_HLLC_flux_LR(const Box &bx, Fab &flux, const Fab &cL,
+                                 const Fab &cR, const Parm &parm, int dir)
+{
+    const int    D       = parm.D;
+    const int    NSTATE  = D + 2;
+    const Box    flux_bx = surrounding_nodes(bx, dir);
+    const int    i_off   = (dir == 0) ? 1 : 0;
+    const int    j_off   = (dir == 1) ? 1 : 0;
+    const int    k_off   = (dir == 2) ? 1 : 0;
+    const double aL = parm.adiabatic, aR = parm.adiabatic, eps = parm.epsilon;
+    for_box(flux_bx, [&](int i, int j, int k) {
+        double speed_L, speed_R, speed_interm;
+        estimate_wave_speed_HLLC_Toro(cL, cR, i, j, k, aL, aR, eps, dir, D,
+                                      speed_L, speed_R, speed_interm);
+        if (0 <= speed_L)
+        {
+            const Vec f = euler_flux(cL, aL, eps, i - i_off, j - j_off,
+                                     k - k_off, dir, D);
+            for (int n = 0; n < NSTATE; ++n)
+                flux(i, j, k, n) = f[n];
+        }
+        else if (speed_L <= 0 && 0 <= speed_interm)
+        {
+            const Vec f = euler_flux(cL, aL, eps, i - i_off, j - j_off,
+                                     k - k_off, dir, D);
+            Vec       pv = primv_from_consv(cL, aL, eps, i - i_off, j - j_off,
+                                            k - k_off, D);
+            HLLC_interm_state_replace(
+                pv, cL(i - i_off, j - j_off, k - k_off, 1 + D), speed_L,
+                speed_interm, eps, dir, D);
+            for (int n = 0; n < NSTATE; ++n)
+                flux(i, j, k, n)
+                    = f[n]
+                      + speed_L * (pv[n] - cL(i - i_off, j - j_off, k - k_off, n));
+        }
+        else if (speed_interm <= 0 && 0 <= speed_R)
+        {
+            const Vec f  = euler_flux(cR, aR, eps, i, j, k, dir, D);
+            Vec       pv = primv_from_consv(cR, aR, eps, i, j, k, D);
+            HLLC_interm_state_replace(pv, cR(i, j, k, 1 + D), speed_R,
+                                      speed_interm, eps, dir, D);
+            for (int n = 0; n < NSTATE; ++n)
+                flux(i, j, k, n) = f[n] + speed_R * (pv[n] - cR(i, j, k, n));
+        }
+        else if (0 >= speed_R)
+        {
+            const Vec f = euler_flux(cR, aR, eps, i, j, k, dir, D);
+            for (int n = 0; n < NSTATE; ++n)
+                flux(i, j, k, n) = f[n];
+        }
+        else
+        {
+            for (int n = 0; n < NSTATE; ++n)
+                flux(i, j, k, n) = std::nan("");
+        }
+    });
+}
+
+// compute_MUSCL_hancock_flux<dir>  MUSCL_Hancock.cpp:15-52
+static void compute_MUSCL_hancock_flux(const Box &bx, Fab &flux,
+                                       const Fab &consv, const double dx[3],
+                                       double dt, const Parm &parm, int dir)
+{
+    const Box bx_g1 = grow(bx, parm.D, 1);
+    Fab       half_L, half_R;
+    half_L.define(bx_g1, parm.D + 2);
+    half_R.define(bx_g1, parm.D + 2);
+    reconstruct_and_half_time_step(bx_g1, half_L, half_R, consv, dx, dt, parm,
+                                   dir);
+    compute_HLLC_flux_LR(bx, flux, half_R, half_L, parm, dir);
+}
+
+// ----------------------------------------------------------------- level
+struct Level
+{
+    Parm             parm;
+    int              D, ng;
+    int              dlo[3], dhi[3];
+    int              bclo[3], bchi[3]; // 0 transmissive, 1 periodic, 2 reflective
+    std::vector<Box> valid;
+    std::vector<Fab> fabs;
+};
+
+static bool inside(const Box &b, int i, int j, int k)
+{
+    return i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1]
+           && k >= b.lo[2] && k <= b.hi[2];
+}
+
+// FillBoundary(periodicity) + PhysBCFunct, BCs.cpp:429-442.
+// FillBoundary copies valid data of every box (and of its periodic images)
+// into the ghost cells of the others; PhysBCFunct then fills, direction by
+// direction, the ghost cells outside a non-periodic domain face:
+// foextrap = nearest interior value, reflect_even/odd = mirror (BCs.cpp:215-397).
+static void fill_consv_boundary(Level &L)
+{
+    const int D  = L.D;
+    const int NC = D + 2;
+    int       len[3];
+    for (int d = 0; d < 3; ++d)
+        len[d] = L.dhi[d] - L.dlo[d] + 1;
+    for (size_t b = 0; b < L.fabs.size(); ++b)
+    {
+        Fab       &F  = L.fabs[b];
+        const Box &vb = L.valid[b];
+        for_box(F.bx, [&](int i, int j, int k) {
+            if (inside(vb, i, j, k))
+                return;
+            int x[3] = { i, j, k };
+            for (int d = 0; d < D; ++d)
+            {
+                if (L.bclo[d] == 1)
+                {
+                    if (x[d] < L.dlo[d])
+                        x[d] += len[d];
+                    else if (x[d] > L.dhi[d])
+                        x[d] -= len[d];
+                }
+                if (x[d] < L.dlo[d] || x[d] > L.dhi[d])
+                    return; // physical boundary: PhysBCFunct below
+            }
+            for (size_t s = 0; s < L.valid.size(); ++s)
+                if (inside(L.valid[s], x[0], x[1], x[2]))
+                {
+                    for (int n = 0; n < NC; ++n)
+                        F(i, j, k, n) = L.fabs[s](x[0], x[1], x[2], n);
+                    return;
+                }
+        });
+        for (int d = 0; d < D; ++d)
+        {
+            if (L.bclo[d] == 1)
+                continue;
+            for_box(F.bx, [&](int i, int j, int k) {
+                int x[3] = { i, j, k };
+                int side;
+                if (x[d] < L.dlo[d])
+                    side = 0;
+                else if (x[d] > L.dhi[d])
+                    side = 1;
+                else
+                    return;
+                const int type = side ? L.bchi[d] : L.bclo[d];
+                int       y[3] = { i, j, k };
+                bool      odd  = false;
+                if (type == 2)
+                {
+                    y[d] = side ? 2 * L.dhi[d] + 1 - x[d] : 2 * L.dlo[d] - 1 - x[d];
+                    odd  = true;
+                }
+                else
+                    y[d] = side ? L.dhi[d] : L.dlo[d];
+                for (int n = 0; n < NC; ++n)
+                {
+                    double v = F(y[0], y[1], y[2], n);
+                    if (odd && n == 1 + d)
+                        v = -v;
+                    F(i, j, k, n) = v;
+                }
+            });
+        }
+    }
+}
+
+// one sweep of advance_explicit (AmrLevelAdv_advance.cpp:275-357) over all
+// boxes, writing the valid cells of out
+static void sweep(Level &L, std::vector<Fab> &in, std::vector<Fab> &out,
+                  int d, double dt, const double dx[3], int num_method)
+{
+    const int D     = L.D;
+    const int ncomp = D + 2;
+    const int iOff = (d == 0), jOff = (d == 1), kOff = (d == 2);
+#pragma omp parallel for schedule(dynamic)
+    for (long b = 0; b < (long)L.valid.size(); ++b)
+    {
+        const Box &bx = L.valid[b];
+        Fab        flux;
+        flux.define(surrounding_nodes(bx, d), ncomp);
+        if (num_method == 0)
+            compute_MUSCL_hancock_flux(bx, flux, in[b], dx, dt, L.parm, d);
+        else
+            compute_HLLC_flux_LR(bx, flux, in[b], in[b], L.parm, d);
+        const Fab &Uin  = in[b];
+        Fab       &Uout = out[b];
+        for (int n = 0; n < ncomp; ++n)
+            for_box(bx, [&](int i, int j, int k) {
+                Uout(i, j, k, n)
+                    = Uin(i, j, k, n)
+                      - (dt / dx[d])
+                            * (flux(i + iOff, j + jOff, k + kOff, n)
+                               - flux(i, j, k, n));
+            });
+    }
+}
+
+// max_wave_speed  Euler.cpp:46-74
+static double max_wave_speed(const Level &L)
+{
+    double       ws = 0; // identity of ReduceOpMax over non-negative values
+    const double a = L.parm.adiabatic, e = L.parm.epsilon;
+    const int    D = L.D;
+    for (size_t b = 0; b < L.valid.size(); ++b)
+        for_box(L.valid[b], [&](int i, int j, int k) {
+            const Vec    q   = primv_from_consv(L.fabs[b], a, e, i, j, k, D);
+            double       vsq = q[1] * q[1];
+            for (int d = 1; d < D; ++d)
+                vsq = vsq + q[1 + d] * q[1 + d];
+            const double velocity = std::sqrt(vsq);
+            const double s = std::sqrt(a * q[1 + D] / (q[0] * e)) + velocity;
+            ws             = std::max(ws, s);
+        });
+    return ws;
+}
+
+// estTimeStep (AmrLevelAdv.cpp:395-477) + computeNewDt clamp (:589-618)
+static double compute_dt(const Level &L, const double dx[3], double cfl,
+                         double cur_time, double stop_time)
+{
+    const double max_speed = max_wave_speed(L);
+    double       dt_est    = 1.0e20;
+    for (int d = 0; d < L.D; ++d)
+        dt_est = std::min(dt_est, dx[d] / max_speed);
+    dt_est *= cfl;
+    double       dt_0 = std::min(1.0e+100, 1 * dt_est);
+    const double eps  = 0.001 * dt_0;
+    if (stop_time >= 0.0)
+        if ((cur_time + dt_0) > (stop_time - eps))
+            dt_0 = stop_time - cur_time;
+    return dt_0 / 1;
+}
+
+static void advance_explicit(Level &L, double dt, const double dx[3],
+                             int num_method)
+{
+    std::vector<Fab> tmp = L.fabs; // Sborder2 = copy of Sborder (:121)
+    std::vector<Fab> *in = &L.fabs, *out = &tmp;
+    for (int d = 0; d < L.D; ++d)
+    {
+        sweep(L, *in, *out, d, dt, dx, num_method);
+        std::swap(L.fabs, *out); // fill boundary needs the level view
+        fill_consv_boundary(L);
+        std::swap(L.fabs, *out);
+        std::swap(in, out);
+    }
+    if (in != &L.fabs)
+        L.fabs = *in;
+}
+
+} // namespace oracle
+
+// ===================================================================== C ABI
+using namespace oracle;
+
+extern "C" {
+
+typedef struct
+{
+    int    dim;
+    int    num_method;
+    double adiabatic;
+    double epsilon;
+    int    limiter;
+    int    limiter_indices[5];
+    int    limit_on_primv[5];
+} or_params;
+
+static Parm make_parm(const or_params *p)
+{
+    Parm m;
+    m.D                           = p->dim;
+    m.adiabatic                   = p->adiabatic;
+    m.epsilon                     = p->epsilon;
+    m.limiter                     = p->limiter;
+    m.need_primitive_for_limiting = false;
+    for (int n = 0; n < 5; ++n)
+    {
+        m.limiter_indices[n] = p->limiter_indices[n];
+        m.limit_on_primv[n]  = p->limit_on_primv[n] != 0;
+        if (n < p->dim + 2)
+            m.need_primitive_for_limiting
+                = m.need_primitive_for_limiting || m.limit_on_primv[n];
+    }
+    return m;
+}
+
+// Level from the packed layout shared with the GPU library:
+// boxes: nbox x 6 ints (lo0 lo1 lo2 hi0 hi1 hi2), offsets: FAB starts.
+static void make_level(Level &L, const or_params *p, const int *boxes,
+                       int nbox, int ng, const int *dlo, const int *dhi,
+                       const int *bclo, const int *bchi)
+{
+    L.parm = make_parm(p);
+    L.D    = p->dim;
+    L.ng   = ng;
+    for (int d = 0; d < 3; ++d)
+    {
+        L.dlo[d]  = d < L.D ? dlo[d] : 0;
+        L.dhi[d]  = d < L.D ? dhi[d] : 0;
+        L.bclo[d] = bclo[d];
+        L.bchi[d] = bchi[d];
+    }
+    L.valid.resize(nbox);
+    L.fabs.resize(nbox);
+    for (int b = 0; b < nbox; ++b)
+    {
+        Box v;
+        for (int d = 0; d < 3; ++d)
+        {
+            v.lo[d] = d < L.D ? boxes[6 * b + d] : 0;
+            v.hi[d] = d < L.D ? boxes[6 * b + 3 + d] : 0;
+        }
+        L.valid[b] = v;
+        L.fabs[b].define(grow(v, L.D, ng), L.D + 2);
+    }
+}
+
+static void level_in(Level &L, const double *U, const int64_t *offsets)
+{
+    for (size_t b = 0; b < L.fabs.size(); ++b)
+        std::memcpy(L.fabs[b].a.data(), U + offsets[b],
+                    sizeof(double) * L.fabs[b].a.size());
+}
+
+static void level_out(const Level &L, double *U, const int64_t *offsets)
+{
+    for (size_t b = 0; b < L.fabs.size(); ++b)
+        std::memcpy(U + offsets[b], L.fabs[b].a.data(),
+                    sizeof(double) * L.fabs[b].a.size());
+}
+
+// The main.cpp loop on a single level.  U: packed level (in/out).
+int or_run(const or_params *p, const int *boxes, int nbox, int ng,
+           const int *dlo, const int *dhi, const int *bclo, const int *bchi,
+           const double *dx, double cfl, double start_time, double stop_time,
+           int max_step, double *U, const int64_t *offsets, double *time_out,
+           int *steps_out)
+{
+    Level L;
+    make_level(L, p, boxes, nbox, ng, dlo, dhi, bclo, bchi);
+    level_in(L, U, offsets);
+    fill_consv_boundary(L);
+    double time  = start_time;
+    int    steps = 0;
+    while ((steps < max_step || max_step < 0)
+           && (time < stop_time || stop_time < 0.0))
+    {
+        const double dt = compute_dt(L, dx, cfl, time, stop_time);
+        advance_explicit(L, dt, dx, p->num_method);
+        time = time + dt;
+        ++steps;
+    }
+    level_out(L, U, offsets);
+    if (time_out)
+        *time_out = time;
+    if (steps_out)
+        *steps_out = steps;
+    return 0;
+}
+
+// One sweep of direction dir: Uout valid cells <- sweep(Uin).  Ghosts of Uin
+// must be filled; Uout ghost cells are left unchanged.
+int or_sweep(const or_params *p, const int *boxes, int nbox, int ng,
+             const int *dlo, const int *dhi, const int *bclo, const int *bchi,
+             int dir, double dt, const double *dx, const double *Uin,
+             double *Uout, const int64_t *offsets)
+{
+    Level L;
+    make_level(L, p, boxes, nbox, ng, dlo, dhi, bclo, bchi);
+    level_in(L, Uin, offsets);
+    std::vector<Fab> out(L.fabs.size());
+    for (size_t b = 0; b < out.size(); ++b)
+    {
+        out[b] = L.fabs[b];
+        std::memcpy(out[b].a.data(), Uout + offsets[b],
+                    sizeof(double) * out[b].a.size());
+    }
+    sweep(L, L.fabs, out, dir, dt, dx, p->num_method);
+    L.fabs = out;
+    level_out(L, Uout, offsets);
+    return 0;
+}
+
+int or_fill_boundary(const or_params *p, const int *boxes, int nbox, int ng,
+                     const int *dlo, const int *dhi, const int *bclo,
+                     const int *bchi, double *U, const int64_t *offsets)
+{
+    Level L;
+    make_level(L, p, boxes, nbox, ng, dlo, dhi, bclo, bchi);
+    level_in(L, U, offsets);
+    fill_consv_boundary(L);
+    level_out(L, U, offsets);
+    return 0;
+}
+
+int or_max_wave_speed(const or_params *p, const int *boxes, int nbox, int ng,
+                      const double *U, const int64_t *offsets, double *out)
+{
+    int   z[3] = { 0, 0, 0 };
+    Level L;
+    make_level(L, p, boxes, nbox, ng, z, z, z, z);
+    level_in(L, U, offsets);
+    *out = max_wave_speed(L);
+    return 0;
+}
+
+int or_est_dt(const or_params *p, const int *boxes, int nbox, int ng,
+              const double *U, const int64_t *offsets, const double *dx,
+              double cfl, double time, double stop_time, double *out)
+{
+    int   z[3] = { 0, 0, 0 };
+    Level L;
+    make_level(L, p, boxes, nbox, ng, z, z, z, z);
+    level_in(L, U, offsets);
+    *out = compute_dt(L, dx, cfl, time, stop_time);
+    return 0;
+}
+
+// Face fluxes of one box on surroundingNodes(bx, dir) (Array4 without ghosts,
+// NC components), as compute_MUSCL_hancock_flux / compute_HLLC_flux write them.
+int or_flux_box(const or_params *p, const int *box, int ng, int dir, double dt,
+                const double *dx, const double *Ufab, double *flux)
+{
+    const int D = p->dim;
+    Box       v;
+    for (int d = 0; d < 3; ++d)
+    {
+        v.lo[d] = d < D ? box[d] : 0;
+        v.hi[d] = d < D ? box[3 + d] : 0;
+    }
+    Parm parm = make_parm(p);
+    Fab  U;
+    U.define(grow(v, D, ng), D + 2);
+    std::memcpy(U.a.data(), Ufab, sizeof(double) * U.a.size());
+    Fab F;
+    F.define(surrounding_nodes(v, dir), D + 2);
+    if (p->num_method == 0)
+        compute_MUSCL_hancock_flux(v, F, U, dx, dt, parm, dir);
+    else
+        compute_HLLC_flux_LR(v, F, U, U, parm, dir);
+    std::memcpy(flux, F.a.data(), sizeof(double) * F.a.size());
+    return 0;
+}
+
+// ---- scalar entry points for known-answer tests (2-cell FABs as in
+// Tests/hllc_test.H; dir 0, face at i = 1)
+int or_wave_speed_toro(int dim, const double *primv_L, const double *primv_R,
+                       double adiabatic, double epsilon, double *out3)
+{
+    Box b{};
+    b.hi[0] = 1;
+    Fab L, R;
+    L.define(b, dim + 2);
+    R.define(b, dim + 2);
+    Vec qL{}, qR{};
+    for (int n = 0; n < dim + 2; ++n)
+    {
+        qL[n] = primv_L[n];
+        qR[n] = primv_R[n];
+    }
+    const Vec cL = consv_from_primv(qL, adiabatic, epsilon, dim);
+    const Vec cR = consv_from_primv(qR, adiabatic, epsilon, dim);
+    for (int i = 0; i < 2; ++i)
+        for (int n = 0; n < dim + 2; ++n)
+        {
+            L(i, 0, 0, n) = cL[n];
+            R(i, 0, 0, n) = cR[n];
+        }
+    estimate_wave_speed_HLLC_Toro(L, R, 1, 0, 0, adiabatic, adiabatic, epsilon,
+                                  0, dim, out3[0], out3[1], out3[2]);
+    return 0;
+}
+
+int or_consv_from_primv(int dim, const double *q, double adiabatic,
+                        double epsilon, double *u)
+{
+    Vec qq{};
+    for (int n = 0; n < dim + 2; ++n)
+        qq[n] = q[n];
+    const Vec c = consv_from_primv(qq, adiabatic, epsilon, dim);
+    for (int n = 0; n < dim + 2; ++n)
+        u[n] = c[n];
+    return 0;
+}
+
+int or_primv_from_consv(int dim, const double *u, double adiabatic,
+                        double epsilon, double *q)
+{
+    Box b{};
+    Fab U;
+    U.define(b, dim + 2);
+    for (int n = 0; n < dim + 2; ++n)
+        U(0, 0, 0, n) = u[n];
+    const Vec r = primv_from_consv(U, adiabatic, epsilon, 0, 0, 0, dim);
+    for (int n = 0; n < dim + 2; ++n)
+        q[n] = r[n];
+    return 0;
+}
+
+// HLLC face flux between conserved states UL (cell i-1) and UR (cell i) in
+// direction dir, through compute_HLLC_flux_LR on two-cell FABs
+int or_hllc_face(int dim, int dir, const double *UL, const double *UR,
+                 double adiabatic, double epsilon, double *flux)
+{
+    Box b{};
+    b.hi[dir] = 1;
+    Fab L, R;
+    L.define(b, dim + 2);
+    R.define(b, dim + 2);
+    const int io = dir == 0, jo = dir == 1, ko = dir == 2;
+    for (int n = 0; n < dim + 2; ++n)
+    {
+        L(0, 0, 0, n)    = UL[n];
+        L(io, jo, ko, n) = UL[n];
+        R(0, 0, 0, n)    = UR[n];
+        R(io, jo, ko, n) = UR[n];
+    }
+    Parm pm{};
+    pm.D         = dim;
+    pm.adiabatic = adiabatic;
+    pm.epsilon   = epsilon;
+    Box cellbox{}; // zero-width cell box whose surroundingNodes is face 1
+    cellbox.lo[dir] = 1;
+    cellbox.hi[dir] = 0;
+    Fab F;
+    F.define(surrounding_nodes(cellbox, dir), dim + 2);
+    compute_HLLC_flux_LR(cellbox, F, L, R, pm, dir);
+    for (int n = 0; n < dim + 2; ++n)
+        flux[n] = F(io, jo, ko, n);
+    return 0;
+}
+
+double or_limiter(int kind, double r) { return slope_limit(kind, r); }
+double or_minmod(double a, double b) { return minmod(a, b); }
+double or_pow_fn(double x, double y) { return or_pow(x, y); }
+
+} // extern "C"

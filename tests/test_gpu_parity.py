@@ -1,0 +1,354 @@
+"""GPU parity of the HIP hot path against the CPU oracle (through the C ABI).
+
+Bar: bit-exact against the oracle built with the shared portable pow
+(liboracle_bxpow.so) -- every other operation is IEEE-identical with
+contraction off -- and <= 1e-13 relative L-inf per conserved variable
+against the oracle that calls glibc's pow as the reference does.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from bamrex_amd import problems
+from bamrex_amd.level import Layout, Level, chop_domain, make_params
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13
+
+
+def rel_linf(a, b, layout, nc):
+    """max over components of max|a-b| / max|b| over valid cells."""
+    worst = 0.0
+    for n in range(nc):
+        num = 0.0
+        den = 0.0
+        for bb in range(len(layout.boxes)):
+            va = layout.valid(a, bb)[n]
+            vb = layout.valid(b, bb)[n]
+            num = max(num, float(np.max(np.abs(va - vb))))
+            den = max(den, float(np.max(np.abs(vb))))
+        worst = max(worst, num / den if den > 0 else num)
+    return worst
+
+
+def assert_valid_equal(a, b, layout):
+    for bb in range(len(layout.boxes)):
+        va = layout.valid(a, bb)
+        vb = layout.valid(b, bb)
+        if not np.array_equal(va, vb):
+            d = np.argwhere(va != vb)
+            raise AssertionError("box %d differs at %d cells, first %s: gpu %r oracle %r" % (
+                bb, len(d), d[0], va[tuple(d[0])], vb[tuple(d[0])]))
+
+
+def setup_params(s, num_method="muscl_hancock"):
+    dim = s["layout"].dim
+    kw = dict(num_method=num_method, limiter=s["limiter"],
+              limiter_indices=s.get("limiter_indices"), limit_on_primv=s.get("limit_on_primv"))
+    return (make_params(dim, s["adiabatic"], s["epsilon"], **kw),
+            pyoracle.params(dim, s["adiabatic"], s["epsilon"], **kw))
+
+
+def random_level(dim, n, mgs, seed, ng=2, shocks=True):
+    boxes = chop_domain((0,) * dim, (n - 1,) * dim, mgs)
+    lay = Layout(dim, boxes, ng)
+    U = lay.new_array()
+    rng = np.random.default_rng(seed)
+    shape = (n,) * dim
+    rho = rng.uniform(0.5, 2.0, shape)
+    p = rng.uniform(0.5, 2.0, shape)
+    vel = [rng.uniform(-1, 1, shape) for _ in range(dim)]
+    if shocks:  # flat patches so the limiter special cases (0/0, x/0) occur
+        sl = tuple(slice(0, n // 2) for _ in range(dim))
+        rho[sl] = 1.0
+        p[sl] = 1.0
+        for v in vel:
+            v[sl] = 0.25
+    full = problems.consv_from_primv([rho] + vel + [p], 1.4, 1.0, dim)
+    for b, (lo, hi) in enumerate(boxes):
+        sl = tuple(slice(lo[d], hi[d] + 1) for d in reversed(range(dim)))
+        for c in range(dim + 2):
+            lay.valid(U, b)[c][...] = full[c][sl]  # arrays are (z, y, x) ordered
+    return lay, U
+
+
+def test_library_loads_and_device_present():
+    import ctypes
+    from bamrex_amd import _lib
+    n = ctypes.c_int()
+    _lib.check(_lib.lib().bx_device_count(ctypes.byref(n)), "bx_device_count")
+    assert n.value >= 1
+    buf = ctypes.create_string_buffer(256)
+    _lib.check(_lib.lib().bx_device_name(buf, 256), "bx_device_name")
+    announce("device " + buf.value.decode())
+
+
+LIMITERS = ["minbee", "van_leer", "van_albada", "superbee"]
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("limiter", LIMITERS)
+def test_sweep_bit_exact(oracle_bxpow, oracle_glibc, dim, limiter):
+    n = {1: 200, 2: 40, 3: 12}[dim]
+    mgs = {1: 64, 2: 16, 3: 6}[dim]
+    lay, U = random_level(dim, n, mgs, seed=dim * 10 + len(limiter))
+    dom_lo, dom_hi = (0,) * dim, (n - 1,) * dim
+    bcs = ["transmissive"] * dim
+    gp = make_params(dim, 1.4, 1.0, limiter=limiter)
+    op = pyoracle.params(dim, 1.4, 1.0, limiter=limiter)
+    oracle_glibc.fill_boundary(op, lay, U, dom_lo, dom_hi, bcs, bcs)
+    dx = (1.0 / n,) * dim
+    dt = 0.3 / n
+    lev = Level(gp, lay, dom_lo, dom_hi, bcs, bcs)
+    try:
+        lev.set_state(U)
+        lev.set_tmp(U)
+        for d in range(dim):
+            announce("sweep dim=%d limiter=%s dir=%d" % (dim, limiter, d))
+            lev.sweep(d, dt, dx)
+            got = lev.get_state("tmp")
+            ref = U.copy()
+            oracle_bxpow.sweep(op, lay, U, ref, d, dt, dx, dom_lo, dom_hi, bcs, bcs)
+            assert_valid_equal(got, ref, lay)
+            ref2 = U.copy()
+            oracle_glibc.sweep(op, lay, U, ref2, d, dt, dx, dom_lo, dom_hi, bcs, bcs)
+            assert rel_linf(got, ref2, lay, dim + 2) <= TOL
+    finally:
+        lev.close()
+
+
+@pytest.mark.parametrize("mode", ["own", "table_prim", "first_order"])
+def test_sweep_limiting_modes(oracle_bxpow, mode):
+    dim, n = 2, 36
+    lay, U = random_level(dim, n, 12, seed=99)
+    dom_lo, dom_hi = (0, 0), (n - 1, n - 1)
+    bcs = ["periodic"] * 2
+    kw = {}
+    num = "muscl_hancock"
+    if mode == "own":
+        kw = dict(limiter_indices=[0, 1, 2, 3], limit_on_primv=[0, 0, 0, 0])
+    elif mode == "table_prim":
+        kw = dict(limiter_indices=[0, 1, 2, 3], limit_on_primv=[0, 1, 1, 1])  # u_x, u_y, p
+    else:
+        num = "hllc"
+    gp = make_params(dim, 1.4, 1.0, num_method=num, limiter="van_albada", **kw)
+    op = pyoracle.params(dim, 1.4, 1.0, num_method=num, limiter="van_albada", **kw)
+    oracle_bxpow.fill_boundary(op, lay, U, dom_lo, dom_hi, bcs, bcs)
+    lev = Level(gp, lay, dom_lo, dom_hi, bcs, bcs)
+    try:
+        lev.set_state(U)
+        lev.set_tmp(U)
+        for d in range(dim):
+            announce("sweep mode=%s dir=%d" % (mode, d))
+            lev.sweep(d, 0.2 / n, (1.0 / n,) * 2)
+            ref = U.copy()
+            oracle_bxpow.sweep(op, lay, U, ref, d, 0.2 / n, (1.0 / n,) * 2, dom_lo, dom_hi,
+                               bcs, bcs)
+            assert_valid_equal(lev.get_state("tmp"), ref, lay)
+    finally:
+        lev.close()
+
+
+@pytest.mark.parametrize("bc", ["transmissive", "reflective", "periodic"])
+def test_fill_boundary_matches_oracle(oracle_bxpow, bc):
+    dim, n = 2, 20
+    lay, U = random_level(dim, n, 7, seed=3)
+    dom_lo, dom_hi = (0, 0), (n - 1, n - 1)
+    gp = make_params(dim, 1.4)
+    op = pyoracle.params(dim, 1.4)
+    lev = Level(gp, lay, dom_lo, dom_hi, [bc] * 2, [bc] * 2)
+    try:
+        lev.set_state(U)
+        announce("fill_boundary bc=%s" % bc)
+        lev.fill_boundary()
+        got = lev.get_state()
+        ref = U.copy()
+        oracle_bxpow.fill_boundary(op, lay, ref, dom_lo, dom_hi, [bc] * 2, [bc] * 2)
+        g = lay.ng
+        for b in range(len(lay.boxes)):
+            fa, fr = lay.fab(got, b), lay.fab(ref, b)
+            ny, nx = fa.shape[1] - 2 * g, fa.shape[2] - 2 * g
+            # face ghosts (the ones the sweeps read); corners are not read
+            np.testing.assert_array_equal(fa[:, g:g + ny, :], fr[:, g:g + ny, :])
+            np.testing.assert_array_equal(fa[:, :, g:g + nx], fr[:, :, g:g + nx])
+    finally:
+        lev.close()
+
+
+def test_max_wave_speed_and_dt(oracle_glibc):
+    s = problems.quadrant_setup(ncell=64, max_grid_size=16)
+    gp, op = setup_params(s)
+    lay = s["layout"]
+    lev = Level(gp, lay, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"])
+    try:
+        lev.set_state(s["U"])
+        announce("max_wave_speed")
+        assert lev.max_wave_speed() == oracle_glibc.max_wave_speed(op, lay, s["U"])
+        assert lev.estTimeStep(s["dx"], 0.8) == oracle_glibc.est_dt(op, lay, s["U"], s["dx"], 0.8)
+        assert lev.estTimeStep(s["dx"], 0.8, 0.1, 0.1001) == oracle_glibc.est_dt(
+            op, lay, s["U"], s["dx"], 0.8, 0.1, 0.1001)
+    finally:
+        lev.close()
+
+
+def _run_both(s, oracle, max_step=-1, stop_time=-1.0, use_graph=False, num_method="muscl_hancock"):
+    gp, op = setup_params(s, num_method)
+    lay = s["layout"]
+    lev = Level(gp, lay, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"])
+    try:
+        lev.set_state(s["U"])
+        t_g, n_g = lev.run(s["dx"], s["cfl"], stop_time=stop_time, max_step=max_step,
+                           use_graph=use_graph)
+        got = lev.get_state()
+    finally:
+        lev.close()
+    ref = s["U"].copy()
+    t_o, n_o = oracle.run(op, lay, ref, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"],
+                          s["dx"], s["cfl"], stop_time=stop_time, max_step=max_step)
+    return got, ref, (t_g, n_g), (t_o, n_o)
+
+
+def test_sod_run_parity(oracle_bxpow, oracle_glibc):
+    """Config 1: Sod, 1024 cells, van Leer, CFL 0.9, to t = 0.25."""
+    announce("sod 1024 to t=0.25")
+    s = problems.sod_setup(ncell=1024, max_grid_size=256)
+    got, ref, g, o = _run_both(s, oracle_bxpow, stop_time=0.25, max_step=100000)
+    assert g == o
+    assert_valid_equal(got, ref, s["layout"])
+    ref2 = s["U"].copy()
+    gp, op = setup_params(s)
+    oracle_glibc.run(op, s["layout"], ref2, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"],
+                     s["dx"], s["cfl"], stop_time=0.25, max_step=100000)
+    assert rel_linf(got, ref2, s["layout"], 3) <= TOL
+
+
+def test_quadrant_run_parity(oracle_bxpow, oracle_glibc):
+    """Config 2 at 256^2 (mgs 64), 40 steps."""
+    announce("quadrant 256^2 40 steps")
+    s = problems.quadrant_setup(ncell=256, max_grid_size=64)
+    got, ref, g, o = _run_both(s, oracle_bxpow, max_step=40)
+    assert g == o
+    assert_valid_equal(got, ref, s["layout"])
+    s2 = problems.quadrant_setup(ncell=256, max_grid_size=64)
+    gp, op = setup_params(s2)
+    oracle_glibc.run(op, s2["layout"], s2["U"], s2["domain_lo"], s2["domain_hi"], s2["bc"],
+                     s2["bc"], s2["dx"], s2["cfl"], max_step=40)
+    assert rel_linf(got, s2["U"], s2["layout"], 4) <= TOL
+
+
+def test_graph_run_equals_host_loop():
+    announce("quadrant 128^2 graph vs host loop")
+    out = []
+    for use_graph in (False, True):
+        s = problems.quadrant_setup(ncell=128, max_grid_size=32)
+        gp, _ = setup_params(s)
+        lev = Level(gp, s["layout"], s["domain_lo"], s["domain_hi"], s["bc"], s["bc"])
+        stream = None
+        try:
+            import ctypes
+            from bamrex_amd import _lib
+            st = ctypes.c_void_p()
+            _lib.check(_lib.lib().bx_stream_create(ctypes.byref(st)), "stream")
+            lev.stream = st
+            lev.set_state(s["U"])
+            lev.run(s["dx"], s["cfl"], max_step=12, use_graph=use_graph)
+            out.append(lev.get_state())
+            stream = st
+        finally:
+            lev.close()
+            if stream:
+                _lib.lib().bx_stream_destroy(stream)
+    assert np.array_equal(out[0], out[1])
+
+
+def test_vortex_3d_run_parity(oracle_bxpow):
+    """Config 3 at 24^3 periodic (mgs 12), superbee limiting each variable on
+    itself, 4 steps."""
+    announce("vortex 3d 24^3")
+    s = problems.vortex_setup(ncell=24, dim=3, max_grid_size=12)
+    got, ref, g, o = _run_both(s, oracle_bxpow, max_step=4)
+    assert g == o
+    assert_valid_equal(got, ref, s["layout"])
+
+
+def test_reflective_hllc_first_order_parity(oracle_bxpow):
+    announce("reflective first-order hllc")
+    s = problems.quadrant_setup(ncell=48, max_grid_size=16)
+    s["bc"] = ["reflective", "reflective"]
+    got, ref, g, o = _run_both(s, oracle_bxpow, max_step=10, num_method="hllc")
+    assert g == o
+    assert_valid_equal(got, ref, s["layout"])
+
+
+def test_flux_box_drop_in(oracle_bxpow):
+    """bx_flux_box writes the same face fluxes as compute_MUSCL_hancock_flux."""
+    from bamrex_amd.level import DeviceBuffer
+    dim, n = 2, 24
+    lay, U = random_level(dim, n, 12, seed=21)
+    dom = ((0, 0), (n - 1, n - 1))
+    bcs = ["transmissive"] * 2
+    gp = make_params(dim, 1.4, limiter="van_leer")
+    op = pyoracle.params(dim, 1.4, limiter="van_leer")
+    oracle_bxpow.fill_boundary(op, lay, U, dom[0], dom[1], bcs, bcs)
+    lev = Level(gp, lay, dom[0], dom[1], bcs, bcs)
+    try:
+        lev.set_state(U)
+        for d in range(dim):
+            for b in range(len(lay.boxes)):
+                announce("flux_box box=%d dir=%d" % (b, d))
+                ref = oracle_bxpow.flux_box(op, lay, U, b, d, 0.01, (1.0 / n,) * 2)
+                fb = DeviceBuffer(ref.nbytes)
+                lev.flux_box(b, d, 0.01, (1.0 / n,) * 2, fb)
+                got = np.zeros_like(ref)
+                fb.download(got)
+                fb.free()
+                np.testing.assert_array_equal(got, ref)
+    finally:
+        lev.close()
+
+
+def test_uneven_boxes_one_sweep_parity(oracle_bxpow):
+    """Config 4 at 60 boxes: one sweep per direction over uneven boxes."""
+    s = problems.uneven_setup(nboxes=60)
+    gp, op = setup_params(s)
+    lay = s["layout"]
+    lev = Level(gp, lay, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"])
+    try:
+        lev.set_state(s["U"])
+        lev.set_tmp(s["U"])
+        for d in range(2):
+            announce("uneven boxes dir=%d" % d)
+            lev.sweep(d, 2e-4, s["dx"])
+            ref = s["U"].copy()
+            oracle_bxpow.sweep(op, lay, s["U"], ref, d, 2e-4, s["dx"], s["domain_lo"],
+                               s["domain_hi"], s["bc"], s["bc"])
+            assert_valid_equal(lev.get_state("tmp"), ref, lay)
+    finally:
+        lev.close()
+
+
+def test_full_size_quadrant_decomposition_invariance():
+    """Config 2 at its full size (4096^2): the result of 3 steps is identical
+    for max_grid_size 128 and 512 (size-independent property)."""
+    res = []
+    for mgs in (128, 512):
+        announce("quadrant 4096^2 mgs=%d 3 steps" % mgs)
+        s = problems.quadrant_setup(ncell=4096, max_grid_size=mgs)
+        gp, _ = setup_params(s)
+        lev = Level(gp, s["layout"], s["domain_lo"], s["domain_hi"], s["bc"], s["bc"])
+        try:
+            lev.set_state(s["U"])
+            lev.run(s["dx"], s["cfl"], max_step=3, use_graph=False)
+            U = lev.get_state()
+        finally:
+            lev.close()
+        lay = s["layout"]
+        full = np.zeros((4, 4096, 4096))
+        for b, (lo, hi) in enumerate(lay.boxes):
+            full[:, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = lay.valid(U, b)
+        res.append(full)
+        del s, U
+    assert np.array_equal(res[0], res[1])

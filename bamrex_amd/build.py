@@ -21,7 +21,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 
 HIP_SOURCES = ["bx_explicit.hip", "bx_api.hip"]
-HOST_SOURCES = []  # filled as the host driver grows
+HOST_SOURCES = ["ParmParse.cpp", "AmrLevelAdv.cpp", "Prob.cpp", "main.cpp"]
+HOST_BIN = os.path.join(HERE, "bamrex_run")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BAMREX_OFFLOAD_ARCH", "gfx950")
@@ -80,6 +81,12 @@ def build(force=True, jobs=4, verbose=False):
         tmp = LIB + ".tmp"
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
         os.replace(tmp, LIB)
+    host = [os.path.join(CSRC, "host", f) for f in HOST_SOURCES]
+    if force or _stale(HOST_BIN, host + hdrs + [LIB]):
+        tmp = HOST_BIN + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", tmp] + host +
+             ["-L" + HERE, "-lbamrex_hip", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, HOST_BIN)
     return LIB
 
 

@@ -212,31 +212,37 @@ def _run_both(s, oracle, max_step=-1, stop_time=-1.0, use_graph=False, num_metho
 
 
 def test_sod_run_parity(oracle_bxpow, oracle_glibc):
-    """Config 1: Sod, 1024 cells, van Leer, CFL 0.9, to t = 0.25."""
+    """Config 1: Sod, 1024 cells, van Leer, CFL 0.9, to t = 0.25: bit-exact
+    against the oracle over the whole run; against the glibc-pow oracle to
+    1e-13 over the first 100 steps (after that the two libm pows have
+    disagreed in the last bit somewhere and the scheme amplifies that
+    difference at the contact -- see test_oracle.py::
+    test_scheme_amplifies_one_ulp_perturbations)."""
     announce("sod 1024 to t=0.25")
     s = problems.sod_setup(ncell=1024, max_grid_size=256)
     got, ref, g, o = _run_both(s, oracle_bxpow, stop_time=0.25, max_step=100000)
-    assert g == o
+    assert g == o and abs(g[0] - 0.25) < 1e-15
     assert_valid_equal(got, ref, s["layout"])
-    ref2 = s["U"].copy()
-    gp, op = setup_params(s)
-    oracle_glibc.run(op, s["layout"], ref2, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"],
-                     s["dx"], s["cfl"], stop_time=0.25, max_step=100000)
-    assert rel_linf(got, ref2, s["layout"], 3) <= TOL
+    announce("sod 1024, 100 steps vs glibc-pow oracle")
+    s2 = problems.sod_setup(ncell=1024, max_grid_size=256)
+    got2, ref2, g2, o2 = _run_both(s2, oracle_glibc, stop_time=0.25, max_step=100)
+    assert g2 == o2
+    assert rel_linf(got2, ref2, s2["layout"], 3) <= TOL
 
 
 def test_quadrant_run_parity(oracle_bxpow, oracle_glibc):
-    """Config 2 at 256^2 (mgs 64), 40 steps."""
+    """Config 2 at 256^2 (mgs 64): 40 steps bit-exact against the oracle;
+    10 steps within 1e-13 of the glibc-pow oracle."""
     announce("quadrant 256^2 40 steps")
     s = problems.quadrant_setup(ncell=256, max_grid_size=64)
     got, ref, g, o = _run_both(s, oracle_bxpow, max_step=40)
     assert g == o
     assert_valid_equal(got, ref, s["layout"])
+    announce("quadrant 256^2 10 steps vs glibc-pow oracle")
     s2 = problems.quadrant_setup(ncell=256, max_grid_size=64)
-    gp, op = setup_params(s2)
-    oracle_glibc.run(op, s2["layout"], s2["U"], s2["domain_lo"], s2["domain_hi"], s2["bc"],
-                     s2["bc"], s2["dx"], s2["cfl"], max_step=40)
-    assert rel_linf(got, s2["U"], s2["layout"], 4) <= TOL
+    got2, ref2, g2, o2 = _run_both(s2, oracle_glibc, max_step=10)
+    assert g2 == o2
+    assert rel_linf(got2, ref2, s2["layout"], 4) <= TOL
 
 
 def test_graph_run_equals_host_loop():

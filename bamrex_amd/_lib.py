@@ -98,6 +98,7 @@ _SIGS = {
     "bx_event_record": (ctypes.c_int, [P, P]),
     "bx_event_elapsed_ms": (ctypes.c_int, [P, P, ctypes.POINTER(ctypes.c_float)]),
     "bx_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "bx_copy_kernel": (ctypes.c_int, [P, P, ctypes.c_size_t, P]),
     "bx_last_error": (ctypes.c_char_p, []),
 }
 

@@ -124,6 +124,17 @@ __global__ __launch_bounds__(256) void k_update_box(
     }
 }
 
+// Plain streaming copy (calibrates the HBM byte counters against a kernel
+// of known traffic: reads and writes `n` doubles once).
+__global__ __launch_bounds__(256) void k_copy(const double *__restrict__ src,
+                                              double *__restrict__ dst,
+                                              long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void k_set_f64(double *p, double v)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0)
@@ -1314,6 +1325,18 @@ extern "C" int bx_event_elapsed_ms(void *start, void *stop, float *ms)
     BX_HIP_CHECK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
     return BX_OK;
 }
+extern "C" int bx_copy_kernel(void *dst, const void *src, size_t bytes,
+                              void *stream)
+{
+    if (!dst || !src || bytes % 8)
+        return set_error(BX_ERR_INVALID, "bx_copy_kernel: bad argument");
+    const long long n = (long long)(bytes / 8);
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, (hipStream_t)stream,
+                       (const double *)src, (double *)dst, n);
+    BX_HIP_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
 extern "C" int bx_device_name(char *buf, int len)
 {
     if (!buf || len <= 0)

@@ -44,6 +44,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--tuning", default="", help="block_x,block_t,seg_k,grid_cap (skips autotune)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run a plain copy kernel of this size (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -71,12 +74,28 @@ def main():
     lev.fill_boundary()
     dt0 = lev.estTimeStep(s["dx"], s["cfl"])
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        bx_, bt_, k_, cap_ = [int(v) for v in args.tuning.split(",")]
+        lev.set_tuning(block_x=bx_, block_t=bt_, seg_k=k_, grid_cap=cap_)
+        tuning = lev.get_tuning()
+    elif not args.no_autotune:
         log("autotuning launch shapes")
         lev.autotune(s["dx"], dt0)
         tuning = lev.get_tuning()
         log("tuning: %s" % tuning)
         lev.set_state(s["U"])
+
+    if args.copy_mib:
+        from bamrex_amd.level import DeviceBuffer
+        nb = args.copy_mib << 20
+        a, b = DeviceBuffer(nb), DeviceBuffer(nb)
+        for _ in range(3):
+            _lib.check(lib.bx_copy_kernel(b.ptr, a.ptr, nb, stream), "copy")
+        _lib.check(lib.bx_stream_sync(stream), "sync")
+        log("copy kernel: %d MiB read + %d MiB written per launch, 3 launches" % (
+            args.copy_mib, args.copy_mib))
+        a.free()
+        b.free()
 
     # ------------------------------------------------- per-kernel timing
     ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()

@@ -203,6 +203,9 @@ int         bx_event_destroy(void *ev);
 int         bx_event_record(void *ev, void *stream);
 int         bx_event_elapsed_ms(void *start, void *stop, float *ms);
 int         bx_device_name(char *buf, int len);
+/* Streaming device copy kernel (reads and writes `bytes` once); used to
+ * calibrate HBM byte counters against a kernel of known traffic. */
+int         bx_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 const char *bx_last_error(void);
 
 #ifdef __cplusplus

@@ -9,7 +9,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libbamrex_hip.so")
+LIB_PATH = os.environ.get("BAMREX_LIB") or os.path.join(_HERE, "libbamrex_hip.so")
 
 BX_OK = 0
 METHOD = {"muscl_hancock": 0, "hllc": 1}

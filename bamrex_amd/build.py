@@ -56,31 +56,37 @@ def _headers():
         os.path.join(ROOT, "include", "bamrex_hip.h")]
 
 
-def build(force=True, jobs=4, verbose=False):
+def build(force=True, jobs=4, verbose=False, defines=(), lib_path=None, obj_dir=None,
+          host=True):
     """Compile every HIP source for gfx950 and link libbamrex_hip.so.
 
     force=True recompiles everything (the graft build check); force=False only
     rebuilds stale objects."""
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found at %s" % HIPCC)
-    os.makedirs(OBJ, exist_ok=True)
+    obj = obj_dir or OBJ
+    lib_out = lib_path or LIB
+    os.makedirs(obj, exist_ok=True)
     hdrs = _headers()
+    flags = HIP_FLAGS + ["-D" + d for d in defines]
     objs = []
     jobs_list = []
     for src in HIP_SOURCES:
         s = os.path.join(CSRC, src)
-        o = os.path.join(OBJ, src.replace(".hip", ".o"))
+        o = os.path.join(obj, src.replace(".hip", ".o"))
         objs.append(o)
         if force or _stale(o, [s] + hdrs):
-            jobs_list.append([HIPCC] + HIP_FLAGS + ["-c", s, "-o", o])
+            jobs_list.append([HIPCC] + flags + ["-c", s, "-o", o])
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         for out in ex.map(_run, jobs_list):
             if verbose and out.strip():
                 print(out)
-    if force or _stale(LIB, objs):
-        tmp = LIB + ".tmp"
+    if force or _stale(lib_out, objs):
+        tmp = lib_out + ".tmp"
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
-        os.replace(tmp, LIB)
+        os.replace(tmp, lib_out)
+    if not host:
+        return lib_out
     host = [os.path.join(CSRC, "host", f) for f in HOST_SOURCES]
     if force or _stale(HOST_BIN, host + hdrs + [LIB]):
         tmp = HOST_BIN + ".tmp"

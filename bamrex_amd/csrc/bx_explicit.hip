@@ -29,6 +29,25 @@
 #include "bx_physics.h"
 #include "bx_kernels.h"
 
+// Occupancy targets of the sweep kernels (waves per SIMD; 0 = compiler's
+// choice).  Raising them trades register spills for latency hiding.
+#ifndef BX_SWEEP_X_WAVES
+#define BX_SWEEP_X_WAVES 0
+#endif
+#ifndef BX_SWEEP_T_WAVES
+#define BX_SWEEP_T_WAVES 0
+#endif
+#if BX_SWEEP_X_WAVES > 0
+#define BX_X_OCC __attribute__((amdgpu_waves_per_eu(BX_SWEEP_X_WAVES)))
+#else
+#define BX_X_OCC
+#endif
+#if BX_SWEEP_T_WAVES > 0
+#define BX_T_OCC __attribute__((amdgpu_waves_per_eu(BX_SWEEP_T_WAVES)))
+#else
+#define BX_T_OCC
+#endif
+
 namespace bx {
 
 __device__ __forceinline__ int find_box(const long long *__restrict__ begin,
@@ -86,98 +105,156 @@ __device__ __forceinline__ void wave_max_to_global(unsigned long long v,
         atomicMax(dst, v);
 }
 
+// Whole-wave lane shifts through DPP (wave_shr:1 / wave_shl:1, a GFX9
+// feature): lane t receives lane t-1 (shr) or lane t+1 (shl).  Lanes at the
+// wave ends keep their own value; those lanes never produce an update.
+__device__ __forceinline__ double wave_shr1(double v)
+{
+    const int2 w = __builtin_bit_cast(int2, v);
+    int2       r;
+    r.x = __builtin_amdgcn_update_dpp(w.x, w.x, 0x138, 0xf, 0xf, false);
+    r.y = __builtin_amdgcn_update_dpp(w.y, w.y, 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, r);
+}
+__device__ __forceinline__ double wave_shl1(double v)
+{
+    const int2 w = __builtin_bit_cast(int2, v);
+    int2       r;
+    r.x = __builtin_amdgcn_update_dpp(w.x, w.x, 0x130, 0xf, 0xf, false);
+    r.y = __builtin_amdgcn_update_dpp(w.y, w.y, 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, r);
+}
+
 // ---------------------------------------------------------------- x sweep
-template <int D, int LIM, int LV, bool FIRST_ORDER>
-__global__ __launch_bounds__(256) void k_sweep_x(
+// Geometry of one x-sweep item (64 consecutive positions of one box plane).
+struct XItem
+{
+    long long pbase; // element offset of position 0 of the plane
+    long long p;     // this lane's position
+    long long P;     // positions in the plane (valid rows x padded row)
+    long long nst;   // component stride
+    int       nx, nxd;
+};
+
+template <int D>
+__device__ __forceinline__ void x_item(const DevBox *__restrict__ boxes,
+                                       const long long *__restrict__ ibeg,
+                                       int nbox, long long item, int ng,
+                                       int lane, int &b, long long &bend,
+                                       XItem &it)
+{
+    constexpr int STEP = 60;
+    if (item >= bend)
+    {
+        b    = find_box(ibeg, nbox, item);
+        bend = ibeg[b + 1];
+    }
+    const DevBox &bx  = boxes[b];
+    const int     nx  = bx.hi[0] - bx.lo[0] + 1;
+    const int     ny  = (D >= 2) ? bx.hi[1] - bx.lo[1] + 1 : 1;
+    const int     nxd = nx + 2 * ng;
+    const long long P = (long long)ny * nxd;
+    const int       cpp   = (int)((P + STEP - 1) / STEP);
+    const unsigned  local = (unsigned)(item - ibeg[b]);
+    const int       kpl   = (int)(local / (unsigned)cpp);
+    const int       chunk = (int)(local - (unsigned)kpl * (unsigned)cpp);
+    const int       ngy   = (D >= 2) ? ng : 0;
+    const int       ngz   = (D >= 3) ? ng : 0;
+    it.pbase = bx.base + (long long)(kpl + ngz) * bx.kstride
+               + (long long)ngy * bx.jstride;
+    it.p   = (long long)chunk * STEP - 2 + lane;
+    it.P   = P;
+    it.nst = bx.nstride;
+    it.nx  = nx;
+    it.nxd = nxd;
+}
+
+template <int D, int LIM, int LV, bool FIRST_ORDER, bool E1>
+__global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     const double *__restrict__ Uin, double *__restrict__ Uout,
     const DevBox *__restrict__ boxes, const long long *__restrict__ ibeg,
     int nbox, long long nitems, int ng, const double *__restrict__ d_dt,
     double dx, EulerParm pm, LimTable lt, unsigned long long *d_maxbits)
 {
-    constexpr int NC   = D + 2;
-    constexpr int STEP = 60; // updated positions per 64-lane item
-    const int     lane = threadIdx.x & 63;
-    const double  dt   = *d_dt;
+    constexpr int NC    = D + 2;
+    const int     lane  = threadIdx.x & 63;
+    const double  dt    = *d_dt;
     const double  hdtdx = 0.5 * dt / dx;
     const double  dtdx  = dt / dx;
-    const int     ngy   = (D >= 2) ? ng : 0;
-    const int     ngz   = (D >= 3) ? ng : 0;
 
     long long first, last;
     wave_item_range(nitems, first, last);
     unsigned long long vmax = 0ULL;
     int                b    = -1;
     long long          bend = -1;
-    DevBox             bx;
-    int nx = 0, ny = 0, nxd = 0, cpp = 0;
-    long long P = 0;
+    XItem              cur, nxt;
+    double             Un_[NC]; // prefetched state of the next item
+    auto issue = [&](const XItem &it, double (&V)[NC]) {
+        const long long pc = it.p < 0 ? 0 : (it.p >= it.P ? it.P - 1 : it.p);
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            V[n] = Uin[it.pbase + pc + n * it.nst];
+    };
+    if (first < last)
+    {
+        x_item<D>(boxes, ibeg, nbox, first, ng, lane, b, bend, nxt);
+        issue(nxt, Un_);
+    }
     for (long long item = first; item < last; ++item)
     {
-        if (item >= bend)
-        {
-            b    = find_box(ibeg, nbox, item);
-            bend = ibeg[b + 1];
-            bx   = boxes[b];
-            nx   = bx.hi[0] - bx.lo[0] + 1;
-            ny   = (D >= 2) ? bx.hi[1] - bx.lo[1] + 1 : 1;
-            nxd  = nx + 2 * ng;
-            P    = (long long)ny * nxd;
-            cpp  = (int)((P + STEP - 1) / STEP);
-        }
-        const unsigned local = (unsigned)(item - ibeg[b]);
-        const int      kpl   = (int)(local / (unsigned)cpp);
-        const int      chunk = (int)(local - (unsigned)kpl * (unsigned)cpp);
-        const long long pbase = bx.base + (long long)(kpl + ngz) * bx.kstride
-                                + (long long)ngy * bx.jstride;
-        const long long p  = (long long)chunk * STEP - 2 + lane;
-        const long long pc = p < 0 ? 0 : (p >= P ? P - 1 : p);
-
+        cur = nxt;
         double U[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n)
-            U[n] = Uin[pbase + pc + n * bx.nstride];
+            U[n] = Un_[n];
+        if (item + 1 < last)
+        {
+            x_item<D>(boxes, ibeg, nbox, item + 1, ng, lane, b, bend, nxt);
+            issue(nxt, Un_);
+        }
         double Um[NC], F[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n)
-            Um[n] = __shfl_up(U[n], 1);
+            Um[n] = wave_shr1(U[n]);
         if (FIRST_ORDER)
         {
-            hllc_flux<D, 0>(Um, U, F, pm);
+            hllc_flux<D, 0, E1>(Um, U, F, pm);
         }
         else
         {
             double Up[NC], hL[NC], hR[NC], hRm[NC];
 #pragma unroll
             for (int n = 0; n < NC; ++n)
-                Up[n] = __shfl_down(U[n], 1);
-            reconstruct_half_step<D, 0, LIM, LV>(Um, U, Up, hL, hR, hdtdx, pm,
-                                                 lt);
+                Up[n] = wave_shl1(U[n]);
+            reconstruct_half_step<D, 0, LIM, LV, E1>(Um, U, Up, hL, hR, hdtdx,
+                                                     pm, lt);
 #pragma unroll
             for (int n = 0; n < NC; ++n)
-                hRm[n] = __shfl_up(hR[n], 1);
-            hllc_flux<D, 0>(hRm, hL, F, pm);
+                hRm[n] = wave_shr1(hR[n]);
+            hllc_flux<D, 0, E1>(hRm, hL, F, pm);
         }
         double Fp[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n)
-            Fp[n] = __shfl_down(F[n], 1);
+            Fp[n] = wave_shl1(F[n]);
 
-        const int  col = (int)((unsigned)pc % (unsigned)nxd);
-        const bool upd = lane >= 2 && lane < 62 && p < P && col >= ng
-                         && col < ng + nx;
+        const long long pc  = cur.p < 0 ? 0 : (cur.p >= cur.P ? cur.P - 1 : cur.p);
+        const int       col = (int)((unsigned)pc % (unsigned)cur.nxd);
+        const bool upd = lane >= 2 && lane < 62 && cur.p < cur.P && col >= ng
+                         && col < ng + cur.nx;
         if (upd)
         {
-            double Un[NC];
+            double Unew[NC];
 #pragma unroll
             for (int n = 0; n < NC; ++n)
             {
-                Un[n] = U[n] - dtdx * (Fp[n] - F[n]);
-                Uout[pbase + p + n * bx.nstride] = Un[n];
+                Unew[n] = U[n] - dtdx * (Fp[n] - F[n]);
+                Uout[cur.pbase + cur.p + n * cur.nst] = Unew[n];
             }
             if (d_maxbits)
             {
-                const double s = cell_wave_speed<D>(Un, pm);
-                vmax = umax64(vmax, __double_as_longlong(s));
+                const double sp = cell_wave_speed<D, E1>(Unew, pm);
+                vmax = umax64(vmax, __double_as_longlong(sp));
             }
         }
     }
@@ -186,8 +263,8 @@ __global__ __launch_bounds__(256) void k_sweep_x(
 }
 
 // -------------------------------------------------------------- y/z sweep
-template <int D, int DIR, int LIM, int LV, bool FIRST_ORDER>
-__global__ __launch_bounds__(256) void k_sweep_t(
+template <int D, int DIR, int LIM, int LV, bool FIRST_ORDER, bool E1>
+__global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
     const double *__restrict__ Uin, double *__restrict__ Uout,
     const DevBox *__restrict__ boxes, const long long *__restrict__ ibeg,
     int nbox, long long nitems, int ng, int K, const double *__restrict__ d_dt,
@@ -256,31 +333,35 @@ __global__ __launch_bounds__(256) void k_sweep_t(
             for (int n = 0; n < NC; ++n)
                 V[n] = Uin[o + n * bx.nstride];
         };
-        double Um1[NC], U0[NC], hRprev[NC], Fprev[NC];
+        // window U(s-1), U(s), and U(s+1) in flight one iteration ahead
+        double Um1[NC], U0[NC], Up[NC], hRprev[NC], Fprev[NC];
         load(s0 - 1, Um1);
         load(s0, U0);
+        load(s0 + 1, Up);
         if (!FIRST_ORDER)
         {
             double Um2[NC], hLd[NC];
             load(s0 - 2, Um2);
-            reconstruct_half_step<D, DIR, LIM, LV>(Um2, Um1, U0, hLd, hRprev,
-                                                   hdtdx, pm, lt);
+            reconstruct_half_step<D, DIR, LIM, LV, E1>(Um2, Um1, U0, hLd,
+                                                       hRprev, hdtdx, pm, lt);
         }
         for (int s = s0; s <= s1; ++s)
         {
+            double Unext[NC];
+            // U(s+2) is only needed by iteration s+1 (s+1 <= s1)
+            if (s < s1)
+                load(s + 2, Unext);
             double F[NC];
-            double Up[NC];
             if (FIRST_ORDER)
             {
-                hllc_flux<D, DIR>(Um1, U0, F, pm);
+                hllc_flux<D, DIR, E1>(Um1, U0, F, pm);
             }
             else
             {
                 double hL0[NC], hR0[NC];
-                load(s + 1, Up);
-                reconstruct_half_step<D, DIR, LIM, LV>(Um1, U0, Up, hL0, hR0,
-                                                       hdtdx, pm, lt);
-                hllc_flux<D, DIR>(hRprev, hL0, F, pm);
+                reconstruct_half_step<D, DIR, LIM, LV, E1>(Um1, U0, Up, hL0,
+                                                           hR0, hdtdx, pm, lt);
+                hllc_flux<D, DIR, E1>(hRprev, hL0, F, pm);
 #pragma unroll
                 for (int n = 0; n < NC; ++n)
                     hRprev[n] = hR0[n];
@@ -298,7 +379,7 @@ __global__ __launch_bounds__(256) void k_sweep_t(
                 }
                 if (d_maxbits)
                 {
-                    const double sp = cell_wave_speed<D>(Un, pm);
+                    const double sp = cell_wave_speed<D, E1>(Un, pm);
                     vmax = umax64(vmax, __double_as_longlong(sp));
                 }
             }
@@ -307,16 +388,12 @@ __global__ __launch_bounds__(256) void k_sweep_t(
             {
                 Fprev[n] = F[n];
                 Um1[n]   = U0[n];
+                U0[n]    = Up[n];
             }
             if (s < s1)
-            {
-                if (FIRST_ORDER)
-                    load(s + 1, U0);
-                else
 #pragma unroll
-                    for (int n = 0; n < NC; ++n)
-                        U0[n] = Up[n];
-            }
+                for (int n = 0; n < NC; ++n)
+                    Up[n] = Unext[n];
         }
     }
     if (d_maxbits)
@@ -465,38 +542,46 @@ __global__ void k_set_u64(unsigned long long *p, unsigned long long v)
 }
 
 // ------------------------------------------------------------ launchers
-template <int D, int LIM, int LV, bool FO>
+template <int D, int LIM, int LV, bool FO, bool E1>
 static hipError_t launch_x(const SweepLaunch &L)
 {
-    hipLaunchKernelGGL((k_sweep_x<D, LIM, LV, FO>), dim3(L.grid),
+    hipLaunchKernelGGL((k_sweep_x<D, LIM, LV, FO, E1>), dim3(L.grid),
                        dim3(L.block), 0, L.stream, L.Uin, L.Uout, L.boxes,
                        L.item_begin, L.nbox, L.nitems, L.ng, L.d_dt, L.dx,
                        L.pm, L.lt, L.d_maxbits);
     return hipGetLastError();
 }
 
-template <int D, int DIR, int LIM, int LV, bool FO>
+template <int D, int DIR, int LIM, int LV, bool FO, bool E1>
 static hipError_t launch_t(const SweepLaunch &L)
 {
-    hipLaunchKernelGGL((k_sweep_t<D, DIR, LIM, LV, FO>), dim3(L.grid),
+    hipLaunchKernelGGL((k_sweep_t<D, DIR, LIM, LV, FO, E1>), dim3(L.grid),
                        dim3(L.block), 0, L.stream, L.Uin, L.Uout, L.boxes,
                        L.item_begin, L.nbox, L.nitems, L.ng, L.K, L.d_dt,
                        L.dx, L.pm, L.lt, L.d_maxbits);
     return hipGetLastError();
 }
 
+template <int D, int LIM, int LV, bool FO, bool E1>
+static hipError_t dispatch_dir_e(int dir, const SweepLaunch &L)
+{
+    if (dir == 0)
+        return launch_x<D, LIM, LV, FO, E1>(L);
+    if constexpr (D >= 2)
+        if (dir == 1)
+            return launch_t<D, 1, LIM, LV, FO, E1>(L);
+    if constexpr (D >= 3)
+        if (dir == 2)
+            return launch_t<D, 2, LIM, LV, FO, E1>(L);
+    return hipErrorInvalidValue;
+}
+
 template <int D, int LIM, int LV, bool FO>
 static hipError_t dispatch_dir(int dir, const SweepLaunch &L)
 {
-    if (dir == 0)
-        return launch_x<D, LIM, LV, FO>(L);
-    if constexpr (D >= 2)
-        if (dir == 1)
-            return launch_t<D, 1, LIM, LV, FO>(L);
-    if constexpr (D >= 3)
-        if (dir == 2)
-            return launch_t<D, 2, LIM, LV, FO>(L);
-    return hipErrorInvalidValue;
+    if (L.pm.eps == 1.0)
+        return dispatch_dir_e<D, LIM, LV, FO, true>(dir, L);
+    return dispatch_dir_e<D, LIM, LV, FO, false>(dir, L);
 }
 
 template <int D, int LIM, int LV>
@@ -552,16 +637,16 @@ const void *sweep_kernel_ptr(int D, int dir)
     if (dir == 0)
     {
         if (D == 1)
-            return (const void *)&k_sweep_x<1, LIM_MINBEE, LV_SHARED, false>;
+            return (const void *)&k_sweep_x<1, LIM_MINBEE, LV_SHARED, false, true>;
         if (D == 2)
-            return (const void *)&k_sweep_x<2, LIM_MINBEE, LV_SHARED, false>;
-        return (const void *)&k_sweep_x<3, LIM_MINBEE, LV_SHARED, false>;
+            return (const void *)&k_sweep_x<2, LIM_MINBEE, LV_SHARED, false, true>;
+        return (const void *)&k_sweep_x<3, LIM_MINBEE, LV_SHARED, false, true>;
     }
     if (D == 2)
-        return (const void *)&k_sweep_t<2, 1, LIM_MINBEE, LV_SHARED, false>;
+        return (const void *)&k_sweep_t<2, 1, LIM_MINBEE, LV_SHARED, false, true>;
     if (dir == 1)
-        return (const void *)&k_sweep_t<3, 1, LIM_MINBEE, LV_SHARED, false>;
-    return (const void *)&k_sweep_t<3, 2, LIM_MINBEE, LV_SHARED, false>;
+        return (const void *)&k_sweep_t<3, 1, LIM_MINBEE, LV_SHARED, false, true>;
+    return (const void *)&k_sweep_t<3, 2, LIM_MINBEE, LV_SHARED, false, true>;
 }
 
 hipError_t launch_fill_ghosts(double *U, const DevBox *boxes,

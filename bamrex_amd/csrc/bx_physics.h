@@ -34,22 +34,52 @@ struct EulerParm
 };
 
 // Q = (rho, u_0..u_{D-1}, p)
-template <int D>
+// Multiplication / division by epsilon.  With E1 (epsilon == 1, the only
+// value the explicit configs use) they are dropped: x*1 and x/1 are exact,
+// so the result is unchanged bit for bit.
+template <bool E1> BX_DEV double mul_eps(double x, double eps)
+{
+    return E1 ? x : x * eps;
+}
+template <bool E1> BX_DEV double div_eps(double x, double eps)
+{
+    return E1 ? x : x / eps;
+}
+
+// a / b for several numerators sharing b: with y = RN(1/b) (one IEEE
+// division) and q0 = RN(a y), the Markstein correction q0 + (a - b q0) y,
+// rounded once, is exactly RN(a / b) whenever nothing under- or overflows
+// (P. Markstein, IBM J. Res. Dev. 34(1), 1990) -- the same bits as a / b.
+BX_DEV double div_by_recip(double a, double b, double y)
+{
+    const double q0 = a * y;
+    const double e  = __builtin_fma(-q0, b, a);
+    return __builtin_fma(e, y, q0);
+}
+
+template <int D, bool E1 = false>
 BX_DEV void primv_from_consv(const double (&U)[D + 2], double (&Q)[D + 2],
                              const EulerParm &pm)
 {
     Q[0] = U[0];
+    if (D == 1)
+        Q[1] = U[1] / U[0];
+    else
+    {
+        const double y = 1.0 / U[0];
 #pragma unroll
-    for (int d = 0; d < D; ++d)
-        Q[1 + d] = U[1 + d] / U[0];
+        for (int d = 0; d < D; ++d)
+            Q[1 + d] = div_by_recip(U[1 + d], U[0], y);
+    }
     double u_sq = Q[1] * Q[1];
 #pragma unroll
     for (int d = 1; d < D; ++d)
         u_sq = u_sq + Q[1 + d] * Q[1 + d];
-    Q[1 + D] = (pm.gamma - 1) * (U[1 + D] - 0.5 * pm.eps * U[0] * u_sq);
+    // 0.5 * eps * rho: (0.5*eps)*rho == 0.5*rho exactly when eps == 1
+    Q[1 + D] = (pm.gamma - 1) * (U[1 + D] - mul_eps<E1>(0.5, pm.eps) * U[0] * u_sq);
 }
 
-template <int D, int dir>
+template <int D, int dir, bool E1 = false>
 BX_DEV void euler_flux(const double (&U)[D + 2], const double (&Q)[D + 2],
                        double (&F)[D + 2], const EulerParm &pm)
 {
@@ -57,7 +87,7 @@ BX_DEV void euler_flux(const double (&U)[D + 2], const double (&Q)[D + 2],
 #pragma unroll
     for (int d = 0; d < D; ++d)
         F[1 + d] = U[1 + dir] * Q[1 + d];
-    F[1 + dir] = F[1 + dir] + Q[1 + D] / pm.eps;
+    F[1 + dir] = F[1 + dir] + div_eps<E1>(Q[1 + D], pm.eps);
     F[1 + D]   = (U[1 + D] + Q[1 + D]) * Q[1 + dir];
 }
 
@@ -124,7 +154,7 @@ BX_DEV double pick_var(const double (&U)[D + 2], const double (&Q)[D + 2],
 // Reconstruct + half time step at one cell (Flux_Helpers.cpp:92-148).
 // Um/U0/Up: conserved states at c-1, c, c+1 along dir.  Outputs the
 // half-stepped low-side (hL) and high-side (hR) states of cell c.
-template <int D, int dir, int LIM, int LV>
+template <int D, int dir, int LIM, int LV, bool E1 = false>
 BX_DEV void reconstruct_half_step(const double (&Um)[D + 2],
                                   const double (&U0)[D + 2],
                                   const double (&Up)[D + 2],
@@ -187,10 +217,10 @@ BX_DEV void reconstruct_half_step(const double (&Um)[D + 2],
         hR[n]                    = U0[n] + 0.5 * phi[n] * delta;
     }
     double QL[NC], QR[NC], FL[NC], FR[NC];
-    primv_from_consv<D>(hL, QL, pm);
-    primv_from_consv<D>(hR, QR, pm);
-    euler_flux<D, dir>(hL, QL, FL, pm);
-    euler_flux<D, dir>(hR, QR, FR, pm);
+    primv_from_consv<D, E1>(hL, QL, pm);
+    primv_from_consv<D, E1>(hR, QR, pm);
+    euler_flux<D, dir, E1>(hL, QL, FL, pm);
+    euler_flux<D, dir, E1>(hR, QR, FR, pm);
 #pragma unroll
     for (int n = 0; n < NC; ++n)
     {
@@ -203,14 +233,14 @@ BX_DEV double amrex_max(double a, double b) { return (a < b) ? b : a; }
 
 // HLLC flux at the face between left state UL and right state UR
 // (HLLC.cpp:90-172 with estimate_wave_speed_HLLC_Toro, HLLC.H:213-309).
-template <int D, int dir>
+template <int D, int dir, bool E1 = false>
 BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
                       double (&F)[D + 2], const EulerParm &pm)
 {
     constexpr int NC = D + 2;
     double        QL[NC], QR[NC];
-    primv_from_consv<D>(UL, QL, pm);
-    primv_from_consv<D>(UR, QR, pm);
+    primv_from_consv<D, E1>(UL, QL, pm);
+    primv_from_consv<D, E1>(UR, QR, pm);
 
     const double adiabatic = pm.gamma;
     const double epsilon   = pm.eps;
@@ -223,8 +253,8 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
     const double u_L       = QL[1 + dir];
     const double u_R       = QR[1 + dir];
 
-    const double c_L = sqrt(adiabatic * p_L / (density_L * epsilon));
-    const double c_R = sqrt(adiabatic * p_R / (density_R * epsilon));
+    const double c_L = sqrt(adiabatic * p_L / mul_eps<E1>(density_L, epsilon));
+    const double c_R = sqrt(adiabatic * p_R / mul_eps<E1>(density_R, epsilon));
 
     const double z = (adiabatic - 1) / (2 * adiabatic);
     const double p_rr
@@ -244,7 +274,7 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
     const double denom
         = (density_L * (speed_L - u_L) - density_R * (speed_R - u_R));
     const double speed_interm
-        = (denom != 0) ? ((p_R - p_L) / epsilon
+        = (denom != 0) ? (div_eps<E1>(p_R - p_L, epsilon)
                           + density_L * u_L * (speed_L - u_L)
                           - density_R * u_R * (speed_R - u_R))
                              / denom
@@ -267,7 +297,7 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
         Qs[n] = left ? QL[n] : QR[n];
     }
     double Fs[NC];
-    euler_flux<D, dir>(Us, Qs, Fs, pm);
+    euler_flux<D, dir, E1>(Us, Qs, Fs, pm);
 
     if (use_L || use_R)
     {
@@ -293,7 +323,7 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
             = density * (speed_K - u_K) / (speed_K - speed_interm);
         st[1 + D] = energy / density
                     + (speed_interm - u_K)
-                          * (speed_interm * epsilon
+                          * (mul_eps<E1>(speed_interm, epsilon)
                              + pressure / (density * (speed_K - u_K)));
 #pragma unroll
         for (int n = 0; n < NC; ++n)
@@ -315,17 +345,17 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
 // states themselves (Fluxes.H:192-204).
 
 // max over a cell of sqrt(gamma p / (rho eps)) + |u| (Euler.cpp:59-67)
-template <int D>
+template <int D, bool E1 = false>
 BX_DEV double cell_wave_speed(const double (&U)[D + 2], const EulerParm &pm)
 {
     double Q[D + 2];
-    primv_from_consv<D>(U, Q, pm);
+    primv_from_consv<D, E1>(U, Q, pm);
     double vsq = Q[1] * Q[1];
 #pragma unroll
     for (int d = 1; d < D; ++d)
         vsq = vsq + Q[1 + d] * Q[1 + d];
     const double velocity = sqrt(vsq);
-    return sqrt(pm.gamma * Q[1 + D] / (Q[0] * pm.eps)) + velocity;
+    return sqrt(pm.gamma * Q[1 + D] / mul_eps<E1>(Q[0], pm.eps)) + velocity;
 }
 
 } // namespace bx

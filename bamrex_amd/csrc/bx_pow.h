@@ -11,8 +11,9 @@
 // (Tang, "Table-driven implementation of the logarithm/exponential function",
 // ACM TOMS 1989/1990; Muller, "Handbook of Floating-Point Arithmetic", ch. 11):
 //
-//   log: x = 2^k m, m in [1,2); j = top 7 bits of m; c_j = 1 + (j+1/2)/128;
-//        m*invc_j = 1 + r computed as an exact double-double (fma), |r| < 2^-8;
+//   log: x = 2^k m, m in [1,2) (m/2 and k+1 when m >= 1.5); j = top 7 bits
+//        of m; c_j the centre of m's interval (c = 1 on the two intervals
+//        next to 1); m*invc_j = 1 + r as an exact double-double, |r| < 2^-7;
 //        log x = k ln2 + (-log invc_j) + log1p(r), kept as a double-double.
 //   exp: t = y log x (double-double); t = (64 k + j) ln2/64 + r, |r| <= ln2/128;
 //        e^t = 2^k * 2^(j/64) * e^r, with e^r - 1 a degree-7 polynomial.
@@ -56,18 +57,22 @@ BX_POW_FN void fast_two_sum(double a, double b, double &s, double &e)
 // log(x) as a double-double, x positive, finite, normal.
 BX_POW_FN void log_dd(double x, double &lhi, double &llo)
 {
-    uint64_t ix = d2u(x);
-    int      k  = (int)((ix >> 52) & 0x7ff) - 1023;
-    double   m  = u2d((ix & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
-    int      j  = (int)((ix >> 45) & 127);
+    const uint64_t ix = d2u(x);
+    const int      j  = (int)((ix >> 45) & 127);
+    // m in [1,2); for j >= 64 use m/2 in [0.75,1) and k+1
+    const int hi = j >> 6;
+    const int k  = (int)((ix >> 52) & 0x7ff) - 1023 + hi;
+    const double m
+        = u2d((ix & 0x000fffffffffffffULL) | ((uint64_t)(1023 - hi) << 52));
 
     const double invc = bx_log_invc[j];
     // m * invc = p_hi + p_lo exactly; r = p - 1
     const double p_hi = m * invc;
     const double p_lo = __builtin_fma(m, invc, -p_hi);
-    const double r0   = p_hi - 1.0; // exact: p_hi in [1 - 2^-7, 1 + 2^-7]
-    double       r_hi, r_lo;
-    two_sum(r0, p_lo, r_hi, r_lo);
+    const double r0   = p_hi - 1.0; // exact (Sterbenz)
+    // r0 is 0 or at least one ulp of p_hi, which bounds |p_lo|
+    double r_hi, r_lo;
+    fast_two_sum(r0, p_lo, r_hi, r_lo);
 
     // log1p(r) = r - r^2/2 + r^3 (1/3 - r/4 + r^2/5 - r^3/6 + r^4/7 - r^5/8)
     const double sq_hi = r_hi * r_hi;
@@ -80,14 +85,14 @@ BX_POW_FN void log_dd(double x, double &lhi, double &llo)
     poly = __builtin_fma(poly, r_hi, -0.25);
     poly = __builtin_fma(poly, r_hi, 0x1.5555555555555p-2);  // 1/3
 
-    // k ln2 + logc: both multiples of 2^-42, the sum is exact
+    // k ln2 + logc: both multiples of 2^-42, the sum is exact; it is 0 or
+    // larger in magnitude than r (table design), and b dominates r^2/2
     const double kd = (double)k;
     const double a  = kd * BX_LN2_HI + bx_log_logc_hi[j];
-
-    double b_hi, b_lo;
-    two_sum(a, r_hi, b_hi, b_lo);
+    double       b_hi, b_lo;
+    fast_two_sum(a, r_hi, b_hi, b_lo);
     double c_hi, c_lo;
-    two_sum(b_hi, -0.5 * sq_hi, c_hi, c_lo);
+    fast_two_sum(b_hi, -0.5 * sq_hi, c_hi, c_lo);
 
     const double small = kd * BX_LN2_LO + bx_log_logc_lo[j] + r_lo
                          - 0.5 * sq_lo - r_hi * r_lo + r3 * poly;
@@ -121,7 +126,7 @@ BX_POW_FN double exp_dd(double t_hi, double t_lo)
     const double plo = __builtin_fma(th, r, -phi);
     const double sml = plo + th * elo + tl + tl * (r + elo);
     double       s_hi, s_lo;
-    two_sum(th, phi, s_hi, s_lo);
+    fast_two_sum(th, phi, s_hi, s_lo); // th >= 1 > |phi|
     double res = s_hi + (s_lo + sml);
 
     // scale by 2^k without overflow in the scale factor itself
@@ -148,9 +153,15 @@ BX_POW_FN bool is_odd_integer(double y)
 
 } // namespace bxpow
 
-// pow(x, y) with C99 Annex F special cases for the finite/NaN inputs this
-// code meets; inf inputs follow Annex F too.
-BX_POW_FN double bx_pow(double x, double y)
+// Special cases of pow (C99 Annex F) and the rare inputs the fast path
+// below does not take: x <= 0, NaN or inf operands, subnormal x, results
+// near the overflow / underflow thresholds.
+#if defined(__HIPCC__)
+__device__ __attribute__((noinline))
+#else
+static
+#endif
+double bx_pow_slow(double x, double y)
 {
     using namespace bxpow;
     if (y == 0.0)
@@ -191,22 +202,43 @@ BX_POW_FN double bx_pow(double x, double y)
         kadj = 54;
     }
     double lhi, llo;
-    bxpow::log_dd(xs, lhi, llo);
+    log_dd(xs, lhi, llo);
     if (kadj)
     {
         // log x = log xs - 54 ln2
         double s, e;
-        bxpow::two_sum(lhi, -54.0 * BX_LN2_HI, s, e);
+        two_sum(lhi, -54.0 * BX_LN2_HI, s, e);
         llo = llo + e - 54.0 * BX_LN2_LO;
-        bxpow::fast_two_sum(s, llo, lhi, llo);
+        fast_two_sum(s, llo, lhi, llo);
     }
     const double t_hi = y * lhi;
     const double t_lo = __builtin_fma(y, lhi, -t_hi) + y * llo;
     double       th, tl;
-    bxpow::fast_two_sum(t_hi, t_lo, th, tl);
+    fast_two_sum(t_hi, t_lo, th, tl);
     if (th > 709.8)
         return sign * __builtin_inf();
     if (th < -745.2)
         return sign * 0.0;
-    return sign * bxpow::exp_dd(th, tl);
+    return sign * exp_dd(th, tl);
+}
+
+// pow(x, y).  Fast path: x positive and normal, y finite and nonzero, and
+// |y log x| < 708 (result well inside the normal range); it performs exactly
+// the same operations as the general path for those inputs.
+BX_POW_FN double bx_pow(double x, double y)
+{
+    using namespace bxpow;
+    if (x >= 0x1p-1022 && x < 0x1p1023 && __builtin_fabs(y) < 0x1p1023
+        && y != 0.0)
+    {
+        double lhi, llo;
+        log_dd(x, lhi, llo);
+        const double t_hi = y * lhi;
+        const double t_lo = __builtin_fma(y, lhi, -t_hi) + y * llo;
+        double       th, tl;
+        fast_two_sum(t_hi, t_lo, th, tl);
+        if (__builtin_fabs(th) < 708.0)
+            return exp_dd(th, tl);
+    }
+    return bx_pow_slow(x, y);
 }

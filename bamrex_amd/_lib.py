@@ -100,7 +100,49 @@ _SIGS = {
     "bx_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
     "bx_copy_kernel": (ctypes.c_int, [P, P, ctypes.c_size_t, P]),
     "bx_last_error": (ctypes.c_char_p, []),
+    # pressure Helmholtz solve + learned preconditioner
+    "bx_helm_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int, ctypes.POINTER(P)]),
+    "bx_helm_destroy": (None, [P]),
+    "bx_helm_set_coeffs": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          P, P, P, P, P, P]),
+    "bx_helm_set_options": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int]),
+    "bx_helm_apply": (ctypes.c_int, [P, P, P, P]),
+    "bx_helm_residual": (ctypes.c_int, [P, P, P, P, P]),
+    "bx_helm_smooth": (ctypes.c_int, [P, P, P, ctypes.c_int, P]),
+    "bx_helm_precond": (ctypes.c_int, [P, P, P, P]),
+    "bx_helm_attach_cnn": (ctypes.c_int, [P, P, ctypes.c_double]),
+    "bx_helm_solve": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, P, P]),
+    "bx_helm_last_error": (ctypes.c_char_p, []),
+    "bx_cnn_create": (ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(P)]),
+    "bx_cnn_destroy": (None, [P]),
+    "bx_cnn_set_params": (ctypes.c_int, [P, P, ctypes.c_int]),
+    "bx_cnn_get_params": (ctypes.c_int, [P, P, ctypes.c_int]),
+    "bx_cnn_set_operator": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, P, P, P, P]),
+    "bx_cnn_infer": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P]),
+    "bx_cnn_net_forward": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]),
+    "bx_cnn_loss_grad": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, ctypes.c_int,
+                                        ctypes.c_int, DP, P, P]),
+    "bx_cnn_train": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_float, ctypes.c_uint, P, P]),
+    "bx_cnn_last_error": (ctypes.c_char_p, []),
 }
+
+BX_SOLVE = {"mg": 0, "fgmres_mg": 1, "fgmres_cnn": 2}
+BX_CNN_NPARAMS = 4896
+
+
+class bx_solve_stats(ctypes.Structure):
+    _fields_ = [
+        ("iters", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("fell_back", ctypes.c_int),
+        ("precond_applies", ctypes.c_int),
+        ("rel_resid", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+    ]
 
 
 def exported_symbols():

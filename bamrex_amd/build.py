@@ -20,7 +20,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 
-HIP_SOURCES = ["bx_explicit.hip", "bx_api.hip"]
+HIP_SOURCES = ["bx_explicit.hip", "bx_api.hip", "bx_helmholtz.hip", "bx_cnn.hip"]
 HOST_SOURCES = ["ParmParse.cpp", "AmrLevelAdv.cpp", "Prob.cpp", "main.cpp"]
 HOST_BIN = os.path.join(HERE, "bamrex_run")
 

@@ -1,0 +1,1034 @@
+// Learned fine-level smoother for the IMEX pressure Helmholtz solve: a small
+// convolutional network, hand-written forward (BF16 MFMA, FP32 accumulate)
+// and backward kernels, and its training loop.
+//
+// Network N: 4 bias-free 3x3 convolutions, 1 -> 16 -> 16 -> 16 -> 1
+// channels, ReLU after the first three.  Bias-free ReLU layers make N
+// positively homogeneous (N(c x) = c N(x), c > 0) up to BF16 rounding, so the
+// correction scales with the residual.  Weights and activations are BF16,
+// every sum is accumulated in FP32.  The correction applied to a residual r is
+//     e = omega * (sigma / D) * N(r / sigma),   sigma = rms(r),
+// with D the diagonal of the Helmholtz operator: N = identity is a Jacobi
+// step, the network learns a better local approximate inverse.
+//
+// Layouts: activations NHWC with 16 channels (bf16, 32 B per pixel).
+// Parameters (FP32 master copy, flat, BX_CNN_NPARAMS):
+//   W1[9][16]        tap t, out n
+//   W2[9][16][16]    tap t, out n, in k
+//   W3[9][16][16]
+//   W4[9][16]        tap t, in k
+// tap t = (dy+1)*3 + (dx+1), neighbour (x+dx, y+dy).
+//
+// The two 16 -> 16 layers run on v_mfma_f32_16x16x16bf16_1k: one wave takes
+// 16 consecutive pixels (M) x 16 output channels (N), K = 16 input channels
+// per tap, 9 MFMAs accumulate the 9 taps.
+//
+// Training loss on crops of (residual, error) pairs harvested from plain
+// multigrid solves (A e = r):
+//     L = sum_crops  ||mask (r - A e_cnn)||^2 / ||mask r||^2
+// (mask = crop interior at distance >= 5 from the crop edge, where the 9x9
+// receptive field sees no padding), i.e. the A^2-norm of the remaining error
+// e - e_cnn.  Gradients: dL/de_cnn = -2 A^T (mask res) (A symmetric for PEEK),
+// then back through the convolutions; Adam on the FP32 master weights.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/bamrex_hip.h"
+#include "bx_helmholtz.h"
+
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int C = 16;
+constexpr int NW1 = 9 * C, NW2 = 9 * C * C, NW3 = 9 * C * C, NW4 = 9 * C;
+constexpr int OFF1 = 0, OFF2 = NW1, OFF3 = OFF2 + NW2, OFF4 = OFF3 + NW3;
+constexpr int NPAR = OFF4 + NW4;
+static_assert(NPAR == BX_CNN_NPARAMS, "parameter count");
+
+__host__ __device__ inline unsigned short f2bf(float f)
+{
+    unsigned int u = __builtin_bit_cast(unsigned int, f);
+    u += 0x7fffu + ((u >> 16) & 1u); // round to nearest even
+    return (unsigned short)(u >> 16);
+}
+__host__ __device__ inline float bf2f(unsigned short h)
+{
+    return __builtin_bit_cast(float, (unsigned int)h << 16);
+}
+
+// Field geometry of one launch: `nimg` images of w x h pixels stored one
+// after the other.  PERIODIC: neighbours wrap; otherwise zero padding.
+struct Geo
+{
+    int w, h, nimg;
+};
+
+template <bool PERIODIC>
+__device__ inline int nbr(int x, int y, int dx, int dy, const Geo &g, int img)
+{
+    int xx = x + dx, yy = y + dy;
+    if (PERIODIC)
+    {
+        xx = xx < 0 ? xx + g.w : (xx >= g.w ? xx - g.w : xx);
+        yy = yy < 0 ? yy + g.h : (yy >= g.h ? yy - g.h : yy);
+    }
+    else if (xx < 0 || yy < 0 || xx >= g.w || yy >= g.h)
+        return -1;
+    return (img * g.h + yy) * g.w + xx;
+}
+
+// ---- inference / training forward --------------------------------------
+// layer 1: x (bf16, 1 channel) -> h1 (bf16, 16, ReLU)
+template <bool PERIODIC>
+__global__ __launch_bounds__(256) void k_conv1(const unsigned short *__restrict__ xin,
+                                               unsigned short *__restrict__ hout,
+                                               const float *__restrict__ W1, Geo g)
+{
+    const long long npx = (long long)g.w * g.h * g.nimg;
+    const long long p   = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npx)
+        return;
+    const int img = (int)(p / ((long long)g.w * g.h));
+    const int rem = (int)(p - (long long)img * g.w * g.h);
+    const int y = rem / g.w, x = rem - y * g.w;
+    float     v[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+    {
+        const int q = nbr<PERIODIC>(x, y, t % 3 - 1, t / 3 - 1, g, img);
+        v[t]        = q < 0 ? 0.f : bf2f(xin[q]);
+    }
+    unsigned short o[C];
+#pragma unroll
+    for (int n = 0; n < C; ++n)
+    {
+        float s = 0.f;
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+            s += W1[t * C + n] * v[t];
+        o[n] = f2bf(s > 0.f ? s : 0.f);
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(hout + p * C);
+    dst[0] = make_uint4(o[0] | (o[1] << 16), o[2] | (o[3] << 16), o[4] | (o[5] << 16),
+                        o[6] | (o[7] << 16));
+    dst[1] = make_uint4(o[8] | (o[9] << 16), o[10] | (o[11] << 16),
+                        o[12] | (o[13] << 16), o[14] | (o[15] << 16));
+}
+
+// layers 2, 3: h (bf16, 16) -> h' (bf16, 16, ReLU) on MFMA.
+// Wb: bf16 weights [9][16 out][16 in].
+template <bool PERIODIC>
+__global__ __launch_bounds__(256) void k_conv16(const unsigned short *__restrict__ hin,
+                                                unsigned short *__restrict__ hout,
+                                                const unsigned short *__restrict__ Wb,
+                                                Geo g)
+{
+    const int       lane  = threadIdx.x & 63;
+    const long long tile  = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long npx   = (long long)g.w * g.h * g.nimg;
+    const long long p0    = tile * 16;
+    if (p0 >= npx) // whole wave exits together (npx is a multiple of 16)
+        return;
+    const int img = (int)(p0 / ((long long)g.w * g.h));
+    const int rem = (int)(p0 - (long long)img * g.w * g.h);
+    const int y = rem / g.w, x0 = rem - y * g.w;
+    const int m = lane & 15, kg = lane >> 4;
+    v4f       acc = { 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+    {
+        const int q = nbr<PERIODIC>(x0 + m, y, t % 3 - 1, t / 3 - 1, g, img);
+        v4s       a = { 0, 0, 0, 0 };
+        if (q >= 0)
+            a = *reinterpret_cast<const v4s *>(hin + (long long)q * C + 4 * kg);
+        const v4s b
+            = *reinterpret_cast<const v4s *>(Wb + (t * C + m) * C + 4 * kg);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    // lane holds D[4*kg + v][m]: pixel p0 + 4*kg + v, channel m
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+        const float s = acc[v];
+        hout[(p0 + 4 * kg + v) * C + m] = f2bf(s > 0.f ? s : 0.f);
+    }
+}
+
+// layer 4: h3 (bf16, 16) -> y (fp32, 1 channel)
+template <bool PERIODIC>
+__global__ __launch_bounds__(256) void k_conv4(const unsigned short *__restrict__ hin,
+                                               float *__restrict__ yout,
+                                               const float *__restrict__ W4, Geo g)
+{
+    const long long npx = (long long)g.w * g.h * g.nimg;
+    const long long p   = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npx)
+        return;
+    const int img = (int)(p / ((long long)g.w * g.h));
+    const int rem = (int)(p - (long long)img * g.w * g.h);
+    const int y = rem / g.w, x = rem - y * g.w;
+    float     s = 0.f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+    {
+        const int q = nbr<PERIODIC>(x, y, t % 3 - 1, t / 3 - 1, g, img);
+        if (q < 0)
+            continue;
+        const uint4 *src = reinterpret_cast<const uint4 *>(hin + (long long)q * C);
+        const uint4  u0 = src[0], u1 = src[1];
+        const unsigned int w8[8] = { u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w };
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+        {
+            s += W4[t * C + 2 * k] * bf2f((unsigned short)(w8[k] & 0xffff));
+            s += W4[t * C + 2 * k + 1] * bf2f((unsigned short)(w8[k] >> 16));
+        }
+    }
+    yout[p] = s;
+}
+
+// x = bf16(r * inv_sigma)
+__global__ __launch_bounds__(256) void k_prep_full(const double *__restrict__ r,
+                                                   unsigned short *__restrict__ x,
+                                                   long long n,
+                                                   const double *__restrict__ ss)
+{
+    const double inv = ss[0] > 0 ? 1.0 / ss[0] : 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        x[i] = f2bf((float)(r[i] * inv));
+}
+
+// e = omega * sigma * y / D
+__global__ __launch_bounds__(256) void k_finish_full(const float *__restrict__ y,
+                                                     const double *__restrict__ Dg,
+                                                     double *__restrict__ e,
+                                                     long long n, double omega,
+                                                     const double *__restrict__ ss)
+{
+    const double sg = ss[0];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        e[i] = omega * sg * (double)y[i] / Dg[i];
+}
+
+// rms of r (two-stage)
+__global__ __launch_bounds__(256) void k_sumsq(const double *__restrict__ r,
+                                               long long n,
+                                               double *__restrict__ partial)
+{
+    __shared__ double sh[256];
+    double            s = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        s += r[i] * r[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1)
+    {
+        if ((int)threadIdx.x < off)
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = sh[0];
+}
+__global__ void k_rms_final(const double *__restrict__ partial, int nb,
+                            long long n, double *__restrict__ out)
+{
+    __shared__ double sh[256];
+    double            s = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        s += partial[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1)
+    {
+        if ((int)threadIdx.x < off)
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[0] = sqrt(sh[0] / (double)n);
+}
+
+// operator diagonal: alpha a + dhx (bXl + bXr) + dhy (bYl + bYr)
+__global__ __launch_bounds__(256) void k_diag(const double *__restrict__ a,
+                                              const double *__restrict__ bX,
+                                              const double *__restrict__ bY,
+                                              double *__restrict__ D, int nx, int ny,
+                                              double alpha, double dhx, double dhy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= nx)
+        return;
+    const long long c = (long long)j * nx + i;
+    D[c] = alpha * a[c] + dhx * (bX[(long long)j * (nx + 1) + i] + bX[(long long)j * (nx + 1) + i + 1])
+           + dhy * (bY[(long long)j * nx + i] + bY[(long long)(j + 1) * nx + i]);
+}
+
+// ---- training: crop gather, loss, backward -------------------------------
+struct CropSet
+{
+    const double *R;  // fields [nf][ny][nx]
+    const int    *xy; // per crop: field, x0, y0
+    int           nx, ny, cs; // field size, crop size
+};
+
+__device__ inline long long gidx(const CropSet &c, int f, int x, int y)
+{
+    x = x % c.nx;
+    y = y % c.ny;
+    if (x < 0) x += c.nx;
+    if (y < 0) y += c.ny;
+    return ((long long)f * c.ny + y) * c.nx + x;
+}
+
+// per crop: sigma (rms over the crop) and ||mask r||^2
+__global__ void k_crop_stats(CropSet c, double *__restrict__ sig,
+                             double *__restrict__ rn2, int border)
+{
+    const int b = blockIdx.x;
+    const int f = c.xy[3 * b], x0 = c.xy[3 * b + 1], y0 = c.xy[3 * b + 2];
+    __shared__ double s1[256], s2[256];
+    double            a1 = 0, a2 = 0;
+    for (int p = threadIdx.x; p < c.cs * c.cs; p += blockDim.x)
+    {
+        const int    y = p / c.cs, x = p - y * c.cs;
+        const double v = c.R[gidx(c, f, x0 + x, y0 + y)];
+        a1 += v * v;
+        if (x >= border && y >= border && x < c.cs - border && y < c.cs - border)
+            a2 += v * v;
+    }
+    s1[threadIdx.x] = a1;
+    s2[threadIdx.x] = a2;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1)
+    {
+        if ((int)threadIdx.x < off)
+        {
+            s1[threadIdx.x] += s1[threadIdx.x + off];
+            s2[threadIdx.x] += s2[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        sig[b] = sqrt(s1[0] / (double)(c.cs * c.cs));
+        rn2[b] = s2[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_crop_prep(CropSet c, const double *__restrict__ sig,
+                                                   unsigned short *__restrict__ x,
+                                                   int batch)
+{
+    const long long n = (long long)batch * c.cs * c.cs;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+    {
+        const int b = (int)(i / (c.cs * c.cs));
+        const int p = (int)(i - (long long)b * c.cs * c.cs);
+        const int y = p / c.cs, xx = p - y * c.cs;
+        const int f = c.xy[3 * b], x0 = c.xy[3 * b + 1], y0 = c.xy[3 * b + 2];
+        const double s = sig[b] > 0 ? 1.0 / sig[b] : 0.0;
+        x[i] = f2bf((float)(c.R[gidx(c, f, x0 + xx, y0 + y)] * s));
+    }
+}
+
+struct OpRef
+{
+    const double *a, *bX, *bY, *D;
+    int           nx, ny;
+    double        alpha, dhx, dhy;
+};
+
+// A applied at global cell (gx, gy) to a crop-local field u (zero outside)
+__device__ inline double apply_local(const OpRef &o, const double *u, int cs,
+                                     int x, int y, int gx, int gy)
+{
+    auto at = [&](int xx, int yy) -> double {
+        return (xx < 0 || yy < 0 || xx >= cs || yy >= cs) ? 0.0 : u[yy * cs + xx];
+    };
+    gx %= o.nx; gy %= o.ny;
+    if (gx < 0) gx += o.nx;
+    if (gy < 0) gy += o.ny;
+    const long long c   = (long long)gy * o.nx + gx;
+    const double    bxl = o.bX[(long long)gy * (o.nx + 1) + gx];
+    const double    bxr = o.bX[(long long)gy * (o.nx + 1) + gx + 1];
+    const double    byl = o.bY[(long long)gy * o.nx + gx];
+    const double    byr = o.bY[(long long)(gy + 1) * o.nx + gx];
+    const double    uc  = at(x, y);
+    return o.alpha * o.a[c] * uc
+           - o.dhx * (bxr * (at(x + 1, y) - uc) - bxl * (uc - at(x - 1, y)))
+           - o.dhy * (byr * (at(x, y + 1) - uc) - byl * (uc - at(x, y - 1)));
+}
+
+__device__ inline double Dat(const OpRef &o, int gx, int gy)
+{
+    gx %= o.nx; gy %= o.ny;
+    if (gx < 0) gx += o.nx;
+    if (gy < 0) gy += o.ny;
+    return o.D[(long long)gy * o.nx + gx];
+}
+
+// e_cnn = sigma * y / D for every crop pixel
+__global__ __launch_bounds__(256) void k_crop_e(CropSet c, OpRef o,
+                                                const double *__restrict__ sig,
+                                                const float *__restrict__ y,
+                                                double *__restrict__ e, int batch)
+{
+    const long long n = (long long)batch * c.cs * c.cs;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+    {
+        const int b = (int)(i / (c.cs * c.cs));
+        const int p = (int)(i - (long long)b * c.cs * c.cs);
+        const int yy = p / c.cs, xx = p - yy * c.cs;
+        e[i] = sig[b] * (double)y[i] / Dat(o, c.xy[3 * b + 1] + xx, c.xy[3 * b + 2] + yy);
+    }
+}
+
+// res = mask (r - A e); loss contribution res^2 / ||mask r||^2
+__global__ __launch_bounds__(256) void k_crop_res(CropSet c, OpRef o,
+                                                  const double *__restrict__ e,
+                                                  const double *__restrict__ rn2,
+                                                  double *__restrict__ res,
+                                                  double *__restrict__ loss,
+                                                  int batch, int border)
+{
+    const long long n = (long long)batch * c.cs * c.cs;
+    double          acc = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+    {
+        const int b = (int)(i / (c.cs * c.cs));
+        const int p = (int)(i - (long long)b * c.cs * c.cs);
+        const int yy = p / c.cs, xx = p - yy * c.cs;
+        double    v  = 0;
+        if (xx >= border && yy >= border && xx < c.cs - border && yy < c.cs - border)
+        {
+            const int f = c.xy[3 * b], gx = c.xy[3 * b + 1] + xx,
+                      gy = c.xy[3 * b + 2] + yy;
+            v = c.R[gidx(c, f, gx, gy)]
+                - apply_local(o, e + (long long)b * c.cs * c.cs, c.cs, xx, yy, gx, gy);
+            acc += v * v / (rn2[b] > 0 ? rn2[b] : 1.0);
+        }
+        res[i] = v;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0 && acc != 0)
+        atomicAdd(loss, acc);
+}
+
+// g_y = dL/dy = -2 A^T(res / ||mask r||^2) * sigma / D   (A symmetric)
+__global__ __launch_bounds__(256) void k_crop_gy(CropSet c, OpRef o,
+                                                 const double *__restrict__ res,
+                                                 const double *__restrict__ rn2,
+                                                 const double *__restrict__ sig,
+                                                 float *__restrict__ gy, int batch,
+                                                 float scale)
+{
+    const long long n = (long long)batch * c.cs * c.cs;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+    {
+        const int b = (int)(i / (c.cs * c.cs));
+        const int p = (int)(i - (long long)b * c.cs * c.cs);
+        const int yy = p / c.cs, xx = p - yy * c.cs;
+        const int gx = c.xy[3 * b + 1] + xx, gyy = c.xy[3 * b + 2] + yy;
+        const double ar = apply_local(o, res + (long long)b * c.cs * c.cs, c.cs, xx,
+                                      yy, gx, gyy);
+        const double ge = -2.0 * ar / (rn2[b] > 0 ? rn2[b] : 1.0);
+        gy[i] = (float)(scale * ge * sig[b] / Dat(o, gx, gyy));
+    }
+}
+
+// layer 4 backward: g_h3 = relu'(h3) * sum_t W4[t] g_y[p - off_t]
+__global__ __launch_bounds__(256) void k_bwd4_data(const float *__restrict__ gy,
+                                                   const unsigned short *__restrict__ h3,
+                                                   const float *__restrict__ W4,
+                                                   float *__restrict__ gh, Geo g)
+{
+    const long long npx = (long long)g.w * g.h * g.nimg;
+    const long long p   = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npx)
+        return;
+    const int img = (int)(p / ((long long)g.w * g.h));
+    const int rem = (int)(p - (long long)img * g.w * g.h);
+    const int y = rem / g.w, x = rem - y * g.w;
+    float     acc[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+        acc[k] = 0.f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+    {
+        // output pixel o with o + off_t = p  ->  o = p - off_t
+        const int q = nbr<false>(x, y, -(t % 3 - 1), -(t / 3 - 1), g, img);
+        if (q < 0)
+            continue;
+        const float gv = gy[q];
+#pragma unroll
+        for (int k = 0; k < C; ++k)
+            acc[k] += W4[t * C + k] * gv;
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+        gh[p * C + k] = bf2f(h3[p * C + k]) > 0.f ? acc[k] : 0.f;
+}
+
+// 16->16 layer backward (data): g_in[p][k] = relu'(h_in) sum_t sum_n W[t][n][k] g_out[p - off_t][n]
+__global__ __launch_bounds__(256) void k_bwd16_data(const float *__restrict__ gout,
+                                                    const unsigned short *__restrict__ hin,
+                                                    const float *__restrict__ W,
+                                                    float *__restrict__ gin, Geo g)
+{
+    const long long npx = (long long)g.w * g.h * g.nimg;
+    const long long p   = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npx)
+        return;
+    const int img = (int)(p / ((long long)g.w * g.h));
+    const int rem = (int)(p - (long long)img * g.w * g.h);
+    const int y = rem / g.w, x = rem - y * g.w;
+    float     acc[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+        acc[k] = 0.f;
+    for (int t = 0; t < 9; ++t)
+    {
+        const int q = nbr<false>(x, y, -(t % 3 - 1), -(t / 3 - 1), g, img);
+        if (q < 0)
+            continue;
+        for (int n = 0; n < C; ++n)
+        {
+            const float gv = gout[(long long)q * C + n];
+#pragma unroll
+            for (int k = 0; k < C; ++k)
+                acc[k] += W[(t * C + n) * C + k] * gv;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k)
+        gin[p * C + k] = bf2f(hin[p * C + k]) > 0.f ? acc[k] : 0.f;
+}
+
+// weight gradient of a 16->16 layer: dW[t][n][k] += sum_p gout[p][n] hin[p+off_t][k]
+// block: 256 threads, thread = (n, k), 9 taps each; chunk of CH pixels
+template <int CH>
+__global__ __launch_bounds__(256) void k_bwd16_weight(const float *__restrict__ gout,
+                                                      const unsigned short *__restrict__ hin,
+                                                      float *__restrict__ dW, Geo g)
+{
+    const long long npx = (long long)g.w * g.h * g.nimg;
+    const int       n = threadIdx.x >> 4, k = threadIdx.x & 15;
+    float           acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+        acc[t] = 0.f;
+    const long long pbeg = (long long)blockIdx.x * CH;
+    for (long long p = pbeg; p < pbeg + CH && p < npx; ++p)
+    {
+        const float gv = gout[p * C + n];
+        if (gv == 0.f)
+            continue;
+        const int img = (int)(p / ((long long)g.w * g.h));
+        const int rem = (int)(p - (long long)img * g.w * g.h);
+        const int y = rem / g.w, x = rem - y * g.w;
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+        {
+            const int q = nbr<false>(x, y, t % 3 - 1, t / 3 - 1, g, img);
+            if (q >= 0)
+                acc[t] += gv * bf2f(hin[(long long)q * C + k]);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+        if (acc[t] != 0.f)
+            atomicAdd(&dW[(t * C + n) * C + k], acc[t]);
+}
+
+// weight gradient of layer 4 (dW4[t][k] += sum_p g_y[p] h3[p+off_t][k]) and
+// layer 1 (dW1[t][n] += sum_p g_h1[p][n] x[p+off_t]); thread = (t, c)
+template <int CH>
+__global__ __launch_bounds__(320) void k_bwd_weight_small(const float *__restrict__ gy,
+                                                          const unsigned short *__restrict__ h3,
+                                                          const float *__restrict__ gh1,
+                                                          const unsigned short *__restrict__ x,
+                                                          float *__restrict__ dW4,
+                                                          float *__restrict__ dW1, Geo g)
+{
+    const long long npx = (long long)g.w * g.h * g.nimg;
+    const int       tid = threadIdx.x; // 0..143 for W4, 144 threads for W1 (second block half)
+    const bool      w4  = tid < 144;
+    const int       idx = w4 ? tid : tid - 144;
+    if (idx >= 144)
+        return;
+    const int t = idx / C, c = idx % C;
+    float     acc = 0.f;
+    const long long pbeg = (long long)blockIdx.x * CH;
+    for (long long p = pbeg; p < pbeg + CH && p < npx; ++p)
+    {
+        const int img = (int)(p / ((long long)g.w * g.h));
+        const int rem = (int)(p - (long long)img * g.w * g.h);
+        const int y = rem / g.w, xx = rem - y * g.w;
+        const int q = nbr<false>(xx, y, t % 3 - 1, t / 3 - 1, g, img);
+        if (q < 0)
+            continue;
+        if (w4)
+            acc += gy[p] * bf2f(h3[(long long)q * C + c]);
+        else
+            acc += gh1[p * C + c] * bf2f(x[q]);
+    }
+    if (acc != 0.f)
+        atomicAdd(w4 ? &dW4[idx] : &dW1[idx], acc);
+}
+
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ gr,
+                       float *__restrict__ m, float *__restrict__ v, int n, float lr,
+                       float b1, float b2, float c1, float c2, float eps, float clip)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float gi = gr[i];
+    gi       = gi > clip ? clip : (gi < -clip ? -clip : gi);
+    m[i]     = b1 * m[i] + (1 - b1) * gi;
+    v[i]     = b2 * v[i] + (1 - b2) * gi * gi;
+    w[i] -= lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps);
+}
+
+// bf16 copies of W2/W3 in the MFMA B layout [9][16 out][16 in] (same order)
+__global__ void k_pack_bf16(const float *__restrict__ w, unsigned short *__restrict__ wb,
+                            int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        wb[i] = f2bf(w[i]);
+}
+// forward uses bf16-rounded weights for layers 1 and 4 as well
+__global__ void k_round_bf16(const float *__restrict__ w, float *__restrict__ wr, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        wr[i] = bf2f(f2bf(w[i]));
+}
+
+} // namespace
+
+// =========================================================== network state
+struct bx_cnn
+{
+    float          *w      = nullptr; // FP32 master parameters
+    float          *wr     = nullptr; // bf16-rounded copy (FP32 storage)
+    unsigned short *wb     = nullptr; // bf16 copy (MFMA operands)
+    float          *grad   = nullptr;
+    float          *adam_m = nullptr, *adam_v = nullptr;
+    int             adam_t = 0;
+    // operator (for D and the training loss)
+    bx_helm *op = nullptr;
+    OpRef    opr{};
+    double  *D  = nullptr;
+    int      onx = 0, ony = 0;
+    // inference buffers
+    long long       cap = 0;
+    unsigned short *x = nullptr, *h1 = nullptr, *h2 = nullptr, *h3 = nullptr;
+    float          *y = nullptr;
+    double         *sig = nullptr, *part = nullptr;
+    // training buffers
+    long long tcap = 0;
+    float    *gy = nullptr, *g3 = nullptr, *g2 = nullptr, *g1 = nullptr;
+    double   *te = nullptr, *tres = nullptr, *tsig = nullptr, *trn2 = nullptr,
+           *tloss = nullptr;
+    int      *txy = nullptr;
+    int       txycap = 0;
+};
+
+static thread_local std::string g_cnn_err;
+static int cerr(int code, const std::string &m)
+{
+    g_cnn_err = m;
+    return code;
+}
+#define C_CHECK(expr)                                                           \
+    do                                                                          \
+    {                                                                           \
+        hipError_t e__ = (expr);                                                \
+        if (e__ != hipSuccess)                                                  \
+            return cerr(BX_ERR_HIP, std::string(#expr) + ": "                   \
+                                        + hipGetErrorString(e__));              \
+    } while (0)
+
+extern "C" const char *bx_cnn_last_error(void) { return g_cnn_err.c_str(); }
+
+static void cfree(void *p)
+{
+    if (p)
+        (void)hipFree(p);
+}
+
+extern "C" void bx_cnn_destroy(bx_cnn *c)
+{
+    if (!c)
+        return;
+    for (void *p : { (void *)c->w, (void *)c->wr, (void *)c->wb, (void *)c->grad,
+                     (void *)c->adam_m, (void *)c->adam_v, (void *)c->D,
+                     (void *)c->x, (void *)c->h1, (void *)c->h2, (void *)c->h3,
+                     (void *)c->y, (void *)c->sig, (void *)c->part, (void *)c->gy,
+                     (void *)c->g3, (void *)c->g2, (void *)c->g1, (void *)c->te,
+                     (void *)c->tres, (void *)c->tsig, (void *)c->trn2,
+                     (void *)c->tloss, (void *)c->txy })
+        cfree(p);
+    delete c;
+}
+
+static int refresh_copies(bx_cnn *c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_round_bf16, dim3((NPAR + 255) / 256), dim3(256), 0, s,
+                       c->w, c->wr, NPAR);
+    hipLaunchKernelGGL(k_pack_bf16, dim3((NPAR + 255) / 256), dim3(256), 0, s,
+                       c->w, c->wb, NPAR);
+    C_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_cnn_create(unsigned seed, bx_cnn **out)
+{
+    if (!out)
+        return cerr(BX_ERR_INVALID, "bx_cnn_create: null");
+    bx_cnn *c = new bx_cnn();
+    if (hipMalloc((void **)&c->w, NPAR * 4) || hipMalloc((void **)&c->wr, NPAR * 4)
+        || hipMalloc((void **)&c->wb, NPAR * 2) || hipMalloc((void **)&c->grad, NPAR * 4)
+        || hipMalloc((void **)&c->adam_m, NPAR * 4)
+        || hipMalloc((void **)&c->adam_v, NPAR * 4)
+        || hipMalloc((void **)&c->sig, 64) || hipMalloc((void **)&c->part, 1024 * 8))
+    {
+        bx_cnn_destroy(c);
+        return cerr(BX_ERR_HIP, "bx_cnn_create: hipMalloc failed");
+    }
+    // He-style initialisation around a Jacobi-like identity path: the centre
+    // tap carries most of the signal so the untrained net is close to
+    // e = sigma/D * relu-split identity.
+    std::mt19937             rng(seed);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    std::vector<float>       w(NPAR);
+    for (int i = 0; i < NPAR; ++i)
+        w[i] = 0.f;
+    const float s1 = std::sqrt(2.f / 9.f), s16 = std::sqrt(2.f / (9.f * 16.f));
+    for (int i = 0; i < NW1; ++i)
+        w[OFF1 + i] = 0.1f * s1 * nd(rng);
+    for (int i = 0; i < NW2; ++i)
+        w[OFF2 + i] = 0.1f * s16 * nd(rng);
+    for (int i = 0; i < NW3; ++i)
+        w[OFF3 + i] = 0.1f * s16 * nd(rng);
+    for (int i = 0; i < NW4; ++i)
+        w[OFF4 + i] = 0.1f * s16 * nd(rng);
+    // identity-like path: channel 0 = relu(x), channel 1 = relu(-x)
+    w[OFF1 + 4 * C + 0] += 1.f;
+    w[OFF1 + 4 * C + 1] -= 1.f;
+    for (int L : { OFF2, OFF3 })
+    {
+        w[L + (4 * C + 0) * C + 0] += 1.f;
+        w[L + (4 * C + 1) * C + 1] += 1.f;
+    }
+    w[OFF4 + 4 * C + 0] += 1.f;
+    w[OFF4 + 4 * C + 1] -= 1.f;
+    if (hipMemcpy(c->w, w.data(), NPAR * 4, hipMemcpyHostToDevice)
+        || hipMemset(c->adam_m, 0, NPAR * 4) || hipMemset(c->adam_v, 0, NPAR * 4))
+    {
+        bx_cnn_destroy(c);
+        return cerr(BX_ERR_HIP, "bx_cnn_create: init copy failed");
+    }
+    int rc = refresh_copies(c, nullptr);
+    if (rc)
+    {
+        bx_cnn_destroy(c);
+        return rc;
+    }
+    (void)hipDeviceSynchronize();
+    *out = c;
+    return BX_OK;
+}
+
+extern "C" int bx_cnn_set_params(bx_cnn *c, const float *p, int n)
+{
+    if (!c || !p || n != NPAR)
+        return cerr(BX_ERR_INVALID, "bx_cnn_set_params: expected BX_CNN_NPARAMS values");
+    C_CHECK(hipMemcpy(c->w, p, NPAR * 4, hipMemcpyHostToDevice));
+    C_CHECK(hipMemset(c->adam_m, 0, NPAR * 4));
+    C_CHECK(hipMemset(c->adam_v, 0, NPAR * 4));
+    c->adam_t = 0;
+    int rc    = refresh_copies(c, nullptr);
+    C_CHECK(hipDeviceSynchronize());
+    return rc;
+}
+
+extern "C" int bx_cnn_get_params(bx_cnn *c, float *p, int n)
+{
+    if (!c || !p || n != NPAR)
+        return cerr(BX_ERR_INVALID, "bx_cnn_get_params: expected BX_CNN_NPARAMS values");
+    C_CHECK(hipMemcpy(p, c->w, NPAR * 4, hipMemcpyDeviceToHost));
+    return BX_OK;
+}
+
+// registers the operator: diagonal for the scaling, coefficients for the loss
+extern "C" int bx_cnn_set_operator(bx_cnn *c, int nx, int ny, double alpha,
+                                   double beta, double dx, double dy,
+                                   const double *a, const double *bX,
+                                   const double *bY, void *stream)
+{
+    if (!c || !a || !bX || !bY || nx < 16 || ny < 1)
+        return cerr(BX_ERR_INVALID, "bx_cnn_set_operator: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (c->onx != nx || c->ony != ny)
+    {
+        cfree(c->D);
+        c->D = nullptr;
+        C_CHECK(hipMalloc((void **)&c->D, sizeof(double) * nx * ny));
+    }
+    c->onx          = nx;
+    c->ony          = ny;
+    const double dhx = beta / (dx * dx), dhy = beta / (dy * dy);
+    hipLaunchKernelGGL(k_diag, dim3((nx + 255) / 256, ny), dim3(256), 0, s, a, bX, bY,
+                       c->D, nx, ny, alpha, dhx, dhy);
+    C_CHECK(hipGetLastError());
+    c->opr = OpRef{ a, bX, bY, c->D, nx, ny, alpha, dhx, dhy };
+    return BX_OK;
+}
+
+static int ensure_infer(bx_cnn *c, long long npx)
+{
+    if (npx <= c->cap)
+        return BX_OK;
+    for (void *p : { (void *)c->x, (void *)c->h1, (void *)c->h2, (void *)c->h3,
+                     (void *)c->y })
+        cfree(p);
+    c->x = c->h1 = c->h2 = c->h3 = nullptr;
+    c->y = nullptr;
+    C_CHECK(hipMalloc((void **)&c->x, npx * 2));
+    C_CHECK(hipMalloc((void **)&c->h1, npx * C * 2));
+    C_CHECK(hipMalloc((void **)&c->h2, npx * C * 2));
+    C_CHECK(hipMalloc((void **)&c->h3, npx * C * 2));
+    C_CHECK(hipMalloc((void **)&c->y, npx * 4));
+    c->cap = npx;
+    return BX_OK;
+}
+
+template <bool PER>
+static void net_forward(bx_cnn *c, const Geo &g, hipStream_t s)
+{
+    const long long npx = (long long)g.w * g.h * g.nimg;
+    const unsigned  nb  = (unsigned)((npx + 255) / 256);
+    hipLaunchKernelGGL((k_conv1<PER>), dim3(nb), dim3(256), 0, s, c->x, c->h1,
+                       c->wr + OFF1, g);
+    const unsigned nbt = (unsigned)((npx / 16 * 64 + 255) / 256);
+    hipLaunchKernelGGL((k_conv16<PER>), dim3(nbt), dim3(256), 0, s, c->h1, c->h2,
+                       c->wb + OFF2, g);
+    hipLaunchKernelGGL((k_conv16<PER>), dim3(nbt), dim3(256), 0, s, c->h2, c->h3,
+                       c->wb + OFF3, g);
+    hipLaunchKernelGGL((k_conv4<PER>), dim3(nb), dim3(256), 0, s, c->h3, c->y,
+                       c->wr + OFF4, g);
+}
+
+int bx_cnn_forward(bx_cnn *c, const double *r, double *e, int nx, int ny,
+                   double omega, hipStream_t s)
+{
+    if (!c || !c->D || nx != c->onx || ny != c->ony || nx % 16)
+        return cerr(BX_ERR_STATE, "bx_cnn_forward: operator not registered or nx % 16 != 0");
+    const long long n = (long long)nx * ny;
+    int             rc;
+    if ((rc = ensure_infer(c, n)))
+        return rc;
+    hipLaunchKernelGGL(k_sumsq, dim3(512), dim3(256), 0, s, r, n, c->part);
+    hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, s, c->part, 512, n, c->sig);
+    hipLaunchKernelGGL(k_prep_full, dim3(2048), dim3(256), 0, s, r, c->x, n, c->sig);
+    net_forward<true>(c, Geo{ nx, ny, 1 }, s);
+    hipLaunchKernelGGL(k_finish_full, dim3(2048), dim3(256), 0, s, c->y, c->D, e, n,
+                       omega, c->sig);
+    C_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_cnn_infer(bx_cnn *c, const double *r, double *e, int nx, int ny,
+                            double omega, void *stream)
+{
+    return bx_cnn_forward(c, r, e, nx, ny, omega, (hipStream_t)stream);
+}
+
+// Raw network on a batch of bf16-quantised inputs (zero padding): y = N(x).
+// x_in: float host-side values are rounded to bf16 first.  For tests.
+extern "C" int bx_cnn_net_forward(bx_cnn *c, const float *x_dev, float *y_dev, int w,
+                                  int h, int nimg, int periodic, void *stream)
+{
+    if (!c || !x_dev || !y_dev || w % 16)
+        return cerr(BX_ERR_INVALID, "bx_cnn_net_forward: bad argument");
+    hipStream_t     s = (hipStream_t)stream;
+    const long long n = (long long)w * h * nimg;
+    int             rc;
+    if ((rc = ensure_infer(c, n)))
+        return rc;
+    hipLaunchKernelGGL(k_pack_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       x_dev, c->x, (int)n);
+    if (periodic)
+        net_forward<true>(c, Geo{ w, h, nimg }, s);
+    else
+        net_forward<false>(c, Geo{ w, h, nimg }, s);
+    C_CHECK(hipMemcpyAsync(y_dev, c->y, n * 4, hipMemcpyDeviceToDevice, s));
+    C_CHECK(hipStreamSynchronize(s));
+    return BX_OK;
+}
+
+static int ensure_train(bx_cnn *c, long long npx, int batch)
+{
+    if (npx > c->tcap)
+    {
+        for (void *p : { (void *)c->gy, (void *)c->g3, (void *)c->g2, (void *)c->g1,
+                         (void *)c->te, (void *)c->tres })
+            cfree(p);
+        C_CHECK(hipMalloc((void **)&c->gy, npx * 4));
+        C_CHECK(hipMalloc((void **)&c->g3, npx * C * 4));
+        C_CHECK(hipMalloc((void **)&c->g2, npx * C * 4));
+        C_CHECK(hipMalloc((void **)&c->g1, npx * C * 4));
+        C_CHECK(hipMalloc((void **)&c->te, npx * 8));
+        C_CHECK(hipMalloc((void **)&c->tres, npx * 8));
+        c->tcap = npx;
+    }
+    if (batch > c->txycap)
+    {
+        for (void *p : { (void *)c->tsig, (void *)c->trn2, (void *)c->txy, (void *)c->tloss })
+            cfree(p);
+        C_CHECK(hipMalloc((void **)&c->tsig, batch * 8));
+        C_CHECK(hipMalloc((void **)&c->trn2, batch * 8));
+        C_CHECK(hipMalloc((void **)&c->txy, batch * 12));
+        C_CHECK(hipMalloc((void **)&c->tloss, 64));
+        c->txycap = batch;
+    }
+    return ensure_infer(c, npx);
+}
+
+// One loss + gradient evaluation on the crops `xy` (device, batch x 3 ints).
+static int loss_grad(bx_cnn *c, const double *R, int nfields, int nx, int ny,
+                     int batch, int cs, double *loss_host, hipStream_t s)
+{
+    (void)nfields;
+    const int       border = 5;
+    const long long npx    = (long long)batch * cs * cs;
+    CropSet         cset{ R, c->txy, nx, ny, cs };
+    Geo             g{ cs, cs, batch };
+    const unsigned  nb = (unsigned)((npx + 255) / 256);
+    hipLaunchKernelGGL(k_crop_stats, dim3(batch), dim3(256), 0, s, cset, c->tsig,
+                       c->trn2, border);
+    hipLaunchKernelGGL(k_crop_prep, dim3(2048), dim3(256), 0, s, cset, c->tsig, c->x,
+                       batch);
+    net_forward<false>(c, g, s);
+    hipLaunchKernelGGL(k_crop_e, dim3(2048), dim3(256), 0, s, cset, c->opr, c->tsig,
+                       c->y, c->te, batch);
+    C_CHECK(hipMemsetAsync(c->tloss, 0, 8, s));
+    hipLaunchKernelGGL(k_crop_res, dim3(2048), dim3(256), 0, s, cset, c->opr, c->te,
+                       c->trn2, c->tres, c->tloss, batch, border);
+    const float inv_b = 1.0f / (float)batch;
+    hipLaunchKernelGGL(k_crop_gy, dim3(2048), dim3(256), 0, s, cset, c->opr, c->tres,
+                       c->trn2, c->tsig, c->gy, batch, inv_b);
+    C_CHECK(hipMemsetAsync(c->grad, 0, NPAR * 4, s));
+    // layer 4
+    hipLaunchKernelGGL(k_bwd4_data, dim3(nb), dim3(256), 0, s, c->gy, c->h3,
+                       c->wr + OFF4, c->g3, g);
+    // layer 3: g3 is dL/d(h3 pre-activation); weights W3 map h2 -> h3
+    hipLaunchKernelGGL(k_bwd16_data, dim3(nb), dim3(256), 0, s, c->g3, c->h2,
+                       c->wr + OFF3, c->g2, g);
+    hipLaunchKernelGGL((k_bwd16_weight<1024>), dim3((unsigned)((npx + 1023) / 1024)),
+                       dim3(256), 0, s, c->g3, c->h2, c->grad + OFF3, g);
+    // layer 2: W2 maps h1 -> h2
+    hipLaunchKernelGGL(k_bwd16_data, dim3(nb), dim3(256), 0, s, c->g2, c->h1,
+                       c->wr + OFF2, c->g1, g);
+    hipLaunchKernelGGL((k_bwd16_weight<1024>), dim3((unsigned)((npx + 1023) / 1024)),
+                       dim3(256), 0, s, c->g2, c->h1, c->grad + OFF2, g);
+    // layers 4 and 1 weights
+    hipLaunchKernelGGL((k_bwd_weight_small<1024>), dim3((unsigned)((npx + 1023) / 1024)),
+                       dim3(288), 0, s, c->gy, c->h3, c->g1, c->x, c->grad + OFF4,
+                       c->grad + OFF1, g);
+    C_CHECK(hipGetLastError());
+    double l = 0;
+    C_CHECK(hipMemcpyAsync(&l, c->tloss, 8, hipMemcpyDeviceToHost, s));
+    C_CHECK(hipStreamSynchronize(s));
+    *loss_host = l / batch;
+    return BX_OK;
+}
+
+extern "C" int bx_cnn_loss_grad(bx_cnn *c, const double *R, int nfields, int nx,
+                                int ny, const int *crops_host, int batch, int crop,
+                                double *loss, float *grad_host, void *stream)
+{
+    if (!c || !c->D || !R || !crops_host || batch < 1 || crop < 16 || crop % 16
+        || !loss)
+        return cerr(BX_ERR_INVALID, "bx_cnn_loss_grad: bad argument");
+    if (nx != c->onx || ny != c->ony)
+        return cerr(BX_ERR_STATE, "bx_cnn_loss_grad: field size differs from the operator");
+    for (int b = 0; b < batch; ++b)
+        if (crops_host[3 * b] < 0 || crops_host[3 * b] >= nfields)
+            return cerr(BX_ERR_INVALID, "bx_cnn_loss_grad: crop field index out of range");
+    hipStream_t s = (hipStream_t)stream;
+    int         rc;
+    if ((rc = ensure_train(c, (long long)batch * crop * crop, batch)))
+        return rc;
+    C_CHECK(hipMemcpyAsync(c->txy, crops_host, batch * 12, hipMemcpyHostToDevice, s));
+    if ((rc = loss_grad(c, R, nfields, nx, ny, batch, crop, loss, s)))
+        return rc;
+    if (grad_host)
+    {
+        C_CHECK(hipMemcpy(grad_host, c->grad, NPAR * 4, hipMemcpyDeviceToHost));
+    }
+    return BX_OK;
+}
+
+extern "C" int bx_cnn_train(bx_cnn *c, const double *R, int nfields, int nx, int ny,
+                            int crop, int batch, int steps, float lr, unsigned seed,
+                            double *loss_history, void *stream)
+{
+    if (!c || !c->D || !R || nfields < 1 || batch < 1 || steps < 1 || crop < 16
+        || crop % 16 || !(lr > 0))
+        return cerr(BX_ERR_INVALID, "bx_cnn_train: bad argument");
+    if (nx != c->onx || ny != c->ony)
+        return cerr(BX_ERR_STATE, "bx_cnn_train: field size differs from the operator");
+    hipStream_t s = (hipStream_t)stream;
+    int         rc;
+    if ((rc = ensure_train(c, (long long)batch * crop * crop, batch)))
+        return rc;
+    std::mt19937     rng(seed);
+    std::vector<int> xy(3 * batch);
+    const float      b1 = 0.9f, b2 = 0.999f;
+    for (int st = 0; st < steps; ++st)
+    {
+        for (int b = 0; b < batch; ++b)
+        {
+            xy[3 * b]     = (int)(rng() % (unsigned)nfields);
+            xy[3 * b + 1] = (int)(rng() % (unsigned)nx);
+            xy[3 * b + 2] = (int)(rng() % (unsigned)ny);
+        }
+        C_CHECK(hipMemcpyAsync(c->txy, xy.data(), batch * 12, hipMemcpyHostToDevice, s));
+        double l = 0;
+        if ((rc = loss_grad(c, R, nfields, nx, ny, batch, crop, &l, s)))
+            return rc;
+        if (loss_history)
+            loss_history[st] = l;
+        ++c->adam_t;
+        const float c1 = 1.f - std::pow(b1, (float)c->adam_t);
+        const float c2 = 1.f - std::pow(b2, (float)c->adam_t);
+        hipLaunchKernelGGL(k_adam, dim3((NPAR + 255) / 256), dim3(256), 0, s, c->w,
+                           c->grad, c->adam_m, c->adam_v, NPAR, lr, b1, b2, c1, c2,
+                           1e-8f, 10.f);
+        if ((rc = refresh_copies(c, s)))
+            return rc;
+    }
+    C_CHECK(hipStreamSynchronize(s));
+    return BX_OK;
+}

@@ -1,0 +1,10 @@
+// Internal interface between the Helmholtz solver and the learned smoother.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/bamrex_hip.h"
+
+// e = omega * (sigma / D) * N(r / sigma) on an nx x ny periodic field, with
+// sigma = rms(r) and D the operator diagonal registered by bx_cnn_set_operator.
+int bx_cnn_forward(bx_cnn *net, const double *r, double *e, int nx, int ny,
+                   double omega, hipStream_t stream);

@@ -1,0 +1,1012 @@
+// Variable-coefficient pressure Helmholtz solve of the IMEX (PEEK/PESK)
+// schemes on one 2D periodic level: the operator, a geometric multigrid with
+// the semantics of AMReX MLMG (the reference's "plain MLMG" solver) and an
+// FP64 flexible GMRES whose preconditioner is a multigrid V-cycle or a
+// V-cycle whose fine-level smoothing is a learned convolutional correction.
+//
+// Reference path (Source/IMEX/IMEX_O1.cpp:392-414, 570-640):
+//   A p = alpha a p - beta div(b grad p) + gamma c . grad p
+//   alpha = 1, a = eps/(gamma-1), beta = adt^2, b = face-averaged enthalpy,
+//   gamma = -eps adt/2 (split KE) or 0 (PEEK)      setup_pressure_op
+//   apply:   mlalapgrad_adotx   Source/LinearSolver/MLALapGrad_2D_K.H:9-33
+//   smooth:  alapgrad_gsrb      Source/LinearSolver/MLALapGrad_2D_K.H:179-229
+//            with dhx = beta/(h*h), cdhx = gamma/(2h) (MLALaplacianGrad.H:1243-1249)
+//   coarse coefficients: average_down of a and c, average_down_faces of b
+//            (MLALaplacianGrad.H:950-1027)
+// MLMG defaults restated (AMReX is not available here): V-cycle, 2 pre- and
+// 2 post-smoothing red-black sweeps, cell-average restriction, piecewise
+// constant interpolation, coarsening by 2 while both extents stay even and
+// >= 8; the bottom level is smoothed to convergence with red-black sweeps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bamrex_hip.h"
+#include "bx_helmholtz.h"
+
+namespace bx {
+
+// ------------------------------------------------------------- kernels
+template <class T>
+__global__ __launch_bounds__(256) void k_apply(
+    const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ a,
+    const T *__restrict__ bX, const T *__restrict__ bY,
+    const T *__restrict__ cX, const T *__restrict__ cY, int nx, int ny,
+    T alpha, T beta, T gamma, T dxinv, T dyinv, const T *__restrict__ rhs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= nx)
+        return;
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+    const long long c   = (long long)j * nx + i;
+    const T         xc  = x[c];
+    const T         dhx = beta * dxinv * dxinv;
+    const T         dhy = beta * dyinv * dyinv;
+    // faces: bX(i,j) at (long)j*(nx+1)+i, bY(i,j) at (long)j*nx+i
+    const T bxl = bX[(long long)j * (nx + 1) + i];
+    const T bxr = bX[(long long)j * (nx + 1) + i + 1];
+    const T byl = bY[(long long)j * nx + i];
+    const T byr = bY[(long long)(j + 1) * nx + i];
+    const T xl = x[(long long)j * nx + im], xr = x[(long long)j * nx + ip];
+    const T xd = x[(long long)jm * nx + i], xu = x[(long long)jp * nx + i];
+    T v = alpha * a[c] * xc - dhx * (bxr * (xr - xc) - bxl * (xc - xl))
+          - dhy * (byr * (xu - xc) - byl * (xc - xd));
+    if (cX)
+    {
+        const T chdx = gamma * cX[c] * T(0.5) * dxinv;
+        const T chdy = gamma * cY[c] * T(0.5) * dyinv;
+        v = v + chdx * (xr - xl) + chdy * (xu - xd);
+    }
+    // residual form when rhs is given: y = rhs - A x
+    y[c] = rhs ? rhs[c] - v : v;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_gsrb(
+    T *__restrict__ phi, const T *__restrict__ rhs, const T *__restrict__ a,
+    const T *__restrict__ bX, const T *__restrict__ bY,
+    const T *__restrict__ cX, const T *__restrict__ cY, int nx, int ny,
+    T alpha, T dhx, T dhy, T cdhx, T cdhy, int redblack)
+{
+    // thread covers cells of one colour: i = 2*t + ((j + redblack) & 1)
+    const int j = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = 2 * t + ((j + redblack) & 1);
+    if (i >= nx)
+        return;
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+    const long long c   = (long long)j * nx + i;
+    const T bxl = bX[(long long)j * (nx + 1) + i];
+    const T bxr = bX[(long long)j * (nx + 1) + i + 1];
+    const T byl = bY[(long long)j * nx + i];
+    const T byr = bY[(long long)(j + 1) * nx + i];
+    const T pl = phi[(long long)j * nx + im], pr = phi[(long long)j * nx + ip];
+    const T pd = phi[(long long)jm * nx + i], pu = phi[(long long)jp * nx + i];
+    // periodic level: no boundary correction terms (delta = 0)
+    const T gamma = alpha * a[c] + dhx * (bxl + bxr) + dhy * (byl + byr);
+    T rho = dhx * (bxl * pl + bxr * pr) + dhy * (byl * pd + byr * pu);
+    if (cX)
+        rho = rho - cdhx * cX[c] * (pr - pl) - cdhy * cY[c] * (pu - pd);
+    phi[c] = (rhs[c] + rho - phi[c] * T(0)) / (gamma - T(0));
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict(const T *__restrict__ f,
+                                                  T *__restrict__ c, int ncx,
+                                                  int ncy)
+{
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y;
+    if (I >= ncx)
+        return;
+    const int       nfx = 2 * ncx;
+    const long long f0  = (long long)(2 * J) * nfx + 2 * I;
+    c[(long long)J * ncx + I]
+        = T(0.25) * (f[f0] + f[f0 + 1] + f[f0 + nfx] + f[f0 + nfx + 1]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_prolong_add(const T *__restrict__ c,
+                                                     T *__restrict__ f, int nfx,
+                                                     int nfy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= nfx)
+        return;
+    f[(long long)j * nfx + i] += c[(long long)(j >> 1) * (nfx >> 1) + (i >> 1)];
+}
+
+// coarse face coefficients: average of the two fine faces on the coarse face
+template <class T>
+__global__ void k_avg_faces_x(const T *__restrict__ f, T *__restrict__ c,
+                              int ncx, int ncy)
+{
+    const int I = blockIdx.x * blockDim.x + threadIdx.x; // 0..ncx
+    const int J = blockIdx.y;
+    if (I > ncx)
+        return;
+    const int nfx = 2 * ncx;
+    c[(long long)J * (ncx + 1) + I]
+        = T(0.5) * (f[(long long)(2 * J) * (nfx + 1) + 2 * I]
+                    + f[(long long)(2 * J + 1) * (nfx + 1) + 2 * I]);
+}
+
+template <class T>
+__global__ void k_avg_faces_y(const T *__restrict__ f, T *__restrict__ c,
+                              int ncx, int ncy)
+{
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y; // 0..ncy
+    if (I >= ncx)
+        return;
+    const int nfx = 2 * ncx;
+    c[(long long)J * ncx + I] = T(0.5)
+                                * (f[(long long)(2 * J) * nfx + 2 * I]
+                                   + f[(long long)(2 * J) * nfx + 2 * I + 1]);
+}
+
+template <class TO, class TI>
+__global__ __launch_bounds__(256) void k_convert(const TI *__restrict__ in,
+                                                 TO *__restrict__ out,
+                                                 long long n, TI scale)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = (TO)(in[i] * scale);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_fill(T *__restrict__ p, long long n,
+                                              T v)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// y += alpha * x  (alpha from a device scalar times a host factor)
+__global__ __launch_bounds__(256) void k_axpy(double *__restrict__ y,
+                                              const double *__restrict__ x,
+                                              long long n, double alpha)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] = y[i] + alpha * x[i];
+}
+
+__global__ __launch_bounds__(256) void k_scale(double *__restrict__ y,
+                                               const double *__restrict__ x,
+                                               long long n, double s)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] = x[i] * s;
+}
+
+// partial dot products of w with k vectors V[0..k-1] (stride n)
+__global__ __launch_bounds__(256) void k_multidot(
+    const double *__restrict__ V, const double *__restrict__ w, long long n,
+    int k, double *__restrict__ partial)
+{
+    __shared__ double sh[8][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int v0 = 0; v0 < k; v0 += 16)
+    {
+        const int nv = min(16, k - v0);
+        double    acc[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            acc[q] = 0;
+        for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+             i < n; i += (long long)gridDim.x * blockDim.x)
+        {
+            const double wi = w[i];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if (q < nv)
+                    acc[q] += V[(long long)(v0 + q) * n + i] * wi;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+        {
+            double s = acc[q];
+            for (int off = 32; off > 0; off >>= 1)
+                s += __shfl_xor(s, off);
+            if (lane == 0)
+                sh[wv][q] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < nv)
+        {
+            double s = 0;
+            for (int q = 0; q < (int)(blockDim.x >> 6); ++q)
+                s += sh[q][threadIdx.x];
+            partial[(long long)(v0 + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_reduce_partials(const double *__restrict__ partial, int nb,
+                                  int k, double *__restrict__ out)
+{
+    // one block per vector
+    __shared__ double sh[256];
+    const int v = blockIdx.x;
+    double    s = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        s += partial[(long long)v * nb + b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1)
+    {
+        if ((int)threadIdx.x < off)
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[v] = sh[0];
+}
+
+// w -= sum_q h[q] V[q]
+__global__ __launch_bounds__(256) void k_subtract_combo(
+    double *__restrict__ w, const double *__restrict__ V,
+    const double *__restrict__ h, int k, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+    {
+        double s = w[i];
+        for (int q = 0; q < k; ++q)
+            s -= h[q] * V[(long long)q * n + i];
+        w[i] = s;
+    }
+}
+
+// x += sum_q y[q] Z[q]
+__global__ __launch_bounds__(256) void k_add_combo(double *__restrict__ x,
+                                                   const double *__restrict__ Z,
+                                                   const double *__restrict__ y,
+                                                   int k, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+    {
+        double s = x[i];
+        for (int q = 0; q < k; ++q)
+            s += y[q] * Z[(long long)q * n + i];
+        x[i] = s;
+    }
+}
+
+// ------------------------------------------------------------ host class
+static dim3 grid2(int nx, int ny, int bs = 256)
+{
+    return dim3((unsigned)((nx + bs - 1) / bs), (unsigned)ny, 1);
+}
+
+template <class T> struct MGLevel
+{
+    int nx = 0, ny = 0;
+    T   hx = 0, hy = 0;       // cell sizes (smoother: beta / h^2)
+    T   dxinv = 0, dyinv = 0; // 1 / h (operator apply)
+    T  *a = nullptr, *bX = nullptr, *bY = nullptr, *cX = nullptr, *cY = nullptr;
+    T  *x = nullptr, *rhs = nullptr, *res = nullptr;
+};
+
+template <class T> struct MGHierarchy
+{
+    std::vector<MGLevel<T>> lev;
+    T                       alpha = 0, beta = 0, gamma = 0;
+    int                     nu1 = 2, nu2 = 2, nu_bottom = 64;
+
+    void free_all()
+    {
+        for (auto &l : lev)
+            for (T *p : { l.a, l.bX, l.bY, l.cX, l.cY, l.x, l.rhs, l.res })
+                if (p)
+                    (void)hipFree(p);
+        lev.clear();
+    }
+};
+
+} // namespace bx
+
+using namespace bx;
+
+struct bx_helm
+{
+    int    nx = 0, ny = 0;
+    double dx = 0, dy = 0;
+    long long n = 0;
+    bool   has_c = false;
+    double alpha = 0, beta = 0, gamma = 0;
+    // fine-level FP64 operator (owned copies)
+    double *a = nullptr, *bX = nullptr, *bY = nullptr, *cX = nullptr,
+           *cY = nullptr;
+    MGHierarchy<double> mg64;
+    MGHierarchy<float>  mg32;
+    int                 precond_fp32 = 1;
+    // FGMRES workspace
+    int     m = 20;
+    double *V = nullptr, *Z = nullptr, *w = nullptr, *r = nullptr,
+           *partial = nullptr, *dots = nullptr, *hdev = nullptr;
+    int     nb_dot = 512;
+    // learned smoother
+    bx_cnn *cnn       = nullptr;
+    double  cnn_omega = 1.0;
+    double *tmp1 = nullptr, *tmp2 = nullptr;
+    float  *fr = nullptr, *fe = nullptr;
+    bool    coeffs_set = false;
+};
+
+static thread_local std::string g_helm_err;
+static int herr(int code, const std::string &m)
+{
+    g_helm_err = m;
+    return code;
+}
+#define H_CHECK(expr)                                                           \
+    do                                                                          \
+    {                                                                           \
+        hipError_t e__ = (expr);                                                \
+        if (e__ != hipSuccess)                                                  \
+            return herr(BX_ERR_HIP, std::string(#expr) + ": "                   \
+                                        + hipGetErrorString(e__));              \
+    } while (0)
+
+extern "C" const char *bx_helm_last_error(void) { return g_helm_err.c_str(); }
+
+template <class T> static int halloc(T **p, long long n)
+{
+    H_CHECK(hipMalloc((void **)p, sizeof(T) * std::max<long long>(n, 1)));
+    return BX_OK;
+}
+
+template <class T>
+static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s)
+{
+    mg.free_all();
+    mg.alpha = (T)h->alpha;
+    mg.beta  = (T)h->beta;
+    mg.gamma = (T)h->gamma;
+    int nx = h->nx, ny = h->ny;
+    double dx = h->dx, dy = h->dy;
+    int rc;
+    while (true)
+    {
+        MGLevel<T> L;
+        L.nx    = nx;
+        L.ny    = ny;
+        L.hx    = (T)dx;
+        L.hy    = (T)dy;
+        L.dxinv = (T)(1.0 / dx);
+        L.dyinv = (T)(1.0 / dy);
+        const long long nc = (long long)nx * ny;
+        if ((rc = halloc(&L.a, nc)) || (rc = halloc(&L.bX, (long long)(nx + 1) * ny))
+            || (rc = halloc(&L.bY, (long long)nx * (ny + 1))) || (rc = halloc(&L.x, nc))
+            || (rc = halloc(&L.rhs, nc)) || (rc = halloc(&L.res, nc)))
+            return rc;
+        if (h->has_c && ((rc = halloc(&L.cX, nc)) || (rc = halloc(&L.cY, nc))))
+            return rc;
+        mg.lev.push_back(L);
+        if (nx % 2 || ny % 2 || nx / 2 < 8 || ny / 2 < 8)
+            break;
+        nx /= 2;
+        ny /= 2;
+        dx *= 2;
+        dy *= 2;
+    }
+    // level 0 coefficients from the FP64 fine arrays
+    MGLevel<T> &L0 = mg.lev[0];
+    const long long n0 = (long long)h->nx * h->ny;
+    hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, s, h->a,
+                       L0.a, n0, 1.0);
+    hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, s, h->bX,
+                       L0.bX, (long long)(h->nx + 1) * h->ny, 1.0);
+    hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, s, h->bY,
+                       L0.bY, (long long)h->nx * (h->ny + 1), 1.0);
+    if (h->has_c)
+    {
+        hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, s,
+                           h->cX, L0.cX, n0, 1.0);
+        hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, s,
+                           h->cY, L0.cY, n0, 1.0);
+    }
+    for (size_t l = 1; l < mg.lev.size(); ++l)
+    {
+        MGLevel<T> &F = mg.lev[l - 1], &C = mg.lev[l];
+        hipLaunchKernelGGL((k_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s,
+                           F.a, C.a, C.nx, C.ny);
+        hipLaunchKernelGGL((k_avg_faces_x<T>), grid2(C.nx + 1, C.ny), dim3(256),
+                           0, s, F.bX, C.bX, C.nx, C.ny);
+        hipLaunchKernelGGL((k_avg_faces_y<T>), grid2(C.nx, C.ny + 1), dim3(256),
+                           0, s, F.bY, C.bY, C.nx, C.ny);
+        if (h->has_c)
+        {
+            hipLaunchKernelGGL((k_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0,
+                               s, F.cX, C.cX, C.nx, C.ny);
+            hipLaunchKernelGGL((k_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0,
+                               s, F.cY, C.cY, C.nx, C.ny);
+        }
+    }
+    H_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+template <class T>
+static void launch_apply(const MGLevel<T> &L, const MGHierarchy<T> &mg,
+                         const T *x, T *y, const T *rhs, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_apply<T>), grid2(L.nx, L.ny), dim3(256), 0, s, x, y,
+                       L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.alpha,
+                       mg.beta, mg.gamma, L.dxinv, L.dyinv, rhs);
+}
+
+template <class T>
+static void launch_smooth(const MGLevel<T> &L, const MGHierarchy<T> &mg,
+                          T *phi, const T *rhs, int redblack, hipStream_t s)
+{
+    const T hx = L.hx, hy = L.hy;
+    const T dhx = mg.beta / (hx * hx), dhy = mg.beta / (hy * hy);
+    const T cdhx = mg.gamma / (2 * hx), cdhy = mg.gamma / (2 * hy);
+    hipLaunchKernelGGL((k_gsrb<T>), grid2((L.nx + 1) / 2, L.ny), dim3(256), 0,
+                       s, phi, rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny,
+                       mg.alpha, dhx, dhy, cdhx, cdhy, redblack);
+}
+
+template <class T>
+static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s)
+{
+    MGLevel<T> &L = mg.lev[l];
+    if (l + 1 == (int)mg.lev.size())
+    {
+        for (int it = 0; it < mg.nu_bottom; ++it)
+        {
+            launch_smooth(L, mg, L.x, L.rhs, 0, s);
+            launch_smooth(L, mg, L.x, L.rhs, 1, s);
+        }
+        return;
+    }
+    for (int it = 0; it < mg.nu1; ++it)
+    {
+        launch_smooth(L, mg, L.x, L.rhs, 0, s);
+        launch_smooth(L, mg, L.x, L.rhs, 1, s);
+    }
+    launch_apply(L, mg, L.x, L.res, L.rhs, s);
+    MGLevel<T> &C = mg.lev[l + 1];
+    hipLaunchKernelGGL((k_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s,
+                       L.res, C.rhs, C.nx, C.ny);
+    hipLaunchKernelGGL((k_fill<T>), dim3(256), dim3(256), 0, s, C.x,
+                       (long long)C.nx * C.ny, (T)0);
+    vcycle(mg, l + 1, s);
+    hipLaunchKernelGGL((k_prolong_add<T>), grid2(L.nx, L.ny), dim3(256), 0, s,
+                       C.x, L.x, L.nx, L.ny);
+    for (int it = 0; it < mg.nu2; ++it)
+    {
+        launch_smooth(L, mg, L.x, L.rhs, 0, s);
+        launch_smooth(L, mg, L.x, L.rhs, 1, s);
+    }
+}
+
+// coarse-grid part of the V-cycle only (levels >= 1), from L.res on level 0
+template <class T>
+static void coarse_correction(MGHierarchy<T> &mg, hipStream_t s)
+{
+    MGLevel<T> &L = mg.lev[0];
+    MGLevel<T> &C = mg.lev[1];
+    hipLaunchKernelGGL((k_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s,
+                       L.res, C.rhs, C.nx, C.ny);
+    hipLaunchKernelGGL((k_fill<T>), dim3(256), dim3(256), 0, s, C.x,
+                       (long long)C.nx * C.ny, (T)0);
+    vcycle(mg, 1, s);
+    hipLaunchKernelGGL((k_prolong_add<T>), grid2(L.nx, L.ny), dim3(256), 0, s,
+                       C.x, L.x, L.nx, L.ny);
+}
+
+static int dot_reduce(bx_helm *h, const double *V, const double *w, int k,
+                      hipStream_t s, double *host_out)
+{
+    hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, V, w, h->n,
+                       k, h->partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(k), dim3(256), 0, s, h->partial,
+                       h->nb_dot, k, h->dots);
+    H_CHECK(hipGetLastError());
+    H_CHECK(hipMemcpyAsync(host_out, h->dots, sizeof(double) * k,
+                           hipMemcpyDeviceToHost, s));
+    H_CHECK(hipStreamSynchronize(s));
+    return BX_OK;
+}
+
+static int norm2(bx_helm *h, const double *v, hipStream_t s, double *out)
+{
+    double d = 0;
+    int    rc = dot_reduce(h, v, v, 1, s, &d);
+    *out      = std::sqrt(d);
+    return rc;
+}
+
+// r = b - A x at FP64
+static void residual64(bx_helm *h, const double *b, const double *x, double *r,
+                       hipStream_t s)
+{
+    launch_apply(h->mg64.lev[0], h->mg64, x, r, b, s);
+}
+
+// ------------------------------------------------------------- C ABI
+extern "C" int bx_helm_create(int nx, int ny, double dx, double dy,
+                              int periodic, bx_helm **out)
+{
+    if (!out || nx < 8 || ny < 8 || !(dx > 0) || !(dy > 0))
+        return herr(BX_ERR_INVALID, "bx_helm_create: bad argument");
+    if (!periodic)
+        return herr(BX_ERR_UNSUPPORTED,
+                    "bx_helm_create: only periodic levels are implemented");
+    bx_helm *h = new bx_helm();
+    h->nx = nx;
+    h->ny = ny;
+    h->dx = dx;
+    h->dy = dy;
+    h->n  = (long long)nx * ny;
+    int rc;
+    if ((rc = halloc(&h->a, h->n)) || (rc = halloc(&h->bX, (long long)(nx + 1) * ny))
+        || (rc = halloc(&h->bY, (long long)nx * (ny + 1)))
+        || (rc = halloc(&h->V, h->n * (h->m + 1))) || (rc = halloc(&h->Z, h->n * h->m))
+        || (rc = halloc(&h->w, h->n)) || (rc = halloc(&h->r, h->n))
+        || (rc = halloc(&h->tmp1, h->n)) || (rc = halloc(&h->tmp2, h->n))
+        || (rc = halloc(&h->fr, h->n)) || (rc = halloc(&h->fe, h->n))
+        || (rc = halloc(&h->partial, (long long)h->nb_dot * (h->m + 1)))
+        || (rc = halloc(&h->dots, h->m + 1)) || (rc = halloc(&h->hdev, h->m + 1)))
+    {
+        bx_helm_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return BX_OK;
+}
+
+extern "C" void bx_helm_destroy(bx_helm *h)
+{
+    if (!h)
+        return;
+    h->mg64.free_all();
+    h->mg32.free_all();
+    for (void *p : { (void *)h->a, (void *)h->bX, (void *)h->bY, (void *)h->cX,
+                     (void *)h->cY, (void *)h->V, (void *)h->Z, (void *)h->w,
+                     (void *)h->r, (void *)h->partial, (void *)h->dots,
+                     (void *)h->hdev, (void *)h->tmp1, (void *)h->tmp2,
+                     (void *)h->fr, (void *)h->fe })
+        if (p)
+            (void)hipFree(p);
+    delete h;
+}
+
+extern "C" int bx_helm_set_coeffs(bx_helm *h, double alpha, double beta,
+                                  double gamma, const double *a,
+                                  const double *bXf, const double *bYf,
+                                  const double *cX, const double *cY,
+                                  void *stream)
+{
+    if (!h || !a || !bXf || !bYf || ((cX == nullptr) != (cY == nullptr)))
+        return herr(BX_ERR_INVALID, "bx_helm_set_coeffs: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    h->alpha = alpha;
+    h->beta  = beta;
+    h->gamma = gamma;
+    h->has_c = cX != nullptr && gamma != 0.0;
+    H_CHECK(hipMemcpyAsync(h->a, a, sizeof(double) * h->n, hipMemcpyDeviceToDevice, s));
+    H_CHECK(hipMemcpyAsync(h->bX, bXf, sizeof(double) * (h->nx + 1) * h->ny,
+                           hipMemcpyDeviceToDevice, s));
+    H_CHECK(hipMemcpyAsync(h->bY, bYf, sizeof(double) * h->nx * (h->ny + 1),
+                           hipMemcpyDeviceToDevice, s));
+    if (h->has_c)
+    {
+        int rc;
+        if (!h->cX && ((rc = halloc(&h->cX, h->n)) || (rc = halloc(&h->cY, h->n))))
+            return rc;
+        H_CHECK(hipMemcpyAsync(h->cX, cX, sizeof(double) * h->n,
+                               hipMemcpyDeviceToDevice, s));
+        H_CHECK(hipMemcpyAsync(h->cY, cY, sizeof(double) * h->n,
+                               hipMemcpyDeviceToDevice, s));
+    }
+    int rc;
+    if ((rc = build_hierarchy(h, h->mg64, s)) || (rc = build_hierarchy(h, h->mg32, s)))
+        return rc;
+    H_CHECK(hipStreamSynchronize(s));
+    h->coeffs_set = true;
+    return BX_OK;
+}
+
+extern "C" int bx_helm_set_options(bx_helm *h, int nu1, int nu2, int nu_bottom,
+                                   int precond_fp32, int restart)
+{
+    if (!h || nu1 < 0 || nu2 < 0 || nu_bottom < 1 || restart < 1 || restart > h->m)
+        return herr(BX_ERR_INVALID, "bx_helm_set_options: bad argument");
+    for (auto *mg : { &h->mg64 })
+    {
+        mg->nu1       = nu1;
+        mg->nu2       = nu2;
+        mg->nu_bottom = nu_bottom;
+    }
+    h->mg32.nu1       = nu1;
+    h->mg32.nu2       = nu2;
+    h->mg32.nu_bottom = nu_bottom;
+    h->precond_fp32   = precond_fp32;
+    h->m              = restart;
+    return BX_OK;
+}
+
+extern "C" int bx_helm_apply(bx_helm *h, const double *x, double *y,
+                             void *stream)
+{
+    if (!h || !h->coeffs_set || !x || !y)
+        return herr(BX_ERR_STATE, "bx_helm_apply: coefficients not set");
+    launch_apply(h->mg64.lev[0], h->mg64, x, y, (const double *)nullptr,
+                 (hipStream_t)stream);
+    H_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_helm_residual(bx_helm *h, const double *b, const double *x,
+                                double *r, void *stream)
+{
+    if (!h || !h->coeffs_set || !b || !x || !r)
+        return herr(BX_ERR_STATE, "bx_helm_residual: coefficients not set");
+    residual64(h, b, x, r, (hipStream_t)stream);
+    H_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_helm_smooth(bx_helm *h, double *phi, const double *rhs,
+                              int redblack, void *stream)
+{
+    if (!h || !h->coeffs_set || !phi || !rhs)
+        return herr(BX_ERR_STATE, "bx_helm_smooth: coefficients not set");
+    launch_smooth(h->mg64.lev[0], h->mg64, phi, rhs, redblack & 1,
+                  (hipStream_t)stream);
+    H_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+// e = M r : one V-cycle from a zero guess (FP32 or FP64), optionally with the
+// learned correction as fine-level smoother:
+//   e  = omega * CNN(r)                 (learned pre-smoothing)
+//   e += P Vcycle_coarse(R (r - A e))
+//   e += omega * CNN(r - A e)           (learned post-smoothing)
+static int precond(bx_helm *h, const double *rv, double *e, hipStream_t s)
+{
+    const long long n = h->n;
+    if (h->cnn)
+    {
+        int rc;
+        // pre: e = omega*CNN(r)
+        if ((rc = bx_cnn_forward(h->cnn, rv, e, h->nx, h->ny, h->cnn_omega, s)))
+            return rc;
+        // residual r1 = r - A e (FP64), coarse correction in FP32 or FP64
+        residual64(h, rv, e, h->tmp1, s);
+        if (h->mg64.lev.size() < 2)
+        {
+            // single-level hierarchy: no coarse correction
+        }
+        else if (h->precond_fp32)
+        {
+            MGHierarchy<float> &mg = h->mg32;
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0,
+                               s, h->tmp1, mg.lev[0].res, n, 1.0);
+            hipLaunchKernelGGL((k_fill<float>), dim3(2048), dim3(256), 0, s,
+                               mg.lev[0].x, n, 0.f);
+            coarse_correction(mg, s);
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0,
+                               s, mg.lev[0].x, h->tmp2, n, 1.0f);
+        }
+        else
+        {
+            MGHierarchy<double> &mg = h->mg64;
+            hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s,
+                               mg.lev[0].res, h->tmp1, n, 1.0);
+            hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s,
+                               mg.lev[0].x, n, 0.0);
+            coarse_correction(mg, s);
+            hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, h->tmp2,
+                               mg.lev[0].x, n, 1.0);
+        }
+        if (h->mg64.lev.size() >= 2)
+            hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, s, e, h->tmp2, n,
+                               1.0);
+        // post: e += omega*CNN(r - A e)
+        residual64(h, rv, e, h->tmp1, s);
+        if ((rc = bx_cnn_forward(h->cnn, h->tmp1, h->tmp2, h->nx, h->ny,
+                                 h->cnn_omega, s)))
+            return rc;
+        hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, s, e, h->tmp2, n, 1.0);
+        H_CHECK(hipGetLastError());
+        return BX_OK;
+    }
+    if (h->precond_fp32)
+    {
+        MGHierarchy<float> &mg = h->mg32;
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0, s,
+                           rv, mg.lev[0].rhs, n, 1.0);
+        hipLaunchKernelGGL((k_fill<float>), dim3(2048), dim3(256), 0, s,
+                           mg.lev[0].x, n, 0.f);
+        vcycle(mg, 0, s);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0, s,
+                           mg.lev[0].x, e, n, 1.0f);
+    }
+    else
+    {
+        MGHierarchy<double> &mg = h->mg64;
+        hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, mg.lev[0].rhs,
+                           rv, n, 1.0);
+        hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s,
+                           mg.lev[0].x, n, 0.0);
+        vcycle(mg, 0, s);
+        hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, e,
+                           mg.lev[0].x, n, 1.0);
+    }
+    H_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_helm_precond(bx_helm *h, const double *r, double *e,
+                               void *stream)
+{
+    if (!h || !h->coeffs_set || !r || !e)
+        return herr(BX_ERR_STATE, "bx_helm_precond: coefficients not set");
+    return precond(h, r, e, (hipStream_t)stream);
+}
+
+extern "C" int bx_helm_attach_cnn(bx_helm *h, bx_cnn *cnn, double omega)
+{
+    if (!h)
+        return herr(BX_ERR_INVALID, "bx_helm_attach_cnn: null solver");
+    h->cnn       = cnn;
+    h->cnn_omega = omega;
+    return BX_OK;
+}
+
+// Plain multigrid solve (MLMG::solve semantics, FP64): x <- x + V(b - A x)
+// until ||b - A x||_2 <= rtol ||b||_2.
+static int solve_mg(bx_helm *h, const double *b, double *x, double rtol,
+                    int maxiter, double bnorm, bx_solve_stats *st,
+                    hipStream_t s)
+{
+    MGHierarchy<double> &mg = h->mg64;
+    MGLevel<double>     &L0 = mg.lev[0];
+    const long long      n  = h->n;
+    double               rn = 0;
+    int                  rc;
+    residual64(h, b, x, h->r, s);
+    if ((rc = norm2(h, h->r, s, &rn)))
+        return rc;
+    int it = 0;
+    while (rn > rtol * bnorm && it < maxiter)
+    {
+        hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, L0.rhs, h->r, n,
+                           1.0);
+        hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s, L0.x, n,
+                           0.0);
+        vcycle(mg, 0, s);
+        hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, s, x, L0.x, n, 1.0);
+        residual64(h, b, x, h->r, s);
+        if ((rc = norm2(h, h->r, s, &rn)))
+            return rc;
+        ++it;
+        ++st->precond_applies;
+    }
+    st->iters += it;
+    st->rel_resid = rn / bnorm;
+    st->converged = rn <= rtol * bnorm;
+    return BX_OK;
+}
+
+// Flexible GMRES(m), right preconditioned, FP64 vectors and dot products,
+// classical Gram-Schmidt applied twice.  Stall detection: if the residual
+// has not dropped by 2x over `stall_window` iterations, or the iteration cap
+// is reached, the caller falls back to plain multigrid.
+static int solve_fgmres(bx_helm *h, const double *b, double *x, double rtol,
+                        int maxiter, double bnorm, bx_solve_stats *st,
+                        hipStream_t s, bool *stalled)
+{
+    const long long n = h->n;
+    const int       m = h->m;
+    std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hv(m + 1),
+        hv2(m + 1), y(m);
+    *stalled      = false;
+    int    total  = 0;
+    double best   = 1e300;
+    int    best_at = 0;
+    int    rc;
+    while (true)
+    {
+        residual64(h, b, x, h->r, s);
+        double beta = 0;
+        if ((rc = norm2(h, h->r, s, &beta)))
+            return rc;
+        st->rel_resid = beta / bnorm;
+        if (beta <= rtol * bnorm)
+        {
+            st->converged = 1;
+            break;
+        }
+        if (total >= maxiter)
+        {
+            *stalled = true;
+            break;
+        }
+        hipLaunchKernelGGL(k_scale, dim3(2048), dim3(256), 0, s, h->V, h->r, n,
+                           1.0 / beta);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0]  = beta;
+        int j = 0;
+        for (; j < m && total < maxiter; ++j)
+        {
+            double *vj = h->V + (long long)j * n;
+            double *zj = h->Z + (long long)j * n;
+            if ((rc = precond(h, vj, zj, s)))
+                return rc;
+            ++st->precond_applies;
+            launch_apply(h->mg64.lev[0], h->mg64, zj, h->w,
+                         (const double *)nullptr, s);
+            // CGS2
+            if ((rc = dot_reduce(h, h->V, h->w, j + 1, s, hv.data())))
+                return rc;
+            H_CHECK(hipMemcpyAsync(h->hdev, hv.data(), sizeof(double) * (j + 1),
+                                   hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_subtract_combo, dim3(2048), dim3(256), 0, s, h->w,
+                               h->V, h->hdev, j + 1, n);
+            if ((rc = dot_reduce(h, h->V, h->w, j + 1, s, hv2.data())))
+                return rc;
+            H_CHECK(hipMemcpyAsync(h->hdev, hv2.data(), sizeof(double) * (j + 1),
+                                   hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_subtract_combo, dim3(2048), dim3(256), 0, s, h->w,
+                               h->V, h->hdev, j + 1, n);
+            for (int i = 0; i <= j; ++i)
+                H[i * m + j] = hv[i] + hv2[i];
+            double hn = 0;
+            if ((rc = norm2(h, h->w, s, &hn)))
+                return rc;
+            H[(j + 1) * m + j] = hn;
+            if (hn > 0)
+                hipLaunchKernelGGL(k_scale, dim3(2048), dim3(256), 0, s,
+                                   h->V + (long long)(j + 1) * n, h->w, n,
+                                   1.0 / hn);
+            // Givens rotations
+            for (int i = 0; i < j; ++i)
+            {
+                const double t0 = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
+                const double t1 = -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
+                H[i * m + j]       = t0;
+                H[(i + 1) * m + j] = t1;
+            }
+            const double den = std::hypot(H[j * m + j], H[(j + 1) * m + j]);
+            cs[j]            = den > 0 ? H[j * m + j] / den : 1.0;
+            sn[j]            = den > 0 ? H[(j + 1) * m + j] / den : 0.0;
+            H[j * m + j]       = den;
+            H[(j + 1) * m + j] = 0;
+            g[j + 1]           = -sn[j] * g[j];
+            g[j]               = cs[j] * g[j];
+            ++total;
+            const double est = std::fabs(g[j + 1]);
+            if (!std::isfinite(est) || !std::isfinite(hn))
+            {
+                // a non-finite preconditioner output: drop this direction
+                *stalled = true;
+                break;
+            }
+            if (est < best * 0.5)
+            {
+                best    = est;
+                best_at = total;
+            }
+            if (est <= rtol * bnorm * 0.5 || hn == 0)
+            {
+                ++j;
+                break;
+            }
+            if (total - best_at >= 8)
+            {
+                *stalled = true;
+                ++j;
+                break;
+            }
+        }
+        // solve the j x j triangular system, x += Z y
+        bool finite = true;
+        for (int i = j - 1; i >= 0; --i)
+        {
+            double t = g[i];
+            for (int q = i + 1; q < j; ++q)
+                t -= H[i * m + q] * y[q];
+            y[i]   = t / H[i * m + i];
+            finite = finite && std::isfinite(y[i]);
+        }
+        if (!finite)
+        {
+            *stalled = true;
+            j        = 0; // keep the current iterate
+        }
+        if (j > 0)
+        {
+            H_CHECK(hipMemcpyAsync(h->hdev, y.data(), sizeof(double) * j,
+                                   hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_add_combo, dim3(2048), dim3(256), 0, s, x, h->Z,
+                               h->hdev, j, n);
+        }
+        st->iters += j;
+        if (*stalled)
+        {
+            residual64(h, b, x, h->r, s);
+            double rn = 0;
+            if ((rc = norm2(h, h->r, s, &rn)))
+                return rc;
+            st->rel_resid = rn / bnorm;
+            st->converged = rn <= rtol * bnorm;
+            if (st->converged)
+                *stalled = false;
+            break;
+        }
+    }
+    return BX_OK;
+}
+
+extern "C" int bx_helm_solve(bx_helm *h, const double *b, double *x, int method,
+                             double rtol, int maxiter, bx_solve_stats *st,
+                             void *stream)
+{
+    if (!h || !h->coeffs_set || !b || !x || !st || !(rtol > 0) || maxiter < 1)
+        return herr(BX_ERR_INVALID, "bx_helm_solve: bad argument");
+    if (method == BX_SOLVE_FGMRES_CNN && !h->cnn)
+        return herr(BX_ERR_STATE, "bx_helm_solve: no network attached");
+    hipStream_t s = (hipStream_t)stream;
+    std::memset(st, 0, sizeof(*st));
+    const auto t0 = std::chrono::steady_clock::now();
+    double     bnorm = 0;
+    int        rc;
+    if ((rc = norm2(h, b, s, &bnorm)))
+        return rc;
+    if (bnorm == 0)
+    {
+        hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s, x, h->n,
+                           0.0);
+        st->converged = 1;
+        H_CHECK(hipStreamSynchronize(s));
+        return BX_OK;
+    }
+    if (method == BX_SOLVE_MG)
+    {
+        if ((rc = solve_mg(h, b, x, rtol, maxiter, bnorm, st, s)))
+            return rc;
+    }
+    else
+    {
+        bx_cnn *saved = h->cnn;
+        if (method == BX_SOLVE_FGMRES_MG)
+            h->cnn = nullptr;
+        bool stalled = false;
+        rc           = solve_fgmres(h, b, x, rtol, maxiter, bnorm, st, s, &stalled);
+        h->cnn       = saved;
+        if (rc)
+            return rc;
+        if (!st->converged)
+        {
+            // fall back to plain multigrid from the current iterate
+            st->fell_back = 1;
+            if ((rc = solve_mg(h, b, x, rtol, 1000, bnorm, st, s)))
+                return rc;
+        }
+    }
+    H_CHECK(hipStreamSynchronize(s));
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                      .count();
+    return BX_OK;
+}

@@ -47,6 +47,14 @@ def main():
     ap.add_argument("--tuning", default="", help="block_x,block_t,seg_k,grid_cap (skips autotune)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain copy kernel of this size (counter calibration)")
+    ap.add_argument("--no-imex", action="store_true", help="skip the IMEX pressure-solve leg")
+    ap.add_argument("--imex-only", action="store_true", help="run only the IMEX pressure-solve leg")
+    ap.add_argument("--imex-n", type=int, default=1024)
+    ap.add_argument("--imex-harvest-steps", type=int, default=8)
+    ap.add_argument("--imex-steps", type=int, default=6)
+    ap.add_argument("--imex-train-steps", type=int, default=640)
+    ap.add_argument("--imex-batch", type=int, default=32)
+    ap.add_argument("--imex-crop", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -61,6 +69,9 @@ def main():
     name = ctypes.create_string_buffer(256)
     _lib.check(lib.bx_device_name(name, 256), "bx_device_name")
     log("device: " + name.value.decode())
+    if args.imex_only:
+        print(json.dumps({"imex": imex_leg(args)}), flush=True)
+        return
 
     log("building the %d^2 quadrant level (mgs %d)" % (args.ncell, args.mgs))
     s = problems.quadrant_setup(ncell=args.ncell, max_grid_size=args.mgs)
@@ -144,11 +155,15 @@ def main():
                 "sweep_ms": [round(x, 4) for x in kern_ms],
                 "algo_bytes_per_launch": algo_bytes}
 
+    lev.close()
+    imex = None
+    if not args.no_imex:
+        imex = imex_leg(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds)
 
-    lev.close()
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": nsteps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / nsteps,
@@ -159,8 +174,89 @@ def main():
             "ghost_cells": lay.ng, "graph": True, "tuning": tuning},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "imex": imex,
     }
     print(json.dumps(out), flush=True)
+
+
+def imex_leg(args):
+    """IMEX PEEK low-Mach Gresho vortex (M = 0.01, gamma 1.4, n^2 periodic):
+    the pressure Helmholtz solve of successive steps to 1e-10 relative
+    residual, plain multigrid (FP64 V-cycles, the reference's MLMG::solve
+    semantics) against FP64 flexible GMRES preconditioned by a V-cycle whose
+    fine-level smoothing is the learned network (BF16 MFMA), and against the
+    same FGMRES with the plain V-cycle.  The network is trained on the GPU
+    first, on crops of residual fields harvested from plain multigrid solves
+    of the preceding steps.  Every solve starts from the previous pressure
+    and is checked by an FP64 residual computed independently."""
+    import numpy as np
+    from bamrex_amd import imex
+    n = args.imex_n
+    log("imex: Gresho %d^2, harvesting %d steps" % (n, args.imex_harvest_steps))
+    wl = imex.GreshoWorkload(n)
+    solver = imex.HelmholtzSolver(n, n, wl.dx)
+    nf_max = 8 * args.imex_harvest_steps
+    R = imex.DeviceArray((nf_max, n, n))
+    nf = 0
+    for k in range(args.imex_harvest_steps):
+        sy = wl.next_system()
+        solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+        db, dx_ = imex.dev(sy["rhs"]), imex.dev(sy["p_prev"])
+        got, st = imex.harvest_residuals(solver, db, dx_, R, nf, nf_max - nf)
+        nf += got
+        wl.advance(sy, dx_.download())
+    log("imex: %d residual fields harvested" % nf)
+    net = imex.LearnedSmoother(seed=1)
+    solver.attach(net, omega=1.0)
+    t0 = time.perf_counter()
+    hist = net.train(R, nf, steps=args.imex_train_steps, crop=args.imex_crop, batch=args.imex_batch,
+                     lr=1e-3, seed=11)
+    train_s = time.perf_counter() - t0
+    log("imex: trained %d steps x %d crops in %.2f s, loss %.4g -> %.4g" % (
+        args.imex_train_steps, args.imex_batch, train_s, float(np.mean(hist[:10])),
+        float(np.mean(hist[-10:]))))
+    methods = ["mg", "fgmres_mg", "fgmres_cnn"]
+    res = {m: {"seconds": [], "iters": [], "rel_resid": [], "fell_back": 0} for m in methods}
+    r = imex.DeviceArray((n, n))
+    for k in range(args.imex_steps):
+        sy = wl.next_system()
+        solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+        db = imex.dev(sy["rhs"])
+        bn = float(np.linalg.norm(sy["rhs"]))
+        p_mg = None
+        for m in methods:
+            dx_ = imex.dev(sy["p_prev"])
+            if k == 0:
+                # untimed first call of each method (first-use allocations)
+                solver.solve(db, dx_, m, rtol=1e-10, maxiter=200)
+                dx_.upload(sy["p_prev"])
+            st = solver.solve(db, dx_, m, rtol=1e-10, maxiter=200)
+            solver.residual(db, dx_, r)
+            rel = float(np.linalg.norm(r.download())) / bn
+            res[m]["seconds"].append(st["seconds"])
+            res[m]["iters"].append(st["iters"])
+            res[m]["rel_resid"].append(rel)
+            res[m]["fell_back"] += int(st["fell_back"])
+            if m == "mg":
+                p_mg = dx_.download()
+        wl.advance(sy, p_mg)
+        log("imex step %d: " % k + "  ".join(
+            "%s %.3f ms/%d it" % (m, res[m]["seconds"][-1] * 1e3, res[m]["iters"][-1]) for m in methods))
+    out = {"workload": "gresho-peek-sp111-%dx%d-M0.01-periodic" % (n, n), "rtol": 1e-10,
+           "harvest_fields": nf, "train_steps": args.imex_train_steps, "train_batch": args.imex_batch,
+           "train_crop": args.imex_crop, "train_crops_total": args.imex_train_steps * args.imex_batch,
+           "train_seconds": round(train_s, 3),
+           "train_loss_first_last": [float(np.mean(hist[:10])), float(np.mean(hist[-10:]))]}
+    for m in methods:
+        out[m] = {"s_per_solve": float(np.mean(res[m]["seconds"])),
+                  "iters_mean": float(np.mean(res[m]["iters"])),
+                  "max_rel_resid": float(np.max(res[m]["rel_resid"])),
+                  "fell_back": res[m]["fell_back"]}
+    out["speedup_cnn_vs_mg"] = out["mg"]["s_per_solve"] / out["fgmres_cnn"]["s_per_solve"]
+    out["speedup_fgmres_mg_vs_mg"] = out["mg"]["s_per_solve"] / out["fgmres_mg"]["s_per_solve"]
+    solver.close()
+    net.close()
+    return out
 
 
 def cpu_baseline(budget_s):

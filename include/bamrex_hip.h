@@ -208,6 +208,118 @@ int         bx_device_name(char *buf, int len);
 int         bx_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 const char *bx_last_error(void);
 
+/* ============================================================================
+ * Pressure Helmholtz solve of the IMEX schemes (PEEK / PESK) on one 2D
+ * periodic level, and the learned preconditioner.
+ *
+ * Operator (MLALaplacianGrad, Source/LinearSolver/MLALapGrad_2D_K.H:9-33, set
+ * up by setup_pressure_op, Source/IMEX/IMEX_O1.cpp:392-414):
+ *   (A p)_ij = alpha a_ij p_ij
+ *            - beta/dx^2 (bX_{i+1,j}(p_{i+1,j}-p_ij) - bX_ij(p_ij-p_{i-1,j}))
+ *            - beta/dy^2 (bY_{i,j+1}(p_{i,j+1}-p_ij) - bY_ij(p_ij-p_{i,j-1}))
+ *            + gamma (cX_ij (p_{i+1,j}-p_{i-1,j})/(2dx) + cY_ij (p_{i,j+1}-p_{i,j-1})/(2dy))
+ * Cell arrays are nx*ny doubles, x fastest.  Face arrays: bX is (nx+1) x ny
+ * (face i of row j at j*(nx+1)+i), bY is nx x (ny+1) (face j of column i at
+ * j*nx+i); on a periodic level face nx equals face 0.  All arrays are device
+ * pointers.  A solver object is not thread-safe.
+ * ========================================================================== */
+typedef struct bx_helm bx_helm;
+typedef struct bx_cnn  bx_cnn;
+
+/* bx_helm_solve methods */
+#define BX_SOLVE_MG 0         /* MLMG::solve: V-cycles until converged (FP64) */
+#define BX_SOLVE_FGMRES_MG 1  /* FGMRES, V-cycle preconditioner */
+#define BX_SOLVE_FGMRES_CNN 2 /* FGMRES, V-cycle with learned fine smoothing */
+
+typedef struct
+{
+    int    iters;           /* outer iterations (V-cycles or Krylov steps) */
+    int    converged;       /* final ||b - A x||_2 <= rtol ||b||_2 */
+    int    fell_back;       /* FGMRES stalled and plain multigrid finished */
+    int    precond_applies; /* V-cycles / preconditioner applications */
+    double rel_resid;       /* final ||b - A x||_2 / ||b||_2 */
+    double seconds;         /* wall time of the solve (host clock) */
+} bx_solve_stats;
+
+/* MLABecLaplacianGradT::define for one periodic level (MLALaplacianGrad.H:312-375).
+ * nx, ny >= 8; non-periodic levels return BX_ERR_UNSUPPORTED. */
+int  bx_helm_create(int nx, int ny, double dx, double dy, int periodic,
+                    bx_helm **out);
+void bx_helm_destroy(bx_helm *h);
+/* setScalars / setACoeffs / setBCoeffs / setCCoeffs (MLALaplacianGrad.H:
+ * 380-560); builds the coarse-level coefficients (averageDownCoeffs,
+ * MLALaplacianGrad.H:930-1027).  cX = cY = NULL (or gamma = 0) drops the
+ * gradient term.  Synchronous. */
+int bx_helm_set_coeffs(bx_helm *h, double alpha, double beta, double gamma,
+                       const double *a, const double *bX, const double *bY,
+                       const double *cX, const double *cY, void *stream);
+/* MLMG::setPreSmooth / setPostSmooth / setFinalSmooth (nu1, nu2, bottom red-
+ * black sweeps), preconditioner precision and FGMRES restart length (<= 20). */
+int bx_helm_set_options(bx_helm *h, int nu1, int nu2, int nu_bottom,
+                        int precond_fp32, int restart);
+/* y = A x (Fapply, MLALaplacianGrad.H:1095-1188). */
+int bx_helm_apply(bx_helm *h, const double *x, double *y, void *stream);
+/* r = b - A x (MLLinOp residual). */
+int bx_helm_residual(bx_helm *h, const double *b, const double *x, double *r,
+                     void *stream);
+/* One red (redblack = 0) or black (1) Gauss-Seidel half sweep on the fine
+ * level (Fsmooth, MLALaplacianGrad.H:1190-1300). */
+int bx_helm_smooth(bx_helm *h, double *phi, const double *rhs, int redblack,
+                   void *stream);
+/* e = M r: one preconditioner application from a zero guess. */
+int bx_helm_precond(bx_helm *h, const double *r, double *e, void *stream);
+/* Use `cnn` (NULL detaches) as the fine-level smoother of the preconditioner,
+ * correction scaled by omega. */
+int bx_helm_attach_cnn(bx_helm *h, bx_cnn *cnn, double omega);
+/* MLMG::solve(phi, rhs, rtol, 0) (IMEX_O1.cpp:602-614) with the chosen method;
+ * x holds the initial guess.  FGMRES that stalls (no 2x residual drop in 8
+ * iterations) or hits maxiter hands its iterate to plain multigrid
+ * (st->fell_back = 1).  Synchronous. */
+int bx_helm_solve(bx_helm *h, const double *b, double *x, int method,
+                  double rtol, int maxiter, bx_solve_stats *st, void *stream);
+const char *bx_helm_last_error(void);
+
+/* Learned smoother: bias-free 3x3 convolutions 1->16->16->16->1 with ReLU,
+ * BF16 weights/activations, FP32 accumulation (MFMA for the 16->16 layers).
+ * Correction e = omega (sigma / D) N(r / sigma), sigma = rms(r), D = diag(A).
+ * Parameters: W1[9][16], W2[9][16][16], W3[9][16][16], W4[9][16] in that
+ * order ([tap][out][in]; tap = (dy+1)*3 + dx+1). */
+#define BX_CNN_NPARAMS 4896
+int  bx_cnn_create(unsigned seed, bx_cnn **out);
+void bx_cnn_destroy(bx_cnn *c);
+/* FP32 host arrays of BX_CNN_NPARAMS values.  set resets the Adam state. */
+int bx_cnn_set_params(bx_cnn *c, const float *params, int n);
+int bx_cnn_get_params(bx_cnn *c, float *params, int n);
+/* Registers the operator the network is trained for and applied with (its
+ * diagonal and, for training, its coefficients; device pointers must stay
+ * valid).  nx must be a multiple of 16. */
+int bx_cnn_set_operator(bx_cnn *c, int nx, int ny, double alpha, double beta,
+                        double dx, double dy, const double *a, const double *bX,
+                        const double *bY, void *stream);
+/* e = omega (sigma/D) N(r/sigma) on the registered periodic level. */
+int bx_cnn_infer(bx_cnn *c, const double *r, double *e, int nx, int ny,
+                 double omega, void *stream);
+/* y = N(x) on nimg images of w x h (w multiple of 16), x rounded to BF16,
+ * zero padding (periodic = 0) or wrap-around.  Device FP32 arrays.
+ * Synchronous. */
+int bx_cnn_net_forward(bx_cnn *c, const float *x, float *y, int w, int h,
+                       int nimg, int periodic, void *stream);
+/* Loss and gradient on crops of the residual fields R (nfields x ny x nx,
+ * device): crop b is the crop x crop window at (x0, y0) of field f, with
+ * crops_host[3b..3b+2] = {f, x0, y0} (periodic wrap).  Loss per crop:
+ * ||m (r - A e)||^2 / ||m r||^2 with e = (sigma/D) N(r/sigma) (omega = 1),
+ * m = cells at distance >= 5 from the crop edge; averaged over the batch.
+ * grad_host (BX_CNN_NPARAMS floats) may be NULL.  Synchronous. */
+int bx_cnn_loss_grad(bx_cnn *c, const double *R, int nfields, int nx, int ny,
+                     const int *crops_host, int batch, int crop, double *loss,
+                     float *grad_host, void *stream);
+/* `steps` Adam steps (lr, beta1 0.9, beta2 0.999) on random crops drawn with
+ * `seed`; loss_history (steps doubles) may be NULL.  Synchronous. */
+int bx_cnn_train(bx_cnn *c, const double *R, int nfields, int nx, int ny,
+                 int crop, int batch, int steps, float lr, unsigned seed,
+                 double *loss_history, void *stream);
+const char *bx_cnn_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

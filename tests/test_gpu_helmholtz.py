@@ -1,0 +1,245 @@
+"""GPU tests of the pressure Helmholtz solver and the learned smoother,
+through the C ABI, against the CPU oracle.
+
+Bars: operator apply, residual, red/black smoothing, the FP64 V-cycle and
+the plain-multigrid iterate are bit-exact against the oracle (same
+operation order, no reductions inside); solves reach 1e-10 relative residual
+checked by the oracle's independent FP64 residual; the network forward is
+bit-exact on exact-arithmetic cases and within BF16 rounding otherwise;
+loss and gradient agree with the oracle to FP32 accumulation-order rounding.
+"""
+import numpy as np
+import pytest
+
+import cnn_oracle as co
+import helm_oracle as ho
+from bamrex_amd import imex
+from bamrex_amd.imex import DeviceArray, HelmholtzSolver, LearnedSmoother, dev
+
+from conftest import announce
+from test_helmholtz_oracle import exact_params, rand_coeffs
+
+pytestmark = pytest.mark.gpu
+
+
+def make_solver(nx, ny, seed=0, beta=5e-3, with_c=False):
+    rng = np.random.default_rng(seed)
+    a, bX, bY = rand_coeffs(rng, nx, ny)
+    s = HelmholtzSolver(nx, ny, 1.0 / nx, 1.0 / ny)
+    kw = {}
+    if with_c:
+        kw = dict(gamma=-0.3, cX=rng.normal(size=(ny, nx)), cY=rng.normal(size=(ny, nx)))
+    s.set_coeffs(1.0, beta, a, bX, bY, **kw)
+    return s, (a, bX, bY), kw, rng
+
+
+@pytest.mark.parametrize("with_c", [False, True])
+def test_apply_and_residual_bit_exact(with_c):
+    announce("helm apply/residual with_c=%s 64x48" % with_c)
+    nx, ny = 64, 48
+    s, (a, bX, bY), kw, rng = make_solver(nx, ny, 1, with_c=with_c)
+    x = rng.normal(size=(ny, nx))
+    b = rng.normal(size=(ny, nx))
+    dx_, dy_, db = dev(x), DeviceArray((ny, nx)), dev(b)
+    s.apply(dx_, dy_)
+    ref = ho.apply(x, a, bX, bY, 1.0, 5e-3, 1 / nx, 1 / ny, **kw)
+    assert np.array_equal(dy_.download(), ref)
+    s.residual(db, dx_, dy_)
+    ref = ho.apply(x, a, bX, bY, 1.0, 5e-3, 1 / nx, 1 / ny, rhs=b, **kw)
+    assert np.array_equal(dy_.download(), ref)
+    s.close()
+
+
+def test_smooth_bit_exact():
+    announce("helm gsrb 64x64")
+    nx = ny = 64
+    s, (a, bX, bY), kw, rng = make_solver(nx, ny, 2)
+    phi = rng.normal(size=(ny, nx))
+    rhs = rng.normal(size=(ny, nx))
+    dp, dr = dev(phi), dev(rhs)
+    ref = phi
+    for rb in (0, 1, 0, 1):
+        s.smooth(dp, dr, rb)
+        ref = ho.gsrb(ref, rhs, a, bX, bY, 1.0, 5e-3, 1 / nx, 1 / ny, rb)
+    assert np.array_equal(dp.download(), ref)
+    s.close()
+
+
+def test_vcycle_fp64_bit_exact():
+    announce("helm fp64 vcycle 128x64")
+    nx, ny = 128, 64
+    s, (a, bX, bY), kw, rng = make_solver(nx, ny, 3)
+    s.set_options(precond_fp32=0)
+    r = rng.normal(size=(ny, nx))
+    dr, de = dev(r), DeviceArray((ny, nx))
+    s.precond(dr, de)
+    ref = ho.vcycle(r, a, bX, bY, 1.0, 5e-3, 1 / nx, 1 / ny)
+    assert np.array_equal(de.download(), ref)
+    # FP32 V-cycle: close to the FP64 one
+    s.set_options(precond_fp32=1)
+    s.precond(dr, de)
+    e32 = de.download()
+    assert np.max(np.abs(e32 - ref)) <= 1e-5 * np.max(np.abs(ref))
+    s.close()
+
+
+def test_mg_solve_bit_exact_iterate():
+    announce("helm mg solve 64x64")
+    nx = ny = 64
+    s, (a, bX, bY), kw, rng = make_solver(nx, ny, 4)
+    b = rng.normal(size=(ny, nx))
+    x0 = rng.normal(size=(ny, nx)) * 0.1
+    db, dx_ = dev(b), dev(x0)
+    st = s.solve(db, dx_, "mg", rtol=1e-10, maxiter=100)
+    xr, its, hist = ho.mg_solve(b, a, bX, bY, 1.0, 5e-3, 1 / nx, 1 / ny, x0=x0, rtol=1e-10, maxiter=100)
+    assert st["converged"] and its == st["iters"], (st, its)
+    assert np.array_equal(dx_.download(), xr)
+    s.close()
+
+
+@pytest.mark.parametrize("method", ["mg", "fgmres_mg"])
+def test_solve_reaches_tolerance_on_gresho_system(method):
+    announce("helm %s gresho 256" % method)
+    n = 256
+    U, dx = imex.gresho_state(n)
+    stage = imex.PeekStage()
+    p = imex.pressure_from_state(U, 1.4, 1.0)
+    dt = imex.advective_dt(U, dx)
+    sys_ = stage.assemble(U, p, dt, dx)
+    s = HelmholtzSolver(n, n, dx)
+    s.set_coeffs(sys_["alpha"], sys_["beta"], sys_["a"], sys_["bX"], sys_["bY"])
+    db, dx_ = dev(sys_["rhs"]), dev(np.zeros((n, n)))
+    st = s.solve(db, dx_, method, rtol=1e-10, maxiter=200)
+    x = dx_.download()
+    r = ho.apply(x, sys_["a"], sys_["bX"], sys_["bY"], 1.0, sys_["beta"], dx, dx, rhs=sys_["rhs"])
+    rel = np.linalg.norm(r) / np.linalg.norm(sys_["rhs"])
+    assert st["converged"] and not st["fell_back"] and rel <= 1e-10, (st, rel)
+    s.close()
+
+
+def test_peek_uniform_state_pressure_is_eos():
+    """A uniform state at rest: rhs = eps E and A p = eps/(gamma-1) p, so the
+    solve returns p = (gamma-1) E exactly up to the tolerance."""
+    announce("helm peek uniform 64")
+    n = 64
+    U = np.zeros((4, n, n))
+    U[0] = 1.3
+    U[3] = 2.7
+    dx = 1.0 / n
+    st_ = imex.PeekStage()
+    p0 = np.full((n, n), 1.0)
+    sys_ = st_.assemble(U, p0, 1e-3, dx)
+    s = HelmholtzSolver(n, n, dx)
+    s.set_coeffs(sys_["alpha"], sys_["beta"], sys_["a"], sys_["bX"], sys_["bY"])
+    db, dx_ = dev(sys_["rhs"]), dev(p0)
+    st = s.solve(db, dx_, "fgmres_mg", rtol=1e-12, maxiter=50)
+    assert st["converged"]
+    assert np.max(np.abs(dx_.download() - 0.4 * 2.7)) <= 1e-11
+    s.close()
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_cnn_forward_exact_case_bit_exact(periodic):
+    announce("cnn forward exact periodic=%s" % periodic)
+    net = LearnedSmoother(seed=1)
+    p = exact_params(np.random.default_rng(11))
+    net.set_params(p.astype(np.float32))
+    x = np.random.default_rng(12).integers(-2, 3, size=(3, 16, 32)).astype(np.float32)
+    y = net.net_forward(x, periodic=periodic)
+    ref, _ = co.forward(p, x, periodic=periodic, quantize=True)
+    assert np.array_equal(y, ref)
+    net.close()
+
+
+def test_cnn_forward_random_weights():
+    announce("cnn forward random 2x48x64")
+    rng = np.random.default_rng(5)
+    net = LearnedSmoother(seed=2)
+    p = (rng.normal(size=co.NPARAMS) * 0.3).astype(np.float32)
+    net.set_params(p)
+    x = rng.normal(size=(2, 48, 64)).astype(np.float32)
+    y = net.net_forward(x)
+    ref, _ = co.forward(p, x, quantize=True)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(y - ref)) <= 3e-2 * scale
+    assert np.mean(np.abs(y - ref)) <= 2e-3 * scale
+    net.close()
+
+
+def test_cnn_loss_and_gradient_match_oracle():
+    announce("cnn loss/grad 96x96 batch 6 crop 32")
+    n = 96
+    rng = np.random.default_rng(9)
+    a, bX, bY = rand_coeffs(rng, n, n)
+    s = HelmholtzSolver(n, n, 1.0 / n)
+    s.set_coeffs(1.0, 2e-3, a, bX, bY)
+    net = LearnedSmoother(seed=3)
+    s.attach(net)
+    p = net.get_params()
+    R = rng.normal(size=(2, n, n))
+    dR = dev(R)
+    crops = np.array([[0, 0, 0], [1, 70, 90], [0, 33, 5], [1, 95, 95], [0, 64, 31], [1, 10, 50]], np.int32)
+    L, g = net.loss_grad(dR, 2, crops, 32)
+    Lr, gr = co.crop_loss_grad(p, R, [tuple(c) for c in crops], 32, a, bX, bY, 1.0, 2e-3, 1 / n, 1 / n)
+    assert abs(L - Lr) <= 2e-3 * abs(Lr), (L, Lr)
+    assert np.linalg.norm(g - gr) <= 2e-2 * np.linalg.norm(gr)
+    # each layer's gradient block separately
+    for off, m in ((0, co.NW1), (co.NW1, co.NW2), (co.NW1 + co.NW2, co.NW3),
+                   (co.NW1 + co.NW2 + co.NW3, co.NW4)):
+        gb, grb = g[off:off + m], gr[off:off + m]
+        assert np.linalg.norm(gb - grb) <= 3e-2 * np.linalg.norm(grb) + 1e-12
+    s.close()
+    net.close()
+
+
+def gresho_systems(n, nsteps, seed_state=None):
+    """(a, bX, bY, beta, rhs, p) for successive PEEK-SP111 steps of the
+    Gresho vortex, solved on the GPU with plain multigrid."""
+    U, dx = imex.gresho_state(n)
+    stage = imex.PeekStage()
+    p = imex.pressure_from_state(U, 1.4, 1.0)
+    out = []
+    s = HelmholtzSolver(n, n, dx)
+    for _ in range(nsteps):
+        dt = imex.advective_dt(U, dx)
+        sy = stage.assemble(U, p, dt, dx)
+        s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+        db, dp = dev(sy["rhs"]), dev(p)
+        st = s.solve(db, dp, "mg", rtol=1e-10, maxiter=100)
+        assert st["converged"]
+        out.append((sy, p.copy()))
+        p = dp.download()
+        U = stage.finish(U, sy, p, dt, dx)
+    s.close()
+    return out, dx
+
+
+def test_training_reduces_loss_and_fallback_on_garbage():
+    announce("cnn train 128 + fallback")
+    n = 128
+    systems, dx = gresho_systems(n, 1)
+    sy, p0 = systems[0]
+    s = HelmholtzSolver(n, n, dx)
+    s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+    rng = np.random.default_rng(1)
+    R = rng.normal(size=(4, n, n))
+    # smooth-ish residual fields: a few V-cycle residuals of random systems
+    dR = dev(R)
+    net = LearnedSmoother(seed=4)
+    s.attach(net, omega=1.0)
+    hist = net.train(dR, 4, steps=60, crop=32, batch=16, lr=2e-3, seed=3)
+    assert np.all(np.isfinite(hist))
+    assert np.mean(hist[-10:]) < np.mean(hist[:10])
+    # garbage weights: the preconditioned solve stalls and falls back
+    net.set_params(np.full(co.NPARAMS, 1e3, np.float32))
+    db, dxv = dev(sy["rhs"]), dev(p0)
+    st = s.solve(db, dxv, "fgmres_cnn", rtol=1e-10, maxiter=60)
+    x = dxv.download()
+    r = ho.apply(x, sy["a"], sy["bX"], sy["bY"], 1.0, sy["beta"], dx, dx, rhs=sy["rhs"])
+    assert st["converged"] and np.linalg.norm(r) <= 1e-10 * np.linalg.norm(sy["rhs"])
+    net.set_params(np.full(co.NPARAMS, np.nan, np.float32))
+    dxv.upload(p0)
+    st = s.solve(db, dxv, "fgmres_cnn", rtol=1e-10, maxiter=60)
+    assert st["fell_back"] and st["converged"]
+    s.close()
+    net.close()

@@ -248,7 +248,7 @@ extern "C" int bx_create(const bx_params *params, bx_ctx **out)
     c->p      = p;
     c->D      = p.dim;
     c->NC     = NC;
-    c->pm     = EulerParm{ p.adiabatic, p.epsilon };
+    c->pm     = EulerParm{ p.adiabatic, p.epsilon, bx_pow_hllc_setup(p.adiabatic) };
     bool shared = true, own = true;
     for (int n = 0; n < NC; ++n)
     {

@@ -29,8 +29,9 @@ enum LimiterKind
 
 struct EulerParm
 {
-    double gamma;
-    double eps;
+    double    gamma;
+    double    eps;
+    BxPowHllc ph; // exponent data of the HLLC pressure estimate
 };
 
 // Q = (rho, u_0..u_{D-1}, p)
@@ -257,10 +258,10 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
     const double c_R = sqrt(adiabatic * p_R / mul_eps<E1>(density_R, epsilon));
 
     const double z = (adiabatic - 1) / (2 * adiabatic);
-    const double p_rr
-        = bx_pow((c_L + c_R - (adiabatic - 1) / 2 * (u_R - u_L))
-                     / (c_L * bx_pow(p_L, -z) + c_R * bx_pow(p_R, -z)),
-                 1 / z);
+    const double p_rr = bx_pow_invz(
+        (c_L + c_R - (adiabatic - 1) / 2 * (u_R - u_L))
+            / (c_L * bx_pow_negz(p_L, -z, pm.ph) + c_R * bx_pow_negz(p_R, -z, pm.ph)),
+        1 / z, pm.ph);
     const double q_L
         = (p_rr > p_L)
               ? sqrt(1 + (adiabatic + 1) / (2 * adiabatic) * (p_rr / p_L - 1))

@@ -242,3 +242,146 @@ BX_POW_FN double bx_pow(double x, double y)
     }
     return bx_pow_slow(x, y);
 }
+
+// ---------------------------------------------------------------------------
+// The two exponents of the HLLC pressure estimate, -z and 1/z with
+// z = (gamma-1)/(2 gamma) rounded to double, are within a few ulps of -1/n
+// and n for the integer n = round(1/z) whenever 2 gamma/(gamma-1) is an
+// integer (gamma = 1.4: n = 7; gamma = 5/3: n = 5).  Then
+//   x^(n + d)     = x^n (1 + d log x + ...),        d = 1/z - n   (exact),
+//   x^(-1/n + w)  = x^(-1/n) (1 + w log x + ...),   w = 1/n - z,
+// with |d|, |w| ~ 1e-15.  x^n is formed in double-double by binary powering
+// and x^(-1/n) by one double-double Newton step on x r^n = 1 from a 1e-12
+// estimate, so both values are known to ~1e-23 relative before the final
+// rounding: the result is the correctly rounded pow(x, y) except when the
+// exact value lies within ~1e-23 of a rounding boundary, the same answer the
+// general routine above gives, for about a third of its instruction count.
+// Inputs outside the stated ranges go to bx_pow.
+namespace bxpow {
+
+BX_POW_FN void dd_sqr(double &h, double &l)
+{
+    const double p = h * h;
+    double       e = __builtin_fma(h, h, -p);
+    e              = __builtin_fma(2.0 * h, l, e);
+    fast_two_sum(p, e, h, l);
+}
+
+BX_POW_FN void dd_mul_d(double &h, double &l, double x)
+{
+    const double p = h * x;
+    double       e = __builtin_fma(h, x, -p);
+    e              = __builtin_fma(l, x, e);
+    fast_two_sum(p, e, h, l);
+}
+
+// x^n (n >= 1) as a double-double, left-to-right binary powering
+BX_POW_FN void dd_powi(double x, int n, double &h, double &l)
+{
+    h             = x;
+    l             = 0.0;
+    const int top = 31 - __builtin_clz((unsigned)n);
+    for (int b = top - 1; b >= 0; --b)
+    {
+        dd_sqr(h, l);
+        if ((n >> b) & 1)
+            dd_mul_d(h, l, x);
+    }
+}
+
+// log(x) to ~1e-11 absolute, x positive and normal (same table split as
+// log_dd; log1p(r) to degree 4 in r, |r| < 2^-7)
+BX_POW_FN double log_approx(double x)
+{
+    const uint64_t ix = d2u(x);
+    const int      j  = (int)((ix >> 45) & 127);
+    const int      hi = j >> 6;
+    const int      k  = (int)((ix >> 52) & 0x7ff) - 1023 + hi;
+    const double   m
+        = u2d((ix & 0x000fffffffffffffULL) | ((uint64_t)(1023 - hi) << 52));
+    const double r = __builtin_fma(m, bx_log_invc[j], -1.0);
+    double       p = -0.25;
+    p              = __builtin_fma(p, r, 0x1.5555555555555p-2); // 1/3
+    p              = __builtin_fma(p, r, -0.5);
+    p              = __builtin_fma(p, r, 1.0);
+    return ((double)k * BX_LN2_HI + bx_log_logc_hi[j]) + r * p;
+}
+
+// e^t to ~1e-13 relative for |t| <= 700
+BX_POW_FN double exp_approx(double t)
+{
+    const double kk = __builtin_rint(t * BX_INV_LN2_64);
+    const int    ki = (int)kk;
+    const int    j  = ki & 63;
+    const int    k  = (ki - j) >> 6;
+    double       r  = __builtin_fma(-kk, BX_LN2_64_HI, t);
+    r               = __builtin_fma(-kk, BX_LN2_64_LO, r);
+    double p        = 0x1.5555555555555p-5; // 1/24
+    p               = __builtin_fma(p, r, 0x1.5555555555555p-3); // 1/6
+    p               = __builtin_fma(p, r, 0.5);
+    p               = __builtin_fma(p, r, 1.0);
+    const double th = bx_exp_t_hi[j];
+    return __builtin_fma(th, r * p, th) * u2d((uint64_t)(k + 1023) << 52);
+}
+
+} // namespace bxpow
+
+// Exponent data of the HLLC estimate for a given gamma.  n = 0 selects the
+// general routine (2 gamma/(gamma-1) not an integer in [2, 32]).
+struct BxPowHllc
+{
+    int    n;
+    double inv_n; // RN(1/n)
+    double d;     // 1/z - n, exact
+    double w;     // 1/n - z to ~1e-16 relative
+};
+
+static inline BxPowHllc bx_pow_hllc_setup(double gamma)
+{
+    BxPowHllc    s  = { 0, 0.0, 0.0, 0.0 };
+    const double z  = (gamma - 1) / (2 * gamma);
+    const double yi = 1 / z;
+    if (!(yi >= 1.5 && yi <= 32.5))
+        return s;
+    const double nd = __builtin_rint(yi);
+    const double d  = yi - nd;
+    // w = (1 - n z) / n with n z as an exact double-double
+    const double nz_hi = nd * z;
+    const double nz_lo = __builtin_fma(nd, z, -nz_hi);
+    const double w     = ((1.0 - nz_hi) - nz_lo) / nd;
+    if (!(__builtin_fabs(d) <= 0x1p-40 && __builtin_fabs(w) <= 0x1p-40))
+        return s;
+    s.n     = (int)nd;
+    s.inv_n = 1.0 / nd;
+    s.d     = d;
+    s.w     = w;
+    return s;
+}
+
+// pow(x, -z) for the estimate's pressures; y = -z is the exact exponent
+BX_POW_FN double bx_pow_negz(double x, double y, const BxPowHllc &s)
+{
+    using namespace bxpow;
+    if (!(s.n != 0 && x >= 0x1p-600 && x <= 0x1p600))
+        return bx_pow(x, y);
+    const double lx = log_approx(x);
+    const double r  = exp_approx(-lx * s.inv_n); // x^(-1/n) to ~1e-12
+    double       h, l;
+    dd_powi(r, s.n, h, l);
+    dd_mul_d(h, l, x); // x r^n = h + l, close to 1
+    const double delta = (1.0 - h) - l;
+    const double c     = r * (delta * s.inv_n); // Newton correction
+    return r + (c + r * (s.w * lx));
+}
+
+// pow(x, 1/z); y = 1/z is the exact exponent
+BX_POW_FN double bx_pow_invz(double x, double y, const BxPowHllc &s)
+{
+    using namespace bxpow;
+    if (!(s.n != 0 && x >= 0x1p-28 && x <= 0x1p28))
+        return bx_pow(x, y);
+    double h, l;
+    dd_powi(x, s.n, h, l);
+    const double t = s.d * log_approx(x);
+    return h + __builtin_fma(h, t, l);
+}

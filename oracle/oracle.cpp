@@ -38,8 +38,19 @@
 #ifdef ORACLE_BXPOW
 #include "../bamrex_amd/csrc/bx_pow.h"
 static inline double or_pow(double x, double y) { return bx_pow(x, y); }
+// the specialised exponents of the HLLC estimate (same operations as the kernels)
+static inline double or_pow_negz(double x, double y, double gamma)
+{
+    return bx_pow_negz(x, y, bx_pow_hllc_setup(gamma));
+}
+static inline double or_pow_invz(double x, double y, double gamma)
+{
+    return bx_pow_invz(x, y, bx_pow_hllc_setup(gamma));
+}
 #else
 static inline double or_pow(double x, double y) { return std::pow(x, y); }
+static inline double or_pow_negz(double x, double y, double) { return std::pow(x, y); }
+static inline double or_pow_invz(double x, double y, double) { return std::pow(x, y); }
 #endif
 
 namespace oracle {
@@ -351,10 +362,10 @@ static void estimate_wave_speed_HLLC_Toro(const Fab &cL, const Fab &cR, int i,
     const double adiabatic = adiabatic_L;
 
     const double z = (adiabatic - 1) / (2 * adiabatic);
-    const double p_rr
-        = or_pow((c_L + c_R - (adiabatic - 1) / 2 * (u_R - u_L))
-                     / (c_L * or_pow(p_L, -z) + c_R * or_pow(p_R, -z)),
-                 1 / z);
+    const double p_rr = or_pow_invz(
+        (c_L + c_R - (adiabatic - 1) / 2 * (u_R - u_L))
+            / (c_L * or_pow_negz(p_L, -z, adiabatic) + c_R * or_pow_negz(p_R, -z, adiabatic)),
+        1 / z, adiabatic);
     const double q_L
         = (p_rr > p_L) ? std::sqrt(1 + (adiabatic + 1) / (2 * adiabatic)
                                            * (p_rr / p_L - 1))
@@ -929,5 +940,13 @@ int or_hllc_face(int dim, int dir, const double *UL, const double *UR,
 double or_limiter(int kind, double r) { return slope_limit(kind, r); }
 double or_minmod(double a, double b) { return minmod(a, b); }
 double or_pow_fn(double x, double y) { return or_pow(x, y); }
+double or_pow_negz_fn(double x, double gamma)
+{
+    return or_pow_negz(x, -((gamma - 1) / (2 * gamma)), gamma);
+}
+double or_pow_invz_fn(double x, double gamma)
+{
+    return or_pow_invz(x, 1 / ((gamma - 1) / (2 * gamma)), gamma);
+}
 
 } // extern "C"

@@ -73,7 +73,7 @@ def _load(bxpow):
     for n in ("or_limiter",):
         getattr(l, n).argtypes = [ctypes.c_int, ctypes.c_double]
         getattr(l, n).restype = ctypes.c_double
-    for n in ("or_minmod", "or_pow_fn"):
+    for n in ("or_minmod", "or_pow_fn", "or_pow_negz_fn", "or_pow_invz_fn"):
         getattr(l, n).argtypes = [ctypes.c_double, ctypes.c_double]
         getattr(l, n).restype = ctypes.c_double
     _libs[key] = l
@@ -208,3 +208,11 @@ class Oracle:
 
     def pow(self, x, y):
         return self.lib.or_pow_fn(x, y)
+
+    def pow_negz(self, x, gamma):
+        """pow(x, -z), z = (gamma-1)/(2 gamma), as the HLLC estimate calls it."""
+        return self.lib.or_pow_negz_fn(x, gamma)
+
+    def pow_invz(self, x, gamma):
+        """pow(x, 1/z) as the HLLC estimate calls it."""
+        return self.lib.or_pow_invz_fn(x, gamma)

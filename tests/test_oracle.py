@@ -226,6 +226,39 @@ def test_pow_correctly_rounded_on_hot_path_inputs(oracle_bxpow, oracle_glibc):
     assert agree >= 396  # glibc itself is off by one ulp on a few of these
 
 
+@pytest.mark.parametrize("gamma", [1.4, 5.0 / 3.0, 1.2])
+def test_pow_hllc_exponents_correctly_rounded(oracle_bxpow, gamma):
+    """The HLLC estimate's pow(p, -z) and pow(X, 1/z) use the near-integer
+    exponent routines when 2 gamma/(gamma-1) is an integer; they are
+    correctly rounded (60-digit reference) on the estimate's input ranges."""
+    decimal.getcontext().prec = 60
+    rng = np.random.default_rng(17)
+    z = (gamma - 1) / (2 * gamma)
+    ps = np.exp(rng.uniform(-32, 32, 300))
+    Xs = np.exp(rng.uniform(-5, 5, 300))
+    D = decimal.Decimal
+    for x, f, y in [(p, oracle_bxpow.pow_negz, -z) for p in ps] + [(X, oracle_bxpow.pow_invz, 1 / z) for X in Xs]:
+        got = f(float(x), gamma)
+        ex = (D(float(x)).ln() * D(y)).exp()
+        err = abs(D(got) - ex) / D(math.ulp(got))
+        assert err <= D("0.5000001"), (x, y, got, err)
+
+
+def test_pow_hllc_exponents_match_general_routine(oracle_bxpow):
+    """Against the general routine: equal except where the exact value is
+    within ~1e-19 of a rounding boundary (tens of cases per million)."""
+    rng = np.random.default_rng(23)
+    same = 0
+    n = 20000
+    for p in np.exp(rng.uniform(-32, 32, n)):
+        same += oracle_bxpow.pow_negz(float(p), 1.4) == oracle_bxpow.pow(float(p), -((1.4 - 1) / (2 * 1.4)))
+    assert same >= n - 3
+    assert oracle_bxpow.pow_negz(-1.0, 1.4) != oracle_bxpow.pow_negz(-1.0, 1.4)   # NaN like pow
+    assert oracle_bxpow.pow_invz(0.0, 1.4) == 0.0
+    assert math.isnan(oracle_bxpow.pow_invz(-2.5, 1.4))
+    assert oracle_bxpow.pow_invz(1e30, 1.4) == oracle_bxpow.pow(1e30, 1 / ((1.4 - 1) / (2 * 1.4)))
+
+
 def test_pow_special_cases(oracle_bxpow):
     p = oracle_bxpow.pow
     assert math.isnan(p(-2.5, 7.000000000000002))

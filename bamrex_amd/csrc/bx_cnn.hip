@@ -640,7 +640,7 @@ struct bx_cnn
     double  *D  = nullptr;
     int      onx = 0, ony = 0;
     // inference buffers
-    long long       cap = 0;
+    long long       cap = 0, epoch = 0;
     unsigned short *x = nullptr, *h1 = nullptr, *h2 = nullptr, *h3 = nullptr;
     float          *y = nullptr;
     double         *sig = nullptr, *part = nullptr;
@@ -780,31 +780,6 @@ extern "C" int bx_cnn_get_params(bx_cnn *c, float *p, int n)
     return BX_OK;
 }
 
-// registers the operator: diagonal for the scaling, coefficients for the loss
-extern "C" int bx_cnn_set_operator(bx_cnn *c, int nx, int ny, double alpha,
-                                   double beta, double dx, double dy,
-                                   const double *a, const double *bX,
-                                   const double *bY, void *stream)
-{
-    if (!c || !a || !bX || !bY || nx < 16 || ny < 1)
-        return cerr(BX_ERR_INVALID, "bx_cnn_set_operator: bad argument");
-    hipStream_t s = (hipStream_t)stream;
-    if (c->onx != nx || c->ony != ny)
-    {
-        cfree(c->D);
-        c->D = nullptr;
-        C_CHECK(hipMalloc((void **)&c->D, sizeof(double) * nx * ny));
-    }
-    c->onx          = nx;
-    c->ony          = ny;
-    const double dhx = beta / (dx * dx), dhy = beta / (dy * dy);
-    hipLaunchKernelGGL(k_diag, dim3((nx + 255) / 256, ny), dim3(256), 0, s, a, bX, bY,
-                       c->D, nx, ny, alpha, dhx, dhy);
-    C_CHECK(hipGetLastError());
-    c->opr = OpRef{ a, bX, bY, c->D, nx, ny, alpha, dhx, dhy };
-    return BX_OK;
-}
-
 static int ensure_infer(bx_cnn *c, long long npx)
 {
     if (npx <= c->cap)
@@ -820,8 +795,46 @@ static int ensure_infer(bx_cnn *c, long long npx)
     C_CHECK(hipMalloc((void **)&c->h3, npx * C * 2));
     C_CHECK(hipMalloc((void **)&c->y, npx * 4));
     c->cap = npx;
+    ++c->epoch;
     return BX_OK;
 }
+
+int bx_cnn_reserve(bx_cnn *c, long long npx)
+{
+    if (!c)
+        return cerr(BX_ERR_INVALID, "bx_cnn_reserve: null network");
+    return ensure_infer(c, npx);
+}
+
+long long bx_cnn_epoch(const bx_cnn *c) { return c ? c->epoch : 0; }
+
+// registers the operator: diagonal for the scaling, coefficients for the loss
+extern "C" int bx_cnn_set_operator(bx_cnn *c, int nx, int ny, double alpha,
+                                   double beta, double dx, double dy,
+                                   const double *a, const double *bX,
+                                   const double *bY, void *stream)
+{
+    if (!c || !a || !bX || !bY || nx < 16 || ny < 1)
+        return cerr(BX_ERR_INVALID, "bx_cnn_set_operator: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (c->onx != nx || c->ony != ny)
+    {
+        cfree(c->D);
+        c->D = nullptr;
+        C_CHECK(hipMalloc((void **)&c->D, sizeof(double) * nx * ny));
+        ++c->epoch;
+    }
+    c->onx          = nx;
+    c->ony          = ny;
+    const double dhx = beta / (dx * dx), dhy = beta / (dy * dy);
+    hipLaunchKernelGGL(k_diag, dim3((nx + 255) / 256, ny), dim3(256), 0, s, a, bX, bY,
+                       c->D, nx, ny, alpha, dhx, dhy);
+    C_CHECK(hipGetLastError());
+    c->opr = OpRef{ a, bX, bY, c->D, nx, ny, alpha, dhx, dhy };
+    return ensure_infer(c, (long long)nx * ny);
+}
+
+
 
 template <bool PER>
 static void net_forward(bx_cnn *c, const Geo &g, hipStream_t s)

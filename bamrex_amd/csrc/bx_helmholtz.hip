@@ -37,8 +37,11 @@ __global__ __launch_bounds__(256) void k_apply(
     const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ a,
     const T *__restrict__ bX, const T *__restrict__ bY,
     const T *__restrict__ cX, const T *__restrict__ cY, int nx, int ny,
-    T alpha, T beta, T gamma, T dxinv, T dyinv, const T *__restrict__ rhs)
+    const T *__restrict__ sc, T dxinv, T dyinv, const T *__restrict__ rhs)
 {
+    // sc = (alpha, beta, gamma) in device memory, so that a captured graph
+    // stays valid when the scalars change from one solve to the next
+    const T alpha = sc[0], beta = sc[1], gamma = sc[2];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= nx)
@@ -73,8 +76,11 @@ __global__ __launch_bounds__(256) void k_gsrb(
     T *__restrict__ phi, const T *__restrict__ rhs, const T *__restrict__ a,
     const T *__restrict__ bX, const T *__restrict__ bY,
     const T *__restrict__ cX, const T *__restrict__ cY, int nx, int ny,
-    T alpha, T dhx, T dhy, T cdhx, T cdhy, int redblack)
+    const T *__restrict__ sc, T hx, T hy, int redblack)
 {
+    const T alpha = sc[0];
+    const T dhx = sc[1] / (hx * hx), dhy = sc[1] / (hy * hy);
+    const T cdhx = sc[2] / (2 * hx), cdhy = sc[2] / (2 * hy);
     // thread covers cells of one colour: i = 2*t + ((j + redblack) & 1)
     const int j = blockIdx.y;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,6 +158,71 @@ __global__ void k_avg_faces_y(const T *__restrict__ f, T *__restrict__ c,
     c[(long long)J * ncx + I] = T(0.5)
                                 * (f[(long long)(2 * J) * nfx + 2 * I]
                                    + f[(long long)(2 * J) * nfx + 2 * I + 1]);
+}
+
+// Bottom solve of the V-cycle in one launch: `iters` red/black sweeps of a
+// level of at most BOTTOM_MAX cells, held in LDS by a single workgroup.  The
+// cell update is the one of k_gsrb, so the result is the same bits as
+// 2 * iters separate colour launches.
+constexpr int BOTTOM_MAX = 1024;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bottom_gsrb(
+    T *__restrict__ phi_g, const T *__restrict__ rhs_g, const T *__restrict__ a_g,
+    const T *__restrict__ bX_g, const T *__restrict__ bY_g,
+    const T *__restrict__ cX_g, const T *__restrict__ cY_g, int nx, int ny,
+    const T *__restrict__ sc, T hx, T hy, int iters)
+{
+    const T alpha = sc[0];
+    const T dhx = sc[1] / (hx * hx), dhy = sc[1] / (hy * hy);
+    const T cdhx = sc[2] / (2 * hx), cdhy = sc[2] / (2 * hy);
+    __shared__ T phi[BOTTOM_MAX], rhs[BOTTOM_MAX], a[BOTTOM_MAX];
+    __shared__ T bX[BOTTOM_MAX + 64], bY[BOTTOM_MAX + 64];
+    __shared__ T cX[BOTTOM_MAX], cY[BOTTOM_MAX];
+    const int n   = nx * ny;
+    const int tid = threadIdx.x;
+    for (int c = tid; c < n; c += blockDim.x)
+    {
+        phi[c] = phi_g[c];
+        rhs[c] = rhs_g[c];
+        a[c]   = a_g[c];
+        if (cX_g)
+        {
+            cX[c] = cX_g[c];
+            cY[c] = cY_g[c];
+        }
+    }
+    for (int c = tid; c < (nx + 1) * ny; c += blockDim.x)
+        bX[c] = bX_g[c];
+    for (int c = tid; c < nx * (ny + 1); c += blockDim.x)
+        bY[c] = bY_g[c];
+    __syncthreads();
+    for (int it = 0; it < iters; ++it)
+        for (int redblack = 0; redblack < 2; ++redblack)
+        {
+            for (int c = tid; c < n; c += blockDim.x)
+            {
+                const int j = c / nx, i = c - j * nx;
+                if ((i + j + redblack) & 1)
+                    continue;
+                const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+                const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+                const T bxl = bX[j * (nx + 1) + i];
+                const T bxr = bX[j * (nx + 1) + i + 1];
+                const T byl = bY[j * nx + i];
+                const T byr = bY[(j + 1) * nx + i];
+                const T pl = phi[j * nx + im], pr = phi[j * nx + ip];
+                const T pd = phi[jm * nx + i], pu = phi[jp * nx + i];
+                const T gamma = alpha * a[c] + dhx * (bxl + bxr) + dhy * (byl + byr);
+                T rho = dhx * (bxl * pl + bxr * pr) + dhy * (byl * pd + byr * pu);
+                if (cX_g)
+                    rho = rho - cdhx * cX[c] * (pr - pl) - cdhy * cY[c] * (pu - pd);
+                phi[c] = (rhs[c] + rho - phi[c] * T(0)) / (gamma - T(0));
+            }
+            __syncthreads();
+        }
+    for (int c = tid; c < n; c += blockDim.x)
+        phi_g[c] = phi[c];
 }
 
 template <class TO, class TI>
@@ -307,6 +378,9 @@ template <class T> struct MGHierarchy
 {
     std::vector<MGLevel<T>> lev;
     T                       alpha = 0, beta = 0, gamma = 0;
+    T                       host_sc[3] = { 0, 0, 0 };
+    T                      *d_sc       = nullptr; // (alpha, beta, gamma) on the device
+    bool                    with_c     = false;
     int                     nu1 = 2, nu2 = 2, nu_bottom = 64;
 
     void free_all()
@@ -316,6 +390,9 @@ template <class T> struct MGHierarchy
                 if (p)
                     (void)hipFree(p);
         lev.clear();
+        if (d_sc)
+            (void)hipFree(d_sc);
+        d_sc = nullptr;
     }
 };
 
@@ -344,9 +421,18 @@ struct bx_helm
     // learned smoother
     bx_cnn *cnn       = nullptr;
     double  cnn_omega = 1.0;
+    int     cnn_mode  = BX_CNN_SMOOTHER;
     double *tmp1 = nullptr, *tmp2 = nullptr;
     float  *fr = nullptr, *fe = nullptr;
     bool    coeffs_set = false;
+    // preconditioner replayed as a captured graph (FGMRES): input pin,
+    // output pout; recaptured when anything it bakes in changes
+    // (slot 0: plain V-cycle, slot 1: with the network)
+    int             use_graph  = 1;
+    hipStream_t     cap_stream = nullptr;
+    hipGraphExec_t  pgraph[2]  = { nullptr, nullptr };
+    long long       gen = 0, pgraph_gen[2] = { -1, -1 }, pgraph_epoch[2] = { -1, -1 };
+    double         *pin = nullptr, *pout = nullptr;
 };
 
 static thread_local std::string g_helm_err;
@@ -373,16 +459,32 @@ template <class T> static int halloc(T **p, long long n)
 }
 
 template <class T>
-static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s)
+static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
+                           bool *reallocated)
 {
-    mg.free_all();
-    mg.alpha = (T)h->alpha;
-    mg.beta  = (T)h->beta;
-    mg.gamma = (T)h->gamma;
+    mg.alpha      = (T)h->alpha;
+    mg.beta       = (T)h->beta;
+    mg.gamma      = (T)h->gamma;
+    mg.host_sc[0] = mg.alpha;
+    mg.host_sc[1] = mg.beta;
+    mg.host_sc[2] = mg.gamma;
+    int rc;
+    // the level arrays depend only on the sizes and on whether the gradient
+    // term is present: keep them (and every pointer a captured graph holds)
+    // when those are unchanged, and refresh the values only
+    const bool keep = !mg.lev.empty() && mg.with_c == h->has_c;
+    if (!keep)
+    {
+        *reallocated = true;
+        mg.free_all();
+        if ((rc = halloc(&mg.d_sc, 3)))
+            return rc;
+        mg.with_c = h->has_c;
+    }
+    H_CHECK(hipMemcpyAsync(mg.d_sc, mg.host_sc, sizeof(T) * 3, hipMemcpyHostToDevice, s));
     int nx = h->nx, ny = h->ny;
     double dx = h->dx, dy = h->dy;
-    int rc;
-    while (true)
+    while (!keep)
     {
         MGLevel<T> L;
         L.nx    = nx;
@@ -448,20 +550,17 @@ static void launch_apply(const MGLevel<T> &L, const MGHierarchy<T> &mg,
                          const T *x, T *y, const T *rhs, hipStream_t s)
 {
     hipLaunchKernelGGL((k_apply<T>), grid2(L.nx, L.ny), dim3(256), 0, s, x, y,
-                       L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.alpha,
-                       mg.beta, mg.gamma, L.dxinv, L.dyinv, rhs);
+                       L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.dxinv,
+                       L.dyinv, rhs);
 }
 
 template <class T>
 static void launch_smooth(const MGLevel<T> &L, const MGHierarchy<T> &mg,
                           T *phi, const T *rhs, int redblack, hipStream_t s)
 {
-    const T hx = L.hx, hy = L.hy;
-    const T dhx = mg.beta / (hx * hx), dhy = mg.beta / (hy * hy);
-    const T cdhx = mg.gamma / (2 * hx), cdhy = mg.gamma / (2 * hy);
     hipLaunchKernelGGL((k_gsrb<T>), grid2((L.nx + 1) / 2, L.ny), dim3(256), 0,
                        s, phi, rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny,
-                       mg.alpha, dhx, dhy, cdhx, cdhy, redblack);
+                       mg.d_sc, L.hx, L.hy, redblack);
 }
 
 template <class T>
@@ -470,6 +569,14 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s)
     MGLevel<T> &L = mg.lev[l];
     if (l + 1 == (int)mg.lev.size())
     {
+        if (L.nx * L.ny <= BOTTOM_MAX && (L.nx + 1) * L.ny <= BOTTOM_MAX + 64
+            && L.nx * (L.ny + 1) <= BOTTOM_MAX + 64)
+        {
+            hipLaunchKernelGGL((k_bottom_gsrb<T>), dim3(1), dim3(256), 0, s, L.x, L.rhs,
+                               L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.hx,
+                               L.hy, mg.nu_bottom);
+            return;
+        }
         for (int it = 0; it < mg.nu_bottom; ++it)
         {
             launch_smooth(L, mg, L.x, L.rhs, 0, s);
@@ -565,7 +672,8 @@ extern "C" int bx_helm_create(int nx, int ny, double dx, double dy,
         || (rc = halloc(&h->tmp1, h->n)) || (rc = halloc(&h->tmp2, h->n))
         || (rc = halloc(&h->fr, h->n)) || (rc = halloc(&h->fe, h->n))
         || (rc = halloc(&h->partial, (long long)h->nb_dot * (h->m + 1)))
-        || (rc = halloc(&h->dots, h->m + 1)) || (rc = halloc(&h->hdev, h->m + 1)))
+        || (rc = halloc(&h->dots, h->m + 1)) || (rc = halloc(&h->hdev, h->m + 1))
+        || (rc = halloc(&h->pin, h->n)) || (rc = halloc(&h->pout, h->n)))
     {
         bx_helm_destroy(h);
         return rc;
@@ -580,11 +688,16 @@ extern "C" void bx_helm_destroy(bx_helm *h)
         return;
     h->mg64.free_all();
     h->mg32.free_all();
+    for (hipGraphExec_t g : h->pgraph)
+        if (g)
+            (void)hipGraphExecDestroy(g);
+    if (h->cap_stream)
+        (void)hipStreamDestroy(h->cap_stream);
     for (void *p : { (void *)h->a, (void *)h->bX, (void *)h->bY, (void *)h->cX,
                      (void *)h->cY, (void *)h->V, (void *)h->Z, (void *)h->w,
                      (void *)h->r, (void *)h->partial, (void *)h->dots,
                      (void *)h->hdev, (void *)h->tmp1, (void *)h->tmp2,
-                     (void *)h->fr, (void *)h->fe })
+                     (void *)h->fr, (void *)h->fe, (void *)h->pin, (void *)h->pout })
         if (p)
             (void)hipFree(p);
     delete h;
@@ -618,9 +731,13 @@ extern "C" int bx_helm_set_coeffs(bx_helm *h, double alpha, double beta,
         H_CHECK(hipMemcpyAsync(h->cY, cY, sizeof(double) * h->n,
                                hipMemcpyDeviceToDevice, s));
     }
-    int rc;
-    if ((rc = build_hierarchy(h, h->mg64, s)) || (rc = build_hierarchy(h, h->mg32, s)))
+    int  rc;
+    bool realloc = false;
+    if ((rc = build_hierarchy(h, h->mg64, s, &realloc))
+        || (rc = build_hierarchy(h, h->mg32, s, &realloc)))
         return rc;
+    if (realloc)
+        ++h->gen;
     H_CHECK(hipStreamSynchronize(s));
     h->coeffs_set = true;
     return BX_OK;
@@ -642,6 +759,15 @@ extern "C" int bx_helm_set_options(bx_helm *h, int nu1, int nu2, int nu_bottom,
     h->mg32.nu_bottom = nu_bottom;
     h->precond_fp32   = precond_fp32;
     h->m              = restart;
+    ++h->gen;
+    return BX_OK;
+}
+
+extern "C" int bx_helm_set_graph(bx_helm *h, int use_graph)
+{
+    if (!h)
+        return herr(BX_ERR_INVALID, "bx_helm_set_graph: null solver");
+    h->use_graph = use_graph ? 1 : 0;
     return BX_OK;
 }
 
@@ -682,9 +808,25 @@ extern "C" int bx_helm_smooth(bx_helm *h, double *phi, const double *rhs,
 //   e  = omega * CNN(r)                 (learned pre-smoothing)
 //   e += P Vcycle_coarse(R (r - A e))
 //   e += omega * CNN(r - A e)           (learned post-smoothing)
+static int precond_vcycle(bx_helm *h, const double *rv, double *e, hipStream_t s);
+
 static int precond(bx_helm *h, const double *rv, double *e, hipStream_t s)
 {
     const long long n = h->n;
+    if (h->cnn && h->cnn_mode == BX_CNN_CORRECTOR)
+    {
+        // e = V(r); e += omega * CNN(r - A e)
+        int rc;
+        if ((rc = precond_vcycle(h, rv, e, s)))
+            return rc;
+        residual64(h, rv, e, h->tmp1, s);
+        if ((rc = bx_cnn_forward(h->cnn, h->tmp1, h->tmp2, h->nx, h->ny,
+                                 h->cnn_omega, s)))
+            return rc;
+        hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, s, e, h->tmp2, n, 1.0);
+        H_CHECK(hipGetLastError());
+        return BX_OK;
+    }
     if (h->cnn)
     {
         int rc;
@@ -731,6 +873,13 @@ static int precond(bx_helm *h, const double *rv, double *e, hipStream_t s)
         H_CHECK(hipGetLastError());
         return BX_OK;
     }
+    return precond_vcycle(h, rv, e, s);
+}
+
+// e = one V-cycle on r from a zero guess, FP32 or FP64 hierarchy
+static int precond_vcycle(bx_helm *h, const double *rv, double *e, hipStream_t s)
+{
+    const long long n = h->n;
     if (h->precond_fp32)
     {
         MGHierarchy<float> &mg = h->mg32;
@@ -765,12 +914,67 @@ extern "C" int bx_helm_precond(bx_helm *h, const double *r, double *e,
     return precond(h, r, e, (hipStream_t)stream);
 }
 
-extern "C" int bx_helm_attach_cnn(bx_helm *h, bx_cnn *cnn, double omega)
+extern "C" int bx_helm_attach_cnn(bx_helm *h, bx_cnn *cnn, double omega, int mode)
 {
-    if (!h)
-        return herr(BX_ERR_INVALID, "bx_helm_attach_cnn: null solver");
+    if (!h || (mode != BX_CNN_SMOOTHER && mode != BX_CNN_CORRECTOR))
+        return herr(BX_ERR_INVALID, "bx_helm_attach_cnn: bad argument");
     h->cnn       = cnn;
     h->cnn_omega = omega;
+    h->cnn_mode  = mode;
+    ++h->gen;
+    return BX_OK;
+}
+
+// e = M r through the captured graph: r -> pin, replay, pout -> e
+static int precond_replay(bx_helm *h, const double *rv, double *e, hipStream_t s)
+{
+    if (!h->use_graph)
+        return precond(h, rv, e, s);
+    const int       slot  = h->cnn ? 1 : 0;
+    const long long epoch = h->cnn ? bx_cnn_epoch(h->cnn) : 0;
+    hipGraphExec_t &gx    = h->pgraph[slot];
+    if (!gx || h->pgraph_gen[slot] != h->gen || h->pgraph_epoch[slot] != epoch)
+    {
+        if (gx)
+        {
+            H_CHECK(hipGraphExecDestroy(gx));
+            gx = nullptr;
+        }
+        if (!h->cap_stream)
+            H_CHECK(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+        // everything the capture touches must be allocated beforehand
+        if (h->cnn)
+        {
+            int rc = bx_cnn_reserve(h->cnn, h->n);
+            if (rc)
+                return rc;
+        }
+        H_CHECK(hipStreamSynchronize(s));
+        hipGraph_t g = nullptr;
+        H_CHECK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = precond(h, h->pin, h->pout, h->cap_stream);
+        hipError_t ec = hipStreamEndCapture(h->cap_stream, &g);
+        if (rc)
+        {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess)
+            return herr(BX_ERR_HIP, std::string("precond capture: ") + hipGetErrorString(ec));
+        ec = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ec != hipSuccess)
+        {
+            gx = nullptr;
+            return herr(BX_ERR_HIP, std::string("precond instantiate: ") + hipGetErrorString(ec));
+        }
+        h->pgraph_gen[slot]   = h->gen;
+        h->pgraph_epoch[slot] = epoch;
+    }
+    H_CHECK(hipMemcpyAsync(h->pin, rv, sizeof(double) * h->n, hipMemcpyDeviceToDevice, s));
+    H_CHECK(hipGraphLaunch(gx, s));
+    H_CHECK(hipMemcpyAsync(e, h->pout, sizeof(double) * h->n, hipMemcpyDeviceToDevice, s));
     return BX_OK;
 }
 
@@ -852,7 +1056,7 @@ static int solve_fgmres(bx_helm *h, const double *b, double *x, double rtol,
         {
             double *vj = h->V + (long long)j * n;
             double *zj = h->Z + (long long)j * n;
-            if ((rc = precond(h, vj, zj, s)))
+            if ((rc = precond_replay(h, vj, zj, s)))
                 return rc;
             ++st->precond_applies;
             launch_apply(h->mg64.lev[0], h->mg64, zj, h->w,

@@ -242,11 +242,15 @@ class HelmholtzSolver:
         _chk(lib().bx_helm_precond(self.h, r_dev.ptr, e_dev.ptr, None), "bx_helm_precond")
         lib().bx_stream_sync(None)
 
-    def attach(self, cnn, omega=1.0):
+    def attach(self, cnn, omega=1.0, mode="corrector"):
         self.cnn = cnn
         if cnn is not None:
             cnn.set_operator(self)
-        _chk(lib().bx_helm_attach_cnn(self.h, cnn.c if cnn else None, omega), "bx_helm_attach_cnn")
+        _chk(lib().bx_helm_attach_cnn(self.h, cnn.c if cnn else None, omega, _lib.BX_CNN_MODE[mode]),
+             "bx_helm_attach_cnn")
+
+    def set_graph(self, use_graph):
+        _chk(lib().bx_helm_set_graph(self.h, int(bool(use_graph))), "bx_helm_set_graph")
 
     def solve(self, b_dev, x_dev, method="mg", rtol=1e-10, maxiter=200):
         st = _lib.bx_solve_stats()

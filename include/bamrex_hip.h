@@ -268,9 +268,19 @@ int bx_helm_smooth(bx_helm *h, double *phi, const double *rhs, int redblack,
                    void *stream);
 /* e = M r: one preconditioner application from a zero guess. */
 int bx_helm_precond(bx_helm *h, const double *r, double *e, void *stream);
-/* Use `cnn` (NULL detaches) as the fine-level smoother of the preconditioner,
- * correction scaled by omega. */
-int bx_helm_attach_cnn(bx_helm *h, bx_cnn *cnn, double omega);
+/* Learned preconditioner variants (bx_helm_attach_cnn mode):
+ *   BX_CNN_SMOOTHER   e = w N(r); e += P V_coarse(R(r - A e)); e += w N(r - A e)
+ *                     (the network replaces the fine-level smoothing)
+ *   BX_CNN_CORRECTOR  e = V(r); e += w N(r - A e)
+ *                     (the network corrects what the V-cycle leaves) */
+#define BX_CNN_SMOOTHER 0
+#define BX_CNN_CORRECTOR 1
+/* Use `cnn` (NULL detaches) in the preconditioner, correction scaled by
+ * omega, in the given mode. */
+int bx_helm_attach_cnn(bx_helm *h, bx_cnn *cnn, double omega, int mode);
+/* FGMRES replays its preconditioner as a captured hipGraph (default 1);
+ * 0 launches every kernel of it directly. */
+int bx_helm_set_graph(bx_helm *h, int use_graph);
 /* MLMG::solve(phi, rhs, rtol, 0) (IMEX_O1.cpp:602-614) with the chosen method;
  * x holds the initial guess.  FGMRES that stalls (no 2x residual drop in 8
  * iterations) or hits maxiter hands its iterate to plain multigrid

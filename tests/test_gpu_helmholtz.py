@@ -243,3 +243,93 @@ def test_training_reduces_loss_and_fallback_on_garbage():
     assert st["fell_back"] and st["converged"]
     s.close()
     net.close()
+
+
+def test_graph_replay_matches_direct_launches():
+    announce("helm fgmres graph vs direct 128")
+    n = 128
+    systems, dx = gresho_systems(n, 1)
+    sy, p0 = systems[0]
+    s = HelmholtzSolver(n, n, dx)
+    s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+    net = LearnedSmoother(seed=5)
+    out = {}
+    for mode in (None, "corrector", "smoother"):
+        for graph in (False, True):
+            if mode:
+                s.attach(net, omega=0.8, mode=mode)
+            s.set_graph(graph)
+            db, dxv = dev(sy["rhs"]), dev(p0)
+            st = s.solve(db, dxv, "fgmres_cnn" if mode else "fgmres_mg", rtol=1e-10, maxiter=60)
+            out[(mode, graph)] = (dxv.download(), st["iters"])
+        assert np.array_equal(out[(mode, False)][0], out[(mode, True)][0])
+        assert out[(mode, False)][1] == out[(mode, True)][1]
+    # coefficient updates of the same sizes keep the captured graph valid
+    s.set_coeffs(sy["alpha"], 0.5 * sy["beta"], sy["a"], sy["bX"], sy["bY"])
+    db, dxv = dev(sy["rhs"]), dev(p0)
+    st = s.solve(db, dxv, "fgmres_cnn", rtol=1e-10, maxiter=60)
+    r = ho.apply(dxv.download(), sy["a"], sy["bX"], sy["bY"], 1.0, 0.5 * sy["beta"], dx, dx, rhs=sy["rhs"])
+    assert st["converged"] and np.linalg.norm(r) <= 1e-10 * np.linalg.norm(sy["rhs"])
+    s.close()
+    net.close()
+
+
+def _reduction(s, sy, dx, fields):
+    """mean ||r - A M r|| / ||r|| of one preconditioner application, FP64
+    residual from the oracle."""
+    out = []
+    for r in fields:
+        dr, de = dev(r), DeviceArray(r.shape)
+        s.precond(dr, de)
+        rr = ho.apply(de.download(), sy["a"], sy["bX"], sy["bY"], 1.0, sy["beta"], dx, dx, rhs=r)
+        out.append(np.linalg.norm(rr) / np.linalg.norm(r))
+    return float(np.mean(out))
+
+
+def test_trained_network_gain():
+    """What the trained network adds to the V-cycle: residual reduction of
+    one preconditioner application on held-out post-V-cycle residuals, for
+    the V-cycle alone, with the untrained network and with the trained one
+    (corrector mode), and FGMRES iteration counts.  The numbers are printed;
+    the assertions are that training helps and that the trained correction
+    does not make the V-cycle worse."""
+    announce("cnn gain 256")
+    n = 256
+    wl = imex.GreshoWorkload(n)
+    dx = wl.dx
+    s = HelmholtzSolver(n, n, dx)
+    R = DeviceArray((48, n, n))
+    nf = 0
+    counts = []
+    for k in range(9):
+        sy = wl.next_system()
+        s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+        db, dxv = dev(sy["rhs"]), dev(sy["p_prev"])
+        got, _ = imex.harvest_residuals(s, db, dxv, R, nf, 48 - nf)
+        counts.append(got)
+        nf += got
+        wl.advance(sy, dxv.download())
+    # the last system's residual fields are held out; the solver still
+    # holds that system's coefficients
+    n_train = nf - counts[-1]
+    held = list(R.download()[n_train:nf])
+    rho_v = _reduction(s, sy, dx, held)
+    net = LearnedSmoother(seed=9)
+    s.attach(net, omega=1.0, mode="corrector")
+    rho_untrained = _reduction(s, sy, dx, held)
+    hist = net.train(R, n_train, steps=400, crop=64, batch=16, lr=1e-3, seed=5)
+    s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+    rho_trained = _reduction(s, sy, dx, held)
+    its = {}
+    for label, meth in (("fgmres_mg", "fgmres_mg"), ("fgmres_cnn", "fgmres_cnn"), ("mg", "mg")):
+        db, dxv = dev(sy["rhs"]), dev(np.zeros((n, n)))
+        its[label] = s.solve(db, dxv, meth, rtol=1e-10, maxiter=100)["iters"]
+    print("[gain] train fields %d held-out %d loss %.4f -> %.4f; one application reduces held-out "
+          "residuals by V %.4g, V+untrained %.4g, V+trained %.4g; iterations from zero %s" % (
+              n_train, nf - n_train, np.mean(hist[:20]), np.mean(hist[-20:]), rho_v, rho_untrained,
+              rho_trained, its), flush=True)
+    assert np.mean(hist[-20:]) < np.mean(hist[:20])
+    assert rho_trained < rho_untrained
+    assert rho_trained <= rho_v * 1.001
+    s.close()
+    net.close()

@@ -58,26 +58,30 @@ BX_DEV double div_by_recip(double a, double b, double y)
     return __builtin_fma(e, y, q0);
 }
 
+// Also returns y = RN(1 / rho) for further quotients by rho.
 template <int D, bool E1 = false>
 BX_DEV void primv_from_consv(const double (&U)[D + 2], double (&Q)[D + 2],
-                             const EulerParm &pm)
+                             const EulerParm &pm, double &y)
 {
     Q[0] = U[0];
-    if (D == 1)
-        Q[1] = U[1] / U[0];
-    else
-    {
-        const double y = 1.0 / U[0];
+    y    = 1.0 / U[0];
 #pragma unroll
-        for (int d = 0; d < D; ++d)
-            Q[1 + d] = div_by_recip(U[1 + d], U[0], y);
-    }
+    for (int d = 0; d < D; ++d)
+        Q[1 + d] = div_by_recip(U[1 + d], U[0], y);
     double u_sq = Q[1] * Q[1];
 #pragma unroll
     for (int d = 1; d < D; ++d)
         u_sq = u_sq + Q[1 + d] * Q[1 + d];
     // 0.5 * eps * rho: (0.5*eps)*rho == 0.5*rho exactly when eps == 1
     Q[1 + D] = (pm.gamma - 1) * (U[1 + D] - mul_eps<E1>(0.5, pm.eps) * U[0] * u_sq);
+}
+
+template <int D, bool E1 = false>
+BX_DEV void primv_from_consv(const double (&U)[D + 2], double (&Q)[D + 2],
+                             const EulerParm &pm)
+{
+    double y;
+    primv_from_consv<D, E1>(U, Q, pm, y);
 }
 
 template <int D, int dir, bool E1 = false>
@@ -239,9 +243,9 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
                       double (&F)[D + 2], const EulerParm &pm)
 {
     constexpr int NC = D + 2;
-    double        QL[NC], QR[NC];
-    primv_from_consv<D, E1>(UL, QL, pm);
-    primv_from_consv<D, E1>(UR, QR, pm);
+    double        QL[NC], QR[NC], yL, yR;
+    primv_from_consv<D, E1>(UL, QL, pm, yL);
+    primv_from_consv<D, E1>(UR, QR, pm, yR);
 
     const double adiabatic = pm.gamma;
     const double epsilon   = pm.eps;
@@ -254,8 +258,21 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
     const double u_L       = QL[1 + dir];
     const double u_R       = QR[1 + dir];
 
-    const double c_L = sqrt(adiabatic * p_L / mul_eps<E1>(density_L, epsilon));
-    const double c_R = sqrt(adiabatic * p_R / mul_eps<E1>(density_R, epsilon));
+    double c_L, c_R;
+    if (E1)
+    {
+        // gamma p / rho with the reciprocal of the (floored) density
+        constexpr double inv_min_rho = 1.0 / min_rho;
+        const double     yLd = QL[0] < min_rho ? inv_min_rho : yL;
+        const double     yRd = QR[0] < min_rho ? inv_min_rho : yR;
+        c_L                  = sqrt(div_by_recip(adiabatic * p_L, density_L, yLd));
+        c_R                  = sqrt(div_by_recip(adiabatic * p_R, density_R, yRd));
+    }
+    else
+    {
+        c_L = sqrt(adiabatic * p_L / mul_eps<E1>(density_L, epsilon));
+        c_R = sqrt(adiabatic * p_R / mul_eps<E1>(density_R, epsilon));
+    }
 
     const double z = (adiabatic - 1) / (2 * adiabatic);
     const double p_rr = bx_pow_invz(
@@ -322,7 +339,7 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
         st[1 + dir] = speed_interm;
         const double scale_factor
             = density * (speed_K - u_K) / (speed_K - speed_interm);
-        st[1 + D] = energy / density
+        st[1 + D] = div_by_recip(energy, density, left ? yL : yR)
                     + (speed_interm - u_K)
                           * (mul_eps<E1>(speed_interm, epsilon)
                              + pressure / (density * (speed_K - u_K)));
@@ -349,13 +366,15 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
 template <int D, bool E1 = false>
 BX_DEV double cell_wave_speed(const double (&U)[D + 2], const EulerParm &pm)
 {
-    double Q[D + 2];
-    primv_from_consv<D, E1>(U, Q, pm);
+    double Q[D + 2], y;
+    primv_from_consv<D, E1>(U, Q, pm, y);
     double vsq = Q[1] * Q[1];
 #pragma unroll
     for (int d = 1; d < D; ++d)
         vsq = vsq + Q[1 + d] * Q[1 + d];
     const double velocity = sqrt(vsq);
+    if (E1)
+        return sqrt(div_by_recip(pm.gamma * Q[1 + D], Q[0], y)) + velocity;
     return sqrt(pm.gamma * Q[1 + D] / mul_eps<E1>(Q[0], pm.eps)) + velocity;
 }
 

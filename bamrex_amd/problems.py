@@ -147,11 +147,19 @@ def isentropic_vortex(layout, U, prob_lo, prob_hi, dx, time=0.0, adiabatic=1.4, 
     dim = layout.dim
     for b in range(len(layout.boxes)):
         cc = layout.cell_centers(b, prob_lo, dx)
-        rho, u, v, p = isentropic_vortex_fields(cc[0], cc[1], time, adiabatic, lam, density_bg,
+        x, y = cc[0], cc[1]
+        if dim == 3:
+            # column vortex: the fields do not depend on z; evaluate one slice
+            x, y = x[0], y[0]
+        rho, u, v, p = isentropic_vortex_fields(x, y, time, adiabatic, lam, density_bg,
                                                 u_bg, v_bg, pressure_bg, x_c, y_c, prob_lo,
                                                 prob_hi)
         q = [rho, u, v] + ([np.full_like(u, w_bg)] if dim == 3 else []) + [p]
-        _set_valid(layout, U, b, consv_from_primv(q, adiabatic, epsilon, dim))
+        Q = consv_from_primv(q, adiabatic, epsilon, dim)
+        if dim == 3:
+            nz = cc[0].shape[0]
+            Q = [np.broadcast_to(c, (nz,) + c.shape) for c in Q]
+        _set_valid(layout, U, b, Q)
 
 
 def vortex_setup(ncell=512, dim=3, max_grid_size=64, ng=2):

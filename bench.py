@@ -55,6 +55,11 @@ def main():
     ap.add_argument("--imex-train-steps", type=int, default=640)
     ap.add_argument("--imex-batch", type=int, default=32)
     ap.add_argument("--imex-crop", type=int, default=64)
+    ap.add_argument("--no-vortex", action="store_true", help="skip the 3D isentropic vortex leg")
+    ap.add_argument("--vortex-n", type=int, default=512)
+    ap.add_argument("--vortex-mgs", type=int, default=128)
+    ap.add_argument("--vortex-steps", type=int, default=50)
+    ap.add_argument("--no-uneven", action="store_true", help="skip the batched uneven-box leg")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -156,6 +161,12 @@ def main():
                 "algo_bytes_per_launch": algo_bytes}
 
     lev.close()
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_leg(args)
+    vortex = None
+    if not args.no_vortex:
+        vortex = vortex_leg(args)
     imex = None
     if not args.no_imex:
         imex = imex_leg(args)
@@ -174,9 +185,128 @@ def main():
             "ghost_cells": lay.ng, "graph": True, "tuning": tuning},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "uneven": uneven,
+        "vortex3d": vortex,
         "imex": imex,
     }
     print(json.dumps(out), flush=True)
+
+
+def _time_sweeps(lib, lev, dt, dx, stream, reps):
+    """Average ms per launch of each directional sweep (HIP events on the
+    sweeps' stream)."""
+    from bamrex_amd import _lib
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(lib.bx_event_create(ctypes.byref(ev0)), "event")
+    _lib.check(lib.bx_event_create(ctypes.byref(ev1)), "event")
+    out = []
+    for d in range(lev.layout.dim):
+        lev.sweep(d, dt, dx)
+        _lib.check(lib.bx_stream_sync(stream), "sync")
+        _lib.check(lib.bx_event_record(ev0, stream), "record")
+        for _ in range(reps):
+            lev.sweep(d, dt, dx)
+        _lib.check(lib.bx_event_record(ev1, stream), "record")
+        ms = ctypes.c_float()
+        _lib.check(lib.bx_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)), "elapsed")
+        out.append(ms.value / reps)
+    lib.bx_event_destroy(ev0)
+    lib.bx_event_destroy(ev1)
+    return out
+
+
+def uneven_leg(args):
+    """Batched uneven AMR level: 3000 2D boxes of 8..96 cells per side
+    (seed 20261202), 2 ghost cells, smooth background plus Mach 1.5-3 shock
+    fronts; one MHM-HLLC sweep per direction over all boxes in one persistent
+    launch, autotuned, and graph-replayed steps."""
+    from bamrex_amd import _lib, problems
+    from bamrex_amd.level import Level, make_params
+    lib = _lib.lib()
+    s = problems.uneven_setup()
+    lay = s["layout"]
+    ncells = lay.ncells()
+    gp = make_params(2, s["adiabatic"], s["epsilon"], limiter=s["limiter"])
+    stream = ctypes.c_void_p()
+    _lib.check(lib.bx_stream_create(ctypes.byref(stream)), "stream")
+    lev = Level(gp, lay, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], stream=stream)
+    lev.set_state(s["U"])
+    lev.set_tmp(s["U"])
+    dt = lev.estTimeStep(s["dx"], s["cfl"])
+    lev.autotune(s["dx"], dt)
+    tuning = lev.get_tuning()
+    lev.set_state(s["U"])
+    lev.set_tmp(s["U"])
+    ms = _time_sweeps(lib, lev, dt, s["dx"], stream, args.kernel_reps)
+    lev.set_state(s["U"])
+    lev.run(s["dx"], s["cfl"], max_step=3, use_graph=True)
+    lev.set_state(s["U"])
+    t0 = time.perf_counter()
+    _, nsteps = lev.run(s["dx"], s["cfl"], max_step=20, use_graph=True)
+    _lib.check(lib.bx_stream_sync(stream), "sync")
+    el = time.perf_counter() - t0
+    lev.close()
+    lib.bx_stream_destroy(stream)
+    bytes_per_sweep = ncells * 2 * 4 * 8
+    out = {"workload": "uneven-2d-3000-boxes-8..96-seed20261202-mhm-hllc", "boxes": len(lay.boxes),
+           "cells": ncells, "sweep_ms": [round(x, 4) for x in ms],
+           "sweep_cell_updates_per_s": [ncells / (x * 1e-3) for x in ms],
+           "sweep_hbm_frac": [round(bytes_per_sweep / (x * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for x in ms],
+           "graph_steps": nsteps, "step_cell_updates_per_s": nsteps * ncells / el, "tuning": tuning}
+    log("uneven: %d boxes, %d cells: sweeps %s ms, %.3e cell-updates/s per step" % (
+        len(lay.boxes), ncells, out["sweep_ms"], out["step_cell_updates_per_s"]))
+    return out
+
+
+def vortex_leg(args):
+    """3D isentropic vortex, periodic n^3 on [-5,5]^3, lambda 5 advected at
+    (1,1,0), MHM-HLLC with superbee limiting each conserved variable on
+    itself, CFL 0.9, graph-replayed steps; L1 error of the final state against
+    the analytic solution at the final time (Exec/IsentropicVortex/Prob.cpp)."""
+    import numpy as np
+    from bamrex_amd import _lib, problems
+    from bamrex_amd.level import Level, make_params
+    lib = _lib.lib()
+    n = args.vortex_n
+    log("vortex: building %d^3 level (mgs %d)" % (n, args.vortex_mgs))
+    s = problems.vortex_setup(ncell=n, dim=3, max_grid_size=args.vortex_mgs)
+    lay = s["layout"]
+    ncells = lay.ncells()
+    gp = make_params(3, s["adiabatic"], s["epsilon"], limiter=s["limiter"],
+                     limiter_indices=s["limiter_indices"], limit_on_primv=s["limit_on_primv"])
+    stream = ctypes.c_void_p()
+    _lib.check(lib.bx_stream_create(ctypes.byref(stream)), "stream")
+    lev = Level(gp, lay, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], stream=stream)
+    lev.set_state(s["U"])
+    lev.fill_boundary()
+    dt0 = lev.estTimeStep(s["dx"], s["cfl"])
+    lev.set_tmp(s["U"])
+    ms = _time_sweeps(lib, lev, dt0, s["dx"], stream, 3)
+    lev.set_state(s["U"])
+    _lib.check(lib.bx_stream_sync(stream), "sync")
+    t0 = time.perf_counter()
+    t_end, nsteps = lev.run(s["dx"], s["cfl"], max_step=args.vortex_steps, use_graph=True)
+    _lib.check(lib.bx_stream_sync(stream), "sync")
+    el = time.perf_counter() - t0
+    U = lev.get_state()
+    lev.close()
+    lib.bx_stream_destroy(stream)
+    ex = lay.new_array()
+    problems.isentropic_vortex(lay, ex, s["prob_lo"], s["prob_hi"], s["dx"], time=t_end)
+    l1 = np.zeros(lay.nc)
+    for b in range(len(lay.boxes)):
+        d = np.abs(lay.valid(U, b) - lay.valid(ex, b))
+        l1 += d.reshape(lay.nc, -1).sum(axis=1)
+    l1 = l1 / ncells          # mean absolute error per cell (volume-weighted L1 / volume)
+    out = {"workload": "isentropic-vortex-3d-%d^3-mgs%d-mhm-hllc-superbee-cfl0.9" % (n, args.vortex_mgs),
+           "cells": ncells, "steps": nsteps, "t_end": t_end,
+           "cell_updates_per_s": nsteps * ncells / el, "ms_per_step": el * 1e3 / nsteps,
+           "sweep_ms": [round(x, 4) for x in ms],
+           "sweep_hbm_frac": [round(ncells * 2 * 5 * 8 / (x * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) for x in ms],
+           "l1_error": [float(x) for x in l1]}
+    log("vortex: %d steps to t=%.4g, %.3e cell-updates/s, L1 rho %.3e" % (
+        nsteps, t_end, out["cell_updates_per_s"], l1[0]))
+    return out
 
 
 def imex_leg(args):

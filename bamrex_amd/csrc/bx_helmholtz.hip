@@ -32,27 +32,24 @@
 namespace bx {
 
 // ------------------------------------------------------------- kernels
+// Cell operations shared by the grid kernels and the single-workgroup tail
+// of the V-cycle, so both give the same bits.
+
+// (A x)_ij on a periodic level (mlalapgrad_adotx)
 template <class T>
-__global__ __launch_bounds__(256) void k_apply(
-    const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ a,
-    const T *__restrict__ bX, const T *__restrict__ bY,
-    const T *__restrict__ cX, const T *__restrict__ cY, int nx, int ny,
-    const T *__restrict__ sc, T dxinv, T dyinv, const T *__restrict__ rhs)
+__device__ __forceinline__ T apply_at(const T *__restrict__ x, const T *__restrict__ a,
+                                      const T *__restrict__ bX, const T *__restrict__ bY,
+                                      const T *__restrict__ cX, const T *__restrict__ cY,
+                                      int nx, int ny, T alpha, T beta, T gamma, T dxinv,
+                                      T dyinv, int i, int j)
 {
-    // sc = (alpha, beta, gamma) in device memory, so that a captured graph
-    // stays valid when the scalars change from one solve to the next
-    const T alpha = sc[0], beta = sc[1], gamma = sc[2];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (i >= nx)
-        return;
     const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
     const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
     const long long c   = (long long)j * nx + i;
     const T         xc  = x[c];
     const T         dhx = beta * dxinv * dxinv;
     const T         dhy = beta * dyinv * dyinv;
-    // faces: bX(i,j) at (long)j*(nx+1)+i, bY(i,j) at (long)j*nx+i
+    // faces: bX(i,j) at j*(nx+1)+i, bY(i,j) at j*nx+i
     const T bxl = bX[(long long)j * (nx + 1) + i];
     const T bxr = bX[(long long)j * (nx + 1) + i + 1];
     const T byl = bY[(long long)j * nx + i];
@@ -67,6 +64,57 @@ __global__ __launch_bounds__(256) void k_apply(
         const T chdy = gamma * cY[c] * T(0.5) * dyinv;
         v = v + chdx * (xr - xl) + chdy * (xu - xd);
     }
+    return v;
+}
+
+// red/black Gauss-Seidel update of cell (i, j) (alapgrad_gsrb; periodic
+// level: no boundary correction, delta = 0)
+template <class T>
+__device__ __forceinline__ void gsrb_at(T *__restrict__ phi, const T *__restrict__ rhs,
+                                        const T *__restrict__ a, const T *__restrict__ bX,
+                                        const T *__restrict__ bY, const T *__restrict__ cX,
+                                        const T *__restrict__ cY, int nx, int ny, T alpha,
+                                        T dhx, T dhy, T cdhx, T cdhy, int i, int j)
+{
+    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+    const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+    const long long c   = (long long)j * nx + i;
+    const T bxl = bX[(long long)j * (nx + 1) + i];
+    const T bxr = bX[(long long)j * (nx + 1) + i + 1];
+    const T byl = bY[(long long)j * nx + i];
+    const T byr = bY[(long long)(j + 1) * nx + i];
+    const T pl = phi[(long long)j * nx + im], pr = phi[(long long)j * nx + ip];
+    const T pd = phi[(long long)jm * nx + i], pu = phi[(long long)jp * nx + i];
+    const T gamma = alpha * a[c] + dhx * (bxl + bxr) + dhy * (byl + byr);
+    T rho = dhx * (bxl * pl + bxr * pr) + dhy * (byl * pd + byr * pu);
+    if (cX)
+        rho = rho - cdhx * cX[c] * (pr - pl) - cdhy * cY[c] * (pu - pd);
+    phi[c] = (rhs[c] + rho - phi[c] * T(0)) / (gamma - T(0));
+}
+
+template <class T>
+__device__ __forceinline__ T restrict_at(const T *__restrict__ f, int ncx, int I, int J)
+{
+    const int       nfx = 2 * ncx;
+    const long long f0  = (long long)(2 * J) * nfx + 2 * I;
+    return T(0.25) * (f[f0] + f[f0 + 1] + f[f0 + nfx] + f[f0 + nfx + 1]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_apply(
+    const T *__restrict__ x, T *__restrict__ y, const T *__restrict__ a,
+    const T *__restrict__ bX, const T *__restrict__ bY,
+    const T *__restrict__ cX, const T *__restrict__ cY, int nx, int ny,
+    const T *__restrict__ sc, T dxinv, T dyinv, const T *__restrict__ rhs)
+{
+    // sc = (alpha, beta, gamma) in device memory, so that a captured graph
+    // stays valid when the scalars change from one solve to the next
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= nx)
+        return;
+    const long long c = (long long)j * nx + i;
+    const T v = apply_at(x, a, bX, bY, cX, cY, nx, ny, sc[0], sc[1], sc[2], dxinv, dyinv, i, j);
     // residual form when rhs is given: y = rhs - A x
     y[c] = rhs ? rhs[c] - v : v;
 }
@@ -87,21 +135,7 @@ __global__ __launch_bounds__(256) void k_gsrb(
     const int i = 2 * t + ((j + redblack) & 1);
     if (i >= nx)
         return;
-    const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
-    const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
-    const long long c   = (long long)j * nx + i;
-    const T bxl = bX[(long long)j * (nx + 1) + i];
-    const T bxr = bX[(long long)j * (nx + 1) + i + 1];
-    const T byl = bY[(long long)j * nx + i];
-    const T byr = bY[(long long)(j + 1) * nx + i];
-    const T pl = phi[(long long)j * nx + im], pr = phi[(long long)j * nx + ip];
-    const T pd = phi[(long long)jm * nx + i], pu = phi[(long long)jp * nx + i];
-    // periodic level: no boundary correction terms (delta = 0)
-    const T gamma = alpha * a[c] + dhx * (bxl + bxr) + dhy * (byl + byr);
-    T rho = dhx * (bxl * pl + bxr * pr) + dhy * (byl * pd + byr * pu);
-    if (cX)
-        rho = rho - cdhx * cX[c] * (pr - pl) - cdhy * cY[c] * (pu - pd);
-    phi[c] = (rhs[c] + rho - phi[c] * T(0)) / (gamma - T(0));
+    gsrb_at(phi, rhs, a, bX, bY, cX, cY, nx, ny, alpha, dhx, dhy, cdhx, cdhy, i, j);
 }
 
 template <class T>
@@ -113,10 +147,7 @@ __global__ __launch_bounds__(256) void k_restrict(const T *__restrict__ f,
     const int J = blockIdx.y;
     if (I >= ncx)
         return;
-    const int       nfx = 2 * ncx;
-    const long long f0  = (long long)(2 * J) * nfx + 2 * I;
-    c[(long long)J * ncx + I]
-        = T(0.25) * (f[f0] + f[f0 + 1] + f[f0 + nfx] + f[f0 + nfx + 1]);
+    c[(long long)J * ncx + I] = restrict_at(f, ncx, I, J);
 }
 
 template <class T>
@@ -129,6 +160,96 @@ __global__ __launch_bounds__(256) void k_prolong_add(const T *__restrict__ c,
     if (i >= nfx)
         return;
     f[(long long)j * nfx + i] += c[(long long)(j >> 1) * (nfx >> 1) + (i >> 1)];
+}
+
+// The coarse end of a V-cycle (every level from l0 down, each of at most
+// TAIL_MAX_CELLS cells) in one workgroup: the same phases as the per-level
+// launches, separated by workgroup barriers instead of kernel boundaries.
+// The data stays in global memory (L2-resident at these sizes); all waves of
+// the workgroup run on one CU, so a barrier makes the stores of one phase
+// visible to the next.
+constexpr int TAIL_MAX_LEVELS = 8;
+constexpr int TAIL_MAX_CELLS  = 4096;
+
+template <class T> struct TailLevel
+{
+    T  *a, *bX, *bY, *cX, *cY, *x, *rhs, *res;
+    int nx, ny;
+    T   hx, hy, dxinv, dyinv;
+};
+
+template <class T> struct TailArgs
+{
+    TailLevel<T> lev[TAIL_MAX_LEVELS];
+};
+
+template <class T>
+__device__ void tail_smooth(const TailLevel<T> &L, T alpha, T beta, T gam, int pairs)
+{
+    const T dhx = beta / (L.hx * L.hx), dhy = beta / (L.hy * L.hy);
+    const T cdhx = gam / (2 * L.hx), cdhy = gam / (2 * L.hy);
+    const int n = L.nx * L.ny;
+    for (int it = 0; it < pairs; ++it)
+        for (int rb = 0; rb < 2; ++rb)
+        {
+            for (int c = threadIdx.x; c < n; c += blockDim.x)
+            {
+                const int j = c / L.nx, i = c - j * L.nx;
+                if ((i + j + rb) & 1)
+                    continue;
+                gsrb_at(L.x, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, alpha, dhx, dhy,
+                        cdhx, cdhy, i, j);
+            }
+            __syncthreads();
+        }
+}
+
+// A: level table in device memory (levels l0.. of the hierarchy)
+template <class T>
+__global__ __launch_bounds__(1024) void k_vcycle_tail(const TailArgs<T> *__restrict__ A,
+                                                      int nlev, int nu1, int nu2,
+                                                      int nu_bottom,
+                                                      const T *__restrict__ sc)
+{
+    const T alpha = sc[0], beta = sc[1], gam = sc[2];
+    const int last = nlev - 1;
+    for (int l = 0; l < last; ++l)
+    {
+        const TailLevel<T> L = A->lev[l];
+        const TailLevel<T> C = A->lev[l + 1];
+        tail_smooth(L, alpha, beta, gam, nu1);
+        const int n = L.nx * L.ny;
+        for (int c = threadIdx.x; c < n; c += blockDim.x)
+        {
+            const int j = c / L.nx, i = c - j * L.nx;
+            L.res[c] = L.rhs[c]
+                       - apply_at(L.x, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, alpha, beta,
+                                  gam, L.dxinv, L.dyinv, i, j);
+        }
+        __syncthreads();
+        const int nc = C.nx * C.ny;
+        for (int c = threadIdx.x; c < nc; c += blockDim.x)
+        {
+            const int J = c / C.nx, I = c - J * C.nx;
+            C.rhs[c] = restrict_at(L.res, C.nx, I, J);
+            C.x[c]   = T(0);
+        }
+        __syncthreads();
+    }
+    tail_smooth(A->lev[last], alpha, beta, gam, nu_bottom);
+    for (int l = last - 1; l >= 0; --l)
+    {
+        const TailLevel<T> L = A->lev[l];
+        const TailLevel<T> C = A->lev[l + 1];
+        const int n = L.nx * L.ny;
+        for (int c = threadIdx.x; c < n; c += blockDim.x)
+        {
+            const int j = c / L.nx, i = c - j * L.nx;
+            L.x[c] += C.x[(j >> 1) * C.nx + (i >> 1)];
+        }
+        __syncthreads();
+        tail_smooth(L, alpha, beta, gam, nu2);
+    }
 }
 
 // coarse face coefficients: average of the two fine faces on the coarse face
@@ -381,6 +502,8 @@ template <class T> struct MGHierarchy
     T                       host_sc[3] = { 0, 0, 0 };
     T                      *d_sc       = nullptr; // (alpha, beta, gamma) on the device
     bool                    with_c     = false;
+    int                     tail_l0    = -1;      // first level run by k_vcycle_tail
+    TailArgs<T>            *d_tail     = nullptr;
     int                     nu1 = 2, nu2 = 2, nu_bottom = 64;
 
     void free_all()
@@ -393,6 +516,10 @@ template <class T> struct MGHierarchy
         if (d_sc)
             (void)hipFree(d_sc);
         d_sc = nullptr;
+        if (d_tail)
+            (void)hipFree(d_tail);
+        d_tail  = nullptr;
+        tail_l0 = -1;
     }
 };
 
@@ -508,6 +635,29 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
         dx *= 2;
         dy *= 2;
     }
+    if (!keep)
+    {
+        // coarse levels that one workgroup runs in a single launch
+        const int nl = (int)mg.lev.size();
+        int       l0 = nl;
+        while (l0 > 0 && (long long)mg.lev[l0 - 1].nx * mg.lev[l0 - 1].ny <= TAIL_MAX_CELLS
+               && nl - (l0 - 1) <= TAIL_MAX_LEVELS)
+            --l0;
+        if (l0 < nl)
+        {
+            TailArgs<T> ta{};
+            for (int l = l0; l < nl; ++l)
+            {
+                const MGLevel<T> &L = mg.lev[l];
+                ta.lev[l - l0] = TailLevel<T>{ L.a,  L.bX, L.bY, L.cX, L.cY, L.x,
+                                               L.rhs, L.res, L.nx, L.ny, L.hx, L.hy,
+                                               L.dxinv, L.dyinv };
+            }
+            H_CHECK(hipMalloc((void **)&mg.d_tail, sizeof(TailArgs<T>)));
+            H_CHECK(hipMemcpy(mg.d_tail, &ta, sizeof(ta), hipMemcpyHostToDevice));
+            mg.tail_l0 = l0;
+        }
+    }
     // level 0 coefficients from the FP64 fine arrays
     MGLevel<T> &L0 = mg.lev[0];
     const long long n0 = (long long)h->nx * h->ny;
@@ -567,6 +717,15 @@ template <class T>
 static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s)
 {
     MGLevel<T> &L = mg.lev[l];
+    if (mg.tail_l0 >= 0 && l >= mg.tail_l0)
+    {
+        // the table starts at level tail_l0; shift it to start at level l
+        const TailArgs<T> *tab = reinterpret_cast<const TailArgs<T> *>(
+            reinterpret_cast<const TailLevel<T> *>(mg.d_tail) + (l - mg.tail_l0));
+        hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(1024), 0, s, tab,
+                           (int)mg.lev.size() - l, mg.nu1, mg.nu2, mg.nu_bottom, mg.d_sc);
+        return;
+    }
     if (l + 1 == (int)mg.lev.size())
     {
         if (L.nx * L.ny <= BOTTOM_MAX && (L.nx + 1) * L.ny <= BOTTOM_MAX + 64

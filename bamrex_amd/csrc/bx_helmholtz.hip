@@ -428,6 +428,21 @@ __global__ __launch_bounds__(256) void k_multidot(
     }
 }
 
+// y = x / sqrt(d[0]) when d[0] > 0 (the new Arnoldi vector)
+__global__ __launch_bounds__(256) void k_scale_inv_norm(double *__restrict__ y,
+                                                        const double *__restrict__ x,
+                                                        long long n,
+                                                        const double *__restrict__ d)
+{
+    const double hn = sqrt(d[0]);
+    if (!(hn > 0))
+        return;
+    const double s = 1.0 / hn;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] = x[i] * s;
+}
+
 __global__ void k_reduce_partials(const double *__restrict__ partial, int nb,
                                   int k, double *__restrict__ out)
 {
@@ -540,8 +555,8 @@ struct bx_helm
     MGHierarchy<double> mg64;
     MGHierarchy<float>  mg32;
     int                 precond_fp32 = 1;
-    // FGMRES workspace
-    int     m = 20;
+    // FGMRES workspace (restart length m <= m_max, the allocated basis)
+    int     m = 20, m_max = 20;
     double *V = nullptr, *Z = nullptr, *w = nullptr, *r = nullptr,
            *partial = nullptr, *dots = nullptr, *hdev = nullptr;
     int     nb_dot = 512;
@@ -826,12 +841,12 @@ extern "C" int bx_helm_create(int nx, int ny, double dx, double dy,
     int rc;
     if ((rc = halloc(&h->a, h->n)) || (rc = halloc(&h->bX, (long long)(nx + 1) * ny))
         || (rc = halloc(&h->bY, (long long)nx * (ny + 1)))
-        || (rc = halloc(&h->V, h->n * (h->m + 1))) || (rc = halloc(&h->Z, h->n * h->m))
+        || (rc = halloc(&h->V, h->n * (h->m_max + 1))) || (rc = halloc(&h->Z, h->n * h->m_max))
         || (rc = halloc(&h->w, h->n)) || (rc = halloc(&h->r, h->n))
         || (rc = halloc(&h->tmp1, h->n)) || (rc = halloc(&h->tmp2, h->n))
         || (rc = halloc(&h->fr, h->n)) || (rc = halloc(&h->fe, h->n))
-        || (rc = halloc(&h->partial, (long long)h->nb_dot * (h->m + 1)))
-        || (rc = halloc(&h->dots, h->m + 1)) || (rc = halloc(&h->hdev, h->m + 1))
+        || (rc = halloc(&h->partial, (long long)h->nb_dot * (h->m_max + 1)))
+        || (rc = halloc(&h->dots, 2 * (h->m_max + 1) + 2)) || (rc = halloc(&h->hdev, h->m_max + 1))
         || (rc = halloc(&h->pin, h->n)) || (rc = halloc(&h->pout, h->n)))
     {
         bx_helm_destroy(h);
@@ -905,7 +920,7 @@ extern "C" int bx_helm_set_coeffs(bx_helm *h, double alpha, double beta,
 extern "C" int bx_helm_set_options(bx_helm *h, int nu1, int nu2, int nu_bottom,
                                    int precond_fp32, int restart)
 {
-    if (!h || nu1 < 0 || nu2 < 0 || nu_bottom < 1 || restart < 1 || restart > h->m)
+    if (!h || nu1 < 0 || nu2 < 0 || nu_bottom < 1 || restart < 1 || restart > h->m_max)
         return herr(BX_ERR_INVALID, "bx_helm_set_options: bad argument");
     for (auto *mg : { &h->mg64 })
     {
@@ -1182,8 +1197,8 @@ static int solve_fgmres(bx_helm *h, const double *b, double *x, double rtol,
 {
     const long long n = h->n;
     const int       m = h->m;
-    std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), hv(m + 1),
-        hv2(m + 1), y(m);
+    std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1),
+        hdots(2 * (m + 1) + 1), y(m);
     *stalled      = false;
     int    total  = 0;
     double best   = 1e300;
@@ -1220,29 +1235,35 @@ static int solve_fgmres(bx_helm *h, const double *b, double *x, double rtol,
             ++st->precond_applies;
             launch_apply(h->mg64.lev[0], h->mg64, zj, h->w,
                          (const double *)nullptr, s);
-            // CGS2
-            if ((rc = dot_reduce(h, h->V, h->w, j + 1, s, hv.data())))
-                return rc;
-            H_CHECK(hipMemcpyAsync(h->hdev, hv.data(), sizeof(double) * (j + 1),
-                                   hipMemcpyHostToDevice, s));
+            // CGS2 chained on the device (the projections are read from
+            // device memory); one host synchronisation per iteration
+            double *d1 = h->dots, *d2 = h->dots + (m + 1), *d3 = h->dots + 2 * (m + 1);
+            hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->V, h->w, n,
+                               j + 1, h->partial);
+            hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(256), 0, s, h->partial,
+                               h->nb_dot, j + 1, d1);
             hipLaunchKernelGGL(k_subtract_combo, dim3(2048), dim3(256), 0, s, h->w,
-                               h->V, h->hdev, j + 1, n);
-            if ((rc = dot_reduce(h, h->V, h->w, j + 1, s, hv2.data())))
-                return rc;
-            H_CHECK(hipMemcpyAsync(h->hdev, hv2.data(), sizeof(double) * (j + 1),
-                                   hipMemcpyHostToDevice, s));
+                               h->V, d1, j + 1, n);
+            hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->V, h->w, n,
+                               j + 1, h->partial);
+            hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(256), 0, s, h->partial,
+                               h->nb_dot, j + 1, d2);
             hipLaunchKernelGGL(k_subtract_combo, dim3(2048), dim3(256), 0, s, h->w,
-                               h->V, h->hdev, j + 1, n);
+                               h->V, d2, j + 1, n);
+            hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->w, h->w, n,
+                               1, h->partial);
+            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, h->partial,
+                               h->nb_dot, 1, d3);
+            hipLaunchKernelGGL(k_scale_inv_norm, dim3(2048), dim3(256), 0, s,
+                               h->V + (long long)(j + 1) * n, h->w, n, d3);
+            H_CHECK(hipGetLastError());
+            H_CHECK(hipMemcpyAsync(hdots.data(), h->dots, sizeof(double) * (2 * (m + 1) + 1),
+                                   hipMemcpyDeviceToHost, s));
+            H_CHECK(hipStreamSynchronize(s));
             for (int i = 0; i <= j; ++i)
-                H[i * m + j] = hv[i] + hv2[i];
-            double hn = 0;
-            if ((rc = norm2(h, h->w, s, &hn)))
-                return rc;
+                H[i * m + j] = hdots[i] + hdots[(m + 1) + i];
+            const double hn    = std::sqrt(hdots[2 * (m + 1)]);
             H[(j + 1) * m + j] = hn;
-            if (hn > 0)
-                hipLaunchKernelGGL(k_scale, dim3(2048), dim3(256), 0, s,
-                                   h->V + (long long)(j + 1) * n, h->w, n,
-                                   1.0 / hn);
             // Givens rotations
             for (int i = 0; i < j; ++i)
             {

@@ -162,20 +162,21 @@ __global__ __launch_bounds__(256) void k_prolong_add(const T *__restrict__ c,
     f[(long long)j * nfx + i] += c[(long long)(j >> 1) * (nfx >> 1) + (i >> 1)];
 }
 
-// The coarse end of a V-cycle (every level from l0 down, each of at most
-// TAIL_MAX_CELLS cells) in one workgroup: the same phases as the per-level
-// launches, separated by workgroup barriers instead of kernel boundaries.
-// The data stays in global memory (L2-resident at these sizes); all waves of
-// the workgroup run on one CU, so a barrier makes the stores of one phase
-// visible to the next.
+// The coarse end of a V-cycle (the coarsest levels, as many as fit in
+// TAIL_LDS_BYTES of LDS) in one workgroup: the coefficients and the work
+// arrays of those levels are copied into LDS, the same phases as the
+// per-level launches run there separated by workgroup barriers, and the
+// correction of the first tail level is written back.
 constexpr int TAIL_MAX_LEVELS = 8;
-constexpr int TAIL_MAX_CELLS  = 4096;
+constexpr int TAIL_LDS_BYTES  = 60 * 1024;
 
 template <class T> struct TailLevel
 {
-    T  *a, *bX, *bY, *cX, *cY, *x, *rhs, *res;
-    int nx, ny;
-    T   hx, hy, dxinv, dyinv;
+    const T *a, *bX, *bY, *cX, *cY; // global coefficients
+    T       *x, *rhs;                // global work arrays (read/written for level 0 only)
+    int      nx, ny;
+    T        hx, hy, dxinv, dyinv;
+    int      o_a, o_bX, o_bY, o_cX, o_cY, o_x, o_rhs, o_res; // LDS offsets (elements)
 };
 
 template <class T> struct TailArgs
@@ -183,8 +184,32 @@ template <class T> struct TailArgs
     TailLevel<T> lev[TAIL_MAX_LEVELS];
 };
 
+// LDS view of one tail level
+template <class T> struct LdsLevel
+{
+    T  *a, *bX, *bY, *cX, *cY, *x, *rhs, *res;
+    int nx, ny;
+    T   hx, hy, dxinv, dyinv;
+};
+
 template <class T>
-__device__ void tail_smooth(const TailLevel<T> &L, T alpha, T beta, T gam, int pairs)
+__device__ LdsLevel<T> lds_level(const TailLevel<T> &G, T *sm)
+{
+    LdsLevel<T> L;
+    L.a   = sm + G.o_a;
+    L.bX  = sm + G.o_bX;
+    L.bY  = sm + G.o_bY;
+    L.cX  = G.cX ? sm + G.o_cX : nullptr;
+    L.cY  = G.cY ? sm + G.o_cY : nullptr;
+    L.x   = sm + G.o_x;
+    L.rhs = sm + G.o_rhs;
+    L.res = sm + G.o_res;
+    L.nx = G.nx; L.ny = G.ny; L.hx = G.hx; L.hy = G.hy; L.dxinv = G.dxinv; L.dyinv = G.dyinv;
+    return L;
+}
+
+template <class T>
+__device__ void tail_smooth(const LdsLevel<T> &L, T alpha, T beta, T gam, int pairs)
 {
     const T dhx = beta / (L.hx * L.hx), dhy = beta / (L.hy * L.hy);
     const T cdhx = gam / (2 * L.hx), cdhy = gam / (2 * L.hy);
@@ -204,19 +229,47 @@ __device__ void tail_smooth(const TailLevel<T> &L, T alpha, T beta, T gam, int p
         }
 }
 
-// A: level table in device memory (levels l0.. of the hierarchy)
+template <class T>
+__device__ void copy_in(T *dst, const T *__restrict__ src, int n)
+{
+    for (int c = threadIdx.x; c < n; c += blockDim.x)
+        dst[c] = src[c];
+}
+
+// A: level table in device memory (the levels from the first tail level)
 template <class T>
 __global__ __launch_bounds__(1024) void k_vcycle_tail(const TailArgs<T> *__restrict__ A,
                                                       int nlev, int nu1, int nu2,
                                                       int nu_bottom,
                                                       const T *__restrict__ sc)
 {
+    extern __shared__ __align__(16) unsigned char tail_smem[];
+    T *sm = reinterpret_cast<T *>(tail_smem);
     const T alpha = sc[0], beta = sc[1], gam = sc[2];
+    for (int l = 0; l < nlev; ++l)
+    {
+        const TailLevel<T> &G = A->lev[l];
+        const int n = G.nx * G.ny;
+        copy_in(sm + G.o_a, G.a, n);
+        copy_in(sm + G.o_bX, G.bX, (G.nx + 1) * G.ny);
+        copy_in(sm + G.o_bY, G.bY, G.nx * (G.ny + 1));
+        if (G.cX)
+        {
+            copy_in(sm + G.o_cX, G.cX, n);
+            copy_in(sm + G.o_cY, G.cY, n);
+        }
+        if (l == 0)
+        {
+            copy_in(sm + G.o_x, G.x, n);
+            copy_in(sm + G.o_rhs, G.rhs, n);
+        }
+    }
+    __syncthreads();
     const int last = nlev - 1;
     for (int l = 0; l < last; ++l)
     {
-        const TailLevel<T> L = A->lev[l];
-        const TailLevel<T> C = A->lev[l + 1];
+        const LdsLevel<T> L = lds_level(A->lev[l], sm);
+        const LdsLevel<T> C = lds_level(A->lev[l + 1], sm);
         tail_smooth(L, alpha, beta, gam, nu1);
         const int n = L.nx * L.ny;
         for (int c = threadIdx.x; c < n; c += blockDim.x)
@@ -236,11 +289,11 @@ __global__ __launch_bounds__(1024) void k_vcycle_tail(const TailArgs<T> *__restr
         }
         __syncthreads();
     }
-    tail_smooth(A->lev[last], alpha, beta, gam, nu_bottom);
+    tail_smooth(lds_level(A->lev[last], sm), alpha, beta, gam, nu_bottom);
     for (int l = last - 1; l >= 0; --l)
     {
-        const TailLevel<T> L = A->lev[l];
-        const TailLevel<T> C = A->lev[l + 1];
+        const LdsLevel<T> L = lds_level(A->lev[l], sm);
+        const LdsLevel<T> C = lds_level(A->lev[l + 1], sm);
         const int n = L.nx * L.ny;
         for (int c = threadIdx.x; c < n; c += blockDim.x)
         {
@@ -250,6 +303,9 @@ __global__ __launch_bounds__(1024) void k_vcycle_tail(const TailArgs<T> *__restr
         __syncthreads();
         tail_smooth(L, alpha, beta, gam, nu2);
     }
+    const TailLevel<T> &G0 = A->lev[0];
+    for (int c = threadIdx.x; c < G0.nx * G0.ny; c += blockDim.x)
+        G0.x[c] = sm[G0.o_x + c];
 }
 
 // coarse face coefficients: average of the two fine faces on the coarse face
@@ -518,6 +574,7 @@ template <class T> struct MGHierarchy
     T                      *d_sc       = nullptr; // (alpha, beta, gamma) on the device
     bool                    with_c     = false;
     int                     tail_l0    = -1;      // first level run by k_vcycle_tail
+    int                     tail_lds   = 0;       // its LDS bytes
     TailArgs<T>            *d_tail     = nullptr;
     int                     nu1 = 2, nu2 = 2, nu_bottom = 64;
 
@@ -652,25 +709,45 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
     }
     if (!keep)
     {
-        // coarse levels that one workgroup runs in a single launch
-        const int nl = (int)mg.lev.size();
-        int       l0 = nl;
-        while (l0 > 0 && (long long)mg.lev[l0 - 1].nx * mg.lev[l0 - 1].ny <= TAIL_MAX_CELLS
-               && nl - (l0 - 1) <= TAIL_MAX_LEVELS)
-            --l0;
+        // the coarsest levels whose arrays fit in LDS run in one launch
+        const int nl   = (int)mg.lev.size();
+        int       l0   = nl;
+        long long used = 0;
+        auto      need = [&](const MGLevel<T> &L) {
+            const long long n = (long long)L.nx * L.ny;
+            return (n + (L.nx + 1) * (long long)L.ny + L.nx * (long long)(L.ny + 1)
+                    + (L.cX ? 2 * n : 0) + 3 * n)
+                   * (long long)sizeof(T);
+        };
+        while (l0 > 0 && nl - (l0 - 1) <= TAIL_MAX_LEVELS
+               && used + need(mg.lev[l0 - 1]) <= TAIL_LDS_BYTES)
+            used += need(mg.lev[--l0]);
         if (l0 < nl)
         {
             TailArgs<T> ta{};
+            int         off = 0;
             for (int l = l0; l < nl; ++l)
             {
-                const MGLevel<T> &L = mg.lev[l];
-                ta.lev[l - l0] = TailLevel<T>{ L.a,  L.bX, L.bY, L.cX, L.cY, L.x,
-                                               L.rhs, L.res, L.nx, L.ny, L.hx, L.hy,
-                                               L.dxinv, L.dyinv };
+                const MGLevel<T> &L  = mg.lev[l];
+                const int         n  = L.nx * L.ny;
+                TailLevel<T>     &tl = ta.lev[l - l0];
+                tl.a = L.a; tl.bX = L.bX; tl.bY = L.bY; tl.cX = L.cX; tl.cY = L.cY;
+                tl.x = L.x; tl.rhs = L.rhs;
+                tl.nx = L.nx; tl.ny = L.ny; tl.hx = L.hx; tl.hy = L.hy;
+                tl.dxinv = L.dxinv; tl.dyinv = L.dyinv;
+                tl.o_a   = off; off += n;
+                tl.o_bX  = off; off += (L.nx + 1) * L.ny;
+                tl.o_bY  = off; off += L.nx * (L.ny + 1);
+                tl.o_cX  = off; off += L.cX ? n : 0;
+                tl.o_cY  = off; off += L.cY ? n : 0;
+                tl.o_x   = off; off += n;
+                tl.o_rhs = off; off += n;
+                tl.o_res = off; off += n;
             }
             H_CHECK(hipMalloc((void **)&mg.d_tail, sizeof(TailArgs<T>)));
             H_CHECK(hipMemcpy(mg.d_tail, &ta, sizeof(ta), hipMemcpyHostToDevice));
-            mg.tail_l0 = l0;
+            mg.tail_l0  = l0;
+            mg.tail_lds = (int)used;
         }
     }
     // level 0 coefficients from the FP64 fine arrays
@@ -732,13 +809,11 @@ template <class T>
 static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s)
 {
     MGLevel<T> &L = mg.lev[l];
-    if (mg.tail_l0 >= 0 && l >= mg.tail_l0)
+    if (mg.tail_l0 >= 0 && l == mg.tail_l0)
     {
-        // the table starts at level tail_l0; shift it to start at level l
-        const TailArgs<T> *tab = reinterpret_cast<const TailArgs<T> *>(
-            reinterpret_cast<const TailLevel<T> *>(mg.d_tail) + (l - mg.tail_l0));
-        hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(1024), 0, s, tab,
-                           (int)mg.lev.size() - l, mg.nu1, mg.nu2, mg.nu_bottom, mg.d_sc);
+        hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(1024), mg.tail_lds, s,
+                           (const TailArgs<T> *)mg.d_tail, (int)mg.lev.size() - l, mg.nu1,
+                           mg.nu2, mg.nu_bottom, mg.d_sc);
         return;
     }
     if (l + 1 == (int)mg.lev.size())

@@ -339,7 +339,7 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
         st[1 + dir] = speed_interm;
         const double scale_factor
             = density * (speed_K - u_K) / (speed_K - speed_interm);
-        st[1 + D] = div_by_recip(energy, density, left ? yL : yR)
+        st[1 + D] = energy / density
                     + (speed_interm - u_K)
                           * (mul_eps<E1>(speed_interm, epsilon)
                              + pressure / (density * (speed_K - u_K)));

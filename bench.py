@@ -153,8 +153,10 @@ def main():
     dom = max(range(2), key=lambda d: kern_ms[d])
     algo_bytes = ncells * 2 * 4 * 8  # read + write NC doubles per valid cell
     achieved = algo_bytes / (kern_ms[dom] * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic("k_sweep_x" if dom == 0 else "k_sweep_t")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": traffic_src,
                 "kernel": "k_sweep_%s" % ("x" if dom == 0 else "t (y)"),
                 "kernel_ms": round(kern_ms[dom], 4),
                 "sweep_ms": [round(x, 4) for x in kern_ms],
@@ -190,6 +192,22 @@ def main():
         "imex": imex,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of a sweep kernel on this workload, as measured
+    by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (calibrated against a copy
+    kernel) and recorded in the profile summary; None when not recorded."""
+    path = os.path.join(ROOT, "profiles", "sweep_quadrant4096.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                fields = dict(f.split("=") for f in line.split(":", 1)[1].split())
+                if kernel in fields:
+                    return int(fields[kernel]), "profiles/sweep_quadrant4096.md"
+    except OSError:
+        pass
+    return None, None
 
 
 def _time_sweeps(lib, lev, dt, dx, stream, reps):

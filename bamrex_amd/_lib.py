@@ -114,6 +114,7 @@ _SIGS = {
     "bx_helm_precond": (ctypes.c_int, [P, P, P, P]),
     "bx_helm_attach_cnn": (ctypes.c_int, [P, P, ctypes.c_double, ctypes.c_int]),
     "bx_helm_set_graph": (ctypes.c_int, [P, ctypes.c_int]),
+    "bx_helm_set_fused": (ctypes.c_int, [P, ctypes.c_int]),
     "bx_helm_solve": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, P, P]),
     "bx_helm_last_error": (ctypes.c_char_p, []),
     "bx_cnn_create": (ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(P)]),
@@ -131,7 +132,7 @@ _SIGS = {
     "bx_cnn_last_error": (ctypes.c_char_p, []),
 }
 
-BX_SOLVE = {"mg": 0, "fgmres_mg": 1, "fgmres_cnn": 2}
+BX_SOLVE = {"mg": 0, "fgmres_mg": 1, "fgmres_cnn": 2, "mg_fused": 3}
 BX_CNN_MODE = {"smoother": 0, "corrector": 1}
 BX_CNN_NPARAMS = 4896
 

@@ -69,6 +69,19 @@ __device__ __forceinline__ T apply_at(const T *__restrict__ x, const T *__restri
 
 // red/black Gauss-Seidel update of cell (i, j) (alapgrad_gsrb; periodic
 // level: no boundary correction, delta = 0)
+// the cell update of red/black Gauss-Seidel from its neighbours' values
+template <class T>
+__device__ __forceinline__ T gsrb_value(T rhs_c, T a_c, T bxl, T bxr, T byl, T byr, T cx,
+                                        T cy, bool with_c, T pl, T pr, T pd, T pu, T phi_c,
+                                        T alpha, T dhx, T dhy, T cdhx, T cdhy)
+{
+    const T gamma = alpha * a_c + dhx * (bxl + bxr) + dhy * (byl + byr);
+    T rho = dhx * (bxl * pl + bxr * pr) + dhy * (byl * pd + byr * pu);
+    if (with_c)
+        rho = rho - cdhx * cx * (pr - pl) - cdhy * cy * (pu - pd);
+    return (rhs_c + rho - phi_c * T(0)) / (gamma - T(0));
+}
+
 template <class T>
 __device__ __forceinline__ void gsrb_at(T *__restrict__ phi, const T *__restrict__ rhs,
                                         const T *__restrict__ a, const T *__restrict__ bX,
@@ -85,11 +98,9 @@ __device__ __forceinline__ void gsrb_at(T *__restrict__ phi, const T *__restrict
     const T byr = bY[(long long)(j + 1) * nx + i];
     const T pl = phi[(long long)j * nx + im], pr = phi[(long long)j * nx + ip];
     const T pd = phi[(long long)jm * nx + i], pu = phi[(long long)jp * nx + i];
-    const T gamma = alpha * a[c] + dhx * (bxl + bxr) + dhy * (byl + byr);
-    T rho = dhx * (bxl * pl + bxr * pr) + dhy * (byl * pd + byr * pu);
-    if (cX)
-        rho = rho - cdhx * cX[c] * (pr - pl) - cdhy * cY[c] * (pu - pd);
-    phi[c] = (rhs[c] + rho - phi[c] * T(0)) / (gamma - T(0));
+    phi[c] = gsrb_value(rhs[c], a[c], bxl, bxr, byl, byr, cX ? cX[c] : T(0),
+                        cY ? cY[c] : T(0), cX != nullptr, pl, pr, pd, pu, phi[c], alpha, dhx,
+                        dhy, cdhx, cdhy);
 }
 
 template <class T>
@@ -160,6 +171,102 @@ __global__ __launch_bounds__(256) void k_prolong_add(const T *__restrict__ c,
     if (i >= nfx)
         return;
     f[(long long)j * nfx + i] += c[(long long)(j >> 1) * (nfx >> 1) + (i >> 1)];
+}
+
+// Red/black smoothing of a whole level in one launch (temporal blocking):
+// a workgroup loads phi on its TX x TY tile plus a halo of NH = 2 * pairs
+// cells into LDS and runs the NH half-sweeps there, updating at half-sweep k
+// the cells at distance > k from the loaded region's edge, whose neighbours
+// are still exact; the tile interior is then written back.  Each cell value
+// is computed by the same gsrb_value as the one-colour launches, from the
+// same neighbour values, so the result is identical to 2*pairs launches of
+// k_gsrb.  With `coarse` the prolongation phi += P e_c is applied at load
+// (the post-smoothing of a V-cycle).  Coefficients are read from global
+// memory (read-only, cache-resident across the half-sweeps).
+constexpr int SM_TX = 64, SM_TY = 16, SM_MAXH = 8;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_gsrb_tiled(
+    T *__restrict__ phi, const T *__restrict__ rhs, const T *__restrict__ a,
+    const T *__restrict__ bX, const T *__restrict__ bY, const T *__restrict__ cX,
+    const T *__restrict__ cY, int nx, int ny, const T *__restrict__ sc, T hx, T hy,
+    int pairs, const T *__restrict__ coarse)
+{
+    __shared__ T ph[(SM_TY + 2 * SM_MAXH) * (SM_TX + 2 * SM_MAXH)];
+    const int NH = 2 * pairs;
+    const int RW = SM_TX + 2 * NH, RH = SM_TY + 2 * NH;
+    const int x0 = blockIdx.x * SM_TX - NH, y0 = blockIdx.y * SM_TY - NH;
+    const T alpha = sc[0];
+    const T dhx = sc[1] / (hx * hx), dhy = sc[1] / (hy * hy);
+    const T cdhx = sc[2] / (2 * hx), cdhy = sc[2] / (2 * hy);
+    auto wrapx = [&](int g) { g %= nx; return g < 0 ? g + nx : g; };
+    auto wrapy = [&](int g) { g %= ny; return g < 0 ? g + ny : g; };
+    for (int c = threadIdx.x; c < RW * RH; c += blockDim.x)
+    {
+        const int lj = c / RW, li = c - lj * RW;
+        const int gi = wrapx(x0 + li), gj = wrapy(y0 + lj);
+        T v = phi[(long long)gj * nx + gi];
+        if (coarse)
+            v += coarse[(long long)(gj >> 1) * (nx >> 1) + (gi >> 1)];
+        ph[c] = v;
+    }
+    __syncthreads();
+    for (int k = 0; k < NH; ++k)
+    {
+        const int rb = k & 1;
+        for (int c = threadIdx.x; c < RW * RH; c += blockDim.x)
+        {
+            const int lj = c / RW, li = c - lj * RW;
+            if (li <= k || lj <= k || li >= RW - 1 - k || lj >= RH - 1 - k)
+                continue;
+            const int gi = wrapx(x0 + li), gj = wrapy(y0 + lj);
+            if ((gi + gj + rb) & 1)
+                continue;
+            const long long g = (long long)gj * nx + gi;
+            ph[c] = gsrb_value(rhs[g], a[g], bX[(long long)gj * (nx + 1) + gi],
+                               bX[(long long)gj * (nx + 1) + gi + 1],
+                               bY[(long long)gj * nx + gi], bY[(long long)(gj + 1) * nx + gi],
+                               cX ? cX[g] : T(0), cY ? cY[g] : T(0), cX != nullptr, ph[c - 1],
+                               ph[c + 1], ph[c - RW], ph[c + RW], ph[c], alpha, dhx, dhy, cdhx,
+                               cdhy);
+        }
+        __syncthreads();
+    }
+    for (int c = threadIdx.x; c < SM_TX * SM_TY; c += blockDim.x)
+    {
+        const int tj = c / SM_TX, ti = c - tj * SM_TX;
+        const int gi = blockIdx.x * SM_TX + ti, gj = blockIdx.y * SM_TY + tj;
+        if (gi < nx && gj < ny)
+            phi[(long long)gj * nx + gi] = ph[(tj + NH) * RW + ti + NH];
+    }
+}
+
+// coarse rhs = R (rhs - A x) on the fine level, and coarse x = 0, in one
+// launch (the residual values and the averaging order of k_apply +
+// k_restrict)
+template <class T>
+__global__ __launch_bounds__(256) void k_resid_restrict(
+    const T *__restrict__ x, const T *__restrict__ rhs, const T *__restrict__ a,
+    const T *__restrict__ bX, const T *__restrict__ bY, const T *__restrict__ cX,
+    const T *__restrict__ cY, int nx, int ny, const T *__restrict__ sc, T dxinv, T dyinv,
+    T *__restrict__ crhs, T *__restrict__ cx)
+{
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y;
+    const int ncx = nx >> 1;
+    if (I >= ncx)
+        return;
+    const T alpha = sc[0], beta = sc[1], gamma = sc[2];
+    T r[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        const int i = 2 * I + (q & 1), j = 2 * J + (q >> 1);
+        r[q] = rhs[(long long)j * nx + i]
+               - apply_at(x, a, bX, bY, cX, cY, nx, ny, alpha, beta, gamma, dxinv, dyinv, i, j);
+    }
+    crhs[(long long)J * ncx + I] = T(0.25) * (r[0] + r[1] + r[2] + r[3]);
+    cx[(long long)J * ncx + I]   = T(0);
 }
 
 // The coarse end of a V-cycle (the coarsest levels, as many as fit in
@@ -628,6 +735,7 @@ struct bx_helm
     // output pout; recaptured when anything it bakes in changes
     // (slot 0: plain V-cycle, slot 1: with the network)
     int             use_graph  = 1;
+    int             fused      = 1; // blocked-smoother V-cycle in the FGMRES preconditioner
     hipStream_t     cap_stream = nullptr;
     hipGraphExec_t  pgraph[2]  = { nullptr, nullptr };
     long long       gen = 0, pgraph_gen[2] = { -1, -1 }, pgraph_epoch[2] = { -1, -1 };
@@ -806,7 +914,7 @@ static void launch_smooth(const MGLevel<T> &L, const MGHierarchy<T> &mg,
 }
 
 template <class T>
-static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s)
+static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
 {
     MGLevel<T> &L = mg.lev[l];
     if (mg.tail_l0 >= 0 && l == mg.tail_l0)
@@ -833,18 +941,38 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s)
         }
         return;
     }
+    MGLevel<T> &C = mg.lev[l + 1];
+    if (fused && mg.nu2 > 0 && 2 * mg.nu1 <= SM_MAXH && 2 * mg.nu2 <= SM_MAXH
+        && L.nx % 2 == 0 && L.ny % 2 == 0)
+    {
+        // three launches per level: blocked pre-smoothing, residual +
+        // restriction + zeroing, prolongation + blocked post-smoothing
+        const dim3 tg((unsigned)((L.nx + SM_TX - 1) / SM_TX),
+                      (unsigned)((L.ny + SM_TY - 1) / SM_TY));
+        if (mg.nu1 > 0)
+            hipLaunchKernelGGL((k_gsrb_tiled<T>), tg, dim3(256), 0, s, L.x, L.rhs, L.a,
+                               L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.hx, L.hy,
+                               mg.nu1, (const T *)nullptr);
+        hipLaunchKernelGGL((k_resid_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s, L.x,
+                           L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.dxinv,
+                           L.dyinv, C.rhs, C.x);
+        vcycle(mg, l + 1, s, fused);
+        hipLaunchKernelGGL((k_gsrb_tiled<T>), tg, dim3(256), 0, s, L.x, L.rhs, L.a, L.bX,
+                           L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.hx, L.hy, mg.nu2,
+                           (const T *)C.x);
+        return;
+    }
     for (int it = 0; it < mg.nu1; ++it)
     {
         launch_smooth(L, mg, L.x, L.rhs, 0, s);
         launch_smooth(L, mg, L.x, L.rhs, 1, s);
     }
     launch_apply(L, mg, L.x, L.res, L.rhs, s);
-    MGLevel<T> &C = mg.lev[l + 1];
     hipLaunchKernelGGL((k_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s,
                        L.res, C.rhs, C.nx, C.ny);
     hipLaunchKernelGGL((k_fill<T>), dim3(256), dim3(256), 0, s, C.x,
                        (long long)C.nx * C.ny, (T)0);
-    vcycle(mg, l + 1, s);
+    vcycle(mg, l + 1, s, fused);
     hipLaunchKernelGGL((k_prolong_add<T>), grid2(L.nx, L.ny), dim3(256), 0, s,
                        C.x, L.x, L.nx, L.ny);
     for (int it = 0; it < mg.nu2; ++it)
@@ -856,7 +984,7 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s)
 
 // coarse-grid part of the V-cycle only (levels >= 1), from L.res on level 0
 template <class T>
-static void coarse_correction(MGHierarchy<T> &mg, hipStream_t s)
+static void coarse_correction(MGHierarchy<T> &mg, hipStream_t s, bool fused)
 {
     MGLevel<T> &L = mg.lev[0];
     MGLevel<T> &C = mg.lev[1];
@@ -864,7 +992,7 @@ static void coarse_correction(MGHierarchy<T> &mg, hipStream_t s)
                        L.res, C.rhs, C.nx, C.ny);
     hipLaunchKernelGGL((k_fill<T>), dim3(256), dim3(256), 0, s, C.x,
                        (long long)C.nx * C.ny, (T)0);
-    vcycle(mg, 1, s);
+    vcycle(mg, 1, s, fused);
     hipLaunchKernelGGL((k_prolong_add<T>), grid2(L.nx, L.ny), dim3(256), 0, s,
                        C.x, L.x, L.nx, L.ny);
 }
@@ -1012,6 +1140,15 @@ extern "C" int bx_helm_set_options(bx_helm *h, int nu1, int nu2, int nu_bottom,
     return BX_OK;
 }
 
+extern "C" int bx_helm_set_fused(bx_helm *h, int fused)
+{
+    if (!h)
+        return herr(BX_ERR_INVALID, "bx_helm_set_fused: null solver");
+    h->fused = fused ? 1 : 0;
+    ++h->gen;
+    return BX_OK;
+}
+
 extern "C" int bx_helm_set_graph(bx_helm *h, int use_graph)
 {
     if (!h)
@@ -1095,7 +1232,7 @@ static int precond(bx_helm *h, const double *rv, double *e, hipStream_t s)
                                s, h->tmp1, mg.lev[0].res, n, 1.0);
             hipLaunchKernelGGL((k_fill<float>), dim3(2048), dim3(256), 0, s,
                                mg.lev[0].x, n, 0.f);
-            coarse_correction(mg, s);
+            coarse_correction(mg, s, h->fused);
             hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0,
                                s, mg.lev[0].x, h->tmp2, n, 1.0f);
         }
@@ -1106,7 +1243,7 @@ static int precond(bx_helm *h, const double *rv, double *e, hipStream_t s)
                                mg.lev[0].res, h->tmp1, n, 1.0);
             hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s,
                                mg.lev[0].x, n, 0.0);
-            coarse_correction(mg, s);
+            coarse_correction(mg, s, h->fused);
             hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, h->tmp2,
                                mg.lev[0].x, n, 1.0);
         }
@@ -1136,7 +1273,7 @@ static int precond_vcycle(bx_helm *h, const double *rv, double *e, hipStream_t s
                            rv, mg.lev[0].rhs, n, 1.0);
         hipLaunchKernelGGL((k_fill<float>), dim3(2048), dim3(256), 0, s,
                            mg.lev[0].x, n, 0.f);
-        vcycle(mg, 0, s);
+        vcycle(mg, 0, s, h->fused != 0);
         hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0, s,
                            mg.lev[0].x, e, n, 1.0f);
     }
@@ -1147,7 +1284,7 @@ static int precond_vcycle(bx_helm *h, const double *rv, double *e, hipStream_t s
                            rv, n, 1.0);
         hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s,
                            mg.lev[0].x, n, 0.0);
-        vcycle(mg, 0, s);
+        vcycle(mg, 0, s, h->fused != 0);
         hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, e,
                            mg.lev[0].x, n, 1.0);
     }
@@ -1231,7 +1368,7 @@ static int precond_replay(bx_helm *h, const double *rv, double *e, hipStream_t s
 // until ||b - A x||_2 <= rtol ||b||_2.
 static int solve_mg(bx_helm *h, const double *b, double *x, double rtol,
                     int maxiter, double bnorm, bx_solve_stats *st,
-                    hipStream_t s)
+                    hipStream_t s, bool fused = false)
 {
     MGHierarchy<double> &mg = h->mg64;
     MGLevel<double>     &L0 = mg.lev[0];
@@ -1248,7 +1385,7 @@ static int solve_mg(bx_helm *h, const double *b, double *x, double rtol,
                            1.0);
         hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s, L0.x, n,
                            0.0);
-        vcycle(mg, 0, s);
+        vcycle(mg, 0, s, fused);
         hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, s, x, L0.x, n, 1.0);
         residual64(h, b, x, h->r, s);
         if ((rc = norm2(h, h->r, s, &rn)))
@@ -1422,7 +1559,8 @@ extern "C" int bx_helm_solve(bx_helm *h, const double *b, double *x, int method,
                              double rtol, int maxiter, bx_solve_stats *st,
                              void *stream)
 {
-    if (!h || !h->coeffs_set || !b || !x || !st || !(rtol > 0) || maxiter < 1)
+    if (!h || !h->coeffs_set || !b || !x || !st || !(rtol > 0) || maxiter < 1
+        || method < BX_SOLVE_MG || method > BX_SOLVE_MG_FUSED)
         return herr(BX_ERR_INVALID, "bx_helm_solve: bad argument");
     if (method == BX_SOLVE_FGMRES_CNN && !h->cnn)
         return herr(BX_ERR_STATE, "bx_helm_solve: no network attached");
@@ -1441,9 +1579,10 @@ extern "C" int bx_helm_solve(bx_helm *h, const double *b, double *x, int method,
         H_CHECK(hipStreamSynchronize(s));
         return BX_OK;
     }
-    if (method == BX_SOLVE_MG)
+    if (method == BX_SOLVE_MG || method == BX_SOLVE_MG_FUSED)
     {
-        if ((rc = solve_mg(h, b, x, rtol, maxiter, bnorm, st, s)))
+        if ((rc = solve_mg(h, b, x, rtol, maxiter, bnorm, st, s,
+                           method == BX_SOLVE_MG_FUSED)))
             return rc;
     }
     else

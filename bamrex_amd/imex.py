@@ -252,6 +252,9 @@ class HelmholtzSolver:
     def set_graph(self, use_graph):
         _chk(lib().bx_helm_set_graph(self.h, int(bool(use_graph))), "bx_helm_set_graph")
 
+    def set_fused(self, fused):
+        _chk(lib().bx_helm_set_fused(self.h, int(bool(fused))), "bx_helm_set_fused")
+
     def solve(self, b_dev, x_dev, method="mg", rtol=1e-10, maxiter=200):
         st = _lib.bx_solve_stats()
         _chk(lib().bx_helm_solve(self.h, b_dev.ptr, x_dev.ptr, _lib.BX_SOLVE[method], rtol, maxiter,

@@ -229,7 +229,8 @@ typedef struct bx_cnn  bx_cnn;
 /* bx_helm_solve methods */
 #define BX_SOLVE_MG 0         /* MLMG::solve: V-cycles until converged (FP64) */
 #define BX_SOLVE_FGMRES_MG 1  /* FGMRES, V-cycle preconditioner */
-#define BX_SOLVE_FGMRES_CNN 2 /* FGMRES, V-cycle with learned fine smoothing */
+#define BX_SOLVE_FGMRES_CNN 2 /* FGMRES, V-cycle with the attached network */
+#define BX_SOLVE_MG_FUSED 3   /* as BX_SOLVE_MG with the blocked-smoother V-cycle */
 
 typedef struct
 {
@@ -281,6 +282,11 @@ int bx_helm_attach_cnn(bx_helm *h, bx_cnn *cnn, double omega, int mode);
 /* FGMRES replays its preconditioner as a captured hipGraph (default 1);
  * 0 launches every kernel of it directly. */
 int bx_helm_set_graph(bx_helm *h, int use_graph);
+/* The preconditioner's V-cycle smooths each level in one launch (red/black
+ * half-sweeps blocked in LDS, residual and restriction fused; default 1) or
+ * with one launch per colour (0), the structure of the plain solve.  Same
+ * results bit for bit.  No reference counterpart. */
+int bx_helm_set_fused(bx_helm *h, int fused);
 /* MLMG::solve(phi, rhs, rtol, 0) (IMEX_O1.cpp:602-614) with the chosen method;
  * x holds the initial guess.  FGMRES that stalls (no 2x residual drop in 8
  * iterations) or hits maxiter hands its iterate to plain multigrid

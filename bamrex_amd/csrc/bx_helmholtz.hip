@@ -16,7 +16,10 @@
 // MLMG defaults restated (AMReX is not available here): V-cycle, 2 pre- and
 // 2 post-smoothing red-black sweeps, cell-average restriction, piecewise
 // constant interpolation, coarsening by 2 while both extents stay even and
-// >= 8; the bottom level is smoothed to convergence with red-black sweeps.
+// >= 8; the bottom level gets nu_bottom (default 16) red-black sweep pairs
+// (AMReX's GPU default bottom solver is BiCGStab; on these mass-term
+// operators the coarsest level is diagonally dominant and the sweeps
+// converge it).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -321,6 +324,27 @@ __device__ void tail_smooth(const LdsLevel<T> &L, T alpha, T beta, T gam, int pa
     const T dhx = beta / (L.hx * L.hx), dhy = beta / (L.hy * L.hy);
     const T cdhx = gam / (2 * L.hx), cdhy = gam / (2 * L.hy);
     const int n = L.nx * L.ny;
+    if (n <= 64)
+    {
+        // one wave owns the level: colours are separated by wave-level
+        // ordering of the LDS accesses instead of workgroup barriers
+        if (threadIdx.x < 64)
+        {
+            const int c = threadIdx.x;
+            const int j = c / L.nx, i = c - j * L.nx;
+            for (int it = 0; it < pairs; ++it)
+                for (int rb = 0; rb < 2; ++rb)
+                {
+                    if (c < n && !((i + j + rb) & 1))
+                        gsrb_at(L.x, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, alpha,
+                                dhx, dhy, cdhx, cdhy, i, j);
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                }
+        }
+        __syncthreads();
+        return;
+    }
     for (int it = 0; it < pairs; ++it)
         for (int rb = 0; rb < 2; ++rb)
         {
@@ -345,7 +369,7 @@ __device__ void copy_in(T *dst, const T *__restrict__ src, int n)
 
 // A: level table in device memory (the levels from the first tail level)
 template <class T>
-__global__ __launch_bounds__(1024) void k_vcycle_tail(const TailArgs<T> *__restrict__ A,
+__global__ __launch_bounds__(256) void k_vcycle_tail(const TailArgs<T> *__restrict__ A,
                                                       int nlev, int nu1, int nu2,
                                                       int nu_bottom,
                                                       const T *__restrict__ sc)
@@ -683,7 +707,7 @@ template <class T> struct MGHierarchy
     int                     tail_l0    = -1;      // first level run by k_vcycle_tail
     int                     tail_lds   = 0;       // its LDS bytes
     TailArgs<T>            *d_tail     = nullptr;
-    int                     nu1 = 2, nu2 = 2, nu_bottom = 64;
+    int                     nu1 = 2, nu2 = 2, nu_bottom = 16;
 
     void free_all()
     {
@@ -919,7 +943,7 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
     MGLevel<T> &L = mg.lev[l];
     if (mg.tail_l0 >= 0 && l == mg.tail_l0)
     {
-        hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(1024), mg.tail_lds, s,
+        hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(256), mg.tail_lds, s,
                            (const TailArgs<T> *)mg.d_tail, (int)mg.lev.size() - l, mg.nu1,
                            mg.nu2, mg.nu_bottom, mg.d_sc);
         return;

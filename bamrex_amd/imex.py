@@ -222,7 +222,7 @@ class HelmholtzSolver:
         if self.cnn is not None:
             self.cnn.set_operator(self)
 
-    def set_options(self, nu1=2, nu2=2, nu_bottom=64, precond_fp32=1, restart=20):
+    def set_options(self, nu1=2, nu2=2, nu_bottom=16, precond_fp32=1, restart=20):
         _chk(lib().bx_helm_set_options(self.h, nu1, nu2, nu_bottom, precond_fp32, restart),
              "bx_helm_set_options")
 

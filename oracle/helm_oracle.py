@@ -88,7 +88,7 @@ def avg_faces(fX, fY):
     return cX, cY
 
 
-def vcycle(r, a, bX, bY, alpha, beta, dx, dy, nu1=2, nu2=2, nu_bottom=64):
+def vcycle(r, a, bX, bY, alpha, beta, dx, dy, nu1=2, nu2=2, nu_bottom=16):
     r, a, bX, bY = map(_c, (r, a, bX, bY))
     ny, nx = r.shape
     e = np.empty_like(r)
@@ -98,7 +98,7 @@ def vcycle(r, a, bX, bY, alpha, beta, dx, dy, nu1=2, nu2=2, nu_bottom=64):
 
 
 def mg_solve(b, a, bX, bY, alpha, beta, dx, dy, x0=None, rtol=1e-10, maxiter=100, nu1=2, nu2=2,
-             nu_bottom=64):
+             nu_bottom=16):
     b, a, bX, bY = map(_c, (b, a, bX, bY))
     ny, nx = b.shape
     x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64, copy=True)

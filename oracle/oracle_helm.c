@@ -18,7 +18,8 @@
  *                    reference tree): nu1 pre- and nu2 post- red/black sweeps,
  *                    residual restriction by cell averaging, piecewise-constant
  *                    prolongation, coarsening while both extents stay even and
- *                    >= 8 after halving, bottom = nu_bottom red/black sweeps.
+ *                    >= 8 after halving, bottom = nu_bottom red/black sweep
+ *                    pairs (16 by default).
  *                    Parity of these MLMG choices with AMReX itself is
  *                    unpinned (no AMReX build here); the pieces they use are
  *                    pinned by the tests (closed forms, symmetry, Fourier

@@ -95,9 +95,10 @@ __global__ __launch_bounds__(256) void k_conv1(const unsigned short *__restrict_
     const long long p   = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npx)
         return;
-    const int img = (int)(p / ((long long)g.w * g.h));
-    const int rem = (int)(p - (long long)img * g.w * g.h);
-    const int y = rem / g.w, x = rem - y * g.w;
+    const unsigned wh_ = (unsigned)g.w * (unsigned)g.h;
+    const int img = (int)((unsigned)p / wh_);
+    const int rem = (int)((unsigned)p - (unsigned)img * wh_);
+    const int y = (int)((unsigned)rem / (unsigned)g.w), x = rem - y * g.w;
     float     v[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t)
@@ -136,9 +137,10 @@ __global__ __launch_bounds__(256) void k_conv16(const unsigned short *__restrict
     const long long p0    = tile * 16;
     if (p0 >= npx) // whole wave exits together (npx is a multiple of 16)
         return;
-    const int img = (int)(p0 / ((long long)g.w * g.h));
-    const int rem = (int)(p0 - (long long)img * g.w * g.h);
-    const int y = rem / g.w, x0 = rem - y * g.w;
+    const unsigned wh_ = (unsigned)g.w * (unsigned)g.h;
+    const int img = (int)((unsigned)p0 / wh_);
+    const int rem = (int)((unsigned)p0 - (unsigned)img * wh_);
+    const int y = (int)((unsigned)rem / (unsigned)g.w), x0 = rem - y * g.w;
     const int m = lane & 15, kg = lane >> 4;
     v4f       acc = { 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
@@ -171,9 +173,10 @@ __global__ __launch_bounds__(256) void k_conv4(const unsigned short *__restrict_
     const long long p   = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npx)
         return;
-    const int img = (int)(p / ((long long)g.w * g.h));
-    const int rem = (int)(p - (long long)img * g.w * g.h);
-    const int y = rem / g.w, x = rem - y * g.w;
+    const unsigned wh_ = (unsigned)g.w * (unsigned)g.h;
+    const int img = (int)((unsigned)p / wh_);
+    const int rem = (int)((unsigned)p - (unsigned)img * wh_);
+    const int y = (int)((unsigned)rem / (unsigned)g.w), x = rem - y * g.w;
     float     s = 0.f;
 #pragma unroll
     for (int t = 0; t < 9; ++t)
@@ -462,9 +465,10 @@ __global__ __launch_bounds__(256) void k_bwd4_data(const float *__restrict__ gy,
     const long long p   = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npx)
         return;
-    const int img = (int)(p / ((long long)g.w * g.h));
-    const int rem = (int)(p - (long long)img * g.w * g.h);
-    const int y = rem / g.w, x = rem - y * g.w;
+    const unsigned wh_ = (unsigned)g.w * (unsigned)g.h;
+    const int img = (int)((unsigned)p / wh_);
+    const int rem = (int)((unsigned)p - (unsigned)img * wh_);
+    const int y = (int)((unsigned)rem / (unsigned)g.w), x = rem - y * g.w;
     float     acc[C];
 #pragma unroll
     for (int k = 0; k < C; ++k)
@@ -496,9 +500,10 @@ __global__ __launch_bounds__(256) void k_bwd16_data(const float *__restrict__ go
     const long long p   = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npx)
         return;
-    const int img = (int)(p / ((long long)g.w * g.h));
-    const int rem = (int)(p - (long long)img * g.w * g.h);
-    const int y = rem / g.w, x = rem - y * g.w;
+    const unsigned wh_ = (unsigned)g.w * (unsigned)g.h;
+    const int img = (int)((unsigned)p / wh_);
+    const int rem = (int)((unsigned)p - (unsigned)img * wh_);
+    const int y = (int)((unsigned)rem / (unsigned)g.w), x = rem - y * g.w;
     float     acc[C];
 #pragma unroll
     for (int k = 0; k < C; ++k)
@@ -540,9 +545,10 @@ __global__ __launch_bounds__(256) void k_bwd16_weight(const float *__restrict__ 
         const float gv = gout[p * C + n];
         if (gv == 0.f)
             continue;
-        const int img = (int)(p / ((long long)g.w * g.h));
-        const int rem = (int)(p - (long long)img * g.w * g.h);
-        const int y = rem / g.w, x = rem - y * g.w;
+        const unsigned wh_ = (unsigned)g.w * (unsigned)g.h;
+        const int img = (int)((unsigned)p / wh_);
+        const int rem = (int)((unsigned)p - (unsigned)img * wh_);
+        const int y = (int)((unsigned)rem / (unsigned)g.w), x = rem - y * g.w;
 #pragma unroll
         for (int t = 0; t < 9; ++t)
         {
@@ -578,9 +584,10 @@ __global__ __launch_bounds__(320) void k_bwd_weight_small(const float *__restric
     const long long pbeg = (long long)blockIdx.x * CH;
     for (long long p = pbeg; p < pbeg + CH && p < npx; ++p)
     {
-        const int img = (int)(p / ((long long)g.w * g.h));
-        const int rem = (int)(p - (long long)img * g.w * g.h);
-        const int y = rem / g.w, xx = rem - y * g.w;
+        const unsigned wh_ = (unsigned)g.w * (unsigned)g.h;
+        const int img = (int)((unsigned)p / wh_);
+        const int rem = (int)((unsigned)p - (unsigned)img * wh_);
+        const int y = (int)((unsigned)rem / (unsigned)g.w), xx = rem - y * g.w;
         const int q = nbr<false>(xx, y, t % 3 - 1, t / 3 - 1, g, img);
         if (q < 0)
             continue;
@@ -782,6 +789,8 @@ extern "C" int bx_cnn_get_params(bx_cnn *c, float *p, int n)
 
 static int ensure_infer(bx_cnn *c, long long npx)
 {
+    if (npx > 0x7fffffffLL)
+        return cerr(BX_ERR_INVALID, "network: more than 2^31 pixels in one launch");
     if (npx <= c->cap)
         return BX_OK;
     for (void *p : { (void *)c->x, (void *)c->h1, (void *)c->h2, (void *)c->h3,

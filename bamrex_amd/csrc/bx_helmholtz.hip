@@ -195,6 +195,9 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
     const T *__restrict__ cY, int nx, int ny, const T *__restrict__ sc, T hx, T hy,
     int pairs, const T *__restrict__ coarse)
 {
+    // each thread owns the region cells tid + 256 m; their coefficients stay
+    // in registers across the half-sweeps, phi lives in LDS
+    constexpr int MAXC = ((SM_TY + 2 * SM_MAXH) * (SM_TX + 2 * SM_MAXH) + 255) / 256;
     __shared__ T ph[(SM_TY + 2 * SM_MAXH) * (SM_TX + 2 * SM_MAXH)];
     const int NH = 2 * pairs;
     const int RW = SM_TX + 2 * NH, RH = SM_TY + 2 * NH;
@@ -204,34 +207,52 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
     const T cdhx = sc[2] / (2 * hx), cdhy = sc[2] / (2 * hy);
     auto wrapx = [&](int g) { g %= nx; return g < 0 ? g + nx : g; };
     auto wrapy = [&](int g) { g %= ny; return g < 0 ? g + ny : g; };
-    for (int c = threadIdx.x; c < RW * RH; c += blockDim.x)
+    T   r_rhs[MAXC], r_a[MAXC], r_bxl[MAXC], r_bxr[MAXC], r_byl[MAXC], r_byr[MAXC];
+    T   r_cx[MAXC], r_cy[MAXC];
+    int r_dist[MAXC], r_par[MAXC];
+#pragma unroll
+    for (int m = 0; m < MAXC; ++m)
     {
+        const int c = threadIdx.x + 256 * m;
+        r_dist[m]   = -1;
+        if (c >= RW * RH)
+            continue;
         const int lj = c / RW, li = c - lj * RW;
         const int gi = wrapx(x0 + li), gj = wrapy(y0 + lj);
-        T v = phi[(long long)gj * nx + gi];
+        const long long g = (long long)gj * nx + gi;
+        T v = phi[g];
         if (coarse)
             v += coarse[(long long)(gj >> 1) * (nx >> 1) + (gi >> 1)];
         ph[c] = v;
+        // distance from the region edge; only interior cells are updated
+        const int d = min(min(li, RW - 1 - li), min(lj, RH - 1 - lj));
+        r_dist[m]   = d;
+        r_par[m]    = (gi + gj) & 1;
+        if (d == 0)
+            continue;
+        r_rhs[m] = rhs[g];
+        r_a[m]   = a[g];
+        r_bxl[m] = bX[(long long)gj * (nx + 1) + gi];
+        r_bxr[m] = bX[(long long)gj * (nx + 1) + gi + 1];
+        r_byl[m] = bY[(long long)gj * nx + gi];
+        r_byr[m] = bY[(long long)(gj + 1) * nx + gi];
+        r_cx[m]  = cX ? cX[g] : T(0);
+        r_cy[m]  = cY ? cY[g] : T(0);
     }
     __syncthreads();
     for (int k = 0; k < NH; ++k)
     {
         const int rb = k & 1;
-        for (int c = threadIdx.x; c < RW * RH; c += blockDim.x)
+#pragma unroll
+        for (int m = 0; m < MAXC; ++m)
         {
-            const int lj = c / RW, li = c - lj * RW;
-            if (li <= k || lj <= k || li >= RW - 1 - k || lj >= RH - 1 - k)
+            // updated at half-sweep k: distance > k and the colour of k
+            if (r_dist[m] <= k || ((r_par[m] + rb) & 1))
                 continue;
-            const int gi = wrapx(x0 + li), gj = wrapy(y0 + lj);
-            if ((gi + gj + rb) & 1)
-                continue;
-            const long long g = (long long)gj * nx + gi;
-            ph[c] = gsrb_value(rhs[g], a[g], bX[(long long)gj * (nx + 1) + gi],
-                               bX[(long long)gj * (nx + 1) + gi + 1],
-                               bY[(long long)gj * nx + gi], bY[(long long)(gj + 1) * nx + gi],
-                               cX ? cX[g] : T(0), cY ? cY[g] : T(0), cX != nullptr, ph[c - 1],
-                               ph[c + 1], ph[c - RW], ph[c + RW], ph[c], alpha, dhx, dhy, cdhx,
-                               cdhy);
+            const int c = threadIdx.x + 256 * m;
+            ph[c] = gsrb_value(r_rhs[m], r_a[m], r_bxl[m], r_bxr[m], r_byl[m], r_byr[m],
+                               r_cx[m], r_cy[m], cX != nullptr, ph[c - 1], ph[c + 1],
+                               ph[c - RW], ph[c + RW], ph[c], alpha, dhx, dhy, cdhx, cdhy);
         }
         __syncthreads();
     }

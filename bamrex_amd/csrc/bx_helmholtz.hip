@@ -183,14 +183,16 @@ __global__ __launch_bounds__(256) void k_prolong_add(const T *__restrict__ c,
 // are still exact; the tile interior is then written back.  Each cell value
 // is computed by the same gsrb_value as the one-colour launches, from the
 // same neighbour values, so the result is identical to 2*pairs launches of
-// k_gsrb.  With `coarse` the prolongation phi += P e_c is applied at load
+// k_gsrb.  Input and output are different arrays: other workgroups of the
+// same launch still read the halo of a tile after it has been written.  With `coarse` the prolongation phi += P e_c is applied at load
 // (the post-smoothing of a V-cycle).  Coefficients are read from global
 // memory (read-only, cache-resident across the half-sweeps).
 constexpr int SM_TX = 64, SM_TY = 16, SM_MAXH = 8;
 
 template <class T>
 __global__ __launch_bounds__(256) void k_gsrb_tiled(
-    T *__restrict__ phi, const T *__restrict__ rhs, const T *__restrict__ a,
+    const T *__restrict__ phi, T *__restrict__ phi_out, const T *__restrict__ rhs,
+    const T *__restrict__ a,
     const T *__restrict__ bX, const T *__restrict__ bY, const T *__restrict__ cX,
     const T *__restrict__ cY, int nx, int ny, const T *__restrict__ sc, T hx, T hy,
     int pairs, const T *__restrict__ coarse)
@@ -261,7 +263,7 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
         const int tj = c / SM_TX, ti = c - tj * SM_TX;
         const int gi = blockIdx.x * SM_TX + ti, gj = blockIdx.y * SM_TY + tj;
         if (gi < nx && gj < ny)
-            phi[(long long)gj * nx + gi] = ph[(tj + NH) * RW + ti + NH];
+            phi_out[(long long)gj * nx + gi] = ph[(tj + NH) * RW + ti + NH];
     }
 }
 
@@ -994,16 +996,22 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
         // restriction + zeroing, prolongation + blocked post-smoothing
         const dim3 tg((unsigned)((L.nx + SM_TX - 1) / SM_TX),
                       (unsigned)((L.ny + SM_TY - 1) / SM_TY));
+        // pre-smoothing x -> res, post-smoothing res + P e_c -> x (the res
+        // array is free in this path: the residual goes straight to C.rhs)
+        const T *xs = L.x;
         if (mg.nu1 > 0)
-            hipLaunchKernelGGL((k_gsrb_tiled<T>), tg, dim3(256), 0, s, L.x, L.rhs, L.a,
-                               L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.hx, L.hy,
+        {
+            hipLaunchKernelGGL((k_gsrb_tiled<T>), tg, dim3(256), 0, s, L.x, L.res, L.rhs,
+                               L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.hx, L.hy,
                                mg.nu1, (const T *)nullptr);
-        hipLaunchKernelGGL((k_resid_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s, L.x,
+            xs = L.res;
+        }
+        hipLaunchKernelGGL((k_resid_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s, xs,
                            L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.dxinv,
                            L.dyinv, C.rhs, C.x);
         vcycle(mg, l + 1, s, fused);
-        hipLaunchKernelGGL((k_gsrb_tiled<T>), tg, dim3(256), 0, s, L.x, L.rhs, L.a, L.bX,
-                           L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.hx, L.hy, mg.nu2,
+        hipLaunchKernelGGL((k_gsrb_tiled<T>), tg, dim3(256), 0, s, xs, L.x, L.rhs, L.a,
+                           L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.hx, L.hy, mg.nu2,
                            (const T *)C.x);
         return;
     }

@@ -339,8 +339,10 @@ def test_blocked_smoother_vcycle_matches_per_colour_launches():
     """The V-cycle with blocked smoothing and fused residual-restriction is
     bit-identical to the per-colour-launch V-cycle, in FP64 (and equal to
     the oracle) and in FP32; the plain solve with it gives the same iterate."""
-    announce("helm blocked smoother 256x128")
-    nx, ny = 256, 128
+    announce("helm blocked smoother 1024x512")
+    # large enough that the smoother launch has more workgroups than run at
+    # once (a tile must not read halo values another tile already wrote)
+    nx, ny = 1024, 512
     s, (a, bX, bY), kw, rng = make_solver(nx, ny, 12)
     r = rng.normal(size=(ny, nx))
     dr, de = dev(r), DeviceArray((ny, nx))

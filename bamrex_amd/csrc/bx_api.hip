@@ -1177,7 +1177,23 @@ extern "C" int bx_autotune(bx_ctx *c, const double dx[3], double dt, double *U,
     bx_tuning best = c->tune;
     const int blocks[3] = { 64, 128, 256 };
     const int caps[4]   = { 0, 1, 2, 4 };
-    const int ks[5]     = { 8, 16, 32, 64, 128 };
+    // segment lengths: powers of two, and the median box length along the
+    // transverse sweeps split into m equal parts (m = 1..8), which lets the
+    // item count divide evenly over 2, 3, ... resident waves per SIMD
+    std::vector<int> ks = { 8, 16, 32, 64, 128 };
+    if (c->D >= 2)
+    {
+        std::vector<int> len;
+        for (const bx_box &bx : c->hboxes)
+            len.push_back(bx.hi[1] - bx.lo[1] + 1);
+        std::nth_element(len.begin(), len.begin() + len.size() / 2, len.end());
+        const int L = len[len.size() / 2];
+        for (int m = 1; m <= 8; ++m)
+            if ((L + m - 1) / m >= 4)
+                ks.push_back((L + m - 1) / m);
+        std::sort(ks.begin(), ks.end());
+        ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+    }
     // x sweep: block size x grid cap
     double bx_ms = 1e30;
     for (int b : blocks)

@@ -83,8 +83,11 @@ __device__ __forceinline__ void wave_item_range(long long nitems,
     int        blk    = blockIdx.x;
     if ((nblk & 7) == 0)
         blk = (blk & 7) * (nblk >> 3) + (blk >> 3);
+    // the wave index is uniform across the wave: readfirstlane lets the
+    // compiler keep all per-item bookkeeping in scalar registers
+    const int       wib  = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long long nw   = (long long)nblk * wpb;
-    const long long w    = (long long)blk * wpb + (threadIdx.x >> 6);
+    const long long w    = (long long)blk * wpb + wib;
     const long long per  = (nitems + nw - 1) / nw;
     first                = w * per;
     last                 = first + per;
@@ -169,17 +172,24 @@ __device__ __forceinline__ void x_item(const DevBox *__restrict__ boxes,
     it.nxd = nxd;
 }
 
-template <int D, int LIM, int LV, bool FIRST_ORDER, bool E1>
+// PM (physics mode) bits: PM_NI -- the HLLC exponents take the near-integer
+// pow forms; PM_E1 -- epsilon == 1 (set only together with PM_NI).
+constexpr int PM_NI = 1;
+constexpr int PM_E1 = 2;
+
+template <int D, int LIM, int LV, bool FIRST_ORDER, int PM>
 __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     const double *__restrict__ Uin, double *__restrict__ Uout,
     const DevBox *__restrict__ boxes, const long long *__restrict__ ibeg,
     int nbox, long long nitems, int ng, const double *__restrict__ d_dt,
     double dx, EulerParm pm, LimTable lt, unsigned long long *d_maxbits)
 {
-    constexpr int NC    = D + 2;
-    const int     lane  = threadIdx.x & 63;
-    const double  dt    = *d_dt;
-    const double  hdtdx = 0.5 * dt / dx;
+    constexpr int  NC    = D + 2;
+    constexpr bool E1    = (PM & PM_E1) != 0;
+    constexpr bool NI    = (PM & PM_NI) != 0;
+    const int      lane  = threadIdx.x & 63;
+    const double   dt    = *d_dt;
+    const double   hdtdx = 0.5 * dt / dx;
     const double  dtdx  = dt / dx;
 
     long long first, last;
@@ -218,7 +228,7 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
             Um[n] = wave_shr1(U[n]);
         if (FIRST_ORDER)
         {
-            hllc_flux<D, 0, E1>(Um, U, F, pm);
+            hllc_flux<D, 0, E1, NI>(Um, U, F, pm);
         }
         else
         {
@@ -231,7 +241,7 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
 #pragma unroll
             for (int n = 0; n < NC; ++n)
                 hRm[n] = wave_shr1(hR[n]);
-            hllc_flux<D, 0, E1>(hRm, hL, F, pm);
+            hllc_flux<D, 0, E1, NI>(hRm, hL, F, pm);
         }
         double Fp[NC];
 #pragma unroll
@@ -263,17 +273,19 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
 }
 
 // -------------------------------------------------------------- y/z sweep
-template <int D, int DIR, int LIM, int LV, bool FIRST_ORDER, bool E1>
+template <int D, int DIR, int LIM, int LV, bool FIRST_ORDER, int PM>
 __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
     const double *__restrict__ Uin, double *__restrict__ Uout,
     const DevBox *__restrict__ boxes, const long long *__restrict__ ibeg,
     int nbox, long long nitems, int ng, int K, const double *__restrict__ d_dt,
     double dx, EulerParm pm, LimTable lt, unsigned long long *d_maxbits)
 {
-    constexpr int NC    = D + 2;
-    const int     lane  = threadIdx.x & 63;
-    const double  dt    = *d_dt;
-    const double  hdtdx = 0.5 * dt / dx;
+    constexpr int  NC    = D + 2;
+    constexpr bool E1    = (PM & PM_E1) != 0;
+    constexpr bool NI    = (PM & PM_NI) != 0;
+    const int      lane  = threadIdx.x & 63;
+    const double   dt    = *d_dt;
+    const double   hdtdx = 0.5 * dt / dx;
     const double  dtdx  = dt / dx;
     const int     ngy   = (D >= 2) ? ng : 0;
     const int     ngz   = (D >= 3) ? ng : 0;
@@ -354,14 +366,14 @@ __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
             double F[NC];
             if (FIRST_ORDER)
             {
-                hllc_flux<D, DIR, E1>(Um1, U0, F, pm);
+                hllc_flux<D, DIR, E1, NI>(Um1, U0, F, pm);
             }
             else
             {
                 double hL0[NC], hR0[NC];
                 reconstruct_half_step<D, DIR, LIM, LV, E1>(Um1, U0, Up, hL0,
                                                            hR0, hdtdx, pm, lt);
-                hllc_flux<D, DIR, E1>(hRprev, hL0, F, pm);
+                hllc_flux<D, DIR, E1, NI>(hRprev, hL0, F, pm);
 #pragma unroll
                 for (int n = 0; n < NC; ++n)
                     hRprev[n] = hR0[n];
@@ -542,46 +554,48 @@ __global__ void k_set_u64(unsigned long long *p, unsigned long long v)
 }
 
 // ------------------------------------------------------------ launchers
-template <int D, int LIM, int LV, bool FO, bool E1>
+template <int D, int LIM, int LV, bool FO, int PM>
 static hipError_t launch_x(const SweepLaunch &L)
 {
-    hipLaunchKernelGGL((k_sweep_x<D, LIM, LV, FO, E1>), dim3(L.grid),
+    hipLaunchKernelGGL((k_sweep_x<D, LIM, LV, FO, PM>), dim3(L.grid),
                        dim3(L.block), 0, L.stream, L.Uin, L.Uout, L.boxes,
                        L.item_begin, L.nbox, L.nitems, L.ng, L.d_dt, L.dx,
                        L.pm, L.lt, L.d_maxbits);
     return hipGetLastError();
 }
 
-template <int D, int DIR, int LIM, int LV, bool FO, bool E1>
+template <int D, int DIR, int LIM, int LV, bool FO, int PM>
 static hipError_t launch_t(const SweepLaunch &L)
 {
-    hipLaunchKernelGGL((k_sweep_t<D, DIR, LIM, LV, FO, E1>), dim3(L.grid),
+    hipLaunchKernelGGL((k_sweep_t<D, DIR, LIM, LV, FO, PM>), dim3(L.grid),
                        dim3(L.block), 0, L.stream, L.Uin, L.Uout, L.boxes,
                        L.item_begin, L.nbox, L.nitems, L.ng, L.K, L.d_dt,
                        L.dx, L.pm, L.lt, L.d_maxbits);
     return hipGetLastError();
 }
 
-template <int D, int LIM, int LV, bool FO, bool E1>
+template <int D, int LIM, int LV, bool FO, int PM>
 static hipError_t dispatch_dir_e(int dir, const SweepLaunch &L)
 {
     if (dir == 0)
-        return launch_x<D, LIM, LV, FO, E1>(L);
+        return launch_x<D, LIM, LV, FO, PM>(L);
     if constexpr (D >= 2)
         if (dir == 1)
-            return launch_t<D, 1, LIM, LV, FO, E1>(L);
+            return launch_t<D, 1, LIM, LV, FO, PM>(L);
     if constexpr (D >= 3)
         if (dir == 2)
-            return launch_t<D, 2, LIM, LV, FO, E1>(L);
+            return launch_t<D, 2, LIM, LV, FO, PM>(L);
     return hipErrorInvalidValue;
 }
 
 template <int D, int LIM, int LV, bool FO>
 static hipError_t dispatch_dir(int dir, const SweepLaunch &L)
 {
+    if (L.pm.ph.n == 0) // general pow (and any epsilon)
+        return dispatch_dir_e<D, LIM, LV, FO, 0>(dir, L);
     if (L.pm.eps == 1.0)
-        return dispatch_dir_e<D, LIM, LV, FO, true>(dir, L);
-    return dispatch_dir_e<D, LIM, LV, FO, false>(dir, L);
+        return dispatch_dir_e<D, LIM, LV, FO, PM_NI | PM_E1>(dir, L);
+    return dispatch_dir_e<D, LIM, LV, FO, PM_NI>(dir, L);
 }
 
 template <int D, int LIM, int LV>
@@ -637,16 +651,16 @@ const void *sweep_kernel_ptr(int D, int dir)
     if (dir == 0)
     {
         if (D == 1)
-            return (const void *)&k_sweep_x<1, LIM_MINBEE, LV_SHARED, false, true>;
+            return (const void *)&k_sweep_x<1, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
         if (D == 2)
-            return (const void *)&k_sweep_x<2, LIM_MINBEE, LV_SHARED, false, true>;
-        return (const void *)&k_sweep_x<3, LIM_MINBEE, LV_SHARED, false, true>;
+            return (const void *)&k_sweep_x<2, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
+        return (const void *)&k_sweep_x<3, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
     }
     if (D == 2)
-        return (const void *)&k_sweep_t<2, 1, LIM_MINBEE, LV_SHARED, false, true>;
+        return (const void *)&k_sweep_t<2, 1, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
     if (dir == 1)
-        return (const void *)&k_sweep_t<3, 1, LIM_MINBEE, LV_SHARED, false, true>;
-    return (const void *)&k_sweep_t<3, 2, LIM_MINBEE, LV_SHARED, false, true>;
+        return (const void *)&k_sweep_t<3, 1, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
+    return (const void *)&k_sweep_t<3, 2, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
 }
 
 hipError_t launch_fill_ghosts(double *U, const DevBox *boxes,

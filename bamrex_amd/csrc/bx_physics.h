@@ -238,7 +238,10 @@ BX_DEV double amrex_max(double a, double b) { return (a < b) ? b : a; }
 
 // HLLC flux at the face between left state UL and right state UR
 // (HLLC.cpp:90-172 with estimate_wave_speed_HLLC_Toro, HLLC.H:213-309).
-template <int D, int dir, bool E1 = false>
+// NI: pm.ph.n != 0, so the two exponents take the near-integer forms with
+// no general pow inlined (same results as the dispatching forms: the floored
+// pressures are >= 1e-14 or NaN).
+template <int D, int dir, bool E1 = false, bool NI = false>
 BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
                       double (&F)[D + 2], const EulerParm &pm)
 {
@@ -275,10 +278,23 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
     }
 
     const double z = (adiabatic - 1) / (2 * adiabatic);
-    const double p_rr = bx_pow_invz(
-        (c_L + c_R - (adiabatic - 1) / 2 * (u_R - u_L))
-            / (c_L * bx_pow_negz(p_L, -z, pm.ph) + c_R * bx_pow_negz(p_R, -z, pm.ph)),
-        1 / z, pm.ph);
+    double       pzL, pzR, p_rr;
+    if (NI)
+    {
+        pzL = bx_pow_negz_ni(p_L, pm.ph);
+        pzR = bx_pow_negz_ni(p_R, pm.ph);
+    }
+    else
+    {
+        pzL = bx_pow_negz(p_L, -z, pm.ph);
+        pzR = bx_pow_negz(p_R, -z, pm.ph);
+    }
+    const double ratio = (c_L + c_R - (adiabatic - 1) / 2 * (u_R - u_L))
+                         / (c_L * pzL + c_R * pzR);
+    if (NI)
+        p_rr = bx_pow_invz_ni(ratio, pm.ph);
+    else
+        p_rr = bx_pow_invz(ratio, 1 / z, pm.ph);
     const double q_L
         = (p_rr > p_L)
               ? sqrt(1 + (adiabatic + 1) / (2 * adiabatic) * (p_rr / p_L - 1))

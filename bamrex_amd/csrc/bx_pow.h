@@ -250,13 +250,17 @@ BX_POW_FN double bx_pow(double x, double y)
 // integer (gamma = 1.4: n = 7; gamma = 5/3: n = 5).  Then
 //   x^(n + d)     = x^n (1 + d log x + ...),        d = 1/z - n   (exact),
 //   x^(-1/n + w)  = x^(-1/n) (1 + w log x + ...),   w = 1/n - z,
-// with |d|, |w| ~ 1e-15.  x^n is formed in double-double by binary powering
+// with |d|, |w| <= 2^-48, so the dropped terms stay below 4e-24 relative
+// for every double x.  x^n is formed in double-double by binary powering
 // and x^(-1/n) by one double-double Newton step on x r^n = 1 from a 1e-12
 // estimate, so both values are known to ~1e-23 relative before the final
 // rounding: the result is the correctly rounded pow(x, y) except when the
 // exact value lies within ~1e-23 of a rounding boundary, the same answer the
 // general routine above gives, for about a third of its instruction count.
-// Inputs outside the stated ranges go to bx_pow.
+// The double-double work runs on operands rescaled by exact powers of two
+// (x r^n is scale-free), so no input range needs the general routine: the
+// *_ni forms below have no branch into it and keep a kernel that calls them
+// free of its registers.
 namespace bxpow {
 
 BX_POW_FN void dd_sqr(double &h, double &l)
@@ -331,6 +335,7 @@ BX_POW_FN double exp_approx(double t)
 struct BxPowHllc
 {
     int    n;
+    int    qmul;  // round(2^16 / n): (k * qmul) >> 16 is within 1 of k / n
     double inv_n; // RN(1/n)
     double d;     // 1/z - n, exact
     double w;     // 1/n - z to ~1e-16 relative
@@ -338,7 +343,7 @@ struct BxPowHllc
 
 static inline BxPowHllc bx_pow_hllc_setup(double gamma)
 {
-    BxPowHllc    s  = { 0, 0.0, 0.0, 0.0 };
+    BxPowHllc    s  = { 0, 0, 0.0, 0.0, 0.0 };
     const double z  = (gamma - 1) / (2 * gamma);
     const double yi = 1 / z;
     if (!(yi >= 1.5 && yi <= 32.5))
@@ -349,39 +354,78 @@ static inline BxPowHllc bx_pow_hllc_setup(double gamma)
     const double nz_hi = nd * z;
     const double nz_lo = __builtin_fma(nd, z, -nz_hi);
     const double w     = ((1.0 - nz_hi) - nz_lo) / nd;
-    if (!(__builtin_fabs(d) <= 0x1p-40 && __builtin_fabs(w) <= 0x1p-40))
+    if (!(__builtin_fabs(d) <= 0x1p-48 && __builtin_fabs(w) <= 0x1p-48))
         return s;
     s.n     = (int)nd;
+    s.qmul  = (65536 + s.n / 2) / s.n;
     s.inv_n = 1.0 / nd;
     s.d     = d;
     s.w     = w;
     return s;
 }
 
-// pow(x, -z) for the estimate's pressures; y = -z is the exact exponent
-BX_POW_FN double bx_pow_negz(double x, double y, const BxPowHllc &s)
+namespace bxpow {
+// unbiased exponent field of x (sign ignored; -1023 for zero and subnormals)
+BX_POW_FN int exponent_field(double x)
+{
+    return (int)((d2u(x) >> 52) & 0x7ff) - 1023;
+}
+} // namespace bxpow
+
+// pow(x, -z) for s.n != 0 and x >= 2^-1022, +inf or NaN (the estimate's
+// pressures, floored at 1e-14).  The Newton residual x r^n - 1 is formed on
+// x 2^(-n q) and r 2^q with q ~ k/n (k the exponent of x): exact rescalings
+// that keep the double-double intermediates normal for every such x.
+BX_POW_FN double bx_pow_negz_ni(double x, const BxPowHllc &s)
 {
     using namespace bxpow;
-    if (!(s.n != 0 && x >= 0x1p-600 && x <= 0x1p600))
-        return bx_pow(x, y);
     const double lx = log_approx(x);
     const double r  = exp_approx(-lx * s.inv_n); // x^(-1/n) to ~1e-12
+    const int    q  = (exponent_field(x) * s.qmul) >> 16;
     double       h, l;
-    dd_powi(r, s.n, h, l);
-    dd_mul_d(h, l, x); // x r^n = h + l, close to 1
+    dd_powi(__builtin_ldexp(r, q), s.n, h, l);
+    dd_mul_d(h, l, __builtin_ldexp(x, -s.n * q)); // x r^n = h + l, close to 1
     const double delta = (1.0 - h) - l;
     const double c     = r * (delta * s.inv_n); // Newton correction
-    return r + (c + r * (s.w * lx));
+    const double res   = r + (c + r * (s.w * lx));
+    return x == __builtin_inf() ? 0.0 : res;
+}
+
+// pow(x, -z); y = -z is the exact exponent
+BX_POW_FN double bx_pow_negz(double x, double y, const BxPowHllc &s)
+{
+    if (!(s.n != 0 && x >= 0x1p-1022))
+        return bx_pow(x, y);
+    return bx_pow_negz_ni(x, s);
+}
+
+// pow(x, 1/z) for s.n != 0 and any x.  x^n is formed on the mantissa
+// m = x 2^-k in [1, 2) and scaled back by 2^(n k) after the final rounding:
+// exact unless the result is below 2^-1022, where that second scaling
+// rounds again (results under 2^-1022 may be off by one subnormal ulp).
+BX_POW_FN double bx_pow_invz_ni(double x, const BxPowHllc &s)
+{
+    using namespace bxpow;
+    const int k = exponent_field(x);
+    double    h, l;
+    dd_powi(__builtin_ldexp(x, -k), s.n, h, l); // signed: (-m)^n
+    const double t   = s.d * log_approx(x);    // log |x|
+    double       res = __builtin_ldexp(h + __builtin_fma(h, t, l), s.n * k);
+    // 1/z is an integer only when d == 0; otherwise x < 0 gives NaN
+    const bool odd_int = s.d == 0.0 && (s.n & 1);
+    if (x == 0.0)
+        res = odd_int ? x : 0.0;
+    else if (__builtin_fabs(x) == __builtin_inf())
+        res = (odd_int && x < 0) ? -__builtin_inf() : __builtin_inf();
+    else if (x < 0 && s.d != 0.0)
+        res = __builtin_nan("");
+    return res;
 }
 
 // pow(x, 1/z); y = 1/z is the exact exponent
 BX_POW_FN double bx_pow_invz(double x, double y, const BxPowHllc &s)
 {
-    using namespace bxpow;
-    if (!(s.n != 0 && x >= 0x1p-28 && x <= 0x1p28))
+    if (s.n == 0)
         return bx_pow(x, y);
-    double h, l;
-    dd_powi(x, s.n, h, l);
-    const double t = s.d * log_approx(x);
-    return h + __builtin_fma(h, t, l);
+    return bx_pow_invz_ni(x, s);
 }

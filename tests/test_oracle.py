@@ -244,6 +244,45 @@ def test_pow_hllc_exponents_correctly_rounded(oracle_bxpow, gamma):
         assert err <= D("0.5000001"), (x, y, got, err)
 
 
+@pytest.mark.parametrize("gamma", [1.4, 5.0 / 3.0])
+def test_pow_hllc_exponents_full_range(oracle_bxpow, gamma):
+    """The near-integer routines rescale by powers of two instead of falling
+    back to the general routine: correctly rounded over the whole normal
+    range of x (60-digit reference), and IEEE pow's special values."""
+    decimal.getcontext().prec = 60
+    D = decimal.Decimal
+    rng = np.random.default_rng(29)
+    z = (gamma - 1) / (2 * gamma)
+    n = round(1 / z)
+    xmax = 1022.0 / n  # |log2 X| keeping X^(1/z) normal
+    ps = np.concatenate([np.exp2(rng.uniform(-1022, 1023.9, 150)),
+                         [2.0 ** -1022, 1e-14, 1.7976931348623157e308, 2.0 ** 600, 2.0 ** -600]])
+    Xs = np.concatenate([np.exp2(rng.uniform(-xmax, xmax, 150)), [2.0 ** 28 * 1.5, 2.0 ** -28 / 1.5]])
+    for x, f, y in [(p, oracle_bxpow.pow_negz, -z) for p in ps] + [(X, oracle_bxpow.pow_invz, 1 / z) for X in Xs]:
+        got = f(float(x), gamma)
+        ex = (D(float(x)).ln() * D(y)).exp()
+        err = abs(D(got) - ex) / D(math.ulp(got))
+        assert err <= D("0.5000001"), (x, y, got, err)
+    inf = math.inf
+    assert oracle_bxpow.pow_negz(inf, gamma) == 0.0
+    assert math.isnan(oracle_bxpow.pow_negz(math.nan, gamma))
+    assert oracle_bxpow.pow_negz(0.0, gamma) == inf
+    assert oracle_bxpow.pow_negz(5e-320, gamma) == oracle_bxpow.pow(5e-320, -z)
+    assert oracle_bxpow.pow_invz(inf, gamma) == inf
+    assert math.isnan(oracle_bxpow.pow_invz(math.nan, gamma))
+    assert oracle_bxpow.pow_invz(2.0 ** (xmax + 1), gamma) == inf
+    assert oracle_bxpow.pow_invz(2.0 ** -(1080.0 / n), gamma) == 0.0
+    assert oracle_bxpow.pow_invz(5e-320, gamma) == 0.0
+    if 1 / z == n:  # integer exponent (gamma = 5/3: 5): odd powers of negatives
+        assert oracle_bxpow.pow_invz(-2.0, gamma) == -(2.0 ** n)
+        assert oracle_bxpow.pow_invz(-inf, gamma) == -inf
+        assert math.copysign(1.0, oracle_bxpow.pow_invz(-0.0, gamma)) == -1.0
+    else:
+        assert math.isnan(oracle_bxpow.pow_invz(-2.0, gamma))
+        assert oracle_bxpow.pow_invz(-inf, gamma) == inf
+        assert math.copysign(1.0, oracle_bxpow.pow_invz(-0.0, gamma)) == 1.0
+
+
 def test_pow_hllc_exponents_match_general_routine(oracle_bxpow):
     """Against the general routine: equal except where the exact value is
     within ~1e-19 of a rounding boundary (tens of cases per million)."""

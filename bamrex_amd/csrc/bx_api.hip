@@ -710,15 +710,15 @@ static int do_sweep(bx_ctx *c, int dir, const double dx[3], const double *Uin,
     return BX_OK;
 }
 
-static int do_fill(bx_ctx *c, double *U, int list, hipStream_t s)
+static int do_fill(bx_ctx *c, double *U, int list, hipStream_t s,
+                   const StepTail &tail = StepTail())
 {
-    const long long n = c->gcells[list];
-    if (n == 0)
+    const int nseg = (int)c->segs[list].size();
+    if (nseg == 0 && !tail.d_time)
         return BX_OK;
-    const int grid = (int)std::min<long long>((n + 255) / 256, (long long)c->ncu * 8);
-    BX_HIP_CHECK(launch_fill_ghosts(U, c->d_boxes, c->d_segs[list],
-                                    (int)c->segs[list].size(), n, c->geom.ng,
-                                    c->D, c->NC, grid, s));
+    const int grid = std::max(1, std::min(nseg, c->ncu * 16));
+    BX_HIP_CHECK(launch_fill_ghosts(U, c->d_boxes, c->d_segs[list], nseg,
+                                    c->geom.ng, c->D, c->NC, grid, tail, s));
     return BX_OK;
 }
 
@@ -931,10 +931,10 @@ extern "C" int bx_est_dt(bx_ctx *c, const double *U, const double dx[3],
 
 // One split step on device buffers.  dt is read from c->d_dt.  When
 // fuse_max is set the last sweep accumulates the CFL maximum of the new
-// state into c->d_maxbits.  Returns in *ends_in_b whether the new state is
-// in B (odd number of sweeps).
+// state into c->d_maxbits.  A non-empty tail runs with the last ghost fill.
 static int step_body(bx_ctx *c, const double dx[3], double *A, double *B,
-                     bool fuse_max, hipStream_t s)
+                     bool fuse_max, hipStream_t s,
+                     const StepTail &tail = StepTail())
 {
     const int D = c->D;
     int       rc;
@@ -948,7 +948,9 @@ static int step_body(bx_ctx *c, const double dx[3], double *A, double *B,
             return rc;
         // ghost cells the next sweep reads: direction d+1, or x for the
         // next step after the last sweep
-        if ((rc = do_fill(c, out, (d + 1) % D, s)) != BX_OK)
+        if ((rc = do_fill(c, out, (d + 1) % D, s,
+                          d == D - 1 ? tail : StepTail()))
+            != BX_OK)
             return rc;
     }
     return BX_OK;
@@ -973,16 +975,24 @@ extern "C" int bx_advance(bx_ctx *c, double dt, const double dx[3], double *U,
     return do_fill_all(c, U, s);
 }
 
+// One step of the device-controlled time loop.  The step starts with the
+// dt already in c->d_dt; the last ghost fill advances the time by it and
+// computes the next step's dt from the maximum the last sweep accumulated
+// (the loop's first dt comes from launch_compute_dt before the first step).
 static int step_device(bx_ctx *c, const double dx[3], double cfl,
                        double stop_time, double *A, double *B, hipStream_t s)
 {
-    int rc;
-    BX_HIP_CHECK(launch_compute_dt(c->d_maxbits, c->d_dt, c->d_time, c->D, dx,
-                                   cfl, stop_time, -1.0, s));
-    if ((rc = step_body(c, dx, A, B, true, s)) != BX_OK)
-        return rc;
-    BX_HIP_CHECK(launch_advance_time(c->d_time, c->d_dt, c->d_step, s));
-    return BX_OK;
+    StepTail t;
+    t.d_maxbits = c->d_maxbits;
+    t.d_dt      = c->d_dt;
+    t.d_time    = c->d_time;
+    t.d_step    = c->d_step;
+    t.D         = c->D;
+    for (int d = 0; d < 3; ++d)
+        t.dx[d] = dx[d];
+    t.cfl       = cfl;
+    t.stop_time = stop_time;
+    return step_body(c, dx, A, B, true, s, t);
 }
 
 static int get_graph(bx_ctx *c, int which, const double dx[3], double cfl,
@@ -1047,6 +1057,8 @@ extern "C" int bx_run(bx_ctx *c, const double dx[3], double cfl,
     if ((rc = set_scalar(c->d_time, start_time, s)) != BX_OK)
         return rc;
     BX_HIP_CHECK(launch_set_u64((unsigned long long *)c->d_step, 0ULL, s));
+    BX_HIP_CHECK(launch_compute_dt(c->d_maxbits, c->d_dt, c->d_time, c->D, dx,
+                                   cfl, stop_time, -1.0, s));
 
     double *A = U, *B = U_tmp;
     const bool odd = (c->D % 2) == 1;

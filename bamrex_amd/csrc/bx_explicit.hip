@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     const int      lane  = threadIdx.x & 63;
     const double   dt    = *d_dt;
     const double   hdtdx = 0.5 * dt / dx;
-    const double  dtdx  = dt / dx;
+    const double   dtdx  = dt / dx;
 
     long long first, last;
     wave_item_range(nitems, first, last);
@@ -286,9 +286,9 @@ __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
     const int      lane  = threadIdx.x & 63;
     const double   dt    = *d_dt;
     const double   hdtdx = 0.5 * dt / dx;
-    const double  dtdx  = dt / dx;
-    const int     ngy   = (D >= 2) ? ng : 0;
-    const int     ngz   = (D >= 3) ? ng : 0;
+    const double   dtdx  = dt / dx;
+    const int      ngy   = (D >= 2) ? ng : 0;
+    const int      ngz   = (D >= 3) ? ng : 0;
 
     long long first, last;
     wave_item_range(nitems, first, last);
@@ -412,52 +412,52 @@ __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
         wave_max_to_global(vmax, d_maxbits);
 }
 
+__device__ void step_tail(const StepTail &t);
+
 // ------------------------------------------------------------- ghost fill
+// One block per ghost segment (block-stride over segments), so the segment
+// and its two boxes are wave-uniform loads; lanes run over the segment's
+// cells, x fastest.
 __global__ __launch_bounds__(256) void k_fill_ghosts(
     double *__restrict__ U, const DevBox *__restrict__ boxes,
-    const GhostSeg *__restrict__ segs, int nseg, long long ncells, int ng,
-    int D, int NC)
+    const GhostSeg *__restrict__ segs, int nseg, int ng, int D, int NC,
+    StepTail tail)
 {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    const int       ngy    = (D >= 2) ? ng : 0;
-    const int       ngz    = (D >= 3) ? ng : 0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-         c < ncells; c += stride)
+    if (tail.d_time && blockIdx.x == 0 && threadIdx.x == 0)
+        step_tail(tail);
+    const int ngy = (D >= 2) ? ng : 0;
+    const int ngz = (D >= 3) ? ng : 0;
+    for (int si = blockIdx.x; si < nseg; si += gridDim.x)
     {
-        int lo = 0, hi = nseg;
-        while (hi - lo > 1)
+        const GhostSeg &g  = segs[si];
+        const int       ex = g.hi[0] - g.lo[0] + 1;
+        const int       ey = g.hi[1] - g.lo[1] + 1;
+        const int       ez = g.hi[2] - g.lo[2] + 1;
+        const int       n  = ex * ey * ez;
+        const DevBox   &db = boxes[g.dst];
+        const DevBox   &sb = boxes[g.src];
+        for (int c = threadIdx.x; c < n; c += blockDim.x)
         {
-            const int mid = (lo + hi) >> 1;
-            if (segs[mid].cell_begin <= c)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        const GhostSeg &g     = segs[lo];
-        long long       local = c - g.cell_begin;
-        const int       ex    = g.hi[0] - g.lo[0] + 1;
-        const int       ey    = g.hi[1] - g.lo[1] + 1;
-        const int       x     = g.lo[0] + (int)(local % ex);
-        local /= ex;
-        const int y = g.lo[1] + (int)(local % ey);
-        const int z = g.lo[2] + (int)(local / ey);
-        const int sx = g.a[0] * x + g.b[0];
-        const int sy = g.a[1] * y + g.b[1];
-        const int sz = g.a[2] * z + g.b[2];
-        const DevBox &db = boxes[g.dst];
-        const DevBox &sb = boxes[g.src];
-        const long long doff = db.base + (x - db.lo[0] + ng)
-                               + (long long)(y - db.lo[1] + ngy) * db.jstride
-                               + (long long)(z - db.lo[2] + ngz) * db.kstride;
-        const long long soff = sb.base + (sx - sb.lo[0] + ng)
-                               + (long long)(sy - sb.lo[1] + ngy) * sb.jstride
-                               + (long long)(sz - sb.lo[2] + ngz) * sb.kstride;
-        for (int n = 0; n < NC; ++n)
-        {
-            double v = U[soff + n * sb.nstride];
-            if ((g.flip_mask >> n) & 1)
-                v = -v;
-            U[doff + n * db.nstride] = v;
+            const int x  = g.lo[0] + c % ex;
+            const int t  = c / ex;
+            const int y  = g.lo[1] + t % ey;
+            const int z  = g.lo[2] + t / ey;
+            const int sx = g.a[0] * x + g.b[0];
+            const int sy = g.a[1] * y + g.b[1];
+            const int sz = g.a[2] * z + g.b[2];
+            const long long doff = db.base + (x - db.lo[0] + ng)
+                                   + (long long)(y - db.lo[1] + ngy) * db.jstride
+                                   + (long long)(z - db.lo[2] + ngz) * db.kstride;
+            const long long soff = sb.base + (sx - sb.lo[0] + ng)
+                                   + (long long)(sy - sb.lo[1] + ngy) * sb.jstride
+                                   + (long long)(sz - sb.lo[2] + ngz) * sb.kstride;
+            for (int m = 0; m < NC; ++m)
+            {
+                double v = U[soff + m * sb.nstride];
+                if ((g.flip_mask >> m) & 1)
+                    v = -v;
+                U[doff + m * db.nstride] = v;
+            }
         }
     }
 }
@@ -501,13 +501,11 @@ __global__ __launch_bounds__(256) void k_max_wave_speed(
 // estTimeStep (AmrLevelAdv.cpp:395-477) for the explicit methods, followed by
 // the stop-time clamp of computeNewDt (AmrLevelAdv.cpp:589-618).  The
 // accumulator is cleared for the next fused reduction.
-__global__ void k_compute_dt(unsigned long long *d_maxbits, double *d_dt,
-                             const double *d_time, int D, double dx0,
-                             double dx1, double dx2, double cfl,
-                             double stop_time, double fixed_dt)
+__device__ void compute_dt(unsigned long long *d_maxbits, double *d_dt,
+                           const double *d_time, int D, double dx0, double dx1,
+                           double dx2, double cfl, double stop_time,
+                           double fixed_dt)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0)
-        return;
     double dt_0;
     if (fixed_dt > 0)
     {
@@ -538,13 +536,26 @@ __global__ void k_compute_dt(unsigned long long *d_maxbits, double *d_dt,
     *d_maxbits = 0ULL;
 }
 
-__global__ void k_advance_time(double *d_time, const double *d_dt,
-                               long long *d_step)
+__global__ void k_compute_dt(unsigned long long *d_maxbits, double *d_dt,
+                             const double *d_time, int D, double dx0,
+                             double dx1, double dx2, double cfl,
+                             double stop_time, double fixed_dt)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
-    *d_time = *d_time + *d_dt;
-    *d_step = *d_step + 1;
+    compute_dt(d_maxbits, d_dt, d_time, D, dx0, dx1, dx2, cfl, stop_time,
+               fixed_dt);
+}
+
+// End of a step, run by one thread of the last ghost fill: advance the time
+// by the step's dt, then the dt of the next step from the maximum the last
+// sweep accumulated (the stop-time clamp uses the advanced time).
+__device__ void step_tail(const StepTail &t)
+{
+    *t.d_time = *t.d_time + *t.d_dt;
+    *t.d_step = *t.d_step + 1;
+    compute_dt(t.d_maxbits, t.d_dt, t.d_time, t.D, t.dx[0], t.dx[1], t.dx[2],
+               t.cfl, t.stop_time, -1.0);
 }
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v)
@@ -664,14 +675,14 @@ const void *sweep_kernel_ptr(int D, int dir)
 }
 
 hipError_t launch_fill_ghosts(double *U, const DevBox *boxes,
-                              const GhostSeg *segs, int nseg, long long ncells,
-                              int ng, int D, int NC, int grid,
+                              const GhostSeg *segs, int nseg, int ng, int D,
+                              int NC, int grid, const StepTail &tail,
                               hipStream_t stream)
 {
-    if (ncells == 0)
+    if (nseg == 0 && !tail.d_time)
         return hipSuccess;
     hipLaunchKernelGGL(k_fill_ghosts, dim3(grid), dim3(256), 0, stream, U,
-                       boxes, segs, nseg, ncells, ng, D, NC);
+                       boxes, segs, nseg, ng, D, NC, tail);
     return hipGetLastError();
 }
 
@@ -711,14 +722,6 @@ hipError_t launch_compute_dt(unsigned long long *d_maxbits, double *d_dt,
     hipLaunchKernelGGL(k_compute_dt, dim3(1), dim3(64), 0, stream, d_maxbits,
                        d_dt, d_time, D, dx[0], dx[1], dx[2], cfl, stop_time,
                        fixed_dt);
-    return hipGetLastError();
-}
-
-hipError_t launch_advance_time(double *d_time, const double *d_dt,
-                               long long *d_step, hipStream_t stream)
-{
-    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(64), 0, stream, d_time,
-                       d_dt, d_step);
     return hipGetLastError();
 }
 

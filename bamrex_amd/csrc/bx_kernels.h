@@ -7,6 +7,21 @@
 
 namespace bx {
 
+// Optional end-of-step work of a ghost fill (d_time == nullptr: none): one
+// thread advances the time by dt, then computes the next dt from the
+// maximum the last sweep accumulated.
+struct StepTail
+{
+    unsigned long long *d_maxbits = nullptr;
+    double             *d_dt      = nullptr;
+    double             *d_time    = nullptr;
+    long long          *d_step    = nullptr;
+    int                 D         = 0;
+    double              dx[3]     = { 0, 0, 0 };
+    double              cfl       = 0;
+    double              stop_time = -1;
+};
+
 struct SweepLaunch
 {
     const double       *Uin;
@@ -32,8 +47,8 @@ hipError_t  launch_sweep(int D, int dir, bool first_order, int lim, int lv,
 const void *sweep_kernel_ptr(int D, int dir);
 
 hipError_t launch_fill_ghosts(double *U, const DevBox *boxes,
-                              const GhostSeg *segs, int nseg, long long ncells,
-                              int ng, int D, int NC, int grid,
+                              const GhostSeg *segs, int nseg, int ng, int D,
+                              int NC, int grid, const StepTail &tail,
                               hipStream_t stream);
 
 hipError_t launch_max_wave_speed(int D, const double *U, const DevBox *boxes,
@@ -46,9 +61,6 @@ hipError_t launch_compute_dt(unsigned long long *d_maxbits, double *d_dt,
                              const double *d_time, int D, const double dx[3],
                              double cfl, double stop_time, double fixed_dt,
                              hipStream_t stream);
-
-hipError_t launch_advance_time(double *d_time, const double *d_dt,
-                               long long *d_step, hipStream_t stream);
 
 hipError_t launch_set_u64(unsigned long long *p, unsigned long long v,
                           hipStream_t stream);

@@ -289,6 +289,33 @@ def test_reflective_hllc_first_order_parity(oracle_bxpow):
     assert_valid_equal(got, ref, s["layout"])
 
 
+def test_non_conforming_level_run_parity(oracle_bxpow):
+    """A level whose box faces do not match cell for cell (a wide box next to
+    two boxes half its height): the ghost fill copies from two source boxes
+    per face, with the same results as the oracle."""
+    boxes = [((0, 0), (7, 15)), ((8, 0), (15, 7)), ((8, 8), (15, 15))]
+    lay = Layout(2, boxes, 2)
+    _, Uf = random_level(2, 16, 16, seed=7)
+    full = Layout(2, [((0, 0), (15, 15))], 2)
+    U = lay.new_array()
+    for b, (lo, hi) in enumerate(boxes):
+        lay.valid(U, b)[...] = full.valid(Uf, 0)[:, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+    dom = ((0, 0), (15, 15))
+    bcs = ["transmissive"] * 2
+    gp = make_params(2, 1.4, limiter="minbee")
+    op = pyoracle.params(2, 1.4, limiter="minbee")
+    lev = Level(gp, lay, dom[0], dom[1], bcs, bcs)
+    try:
+        lev.set_state(U)
+        lev.run((1 / 16, 1 / 16), 0.8, max_step=5, use_graph=False)
+        got = lev.get_state()
+    finally:
+        lev.close()
+    ref = U.copy()
+    oracle_bxpow.run(op, lay, ref, dom[0], dom[1], bcs, bcs, (1 / 16, 1 / 16), 0.8, max_step=5)
+    assert_valid_equal(got, ref, lay)
+
+
 def test_flux_box_drop_in(oracle_bxpow):
     """bx_flux_box writes the same face fluxes as compute_MUSCL_hancock_flux."""
     from bamrex_amd.level import DeviceBuffer

@@ -19,9 +19,10 @@
 //   W4[9][16]        tap t, in k
 // tap t = (dy+1)*3 + (dx+1), neighbour (x+dx, y+dy).
 //
-// The two 16 -> 16 layers run on v_mfma_f32_16x16x16bf16_1k: one wave takes
-// 16 consecutive pixels (M) x 16 output channels (N), K = 16 input channels
-// per tap, 9 MFMAs accumulate the 9 taps.
+// All four layers run on v_mfma_f32_16x16x16bf16_1k (see the forward
+// kernels): one wave takes 16 pixels (M); the 16 -> 16 layers accumulate 9
+// MFMAs (one per tap, K = 16 input channels), the first layer takes the 9
+// taps as K, the last one forms 9 per-tap partials per pixel as N.
 //
 // Training loss on crops of (residual, error) pairs harvested from plain
 // multigrid solves (A e = r):
@@ -85,42 +86,80 @@ __device__ inline int nbr(int x, int y, int dx, int dy, const Geo &g, int img)
 }
 
 // ---- inference / training forward --------------------------------------
+// Every layer runs on v_mfma_f32_16x16x16bf16_1k with one wave per 16 pixels:
+//   A[i][k] (lane l: i = l%16, k = 4(l/16)+v), B[k][j] (lane l: j = l%16),
+//   D[i][j] (lane l: i = 4(l/16)+v, j = l%16).
+// Each D[i][j] is a dot product over k of row i of A with column j of B, so a
+// pixel's result does not depend on which pixels share its tile: the fused
+// inference kernel below and these layer-by-layer kernels agree bit for bit.
+//   layer 1: A = the 9 taps of x at a pixel (k = tap, 7 zero), B = W1 (tap x out)
+//   layers 2, 3: A = 16 channels of a neighbour, B = W (in x out), 9 MFMAs
+//   layer 4: A = 16 channels of h3 at a pixel, B = W4 (in x tap), giving the
+//            per-tap partials P[pixel][tap]; y = sum over taps of P at the
+//            neighbours, in tap order.
+
+// B operand of layer 1: B[k = tap][j = out] = W1[tap][out] (tap < 9)
+__device__ inline v4s w1_operand(const unsigned short *__restrict__ Wb1, int m, int kg)
+{
+    v4s b;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+        const int t = 4 * kg + v;
+        b[v] = t < 9 ? (short)Wb1[t * C + m] : (short)0;
+    }
+    return b;
+}
+// B operand of layer 4: B[k = in channel][j = tap] = W4[tap][in] (tap < 9)
+__device__ inline v4s w4_operand(const unsigned short *__restrict__ Wb4, int m, int kg)
+{
+    v4s b;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        b[v] = m < 9 ? (short)Wb4[m * C + 4 * kg + v] : (short)0;
+    return b;
+}
+
 // layer 1: x (bf16, 1 channel) -> h1 (bf16, 16, ReLU)
 template <bool PERIODIC>
 __global__ __launch_bounds__(256) void k_conv1(const unsigned short *__restrict__ xin,
                                                unsigned short *__restrict__ hout,
-                                               const float *__restrict__ W1, Geo g)
+                                               const unsigned short *__restrict__ Wb1,
+                                               Geo g)
 {
-    const long long npx = (long long)g.w * g.h * g.nimg;
-    const long long p   = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npx)
+    const int       lane = threadIdx.x & 63;
+    const long long tile = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long npx  = (long long)g.w * g.h * g.nimg;
+    const long long p0   = tile * 16;
+    if (p0 >= npx) // whole wave exits together (npx is a multiple of 16)
         return;
     const unsigned wh_ = (unsigned)g.w * (unsigned)g.h;
-    const int img = (int)((unsigned)p / wh_);
-    const int rem = (int)((unsigned)p - (unsigned)img * wh_);
-    const int y = (int)((unsigned)rem / (unsigned)g.w), x = rem - y * g.w;
-    float     v[9];
+    const int img = (int)((unsigned)p0 / wh_);
+    const int rem = (int)((unsigned)p0 - (unsigned)img * wh_);
+    const int y = (int)((unsigned)rem / (unsigned)g.w), x0 = rem - y * g.w;
+    const int m = lane & 15, kg = lane >> 4;
+    v4s       a;
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
+    for (int v = 0; v < 4; ++v)
     {
-        const int q = nbr<PERIODIC>(x, y, t % 3 - 1, t / 3 - 1, g, img);
-        v[t]        = q < 0 ? 0.f : bf2f(xin[q]);
+        const int t = 4 * kg + v;
+        short     xv = 0;
+        if (t < 9)
+        {
+            const int q = nbr<PERIODIC>(x0 + m, y, t % 3 - 1, t / 3 - 1, g, img);
+            if (q >= 0)
+                xv = (short)xin[q];
+        }
+        a[v] = xv;
     }
-    unsigned short o[C];
+    v4f acc = { 0.f, 0.f, 0.f, 0.f };
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w1_operand(Wb1, m, kg), acc, 0, 0, 0);
 #pragma unroll
-    for (int n = 0; n < C; ++n)
+    for (int v = 0; v < 4; ++v)
     {
-        float s = 0.f;
-#pragma unroll
-        for (int t = 0; t < 9; ++t)
-            s += W1[t * C + n] * v[t];
-        o[n] = f2bf(s > 0.f ? s : 0.f);
+        const float s = acc[v];
+        hout[(p0 + 4 * kg + v) * C + m] = f2bf(s > 0.f ? s : 0.f);
     }
-    uint4 *dst = reinterpret_cast<uint4 *>(hout + p * C);
-    dst[0] = make_uint4(o[0] | (o[1] << 16), o[2] | (o[3] << 16), o[4] | (o[5] << 16),
-                        o[6] | (o[7] << 16));
-    dst[1] = make_uint4(o[8] | (o[9] << 16), o[10] | (o[11] << 16),
-                        o[12] | (o[13] << 16), o[14] | (o[15] << 16));
 }
 
 // layers 2, 3: h (bf16, 16) -> h' (bf16, 16, ReLU) on MFMA.
@@ -163,11 +202,30 @@ __global__ __launch_bounds__(256) void k_conv16(const unsigned short *__restrict
     }
 }
 
-// layer 4: h3 (bf16, 16) -> y (fp32, 1 channel)
+// layer 4, part 1: per-tap partials P[p][t] = sum_k W4[t][k] h3[p][k]
+__global__ __launch_bounds__(256) void k_conv4_taps(const unsigned short *__restrict__ hin,
+                                                    float *__restrict__ P,
+                                                    const unsigned short *__restrict__ Wb4,
+                                                    long long npx)
+{
+    const int       lane = threadIdx.x & 63;
+    const long long tile = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long p0   = tile * 16;
+    if (p0 >= npx)
+        return;
+    const int m = lane & 15, kg = lane >> 4;
+    const v4s a = *reinterpret_cast<const v4s *>(hin + (p0 + m) * C + 4 * kg);
+    v4f       acc = { 0.f, 0.f, 0.f, 0.f };
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w4_operand(Wb4, m, kg), acc, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        P[(p0 + 4 * kg + v) * C + m] = acc[v];
+}
+
+// layer 4, part 2: y[p] = sum over taps t (in order) of P[p + off_t][t]
 template <bool PERIODIC>
-__global__ __launch_bounds__(256) void k_conv4(const unsigned short *__restrict__ hin,
-                                               float *__restrict__ yout,
-                                               const float *__restrict__ W4, Geo g)
+__global__ __launch_bounds__(256) void k_conv4_sum(const float *__restrict__ P,
+                                                   float *__restrict__ yout, Geo g)
 {
     const long long npx = (long long)g.w * g.h * g.nimg;
     const long long p   = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -184,15 +242,7 @@ __global__ __launch_bounds__(256) void k_conv4(const unsigned short *__restrict_
         const int q = nbr<PERIODIC>(x, y, t % 3 - 1, t / 3 - 1, g, img);
         if (q < 0)
             continue;
-        const uint4 *src = reinterpret_cast<const uint4 *>(hin + (long long)q * C);
-        const uint4  u0 = src[0], u1 = src[1];
-        const unsigned int w8[8] = { u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w };
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-        {
-            s += W4[t * C + 2 * k] * bf2f((unsigned short)(w8[k] & 0xffff));
-            s += W4[t * C + 2 * k + 1] * bf2f((unsigned short)(w8[k] >> 16));
-        }
+        s += P[(long long)q * C + t];
     }
     yout[p] = s;
 }
@@ -630,6 +680,234 @@ __global__ void k_round_bf16(const float *__restrict__ w, float *__restrict__ wr
         wr[i] = bf2f(f2bf(w[i]));
 }
 
+// ---- fused inference: all four layers of one output tile in LDS --------
+// Output tile FT x FT.  Each layer is evaluated on the region its successor
+// needs (the halo shrinks by one per layer): x on (FT+8)^2, h1 on (FT+6)^2,
+// h2 on (FT+4)^2, h3 on (FT+2)^2, the layer-4 partials on (FT+2)^2.  Each
+// MFMA tile takes 16 consecutive pixels of the flattened region.  Activations
+// sit in LDS as four planes of 4 channels (8 bytes per pixel and plane), with
+// plane strides of 16 (mod 32) pixels, so the 8-byte A-operand reads of the
+// 64 lanes fall in distinct banks.  Zero-padded fields (not PERIODIC) store 0
+// for every activation outside the image, as the layered kernels read their
+// padding.
+constexpr int FT   = 28;
+constexpr int FX   = FT + 8; // 36
+constexpr int FR1  = FT + 6; // 34
+constexpr int FR2  = FT + 4; // 32
+constexpr int FR3  = FT + 2; // 30
+constexpr int NP1  = 1168;   // >= FR1^2, = 16 (mod 32)
+constexpr int NP2  = 1040;   // >= FR2^2
+constexpr int NP3  = 912;    // >= FR3^2
+static_assert(NP1 >= FR1 * FR1 && NP2 >= FR2 * FR2 && NP3 >= FR3 * FR3, "planes");
+constexpr int F_LDS = FX * FX * 2 + 4 * NP1 * 8 + 4 * NP2 * 8;
+static_assert(4 * NP3 * 8 <= 4 * NP1 * 8, "h3 aliases h1");
+static_assert(9 * NP3 * 4 <= 4 * NP2 * 8, "layer-4 partials alias h2");
+
+struct FusedIO
+{
+    const double *r;   // INFER: residual (FP64)
+    const float  *xin; // raw mode: network input (rounded to bf16 on load)
+    const double *ss;  // INFER: sigma (rms of r)
+    const double *D;   // INFER: operator diagonal
+    double       *e;   // INFER: correction
+    float        *y;   // raw mode: network output
+    double        omega;
+};
+
+__device__ inline int wrapi(int v, int n)
+{
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+// D of a 16 -> 16 or 1 -> 16 layer (lane: pixels 4kg+v of the tile,
+// channel m) with ReLU, into the planes of the WO x WO region at origin org
+template <bool PERIODIC, int WO, int NP>
+__device__ inline void store_act(unsigned short *__restrict__ hout, int ox, int oy, int org,
+                                 const Geo &g, int tile, const v4f &acc, int m, int kg)
+{
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+        const int po = tile * 16 + 4 * kg + v;
+        if (po >= WO * WO)
+            continue;
+        const float    sv = acc[v];
+        unsigned short h  = f2bf(sv > 0.f ? sv : 0.f);
+        if (!PERIODIC)
+        {
+            const int x = ox + org + po % WO, y = oy + org + po / WO;
+            if (x < 0 || y < 0 || x >= g.w || y >= g.h)
+                h = 0;
+        }
+        hout[((m >> 2) * NP + po) * 4 + (m & 3)] = h;
+    }
+}
+
+// A 16 -> 16 layer on the WO x WO region (input (WO+2)^2): 9 MFMAs (one per
+// tap) per 16 output pixels, two tiles interleaved per wave.
+template <bool PERIODIC, int WO, int NPI, int NPO>
+__device__ inline void conv16_tiles(const unsigned short *__restrict__ hin,
+                                    unsigned short *__restrict__ hout, int ox, int oy,
+                                    int org, const Geo &g,
+                                    const unsigned short *__restrict__ W, int wave, int m,
+                                    int kg)
+{
+    constexpr int WI = WO + 2, NPX = WO * WO, NT = (NPX + 15) / 16;
+    v4s           b[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+        b[t] = *reinterpret_cast<const v4s *>(W + (t * C + m) * C + 4 * kg);
+    for (int tile = wave; tile < NT; tile += 8)
+    {
+        const int  t2  = tile + 4;
+        const int  p1  = tile * 16 + m, p2 = t2 * 16 + m;
+        const bool a1v = p1 < NPX, a2v = p2 < NPX;
+        const int  o1  = a1v ? (p1 / WO) * WI + p1 % WO : 0;
+        const int  o2  = a2v ? (p2 / WO) * WI + p2 % WO : 0;
+        v4f        c1 = { 0.f, 0.f, 0.f, 0.f }, c2 = { 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+        {
+            const int d  = (t / 3) * WI + t % 3;
+            v4s       a1 = { 0, 0, 0, 0 }, a2 = { 0, 0, 0, 0 };
+            if (a1v)
+                a1 = *reinterpret_cast<const v4s *>(hin + (kg * NPI + o1 + d) * 4);
+            if (a2v)
+                a2 = *reinterpret_cast<const v4s *>(hin + (kg * NPI + o2 + d) * 4);
+            c1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b[t], c1, 0, 0, 0);
+            c2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b[t], c2, 0, 0, 0);
+        }
+        store_act<PERIODIC, WO, NPO>(hout, ox, oy, org, g, tile, c1, m, kg);
+        if (t2 < NT)
+            store_act<PERIODIC, WO, NPO>(hout, ox, oy, org, g, t2, c2, m, kg);
+    }
+}
+
+template <bool PERIODIC, bool INFER>
+__global__ __launch_bounds__(256) void k_net_fused(FusedIO io,
+                                                   const unsigned short *__restrict__ Wb, Geo g)
+{
+    __shared__ __align__(16) unsigned char smem[F_LDS];
+    unsigned short *sx  = reinterpret_cast<unsigned short *>(smem);
+    unsigned short *sh1 = sx + FX * FX;
+    unsigned short *sh2 = sh1 + 4 * NP1 * 4;
+    unsigned short *sh3 = sh1;                          // h1 is dead once h2 is formed
+    float          *sP  = reinterpret_cast<float *>(sh2); // h2 is dead once h3 is formed
+    const int ox = blockIdx.x * FT, oy = blockIdx.y * FT, img = blockIdx.z;
+    const int tid = threadIdx.x;
+    const long long ibase = (long long)img * g.h * g.w;
+    auto inside = [&](int x, int y) { return x >= 0 && y >= 0 && x < g.w && y < g.h; };
+
+    // x = bf16(r / sigma) on [-4, FT+4)^2
+    double inv = 1.0;
+    if (INFER)
+        inv = io.ss[0] > 0 ? 1.0 / io.ss[0] : 0.0;
+    const bool big = g.w >= FX && g.h >= FX;
+    for (int q = tid; q < FX * FX; q += blockDim.x)
+    {
+        const int i = q % FX, j = q / FX;
+        int       x = ox - 4 + i, y = oy - 4 + j;
+        unsigned short v = 0;
+        if (PERIODIC || inside(x, y))
+        {
+            if (PERIODIC)
+            {
+                if (big) // the region wraps at most once
+                {
+                    x = x < 0 ? x + g.w : (x >= g.w ? x - g.w : x);
+                    y = y < 0 ? y + g.h : (y >= g.h ? y - g.h : y);
+                }
+                else
+                {
+                    x = wrapi(x, g.w);
+                    y = wrapi(y, g.h);
+                }
+            }
+            const long long p = ibase + (long long)y * g.w + x;
+            v = INFER ? f2bf((float)(io.r[p] * inv)) : f2bf(io.xin[p]);
+        }
+        sx[q] = v;
+    }
+    __syncthreads();
+
+    const int lane = tid & 63, wave = tid >> 6, m = lane & 15, kg = lane >> 4;
+    // layer 1 on [-3, FT+3)^2: A = the 9 taps of x at the pixel
+    {
+        const v4s b = w1_operand(Wb + OFF1, m, kg);
+        for (int tile = wave; tile * 16 < FR1 * FR1; tile += 4)
+        {
+            const int pm = tile * 16 + m;
+            v4s       a  = { 0, 0, 0, 0 };
+            if (pm < FR1 * FR1)
+            {
+                const int im = pm % FR1, jm = pm / FR1;
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int t = 4 * kg + v;
+                    a[v] = t < 9 ? (short)sx[(jm + t / 3) * FX + im + t % 3] : (short)0;
+                }
+            }
+            v4f acc = { 0.f, 0.f, 0.f, 0.f };
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            store_act<PERIODIC, FR1, NP1>(sh1, ox, oy, -3, g, tile, acc, m, kg);
+        }
+    }
+    __syncthreads();
+
+    // layers 2 and 3
+    conv16_tiles<PERIODIC, FR2, NP1, NP2>(sh1, sh2, ox, oy, -2, g, Wb + OFF2, wave, m, kg);
+    __syncthreads();
+    conv16_tiles<PERIODIC, FR3, NP2, NP3>(sh2, sh3, ox, oy, -1, g, Wb + OFF3, wave, m, kg);
+    __syncthreads();
+
+    // layer 4, part 1: per-tap partials on [-1, FT+1)^2
+    {
+        const v4s b = w4_operand(Wb + OFF4, m, kg);
+        for (int tile = wave; tile * 16 < FR3 * FR3; tile += 4)
+        {
+            const int pm = tile * 16 + m;
+            v4s       a  = { 0, 0, 0, 0 };
+            if (pm < FR3 * FR3)
+                a = *reinterpret_cast<const v4s *>(sh3 + (kg * NP3 + pm) * 4);
+            v4f acc = { 0.f, 0.f, 0.f, 0.f };
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            if (m < 9)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                {
+                    const int po = tile * 16 + 4 * kg + v;
+                    if (po < FR3 * FR3)
+                        sP[m * NP3 + po] = acc[v];
+                }
+        }
+    }
+    __syncthreads();
+
+    // layer 4, part 2: y = sum over taps of the partials at the neighbours
+    for (int q = tid; q < FT * FT; q += blockDim.x)
+    {
+        const int i = q % FT, j = q / FT;
+        const int x = ox + i, y = oy + j;
+        if (x >= g.w || y >= g.h)
+            continue;
+        float s = 0.f;
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+        {
+            if (!PERIODIC && !inside(x + t % 3 - 1, y + t / 3 - 1))
+                continue;
+            s += sP[t * NP3 + (j + t / 3) * FR3 + i + t % 3];
+        }
+        const long long p = ibase + (long long)y * g.w + x;
+        if (INFER)
+            io.e[p] = io.omega * io.ss[0] * (double)s / io.D[p];
+        else
+            io.y[p] = s;
+    }
+}
+
 } // namespace
 
 // =========================================================== network state
@@ -649,7 +927,7 @@ struct bx_cnn
     // inference buffers
     long long       cap = 0, epoch = 0;
     unsigned short *x = nullptr, *h1 = nullptr, *h2 = nullptr, *h3 = nullptr;
-    float          *y = nullptr;
+    float          *y = nullptr, *P = nullptr; // P: layer-4 partials, 16 per pixel
     double         *sig = nullptr, *part = nullptr;
     // training buffers
     long long tcap = 0;
@@ -690,7 +968,7 @@ extern "C" void bx_cnn_destroy(bx_cnn *c)
     for (void *p : { (void *)c->w, (void *)c->wr, (void *)c->wb, (void *)c->grad,
                      (void *)c->adam_m, (void *)c->adam_v, (void *)c->D,
                      (void *)c->x, (void *)c->h1, (void *)c->h2, (void *)c->h3,
-                     (void *)c->y, (void *)c->sig, (void *)c->part, (void *)c->gy,
+                     (void *)c->y, (void *)c->P, (void *)c->sig, (void *)c->part, (void *)c->gy,
                      (void *)c->g3, (void *)c->g2, (void *)c->g1, (void *)c->te,
                      (void *)c->tres, (void *)c->tsig, (void *)c->trn2,
                      (void *)c->tloss, (void *)c->txy })
@@ -794,15 +1072,16 @@ static int ensure_infer(bx_cnn *c, long long npx)
     if (npx <= c->cap)
         return BX_OK;
     for (void *p : { (void *)c->x, (void *)c->h1, (void *)c->h2, (void *)c->h3,
-                     (void *)c->y })
+                     (void *)c->y, (void *)c->P })
         cfree(p);
     c->x = c->h1 = c->h2 = c->h3 = nullptr;
-    c->y = nullptr;
+    c->y = c->P = nullptr;
     C_CHECK(hipMalloc((void **)&c->x, npx * 2));
     C_CHECK(hipMalloc((void **)&c->h1, npx * C * 2));
     C_CHECK(hipMalloc((void **)&c->h2, npx * C * 2));
     C_CHECK(hipMalloc((void **)&c->h3, npx * C * 2));
     C_CHECK(hipMalloc((void **)&c->y, npx * 4));
+    C_CHECK(hipMalloc((void **)&c->P, npx * C * 4));
     c->cap = npx;
     ++c->epoch;
     return BX_OK;
@@ -845,20 +1124,27 @@ extern "C" int bx_cnn_set_operator(bx_cnn *c, int nx, int ny, double alpha,
 
 
 
+static dim3 fused_grid(const Geo &g)
+{
+    return dim3((unsigned)((g.w + FT - 1) / FT), (unsigned)((g.h + FT - 1) / FT),
+                (unsigned)g.nimg);
+}
+
 template <bool PER>
 static void net_forward(bx_cnn *c, const Geo &g, hipStream_t s)
 {
     const long long npx = (long long)g.w * g.h * g.nimg;
     const unsigned  nb  = (unsigned)((npx + 255) / 256);
-    hipLaunchKernelGGL((k_conv1<PER>), dim3(nb), dim3(256), 0, s, c->x, c->h1,
-                       c->wr + OFF1, g);
-    const unsigned nbt = (unsigned)((npx / 16 * 64 + 255) / 256);
+    const unsigned  nbt = (unsigned)((npx / 16 * 64 + 255) / 256);
+    hipLaunchKernelGGL((k_conv1<PER>), dim3(nbt), dim3(256), 0, s, c->x, c->h1,
+                       c->wb + OFF1, g);
     hipLaunchKernelGGL((k_conv16<PER>), dim3(nbt), dim3(256), 0, s, c->h1, c->h2,
                        c->wb + OFF2, g);
     hipLaunchKernelGGL((k_conv16<PER>), dim3(nbt), dim3(256), 0, s, c->h2, c->h3,
                        c->wb + OFF3, g);
-    hipLaunchKernelGGL((k_conv4<PER>), dim3(nb), dim3(256), 0, s, c->h3, c->y,
-                       c->wr + OFF4, g);
+    hipLaunchKernelGGL(k_conv4_taps, dim3(nbt), dim3(256), 0, s, c->h3, c->P,
+                       c->wb + OFF4, npx);
+    hipLaunchKernelGGL((k_conv4_sum<PER>), dim3(nb), dim3(256), 0, s, c->P, c->y, g);
 }
 
 int bx_cnn_forward(bx_cnn *c, const double *r, double *e, int nx, int ny,
@@ -872,10 +1158,9 @@ int bx_cnn_forward(bx_cnn *c, const double *r, double *e, int nx, int ny,
         return rc;
     hipLaunchKernelGGL(k_sumsq, dim3(512), dim3(256), 0, s, r, n, c->part);
     hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, s, c->part, 512, n, c->sig);
-    hipLaunchKernelGGL(k_prep_full, dim3(2048), dim3(256), 0, s, r, c->x, n, c->sig);
-    net_forward<true>(c, Geo{ nx, ny, 1 }, s);
-    hipLaunchKernelGGL(k_finish_full, dim3(2048), dim3(256), 0, s, c->y, c->D, e, n,
-                       omega, c->sig);
+    FusedIO io{ r, nullptr, c->sig, c->D, e, nullptr, omega };
+    hipLaunchKernelGGL((k_net_fused<true, true>), fused_grid(Geo{ nx, ny, 1 }), dim3(256),
+                       0, s, io, c->wb, Geo{ nx, ny, 1 });
     C_CHECK(hipGetLastError());
     return BX_OK;
 }
@@ -889,21 +1174,41 @@ extern "C" int bx_cnn_infer(bx_cnn *c, const double *r, double *e, int nx, int n
 // Raw network on a batch of bf16-quantised inputs (zero padding): y = N(x).
 // x_in: float host-side values are rounded to bf16 first.  For tests.
 extern "C" int bx_cnn_net_forward(bx_cnn *c, const float *x_dev, float *y_dev, int w,
-                                  int h, int nimg, int periodic, void *stream)
+                                  int h, int nimg, int flags, void *stream)
 {
-    if (!c || !x_dev || !y_dev || w % 16)
+    if (!c || !x_dev || !y_dev || w < 1 || h < 1 || nimg < 1)
         return cerr(BX_ERR_INVALID, "bx_cnn_net_forward: bad argument");
+    const bool      periodic = (flags & BX_CNN_PERIODIC) != 0;
+    const bool      layered  = (flags & BX_CNN_LAYERED) != 0;
     hipStream_t     s = (hipStream_t)stream;
     const long long n = (long long)w * h * nimg;
+    const Geo       g{ w, h, nimg };
     int             rc;
+    if (!layered)
+    {
+        if (n > 0x7fffffffLL)
+            return cerr(BX_ERR_INVALID, "network: more than 2^31 pixels in one launch");
+        FusedIO io{ nullptr, x_dev, nullptr, nullptr, nullptr, y_dev, 0.0 };
+        if (periodic)
+            hipLaunchKernelGGL((k_net_fused<true, false>), fused_grid(g), dim3(256), 0, s,
+                               io, c->wb, g);
+        else
+            hipLaunchKernelGGL((k_net_fused<false, false>), fused_grid(g), dim3(256), 0, s,
+                               io, c->wb, g);
+        C_CHECK(hipGetLastError());
+        C_CHECK(hipStreamSynchronize(s));
+        return BX_OK;
+    }
+    if (w % 16)
+        return cerr(BX_ERR_INVALID, "bx_cnn_net_forward: layered kernels need w % 16 == 0");
     if ((rc = ensure_infer(c, n)))
         return rc;
     hipLaunchKernelGGL(k_pack_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
                        x_dev, c->x, (int)n);
     if (periodic)
-        net_forward<true>(c, Geo{ w, h, nimg }, s);
+        net_forward<true>(c, g, s);
     else
-        net_forward<false>(c, Geo{ w, h, nimg }, s);
+        net_forward<false>(c, g, s);
     C_CHECK(hipMemcpyAsync(y_dev, c->y, n * 4, hipMemcpyDeviceToDevice, s));
     C_CHECK(hipStreamSynchronize(s));
     return BX_OK;

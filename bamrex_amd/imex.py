@@ -298,12 +298,14 @@ class LearnedSmoother:
         _chk(lib().bx_cnn_set_params(self.c, w.ctypes.data, self.NPARAMS), "bx_cnn_set_params",
              "bx_cnn_last_error")
 
-    def net_forward(self, x, periodic=False):
-        """y = N(x) for x of shape (nimg, h, w) float32 (rounded to bf16)."""
+    def net_forward(self, x, periodic=False, layered=False):
+        """y = N(x) for x of shape (nimg, h, w) float32 (rounded to bf16):
+        the fused inference kernel, or the layer-by-layer training kernels."""
         x = np.ascontiguousarray(x, np.float32)
         nimg, hh, ww = x.shape
         dx_, dy_ = dev(x, np.float32), DeviceArray(x.shape, np.float32)
-        _chk(lib().bx_cnn_net_forward(self.c, dx_.ptr, dy_.ptr, ww, hh, nimg, int(periodic), None),
+        flags = (1 if periodic else 0) | (2 if layered else 0)
+        _chk(lib().bx_cnn_net_forward(self.c, dx_.ptr, dy_.ptr, ww, hh, nimg, flags, None),
              "bx_cnn_net_forward", "bx_cnn_last_error")
         return dy_.download()
 

@@ -315,11 +315,16 @@ int bx_cnn_set_operator(bx_cnn *c, int nx, int ny, double alpha, double beta,
 /* e = omega (sigma/D) N(r/sigma) on the registered periodic level. */
 int bx_cnn_infer(bx_cnn *c, const double *r, double *e, int nx, int ny,
                  double omega, void *stream);
-/* y = N(x) on nimg images of w x h (w multiple of 16), x rounded to BF16,
- * zero padding (periodic = 0) or wrap-around.  Device FP32 arrays.
+/* y = N(x) on nimg images of w x h, x rounded to BF16, zero padding or
+ * wrap-around (BX_CNN_PERIODIC).  Device FP32 arrays.  By default the fused
+ * inference kernel runs (all four layers per output tile in LDS, the kernel
+ * bx_cnn_infer uses); BX_CNN_LAYERED runs the layer-by-layer kernels of the
+ * training forward instead (w multiple of 16).  Both give identical results.
  * Synchronous. */
+#define BX_CNN_PERIODIC 1
+#define BX_CNN_LAYERED 2
 int bx_cnn_net_forward(bx_cnn *c, const float *x, float *y, int w, int h,
-                       int nimg, int periodic, void *stream);
+                       int nimg, int flags, void *stream);
 /* Loss and gradient on crops of the residual fields R (nfields x ny x nx,
  * device): crop b is the crop x crop window at (x0, y0) of field f, with
  * crops_host[3b..3b+2] = {f, x0, y0} (periodic wrap).  Loss per crop:

@@ -151,6 +151,24 @@ def test_cnn_forward_exact_case_bit_exact(periodic):
     net.close()
 
 
+@pytest.mark.parametrize("shape,periodic", [((2, 37, 48), False), ((2, 37, 48), True),
+                                            ((1, 80, 96), False), ((1, 5, 16), True),
+                                            ((3, 30, 32), False)])
+def test_cnn_fused_forward_equals_layered(shape, periodic):
+    """The fused inference kernel (all layers of a tile in LDS) gives the
+    same bits as the layer-by-layer kernels, for zero padding and wrap-around,
+    partial tiles, several images, and images smaller than a tile's halo."""
+    announce("cnn fused vs layered %s periodic=%s" % (shape, periodic))
+    rng = np.random.default_rng(sum(shape) + periodic)
+    net = LearnedSmoother(seed=3)
+    net.set_params((rng.normal(size=co.NPARAMS) * 0.3).astype(np.float32))
+    x = rng.normal(size=shape).astype(np.float32)
+    yf = net.net_forward(x, periodic=periodic)
+    yl = net.net_forward(x, periodic=periodic, layered=True)
+    assert np.array_equal(yf, yl)
+    net.close()
+
+
 def test_cnn_forward_random_weights():
     announce("cnn forward random 2x48x64")
     rng = np.random.default_rng(5)

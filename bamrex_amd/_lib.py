@@ -12,7 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("BAMREX_LIB") or os.path.join(_HERE, "libbamrex_hip.so")
 
 BX_OK = 0
-METHOD = {"muscl_hancock": 0, "hllc": 1}
+METHOD = {"muscl_hancock": 0, "hllc": 1, "slic": 2, "force": 3, "lax_friedrichs": 4}
 LIMITER = {"minbee": 0, "van_leer": 1, "van_albada": 2, "superbee": 3}
 BC = {"transmissive": 0, "periodic": 1, "reflective": 2}
 

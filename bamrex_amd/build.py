@@ -20,7 +20,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 
-HIP_SOURCES = ["bx_explicit.hip", "bx_api.hip", "bx_helmholtz.hip", "bx_cnn.hip"]
+HIP_SOURCES = ["bx_explicit.hip", "bx_sweep2d.hip", "bx_sweep3d.hip", "bx_api.hip",
+               "bx_helmholtz.hip", "bx_cnn.hip"]
 HOST_SOURCES = ["ParmParse.cpp", "AmrLevelAdv.cpp", "Prob.cpp", "main.cpp"]
 HOST_BIN = os.path.join(HERE, "bamrex_run")
 
@@ -56,7 +57,19 @@ def _headers():
         os.path.join(ROOT, "include", "bamrex_hip.h")]
 
 
-def build(force=True, jobs=4, verbose=False, defines=(), lib_path=None, obj_dir=None,
+def _included_sources(src):
+    """Sources a translation unit pulls in with #include "x.hip" (the
+    per-dimension sweep units include the explicit-scheme source)."""
+    out = []
+    with open(src) as f:
+        for line in f:
+            line = line.strip()
+            if line.startswith("#include") and '.hip"' in line:
+                out.append(os.path.join(os.path.dirname(src), line.split('"')[1]))
+    return out
+
+
+def build(force=True, jobs=6, verbose=False, defines=(), lib_path=None, obj_dir=None,
           host=True):
     """Compile every HIP source for gfx950 and link libbamrex_hip.so.
 
@@ -75,7 +88,7 @@ def build(force=True, jobs=4, verbose=False, defines=(), lib_path=None, obj_dir=
         s = os.path.join(CSRC, src)
         o = os.path.join(obj, src.replace(".hip", ".o"))
         objs.append(o)
-        if force or _stale(o, [s] + hdrs):
+        if force or _stale(o, [s] + _included_sources(s) + hdrs):
             jobs_list.append([HIPCC] + flags + ["-c", s, "-o", o])
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         for out in ex.map(_run, jobs_list):

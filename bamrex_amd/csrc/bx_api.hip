@@ -40,12 +40,13 @@ static int set_error(int code, const std::string &msg)
 extern "C" const char *bx_last_error(void) { return g_last_error.c_str(); }
 
 // ------------------------------------------------------ per-box kernels
-// Drop-in of compute_MUSCL_hancock_flux / compute_HLLC_flux for one box:
-// one thread per face of surroundingNodes(bx, dir).
-template <int D, int DIR, int LIM, int LV, bool FO>
+// Drop-in of compute_MUSCL_hancock_flux / compute_HLLC_flux /
+// compute_SLIC_flux / compute_force_flux / compute_LF_flux for one box: one
+// thread per face of surroundingNodes(bx, dir).
+template <int D, int DIR, int LIM, int LV, int SCH>
 __global__ __launch_bounds__(256) void k_flux_box(
     const double *__restrict__ U, DevBox bx, int ng, double *__restrict__ flux,
-    double hdtdx, EulerParm pm, LimTable lt)
+    double hdtdx, double fc, double rc, EulerParm pm, LimTable lt)
 {
     constexpr int NC = D + 2;
     const int     fx = bx.hi[0] - bx.lo[0] + 1 + (DIR == 0);
@@ -71,9 +72,9 @@ __global__ __launch_bounds__(256) void k_flux_box(
             for (int n = 0; n < NC; ++n)
                 W[s][n] = U[c + (s - 2) * sd + n * bx.nstride];
         double F[NC];
-        if (FO)
+        if (!scheme_reconstructs(SCH))
         {
-            hllc_flux<D, DIR>(W[1], W[2], F, pm);
+            face_flux<D, DIR, false, false, SCH>(W[1], W[2], F, pm, fc, rc);
         }
         else
         {
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(256) void k_flux_box(
             reconstruct_half_step<D, DIR, LIM, LV>(W[1], W[2], W[3], hL1, hR1,
                                                    hdtdx, pm, lt);
             // swapped arrays: left state = half_R(c-1), right = half_L(c)
-            hllc_flux<D, DIR>(hR0, hL1, F, pm);
+            face_flux<D, DIR, false, false, SCH>(hR0, hL1, F, pm, fc, rc);
         }
         const long long fvol = nf;
 #pragma unroll
@@ -225,10 +226,10 @@ extern "C" int bx_create(const bx_params *params, bx_ctx **out)
     const bx_params &p = *params;
     if (p.dim < 1 || p.dim > 3)
         return set_error(BX_ERR_INVALID, "bx_create: dim must be 1, 2 or 3");
-    if (p.num_method != BX_METHOD_MUSCL_HANCOCK && p.num_method != BX_METHOD_HLLC)
+    if (p.num_method < BX_METHOD_MUSCL_HANCOCK || p.num_method > BX_METHOD_LAX_FRIEDRICHS)
         return set_error(BX_ERR_INVALID,
                          "Invalid numerical method: valid explicit options are "
-                         "muscl_hancock and hllc");
+                         "muscl_hancock, hllc, slic, force and lax_friedrichs");
     if (p.limiter < BX_LIMITER_MINBEE || p.limiter > BX_LIMITER_SUPERBEE)
         return set_error(BX_ERR_INVALID, "bx_create: invalid limiter");
     if (!(p.adiabatic > 1.0))
@@ -705,7 +706,7 @@ static int do_sweep(bx_ctx *c, int dir, const double dx[3], const double *Uin,
     }
     if (L.nitems == 0)
         return BX_OK;
-    BX_HIP_CHECK(launch_sweep(c->D, dir, c->p.num_method == BX_METHOD_HLLC,
+    BX_HIP_CHECK(launch_sweep(c->D, dir, c->p.num_method,
                               c->p.limiter, c->lv_mode, L));
     return BX_OK;
 }
@@ -783,45 +784,66 @@ extern "C" int bx_sweep(bx_ctx *c, int dir, double dt, const double dx[3],
     return do_sweep(c, dir, dx, U_in, U_out, nullptr, s);
 }
 
-template <int D, int DIR, int LIM, int LV, bool FO>
+template <int D, int DIR, int LIM, int LV, int SCH>
 static hipError_t launch_flux_box(const double *U, const DevBox &bx, int ng,
-                                  double *flux, double hdtdx, EulerParm pm,
-                                  LimTable lt, hipStream_t s)
+                                  double *flux, double hdtdx, double fc, double rc,
+                                  EulerParm pm, LimTable lt, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_flux_box<D, DIR, LIM, LV, FO>), dim3(256), dim3(256),
-                       0, s, U, bx, ng, flux, hdtdx, pm, lt);
+    hipLaunchKernelGGL((k_flux_box<D, DIR, LIM, LV, SCH>), dim3(256), dim3(256),
+                       0, s, U, bx, ng, flux, hdtdx, fc, rc, pm, lt);
     return hipGetLastError();
 }
 
-template <int D, int DIR>
-static hipError_t flux_box_lim(int lim, int lv, bool fo, const double *U,
-                               const DevBox &bx, int ng, double *flux,
-                               double hdtdx, EulerParm pm, LimTable lt,
-                               hipStream_t s)
+template <int D, int DIR, int SCH>
+static hipError_t flux_box_lim(int lim, int lv, const double *U, const DevBox &bx,
+                               int ng, double *flux, double hdtdx, double fc,
+                               double rc, EulerParm pm, LimTable lt, hipStream_t s)
 {
 #define BX_FB(LIMV)                                                            \
     do                                                                         \
     {                                                                          \
         if (lv == LV_SHARED)                                                   \
-            return launch_flux_box<D, DIR, LIMV, LV_SHARED, false>(            \
-                U, bx, ng, flux, hdtdx, pm, lt, s);                            \
+            return launch_flux_box<D, DIR, LIMV, LV_SHARED, SCH>(              \
+                U, bx, ng, flux, hdtdx, fc, rc, pm, lt, s);                    \
         if (lv == LV_OWN)                                                      \
-            return launch_flux_box<D, DIR, LIMV, LV_OWN, false>(               \
-                U, bx, ng, flux, hdtdx, pm, lt, s);                            \
-        return launch_flux_box<D, DIR, LIMV, LV_TABLE, false>(                 \
-            U, bx, ng, flux, hdtdx, pm, lt, s);                                \
+            return launch_flux_box<D, DIR, LIMV, LV_OWN, SCH>(                 \
+                U, bx, ng, flux, hdtdx, fc, rc, pm, lt, s);                    \
+        return launch_flux_box<D, DIR, LIMV, LV_TABLE, SCH>(                   \
+            U, bx, ng, flux, hdtdx, fc, rc, pm, lt, s);                        \
     } while (0)
-    if (fo)
-        return launch_flux_box<D, DIR, LIM_MINBEE, LV_SHARED, true>(
-            U, bx, ng, flux, hdtdx, pm, lt, s);
-    switch (lim)
-    {
-    case LIM_MINBEE: BX_FB(LIM_MINBEE);
-    case LIM_VAN_LEER: BX_FB(LIM_VAN_LEER);
-    case LIM_VAN_ALBADA: BX_FB(LIM_VAN_ALBADA);
-    default: BX_FB(LIM_SUPERBEE);
-    }
+    if constexpr (!scheme_reconstructs(SCH))
+        return launch_flux_box<D, DIR, LIM_MINBEE, LV_SHARED, SCH>(
+            U, bx, ng, flux, hdtdx, fc, rc, pm, lt, s);
+    else
+        switch (lim)
+        {
+        case LIM_MINBEE: BX_FB(LIM_MINBEE);
+        case LIM_VAN_LEER: BX_FB(LIM_VAN_LEER);
+        case LIM_VAN_ALBADA: BX_FB(LIM_VAN_ALBADA);
+        default: BX_FB(LIM_SUPERBEE);
+        }
 #undef BX_FB
+}
+
+template <int D, int DIR>
+static hipError_t flux_box_scheme(int sch, int lim, int lv, const double *U,
+                                  const DevBox &bx, int ng, double *flux, double hdtdx,
+                                  double fc, double rc, EulerParm pm, LimTable lt,
+                                  hipStream_t s)
+{
+    switch (sch)
+    {
+    case SCH_MHM:
+        return flux_box_lim<D, DIR, SCH_MHM>(lim, lv, U, bx, ng, flux, hdtdx, fc, rc, pm, lt, s);
+    case SCH_HLLC:
+        return flux_box_lim<D, DIR, SCH_HLLC>(lim, lv, U, bx, ng, flux, hdtdx, fc, rc, pm, lt, s);
+    case SCH_SLIC:
+        return flux_box_lim<D, DIR, SCH_SLIC>(lim, lv, U, bx, ng, flux, hdtdx, fc, rc, pm, lt, s);
+    case SCH_FORCE:
+        return flux_box_lim<D, DIR, SCH_FORCE>(lim, lv, U, bx, ng, flux, hdtdx, fc, rc, pm, lt, s);
+    default:
+        return flux_box_lim<D, DIR, SCH_LF>(lim, lv, U, bx, ng, flux, hdtdx, fc, rc, pm, lt, s);
+    }
 }
 
 extern "C" int bx_flux_box(bx_ctx *c, int ibox, int dir, double dt,
@@ -834,21 +856,27 @@ extern "C" int bx_flux_box(bx_ctx *c, int ibox, int dir, double dt,
         return set_error(BX_ERR_INVALID, "bx_flux_box: bad argument");
     const DevBox &bx    = c->hdev[ibox];
     const double  hdtdx = 0.5 * dt / dx[dir];
+    const double  fc    = 0.5 * dx[dir] / dt; // FORCE / LF
+    const double  rc    = dt / (2 * dx[dir]); // FORCE
     double       *flux  = flux_base + flux_offset;
-    const bool    fo    = c->p.num_method == BX_METHOD_HLLC;
+    const int     sch   = c->p.num_method;
     hipStream_t   s     = (hipStream_t)stream;
     hipError_t    e     = hipErrorInvalidValue;
     const int     lim = c->p.limiter, lv = c->lv_mode, ng = c->geom.ng;
+#define BX_FBS(DD, DIR)                                                         \
+    flux_box_scheme<DD, DIR>(sch, lim, lv, U, bx, ng, flux, hdtdx, fc, rc, c->pm, \
+                             c->lt, s)
     switch (c->D * 4 + dir)
     {
-    case 4: e = flux_box_lim<1, 0>(lim, lv, fo, U, bx, ng, flux, hdtdx, c->pm, c->lt, s); break;
-    case 8: e = flux_box_lim<2, 0>(lim, lv, fo, U, bx, ng, flux, hdtdx, c->pm, c->lt, s); break;
-    case 9: e = flux_box_lim<2, 1>(lim, lv, fo, U, bx, ng, flux, hdtdx, c->pm, c->lt, s); break;
-    case 12: e = flux_box_lim<3, 0>(lim, lv, fo, U, bx, ng, flux, hdtdx, c->pm, c->lt, s); break;
-    case 13: e = flux_box_lim<3, 1>(lim, lv, fo, U, bx, ng, flux, hdtdx, c->pm, c->lt, s); break;
-    case 14: e = flux_box_lim<3, 2>(lim, lv, fo, U, bx, ng, flux, hdtdx, c->pm, c->lt, s); break;
+    case 4: e = BX_FBS(1, 0); break;
+    case 8: e = BX_FBS(2, 0); break;
+    case 9: e = BX_FBS(2, 1); break;
+    case 12: e = BX_FBS(3, 0); break;
+    case 13: e = BX_FBS(3, 1); break;
+    case 14: e = BX_FBS(3, 2); break;
     default: break;
     }
+#undef BX_FBS
     BX_HIP_CHECK(e);
     return BX_OK;
 }

@@ -71,6 +71,16 @@ __device__ __forceinline__ unsigned long long umax64(unsigned long long a,
     return a > b ? a : b;
 }
 
+// Running maximum of non-negative wave speeds as ordered bits.  A NaN speed
+// (negative pressure) is skipped, as the reference's reduction skips it:
+// amrex::max(acc, s) = (acc < s) ? s : acc keeps acc when s is NaN
+// (max_wave_speed, Euler.cpp:53-68).
+__device__ __forceinline__ unsigned long long speed_max(unsigned long long acc,
+                                                        double s)
+{
+    return s == s ? umax64(acc, (unsigned long long)__double_as_longlong(s)) : acc;
+}
+
 // Contiguous item range of this wave, XCD-aware: blocks are dealt to the 8
 // XCDs round-robin, so block b runs on XCD b%8; renumber so that XCD x owns
 // a contiguous slice of the item space.
@@ -177,7 +187,7 @@ __device__ __forceinline__ void x_item(const DevBox *__restrict__ boxes,
 constexpr int PM_NI = 1;
 constexpr int PM_E1 = 2;
 
-template <int D, int LIM, int LV, bool FIRST_ORDER, int PM>
+template <int D, int LIM, int LV, int SCH, int PM>
 __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     const double *__restrict__ Uin, double *__restrict__ Uout,
     const DevBox *__restrict__ boxes, const long long *__restrict__ ibeg,
@@ -191,6 +201,9 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     const double   dt    = *d_dt;
     const double   hdtdx = 0.5 * dt / dx;
     const double   dtdx  = dt / dx;
+    const double   fc    = 0.5 * dx / dt;   // FORCE / LF
+    const double   rc    = dt / (2 * dx);   // FORCE
+    constexpr bool RECON = scheme_reconstructs(SCH);
 
     long long first, last;
     wave_item_range(nitems, first, last);
@@ -226,9 +239,9 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
 #pragma unroll
         for (int n = 0; n < NC; ++n)
             Um[n] = wave_shr1(U[n]);
-        if (FIRST_ORDER)
+        if (!RECON)
         {
-            hllc_flux<D, 0, E1, NI>(Um, U, F, pm);
+            face_flux<D, 0, E1, NI, SCH>(Um, U, F, pm, fc, rc);
         }
         else
         {
@@ -241,7 +254,7 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
 #pragma unroll
             for (int n = 0; n < NC; ++n)
                 hRm[n] = wave_shr1(hR[n]);
-            hllc_flux<D, 0, E1, NI>(hRm, hL, F, pm);
+            face_flux<D, 0, E1, NI, SCH>(hRm, hL, F, pm, fc, rc);
         }
         double Fp[NC];
 #pragma unroll
@@ -264,7 +277,7 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
             if (d_maxbits)
             {
                 const double sp = cell_wave_speed<D, E1>(Unew, pm);
-                vmax = umax64(vmax, __double_as_longlong(sp));
+                vmax = speed_max(vmax, sp);
             }
         }
     }
@@ -273,7 +286,7 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
 }
 
 // -------------------------------------------------------------- y/z sweep
-template <int D, int DIR, int LIM, int LV, bool FIRST_ORDER, int PM>
+template <int D, int DIR, int LIM, int LV, int SCH, int PM>
 __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
     const double *__restrict__ Uin, double *__restrict__ Uout,
     const DevBox *__restrict__ boxes, const long long *__restrict__ ibeg,
@@ -287,6 +300,9 @@ __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
     const double   dt    = *d_dt;
     const double   hdtdx = 0.5 * dt / dx;
     const double   dtdx  = dt / dx;
+    const double   fc    = 0.5 * dx / dt;   // FORCE / LF
+    const double   rc    = dt / (2 * dx);   // FORCE
+    constexpr bool RECON = scheme_reconstructs(SCH);
     const int      ngy   = (D >= 2) ? ng : 0;
     const int      ngz   = (D >= 3) ? ng : 0;
 
@@ -350,7 +366,7 @@ __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
         load(s0 - 1, Um1);
         load(s0, U0);
         load(s0 + 1, Up);
-        if (!FIRST_ORDER)
+        if (RECON)
         {
             double Um2[NC], hLd[NC];
             load(s0 - 2, Um2);
@@ -364,16 +380,16 @@ __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
             if (s < s1)
                 load(s + 2, Unext);
             double F[NC];
-            if (FIRST_ORDER)
+            if (!RECON)
             {
-                hllc_flux<D, DIR, E1, NI>(Um1, U0, F, pm);
+                face_flux<D, DIR, E1, NI, SCH>(Um1, U0, F, pm, fc, rc);
             }
             else
             {
                 double hL0[NC], hR0[NC];
                 reconstruct_half_step<D, DIR, LIM, LV, E1>(Um1, U0, Up, hL0,
                                                            hR0, hdtdx, pm, lt);
-                hllc_flux<D, DIR, E1, NI>(hRprev, hL0, F, pm);
+                face_flux<D, DIR, E1, NI, SCH>(hRprev, hL0, F, pm, fc, rc);
 #pragma unroll
                 for (int n = 0; n < NC; ++n)
                     hRprev[n] = hR0[n];
@@ -392,7 +408,7 @@ __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
                 if (d_maxbits)
                 {
                     const double sp = cell_wave_speed<D, E1>(Un, pm);
-                    vmax = umax64(vmax, __double_as_longlong(sp));
+                    vmax = speed_max(vmax, sp);
                 }
             }
 #pragma unroll
@@ -412,6 +428,7 @@ __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
         wave_max_to_global(vmax, d_maxbits);
 }
 
+#ifndef BX_SWEEP_PART // kernels outside the per-dimension sweep instantiations
 __device__ void step_tail(const StepTail &t);
 
 // ------------------------------------------------------------- ghost fill
@@ -492,7 +509,7 @@ __global__ __launch_bounds__(256) void k_max_wave_speed(
 #pragma unroll
         for (int n = 0; n < NC; ++n)
             V[n] = U[o + n * bx.nstride];
-        vmax = umax64(vmax, __double_as_longlong(cell_wave_speed<D>(V, pm)));
+        vmax = speed_max(vmax, cell_wave_speed<D>(V, pm));
     }
     wave_max_to_global(vmax, d_maxbits);
 }
@@ -564,94 +581,154 @@ __global__ void k_set_u64(unsigned long long *p, unsigned long long v)
         *p = v;
 }
 
+#endif // BX_SWEEP_PART
+
 // ------------------------------------------------------------ launchers
-template <int D, int LIM, int LV, bool FO, int PM>
+template <int D, int LIM, int LV, int SCH, int PM>
 static hipError_t launch_x(const SweepLaunch &L)
 {
-    hipLaunchKernelGGL((k_sweep_x<D, LIM, LV, FO, PM>), dim3(L.grid),
+    hipLaunchKernelGGL((k_sweep_x<D, LIM, LV, SCH, PM>), dim3(L.grid),
                        dim3(L.block), 0, L.stream, L.Uin, L.Uout, L.boxes,
                        L.item_begin, L.nbox, L.nitems, L.ng, L.d_dt, L.dx,
                        L.pm, L.lt, L.d_maxbits);
     return hipGetLastError();
 }
 
-template <int D, int DIR, int LIM, int LV, bool FO, int PM>
+template <int D, int DIR, int LIM, int LV, int SCH, int PM>
 static hipError_t launch_t(const SweepLaunch &L)
 {
-    hipLaunchKernelGGL((k_sweep_t<D, DIR, LIM, LV, FO, PM>), dim3(L.grid),
+    hipLaunchKernelGGL((k_sweep_t<D, DIR, LIM, LV, SCH, PM>), dim3(L.grid),
                        dim3(L.block), 0, L.stream, L.Uin, L.Uout, L.boxes,
                        L.item_begin, L.nbox, L.nitems, L.ng, L.K, L.d_dt,
                        L.dx, L.pm, L.lt, L.d_maxbits);
     return hipGetLastError();
 }
 
-template <int D, int LIM, int LV, bool FO, int PM>
+template <int D, int LIM, int LV, int SCH, int PM>
 static hipError_t dispatch_dir_e(int dir, const SweepLaunch &L)
 {
     if (dir == 0)
-        return launch_x<D, LIM, LV, FO, PM>(L);
+        return launch_x<D, LIM, LV, SCH, PM>(L);
     if constexpr (D >= 2)
         if (dir == 1)
-            return launch_t<D, 1, LIM, LV, FO, PM>(L);
+            return launch_t<D, 1, LIM, LV, SCH, PM>(L);
     if constexpr (D >= 3)
         if (dir == 2)
-            return launch_t<D, 2, LIM, LV, FO, PM>(L);
+            return launch_t<D, 2, LIM, LV, SCH, PM>(L);
     return hipErrorInvalidValue;
 }
 
-template <int D, int LIM, int LV, bool FO>
-static hipError_t dispatch_dir(int dir, const SweepLaunch &L)
+// Physics mode.  The HLLC-based schemes take the near-integer pow forms when
+// gamma allows (PM_NI) and drop epsilon when it is 1 (PM_E1); FORCE, LF and
+// SLIC use no pow, so only epsilon == 1 (PM_NI | PM_E1) or not (0) matters.
+template <int D, int LIM, int LV, int SCH>
+static hipError_t dispatch_pm(int dir, const SweepLaunch &L)
 {
-    if (L.pm.ph.n == 0) // general pow (and any epsilon)
-        return dispatch_dir_e<D, LIM, LV, FO, 0>(dir, L);
-    if (L.pm.eps == 1.0)
-        return dispatch_dir_e<D, LIM, LV, FO, PM_NI | PM_E1>(dir, L);
-    return dispatch_dir_e<D, LIM, LV, FO, PM_NI>(dir, L);
-}
-
-template <int D, int LIM, int LV>
-static hipError_t dispatch_fo(int dir, bool first_order, const SweepLaunch &L)
-{
-    if (first_order)
-        return dispatch_dir<D, LIM, LV, true>(dir, L);
-    return dispatch_dir<D, LIM, LV, false>(dir, L);
-}
-
-template <int D, int LIM>
-static hipError_t dispatch_lv(int dir, bool fo, int lv, const SweepLaunch &L)
-{
-    switch (lv)
+    if constexpr (scheme_uses_hllc(SCH))
     {
-    case LV_SHARED: return dispatch_fo<D, LIM, LV_SHARED>(dir, fo, L);
-    case LV_OWN: return dispatch_fo<D, LIM, LV_OWN>(dir, fo, L);
-    default: return dispatch_fo<D, LIM, LV_TABLE>(dir, fo, L);
+        if (L.pm.ph.n == 0) // general pow (and any epsilon)
+            return dispatch_dir_e<D, LIM, LV, SCH, 0>(dir, L);
+        if (L.pm.eps == 1.0)
+            return dispatch_dir_e<D, LIM, LV, SCH, PM_NI | PM_E1>(dir, L);
+        return dispatch_dir_e<D, LIM, LV, SCH, PM_NI>(dir, L);
+    }
+    else
+    {
+        if (L.pm.eps == 1.0)
+            return dispatch_dir_e<D, LIM, LV, SCH, PM_NI | PM_E1>(dir, L);
+        return dispatch_dir_e<D, LIM, LV, SCH, 0>(dir, L);
     }
 }
 
-template <int D>
-static hipError_t dispatch_lim(int dir, bool fo, int lim, int lv,
-                               const SweepLaunch &L)
+template <int D, int LIM, int SCH>
+static hipError_t dispatch_lv(int dir, int lv, const SweepLaunch &L)
 {
-    if (fo) // first order: no reconstruction, limiter irrelevant
-        return dispatch_fo<D, LIM_MINBEE, LV_SHARED>(dir, true, L);
-    switch (lim)
+    switch (lv)
     {
-    case LIM_MINBEE: return dispatch_lv<D, LIM_MINBEE>(dir, fo, lv, L);
-    case LIM_VAN_LEER: return dispatch_lv<D, LIM_VAN_LEER>(dir, fo, lv, L);
-    case LIM_VAN_ALBADA: return dispatch_lv<D, LIM_VAN_ALBADA>(dir, fo, lv, L);
-    case LIM_SUPERBEE: return dispatch_lv<D, LIM_SUPERBEE>(dir, fo, lv, L);
+    case LV_SHARED: return dispatch_pm<D, LIM, LV_SHARED, SCH>(dir, L);
+    case LV_OWN: return dispatch_pm<D, LIM, LV_OWN, SCH>(dir, L);
+    default: return dispatch_pm<D, LIM, LV_TABLE, SCH>(dir, L);
+    }
+}
+
+template <int D, int SCH>
+static hipError_t dispatch_lim(int dir, int lim, int lv, const SweepLaunch &L)
+{
+    if constexpr (!scheme_reconstructs(SCH)) // no reconstruction: no limiter
+        return dispatch_pm<D, LIM_MINBEE, LV_SHARED, SCH>(dir, L);
+    else
+        switch (lim)
+        {
+        case LIM_MINBEE: return dispatch_lv<D, LIM_MINBEE, SCH>(dir, lv, L);
+        case LIM_VAN_LEER: return dispatch_lv<D, LIM_VAN_LEER, SCH>(dir, lv, L);
+        case LIM_VAN_ALBADA: return dispatch_lv<D, LIM_VAN_ALBADA, SCH>(dir, lv, L);
+        case LIM_SUPERBEE: return dispatch_lv<D, LIM_SUPERBEE, SCH>(dir, lv, L);
+        default: return hipErrorInvalidValue;
+        }
+}
+
+template <int D>
+static hipError_t dispatch_scheme(int dir, int sch, int lim, int lv,
+                                  const SweepLaunch &L)
+{
+    switch (sch)
+    {
+    case SCH_MHM: return dispatch_lim<D, SCH_MHM>(dir, lim, lv, L);
+    case SCH_HLLC: return dispatch_lim<D, SCH_HLLC>(dir, lim, lv, L);
+    case SCH_SLIC: return dispatch_lim<D, SCH_SLIC>(dir, lim, lv, L);
+    case SCH_FORCE: return dispatch_lim<D, SCH_FORCE>(dir, lim, lv, L);
+    case SCH_LF: return dispatch_lim<D, SCH_LF>(dir, lim, lv, L);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_sweep(int D, int dir, bool first_order, int lim, int lv,
+// One dimension's instantiations per translation unit: bx_explicit.hip
+// holds 1D and everything else, bx_sweep2d.hip / bx_sweep3d.hip include this
+// file with BX_SWEEP_PART 2 / 3, so the three build in parallel.
+#if !defined(BX_SWEEP_PART)
+hipError_t launch_sweep_d1(int dir, int sch, int lim, int lv, const SweepLaunch &L)
+{
+    return dispatch_scheme<1>(dir, sch, lim, lv, L);
+}
+const void *sweep_kernel_ptr_d1(int)
+{
+    return (const void *)&k_sweep_x<1, LIM_MINBEE, LV_SHARED, SCH_MHM, PM_NI | PM_E1>;
+}
+#elif BX_SWEEP_PART == 2
+hipError_t launch_sweep_d2(int dir, int sch, int lim, int lv, const SweepLaunch &L)
+{
+    return dispatch_scheme<2>(dir, sch, lim, lv, L);
+}
+const void *sweep_kernel_ptr_d2(int dir)
+{
+    if (dir == 0)
+        return (const void *)&k_sweep_x<2, LIM_MINBEE, LV_SHARED, SCH_MHM, PM_NI | PM_E1>;
+    return (const void *)&k_sweep_t<2, 1, LIM_MINBEE, LV_SHARED, SCH_MHM, PM_NI | PM_E1>;
+}
+#elif BX_SWEEP_PART == 3
+hipError_t launch_sweep_d3(int dir, int sch, int lim, int lv, const SweepLaunch &L)
+{
+    return dispatch_scheme<3>(dir, sch, lim, lv, L);
+}
+const void *sweep_kernel_ptr_d3(int dir)
+{
+    if (dir == 0)
+        return (const void *)&k_sweep_x<3, LIM_MINBEE, LV_SHARED, SCH_MHM, PM_NI | PM_E1>;
+    if (dir == 1)
+        return (const void *)&k_sweep_t<3, 1, LIM_MINBEE, LV_SHARED, SCH_MHM, PM_NI | PM_E1>;
+    return (const void *)&k_sweep_t<3, 2, LIM_MINBEE, LV_SHARED, SCH_MHM, PM_NI | PM_E1>;
+}
+#endif
+
+#ifndef BX_SWEEP_PART
+hipError_t launch_sweep(int D, int dir, int scheme, int lim, int lv,
                         const SweepLaunch &L)
 {
     switch (D)
     {
-    case 1: return dispatch_lim<1>(dir, first_order, lim, lv, L);
-    case 2: return dispatch_lim<2>(dir, first_order, lim, lv, L);
-    case 3: return dispatch_lim<3>(dir, first_order, lim, lv, L);
+    case 1: return launch_sweep_d1(dir, scheme, lim, lv, L);
+    case 2: return launch_sweep_d2(dir, scheme, lim, lv, L);
+    case 3: return launch_sweep_d3(dir, scheme, lim, lv, L);
     default: return hipErrorInvalidValue;
     }
 }
@@ -659,19 +736,12 @@ hipError_t launch_sweep(int D, int dir, bool first_order, int lim, int lv,
 const void *sweep_kernel_ptr(int D, int dir)
 {
     // representative instantiation for occupancy queries
-    if (dir == 0)
+    switch (D)
     {
-        if (D == 1)
-            return (const void *)&k_sweep_x<1, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
-        if (D == 2)
-            return (const void *)&k_sweep_x<2, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
-        return (const void *)&k_sweep_x<3, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
+    case 1: return sweep_kernel_ptr_d1(dir);
+    case 2: return sweep_kernel_ptr_d2(dir);
+    default: return sweep_kernel_ptr_d3(dir);
     }
-    if (D == 2)
-        return (const void *)&k_sweep_t<2, 1, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
-    if (dir == 1)
-        return (const void *)&k_sweep_t<3, 1, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
-    return (const void *)&k_sweep_t<3, 2, LIM_MINBEE, LV_SHARED, false, PM_NI | PM_E1>;
 }
 
 hipError_t launch_fill_ghosts(double *U, const DevBox *boxes,
@@ -731,5 +801,7 @@ hipError_t launch_set_u64(unsigned long long *p, unsigned long long v,
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, stream, p, v);
     return hipGetLastError();
 }
+
+#endif // BX_SWEEP_PART
 
 } // namespace bx

@@ -42,9 +42,17 @@ struct SweepLaunch
     hipStream_t         stream;
 };
 
-hipError_t  launch_sweep(int D, int dir, bool first_order, int lim, int lv,
+// scheme: SCH_* (bx_physics.h), the BX_METHOD_* code
+hipError_t  launch_sweep(int D, int dir, int scheme, int lim, int lv,
                          const SweepLaunch &L);
 const void *sweep_kernel_ptr(int D, int dir);
+// per-dimension instantiations (bx_explicit.hip, bx_sweep2d.hip, bx_sweep3d.hip)
+hipError_t  launch_sweep_d1(int dir, int scheme, int lim, int lv, const SweepLaunch &L);
+hipError_t  launch_sweep_d2(int dir, int scheme, int lim, int lv, const SweepLaunch &L);
+hipError_t  launch_sweep_d3(int dir, int scheme, int lim, int lv, const SweepLaunch &L);
+const void *sweep_kernel_ptr_d1(int dir);
+const void *sweep_kernel_ptr_d2(int dir);
+const void *sweep_kernel_ptr_d3(int dir);
 
 hipError_t launch_fill_ghosts(double *U, const DevBox *boxes,
                               const GhostSeg *segs, int nseg, int ng, int D,

@@ -11,6 +11,7 @@
 //   HLLC star state    Source/Fluxes/HLLC.H:311-337
 //   HLLC flux          Source/Fluxes/HLLC.cpp:90-172
 //   max wave speed     Source/System/Euler/Euler.cpp:53-68
+//   FORCE / LF flux    Source/Fluxes/Force.cpp:24-104, 106-146
 // State vectors are conserved variables U = (rho, m_0..m_{D-1}, E), NC = D+2.
 #pragma once
 #include "bx_pow.h"
@@ -373,6 +374,80 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
         for (int n = 0; n < NC; ++n)
             F[n] = __builtin_nan("");
     }
+}
+
+// Lax-Friedrichs flux at the face between UL and UR (compute_LF_flux,
+// Force.cpp:106-146): 0.5 dx/dt (UL - UR) + 0.5 (F(UL) + F(UR)).
+// lfc = 0.5 * dx / dt, formed once per launch as the reference's expression
+// evaluates it (left to right).
+template <int D, int dir, bool E1 = false>
+BX_DEV void lf_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
+                    double (&F)[D + 2], const EulerParm &pm, double lfc)
+{
+    constexpr int NC = D + 2;
+    double        QL[NC], QR[NC], FL[NC], FR[NC];
+    primv_from_consv<D, E1>(UL, QL, pm);
+    primv_from_consv<D, E1>(UR, QR, pm);
+    euler_flux<D, dir, E1>(UL, QL, FL, pm);
+    euler_flux<D, dir, E1>(UR, QR, FR, pm);
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+        F[n] = lfc * (UL[n] - UR[n]) + 0.5 * (FL[n] + FR[n]);
+}
+
+// FORCE flux (compute_force_flux_LR, Force.cpp:24-104): half the
+// Lax-Friedrichs flux plus half the flux of the Richtmyer half-step state,
+//   F = 0.5 (fc (UL - UR) + 0.5 (FL + FR)) + 0.5 F(0.5 (UL + UR) + rc (FL - FR))
+// with fc = (0.5 dx) / dt and rc = dt / (2 dx), in the reference's order.
+template <int D, int dir, bool E1 = false>
+BX_DEV void force_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
+                       double (&F)[D + 2], const EulerParm &pm, double fc,
+                       double rc)
+{
+    constexpr int NC = D + 2;
+    double        Q[NC], FL[NC], FR[NC], H[NC];
+    primv_from_consv<D, E1>(UL, Q, pm);
+    euler_flux<D, dir, E1>(UL, Q, FL, pm);
+    primv_from_consv<D, E1>(UR, Q, pm);
+    euler_flux<D, dir, E1>(UR, Q, FR, pm);
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+    {
+        F[n] = 0.5 * (fc * (UL[n] - UR[n]) + 0.5 * (FL[n] + FR[n]));
+        H[n] = 0.5 * (UL[n] + UR[n]) + rc * (FL[n] - FR[n]);
+    }
+    primv_from_consv<D, E1>(H, Q, pm);
+    euler_flux<D, dir, E1>(H, Q, FL, pm);
+#pragma unroll
+    for (int n = 0; n < NC; ++n)
+        F[n] += 0.5 * FL[n];
+}
+
+// Explicit schemes (adv.num_method, NumericalMethods.H:10-20), numbered as
+// the BX_METHOD_* codes of the C ABI.
+enum Scheme
+{
+    SCH_MHM   = 0, // MUSCL-Hancock: reconstruct + half step, HLLC flux
+    SCH_HLLC  = 1, // first-order HLLC on the cell states
+    SCH_SLIC  = 2, // reconstruct + half step, FORCE flux
+    SCH_FORCE = 3, // first-order FORCE
+    SCH_LF    = 4  // first-order Lax-Friedrichs
+};
+constexpr bool scheme_reconstructs(int s) { return s == SCH_MHM || s == SCH_SLIC; }
+constexpr bool scheme_uses_hllc(int s) { return s == SCH_MHM || s == SCH_HLLC; }
+
+// Face flux of scheme SCH between the left and right states of a face.
+// fc = (0.5 dx) / dt, rc = dt / (2 dx) (FORCE / LF only).
+template <int D, int dir, bool E1, bool NI, int SCH>
+BX_DEV void face_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
+                      double (&F)[D + 2], const EulerParm &pm, double fc, double rc)
+{
+    if (scheme_uses_hllc(SCH))
+        hllc_flux<D, dir, E1, NI>(UL, UR, F, pm);
+    else if (SCH == SCH_LF)
+        lf_flux<D, dir, E1>(UL, UR, F, pm, fc);
+    else
+        force_flux<D, dir, E1>(UL, UR, F, pm, fc, rc);
 }
 
 // First-order HLLC (num_method = hllc) uses the same face flux on the cell

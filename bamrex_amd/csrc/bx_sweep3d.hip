@@ -1,0 +1,4 @@
+// 3D instantiations of the directional sweep kernels, compiled apart from
+// bx_explicit.hip so that the dimensions build in parallel.
+#define BX_SWEEP_PART 3
+#include "bx_explicit.hip"

@@ -81,10 +81,18 @@ void AmrLevelAdv::read_params(const ParmParse &root)
     }
 
     m_parm.dim = dim;
+    // NumericalMethods::enum_from_string (Source/NumericalMethods.H:22-44),
+    // restricted to the explicit methods
     if (num_method == "muscl_hancock")
         m_parm.num_method = BX_METHOD_MUSCL_HANCOCK;
     else if (num_method == "hllc")
         m_parm.num_method = BX_METHOD_HLLC;
+    else if (num_method == "slic")
+        m_parm.num_method = BX_METHOD_SLIC;
+    else if (num_method == "force")
+        m_parm.num_method = BX_METHOD_FORCE;
+    else if (num_method == "lax_friedrichs")
+        m_parm.num_method = BX_METHOD_LAX_FRIEDRICHS;
     else
         throw std::runtime_error("Invalid numerical method \"" + num_method
                                  + "\" for the explicit hot path");

@@ -34,9 +34,14 @@ extern "C" {
 #define BX_ERR_STATE 4       /* call order violated (e.g. no level set) */
 
 /* adv.num_method (Source/NumericalMethods.H:10-44); the explicit methods
- * implemented here. */
+ * AmrLevelAdv::advance_explicit dispatches (AmrLevelAdv_advance.cpp:311-326):
+ * compute_MUSCL_hancock_flux, compute_HLLC_flux, compute_SLIC_flux,
+ * compute_force_flux, compute_LF_flux (Fluxes.H, Force.cpp, SLIC.cpp). */
 #define BX_METHOD_MUSCL_HANCOCK 0
 #define BX_METHOD_HLLC 1
+#define BX_METHOD_SLIC 2
+#define BX_METHOD_FORCE 3
+#define BX_METHOD_LAX_FRIEDRICHS 4
 
 /* limiting.limiter (Source/Fluxes/Limiters.H:13) */
 #define BX_LIMITER_MINBEE 0

@@ -16,6 +16,8 @@
 //   HLLC.cpp:64-173        compute_HLLC_flux_LR_euler
 //   MUSCL_Hancock.cpp:15-52 compute_MUSCL_hancock_flux (swapped L/R arrays)
 //   Fluxes.H:192-227       compute_HLLC_flux (first order)
+//   Force.cpp:24-146       compute_force_flux_LR, compute_LF_flux
+//   SLIC.cpp:12-50         compute_SLIC_flux
 //   AmrLevelAdv_advance.cpp:261-360 advance_explicit (x,y,z order, full dt,
 //                          update, fill_consv_boundary after every sweep)
 //   BCs.cpp:215-442        bc types -> FillBoundary + PhysBCFunct
@@ -478,6 +480,110 @@ static void compute_MUSCL_hancock_flux(const Box &bx, Fab &flux,
     compute_HLLC_flux_LR(bx, flux, half_R, half_L, parm, dir);
 }
 
+// compute_force_flux_LR<dir>  Force.cpp:24-104.  Face i takes cL at i-1
+// and cR at i; the flux array covers surroundingNodes(bx, dir) (the
+// reference's flux_bx = growHi(bx, dir, 1), the same index range).
+static void compute_force_flux_LR(const Box &bx, Fab &flux, const Fab &cL,
+                                  const Fab &cR, const double dx[3], double dt,
+                                  const Parm &parm, int dir)
+{
+    const int    D      = parm.D;
+    const int    NSTATE = D + 2;
+    const double dxd    = dx[dir];
+    const double a = parm.adiabatic, e = parm.epsilon;
+    const Box    ghost_bx = grow_dir(bx, dir, 1);
+    const Box    flux_bx  = surrounding_nodes(bx, dir);
+    Fab          flux_func_values_L, flux_func_values_R, half_updated_values;
+    flux_func_values_L.define(ghost_bx, NSTATE);
+    flux_func_values_R.define(ghost_bx, NSTATE);
+    half_updated_values.define(ghost_bx, NSTATE);
+    compute_flux_function(ghost_bx, flux_func_values_L, cL, a, e, dir, D);
+    compute_flux_function(ghost_bx, flux_func_values_R, cR, a, e, dir, D);
+    const int io = (dir == 0) ? 1 : 0;
+    const int jo = (dir == 1) ? 1 : 0;
+    const int ko = (dir == 2) ? 1 : 0;
+    // half of the Lax-Friedrichs flux, and the Richtmyer half-step state
+    for (int n = 0; n < NSTATE; ++n)
+        for_box(flux_bx, [&](int i, int j, int k) {
+            flux(i, j, k, n)
+                = 0.5
+                  * ((0.5 * dxd / dt) * (cL(i - io, j - jo, k - ko, n) - cR(i, j, k, n))
+                     + 0.5
+                           * (flux_func_values_L(i - io, j - jo, k - ko, n)
+                              + flux_func_values_R(i, j, k, n)));
+            half_updated_values(i, j, k, n)
+                = 0.5 * (cL(i - io, j - jo, k - ko, n) + cR(i, j, k, n))
+                  + (dt / (2 * dxd))
+                        * (flux_func_values_L(i - io, j - jo, k - ko, n)
+                           - flux_func_values_R(i, j, k, n));
+        });
+    // plus half of the Richtmyer flux (the L buffer is reused, as there)
+    compute_flux_function(flux_bx, flux_func_values_L, half_updated_values, a, e,
+                          dir, D);
+    for (int n = 0; n < NSTATE; ++n)
+        for_box(flux_bx, [&](int i, int j, int k) {
+            flux(i, j, k, n) += 0.5 * flux_func_values_L(i, j, k, n);
+        });
+}
+
+// compute_LF_flux<dir>  Force.cpp:106-146
+static void compute_LF_flux(const Box &bx, Fab &flux, const Fab &consv,
+                            const double dx[3], double dt, const Parm &parm,
+                            int dir)
+{
+    const int    D        = parm.D;
+    const int    NSTATE   = D + 2;
+    const double dxd      = dx[dir];
+    const Box    ghost_bx = grow(bx, D, 1);
+    const Box    flux_bx  = surrounding_nodes(bx, dir);
+    Fab          flux_func_values;
+    flux_func_values.define(ghost_bx, NSTATE);
+    compute_flux_function(ghost_bx, flux_func_values, consv, parm.adiabatic,
+                          parm.epsilon, dir, D);
+    const int io = (dir == 0) ? 1 : 0;
+    const int jo = (dir == 1) ? 1 : 0;
+    const int ko = (dir == 2) ? 1 : 0;
+    for (int n = 0; n < NSTATE; ++n)
+        for_box(flux_bx, [&](int i, int j, int k) {
+            flux(i, j, k, n)
+                = 0.5 * dxd / dt * (consv(i - io, j - jo, k - ko, n) - consv(i, j, k, n))
+                  + 0.5 * (flux_func_values(i - io, j - jo, k - ko, n)
+                           + flux_func_values(i, j, k, n));
+        });
+}
+
+// compute_SLIC_flux<dir>  SLIC.cpp:12-50: reconstruction + half step on
+// grow(bx, dir, 1), then the FORCE flux of (half_R at i-1, half_L at i)
+static void compute_SLIC_flux(const Box &bx, Fab &flux, const Fab &consv,
+                              const double dx[3], double dt, const Parm &parm,
+                              int dir)
+{
+    const Box bx_g1 = grow_dir(bx, dir, 1);
+    Fab       half_stepped_L, half_stepped_R;
+    half_stepped_L.define(bx_g1, parm.D + 2);
+    half_stepped_R.define(bx_g1, parm.D + 2);
+    reconstruct_and_half_time_step(bx_g1, half_stepped_L, half_stepped_R, consv,
+                                   dx, dt, parm, dir);
+    compute_force_flux_LR(bx, flux, half_stepped_R, half_stepped_L, dx, dt, parm,
+                          dir);
+}
+
+// advance_explicit's flux dispatch (AmrLevelAdv_advance.cpp:311-326):
+// 0 muscl_hancock, 1 hllc, 2 slic, 3 force, 4 lax_friedrichs
+static void compute_flux(int num_method, const Box &bx, Fab &flux, const Fab &consv,
+                         const double dx[3], double dt, const Parm &parm, int dir)
+{
+    switch (num_method)
+    {
+    case 0: compute_MUSCL_hancock_flux(bx, flux, consv, dx, dt, parm, dir); break;
+    case 1: compute_HLLC_flux_LR(bx, flux, consv, consv, parm, dir); break;
+    case 2: compute_SLIC_flux(bx, flux, consv, dx, dt, parm, dir); break;
+    case 3: compute_force_flux_LR(bx, flux, consv, consv, dx, dt, parm, dir); break;
+    case 4: compute_LF_flux(bx, flux, consv, dx, dt, parm, dir); break;
+    default: throw std::logic_error("Internal error");
+    }
+}
+
 // ----------------------------------------------------------------- level
 struct Level
 {
@@ -584,10 +690,7 @@ static void sweep(Level &L, std::vector<Fab> &in, std::vector<Fab> &out,
         const Box &bx = L.valid[b];
         Fab        flux;
         flux.define(surrounding_nodes(bx, d), ncomp);
-        if (num_method == 0)
-            compute_MUSCL_hancock_flux(bx, flux, in[b], dx, dt, L.parm, d);
-        else
-            compute_HLLC_flux_LR(bx, flux, in[b], in[b], L.parm, d);
+        compute_flux(num_method, bx, flux, in[b], dx, dt, L.parm, d);
         const Fab &Uin  = in[b];
         Fab       &Uout = out[b];
         for (int n = 0; n < ncomp; ++n)
@@ -841,10 +944,7 @@ int or_flux_box(const or_params *p, const int *box, int ng, int dir, double dt,
     std::memcpy(U.a.data(), Ufab, sizeof(double) * U.a.size());
     Fab F;
     F.define(surrounding_nodes(v, dir), D + 2);
-    if (p->num_method == 0)
-        compute_MUSCL_hancock_flux(v, F, U, dx, dt, parm, dir);
-    else
-        compute_HLLC_flux_LR(v, F, U, U, parm, dir);
+    compute_flux(p->num_method, v, F, U, dx, dt, parm, dir);
     std::memcpy(flux, F.a.data(), sizeof(double) * F.a.size());
     return 0;
 }

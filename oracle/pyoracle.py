@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 
 BC = {"transmissive": 0, "periodic": 1, "reflective": 2}
 LIMITER = {"minbee": 0, "van_leer": 1, "van_albada": 2, "superbee": 3}
-METHOD = {"muscl_hancock": 0, "hllc": 1}
+METHOD = {"muscl_hancock": 0, "hllc": 1, "slic": 2, "force": 3, "lax_friedrichs": 4}
 
 
 class or_params(ctypes.Structure):

@@ -476,3 +476,124 @@ def test_scheme_amplifies_one_ulp_perturbations(oracle_glibc, oracle_bxpow):
               s["dx"], s["cfl"], stop_time=0.25, max_step=100)
         res.append(s["layout"].valid(s["U"], 0).copy())
     assert np.array_equal(res[0], res[1])
+
+
+# ------------------------------------------- SLIC / FORCE / Lax-Friedrichs
+def _face_flux(o, method, dim, UL, UR, dt, dx, gamma=1.4, eps=1.0, lim="minbee"):
+    """Flux at the face between two cells (states UL | UR, each repeated on
+    its side of a 2-cell box with 2 ghost layers), through or_flux_box."""
+    lay = Layout(1, [((0,), (1,))], 2)
+    U = lay.new_array()
+    f = lay.fab(U, 0)
+    for n in range(3):
+        f[n][:3] = UL[n]
+        f[n][3:] = UR[n]
+    p = pyoracle.params(1, gamma, eps, num_method=method, limiter=lim)
+    return o.flux_box(p, lay, U, 0, 0, dt, (dx,))[:, 1]
+
+
+def _F_exact(U, gamma):
+    """Euler flux of a 1D conserved state in exact rational arithmetic."""
+    from fractions import Fraction as Fr
+    rho, m, E = (Fr(float(v)) for v in U)
+    g = Fr(float(gamma))
+    u = m / rho
+    p = (g - 1) * (E - Fr(1, 2) * rho * u * u)
+    return [m, m * u + p, (E + p) * u]
+
+
+@pytest.mark.parametrize("method", ["force", "lax_friedrichs"])
+def test_force_lf_against_textbook_definition(oracle_glibc, method):
+    """FORCE = (F_LF + F_RI) / 2 with F_LF = (F_L + F_R)/2 + (dx/dt)(U_L - U_R)/2
+    and F_RI = F((U_L + U_R)/2 + (dt/dx)(F_L - F_R)/2) (Toro, Riemann Solvers
+    and Numerical Methods for Fluid Dynamics, 3rd ed., sec. 7.4.2), evaluated
+    exactly on random states: a swapped state, dropped half or wrong factor
+    in the oracle shows up far above the rounding level."""
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(31 if method == "force" else 37)
+    for _ in range(40):
+        qL = [rng.uniform(0.2, 3), rng.uniform(-2, 2), rng.uniform(0.1, 3)]
+        qR = [rng.uniform(0.2, 3), rng.uniform(-2, 2), rng.uniform(0.1, 3)]
+        UL = problems.consv_from_primv([np.float64(x) for x in qL], 1.4, 1.0, 1)
+        UR = problems.consv_from_primv([np.float64(x) for x in qR], 1.4, 1.0, 1)
+        dt, dx = float(rng.uniform(1e-4, 1e-2)), 1.0 / 64
+        got = _face_flux(oracle_glibc, method, 1, UL, UR, dt, dx)
+        FL, FR = _F_exact(UL, 1.4), _F_exact(UR, 1.4)
+        uL = [Fr(float(v)) for v in UL]
+        uR = [Fr(float(v)) for v in UR]
+        lam = Fr(dx) / Fr(dt)
+        lf = [(FL[n] + FR[n]) / 2 + lam * (uL[n] - uR[n]) / 2 for n in range(3)]
+        if method == "lax_friedrichs":
+            ex = lf
+        else:
+            uri = [(uL[n] + uR[n]) / 2 + (FL[n] - FR[n]) / (2 * lam) for n in range(3)]
+            FRI = _F_exact([float(v) for v in uri], 1.4)
+            ex = [(lf[n] + FRI[n]) / 2 for n in range(3)]
+        scale = max(abs(float(v)) for v in lf)
+        for n in range(3):
+            assert abs(got[n] - float(ex[n])) <= 1e-12 * scale, (method, n, got, ex)
+
+
+@pytest.mark.parametrize("method", ["slic", "force", "lax_friedrichs"])
+def test_new_methods_consistent_and_free_stream(oracle_glibc, method):
+    """Equal states: every face flux is F(U) exactly (consistency), so a
+    uniform level stays uniform bit for bit in 1, 2 and 3 dimensions."""
+    U0 = problems.consv_from_primv([np.float64(1.3), np.float64(-0.4), np.float64(2.1)], 1.4, 1.0, 1)
+    got = _face_flux(oracle_glibc, method, 1, U0, U0, 1e-3, 1.0 / 64)
+    assert np.array_equal(got, [float(v) for v in _F_exact(U0, 1.4)]) or \
+        np.allclose(got, [float(v) for v in _F_exact(U0, 1.4)], rtol=1e-15, atol=0)
+    for dim in (1, 2, 3):
+        n = 12
+        boxes = chop_domain((0,) * dim, (n - 1,) * dim, 6)
+        lay = Layout(dim, boxes, 2)
+        U = lay.new_array()
+        q = [1.3] + [0.3 * (d + 1) for d in range(dim)] + [2.1]
+        c = problems.consv_from_primv([np.float64(v) for v in q], 1.4, 1.0, dim)
+        for b in range(len(boxes)):
+            for k in range(dim + 2):
+                lay.fab(U, b)[k][...] = c[k]
+        ref = U.copy()
+        p = pyoracle.params(dim, 1.4, 1.0, num_method=method, limiter="van_leer")
+        oracle_glibc.run(p, lay, U, (0,) * dim, (n - 1,) * dim, ["periodic"] * dim,
+                         ["periodic"] * dim, (1.0 / n,) * dim, 0.8, max_step=3)
+        for b in range(len(boxes)):
+            assert np.array_equal(lay.valid(U, b), lay.valid(ref, b)), (method, dim)
+
+
+def test_new_methods_sod_accuracy_ordering(oracle_glibc):
+    """Sod at 256 cells to t = 0.2: every method approaches the exact
+    solution, Lax-Friedrichs is the most diffusive, FORCE halves its
+    numerical viscosity, SLIC (second order) is the most accurate; all
+    conserve mass to rounding (transmissive ends, waves not yet out)."""
+    x = (np.arange(256) + 0.5) / 256
+    ex = sample((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), (x - 0.5) / 0.2)
+    err = {}
+    for method in ("lax_friedrichs", "force", "slic", "muscl_hancock"):
+        s = problems.sod_setup(ncell=256)
+        p = pyoracle.params(1, 1.4, 1.0, num_method=method, limiter="van_leer")
+        lay, U = s["layout"], s["U"]
+        m0 = sum(lay.valid(U, b)[0].sum() for b in range(len(lay.boxes)))
+        oracle_glibc.run(p, lay, U, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], s["dx"],
+                         0.8, stop_time=0.2, max_step=100000)
+        rho = np.concatenate([lay.valid(U, b)[0] for b in range(len(lay.boxes))])
+        err[method] = np.abs(rho - ex[0]).mean()
+        assert abs(rho.sum() - m0) <= 1e-12 * m0
+    assert err["lax_friedrichs"] > err["force"] > err["slic"]
+    assert err["slic"] < 0.02 and err["muscl_hancock"] < 0.02
+
+
+def test_max_wave_speed_skips_nan(oracle_glibc):
+    """max_wave_speed (Euler.cpp:53-68) reduces with amrex::max(acc, s) =
+    (acc < s) ? s : acc, which keeps acc when s is NaN: a negative-pressure
+    cell is skipped, the others still count."""
+    dim, n = 2, 8
+    boxes = chop_domain((0, 0), (n - 1, n - 1), 8)
+    lay = Layout(dim, boxes, 2)
+    U = lay.new_array()
+    c = problems.consv_from_primv([np.float64(1.0), np.float64(0.5), np.float64(0.0),
+                                   np.float64(1.0)], 1.4, 1.0, dim)
+    for k in range(4):
+        lay.valid(U, 0)[k][...] = c[k]
+    lay.valid(U, 0)[3][4, 4] = -1.0
+    p = pyoracle.params(dim, 1.4)
+    assert oracle_glibc.max_wave_speed(p, lay, U) == math.sqrt(1.4) + 0.5

@@ -117,6 +117,15 @@ _SIGS = {
     "bx_helm_set_fused": (ctypes.c_int, [P, ctypes.c_int]),
     "bx_helm_solve": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, P, P]),
     "bx_helm_last_error": (ctypes.c_char_p, []),
+    "bx_imex_pressure": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, P,
+                                        P]),
+    "bx_imex_advective_dt": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.POINTER(ctypes.c_double), P]),
+    "bx_imex_peek_assemble": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P, P, P, P,
+                                             P]),
+    "bx_imex_peek_finish": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P]),
     "bx_cnn_create": (ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(P)]),
     "bx_cnn_destroy": (None, [P]),
     "bx_cnn_set_params": (ctypes.c_int, [P, P, ctypes.c_int]),

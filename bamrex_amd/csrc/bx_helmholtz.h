@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "../../include/bamrex_hip.h"
 
 // e = omega * (sigma / D) * N(r / sigma) on an nx x ny periodic field, with
@@ -14,3 +16,7 @@ int bx_cnn_forward(bx_cnn *net, const double *r, double *e, int nx, int ny,
 int bx_cnn_reserve(bx_cnn *net, long long npx);
 // Changes whenever a buffer bx_cnn_forward uses is reallocated.
 long long bx_cnn_epoch(const bx_cnn *net);
+
+// Records msg as the text bx_helm_last_error returns and gives back code
+// (shared by the bx_helm_*, bx_cnn_* and bx_imex_* entry points).
+int bx_helm_fail(int code, const std::string &msg);

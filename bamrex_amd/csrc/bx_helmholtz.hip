@@ -806,6 +806,8 @@ static int herr(int code, const std::string &m)
 
 extern "C" const char *bx_helm_last_error(void) { return g_helm_err.c_str(); }
 
+int bx_helm_fail(int code, const std::string &msg) { return herr(code, msg); }
+
 template <class T> static int halloc(T **p, long long n)
 {
     H_CHECK(hipMalloc((void **)p, sizeof(T) * std::max<long long>(n, 1)));

@@ -1,0 +1,377 @@
+// Explicit work of the IMEX PEEK-SP111 stage on one 2D periodic level: the
+// Rusanov advective update, the enthalpy, the pressure-equation source and
+// coefficients, and after the pressure solve the momentum/energy flux
+// corrections with the conservative update.  In the reference this is host
+// code around MLMG (CPU-only update kernels):
+//   compute_flux_imex_o1            Source/IMEX/IMEX_O1.cpp:18-175
+//   advance_rusanov_adv             Source/IMEX/IMEX_Fluxes.cpp:10-118
+//   adv_flux / kinetic_energy / pressure
+//                                   Source/IMEX_K/euler_K.H:8-65
+//   compute_enthalpy (reuse_pressure)
+//                                   Source/IMEX/IMEX_O1.cpp:177-228
+//   pressure_source_vector          Source/IMEX/IMEX_O1.cpp:416-452
+//   setup_pressure_op (a, face b)   Source/IMEX/IMEX_O1.cpp:570-614
+//   update_momentum_flux / update_energy_flux
+//                                   Source/IMEX/IMEX_O1.cpp:480-566
+//   conservative_update (SP111)     Source/IMEX/IMEX_RK.cpp:22-96
+//   max_dx_scaled_speed             Source/System/Euler/Euler.cpp:107-129
+//
+// Layout: component-major (rho, m_x, m_y, E), x fastest, nx*ny cells per
+// component, no ghost cells (periodic neighbours are wrapped in the
+// kernels).  Face arrays: bX is ny rows of nx+1 faces, bY ny+1 rows of nx.
+// Every expression keeps the reference's operation order, so results are
+// bit-identical to a CPU evaluation without contraction.
+//
+// Kernels and HBM bytes per cell (reads + writes, neighbours from cache):
+//   k_peek_advect   U -> Uex                       32 + 32
+//   k_peek_system   Uex, p -> enth, rhs, a, bX, bY 40 + 40
+//   k_peek_finish   U, Uex(m), enth, p -> U_new    64 + 32
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/bamrex_hip.h"
+#include "bx_helmholtz.h"
+
+namespace bx {
+
+struct PeekGeom
+{
+    int       nx, ny;
+    long long n; // cells per component
+};
+
+__device__ __forceinline__ int wrap_lo(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+__device__ __forceinline__ int wrap_hi(int i, int n) { return i == n - 1 ? 0 : i + 1; }
+
+// kinetic_energy (euler_K.H:8-20): eps (m_x^2 + m_y^2) / (2 rho)
+__device__ __forceinline__ double peek_ke(double r, double mx, double my, double eps)
+{
+    return eps * (mx * mx + my * my) / (2 * r);
+}
+
+// Cell state and its advective flux in direction DIR (adv_flux, euler_K.H:
+// 53-65), with |u_DIR| for the Rusanov speed.
+template <int DIR>
+struct AdvCell
+{
+    double U[4], F[4], au;
+    __device__ __forceinline__ void load(const double *__restrict__ S, long long c,
+                                         long long n, double eps)
+    {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            U[k] = S[c + k * n];
+        const double u = U[1 + DIR] / U[0];
+        F[0]           = U[1 + DIR];
+        F[1]           = U[1] * u;
+        F[2]           = U[2] * u;
+        F[3]           = u * peek_ke(U[0], U[1], U[2], eps);
+        au             = fabs(u);
+    }
+};
+
+// Rusanov face flux between cell m (lower) and c (upper)
+// (IMEX_Fluxes.cpp:70-90): 0.5 ((F_c + F_m) - max(|u_c|,|u_m|) (U_c - U_m))
+template <int DIR>
+__device__ __forceinline__ void rusanov(const AdvCell<DIR> &m, const AdvCell<DIR> &c,
+                                        double (&F)[4])
+{
+    // amrex::max(a, b) = (a < b) ? b : a
+    const double umax = (c.au < m.au) ? m.au : c.au;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        F[k] = 0.5 * ((c.F[k] + m.F[k]) - umax * (c.U[k] - m.U[k]));
+}
+
+// Rusanov fluxes at the four faces of cell (i, j).
+__device__ __forceinline__ void cell_faces(const double *__restrict__ U, const PeekGeom &g,
+                                           int i, int j, double eps, double (&Fxl)[4],
+                                           double (&Fxr)[4], double (&Fyl)[4],
+                                           double (&Fyr)[4])
+{
+    const long long row = (long long)j * g.nx;
+    const long long c   = row + i;
+    AdvCell<0>      xl, xc, xr;
+    xl.load(U, row + wrap_lo(i, g.nx), g.n, eps);
+    xc.load(U, c, g.n, eps);
+    xr.load(U, row + wrap_hi(i, g.nx), g.n, eps);
+    rusanov(xl, xc, Fxl);
+    rusanov(xc, xr, Fxr);
+    AdvCell<1> yl, yc, yr;
+    yl.load(U, (long long)wrap_lo(j, g.ny) * g.nx + i, g.n, eps);
+    yc.load(U, c, g.n, eps);
+    yr.load(U, (long long)wrap_hi(j, g.ny) * g.nx + i, g.n, eps);
+    rusanov(yl, yc, Fyl);
+    rusanov(yc, yr, Fyr);
+}
+
+// Explicit advective update (IMEX_Fluxes.cpp:92-111):
+//   Uex = U - (dtdx (Fx(i+1) - Fx(i)) + dtdy (Fy(j+1) - Fy(j)))
+__global__ __launch_bounds__(256) void k_peek_advect(const double *__restrict__ U,
+                                                     double *__restrict__ Uex, PeekGeom g,
+                                                     double eps, double dtdx, double dtdy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    double Fxl[4], Fxr[4], Fyl[4], Fyr[4];
+    cell_faces(U, g, i, j, eps, Fxl, Fxr, Fyl, Fyr);
+    const long long c = (long long)j * g.nx + i;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        Uex[c + k * g.n] = U[c + k * g.n] - (dtdx * (Fxr[k] - Fxl[k]) + dtdy * (Fyr[k] - Fyl[k]));
+}
+
+// enth = p gamma / ((gamma - 1) rho_ex)   (compute_enthalpy, reuse_pressure)
+__device__ __forceinline__ double peek_enth(const double *__restrict__ p,
+                                            const double *__restrict__ Uex, long long c,
+                                            double adia)
+{
+    return p[c] * adia / ((adia - 1) * Uex[c]);
+}
+
+// Pressure system of the stage: enthalpy, the source b
+// (pressure_source_vector) and the operator coefficients a = eps/(gamma-1),
+// b = face average of the enthalpy (average_cellcenter_to_face:
+// 0.5 (h(i-1) + h(i))).  The periodic image faces (bX column nx, bY row ny)
+// repeat faces 0.
+__global__ __launch_bounds__(256) void k_peek_system(
+    const double *__restrict__ Uex, const double *__restrict__ p, double *__restrict__ enth,
+    double *__restrict__ rhs, double *__restrict__ acoef, double *__restrict__ bX,
+    double *__restrict__ bY, PeekGeom g, double adia, double eps, double hdtdx, double hdtdy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    const long long row = (long long)j * g.nx;
+    const long long c   = row + i;
+    const long long cw  = row + wrap_lo(i, g.nx);
+    const long long ce  = row + wrap_hi(i, g.nx);
+    const long long cs  = (long long)wrap_lo(j, g.ny) * g.nx + i;
+    const long long cn  = (long long)wrap_hi(j, g.ny) * g.nx + i;
+    const long long n   = g.n;
+    const double    h   = peek_enth(p, Uex, c, adia);
+    const double    hw  = peek_enth(p, Uex, cw, adia);
+    const double    he  = peek_enth(p, Uex, ce, adia);
+    const double    hs  = peek_enth(p, Uex, cs, adia);
+    const double    hn  = peek_enth(p, Uex, cn, adia);
+    const double    ke  = peek_ke(Uex[c], Uex[c + n], Uex[c + 2 * n], eps);
+    rhs[c] = eps
+             * (Uex[c + 3 * n] - ke
+                - (hdtdx * (he * Uex[ce + n] - hw * Uex[cw + n])
+                   + hdtdy * (hn * Uex[cn + 2 * n] - hs * Uex[cs + 2 * n])));
+    enth[c]           = h;
+    acoef[c]          = eps / (adia - 1);
+    const double bx_f = 0.5 * (hw + h);
+    const double by_f = 0.5 * (hs + h);
+    const int    bxw  = g.nx + 1;
+    bX[(long long)j * bxw + i] = bx_f;
+    if (i == 0)
+        bX[(long long)j * bxw + g.nx] = bx_f;
+    bY[c] = by_f;
+    if (j == 0)
+        bY[(long long)g.ny * g.nx + i] = by_f;
+}
+
+// Face fluxes after the pressure solve (update_momentum_flux,
+// update_energy_flux) for the face between cells m (lower) and c (upper) in
+// direction DIR; F holds the Rusanov flux of that face on entry.
+template <int DIR>
+__device__ __forceinline__ void peek_correct(double (&F)[4], const double *__restrict__ Uex,
+                                             const double *__restrict__ enth,
+                                             const double *__restrict__ p, long long m,
+                                             long long c, long long n, double heps,
+                                             double hdtde)
+{
+    F[1 + DIR] += heps * (p[c] + p[m]);
+    const double hc = enth[c], hm = enth[m];
+    F[3] += 0.5 * (hc * Uex[c + (1 + DIR) * n] + hm * Uex[m + (1 + DIR) * n])
+            - hdtde * ((hc + hm) * (p[c] - p[m]));
+}
+
+// U_new = U - (dtdx (Fx(i+1) - Fx(i)) + dtdy (Fy(j+1) - Fy(j)))
+// (conservative_update with the SP111 weight b = 1, IMEX_RK.cpp:22-96)
+__global__ __launch_bounds__(256) void k_peek_finish(
+    const double *__restrict__ U, const double *__restrict__ Uex,
+    const double *__restrict__ enth, const double *__restrict__ p, double *__restrict__ Unew,
+    PeekGeom g, double eps, double dtdx, double dtdy, double heps, double hdtdxe,
+    double hdtdye)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    double Fxl[4], Fxr[4], Fyl[4], Fyr[4];
+    cell_faces(U, g, i, j, eps, Fxl, Fxr, Fyl, Fyr);
+    const long long row = (long long)j * g.nx;
+    const long long c   = row + i;
+    const long long cw  = row + wrap_lo(i, g.nx);
+    const long long ce  = row + wrap_hi(i, g.nx);
+    const long long cs  = (long long)wrap_lo(j, g.ny) * g.nx + i;
+    const long long cn  = (long long)wrap_hi(j, g.ny) * g.nx + i;
+    peek_correct<0>(Fxl, Uex, enth, p, cw, c, g.n, heps, hdtdxe);
+    peek_correct<0>(Fxr, Uex, enth, p, c, ce, g.n, heps, hdtdxe);
+    peek_correct<1>(Fyl, Uex, enth, p, cs, c, g.n, heps, hdtdye);
+    peek_correct<1>(Fyr, Uex, enth, p, c, cn, g.n, heps, hdtdye);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        Unew[c + k * g.n] = U[c + k * g.n] - (dtdx * (Fxr[k] - Fxl[k]) + dtdy * (Fyr[k] - Fyl[k]));
+}
+
+// p = (gamma - 1) (E - ke)   (pressure, euler_K.H:23-35)
+__global__ __launch_bounds__(256) void k_peek_pressure(const double *__restrict__ U,
+                                                       double *__restrict__ p, long long n,
+                                                       double adia, double eps)
+{
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n)
+        return;
+    p[c] = (adia - 1) * (U[c + 3 * n] - peek_ke(U[c], U[c + n], U[c + 2 * n], eps));
+}
+
+// max over cells of (|m_x|/dx + |m_y|/dy) / rho (max_dx_scaled_speed): the
+// values are non-negative, so their bit patterns order like the values; a
+// NaN (all exponent bits set) orders above every number and is propagated.
+__global__ __launch_bounds__(256) void k_peek_speed(const double *__restrict__ U, long long n,
+                                                    double dx, double dy,
+                                                    unsigned long long *__restrict__ out)
+{
+    unsigned long long m = 0ULL;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x)
+    {
+        const double s = (fabs(U[c + n]) / dx + fabs(U[c + 2 * n]) / dy) / U[c];
+        const unsigned long long b = (unsigned long long)__double_as_longlong(s);
+        m = b > m ? b : m;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const unsigned long long o = __shfl_xor(m, off);
+        m                          = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m != 0ULL)
+        atomicMax(out, m);
+}
+
+__global__ void k_peek_zero_u64(unsigned long long *p) { *p = 0ULL; }
+
+} // namespace bx
+
+using namespace bx;
+
+// the 2D launches put 4 rows in a block and the rows on gridDim.y (<= 65535)
+static bool peek_sizes_ok(int nx, int ny)
+{
+    return nx >= 1 && ny >= 1 && ny <= 4 * 65535 && (long long)nx * ny <= (1LL << 31);
+}
+
+static dim3 peek_grid(int nx, int ny) { return dim3((nx + 63) / 64, (ny + 3) / 4); }
+
+#define PEEK_CHECK(expr)                                                         \
+    do                                                                           \
+    {                                                                            \
+        hipError_t e_ = (expr);                                                  \
+        if (e_ != hipSuccess)                                                    \
+            return bx_helm_fail(BX_ERR_HIP, std::string(#expr) + ": "            \
+                                                + hipGetErrorString(e_));        \
+    } while (0)
+
+extern "C" int bx_imex_pressure(const double *U, int nx, int ny, double gamma, double eps,
+                                double *p, void *stream)
+{
+    if (!U || !p || !peek_sizes_ok(nx, ny))
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_pressure: bad argument");
+    const long long n = (long long)nx * ny;
+    hipLaunchKernelGGL(k_peek_pressure, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, U, p, n, gamma, eps);
+    PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, double dy,
+                                    double cfl, double *dt, void *stream)
+{
+    if (!U || !dt || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0))
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_advective_dt: bad argument");
+    hipStream_t         s = (hipStream_t)stream;
+    unsigned long long *d = nullptr;
+    {
+        // one 8-byte accumulator per device, kept until the process ends
+        static std::mutex          mu;
+        static unsigned long long *acc[64] = {};
+        int                        dev     = 0;
+        PEEK_CHECK(hipGetDevice(&dev));
+        if (dev < 0 || dev >= 64)
+            return bx_helm_fail(BX_ERR_UNSUPPORTED, "bx_imex_advective_dt: device id");
+        std::lock_guard<std::mutex> lk(mu);
+        if (!acc[dev])
+            PEEK_CHECK(hipMalloc((void **)&acc[dev], sizeof(unsigned long long)));
+        d = acc[dev];
+    }
+    hipLaunchKernelGGL(k_peek_zero_u64, dim3(1), dim3(1), 0, s, d);
+    const long long n    = (long long)nx * ny;
+    const int       grid = (int)std::min<long long>((n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_peek_speed, dim3(grid), dim3(256), 0, s, U, n, dx, dy, d);
+    PEEK_CHECK(hipGetLastError());
+    unsigned long long bits = 0;
+    PEEK_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, s));
+    PEEK_CHECK(hipStreamSynchronize(s));
+    // dt_est = 1 / max_dx_scaled_speed; dt_est *= cfl (AmrLevelAdv.cpp:435, 466)
+    double smax;
+    static_assert(sizeof(smax) == sizeof(bits), "");
+    memcpy(&smax, &bits, sizeof(smax));
+    double dt_est = 1 / smax;
+    dt_est *= cfl;
+    *dt = dt_est;
+    return BX_OK;
+}
+
+extern "C" int bx_imex_peek_assemble(const double *U, const double *p, int nx, int ny,
+                                     double dx, double dy, double dt, double gamma,
+                                     double eps, double *Uex, double *enth, double *rhs,
+                                     double *a, double *bX, double *bY, void *stream)
+{
+    if (!U || !p || !Uex || !enth || !rhs || !a || !bX || !bY || !peek_sizes_ok(nx, ny)
+        || !(dx > 0) || !(dy > 0) || U == Uex)
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_assemble: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    PeekGeom    g{ nx, ny, (long long)nx * ny };
+    // dtdx = dt / dx (IMEX_Fluxes.cpp:93); hdtdx = 0.5 dt idx (IMEX_O1.cpp:66)
+    const double dtdx  = dt / dx;
+    const double dtdy  = dt / dy;
+    const double hdtdx = 0.5 * dt * (1.0 / dx);
+    const double hdtdy = 0.5 * dt * (1.0 / dy);
+    hipLaunchKernelGGL(k_peek_advect, peek_grid(nx, ny), dim3(64, 4), 0, s, U, Uex, g, eps,
+                       dtdx, dtdy);
+    PEEK_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_peek_system, peek_grid(nx, ny), dim3(64, 4), 0, s, Uex, p, enth, rhs,
+                       a, bX, bY, g, gamma, eps, hdtdx, hdtdy);
+    PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
+                                   const double *p_new, int nx, int ny, double dx, double dy,
+                                   double dt, double eps, double *U_new, void *stream)
+{
+    if (!U || !Uex || !enth || !p_new || !U_new || !peek_sizes_ok(nx, ny) || !(dx > 0)
+        || !(dy > 0) || U == U_new || Uex == U_new)
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_finish: bad argument");
+    PeekGeom     g{ nx, ny, (long long)nx * ny };
+    const double dtdx   = dt / dx;
+    const double dtdy   = dt / dy;
+    const double heps   = 0.5 / eps;                   // IMEX_O1.cpp:499
+    const double hdtdxe = 0.5 * dt * (1.0 / dx) / eps; // IMEX_O1.cpp:66-70
+    const double hdtdye = 0.5 * dt * (1.0 / dy) / eps;
+    hipLaunchKernelGGL(k_peek_finish, peek_grid(nx, ny), dim3(64, 4), 0, (hipStream_t)stream,
+                       U, Uex, enth, p_new, U_new, g, eps, dtdx, dtdy, heps, hdtdxe, hdtdye);
+    PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}

@@ -1,24 +1,22 @@
 """IMEX PEEK stage on a 2D periodic level and the pressure Helmholtz solve.
 
-Host side of the learned-preconditioner path.  The stage assembly is the
-caller of the pressure solve (in the reference, AMReX host code around
-MLMG); the solve itself runs on the GPU through the C ABI
-(`bx_helm_*`, `bx_cnn_*`).
+Host side of the learned-preconditioner path.  Every array of a step lives
+on the GPU: the stage's explicit work runs in the bx_imex_* kernels, the
+pressure solve in bx_helm_* / bx_cnn_*; the host only sequences the calls
+and holds the scalars (dt, alpha, beta).
 
-PEEK-SP111 stage (one implicit stage, adt = dt), following
+PEEK-SP111 stage (one implicit stage, adt = dt), linearised, enthalpy from
+the previous pressure, kinetic energy of the explicit state:
   compute_flux_imex_o1            Source/IMEX/IMEX_O1.cpp:18-175
   advance_rusanov_adv             Source/IMEX/IMEX_Fluxes.cpp:10-118
-  adv_flux / kinetic_energy       Source/IMEX_K/euler_K.H:10-65
-  compute_enthalpy (reuse_pressure), compute_ke (ex)
-                                  Source/IMEX/IMEX_O1.cpp:177-262
-  pressure_source_vector          Source/IMEX/IMEX_O1.cpp:416-452
-  setup_pressure_op               Source/IMEX/IMEX_O1.cpp:570-614
+  compute_enthalpy / pressure_source_vector / setup_pressure_op
+                                  Source/IMEX/IMEX_O1.cpp:177-228, 416-452, 570-614
   update_momentum_flux / update_energy_flux
                                   Source/IMEX/IMEX_O1.cpp:480-566
   conservative_update + sum_fluxes (SP111: b = 1)
                                   Source/IMEX/IMEX_RK.cpp:22-96
-Arrays are numpy (ny, nx) in x-fastest order, the same order as the device
-buffers; U has 4 components (rho, m_x, m_y, E).
+Device arrays are (ny, nx) in x-fastest order; U has 4 components
+(rho, m_x, m_y, E).
 """
 import ctypes
 import math
@@ -37,8 +35,9 @@ def _chk(rc, what, err="bx_helm_last_error"):
 
 # ------------------------------------------------------------------ problem
 def gresho_state(n, mach=0.01, gamma=1.4, eps=1.0):
-    """Gresho vortex on [-0.5,0.5]^2 (Exec/GreshoVortex/Prob.cpp:17-106),
-    rho = 1, p0 = rho / (gamma M^2); periodic variant of the problem."""
+    """Initial data of the Gresho vortex on [-0.5,0.5]^2
+    (Exec/GreshoVortex/Prob.cpp:17-106), rho = 1, p0 = rho / (gamma M^2);
+    periodic variant of the problem.  Host arrays (problem set-up)."""
     dx = 1.0 / n
     c = (np.arange(n) + 0.5) * dx - 0.5
     x, y = np.meshgrid(c, c)          # (ny, nx), x fastest
@@ -61,102 +60,6 @@ def gresho_state(n, mach=0.01, gamma=1.4, eps=1.0):
     return U, dx
 
 
-def kinetic_energy(U, eps):
-    return eps * (U[1] * U[1] + U[2] * U[2]) / (2 * U[0])
-
-
-def pressure_from_state(U, gamma, eps):
-    return (gamma - 1) * (U[3] - kinetic_energy(U, eps))
-
-
-def advective_dt(U, dx, cfl=0.45):
-    """Advective time step of the IMEX schemes (AmrLevelAdv.cpp:435):
-    cfl / max(|u|/dx + |v|/dy)."""
-    s = np.max(np.abs(U[1] / U[0]) / dx + np.abs(U[2] / U[0]) / dx)
-    return cfl / s
-
-
-def _sh(a, d, k):
-    """a at index i + k along direction d (periodic); arrays are (.., ny, nx)."""
-    return np.roll(a, -k, axis=-1 - d)
-
-
-def _rusanov_faces(U, eps):
-    """Rusanov advective face fluxes F_d at faces i-1/2 (index i),
-    IMEX_Fluxes.cpp:73-99."""
-    ke = kinetic_energy(U, eps)
-    out = []
-    for d in range(2):
-        u = U[1 + d] / U[0]
-        F = np.empty_like(U)
-        F[0] = U[1 + d]
-        F[1] = U[1] * u
-        F[2] = U[2] * u
-        F[3] = u * ke
-        Fm = _sh(F, d, -1)
-        Um = _sh(U, d, -1)
-        umax = np.maximum(np.abs(u), np.abs(_sh(u, d, -1)))
-        out.append(0.5 * ((F + Fm) - umax * (U - Um)))
-    return out
-
-
-class PeekStage:
-    """One PEEK-SP111 step: explicit advection, pressure system, solve,
-    flux correction and conservative update."""
-
-    def __init__(self, gamma=1.4, eps=1.0):
-        self.gamma = gamma
-        self.eps = eps
-
-    def assemble(self, U, p, dt, dx):
-        """Returns (U_ex, enthalpy, rhs, a, bX, bY, beta) for the stage with
-        adt = dt.  bX is (ny, nx+1), bY is (ny+1, nx)."""
-        g, eps = self.gamma, self.eps
-        Fx, Fy = _rusanov_faces(U, eps)
-        dtdx = dt / dx
-        Uex = U - (dtdx * (_sh(Fx, 0, 1) - Fx) + dtdx * (_sh(Fy, 1, 1) - Fy))
-        enth = p * g / ((g - 1) * Uex[0])
-        ke = kinetic_energy(Uex, eps)
-        hdt = 0.5 * dt / dx
-        rhs = eps * (Uex[3] - ke - (hdt * (_sh(enth, 0, 1) * _sh(Uex[1], 0, 1)
-                                           - _sh(enth, 0, -1) * _sh(Uex[1], 0, -1))
-                                    + hdt * (_sh(enth, 1, 1) * _sh(Uex[2], 1, 1)
-                                             - _sh(enth, 1, -1) * _sh(Uex[2], 1, -1))))
-        # face enthalpy: arithmetic average of the two neighbours
-        hx = 0.5 * (_sh(enth, 0, -1) + enth)       # face i-1/2 at index i
-        hy = 0.5 * (_sh(enth, 1, -1) + enth)
-        ny, nx = p.shape
-        bX = np.empty((ny, nx + 1))
-        bX[:, :nx] = hx
-        bX[:, nx] = hx[:, 0]
-        bY = np.empty((ny + 1, nx))
-        bY[:ny] = hy
-        bY[ny] = hy[0]
-        a = np.full_like(p, eps / (g - 1))
-        return dict(Uex=Uex, enth=enth, rhs=rhs, a=a, bX=bX, bY=bY,
-                    alpha=1.0, beta=dt * dt, Fx=Fx, Fy=Fy)
-
-    def finish(self, U, st, p_new, dt, dx):
-        """Pressure and enthalpy flux corrections, then U_new = U - dt/dx div F
-        (IMEX_O1.cpp:480-566, IMEX_RK.cpp:93-95)."""
-        eps = self.eps
-        heps = 0.5 / eps
-        hdtdxe = 0.5 * dt / dx / eps
-        Fx = st["Fx"].copy()
-        Fy = st["Fy"].copy()
-        h, ex = st["enth"], st["Uex"]
-        pm = _sh(p_new, 0, -1)
-        Fx[1] += heps * (p_new + pm)
-        Fx[3] += (0.5 * (h * ex[1] + _sh(h, 0, -1) * _sh(ex[1], 0, -1))
-                  - hdtdxe * ((h + _sh(h, 0, -1)) * (p_new - pm)))
-        pm = _sh(p_new, 1, -1)
-        Fy[2] += heps * (p_new + pm)
-        Fy[3] += (0.5 * (h * ex[2] + _sh(h, 1, -1) * _sh(ex[2], 1, -1))
-                  - hdtdxe * ((h + _sh(h, 1, -1)) * (p_new - pm)))
-        dtdx = dt / dx
-        return U - (dtdx * (_sh(Fx, 0, 1) - Fx) + dtdx * (_sh(Fy, 1, 1) - Fy))
-
-
 # ------------------------------------------------------------- GPU solver
 class DeviceArray:
     """A device allocation holding one float64/float32 numpy-shaped array."""
@@ -173,6 +76,12 @@ class DeviceArray:
         a = np.ascontiguousarray(a, dtype=self.dtype)
         assert a.nbytes == self.nbytes, (a.shape, self.shape)
         _lib.check(lib().bx_memcpy_htod(self.ptr, a.ctypes.data, self.nbytes), "htod")
+        return self
+
+    def copy_from(self, other):
+        """Device-to-device copy of a DeviceArray of the same size."""
+        assert other.nbytes == self.nbytes, (other.shape, self.shape)
+        _lib.check(lib().bx_memcpy_dtod(self.ptr, other.ptr, self.nbytes), "dtod")
         return self
 
     def download(self):
@@ -193,7 +102,67 @@ class DeviceArray:
 
 
 def dev(a, dtype=np.float64):
+    """A new device array holding a (host array) or a copy of a (DeviceArray)."""
+    if isinstance(a, DeviceArray):
+        return _copy_of(a)
     return DeviceArray(np.shape(a), dtype).upload(a)
+
+
+def _copy_of(d):
+    """A new device array holding a device-to-device copy of d."""
+    out = DeviceArray(d.shape, d.dtype)
+    _lib.check(lib().bx_memcpy_dtod(out.ptr, d.ptr, d.nbytes), "dtod")
+    return out
+
+
+class PeekStage:
+    """Explicit work of one PEEK-SP111 step on an nx x ny periodic level
+    (bx_imex_*): owns the stage's device arrays Uex, enth, rhs, a, bX, bY."""
+
+    def __init__(self, nx, ny, dx, dy=None, gamma=1.4, eps=1.0):
+        self.nx, self.ny = nx, ny
+        self.dx = dx
+        self.dy = dx if dy is None else dy
+        self.gamma, self.eps = gamma, eps
+        self.Uex = DeviceArray((4, ny, nx))
+        self.enth = DeviceArray((ny, nx))
+        self.rhs = DeviceArray((ny, nx))
+        self.a = DeviceArray((ny, nx))
+        self.bX = DeviceArray((ny, nx + 1))
+        self.bY = DeviceArray((ny + 1, nx))
+
+    def pressure(self, U, p):
+        """p = (gamma - 1)(E - ke) on the device."""
+        _chk(lib().bx_imex_pressure(U.ptr, self.nx, self.ny, self.gamma, self.eps, p.ptr, None),
+             "bx_imex_pressure")
+
+    def advective_dt(self, U, cfl=0.45):
+        out = ctypes.c_double()
+        _chk(lib().bx_imex_advective_dt(U.ptr, self.nx, self.ny, self.dx, self.dy, cfl,
+                                        ctypes.byref(out), None), "bx_imex_advective_dt")
+        return out.value
+
+    def assemble(self, U, p, dt):
+        """Explicit update and pressure system from the state U and the
+        previous pressure p; returns the system (device arrays and the
+        operator scalars alpha = 1, beta = dt^2)."""
+        _chk(lib().bx_imex_peek_assemble(U.ptr, p.ptr, self.nx, self.ny, self.dx, self.dy, dt,
+                                         self.gamma, self.eps, self.Uex.ptr, self.enth.ptr,
+                                         self.rhs.ptr, self.a.ptr, self.bX.ptr, self.bY.ptr, None),
+             "bx_imex_peek_assemble")
+        return dict(Uex=self.Uex, enth=self.enth, rhs=self.rhs, a=self.a, bX=self.bX, bY=self.bY,
+                    alpha=1.0, beta=dt * dt, dt=dt)
+
+    def finish(self, U, p_new, dt, U_out):
+        """Flux corrections with the solved pressure and the conservative
+        update into U_out (not U)."""
+        _chk(lib().bx_imex_peek_finish(U.ptr, self.Uex.ptr, self.enth.ptr, p_new.ptr, self.nx,
+                                       self.ny, self.dx, self.dy, dt, self.eps, U_out.ptr, None),
+             "bx_imex_peek_finish")
+
+    def free(self):
+        for d in (self.Uex, self.enth, self.rhs, self.a, self.bX, self.bY):
+            d.free()
 
 
 class HelmholtzSolver:
@@ -210,9 +179,13 @@ class HelmholtzSolver:
         self.cnn = None
 
     def set_coeffs(self, alpha, beta, a, bX, bY, gamma=0.0, cX=None, cY=None):
-        da, dbx, dby = dev(a), dev(bX), dev(bY)
-        dcx = dev(cX) if cX is not None else None
-        dcy = dev(cY) if cY is not None else None
+        """Coefficients as host arrays or DeviceArrays (device arrays are
+        copied: the attached network keeps pointers to the solver's own)."""
+        def own(x):
+            return _copy_of(x) if isinstance(x, DeviceArray) else dev(x)
+        da, dbx, dby = own(a), own(bX), own(bY)
+        dcx = own(cX) if cX is not None else None
+        dcy = own(cY) if cY is not None else None
         _chk(lib().bx_helm_set_coeffs(self.h, alpha, beta, gamma, da.ptr, dbx.ptr, dby.ptr,
                                       dcx.ptr if dcx else None, dcy.ptr if dcy else None, None),
              "bx_helm_set_coeffs")
@@ -364,23 +337,38 @@ def harvest_residuals(solver, b_dev, x_dev, R_dev, first, count, rtol=1e-10):
 
 class GreshoWorkload:
     """Successive PEEK-SP111 pressure systems of the low-Mach Gresho vortex
-    on an n x n periodic level: the state advances with the plain multigrid
-    pressure so every method solves the same sequence of systems."""
+    on an n x n periodic level, all on the GPU.  The state advances with
+    the pressure passed to advance() (the benchmark passes the plain
+    multigrid solution, so every method solves the same sequence)."""
 
     def __init__(self, n=1024, mach=0.01, gamma=1.4, eps=1.0, cfl=0.45):
-        self.U, self.dx = gresho_state(n, mach, gamma, eps)
+        U, self.dx = gresho_state(n, mach, gamma, eps)
         self.n = n
-        self.stage = PeekStage(gamma, eps)
-        self.p = pressure_from_state(self.U, gamma, eps)
+        self.U = dev(U)
+        self.U_next = DeviceArray(U.shape)
+        self.p = DeviceArray((n, n))
+        self.stage = PeekStage(n, n, self.dx, gamma=gamma, eps=eps)
+        self.stage.pressure(self.U, self.p)
         self.cfl = cfl
 
     def next_system(self):
-        dt = advective_dt(self.U, self.dx, self.cfl)
-        sy = self.stage.assemble(self.U, self.p, dt, self.dx)
-        sy["dt"] = dt
-        sy["p_prev"] = self.p.copy()
+        """Assembles the next step's system; "p_prev" is the previous
+        pressure (the solve's initial guess, device array)."""
+        dt = self.stage.advective_dt(self.U, self.cfl)
+        sy = self.stage.assemble(self.U, self.p, dt)
+        sy["p_prev"] = self.p
         return sy
 
     def advance(self, sy, p_new):
-        self.U = self.stage.finish(self.U, sy, p_new, sy["dt"], self.dx)
-        self.p = p_new
+        """U <- stage update with the solved pressure p_new (DeviceArray);
+        p <- p_new."""
+        self.stage.finish(self.U, p_new, sy["dt"], self.U_next)
+        self.U, self.U_next = self.U_next, self.U
+        if p_new.ptr.value != self.p.ptr.value:
+            _lib.check(lib().bx_memcpy_dtod(self.p.ptr, p_new.ptr, self.p.nbytes), "dtod")
+        lib().bx_stream_sync(None)
+
+    def free(self):
+        for d in (self.U, self.U_next, self.p):
+            d.free()
+        self.stage.free()

@@ -354,7 +354,7 @@ def imex_leg(args):
         db, dx_ = imex.dev(sy["rhs"]), imex.dev(sy["p_prev"])
         got, st = imex.harvest_residuals(solver, db, dx_, R, nf, nf_max - nf)
         nf += got
-        wl.advance(sy, dx_.download())
+        wl.advance(sy, dx_)
     log("imex: %d residual fields harvested" % nf)
     net = imex.LearnedSmoother(seed=1)
     solver.attach(net, omega=1.0)
@@ -373,11 +373,21 @@ def imex_leg(args):
                ("fgmres_cnn_smoother", "fgmres_cnn", "smoother", True)]
     res = {m[0]: {"seconds": [], "iters": [], "rel_resid": [], "fell_back": 0} for m in methods}
     r = imex.DeviceArray((n, n))
+    stage_ms = {"advective_dt": [], "assemble": [], "set_coeffs": [], "finish": []}
     for k in range(args.imex_steps):
-        sy = wl.next_system()
+        t0 = time.perf_counter()
+        dt = wl.stage.advective_dt(wl.U, wl.cfl)
+        t1 = time.perf_counter()
+        sy = wl.stage.assemble(wl.U, wl.p, dt)
+        sy["p_prev"] = wl.p
+        imex.lib().bx_stream_sync(None)
+        t2 = time.perf_counter()
         solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+        t3 = time.perf_counter()
+        for key, v in (("advective_dt", t1 - t0), ("assemble", t2 - t1), ("set_coeffs", t3 - t2)):
+            stage_ms[key].append(v * 1e3)
         db = imex.dev(sy["rhs"])
-        bn = float(np.linalg.norm(sy["rhs"]))
+        bn = float(np.linalg.norm(sy["rhs"].download()))
         p_mg = None
         for label, meth, mode, graph in methods:
             if mode is not None:
@@ -386,7 +396,7 @@ def imex_leg(args):
             dx_ = imex.dev(sy["p_prev"])
             # untimed solve first: graph capture after a configuration change
             solver.solve(db, dx_, meth, rtol=1e-10, maxiter=200)
-            dx_.upload(sy["p_prev"])
+            dx_.copy_from(sy["p_prev"])
             st = solver.solve(db, dx_, meth, rtol=1e-10, maxiter=200)
             solver.residual(db, dx_, r)
             rel = float(np.linalg.norm(r.download())) / bn
@@ -395,7 +405,7 @@ def imex_leg(args):
             res[label]["rel_resid"].append(rel)
             res[label]["fell_back"] += int(st["fell_back"])
             if label == "mg":
-                p_mg = dx_.download()
+                p_mg = dx_
         if k == args.imex_steps - 1:
             # cold start (zero initial guess) on the last system: where a
             # better preconditioner saves iterations
@@ -410,7 +420,9 @@ def imex_leg(args):
                 st = solver.solve(db, dx_, meth, rtol=1e-10, maxiter=200)
                 cold[label] = {"s_per_solve": st["seconds"], "iters": st["iters"],
                                "fell_back": int(st["fell_back"])}
+        t0 = time.perf_counter()
         wl.advance(sy, p_mg)
+        stage_ms["finish"].append((time.perf_counter() - t0) * 1e3)
         log("imex step %d: " % k + "  ".join(
             "%s %.3f ms/%d it" % (m[0], res[m[0]]["seconds"][-1] * 1e3, res[m[0]]["iters"][-1])
             for m in methods))
@@ -426,10 +438,15 @@ def imex_leg(args):
                       "fell_back": res[label]["fell_back"], "network_mode": mode,
                       "graph_preconditioner": graph}
     out["cold_start"] = cold
+    # explicit work of a step on the GPU (bx_imex_*; host clock around
+    # synchronised calls): dt reduction, stage assembly, coefficient upload
+    # and multigrid hierarchy build, flux correction + update
+    out["stage_ms_median"] = {k: round(float(np.median(v)), 4) for k, v in stage_ms.items()}
     out["speedup_cnn_vs_mg"] = out["mg"]["s_per_solve"] / out["fgmres_cnn"]["s_per_solve"]
     out["speedup_fgmres_mg_vs_mg"] = out["mg"]["s_per_solve"] / out["fgmres_mg"]["s_per_solve"]
     solver.close()
     net.close()
+    wl.free()
     return out
 
 

@@ -298,7 +298,46 @@ int bx_helm_set_fused(bx_helm *h, int fused);
  * (st->fell_back = 1).  Synchronous. */
 int bx_helm_solve(bx_helm *h, const double *b, double *x, int method,
                   double rtol, int maxiter, bx_solve_stats *st, void *stream);
+/* Describes the last failure of a bx_helm_*, bx_cnn_* or bx_imex_* call. */
 const char *bx_helm_last_error(void);
+
+/* Explicit work of the IMEX PEEK-SP111 stage (linearised, enthalpy from the
+ * previous pressure, kinetic energy of the explicit state) on one nx x ny
+ * periodic 2D level.  Device arrays: U, Uex, U_new hold 4 components
+ * (rho, m_x, m_y, E) of nx*ny doubles each, x fastest, no ghost cells;
+ * p, enth, rhs, a hold nx*ny; bX is ny rows of nx+1 faces, bY ny+1 rows of
+ * nx (the layout bx_helm_set_coeffs takes).  Asynchronous on `stream`
+ * unless stated.
+ *
+ * pressure (Source/IMEX_K/euler_K.H:23-35): p = (gamma-1)(E - ke). */
+int bx_imex_pressure(const double *U, int nx, int ny, double gamma, double eps,
+                     double *p, void *stream);
+/* Advective time step of the IMEX schemes (estTimeStep, AmrLevelAdv.cpp:
+ * 431-436 and 466): cfl / max over cells of (|m_x|/dx + |m_y|/dy)/rho
+ * (max_dx_scaled_speed, Source/System/Euler/Euler.cpp:107-129).
+ * Synchronous; writes the host double *dt. */
+int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, double dy,
+                         double cfl, double *dt, void *stream);
+/* First half of compute_flux_imex_o1 (Source/IMEX/IMEX_O1.cpp:18-131):
+ * Rusanov advective update U -> Uex (advance_rusanov_adv,
+ * Source/IMEX/IMEX_Fluxes.cpp:10-118), enthalpy p gamma/((gamma-1) rho_ex)
+ * from the previous pressure p (compute_enthalpy, IMEX_O1.cpp:177-228), the
+ * pressure equation's source rhs (pressure_source_vector, :416-452) and its
+ * coefficients a = eps/(gamma-1), b = face-averaged enthalpy
+ * (setup_pressure_op, :570-614).  The operator's scalars are alpha = 1,
+ * beta = dt*dt. */
+int bx_imex_peek_assemble(const double *U, const double *p, int nx, int ny,
+                          double dx, double dy, double dt, double gamma,
+                          double eps, double *Uex, double *enth, double *rhs,
+                          double *a, double *bX, double *bY, void *stream);
+/* Second half (IMEX_O1.cpp:132-175 with update_momentum_flux /
+ * update_energy_flux, :480-566, and conservative_update, Source/IMEX/
+ * IMEX_RK.cpp:22-96): the Rusanov fluxes of U corrected with the solved
+ * pressure p_new, then U_new = U - dt div F.  U_new must not alias U or
+ * Uex. */
+int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
+                        const double *p_new, int nx, int ny, double dx, double dy,
+                        double dt, double eps, double *U_new, void *stream);
 
 /* Learned smoother: bias-free 3x3 convolutions 1->16->16->16->1 with ReLU,
  * BF16 weights/activations, FP32 accumulation (MFMA for the 16->16 layers).

@@ -13,6 +13,7 @@ import pytest
 
 import cnn_oracle as co
 import helm_oracle as ho
+import imex_oracle as io
 from bamrex_amd import imex
 from bamrex_amd.imex import DeviceArray, HelmholtzSolver, LearnedSmoother, dev
 
@@ -102,9 +103,9 @@ def test_solve_reaches_tolerance_on_gresho_system(method):
     announce("helm %s gresho 256" % method)
     n = 256
     U, dx = imex.gresho_state(n)
-    stage = imex.PeekStage()
-    p = imex.pressure_from_state(U, 1.4, 1.0)
-    dt = imex.advective_dt(U, dx)
+    stage = io.PeekStage()
+    p = io.pressure_from_state(U, 1.4, 1.0)
+    dt = io.advective_dt(U, dx)
     sys_ = stage.assemble(U, p, dt, dx)
     s = HelmholtzSolver(n, n, dx)
     s.set_coeffs(sys_["alpha"], sys_["beta"], sys_["a"], sys_["bX"], sys_["bY"])
@@ -126,7 +127,7 @@ def test_peek_uniform_state_pressure_is_eos():
     U[0] = 1.3
     U[3] = 2.7
     dx = 1.0 / n
-    st_ = imex.PeekStage()
+    st_ = io.PeekStage()
     p0 = np.full((n, n), 1.0)
     sys_ = st_.assemble(U, p0, 1e-3, dx)
     s = HelmholtzSolver(n, n, dx)
@@ -214,12 +215,12 @@ def gresho_systems(n, nsteps, seed_state=None):
     """(a, bX, bY, beta, rhs, p) for successive PEEK-SP111 steps of the
     Gresho vortex, solved on the GPU with plain multigrid."""
     U, dx = imex.gresho_state(n)
-    stage = imex.PeekStage()
-    p = imex.pressure_from_state(U, 1.4, 1.0)
+    stage = io.PeekStage()
+    p = io.pressure_from_state(U, 1.4, 1.0)
     out = []
     s = HelmholtzSolver(n, n, dx)
     for _ in range(nsteps):
-        dt = imex.advective_dt(U, dx)
+        dt = io.advective_dt(U, dx)
         sy = stage.assemble(U, p, dt, dx)
         s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
         db, dp = dev(sy["rhs"]), dev(p)
@@ -320,13 +321,14 @@ def test_trained_network_gain():
     nf = 0
     counts = []
     for k in range(9):
-        sy = wl.next_system()
-        s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
-        db, dxv = dev(sy["rhs"]), dev(sy["p_prev"])
+        syd = wl.next_system()
+        s.set_coeffs(syd["alpha"], syd["beta"], syd["a"], syd["bX"], syd["bY"])
+        db, dxv = dev(syd["rhs"]), dev(syd["p_prev"])
         got, _ = imex.harvest_residuals(s, db, dxv, R, nf, 48 - nf)
         counts.append(got)
         nf += got
-        wl.advance(sy, dxv.download())
+        sy = {k_: (v.download() if isinstance(v, DeviceArray) else v) for k_, v in syd.items()}
+        wl.advance(syd, dxv)
     # the last system's residual fields are held out; the solver still
     # holds that system's coefficients
     n_train = nf - counts[-1]

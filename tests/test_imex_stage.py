@@ -1,5 +1,5 @@
-"""CPU tests of the PEEK stage assembly (bamrex_amd/imex.py), the host side
-of the pressure solve, restating the reference's own IMEX tests."""
+"""CPU tests of the oracle's PEEK-SP111 stage (oracle/imex_oracle.py), the
+checker of the bx_imex_* kernels, restating the reference's own IMEX tests."""
 import os
 import sys
 
@@ -9,6 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
 
 import helm_oracle as ho  # noqa: E402
+import imex_oracle as io  # noqa: E402
 from bamrex_amd import imex  # noqa: E402
 
 
@@ -28,7 +29,7 @@ def test_rusanov_flux_known_answers():
     nx, ny = 8, 2
     u = np.where(np.arange(nx) < 4, -2.0, 2.0)[None, :].repeat(ny, 0)
     U = state_from_primv(np.ones((ny, nx)), u, np.zeros((ny, nx)), np.full((ny, nx), 0.4), 1.4, 1.0)
-    Fx, _ = imex._rusanov_faces(U, 1.0)
+    Fx, _ = io.rusanov_faces(U, 1.0)
     # face index i is the face between cells i-1 and i
     for i, exp in ((1, (-2, 4, -4)), (2, (-2, 4, -4)), (4, (0, 0, 0)), (5, (2, 4, 4)), (6, (2, 4, 4))):
         got = (Fx[0][0, i], Fx[1][0, i], Fx[3][0, i])
@@ -54,7 +55,7 @@ def test_peek_energy_matches_pressure_and_explicit_ke():
     p = 1.0 + 0.2 * np.cos(2 * np.pi * (X + Y))
     U = state_from_primv(rho, u, v, p, gamma, eps)
     dx = 1.0 / n
-    stage = imex.PeekStage(gamma, eps)
+    stage = io.PeekStage(gamma, eps)
     sy = stage.assemble(U, p, dt, dx)
     pn, its, _ = ho.mg_solve(sy["rhs"], sy["a"], sy["bX"], sy["bY"], sy["alpha"], sy["beta"], dx, dx,
                              x0=p, rtol=1e-15, maxiter=60)
@@ -71,8 +72,8 @@ def test_peek_step_conserves_and_keeps_uniform_flow():
     gamma, eps = 1.4, 1.0
     U = state_from_primv(np.full((n, n), 1.2), np.full((n, n), 0.3), np.full((n, n), -0.2),
                          np.full((n, n), 2.0), gamma, eps)
-    p = imex.pressure_from_state(U, gamma, eps)
-    stage = imex.PeekStage(gamma, eps)
+    p = io.pressure_from_state(U, gamma, eps)
+    stage = io.PeekStage(gamma, eps)
     dx = 1.0 / n
     sy = stage.assemble(U, p, 0.02, dx)
     pn, its, _ = ho.mg_solve(sy["rhs"], sy["a"], sy["bX"], sy["bY"], sy["alpha"], sy["beta"], dx, dx,
@@ -81,8 +82,8 @@ def test_peek_step_conserves_and_keeps_uniform_flow():
     assert np.allclose(Unew, U, rtol=1e-13, atol=0)
     # Gresho vortex: mass, momentum and energy totals are conserved
     Ug, dxg = imex.gresho_state(64)
-    pg = imex.pressure_from_state(Ug, gamma, eps)
-    dtg = imex.advective_dt(Ug, dxg)
+    pg = io.pressure_from_state(Ug, gamma, eps)
+    dtg = io.advective_dt(Ug, dxg)
     sy = stage.assemble(Ug, pg, dtg, dxg)
     pn, its, _ = ho.mg_solve(sy["rhs"], sy["a"], sy["bX"], sy["bY"], sy["alpha"], sy["beta"], dxg, dxg,
                              x0=pg, rtol=1e-12, maxiter=60)
@@ -90,3 +91,44 @@ def test_peek_step_conserves_and_keeps_uniform_flow():
     tot0, tot1 = Ug.sum(axis=(1, 2)), Un.sum(axis=(1, 2))
     scale = np.abs(Ug).sum(axis=(1, 2))
     assert np.all(np.abs(tot1 - tot0) <= 1e-13 * scale)
+
+
+def test_advective_dt_brute_force_and_anisotropic_cells():
+    """cfl / max (|m_x|/dx + |m_y|/dy)/rho, cell by cell in plain Python;
+    with dy != dx a transposed operand or swapped spacing changes it."""
+    rng = np.random.default_rng(11)
+    U = np.empty((4, 5, 7))
+    U[0] = rng.uniform(0.5, 2.0, (5, 7))
+    U[1] = rng.uniform(-1, 1, (5, 7))
+    U[2] = rng.uniform(-3, 3, (5, 7))
+    U[3] = 10.0
+    dx, dy = 0.1, 0.025
+    best = 0.0
+    for j in range(5):
+        for i in range(7):
+            best = max(best, (abs(U[1, j, i]) / dx + abs(U[2, j, i]) / dy) / U[0, j, i])
+    assert io.advective_dt(U, dx, dy, cfl=0.45) == (1 / best) * 0.45
+    assert io.advective_dt(U, dx, dy, cfl=0.45) != io.advective_dt(U, dy, dx, cfl=0.45)
+
+
+def test_stage_anisotropic_free_stream_and_conservation():
+    """Uniform flow stays uniform with dx != dy, and a non-uniform state's
+    totals are conserved (periodic fluxes cancel) with dx != dy."""
+    ny, nx = 12, 20
+    gamma, eps = 1.4, 0.3
+    U = state_from_primv(np.full((ny, nx), 0.8), np.full((ny, nx), 0.5),
+                         np.full((ny, nx), -0.25), np.full((ny, nx), 3.0), gamma, eps)
+    st = io.PeekStage(gamma, eps)
+    dx, dy, dt = 1.0 / nx, 0.5 / ny, 0.01
+    p = io.pressure_from_state(U, gamma, eps)
+    sy = st.assemble(U, p, dt, dx, dy)
+    Un = st.finish(U, sy, p, dt, dx, dy)
+    assert np.allclose(Un, U, rtol=1e-14, atol=0)
+    rng = np.random.default_rng(2)
+    U2 = state_from_primv(1 + 0.2 * rng.random((ny, nx)), rng.uniform(-1, 1, (ny, nx)),
+                          rng.uniform(-1, 1, (ny, nx)), 2 + rng.random((ny, nx)), gamma, eps)
+    p2 = io.pressure_from_state(U2, gamma, eps)
+    sy2 = st.assemble(U2, p2, dt, dx, dy)
+    Un2 = st.finish(U2, sy2, 1.01 * p2, dt, dx, dy)
+    scale = np.abs(U2).sum(axis=(1, 2))
+    assert np.all(np.abs(Un2.sum(axis=(1, 2)) - U2.sum(axis=(1, 2))) <= 1e-13 * scale)

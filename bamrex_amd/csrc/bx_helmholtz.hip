@@ -477,6 +477,42 @@ __global__ void k_avg_faces_x(const T *__restrict__ f, T *__restrict__ c,
                     + f[(long long)(2 * J + 1) * (nfx + 1) + 2 * I]);
 }
 
+// One launch per coarse level for all its coefficients: a (and c) by cell
+// average (k_restrict), bX / bY by face average (k_avg_faces_x / _y); the
+// same expressions, so the same bits.  Grid (ncx + 1) x (ncy + 1).
+template <class T>
+__global__ __launch_bounds__(256) void k_coarsen_coeffs(
+    const T *__restrict__ fa, const T *__restrict__ fbX, const T *__restrict__ fbY,
+    const T *__restrict__ fcX, const T *__restrict__ fcY, T *__restrict__ ca,
+    T *__restrict__ cbX, T *__restrict__ cbY, T *__restrict__ ccX, T *__restrict__ ccY,
+    int ncx, int ncy)
+{
+    const int I = blockIdx.x * blockDim.x + threadIdx.x; // 0..ncx
+    const int J = blockIdx.y;                            // 0..ncy
+    if (I > ncx)
+        return;
+    const int nfx = 2 * ncx;
+    if (J < ncy)
+    {
+        cbX[(long long)J * (ncx + 1) + I]
+            = T(0.5) * (fbX[(long long)(2 * J) * (nfx + 1) + 2 * I]
+                        + fbX[(long long)(2 * J + 1) * (nfx + 1) + 2 * I]);
+        if (I < ncx)
+        {
+            ca[(long long)J * ncx + I] = restrict_at(fa, ncx, I, J);
+            if (fcX)
+            {
+                ccX[(long long)J * ncx + I] = restrict_at(fcX, ncx, I, J);
+                ccY[(long long)J * ncx + I] = restrict_at(fcY, ncx, I, J);
+            }
+        }
+    }
+    if (I < ncx)
+        cbY[(long long)J * ncx + I] = T(0.5)
+                                      * (fbY[(long long)(2 * J) * nfx + 2 * I]
+                                         + fbY[(long long)(2 * J) * nfx + 2 * I + 1]);
+}
+
 template <class T>
 __global__ void k_avg_faces_y(const T *__restrict__ f, T *__restrict__ c,
                               int ncx, int ncy)
@@ -564,6 +600,38 @@ __global__ __launch_bounds__(256) void k_convert(const TI *__restrict__ in,
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
         out[i] = (TO)(in[i] * scale);
+}
+
+// Level-0 coefficients of a hierarchy from the FP64 fine arrays, one launch:
+// a, c over nx*ny cells, bX over (nx+1)*ny faces, bY over nx*(ny+1).
+template <class T>
+__global__ __launch_bounds__(256) void k_convert_coeffs(
+    const double *__restrict__ a, const double *__restrict__ bX,
+    const double *__restrict__ bY, const double *__restrict__ cX,
+    const double *__restrict__ cY, T *__restrict__ oa, T *__restrict__ obX,
+    T *__restrict__ obY, T *__restrict__ ocX, T *__restrict__ ocY, int nx, int ny)
+{
+    const long long n   = (long long)nx * ny;
+    const long long nbx = (long long)(nx + 1) * ny;
+    const long long nby = (long long)nx * (ny + 1);
+    const long long nm  = nbx > nby ? nbx : nby;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nm;
+         i += (long long)gridDim.x * blockDim.x)
+    {
+        if (i < n)
+        {
+            oa[i] = (T)a[i];
+            if (cX)
+            {
+                ocX[i] = (T)cX[i];
+                ocY[i] = (T)cY[i];
+            }
+        }
+        if (i < nbx)
+            obX[i] = (T)bX[i];
+        if (i < nby)
+            obY[i] = (T)bY[i];
+    }
 }
 
 template <class T>
@@ -907,38 +975,19 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
             mg.tail_lds = (int)used;
         }
     }
-    // level 0 coefficients from the FP64 fine arrays
+    // level 0 coefficients from the FP64 fine arrays, then one averaging
+    // launch per coarse level
     MGLevel<T> &L0 = mg.lev[0];
-    const long long n0 = (long long)h->nx * h->ny;
-    hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, s, h->a,
-                       L0.a, n0, 1.0);
-    hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, s, h->bX,
-                       L0.bX, (long long)(h->nx + 1) * h->ny, 1.0);
-    hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, s, h->bY,
-                       L0.bY, (long long)h->nx * (h->ny + 1), 1.0);
-    if (h->has_c)
-    {
-        hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, s,
-                           h->cX, L0.cX, n0, 1.0);
-        hipLaunchKernelGGL((k_convert<T, double>), dim3(1024), dim3(256), 0, s,
-                           h->cY, L0.cY, n0, 1.0);
-    }
+    hipLaunchKernelGGL((k_convert_coeffs<T>), dim3(1024), dim3(256), 0, s, h->a, h->bX,
+                       h->bY, h->has_c ? h->cX : nullptr, h->has_c ? h->cY : nullptr, L0.a,
+                       L0.bX, L0.bY, L0.cX, L0.cY, h->nx, h->ny);
     for (size_t l = 1; l < mg.lev.size(); ++l)
     {
         MGLevel<T> &F = mg.lev[l - 1], &C = mg.lev[l];
-        hipLaunchKernelGGL((k_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s,
-                           F.a, C.a, C.nx, C.ny);
-        hipLaunchKernelGGL((k_avg_faces_x<T>), grid2(C.nx + 1, C.ny), dim3(256),
-                           0, s, F.bX, C.bX, C.nx, C.ny);
-        hipLaunchKernelGGL((k_avg_faces_y<T>), grid2(C.nx, C.ny + 1), dim3(256),
-                           0, s, F.bY, C.bY, C.nx, C.ny);
-        if (h->has_c)
-        {
-            hipLaunchKernelGGL((k_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0,
-                               s, F.cX, C.cX, C.nx, C.ny);
-            hipLaunchKernelGGL((k_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0,
-                               s, F.cY, C.cY, C.nx, C.ny);
-        }
+        const bool  c = h->has_c;
+        hipLaunchKernelGGL((k_coarsen_coeffs<T>), grid2(C.nx + 1, C.ny + 1), dim3(256), 0, s,
+                           F.a, F.bX, F.bY, c ? F.cX : nullptr, c ? F.cY : nullptr, C.a, C.bX,
+                           C.bY, C.cX, C.cY, C.nx, C.ny);
     }
     H_CHECK(hipGetLastError());
     return BX_OK;

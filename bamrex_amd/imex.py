@@ -176,16 +176,25 @@ class HelmholtzSolver:
         _chk(lib().bx_helm_create(nx, ny, dx, self.dy, 1, ctypes.byref(h)), "bx_helm_create")
         self.h = h
         self._keep = []
+        self._own = {}
         self.cnn = None
 
     def set_coeffs(self, alpha, beta, a, bX, bY, gamma=0.0, cX=None, cY=None):
         """Coefficients as host arrays or DeviceArrays (device arrays are
         copied: the attached network keeps pointers to the solver's own)."""
-        def own(x):
-            return _copy_of(x) if isinstance(x, DeviceArray) else dev(x)
-        da, dbx, dby = own(a), own(bX), own(bY)
-        dcx = own(cX) if cX is not None else None
-        dcy = own(cY) if cY is not None else None
+        def own(name, x):
+            cur = self._own.get(name)
+            if isinstance(x, DeviceArray):
+                if cur is None or cur.nbytes != x.nbytes:
+                    cur = self._own[name] = DeviceArray(x.shape)
+                return cur.copy_from(x)
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            if cur is None or cur.nbytes != x.nbytes:
+                cur = self._own[name] = DeviceArray(x.shape)
+            return cur.upload(x)
+        da, dbx, dby = own("a", a), own("bX", bX), own("bY", bY)
+        dcx = own("cX", cX) if cX is not None else None
+        dcy = own("cY", cY) if cY is not None else None
         _chk(lib().bx_helm_set_coeffs(self.h, alpha, beta, gamma, da.ptr, dbx.ptr, dby.ptr,
                                       dcx.ptr if dcx else None, dcy.ptr if dcy else None, None),
              "bx_helm_set_coeffs")
@@ -239,6 +248,9 @@ class HelmholtzSolver:
         if self.h:
             lib().bx_helm_destroy(self.h)
             self.h = None
+        for d in self._own.values():
+            d.free()
+        self._own = {}
 
 
 class LearnedSmoother:

@@ -18,5 +18,5 @@ int bx_cnn_reserve(bx_cnn *net, long long npx);
 long long bx_cnn_epoch(const bx_cnn *net);
 
 // Records msg as the text bx_helm_last_error returns and gives back code
-// (shared by the bx_helm_*, bx_cnn_* and bx_imex_* entry points).
+// (shared by the bx_helm_* and bx_imex_* entry points).
 int bx_helm_fail(int code, const std::string &msg);

@@ -298,7 +298,7 @@ int bx_helm_set_fused(bx_helm *h, int fused);
  * (st->fell_back = 1).  Synchronous. */
 int bx_helm_solve(bx_helm *h, const double *b, double *x, int method,
                   double rtol, int maxiter, bx_solve_stats *st, void *stream);
-/* Describes the last failure of a bx_helm_*, bx_cnn_* or bx_imex_* call. */
+/* Describes the last failure of a bx_helm_* or bx_imex_* call. */
 const char *bx_helm_last_error(void);
 
 /* Explicit work of the IMEX PEEK-SP111 stage (linearised, enthalpy from the

@@ -116,6 +116,18 @@ _SIGS = {
     "bx_helm_set_graph": (ctypes.c_int, [P, ctypes.c_int]),
     "bx_helm_set_fused": (ctypes.c_int, [P, ctypes.c_int]),
     "bx_helm_solve": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, P, P]),
+    "bx_amr_create": (ctypes.c_int, [P, P, ctypes.POINTER(P)]),
+    "bx_amr_destroy": (None, [P]),
+    "bx_amr_info": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)]),
+    "bx_amr_avg_down": (ctypes.c_int, [P, P, P, P]),
+    "bx_amr_fill_fine": (ctypes.c_int, [P, P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        P]),
+    "bx_amr_advance": (ctypes.c_int, [P, ctypes.c_double, DP, ctypes.c_double, P, P, P, P, P, P]),
+    "bx_amr_est_dt": (ctypes.c_int, [P, DP, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P,
+                                     ctypes.POINTER(ctypes.c_double), P]),
+    "bx_amr_run": (ctypes.c_int, [P, DP, ctypes.c_double, ctypes.c_double, ctypes.c_int, P, P, P, P, P,
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), P]),
     "bx_helm_last_error": (ctypes.c_char_p, []),
     "bx_imex_pressure": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, P,
                                         P]),

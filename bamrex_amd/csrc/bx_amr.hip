@@ -1,0 +1,1011 @@
+// Two AMR levels of the explicit path, refinement ratio 2, fine level
+// subcycled: the coupling the reference gets from AMReX's Amr/AmrLevel/
+// FluxRegister machinery around AmrLevelAdv.
+//
+//   coarse advance, then two fine advances of dt/2   Amr::timeStep (AMReX)
+//   fine ghost cells at the coarse-fine boundary      FillPatcherFill
+//     (AmrLevelAdv_advance.cpp:121-129): coarse data interpolated in time
+//     (linear between the coarse old and new states) and space (conservative
+//     linear, cell_cons_interp, AmrLevelAdv.cpp:171-175)
+//   flux registers                                    _advance.cpp:61-81, 192-254
+//     (fluxes scaled by dt and the face area; CrseInit with -1, FineAdd with +1)
+//   reflux, avgDown after the fine advances           post_timestep
+//     (AmrLevelAdv.cpp:628-652, 918-987)
+//   dt over both levels                               computeNewDt (:589-618)
+//
+// Fine boxes are in fine index space, aligned to the ratio (even lo, odd
+// hi), inside the fine domain; each coarse cell is either covered by fine
+// cells or not.  The split sweeps run exactly as on a single level; the
+// coarse-fine ghost cells are set once per fine step (on both ping-pong
+// buffers) and, as in the reference, are not refreshed between the
+// directions (fill_consv_boundary fills fine-fine and physical ghosts only).
+// Edge and corner ghost cells at the coarse-fine boundary are not filled:
+// the sweeps read face ghost cells only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bx_level.h"
+
+namespace bx {
+
+// Fine ghost cells [lo, hi] of fine box dst (global fine indices) take the
+// interpolated value of the coarse data around their image x + shift
+// (shift: periodic wrap into the domain), coarse cells of box src.
+struct CFSeg
+{
+    int dst, src;
+    int lo[3], hi[3];
+    int shift[3];
+    int pad;
+};
+
+// The face on the low side of cell i (global index) of box `box`.
+struct RegFace
+{
+    int box;
+    int i[3];
+};
+
+// A coarse cell receiving reflux corrections terms[begin, end).
+struct RefluxTarget
+{
+    int box;
+    int i[3];
+    int begin, end;
+};
+
+struct RefluxTerm
+{
+    int slot;
+    int sign;
+};
+
+// Coarse cells [lo, hi] of box cbox covered by fine box fbox.
+struct AvgSeg
+{
+    int fbox, cbox;
+    int lo[3], hi[3];
+};
+
+__device__ __forceinline__ long long cell_off(const DevBox &b, int ng, int D, int i,
+                                              int j, int k)
+{
+    const int ngy = (D >= 2) ? ng : 0;
+    const int ngz = (D >= 3) ? ng : 0;
+    return b.base + (i - b.lo[0] + ng) + (long long)(j - b.lo[1] + ngy) * b.jstride
+           + (long long)(k - b.lo[2] + ngz) * b.kstride;
+}
+
+// Monotonised-central slope of the coarse data: min(|c|, |l|, |r|) with the
+// sign of the central difference c, zero at an extremum.
+__device__ __forceinline__ double cf_mc_slope(double um, double uc, double up)
+{
+    const double dc = 0.5 * (up - um);
+    const double dl = 2 * (uc - um);
+    const double dr = 2 * (up - uc);
+    if (!(dl * dr > 0))
+        return 0.0;
+    const double m = fmin(fabs(dc), fmin(fabs(dl), fabs(dr)));
+    return dc < 0 ? -m : m;
+}
+
+// Coarse-fine ghost fill.  One block per segment; threads over its cells.
+// Time weights: mode 0 = the coarse old state, 1 = alpha old + beta new.
+// Per component: MC slopes in each direction, scaled by the largest
+// alpha <= 1 that keeps the four (2D) / eight (3D) children of the coarse
+// cell within the minimum and maximum of its 3^D neighbourhood; the child
+// value is u_c + sum_d off_d (alpha s_d), off_d = -1/4 or +1/4.
+template <int D>
+__global__ __launch_bounds__(256) void k_cf_fill(
+    double *__restrict__ Uf, const DevBox *__restrict__ fboxes,
+    const double *__restrict__ Uold, const double *__restrict__ Unew,
+    const DevBox *__restrict__ cboxes, const CFSeg *__restrict__ segs, int nseg, int ngf,
+    int ngc, int mode, double alpha, double beta)
+{
+    constexpr int NC = D + 2;
+    for (int si = blockIdx.x; si < nseg; si += gridDim.x)
+    {
+        const CFSeg  &g  = segs[si];
+        const DevBox &fb = fboxes[g.dst];
+        const DevBox &cb = cboxes[g.src];
+        const int     ex = g.hi[0] - g.lo[0] + 1;
+        const int     ey = g.hi[1] - g.lo[1] + 1;
+        const int     ez = g.hi[2] - g.lo[2] + 1;
+        const int     n  = ex * ey * ez;
+        for (int c = threadIdx.x; c < n; c += blockDim.x)
+        {
+            int x[3];
+            x[0] = g.lo[0] + c % ex;
+            x[1] = g.lo[1] + (c / ex) % ey;
+            x[2] = g.lo[2] + c / (ex * ey);
+            int    ci[3] = { 0, 0, 0 };
+            double off[3] = { 0, 0, 0 };
+            for (int d = 0; d < D; ++d)
+            {
+                const int img = x[d] + g.shift[d];
+                ci[d]         = img >> 1; // floor(img / 2)
+                off[d]        = (img & 1) ? 0.25 : -0.25;
+            }
+            const long long doff = cell_off(fb, ngf, D, x[0], x[1], x[2]);
+            const long long cc   = cell_off(cb, ngc, D, ci[0], ci[1], ci[2]);
+            const long long sd[3] = { 1, cb.jstride, cb.kstride };
+            for (int m = 0; m < NC; ++m)
+            {
+                auto val = [&](long long o) {
+                    const long long a = o + m * cb.nstride;
+                    return mode == 0 ? Uold[a] : alpha * Uold[a] + beta * Unew[a];
+                };
+                const double uc = val(cc);
+                double       s[3] = { 0, 0, 0 };
+                for (int d = 0; d < D; ++d)
+                    s[d] = cf_mc_slope(val(cc - sd[d]), uc, val(cc + sd[d]));
+                double umin = uc, umax = uc;
+                for (int kk = (D >= 3 ? -1 : 0); kk <= (D >= 3 ? 1 : 0); ++kk)
+                    for (int jj = (D >= 2 ? -1 : 0); jj <= (D >= 2 ? 1 : 0); ++jj)
+                        for (int ii = -1; ii <= 1; ++ii)
+                        {
+                            const double v = val(cc + ii + jj * sd[1] + kk * sd[2]);
+                            umin           = fmin(umin, v);
+                            umax           = fmax(umax, v);
+                        }
+                double ext = 0.0;
+                for (int d = 0; d < D; ++d)
+                    ext = ext + 0.25 * fabs(s[d]);
+                double lim = 1.0;
+                if (ext > 0)
+                    lim = fmin(1.0, fmin((umax - uc) / ext, (uc - umin) / ext));
+                double r = uc;
+                for (int d = 0; d < D; ++d)
+                    r = r + off[d] * (lim * s[d]);
+                Uf[doff + m * fb.nstride] = r;
+            }
+        }
+    }
+}
+
+// Flux-register contributions of direction DIR: slot s gets the fluxes of
+// faces[s*nsub .. s*nsub+nsub-1] scaled by `scale` (dt times the face
+// area), each evaluated by the same face computation the sweeps use.
+// accumulate = 0: reg = -F (CrseInit with -1, nsub = 1);
+// accumulate = 1: reg += F for each face in order (FineAdd with +1).
+template <int D, int DIR, int LIM, int LV, int SCH>
+__global__ __launch_bounds__(256) void k_reg_flux(
+    const double *__restrict__ U, const DevBox *__restrict__ boxes,
+    const RegFace *__restrict__ faces, int nslot, int nsub, double *__restrict__ reg,
+    int accumulate, int ng, double hdtdx, double fc, double rc, double scale,
+    EulerParm pm, LimTable lt)
+{
+    constexpr int NC = D + 2;
+    const int     sl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= nslot)
+        return;
+    for (int q = 0; q < nsub; ++q)
+    {
+        const RegFace  &f  = faces[(long long)sl * nsub + q];
+        const DevBox   &bx = boxes[f.box];
+        const long long sd = (DIR == 0) ? 1 : (DIR == 1 ? bx.jstride : bx.kstride);
+        const long long c  = cell_off(bx, ng, D, f.i[0], f.i[1], f.i[2]);
+        double          W[4][NC];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int n = 0; n < NC; ++n)
+                W[s][n] = U[c + (s - 2) * sd + n * bx.nstride];
+        double F[NC];
+        if (!scheme_reconstructs(SCH))
+        {
+            face_flux<D, DIR, false, false, SCH>(W[1], W[2], F, pm, fc, rc);
+        }
+        else
+        {
+            double hL0[NC], hR0[NC], hL1[NC], hR1[NC];
+            reconstruct_half_step<D, DIR, LIM, LV>(W[0], W[1], W[2], hL0, hR0, hdtdx, pm, lt);
+            reconstruct_half_step<D, DIR, LIM, LV>(W[1], W[2], W[3], hL1, hR1, hdtdx, pm, lt);
+            face_flux<D, DIR, false, false, SCH>(hR0, hL1, F, pm, fc, rc);
+        }
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+        {
+            const double v = F[n] * scale;
+            double      &r = reg[(long long)sl * NC + n];
+            if (accumulate)
+                r = r + v;
+            else
+                r = -1.0 * v;
+        }
+    }
+}
+
+// Reflux (FluxRegister::Reflux with scale 1): each coarse cell next to the
+// fine region gets sign * reg / vol per register face, faces in the order
+// (x lo, y lo, z lo, x hi, y hi, z hi); sign = -1 on the low faces of the
+// fine boxes, +1 on the high faces.
+template <int D>
+__global__ __launch_bounds__(256) void k_reflux(double *__restrict__ U,
+                                                const DevBox *__restrict__ boxes,
+                                                const RefluxTarget *__restrict__ tg, int nt,
+                                                const RefluxTerm *__restrict__ terms,
+                                                const double *__restrict__ reg, int ng,
+                                                double vol)
+{
+    constexpr int NC = D + 2;
+    const int     t  = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt)
+        return;
+    const RefluxTarget &g  = tg[t];
+    const DevBox       &bx = boxes[g.box];
+    const long long     o  = cell_off(bx, ng, D, g.i[0], g.i[1], g.i[2]);
+    for (int n = 0; n < NC; ++n)
+    {
+        double u = U[o + n * bx.nstride];
+        for (int q = g.begin; q < g.end; ++q)
+            u = u + (terms[q].sign * reg[(long long)terms[q].slot * NC + n]) / vol;
+        U[o + n * bx.nstride] = u;
+    }
+}
+
+// avgDown (average_down, AmrLevelAdv.cpp:961-987): a covered coarse cell
+// becomes volfrac * (sum of its 2^D children), children summed with x
+// fastest.  One block per segment.
+template <int D>
+__global__ __launch_bounds__(256) void k_avg_down(double *__restrict__ Uc,
+                                                  const DevBox *__restrict__ cboxes,
+                                                  const double *__restrict__ Uf,
+                                                  const DevBox *__restrict__ fboxes,
+                                                  const AvgSeg *__restrict__ segs, int nseg,
+                                                  int ngc, int ngf, double volfrac)
+{
+    constexpr int NC = D + 2;
+    for (int si = blockIdx.x; si < nseg; si += gridDim.x)
+    {
+        const AvgSeg &g  = segs[si];
+        const DevBox &cb = cboxes[g.cbox];
+        const DevBox &fb = fboxes[g.fbox];
+        const int     ex = g.hi[0] - g.lo[0] + 1;
+        const int     ey = g.hi[1] - g.lo[1] + 1;
+        const int     ez = g.hi[2] - g.lo[2] + 1;
+        const int     n  = ex * ey * ez;
+        for (int c = threadIdx.x; c < n; c += blockDim.x)
+        {
+            const int       i  = g.lo[0] + c % ex;
+            const int       j  = g.lo[1] + (c / ex) % ey;
+            const int       k  = g.lo[2] + c / (ex * ey);
+            const long long co = cell_off(cb, ngc, D, i, j, k);
+            for (int m = 0; m < NC; ++m)
+            {
+                double s = 0;
+                for (int kr = 0; kr < (D >= 3 ? 2 : 1); ++kr)
+                    for (int jr = 0; jr < (D >= 2 ? 2 : 1); ++jr)
+                        for (int ir = 0; ir < 2; ++ir)
+                        {
+                            const long long fo
+                                = cell_off(fb, ngf, D, 2 * i + ir, D >= 2 ? 2 * j + jr : 0,
+                                           D >= 3 ? 2 * k + kr : 0);
+                            s = s + Uf[fo + m * fb.nstride];
+                        }
+                Uc[co + m * cb.nstride] = volfrac * s;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------ launch dispatch
+template <int D, int DIR, int LIM, int LV, int SCH>
+static hipError_t launch_reg(const double *U, const DevBox *boxes, const RegFace *faces,
+                             int nslot, int nsub, double *reg, int acc, int ng,
+                             double hdtdx, double fc, double rc, double scale,
+                             EulerParm pm, LimTable lt, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_reg_flux<D, DIR, LIM, LV, SCH>), dim3((nslot + 127) / 128),
+                       dim3(128), 0, s, U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx,
+                       fc, rc, scale, pm, lt);
+    return hipGetLastError();
+}
+
+template <int D, int DIR, int SCH>
+static hipError_t reg_lim(int lim, int lv, const double *U, const DevBox *boxes,
+                          const RegFace *faces, int nslot, int nsub, double *reg, int acc,
+                          int ng, double hdtdx, double fc, double rc, double scale,
+                          EulerParm pm, LimTable lt, hipStream_t s)
+{
+#define BX_RL(LIMV)                                                                 \
+    do                                                                              \
+    {                                                                               \
+        if (lv == LV_SHARED)                                                        \
+            return launch_reg<D, DIR, LIMV, LV_SHARED, SCH>(                        \
+                U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, pm, lt, s); \
+        if (lv == LV_OWN)                                                           \
+            return launch_reg<D, DIR, LIMV, LV_OWN, SCH>(                           \
+                U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, pm, lt, s); \
+        return launch_reg<D, DIR, LIMV, LV_TABLE, SCH>(                             \
+            U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, pm, lt, s); \
+    } while (0)
+    if constexpr (!scheme_reconstructs(SCH))
+        return launch_reg<D, DIR, LIM_MINBEE, LV_SHARED, SCH>(
+            U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, pm, lt, s);
+    else
+        switch (lim)
+        {
+        case LIM_MINBEE: BX_RL(LIM_MINBEE);
+        case LIM_VAN_LEER: BX_RL(LIM_VAN_LEER);
+        case LIM_VAN_ALBADA: BX_RL(LIM_VAN_ALBADA);
+        default: BX_RL(LIM_SUPERBEE);
+        }
+#undef BX_RL
+}
+
+template <int D, int DIR>
+static hipError_t reg_scheme(int sch, int lim, int lv, const double *U, const DevBox *boxes,
+                             const RegFace *faces, int nslot, int nsub, double *reg, int acc,
+                             int ng, double hdtdx, double fc, double rc, double scale,
+                             EulerParm pm, LimTable lt, hipStream_t s)
+{
+#define BX_RS(S)                                                                        \
+    return reg_lim<D, DIR, S>(lim, lv, U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, \
+                              fc, rc, scale, pm, lt, s)
+    switch (sch)
+    {
+    case SCH_MHM: BX_RS(SCH_MHM);
+    case SCH_HLLC: BX_RS(SCH_HLLC);
+    case SCH_SLIC: BX_RS(SCH_SLIC);
+    case SCH_FORCE: BX_RS(SCH_FORCE);
+    default: BX_RS(SCH_LF);
+    }
+#undef BX_RS
+}
+
+} // namespace bx
+
+// ================================================================ host side
+struct bx_amr
+{
+    bx_ctx *crse = nullptr, *fine = nullptr;
+    int     D = 0, NC = 0;
+    // coarse-fine ghost fill
+    std::vector<CFSeg> cf;
+    CFSeg             *d_cf = nullptr;
+    // flux registers: slots of direction d are [slot0[d], slot0[d+1])
+    int                  nslot = 0, slot0[4] = { 0, 0, 0, 0 };
+    std::vector<RegFace> cface, fface; // 1 and 2^(D-1) faces per slot
+    RegFace             *d_cface = nullptr, *d_fface = nullptr;
+    double              *d_reg   = nullptr;
+    std::vector<RefluxTarget> tg;
+    std::vector<RefluxTerm>   terms;
+    RefluxTarget             *d_tg    = nullptr;
+    RefluxTerm               *d_terms = nullptr;
+    std::vector<AvgSeg>       avg;
+    AvgSeg                   *d_avg = nullptr;
+};
+
+template <class T>
+static int amr_upload(T **dptr, const std::vector<T> &v)
+{
+    if (*dptr)
+        (void)hipFree(*dptr);
+    *dptr = nullptr;
+    if (v.empty())
+        return BX_OK;
+    BX_HIP_CHECK(hipMalloc((void **)dptr, sizeof(T) * v.size()));
+    BX_HIP_CHECK(hipMemcpy(*dptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return BX_OK;
+}
+
+static bool in_box(const bx_box &b, const int x[3], int D)
+{
+    for (int d = 0; d < D; ++d)
+        if (x[d] < b.lo[d] || x[d] > b.hi[d])
+            return false;
+    return true;
+}
+
+static int find_box(const std::vector<bx_box> &B, const int x[3], int D)
+{
+    for (size_t i = 0; i < B.size(); ++i)
+        if (in_box(B[i], x, D))
+            return (int)i;
+    return -1;
+}
+
+// Wraps x into the domain along periodic directions; false if x lies
+// outside the domain along a non-periodic direction.
+static bool wrap_into(const bx_geometry &g, int D, int x[3])
+{
+    for (int d = 0; d < D; ++d)
+    {
+        const int L = g.domain_hi[d] - g.domain_lo[d] + 1;
+        if (x[d] < g.domain_lo[d] || x[d] > g.domain_hi[d])
+        {
+            if (g.bc_lo[d] != BX_BC_PERIODIC)
+                return false;
+            x[d] = x[d] < g.domain_lo[d] ? x[d] + L : x[d] - L;
+        }
+    }
+    return true;
+}
+
+static int amr_plan(bx_amr *a)
+{
+    const bx_ctx      *C = a->crse, *F = a->fine;
+    const int          D = a->D;
+    const bx_geometry &gc = C->geom, &gf = F->geom;
+    const auto        &CB = C->hboxes, &FB = F->hboxes;
+    auto               odd_cell = [](int v) { return (v & 1) != 0; };
+    // ---- validation
+    for (int d = 0; d < D; ++d)
+    {
+        if (gf.domain_lo[d] != 2 * gc.domain_lo[d] || gf.domain_hi[d] != 2 * gc.domain_hi[d] + 1
+            || gf.bc_lo[d] != gc.bc_lo[d] || gf.bc_hi[d] != gc.bc_hi[d])
+            return set_error(BX_ERR_INVALID,
+                             "bx_amr_create: the fine geometry must be the coarse one refined by 2");
+    }
+    for (const bx_box &b : FB)
+        for (int d = 0; d < D; ++d)
+            if (odd_cell(b.lo[d]) || !odd_cell(b.hi[d]) || b.lo[d] < gf.domain_lo[d]
+                || b.hi[d] > gf.domain_hi[d])
+                return set_error(BX_ERR_INVALID,
+                                 "bx_amr_create: fine boxes must be ratio-2 aligned and inside the domain");
+    if (gc.ng < 2 || gf.ng < 2)
+        return set_error(BX_ERR_INVALID, "bx_amr_create: both levels need 2 ghost cells");
+
+    // ---- coarse-fine ghost cells: face ghost slabs of each fine box, cell
+    // by cell, those whose (periodically wrapped) image is inside the domain
+    // and not covered by a fine box; runs along x become segments
+    a->cf.clear();
+    const int ngf = gf.ng;
+    for (int ib = 0; ib < (int)FB.size(); ++ib)
+    {
+        const bx_box &b = FB[ib];
+        for (int d = 0; d < D; ++d)
+            for (int side = 0; side < 2; ++side)
+            {
+                int lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
+                for (int e = 0; e < D; ++e)
+                {
+                    lo[e] = b.lo[e];
+                    hi[e] = b.hi[e];
+                }
+                if (side == 0)
+                {
+                    lo[d] = b.lo[d] - ngf;
+                    hi[d] = b.lo[d] - 1;
+                }
+                else
+                {
+                    lo[d] = b.hi[d] + 1;
+                    hi[d] = b.hi[d] + ngf;
+                }
+                for (int z = lo[2]; z <= hi[2]; ++z)
+                    for (int y = lo[1]; y <= hi[1]; ++y)
+                    {
+                        CFSeg cur{};
+                        bool  open = false;
+                        for (int x = lo[0]; x <= hi[0]; ++x)
+                        {
+                            int p[3] = { x, y, z }, img[3] = { x, y, z };
+                            bool cf = wrap_into(gf, D, img) && find_box(FB, img, D) < 0;
+                            int  src = -1, sh[3] = { img[0] - p[0], img[1] - p[1], img[2] - p[2] };
+                            if (cf)
+                            {
+                                int cc[3] = { img[0] >> 1, img[1] >> 1, img[2] >> 1 };
+                                src       = find_box(CB, cc, D);
+                                if (src < 0)
+                                    return set_error(BX_ERR_INVALID,
+                                                     "bx_amr_create: coarse level does not cover the domain");
+                            }
+                            const bool extend = open && cf && src == cur.src
+                                                && sh[0] == cur.shift[0] && sh[1] == cur.shift[1]
+                                                && sh[2] == cur.shift[2] && x == cur.hi[0] + 1;
+                            if (extend)
+                            {
+                                cur.hi[0] = x;
+                                continue;
+                            }
+                            if (open)
+                                a->cf.push_back(cur);
+                            open = cf;
+                            if (cf)
+                            {
+                                cur       = CFSeg{};
+                                cur.dst   = ib;
+                                cur.src   = src;
+                                cur.lo[0] = cur.hi[0] = x;
+                                cur.lo[1] = cur.hi[1] = y;
+                                cur.lo[2] = cur.hi[2] = z;
+                                for (int e = 0; e < 3; ++e)
+                                    cur.shift[e] = sh[e];
+                            }
+                        }
+                        if (open)
+                            a->cf.push_back(cur);
+                    }
+            }
+    }
+
+    // ---- flux registers: per direction, per fine box side, per coarse
+    // face of that side whose outside neighbour is not fine
+    a->cface.clear();
+    a->fface.clear();
+    struct Tgt
+    {
+        int box, i[3], orient;
+        RefluxTerm t;
+    };
+    std::vector<Tgt> tgts;
+    const int nsub = 1 << (D - 1);
+    a->nslot       = 0;
+    for (int d = 0; d < D; ++d)
+    {
+        a->slot0[d] = a->nslot;
+        for (int side = 0; side < 2; ++side)
+            for (int ib = 0; ib < (int)FB.size(); ++ib)
+            {
+                const bx_box &b = FB[ib];
+                int           clo[3] = { 0, 0, 0 }, chi[3] = { 0, 0, 0 };
+                for (int e = 0; e < D; ++e)
+                {
+                    clo[e] = b.lo[e] >> 1;
+                    chi[e] = b.hi[e] >> 1;
+                }
+                const int Ff = side == 0 ? b.lo[d] : b.hi[d] + 1; // fine face
+                const int Fc = Ff >> 1;                           // coarse face
+                clo[d] = chi[d] = 0;
+                for (int z = clo[2]; z <= chi[2]; ++z)
+                    for (int y = clo[1]; y <= chi[1]; ++y)
+                        for (int x = clo[0]; x <= chi[0]; ++x)
+                        {
+                            int t[3] = { x, y, z };
+                            // outside coarse cell and one of its fine children
+                            int oc[3] = { t[0], t[1], t[2] };
+                            oc[d]     = side == 0 ? Fc - 1 : Fc;
+                            int ofine[3] = { 2 * oc[0], 2 * oc[1], 2 * oc[2] };
+                            if (!wrap_into(gf, D, ofine) || find_box(FB, ofine, D) >= 0)
+                                continue;
+                            int ocw[3] = { oc[0], oc[1], oc[2] };
+                            wrap_into(gc, D, ocw);
+                            // coarse face: high-side cell index Fc, evaluated in the
+                            // coarse box holding the inside cell
+                            int inside[3] = { t[0], t[1], t[2] };
+                            inside[d]     = side == 0 ? Fc : Fc - 1;
+                            const int cbox = find_box(CB, inside, D);
+                            const int tbox = find_box(CB, ocw, D);
+                            if (cbox < 0 || tbox < 0)
+                                return set_error(BX_ERR_INVALID,
+                                                 "bx_amr_create: coarse level does not cover the domain");
+                            RegFace cf{ cbox, { t[0], t[1], t[2] } };
+                            cf.i[d] = Fc;
+                            a->cface.push_back(cf);
+                            // fine faces: children in the transverse directions, x then y then z
+                            for (int q = 0; q < nsub; ++q)
+                            {
+                                RegFace ff{ ib, { 0, 0, 0 } };
+                                int     bit = 0;
+                                for (int e = 0; e < D; ++e)
+                                {
+                                    if (e == d)
+                                    {
+                                        ff.i[e] = Ff;
+                                        continue;
+                                    }
+                                    ff.i[e] = 2 * t[e] + ((q >> bit) & 1);
+                                    ++bit;
+                                }
+                                a->fface.push_back(ff);
+                            }
+                            Tgt tt;
+                            tt.box = tbox;
+                            for (int e = 0; e < 3; ++e)
+                                tt.i[e] = ocw[e];
+                            tt.orient = side * 3 + d; // x lo, y lo, z lo, x hi, ...
+                            tt.t      = RefluxTerm{ a->nslot, side == 0 ? -1 : 1 };
+                            tgts.push_back(tt);
+                            ++a->nslot;
+                        }
+            }
+    }
+    a->slot0[D] = a->nslot;
+    // group the reflux terms by target cell, orientations in order
+    std::stable_sort(tgts.begin(), tgts.end(), [](const Tgt &p, const Tgt &q) {
+        if (p.box != q.box)
+            return p.box < q.box;
+        for (int e = 2; e >= 0; --e)
+            if (p.i[e] != q.i[e])
+                return p.i[e] < q.i[e];
+        return p.orient < q.orient;
+    });
+    a->tg.clear();
+    a->terms.clear();
+    for (size_t k = 0; k < tgts.size(); ++k)
+    {
+        const bool same = !a->tg.empty() && a->tg.back().box == tgts[k].box
+                          && a->tg.back().i[0] == tgts[k].i[0] && a->tg.back().i[1] == tgts[k].i[1]
+                          && a->tg.back().i[2] == tgts[k].i[2];
+        if (!same)
+            a->tg.push_back(RefluxTarget{ tgts[k].box, { tgts[k].i[0], tgts[k].i[1], tgts[k].i[2] },
+                                          (int)a->terms.size(), (int)a->terms.size() });
+        a->terms.push_back(tgts[k].t);
+        a->tg.back().end = (int)a->terms.size();
+    }
+
+    // ---- avgDown: coarsened fine boxes intersected with the coarse boxes
+    a->avg.clear();
+    for (int ib = 0; ib < (int)FB.size(); ++ib)
+        for (int jb = 0; jb < (int)CB.size(); ++jb)
+        {
+            AvgSeg s{ ib, jb, { 0, 0, 0 }, { 0, 0, 0 } };
+            bool   ok = true;
+            for (int e = 0; e < D && ok; ++e)
+            {
+                s.lo[e] = std::max(FB[ib].lo[e] >> 1, CB[jb].lo[e]);
+                s.hi[e] = std::min(FB[ib].hi[e] >> 1, CB[jb].hi[e]);
+                ok      = s.lo[e] <= s.hi[e];
+            }
+            if (ok)
+                a->avg.push_back(s);
+        }
+
+    int rc;
+    if ((rc = amr_upload(&a->d_cf, a->cf)) || (rc = amr_upload(&a->d_cface, a->cface))
+        || (rc = amr_upload(&a->d_fface, a->fface)) || (rc = amr_upload(&a->d_tg, a->tg))
+        || (rc = amr_upload(&a->d_terms, a->terms)) || (rc = amr_upload(&a->d_avg, a->avg)))
+        return rc;
+    if (a->d_reg)
+        (void)hipFree(a->d_reg);
+    a->d_reg = nullptr;
+    if (a->nslot)
+        BX_HIP_CHECK(hipMalloc((void **)&a->d_reg, sizeof(double) * a->nslot * a->NC));
+    return BX_OK;
+}
+
+extern "C" void bx_amr_destroy(bx_amr *a)
+{
+    if (!a)
+        return;
+    for (void *p : { (void *)a->d_cf, (void *)a->d_cface, (void *)a->d_fface, (void *)a->d_reg,
+                     (void *)a->d_tg, (void *)a->d_terms, (void *)a->d_avg })
+        if (p)
+            (void)hipFree(p);
+    delete a;
+}
+
+extern "C" int bx_amr_create(bx_ctx *coarse, bx_ctx *fine, bx_amr **out)
+{
+    if (!coarse || !fine || !out || coarse == fine)
+        return set_error(BX_ERR_INVALID, "bx_amr_create: bad argument");
+    if (!coarse->level_set || !fine->level_set)
+        return set_error(BX_ERR_INVALID, "bx_amr_create: both levels need bx_set_level first");
+    const bx_params &pc = coarse->p, &pf = fine->p;
+    bool same = pc.dim == pf.dim && pc.num_method == pf.num_method
+                && pc.adiabatic == pf.adiabatic && pc.epsilon == pf.epsilon
+                && pc.limiter == pf.limiter;
+    for (int n = 0; n < 5; ++n)
+        same = same && pc.limiter_indices[n] == pf.limiter_indices[n]
+               && pc.limit_on_primv[n] == pf.limit_on_primv[n];
+    if (!same)
+        return set_error(BX_ERR_INVALID, "bx_amr_create: the levels must share bx_params");
+    bx_amr *a = new bx_amr;
+    a->crse   = coarse;
+    a->fine   = fine;
+    a->D      = coarse->D;
+    a->NC     = coarse->NC;
+    int rc    = amr_plan(a);
+    if (rc != BX_OK)
+    {
+        bx_amr_destroy(a);
+        return rc;
+    }
+    *out = a;
+    return BX_OK;
+}
+
+extern "C" int bx_amr_info(const bx_amr *a, int *n_cf_segs, int *n_register_faces,
+                           int *n_reflux_cells)
+{
+    if (!a)
+        return set_error(BX_ERR_INVALID, "bx_amr_info: null");
+    if (n_cf_segs)
+        *n_cf_segs = (int)a->cf.size();
+    if (n_register_faces)
+        *n_register_faces = a->nslot;
+    if (n_reflux_cells)
+        *n_reflux_cells = (int)a->tg.size();
+    return BX_OK;
+}
+
+// Flux-register contribution of one level's sweep input in direction d.
+static int reg_fluxes(bx_amr *a, bool fine_level, int d, const double *U, double dt,
+                      const double dx[3], hipStream_t s)
+{
+    const int nslot = a->slot0[d + 1] - a->slot0[d];
+    if (nslot == 0)
+        return BX_OK;
+    bx_ctx *L = fine_level ? a->fine : a->crse;
+    // the sweeps' coefficients (same expressions, so the same bits)
+    const double hdtdx = 0.5 * dt / dx[d];
+    const double fc    = 0.5 * dx[d] / dt;
+    const double rc    = dt / (2 * dx[d]);
+    // fluxes[d].mult(dt * prod of the other cell sizes) (_advance.cpp:192-207)
+    double scale = dt;
+    for (int e = 0; e < a->D; ++e)
+        if (e != d)
+            scale *= dx[e];
+    const int      nsub  = fine_level ? (1 << (a->D - 1)) : 1;
+    const RegFace *faces = (fine_level ? a->d_fface : a->d_cface) + (long long)a->slot0[d] * nsub;
+    double        *reg   = a->d_reg + (long long)a->slot0[d] * a->NC;
+    const int      sch = L->p.num_method, lim = L->p.limiter, lv = L->lv_mode, ng = L->geom.ng;
+    hipError_t     e   = hipErrorInvalidValue;
+#define BX_R(DD, DIR)                                                                    \
+    reg_scheme<DD, DIR>(sch, lim, lv, U, L->d_boxes, faces, nslot, nsub, reg, fine_level, \
+                        ng, hdtdx, fc, rc, scale, L->pm, L->lt, s)
+    switch (a->D * 4 + d)
+    {
+    case 4: e = BX_R(1, 0); break;
+    case 8: e = BX_R(2, 0); break;
+    case 9: e = BX_R(2, 1); break;
+    case 12: e = BX_R(3, 0); break;
+    case 13: e = BX_R(3, 1); break;
+    case 14: e = BX_R(3, 2); break;
+    default: break;
+    }
+#undef BX_R
+    BX_HIP_CHECK(e);
+    return BX_OK;
+}
+
+// One level's split step from A (valid + ghosts filled), result in A;
+// B is the ping-pong buffer.  Registers are fed from each sweep's input.
+static int level_step(bx_amr *a, bool fine_level, double dt, const double dx[3], double *A,
+                      double *B, hipStream_t s)
+{
+    bx_ctx   *L = fine_level ? a->fine : a->crse;
+    const int D = a->D;
+    int       rc;
+    if ((rc = set_scalar(L->d_dt, dt, s)) != BX_OK)
+        return rc;
+    for (int d = 0; d < D; ++d)
+    {
+        const double *in  = (d % 2 == 0) ? A : B;
+        double       *out = (d % 2 == 0) ? B : A;
+        if ((rc = do_sweep(L, d, dx, in, out, nullptr, s)) != BX_OK)
+            return rc;
+        if ((rc = reg_fluxes(a, fine_level, d, in, dt, dx, s)) != BX_OK)
+            return rc;
+        if ((rc = do_fill(L, out, (d + 1) % D, s)) != BX_OK)
+            return rc;
+    }
+    if (D % 2 == 1)
+        BX_HIP_CHECK(hipMemcpyAsync(A, B, sizeof(double) * L->level_extent,
+                                    hipMemcpyDeviceToDevice, s));
+    return BX_OK;
+}
+
+static int cf_fill(bx_amr *a, double *Uf, const double *Uold, const double *Unew, int mode,
+                   double alpha, double beta, hipStream_t s)
+{
+    const int nseg = (int)a->cf.size();
+    if (nseg == 0)
+        return BX_OK;
+    const int grid = std::min(nseg, 4096);
+    bx_ctx   *C = a->crse, *F = a->fine;
+#define BX_CF(DD)                                                                          \
+    hipLaunchKernelGGL((k_cf_fill<DD>), dim3(grid), dim3(64), 0, s, Uf, F->d_boxes, Uold,  \
+                       Unew, C->d_boxes, a->d_cf, nseg, F->geom.ng, C->geom.ng, mode, alpha, \
+                       beta)
+    if (a->D == 1)
+        BX_CF(1);
+    else if (a->D == 2)
+        BX_CF(2);
+    else
+        BX_CF(3);
+#undef BX_CF
+    BX_HIP_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+static int avg_down(bx_amr *a, const double *Uf, double *Uc, hipStream_t s)
+{
+    const int nseg = (int)a->avg.size();
+    if (nseg == 0)
+        return BX_OK;
+    bx_ctx      *C = a->crse, *F = a->fine;
+    const double volfrac = 1.0 / (double)(1 << a->D);
+#define BX_AV(DD)                                                                           \
+    hipLaunchKernelGGL((k_avg_down<DD>), dim3(std::min(nseg, 4096)), dim3(256), 0, s, Uc,    \
+                       C->d_boxes, Uf, F->d_boxes, a->d_avg, nseg, C->geom.ng, F->geom.ng,  \
+                       volfrac)
+    if (a->D == 1)
+        BX_AV(1);
+    else if (a->D == 2)
+        BX_AV(2);
+    else
+        BX_AV(3);
+#undef BX_AV
+    BX_HIP_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+static bool amr_dx_ok(const bx_amr *a, const double dxc[3])
+{
+    for (int d = 0; d < a->D; ++d)
+        if (!(dxc[d] > 0))
+            return false;
+    return true;
+}
+
+extern "C" int bx_amr_avg_down(bx_amr *a, const double *Uf, double *Uc, void *stream)
+{
+    if (!a || !Uf || !Uc)
+        return set_error(BX_ERR_INVALID, "bx_amr_avg_down: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    int         rc;
+    if ((rc = avg_down(a, Uf, Uc, s)) != BX_OK)
+        return rc;
+    return do_fill_all(a->crse, Uc, s);
+}
+
+extern "C" int bx_amr_fill_fine(bx_amr *a, double *Uf, const double *Uc_old,
+                                const double *Uc_new, double t_old, double t_new, double time,
+                                void *stream)
+{
+    if (!a || !Uf || !Uc_old || !Uc_new)
+        return set_error(BX_ERR_INVALID, "bx_amr_fill_fine: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    int         rc;
+    if ((rc = do_fill_all(a->fine, Uf, s)) != BX_OK)
+        return rc;
+    if (time == t_old || !(t_new != t_old))
+        return cf_fill(a, Uf, Uc_old, Uc_new, 0, 1.0, 0.0, s);
+    // FillPatchTwoLevels time interpolation: alpha old + beta new
+    const double alpha = (t_new - time) / (t_new - t_old);
+    const double beta  = (time - t_old) / (t_new - t_old);
+    return cf_fill(a, Uf, Uc_old, Uc_new, 1, alpha, beta, s);
+}
+
+extern "C" int bx_amr_advance(bx_amr *a, double dt, const double dx_coarse[3], double time,
+                              double *Uc, double *Uc_tmp, double *Uc_old, double *Uf,
+                              double *Uf_tmp, void *stream)
+{
+    if (!a || !Uc || !Uc_tmp || !Uc_old || !Uf || !Uf_tmp || !amr_dx_ok(a, dx_coarse)
+        || !(dt > 0))
+        return set_error(BX_ERR_INVALID, "bx_amr_advance: bad argument");
+    hipStream_t s  = (hipStream_t)stream;
+    bx_ctx     *C  = a->crse, *F = a->fine;
+    const int   D  = a->D;
+    double      dxf[3] = { 0, 0, 0 };
+    for (int d = 0; d < D; ++d)
+        dxf[d] = dx_coarse[d] / 2;
+    int rc;
+    // coarse level: keep the old state (ghosts filled) for the time interpolation
+    BX_HIP_CHECK(hipMemcpyAsync(Uc_old, Uc, sizeof(double) * C->level_extent,
+                                hipMemcpyDeviceToDevice, s));
+    if ((rc = level_step(a, false, dt, dx_coarse, Uc, Uc_tmp, s)) != BX_OK)
+        return rc;
+    if ((rc = do_fill_all(C, Uc, s)) != BX_OK)
+        return rc;
+    // fine level: two steps of dt / 2 (dt_level[1] = dt_0 / 2)
+    const double dtf   = dt / 2;
+    const double t_new = time + dt;
+    for (int sub = 0; sub < 2; ++sub)
+    {
+        const double tf = sub == 0 ? time : time + dtf;
+        if ((rc = bx_amr_fill_fine(a, Uf, Uc_old, Uc, time, t_new, tf, stream)) != BX_OK)
+            return rc;
+        // the ping-pong buffer starts as a copy of the filled state (Sborder2)
+        BX_HIP_CHECK(hipMemcpyAsync(Uf_tmp, Uf, sizeof(double) * F->level_extent,
+                                    hipMemcpyDeviceToDevice, s));
+        if ((rc = level_step(a, true, dtf, dxf, Uf, Uf_tmp, s)) != BX_OK)
+            return rc;
+    }
+    // post_timestep of the coarse level: reflux, then avgDown
+    if (!a->tg.empty())
+    {
+        double vol = dx_coarse[0];
+        for (int d = 1; d < D; ++d)
+            vol *= dx_coarse[d];
+        const int nt = (int)a->tg.size();
+#define BX_RF(DD)                                                                          \
+    hipLaunchKernelGGL((k_reflux<DD>), dim3((nt + 127) / 128), dim3(128), 0, s, Uc,          \
+                       C->d_boxes, a->d_tg, nt, a->d_terms, a->d_reg, C->geom.ng, vol)
+        if (D == 1)
+            BX_RF(1);
+        else if (D == 2)
+            BX_RF(2);
+        else
+            BX_RF(3);
+#undef BX_RF
+        BX_HIP_CHECK(hipGetLastError());
+    }
+    if ((rc = avg_down(a, Uf, Uc, s)) != BX_OK)
+        return rc;
+    if ((rc = do_fill_all(C, Uc, s)) != BX_OK)
+        return rc;
+    return do_fill_all(F, Uf, s);
+}
+
+// estTimeStep of one level (AmrLevelAdv.cpp:395-477, explicit methods)
+static int level_dt_est(bx_ctx *L, const double *U, const double dx[3], double cfl,
+                        hipStream_t s, double *out)
+{
+    int rc;
+    if ((rc = do_maxspeed(L, U, s)) != BX_OK)
+        return rc;
+    unsigned long long bits = 0;
+    BX_HIP_CHECK(hipMemcpyAsync(&bits, L->d_maxbits, sizeof(bits), hipMemcpyDeviceToHost, s));
+    BX_HIP_CHECK(hipStreamSynchronize(s));
+    double max_speed;
+    std::memcpy(&max_speed, &bits, sizeof(max_speed));
+    double dt_est = 1.0e20;
+    for (int d = 0; d < L->D; ++d)
+    {
+        const double c = dx[d] / max_speed;
+        dt_est         = (c < dt_est) ? c : dt_est;
+    }
+    *out = dt_est * cfl;
+    return BX_OK;
+}
+
+extern "C" int bx_amr_est_dt(bx_amr *a, const double dx_coarse[3], double cfl, double time,
+                             double stop_time, const double *Uc, const double *Uf, double *dt,
+                             void *stream)
+{
+    if (!a || !Uc || !Uf || !dt || !amr_dx_ok(a, dx_coarse))
+        return set_error(BX_ERR_INVALID, "bx_amr_est_dt: bad argument");
+    hipStream_t s      = (hipStream_t)stream;
+    double      dxf[3] = { 0, 0, 0 };
+    for (int d = 0; d < a->D; ++d)
+        dxf[d] = dx_coarse[d] / 2;
+    double dmin[2];
+    int    rc;
+    if ((rc = level_dt_est(a->crse, Uc, dx_coarse, cfl, s, &dmin[0])) != BX_OK
+        || (rc = level_dt_est(a->fine, Uf, dxf, cfl, s, &dmin[1])) != BX_OK)
+        return rc;
+    // computeNewDt (AmrLevelAdv.cpp:589-618): n_cycle = {1, 2}
+    double dt_0     = 1.0e+100;
+    int    n_factor = 1;
+    for (int i = 0; i < 2; ++i)
+    {
+        n_factor *= (i == 0 ? 1 : 2);
+        dt_0 = std::min(dt_0, n_factor * dmin[i]);
+    }
+    const double eps = 0.001 * dt_0;
+    if (stop_time >= 0.0)
+        if ((time + dt_0) > (stop_time - eps))
+            dt_0 = stop_time - time;
+    *dt = dt_0;
+    return BX_OK;
+}
+
+extern "C" int bx_amr_run(bx_amr *a, const double dx_coarse[3], double cfl, double stop_time,
+                          int max_step, double *Uc, double *Uc_tmp, double *Uc_old, double *Uf,
+                          double *Uf_tmp, double *time, int *steps, void *stream)
+{
+    if (!a || !time || !amr_dx_ok(a, dx_coarse))
+        return set_error(BX_ERR_INVALID, "bx_amr_run: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    int         rc;
+    // post_init: average down, then every ghost cell of both levels
+    if ((rc = avg_down(a, Uf, Uc, s)) != BX_OK || (rc = do_fill_all(a->crse, Uc, s)) != BX_OK
+        || (rc = do_fill_all(a->fine, Uf, s)) != BX_OK)
+        return rc;
+    double t = *time;
+    int    n = 0;
+    while ((n < max_step || max_step < 0) && (t < stop_time || stop_time < 0.0))
+    {
+        double dt;
+        if ((rc = bx_amr_est_dt(a, dx_coarse, cfl, t, stop_time, Uc, Uf, &dt, stream)) != BX_OK)
+            return rc;
+        if ((rc = bx_amr_advance(a, dt, dx_coarse, t, Uc, Uc_tmp, Uc_old, Uf, Uf_tmp, stream))
+            != BX_OK)
+            return rc;
+        t = t + dt;
+        ++n;
+    }
+    BX_HIP_CHECK(hipStreamSynchronize(s));
+    *time = t;
+    if (steps)
+        *steps = n;
+    return BX_OK;
+}

@@ -15,6 +15,7 @@
 #include "../../include/bamrex_hip.h"
 #include "bx_internal.h"
 #include "bx_kernels.h"
+#include "bx_level.h"
 #include "bx_physics.h"
 
 using namespace bx;
@@ -22,20 +23,11 @@ using namespace bx;
 // ------------------------------------------------------------ error state
 static thread_local std::string g_last_error;
 
-static int set_error(int code, const std::string &msg)
+int bx::set_error(int code, const std::string &msg)
 {
     g_last_error = msg;
     return code;
 }
-
-#define BX_HIP_CHECK(expr)                                                      \
-    do                                                                          \
-    {                                                                           \
-        hipError_t e__ = (expr);                                                \
-        if (e__ != hipSuccess)                                                  \
-            return set_error(BX_ERR_HIP, std::string(#expr) + ": "              \
-                                             + hipGetErrorString(e__));         \
-    } while (0)
 
 extern "C" const char *bx_last_error(void) { return g_last_error.c_str(); }
 
@@ -141,52 +133,6 @@ __global__ void k_set_f64(double *p, double v)
     if (threadIdx.x == 0 && blockIdx.x == 0)
         *p = v;
 }
-
-// -------------------------------------------------------------- context
-struct StepGraph
-{
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t     graph = nullptr;
-    const double  *a = nullptr, *b = nullptr;
-    double         dx[3] = { 0, 0, 0 };
-    double         cfl = 0, stop = 0;
-    void          *stream = nullptr;
-};
-
-struct bx_ctx
-{
-    bx_params p;
-    int       D = 0, NC = 0;
-    EulerParm pm;
-    LimTable  lt;
-    int       lv_mode = LV_SHARED;
-
-    bool                 level_set = false;
-    bx_geometry          geom;
-    std::vector<bx_box>  hboxes;
-    std::vector<DevBox>  hdev;
-    DevBox              *d_boxes = nullptr;
-    int64_t              level_extent = 0; // doubles spanned by all FABs
-
-    std::vector<long long> ib_x, ib_t[3], cbeg;
-    long long *d_ib_x = nullptr, *d_ib_t[3] = { nullptr, nullptr, nullptr },
-              *d_cbeg = nullptr;
-    long long  ncells = 0;
-
-    std::vector<GhostSeg> segs[4]; // [0..D-1]: ghosts in dir d only; [3]: rest
-    GhostSeg             *d_segs[4] = { nullptr, nullptr, nullptr, nullptr };
-    long long             gcells[4] = { 0, 0, 0, 0 };
-
-    unsigned long long *d_maxbits = nullptr;
-    double             *d_dt = nullptr, *d_time = nullptr;
-    long long          *d_step = nullptr;
-
-    bx_tuning tune;
-    int       ncu   = 0;
-    int       occ_x = 1, occ_t = 1;
-
-    StepGraph graphs[2];
-};
 
 static void free_dev(void *p)
 {
@@ -674,7 +620,7 @@ static int persistent_grid(const bx_ctx *c, long long nitems, int block, int occ
     return (int)std::max<long long>(g, 1);
 }
 
-static int do_sweep(bx_ctx *c, int dir, const double dx[3], const double *Uin,
+int do_sweep(bx_ctx *c, int dir, const double dx[3], const double *Uin,
                     double *Uout, unsigned long long *maxbits, hipStream_t s)
 {
     SweepLaunch L;
@@ -711,8 +657,7 @@ static int do_sweep(bx_ctx *c, int dir, const double dx[3], const double *Uin,
     return BX_OK;
 }
 
-static int do_fill(bx_ctx *c, double *U, int list, hipStream_t s,
-                   const StepTail &tail = StepTail())
+int do_fill(bx_ctx *c, double *U, int list, hipStream_t s, const StepTail &tail)
 {
     const int nseg = (int)c->segs[list].size();
     if (nseg == 0 && !tail.d_time)
@@ -723,7 +668,7 @@ static int do_fill(bx_ctx *c, double *U, int list, hipStream_t s,
     return BX_OK;
 }
 
-static int do_fill_all(bx_ctx *c, double *U, hipStream_t s)
+int do_fill_all(bx_ctx *c, double *U, hipStream_t s)
 {
     int rc;
     for (int l = 0; l < c->D; ++l)
@@ -732,7 +677,7 @@ static int do_fill_all(bx_ctx *c, double *U, hipStream_t s)
     return do_fill(c, U, 3, s);
 }
 
-static int do_maxspeed(bx_ctx *c, const double *U, hipStream_t s)
+int do_maxspeed(bx_ctx *c, const double *U, hipStream_t s)
 {
     BX_HIP_CHECK(launch_set_u64(c->d_maxbits, 0ULL, s));
     const int grid = (int)std::min<long long>((c->ncells + 255) / 256,
@@ -753,7 +698,7 @@ static int do_maxspeed(bx_ctx *c, const double *U, hipStream_t s)
             return set_error(BX_ERR_STATE, "bx_set_level has not been called"); \
     } while (0)
 
-static bool dx_ok(const bx_ctx *c, const double dx[3])
+bool dx_ok(const bx_ctx *c, const double dx[3])
 {
     if (!dx)
         return false;
@@ -763,7 +708,7 @@ static bool dx_ok(const bx_ctx *c, const double dx[3])
     return true;
 }
 
-static int set_scalar(double *p, double v, hipStream_t s)
+int set_scalar(double *p, double v, hipStream_t s)
 {
     hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(64), 0, s, p, v);
     BX_HIP_CHECK(hipGetLastError());

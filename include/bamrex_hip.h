@@ -213,6 +213,53 @@ int         bx_device_name(char *buf, int len);
 int         bx_copy_kernel(void *dst, const void *src, size_t bytes, void *stream);
 const char *bx_last_error(void);
 
+/* ==========================================================================
+ * Two AMR levels (refinement ratio 2, fine level subcycled)
+ * ==========================================================================
+ * What AMReX's Amr / FluxRegister / FillPatch machinery does around
+ * AmrLevelAdv for a coarse level and one fine level: the coarse advance, two
+ * fine advances of dt/2 whose coarse-fine ghost cells are interpolated from
+ * the coarse old and new states (FillPatcherFill, AmrLevelAdv_advance.cpp:
+ * 121-129; cell_cons_interp, AmrLevelAdv.cpp:171-175), flux registers
+ * (_advance.cpp:61-81, 192-254), then reflux and avgDown (post_timestep,
+ * AmrLevelAdv.cpp:628-652, 918-987).  Both levels are bx_ctx with the same
+ * bx_params; the fine geometry is the coarse one refined by 2 and its boxes
+ * are ratio-aligned (even lo, odd hi) in fine index space.  Device state
+ * buffers belong to the caller, each level's packed layout as for bx_run. */
+typedef struct bx_amr bx_amr;
+/* Plans the coarse-fine ghost cells, register faces and covered cells. */
+int  bx_amr_create(bx_ctx *coarse, bx_ctx *fine, bx_amr **out);
+void bx_amr_destroy(bx_amr *a);
+int  bx_amr_info(const bx_amr *a, int *n_cf_segs, int *n_register_faces,
+                 int *n_reflux_cells);
+/* AmrLevelAdv::avgDown (AmrLevelAdv.cpp:947-987): covered coarse cells
+ * become the average of their fine cells; coarse ghosts refilled. */
+int bx_amr_avg_down(bx_amr *a, const double *Uf, double *Uc, void *stream);
+/* FillPatch of the fine level at `time` (t_old <= time <= t_new): fine-fine
+ * and physical ghosts, then the coarse-fine face ghosts from the coarse
+ * states at t_old and t_new (ghosts of both filled). */
+int bx_amr_fill_fine(bx_amr *a, double *Uf, const double *Uc_old,
+                     const double *Uc_new, double t_old, double t_new,
+                     double time, void *stream);
+/* One coarse step of dt at `time` (Amr::timeStep for levels 0 and 1 plus
+ * post_timestep).  Uc, Uf: states with ghosts filled (in/out); Uc_tmp,
+ * Uf_tmp: ping-pong buffers; Uc_old receives the coarse state at `time`. */
+int bx_amr_advance(bx_amr *a, double dt, const double dx_coarse[3], double time,
+                   double *Uc, double *Uc_tmp, double *Uc_old, double *Uf,
+                   double *Uf_tmp, void *stream);
+/* computeNewDt over both levels (AmrLevelAdv.cpp:589-618): min(dt_est(0),
+ * 2 dt_est(1)) with the stop-time clamp; *dt is the coarse dt.  Synchronous. */
+int bx_amr_est_dt(bx_amr *a, const double dx_coarse[3], double cfl, double time,
+                  double stop_time, const double *Uc, const double *Uf,
+                  double *dt, void *stream);
+/* The main.cpp:76-94 loop on two levels: post_init avgDown, then steps until
+ * max_step or stop_time.  *time in: start time, out: final time.
+ * Synchronous. */
+int bx_amr_run(bx_amr *a, const double dx_coarse[3], double cfl,
+               double stop_time, int max_step, double *Uc, double *Uc_tmp,
+               double *Uc_old, double *Uf, double *Uf_tmp, double *time,
+               int *steps, void *stream);
+
 /* ============================================================================
  * Pressure Helmholtz solve of the IMEX schemes (PEEK / PESK) on one 2D
  * periodic level, and the learned preconditioner.

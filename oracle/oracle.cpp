@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <vector>
 
@@ -678,8 +679,13 @@ static void fill_consv_boundary(Level &L)
 
 // one sweep of advance_explicit (AmrLevelAdv_advance.cpp:275-357) over all
 // boxes, writing the valid cells of out
+// Sees the face fluxes of one box and direction (the flux-register hook of
+// the two-level driver; called concurrently for different boxes).
+typedef std::function<void(int box, int dir, const Fab &flux)> FluxHook;
+
 static void sweep(Level &L, std::vector<Fab> &in, std::vector<Fab> &out,
-                  int d, double dt, const double dx[3], int num_method)
+                  int d, double dt, const double dx[3], int num_method,
+                  const FluxHook *hook = nullptr)
 {
     const int D     = L.D;
     const int ncomp = D + 2;
@@ -691,6 +697,8 @@ static void sweep(Level &L, std::vector<Fab> &in, std::vector<Fab> &out,
         Fab        flux;
         flux.define(surrounding_nodes(bx, d), ncomp);
         compute_flux(num_method, bx, flux, in[b], dx, dt, L.parm, d);
+        if (hook)
+            (*hook)((int)b, d, flux);
         const Fab &Uin  = in[b];
         Fab       &Uout = out[b];
         for (int n = 0; n < ncomp; ++n)
@@ -741,13 +749,13 @@ static double compute_dt(const Level &L, const double dx[3], double cfl,
 }
 
 static void advance_explicit(Level &L, double dt, const double dx[3],
-                             int num_method)
+                             int num_method, const FluxHook *hook = nullptr)
 {
     std::vector<Fab> tmp = L.fabs; // Sborder2 = copy of Sborder (:121)
     std::vector<Fab> *in = &L.fabs, *out = &tmp;
     for (int d = 0; d < L.D; ++d)
     {
-        sweep(L, *in, *out, d, dt, dx, num_method);
+        sweep(L, *in, *out, d, dt, dx, num_method, hook);
         std::swap(L.fabs, *out); // fill boundary needs the level view
         fill_consv_boundary(L);
         std::swap(L.fabs, *out);
@@ -755,6 +763,334 @@ static void advance_explicit(Level &L, double dt, const double dx[3],
     }
     if (in != &L.fabs)
         L.fabs = *in;
+}
+
+
+// ============================================================ two levels
+// Refinement ratio 2, fine level subcycled (Amr::timeStep), restated from
+// AMReX's documented semantics (AMReX is not available here):
+//   FillPatchTwoLevels: fine ghost cells not covered by fine boxes and inside
+//     the (periodic) domain get coarse data interpolated linearly in time
+//     (copy at t_old; else alpha*old + beta*new) and conservatively in space
+//     (cell_cons_interp with MC slopes limited to the neighbourhood range);
+//     face ghost cells only (the sweeps read no others);
+//   FluxRegister: fluxes scaled by dt * face area (_advance.cpp:192-207);
+//     CrseInit(-1) sets -F_c, FineAdd(+1) adds the fine faces of each coarse
+//     face (transverse children x fastest), one FineAdd per fine step;
+//   Reflux (scale 1): coarse cell outside the fine region += sign*reg/vol,
+//     sign -1 on the fine boxes' low faces, +1 on their high faces, in the
+//     orientation order x lo, y lo, z lo, x hi, y hi, z hi;
+//   avgDown: covered coarse cell = volfrac * sum of children (x fastest);
+//   computeNewDt over both levels (AmrLevelAdv.cpp:589-618).
+// Parity against AMReX of the interpolation formula and of these summation
+// orders is unpinned; the tests pin conservation, free streams, linear
+// exactness and bounds of the interpolation, and the sign of the reflux.
+static int box_of(const Level &L, const int x[3])
+{
+    for (size_t s = 0; s < L.valid.size(); ++s)
+        if (inside(L.valid[s], x[0], x[1], x[2]))
+            return (int)s;
+    return -1;
+}
+
+static bool wrap_level(const Level &L, int x[3])
+{
+    for (int d = 0; d < L.D; ++d)
+    {
+        const int len = L.dhi[d] - L.dlo[d] + 1;
+        if (x[d] < L.dlo[d] || x[d] > L.dhi[d])
+        {
+            if (L.bclo[d] != 1)
+                return false;
+            x[d] += x[d] < L.dlo[d] ? len : -len;
+        }
+    }
+    return true;
+}
+
+struct AmrSlot
+{
+    int d, orient, sign;
+    int cbox, cface[3];     // coarse face: low face of cell cface in cbox
+    int fbox, fface[4][3];  // fine faces (low faces of these cells) in fbox
+    int tbox, tcell[3];     // coarse cell receiving the correction
+};
+
+// Register faces: for every fine box, direction and side, each coarse face
+// of that side whose outside neighbour (periodically wrapped) is in the
+// domain and not refined.
+static std::vector<AmrSlot> amr_slots(const Level &C, const Level &F)
+{
+    const int            D = C.D;
+    std::vector<AmrSlot> out;
+    for (int side = 0; side < 2; ++side)
+        for (int d = 0; d < D; ++d)
+            for (size_t fb = 0; fb < F.valid.size(); ++fb)
+            {
+                const Box &v = F.valid[fb];
+                Box        cv;
+                for (int e = 0; e < 3; ++e)
+                {
+                    cv.lo[e] = e < D ? v.lo[e] / 2 : 0;
+                    cv.hi[e] = e < D ? v.hi[e] / 2 : 0;
+                }
+                const int fine_face = side == 0 ? v.lo[d] : v.hi[d] + 1;
+                const int crse_face = fine_face / 2;
+                cv.lo[d] = cv.hi[d] = 0;
+                for_box(cv, [&](int i, int j, int k) {
+                    AmrSlot s;
+                    int     t[3] = { i, j, k };
+                    int     o[3] = { i, j, k };
+                    o[d]         = side == 0 ? crse_face - 1 : crse_face;
+                    int child[3] = { 2 * o[0], D > 1 ? 2 * o[1] : 0, D > 2 ? 2 * o[2] : 0 };
+                    if (!wrap_level(F, child) || box_of(F, child) >= 0)
+                        return;
+                    int ow[3] = { o[0], o[1], o[2] };
+                    wrap_level(C, ow);
+                    int in[3] = { t[0], t[1], t[2] };
+                    in[d]     = side == 0 ? crse_face : crse_face - 1;
+                    s.d       = d;
+                    s.orient  = side * 3 + d;
+                    s.sign    = side == 0 ? -1 : 1;
+                    s.cbox    = box_of(C, in);
+                    s.tbox    = box_of(C, ow);
+                    if (s.cbox < 0 || s.tbox < 0)
+                        throw std::runtime_error("coarse level does not cover the domain");
+                    for (int e = 0; e < 3; ++e)
+                    {
+                        s.cface[e] = t[e];
+                        s.tcell[e] = ow[e];
+                    }
+                    s.cface[d] = crse_face;
+                    s.fbox     = (int)fb;
+                    int q      = 0;
+                    // transverse children, x fastest
+                    for (int c2 = 0; c2 < (D > 2 && d != 2 ? 2 : 1); ++c2)
+                        for (int c1 = 0; c1 < (D > 1 && d != 1 ? 2 : 1); ++c1)
+                            for (int c0 = 0; c0 < (d != 0 ? 2 : 1); ++c0)
+                            {
+                                s.fface[q][0] = d == 0 ? fine_face : 2 * t[0] + c0;
+                                s.fface[q][1] = D < 2 ? 0 : (d == 1 ? fine_face : 2 * t[1] + c1);
+                                s.fface[q][2] = D < 3 ? 0 : (d == 2 ? fine_face : 2 * t[2] + c2);
+                                ++q;
+                            }
+                    out.push_back(s);
+                });
+            }
+    return out;
+}
+
+static double mc_slope(double um, double uc, double up)
+{
+    const double dc = 0.5 * (up - um);
+    const double dl = 2 * (uc - um);
+    const double dr = 2 * (up - uc);
+    if (!(dl * dr > 0))
+        return 0.0;
+    const double m = std::fmin(std::fabs(dc), std::fmin(std::fabs(dl), std::fabs(dr)));
+    return dc < 0 ? -m : m;
+}
+
+// Coarse-fine face ghost cells of the fine level at a time between the two
+// coarse states (mode 0: Cold; mode 1: alpha*Cold + beta*Cnew).
+static void cf_fill(Level &F, const Level &Cold, const Level &Cnew, int mode,
+                    double alpha, double beta)
+{
+    const int D  = F.D;
+    const int NC = D + 2;
+    for (size_t b = 0; b < F.fabs.size(); ++b)
+    {
+        const Box &v = F.valid[b];
+        for_box(F.fabs[b].bx, [&](int i, int j, int k) {
+            int x[3] = { i, j, k }, nout = 0;
+            for (int d = 0; d < D; ++d)
+                nout += (x[d] < v.lo[d] || x[d] > v.hi[d]);
+            if (nout != 1)
+                return;
+            int img[3] = { i, j, k };
+            if (!wrap_level(F, img) || box_of(F, img) >= 0)
+                return;
+            int    ci[3] = { 0, 0, 0 };
+            double off[3] = { 0, 0, 0 };
+            for (int d = 0; d < D; ++d)
+            {
+                ci[d]  = img[d] >> 1;
+                off[d] = (img[d] & 1) ? 0.25 : -0.25;
+            }
+            const int cb = box_of(Cold, ci);
+            for (int m = 0; m < NC; ++m)
+            {
+                auto val = [&](int a0, int a1, int a2) {
+                    const double o = Cold.fabs[cb](ci[0] + a0, ci[1] + a1, ci[2] + a2, m);
+                    if (mode == 0)
+                        return o;
+                    const double nw = Cnew.fabs[cb](ci[0] + a0, ci[1] + a1, ci[2] + a2, m);
+                    return alpha * o + beta * nw;
+                };
+                const double uc   = val(0, 0, 0);
+                double       s[3] = { 0, 0, 0 };
+                s[0]              = mc_slope(val(-1, 0, 0), uc, val(1, 0, 0));
+                if (D > 1)
+                    s[1] = mc_slope(val(0, -1, 0), uc, val(0, 1, 0));
+                if (D > 2)
+                    s[2] = mc_slope(val(0, 0, -1), uc, val(0, 0, 1));
+                double umin = uc, umax = uc;
+                for (int kk = (D > 2 ? -1 : 0); kk <= (D > 2 ? 1 : 0); ++kk)
+                    for (int jj = (D > 1 ? -1 : 0); jj <= (D > 1 ? 1 : 0); ++jj)
+                        for (int ii = -1; ii <= 1; ++ii)
+                        {
+                            const double w = val(ii, jj, kk);
+                            umin           = std::fmin(umin, w);
+                            umax           = std::fmax(umax, w);
+                        }
+                double ext = 0.0;
+                for (int d = 0; d < D; ++d)
+                    ext = ext + 0.25 * std::fabs(s[d]);
+                double lim = 1.0;
+                if (ext > 0)
+                    lim = std::fmin(1.0, std::fmin((umax - uc) / ext, (uc - umin) / ext));
+                double r = uc;
+                for (int d = 0; d < D; ++d)
+                    r = r + off[d] * (lim * s[d]);
+                F.fabs[b](i, j, k, m) = r;
+            }
+        });
+    }
+}
+
+static void avg_down(Level &C, const Level &F)
+{
+    const int    D       = C.D;
+    const double volfrac = 1.0 / (double)(1 << D);
+    for (size_t fb = 0; fb < F.valid.size(); ++fb)
+    {
+        const Box &v = F.valid[fb];
+        Box        cv;
+        for (int e = 0; e < 3; ++e)
+        {
+            cv.lo[e] = e < D ? v.lo[e] / 2 : 0;
+            cv.hi[e] = e < D ? v.hi[e] / 2 : 0;
+        }
+        for_box(cv, [&](int i, int j, int k) {
+            int       c[3] = { i, j, k };
+            const int cb   = box_of(C, c);
+            for (int m = 0; m < D + 2; ++m)
+            {
+                double sum = 0;
+                for (int kr = 0; kr < (D > 2 ? 2 : 1); ++kr)
+                    for (int jr = 0; jr < (D > 1 ? 2 : 1); ++jr)
+                        for (int ir = 0; ir < 2; ++ir)
+                            sum = sum + F.fabs[fb](2 * i + ir, D > 1 ? 2 * j + jr : 0,
+                                                   D > 2 ? 2 * k + kr : 0, m);
+                C.fabs[cb](i, j, k, m) = volfrac * sum;
+            }
+        });
+    }
+}
+
+static double level_dt_est(const Level &L, const double dx[3], double cfl)
+{
+    const double max_speed = max_wave_speed(L);
+    double       dt_est    = 1.0e20;
+    for (int d = 0; d < L.D; ++d)
+        dt_est = std::min(dt_est, dx[d] / max_speed);
+    return dt_est * cfl;
+}
+
+static double amr_dt(const Level &C, const Level &F, const double dxc[3], double cfl,
+                     double cur_time, double stop_time)
+{
+    double dxf[3];
+    for (int d = 0; d < 3; ++d)
+        dxf[d] = dxc[d] / 2;
+    const double dt_min[2] = { level_dt_est(C, dxc, cfl), level_dt_est(F, dxf, cfl) };
+    double       dt_0      = 1.0e+100;
+    int          n_factor  = 1;
+    for (int i = 0; i < 2; ++i)
+    {
+        n_factor *= (i == 0 ? 1 : 2);
+        dt_0 = std::min(dt_0, n_factor * dt_min[i]);
+    }
+    const double eps = 0.001 * dt_0;
+    if (stop_time >= 0.0)
+        if ((cur_time + dt_0) > (stop_time - eps))
+            dt_0 = stop_time - cur_time;
+    return dt_0;
+}
+
+static void amr_advance(Level &C, Level &F, const std::vector<AmrSlot> &slots, double dt,
+                        const double dxc[3], double time, int num_method, bool do_reflux)
+{
+    const int           D  = C.D;
+    const int           NC = D + 2;
+    std::vector<double> reg(slots.size() * NC, 0.0);
+    double              dxf[3];
+    for (int d = 0; d < 3; ++d)
+        dxf[d] = dxc[d] / 2;
+    auto scale_of = [&](int d, double h, const double *dxl) {
+        double sc = h;
+        for (int e = 0; e < D; ++e)
+            if (e != d)
+                sc *= dxl[e];
+        return sc;
+    };
+    const Level Cold = C;
+    FluxHook crse_hook = [&](int box, int dir, const Fab &flux) {
+        const double sc = scale_of(dir, dt, dxc);
+        for (size_t q = 0; q < slots.size(); ++q)
+            if (slots[q].d == dir && slots[q].cbox == box)
+                for (int m = 0; m < NC; ++m)
+                    reg[q * NC + m] = -1.0 * (flux(slots[q].cface[0], slots[q].cface[1],
+                                                   slots[q].cface[2], m)
+                                              * sc);
+    };
+    advance_explicit(C, dt, dxc, num_method, &crse_hook);
+    const double dtf   = dt / 2;
+    const double t_new = time + dt;
+    FluxHook fine_hook = [&](int box, int dir, const Fab &flux) {
+        const double sf   = scale_of(dir, dtf, dxf);
+        const int    nsub = 1 << (D - 1);
+        for (size_t q = 0; q < slots.size(); ++q)
+            if (slots[q].d == dir && slots[q].fbox == box)
+                for (int c = 0; c < nsub; ++c)
+                    for (int m = 0; m < NC; ++m)
+                        reg[q * NC + m]
+                            = reg[q * NC + m]
+                              + flux(slots[q].fface[c][0], slots[q].fface[c][1],
+                                     slots[q].fface[c][2], m)
+                                    * sf;
+    };
+    for (int sub = 0; sub < 2; ++sub)
+    {
+        const double tf = sub == 0 ? time : time + dtf;
+        fill_consv_boundary(F);
+        if (tf == time)
+            cf_fill(F, Cold, C, 0, 1.0, 0.0);
+        else
+            cf_fill(F, Cold, C, 1, (t_new - tf) / (t_new - time), (tf - time) / (t_new - time));
+        advance_explicit(F, dtf, dxf, num_method, &fine_hook);
+    }
+    if (do_reflux)
+    {
+        double vol = dxc[0];
+        for (int d = 1; d < D; ++d)
+            vol *= dxc[d];
+        for (int orient = 0; orient < 6; ++orient)
+            for (size_t q = 0; q < slots.size(); ++q)
+            {
+                const AmrSlot &s = slots[q];
+                if (s.orient != orient)
+                    continue;
+                for (int m = 0; m < NC; ++m)
+                {
+                    double &u = C.fabs[s.tbox](s.tcell[0], s.tcell[1], s.tcell[2], m);
+                    u         = u + (s.sign * reg[q * NC + m]) / vol;
+                }
+            }
+    }
+    avg_down(C, F);
+    fill_consv_boundary(C);
+    fill_consv_boundary(F);
 }
 
 } // namespace oracle
@@ -1047,6 +1383,80 @@ double or_pow_negz_fn(double x, double gamma)
 double or_pow_invz_fn(double x, double gamma)
 {
     return or_pow_invz(x, 1 / ((gamma - 1) / (2 * gamma)), gamma);
+}
+
+// Two levels (ratio 2): the main.cpp loop with Amr::timeStep subcycling,
+// post_init avgDown first.  Coarse boxes/U as or_run; fine boxes in fine
+// index space, fine domain = coarse domain refined by 2, same BCs.
+int or_amr_run(const or_params *p, const int *cboxes, int ncbox, const int *fboxes,
+               int nfbox, int ng, const int *dlo, const int *dhi, const int *bclo,
+               const int *bchi, const double *dx, double cfl, double start_time,
+               double stop_time, int max_step, double *Uc, const int64_t *coffsets,
+               double *Uf, const int64_t *foffsets, int do_reflux, double *time_out,
+               int *steps_out)
+{
+    Level C, F;
+    int   fdlo[3], fdhi[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        fdlo[d] = 2 * dlo[d];
+        fdhi[d] = 2 * dhi[d] + 1;
+    }
+    make_level(C, p, cboxes, ncbox, ng, dlo, dhi, bclo, bchi);
+    make_level(F, p, fboxes, nfbox, ng, fdlo, fdhi, bclo, bchi);
+    level_in(C, Uc, coffsets);
+    level_in(F, Uf, foffsets);
+    const std::vector<AmrSlot> slots = amr_slots(C, F);
+    avg_down(C, F);
+    fill_consv_boundary(C);
+    fill_consv_boundary(F);
+    double time  = start_time;
+    int    steps = 0;
+    while ((steps < max_step || max_step < 0)
+           && (time < stop_time || stop_time < 0.0))
+    {
+        const double dt = amr_dt(C, F, dx, cfl, time, stop_time);
+        amr_advance(C, F, slots, dt, dx, time, p->num_method, do_reflux != 0);
+        time = time + dt;
+        ++steps;
+    }
+    level_out(C, Uc, coffsets);
+    level_out(F, Uf, foffsets);
+    if (time_out)
+        *time_out = time;
+    if (steps_out)
+        *steps_out = steps;
+    return 0;
+}
+
+// The fine level's ghost cells at `time` between coarse states at t_old and
+// t_new: fine-fine and physical (fill_consv_boundary), then coarse-fine.
+int or_amr_fill_fine(const or_params *p, const int *cboxes, int ncbox, const int *fboxes,
+                     int nfbox, int ng, const int *dlo, const int *dhi, const int *bclo,
+                     const int *bchi, const double *Uc_old, const double *Uc_new,
+                     const int64_t *coffsets, double *Uf, const int64_t *foffsets,
+                     double t_old, double t_new, double time)
+{
+    Level Co, Cn, F;
+    int   fdlo[3], fdhi[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        fdlo[d] = 2 * dlo[d];
+        fdhi[d] = 2 * dhi[d] + 1;
+    }
+    make_level(Co, p, cboxes, ncbox, ng, dlo, dhi, bclo, bchi);
+    make_level(Cn, p, cboxes, ncbox, ng, dlo, dhi, bclo, bchi);
+    make_level(F, p, fboxes, nfbox, ng, fdlo, fdhi, bclo, bchi);
+    level_in(Co, Uc_old, coffsets);
+    level_in(Cn, Uc_new, coffsets);
+    level_in(F, Uf, foffsets);
+    fill_consv_boundary(F);
+    if (time == t_old || !(t_new != t_old))
+        cf_fill(F, Co, Cn, 0, 1.0, 0.0);
+    else
+        cf_fill(F, Co, Cn, 1, (t_new - time) / (t_new - t_old), (time - t_old) / (t_new - t_old));
+    level_out(F, Uf, foffsets);
+    return 0;
 }
 
 } // extern "C"

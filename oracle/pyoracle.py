@@ -56,6 +56,10 @@ _SIGS = {
     "or_consv_from_primv": [ctypes.c_int, D, ctypes.c_double, ctypes.c_double, D],
     "or_primv_from_consv": [ctypes.c_int, D, ctypes.c_double, ctypes.c_double, D],
     "or_hllc_face": [ctypes.c_int, ctypes.c_int, D, D, ctypes.c_double, ctypes.c_double, D],
+    "or_amr_run": [PP, I, ctypes.c_int, I, ctypes.c_int, ctypes.c_int, I, I, I, I, D, ctypes.c_double,
+                   ctypes.c_double, ctypes.c_double, ctypes.c_int, D, I64, D, I64, ctypes.c_int, D, I],
+    "or_amr_fill_fine": [PP, I, ctypes.c_int, I, ctypes.c_int, ctypes.c_int, I, I, I, I, D, D, I64, D,
+                         I64, ctypes.c_double, ctypes.c_double, ctypes.c_double],
 }
 
 
@@ -133,6 +137,30 @@ class Oracle:
                         _dp(dxa), cfl, start_time, stop_time, max_step, _dp(U),
                         offs.ctypes.data_as(I64), ctypes.byref(t), ctypes.byref(n))
         return t.value, n.value
+
+    def amr_run(self, p, clayout, Uc, flayout, Uf, domain_lo, domain_hi, bc_lo, bc_hi, dx, cfl,
+                stop_time=-1.0, max_step=-1, start_time=0.0, do_reflux=True):
+        """Two levels, ratio 2 (or_amr_run); Uc, Uf updated in place."""
+        cb, dlo, dhi, bl, bh, coffs = self._geom(clayout, domain_lo, domain_hi, bc_lo, bc_hi)
+        fb, *_, foffs = self._geom(flayout, domain_lo, domain_hi, bc_lo, bc_hi)
+        t = ctypes.c_double()
+        n = ctypes.c_int()
+        dxa = np.array(list(dx) + [0.0] * (3 - len(dx)), dtype=np.float64)
+        self.lib.or_amr_run(ctypes.byref(p), cb, len(clayout.boxes), fb, len(flayout.boxes),
+                            clayout.ng, dlo, dhi, bl, bh, _dp(dxa), cfl, start_time, stop_time,
+                            max_step, _dp(Uc), coffs.ctypes.data_as(I64), _dp(Uf),
+                            foffs.ctypes.data_as(I64), int(bool(do_reflux)), ctypes.byref(t),
+                            ctypes.byref(n))
+        return t.value, n.value
+
+    def amr_fill_fine(self, p, clayout, Uc_old, Uc_new, flayout, Uf, domain_lo, domain_hi, bc_lo,
+                      bc_hi, t_old, t_new, time):
+        cb, dlo, dhi, bl, bh, coffs = self._geom(clayout, domain_lo, domain_hi, bc_lo, bc_hi)
+        fb, *_, foffs = self._geom(flayout, domain_lo, domain_hi, bc_lo, bc_hi)
+        self.lib.or_amr_fill_fine(ctypes.byref(p), cb, len(clayout.boxes), fb, len(flayout.boxes),
+                                  clayout.ng, dlo, dhi, bl, bh, _dp(Uc_old), _dp(Uc_new),
+                                  coffs.ctypes.data_as(I64), _dp(Uf), foffs.ctypes.data_as(I64),
+                                  t_old, t_new, time)
 
     def sweep(self, p, layout, Uin, Uout, dir, dt, dx, domain_lo, domain_hi, bc_lo, bc_hi):
         barr, dlo, dhi, bl, bh, offs = self._geom(layout, domain_lo, domain_hi, bc_lo, bc_hi)

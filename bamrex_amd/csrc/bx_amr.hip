@@ -525,6 +525,38 @@ static int amr_plan(bx_amr *a)
             }
     }
 
+    // merge runs of consecutive rows (then planes) with the same x range,
+    // boxes and shift into rectangles: fewer, larger segments
+    for (int dim_merge = 1; dim_merge < 3; ++dim_merge)
+    {
+        std::vector<CFSeg> merged;
+        for (const CFSeg &g : a->cf)
+        {
+            if (!merged.empty())
+            {
+                CFSeg &m    = merged.back();
+                bool   same = m.dst == g.dst && m.src == g.src;
+                for (int e = 0; e < 3 && same; ++e)
+                {
+                    same = same && m.shift[e] == g.shift[e];
+                    if (e == dim_merge)
+                        same = same && g.lo[e] == m.hi[e] + 1 && g.hi[e] == g.lo[e];
+                    else if (e < dim_merge)
+                        same = same && m.lo[e] == g.lo[e] && m.hi[e] == g.hi[e];
+                    else
+                        same = same && m.lo[e] == g.lo[e] && m.hi[e] == g.hi[e];
+                }
+                if (same)
+                {
+                    m.hi[dim_merge] = g.hi[dim_merge];
+                    continue;
+                }
+            }
+            merged.push_back(g);
+        }
+        a->cf.swap(merged);
+    }
+
     // ---- flux registers: per direction, per fine box side, per coarse
     // face of that side whose outside neighbour is not fine
     a->cface.clear();
@@ -893,9 +925,18 @@ extern "C" int bx_amr_advance(bx_amr *a, double dt, const double dx_coarse[3], d
         const double tf = sub == 0 ? time : time + dtf;
         if ((rc = bx_amr_fill_fine(a, Uf, Uc_old, Uc, time, t_new, tf, stream)) != BX_OK)
             return rc;
-        // the ping-pong buffer starts as a copy of the filled state (Sborder2)
-        BX_HIP_CHECK(hipMemcpyAsync(Uf_tmp, Uf, sizeof(double) * F->level_extent,
-                                    hipMemcpyDeviceToDevice, s));
+        // The reference's second buffer starts as a copy of the filled state
+        // (Sborder2).  Of it the sweeps read the valid cells they write
+        // first and the ghost cells of the directions after the first: the
+        // fills after each sweep provide the fine-fine and physical ones,
+        // and the coarse-fine ones are interpolated here, to the same values.
+        if (tf == time)
+            rc = cf_fill(a, Uf_tmp, Uc_old, Uc, 0, 1.0, 0.0, s);
+        else
+            rc = cf_fill(a, Uf_tmp, Uc_old, Uc, 1, (t_new - tf) / (t_new - time),
+                         (tf - time) / (t_new - time), s);
+        if (rc != BX_OK)
+            return rc;
         if ((rc = level_step(a, true, dtf, dxf, Uf, Uf_tmp, s)) != BX_OK)
             return rc;
     }
@@ -925,16 +966,11 @@ extern "C" int bx_amr_advance(bx_amr *a, double dt, const double dx_coarse[3], d
     return do_fill_all(F, Uf, s);
 }
 
-// estTimeStep of one level (AmrLevelAdv.cpp:395-477, explicit methods)
-static int level_dt_est(bx_ctx *L, const double *U, const double dx[3], double cfl,
-                        hipStream_t s, double *out)
+// estTimeStep of one level (AmrLevelAdv.cpp:395-477, explicit methods) from
+// the bits of its maximum wave speed
+static void level_dt_est(const bx_ctx *L, unsigned long long bits, const double dx[3],
+                         double cfl, double *out)
 {
-    int rc;
-    if ((rc = do_maxspeed(L, U, s)) != BX_OK)
-        return rc;
-    unsigned long long bits = 0;
-    BX_HIP_CHECK(hipMemcpyAsync(&bits, L->d_maxbits, sizeof(bits), hipMemcpyDeviceToHost, s));
-    BX_HIP_CHECK(hipStreamSynchronize(s));
     double max_speed;
     std::memcpy(&max_speed, &bits, sizeof(max_speed));
     double dt_est = 1.0e20;
@@ -944,7 +980,6 @@ static int level_dt_est(bx_ctx *L, const double *U, const double dx[3], double c
         dt_est         = (c < dt_est) ? c : dt_est;
     }
     *out = dt_est * cfl;
-    return BX_OK;
 }
 
 extern "C" int bx_amr_est_dt(bx_amr *a, const double dx_coarse[3], double cfl, double time,
@@ -959,9 +994,16 @@ extern "C" int bx_amr_est_dt(bx_amr *a, const double dx_coarse[3], double cfl, d
         dxf[d] = dx_coarse[d] / 2;
     double dmin[2];
     int    rc;
-    if ((rc = level_dt_est(a->crse, Uc, dx_coarse, cfl, s, &dmin[0])) != BX_OK
-        || (rc = level_dt_est(a->fine, Uf, dxf, cfl, s, &dmin[1])) != BX_OK)
+    if ((rc = do_maxspeed(a->crse, Uc, s)) != BX_OK || (rc = do_maxspeed(a->fine, Uf, s)) != BX_OK)
         return rc;
+    unsigned long long bits[2] = { 0, 0 };
+    BX_HIP_CHECK(hipMemcpyAsync(&bits[0], a->crse->d_maxbits, sizeof(bits[0]),
+                                hipMemcpyDeviceToHost, s));
+    BX_HIP_CHECK(hipMemcpyAsync(&bits[1], a->fine->d_maxbits, sizeof(bits[1]),
+                                hipMemcpyDeviceToHost, s));
+    BX_HIP_CHECK(hipStreamSynchronize(s));
+    level_dt_est(a->crse, bits[0], dx_coarse, cfl, &dmin[0]);
+    level_dt_est(a->fine, bits[1], dxf, cfl, &dmin[1]);
     // computeNewDt (AmrLevelAdv.cpp:589-618): n_cycle = {1, 2}
     double dt_0     = 1.0e+100;
     int    n_factor = 1;

@@ -680,7 +680,8 @@ int do_fill_all(bx_ctx *c, double *U, hipStream_t s)
 int do_maxspeed(bx_ctx *c, const double *U, hipStream_t s)
 {
     BX_HIP_CHECK(launch_set_u64(c->d_maxbits, 0ULL, s));
-    const int grid = (int)std::min<long long>((c->ncells + 255) / 256,
+    // chunks of 4096 cells per block (k_max_wave_speed)
+    const int grid = (int)std::min<long long>((c->ncells + 4095) / 4096,
                                               (long long)c->ncu * 8);
     BX_HIP_CHECK(launch_max_wave_speed(c->D, U, c->d_boxes, c->d_cbeg,
                                        (int)c->hboxes.size(), c->ncells,

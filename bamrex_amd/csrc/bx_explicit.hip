@@ -480,36 +480,60 @@ __global__ __launch_bounds__(256) void k_fill_ghosts(
 }
 
 // ------------------------------------------------------ max wave speed
+// Blocks take chunks of MWS_CHUNK consecutive cells of the flattened level
+// (boxes in order, x fastest); a thread locates its box once per chunk and
+// walks forward from there, so the per-cell index work is 32-bit.
+constexpr int MWS_CHUNK = 4096;
+
 template <int D>
 __global__ __launch_bounds__(256) void k_max_wave_speed(
     const double *__restrict__ U, const DevBox *__restrict__ boxes,
     const long long *__restrict__ cbeg, int nbox, long long ncells, int ng,
     EulerParm pm, unsigned long long *d_maxbits)
 {
-    constexpr int      NC     = D + 2;
-    const long long    stride = (long long)gridDim.x * blockDim.x;
-    const int          ngy    = (D >= 2) ? ng : 0;
-    const int          ngz    = (D >= 3) ? ng : 0;
-    unsigned long long vmax   = 0ULL;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-         c < ncells; c += stride)
+    constexpr int      NC   = D + 2;
+    const int          ngy  = (D >= 2) ? ng : 0;
+    const int          ngz  = (D >= 3) ? ng : 0;
+    unsigned long long vmax = 0ULL;
+    for (long long c0 = (long long)blockIdx.x * MWS_CHUNK; c0 < ncells;
+         c0 += (long long)gridDim.x * MWS_CHUNK)
     {
-        const int     b     = find_box(cbeg, nbox, c);
-        const DevBox &bx    = boxes[b];
-        long long     local = c - cbeg[b];
-        const int     nx    = bx.hi[0] - bx.lo[0] + 1;
-        const int     ny    = (D >= 2) ? bx.hi[1] - bx.lo[1] + 1 : 1;
-        const int     i     = (int)(local % nx);
-        local /= nx;
-        const int       j = (int)(local % ny);
-        const int       k = (int)(local / ny);
-        const long long o = bx.base + (i + ng) + (long long)(j + ngy) * bx.jstride
-                            + (long long)(k + ngz) * bx.kstride;
-        double V[NC];
+        const long long cend = c0 + MWS_CHUNK < ncells ? c0 + MWS_CHUNK : ncells;
+        const long long cfst = c0 + threadIdx.x;
+        if (cfst >= cend)
+            continue;
+        // box geometry held in registers, reloaded only when the walk
+        // crosses into the next box
+        int       b    = find_box(cbeg, nbox, cfst);
+        long long bbeg = cbeg[b], bend = cbeg[b + 1];
+        DevBox    bx   = boxes[b];
+        for (long long c = cfst; c < cend; c += blockDim.x)
+        {
+            if (c >= bend)
+            {
+                do
+                {
+                    ++b;
+                    bbeg = bend;
+                    bend = cbeg[b + 1];
+                } while (c >= bend);
+                bx = boxes[b];
+            }
+            const unsigned local = (unsigned)(c - bbeg);
+            const unsigned nx    = (unsigned)(bx.hi[0] - bx.lo[0] + 1);
+            const unsigned ny    = (D >= 2) ? (unsigned)(bx.hi[1] - bx.lo[1] + 1) : 1u;
+            const unsigned i     = local % nx;
+            const unsigned r     = local / nx;
+            const unsigned j     = r % ny;
+            const unsigned k     = r / ny;
+            const long long o = bx.base + (i + ng) + (long long)(j + ngy) * bx.jstride
+                                + (long long)(k + ngz) * bx.kstride;
+            double V[NC];
 #pragma unroll
-        for (int n = 0; n < NC; ++n)
-            V[n] = U[o + n * bx.nstride];
-        vmax = speed_max(vmax, cell_wave_speed<D>(V, pm));
+            for (int n = 0; n < NC; ++n)
+                V[n] = U[o + n * bx.nstride];
+            vmax = speed_max(vmax, cell_wave_speed<D>(V, pm));
+        }
     }
     wave_max_to_global(vmax, d_maxbits);
 }

@@ -60,6 +60,10 @@ def main():
     ap.add_argument("--vortex-mgs", type=int, default=128)
     ap.add_argument("--vortex-steps", type=int, default=50)
     ap.add_argument("--no-uneven", action="store_true", help="skip the batched uneven-box leg")
+    ap.add_argument("--no-amr", action="store_true", help="skip the two-level AMR leg")
+    ap.add_argument("--amr-only", action="store_true", help="run only the two-level AMR leg")
+    ap.add_argument("--amr-n", type=int, default=2048)
+    ap.add_argument("--amr-steps", type=int, default=20)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -76,6 +80,9 @@ def main():
     log("device: " + name.value.decode())
     if args.imex_only:
         print(json.dumps({"imex": imex_leg(args)}), flush=True)
+        return
+    if args.amr_only:
+        print(json.dumps({"amr": amr_leg(args)}), flush=True)
         return
 
     log("building the %d^2 quadrant level (mgs %d)" % (args.ncell, args.mgs))
@@ -169,6 +176,9 @@ def main():
     vortex = None
     if not args.no_vortex:
         vortex = vortex_leg(args)
+    amr_out = None
+    if not args.no_amr:
+        amr_out = amr_leg(args)
     imex = None
     if not args.no_imex:
         imex = imex_leg(args)
@@ -189,6 +199,7 @@ def main():
         "cpu_baseline": cpu,
         "uneven": uneven,
         "vortex3d": vortex,
+        "amr": amr_out,
         "imex": imex,
     }
     print(json.dumps(out), flush=True)
@@ -324,6 +335,53 @@ def vortex_leg(args):
            "l1_error": [float(x) for x in l1]}
     log("vortex: %d steps to t=%.4g, %.3e cell-updates/s, L1 rho %.3e" % (
         nsteps, t_end, out["cell_updates_per_s"], l1[0]))
+    return out
+
+
+def amr_leg(args):
+    """Two levels, ratio 2: the quadrant problem on an n^2 coarse level (mgs
+    128) with a fixed fine level over the central half of the domain in
+    each direction (an n^2 fine region, boxes of 128), MHM-HLLC, minbee,
+    CFL 0.8.  One coarse step = coarse advance + two fine advances of dt/2
+    with coarse-fine interpolation, flux registers, reflux, avgDown and the
+    two-level dt (host-driven launches, one host read of the dt per step)."""
+    from bamrex_amd import _lib, amr, problems
+    from bamrex_amd.level import make_params
+    lib = _lib.lib()
+    n = args.amr_n
+    log("amr: building %d^2 coarse + %d^2 fine levels" % (n, n))
+
+    def quadrant(ncell, max_grid_size):
+        return problems.quadrant_setup(ncell=ncell, max_grid_size=max_grid_size)
+
+    s = amr.refine_setup(quadrant, n, 128, (n // 4, n // 4), (3 * n // 4 - 1, 3 * n // 4 - 1),
+                         fine_max_grid_size=128)
+    gp = make_params(2, s["adiabatic"], s["epsilon"], limiter=s["limiter"])
+    h = amr.AmrHierarchy(gp, s["layout"], s["fine_layout"], s["domain_lo"], s["domain_hi"],
+                         s["bc"], s["bc"])
+    # launch shapes tuned per level on the initial state, which is then reloaded
+    h.set_state(s["U"], s["Uf"])
+    h.avg_down()
+    dt = h.est_dt(s["dx"], 0.8)
+    h.coarse.autotune(s["dx"], dt)
+    h.fine.autotune(tuple(x / 2 for x in s["dx"]), dt / 2)
+    h.set_state(s["U"], s["Uf"])
+    del s["U"], s["Uf"]
+    h.run(s["dx"], 0.8, max_step=2)
+    _lib.check(lib.bx_stream_sync(h.stream), "sync")
+    t0 = time.perf_counter()
+    t_end, nsteps = h.run(s["dx"], 0.8, max_step=args.amr_steps, start_time=0.0)
+    el = time.perf_counter() - t0
+    info = h.info()
+    h.close()
+    nc, nf = s["layout"].ncells(), s["fine_layout"].ncells()
+    out = {"workload": "quadrant-2d-%d^2-mgs128+fine-%d^2-ratio2-mhm-hllc-minbee-cfl0.8" % (n, n),
+           "coarse_cells": nc, "fine_cells": nf, "coarse_steps": nsteps,
+           "ms_per_coarse_step": round(el * 1e3 / nsteps, 4),
+           "cell_updates_per_s": (nc + 2 * nf) * nsteps / el,
+           "cf_segments": info["cf_segments"], "register_faces": info["register_faces"]}
+    log("amr: %d coarse steps, %.3f ms/step, %.3e cell-updates/s" % (
+        nsteps, out["ms_per_coarse_step"], out["cell_updates_per_s"]))
     return out
 
 

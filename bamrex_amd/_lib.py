@@ -37,6 +37,12 @@ class bx_box(ctypes.Structure):
     ]
 
 
+class bx_plot_level(ctypes.Structure):
+    _fields_ = [("boxes", ctypes.c_void_p), ("nbox", ctypes.c_int), ("ng", ctypes.c_int),
+                ("U", ctypes.c_void_p), ("domain_lo", ctypes.c_int * 3), ("domain_hi", ctypes.c_int * 3),
+                ("dx", ctypes.c_double * 3), ("level_steps", ctypes.c_int)]
+
+
 class bx_geometry(ctypes.Structure):
     _fields_ = [
         ("domain_lo", ctypes.c_int * 3),
@@ -116,6 +122,8 @@ _SIGS = {
     "bx_helm_set_graph": (ctypes.c_int, [P, ctypes.c_int]),
     "bx_helm_set_fused": (ctypes.c_int, [P, ctypes.c_int]),
     "bx_helm_solve": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, P, P]),
+    "bx_write_plotfile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, DP, DP,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "bx_amr_create": (ctypes.c_int, [P, P, ctypes.POINTER(P)]),
     "bx_amr_destroy": (None, [P]),
     "bx_amr_info": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),

@@ -100,6 +100,20 @@ class AmrHierarchy:
                                  ctypes.byref(n), self.stream), "bx_amr_run")
         return t.value, n.value
 
+    def write_plotfile(self, path, time, dx_coarse, prob_lo, prob_hi, steps=0, derived=False):
+        """Both levels as one AMReX plotfile (AmrLevel::writePlotFile)."""
+        from .plotfile import write_plotfile
+        Uc, Uf = self.get_state()
+        g, gf = self.coarse.geom, self.fine.geom
+        dim = self.dim
+        levels = [dict(layout=self.coarse.layout, U=Uc, domain_lo=list(g.domain_lo)[:dim],
+                       domain_hi=list(g.domain_hi)[:dim], dx=tuple(dx_coarse), steps=steps),
+                  dict(layout=self.fine.layout, U=Uf, domain_lo=list(gf.domain_lo)[:dim],
+                       domain_hi=list(gf.domain_hi)[:dim], dx=tuple(x / 2 for x in dx_coarse),
+                       steps=2 * steps)]
+        p = self.coarse.params
+        write_plotfile(path, levels, prob_lo, prob_hi, time, p.adiabatic, p.epsilon, derived=derived)
+
     def close(self):
         if self.a:
             L.lib().bx_amr_destroy(self.a)

@@ -339,29 +339,28 @@ int AmrLevelAdv::run(double stop_time, int max_step, double &time)
     return steps;
 }
 
-// Restartable raw dump: a text header (dim, ng, ncomp, time, step, boxes
-// with offsets) followed by the packed level as little-endian doubles.
+// AmrLevel::writePlotFile (AmrLevelAdv.cpp:116-136): an AMReX plotfile
+// directory of this level's state, written by the library.
 void AmrLevelAdv::writePlotFile(const std::string &path, double time,
                                 int step) const
 {
     std::vector<double> U;
     download(U);
-    std::ofstream f(path, std::ios::binary);
-    if (!f)
-        throw std::runtime_error("cannot write " + path);
-    f << "BAMREX_AMD_LEVEL 1\n";
-    f << "dim " << m_geom.dim << " ng " << m_layout.ng << " ncomp " << m_layout.nc
-      << "\n";
-    char buf[64];
-    std::snprintf(buf, sizeof buf, "%.17g", time);
-    f << "time " << buf << " step " << step << "\n";
-    f << "nboxes " << m_layout.boxes.size() << " total " << m_layout.total << "\n";
-    for (auto &b : m_layout.boxes)
-        f << b.lo[0] << " " << b.lo[1] << " " << b.lo[2] << " " << b.hi[0] << " "
-          << b.hi[1] << " " << b.hi[2] << " " << b.offset << "\n";
-    f << "data\n";
-    f.write(reinterpret_cast<const char *>(U.data()),
-            (std::streamsize)(sizeof(double) * U.size()));
+    bx_plot_level lv{};
+    lv.boxes = m_layout.boxes.data();
+    lv.nbox  = (int)m_layout.boxes.size();
+    lv.ng    = m_layout.ng;
+    lv.U     = U.data();
+    for (int d = 0; d < 3; ++d)
+    {
+        lv.domain_lo[d] = 0;
+        lv.domain_hi[d] = d < m_geom.dim ? m_geom.n_cell[d] - 1 : 0;
+        lv.dx[d]        = d < m_geom.dim ? m_geom.dx[d] : 0.0;
+    }
+    lv.level_steps = step;
+    check(bx_write_plotfile(path.c_str(), m_geom.dim, 1, &lv, 2, m_geom.prob_lo,
+                            m_geom.prob_hi, time, m_parm.adiabatic, m_parm.epsilon, 0),
+          "bx_write_plotfile");
 }
 
 } // namespace bamrex

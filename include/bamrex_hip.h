@@ -214,6 +214,33 @@ int         bx_copy_kernel(void *dst, const void *src, size_t bytes, void *strea
 const char *bx_last_error(void);
 
 /* ==========================================================================
+ * Plotfiles
+ * ========================================================================== */
+/* One level of a plotfile: a host copy of a packed level (bx_pack_offsets
+ * layout, ng ghost cells per FAB; only valid cells are written). */
+typedef struct
+{
+    const bx_box *boxes;
+    int           nbox;
+    int           ng;
+    const double *U;
+    int           domain_lo[3];
+    int           domain_hi[3];
+    double        dx[3];
+    int           level_steps;
+} bx_plot_level;
+/* AmrLevel::writePlotFile (Source/AmrLevelAdv.cpp:116-136): an AMReX
+ * plotfile directory (HyperCLaw-V1.1: Header, Level_<l>/Cell_H,
+ * Level_<l>/Cell_D_00000) of nlev levels (refinement ratio ref_ratio).
+ * Variables: density, mom_x[, mom_y[, mom_z]], energy
+ * (FluidSystem::VariableNames); derived != 0 adds pressure and u_x[, u_y[,
+ * u_z]] (FluidSystem::SetupDeriveList).  Host code, synchronous. */
+int bx_write_plotfile(const char *dir, int dim, int nlev,
+                      const bx_plot_level *levels, int ref_ratio,
+                      const double prob_lo[3], const double prob_hi[3],
+                      double time, double gamma, double eps, int derived);
+
+/* ==========================================================================
  * Two AMR levels (refinement ratio 2, fine level subcycled)
  * ==========================================================================
  * What AMReX's Amr / FluxRegister / FillPatch machinery does around

@@ -133,3 +133,25 @@ def test_amr_conservation_on_gpu():
     finally:
         h.close()
     assert np.all(np.abs(tot1 - tot0) <= 2e-14 * np.abs(tot0).max())
+
+
+def test_amr_plotfile_after_run(tmp_path):
+    """A two-level plotfile written after a few steps holds both levels'
+    valid cells as the device state has them."""
+    from plotfile_reader import read_plotfile
+    s = amr.refine_setup(quadrant, 32, 16, (8, 8), (23, 23), fine_max_grid_size=8)
+    announce("amr plotfile 32^2")
+    h = amr.AmrHierarchy(gparams(s), s["layout"], s["fine_layout"], s["domain_lo"], s["domain_hi"],
+                         s["bc"], s["bc"])
+    try:
+        h.set_state(s["U"], s["Uf"])
+        t, n = h.run(s["dx"], 0.8, max_step=3)
+        h.write_plotfile(str(tmp_path / "plt00003"), t, s["dx"], (0.0, 0.0), (1.0, 1.0), steps=n)
+        Uc, Uf = h.get_state()
+    finally:
+        h.close()
+    pf = read_plotfile(str(tmp_path / "plt00003"))
+    assert pf["time"] == t and pf["steps"] == [3, 6]
+    for lay, U, lv in ((s["layout"], Uc, pf["levels"][0]), (s["fine_layout"], Uf, pf["levels"][1])):
+        for b, fab in enumerate(lv["fabs"]):
+            assert np.array_equal(fab["data"], lay.valid(U, b))

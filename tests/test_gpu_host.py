@@ -13,6 +13,7 @@ from bamrex_amd import problems
 from bamrex_amd.level import Layout, make_params
 
 from conftest import ROOT, announce
+from plotfile_reader import read_plotfile
 
 pytestmark = pytest.mark.gpu
 
@@ -20,43 +21,37 @@ HOST = os.path.join(ROOT, "bamrex_amd", "bamrex_run")
 INPUTS = os.path.join(ROOT, "bamrex_amd", "inputs")
 
 
-def read_dump(path):
-    """The host's raw level dump: header lines, then the packed level."""
-    with open(path, "rb") as f:
-        assert f.readline().split()[0] == b"BAMREX_AMD_LEVEL"
-        h = f.readline().split()
-        dim, ng, nc = int(h[1]), int(h[3]), int(h[5])
-        h = f.readline().split()
-        t, step = float(h[1]), int(h[3])
-        h = f.readline().split()
-        nb, total = int(h[1]), int(h[3])
-        boxes, offs = [], []
-        for _ in range(nb):
-            v = [int(x) for x in f.readline().split()]
-            boxes.append((tuple(v[0:dim]), tuple(v[3:3 + dim])))
-            offs.append(v[6])
-        assert f.readline().strip() == b"data"
-        U = np.frombuffer(f.read(), dtype=np.float64)
-    assert U.size == total
-    return dict(dim=dim, ng=ng, nc=nc, time=t, step=step, boxes=boxes, offsets=offs, U=U.copy())
+def read_final_plotfile(tmp_path):
+    """The last plotfile the host wrote (plt<step>), as the packed level."""
+    names = sorted(p.name for p in tmp_path.iterdir() if p.name.startswith("plt"))
+    pf = read_plotfile(str(tmp_path / names[-1]))
+    dim = pf["dim"]
+    fabs = pf["levels"][0]["fabs"]
+    boxes = [(f["lo"], f["hi"]) for f in fabs]
+    lay = Layout(dim, boxes, 2)
+    U = lay.new_array()
+    for b, f in enumerate(fabs):
+        lay.valid(U, b)[...] = f["data"][:dim + 2]
+    return dict(dim=dim, time=pf["time"], step=pf["steps"][0], boxes=boxes, layout=lay, U=U,
+                names=names)
 
 
-def run_host(tmp_path, inputs, *overrides):
-    plot = str(tmp_path / "level.dump")
-    cmd = [HOST, os.path.join(INPUTS, inputs)] + list(overrides) + ["amr.plot_file=" + plot]
+def run_host(tmp_path, inputs, *overrides, plot_int=1000):
+    cmd = [HOST, os.path.join(INPUTS, inputs)] + list(overrides) + [
+        "amr.plot_file=" + str(tmp_path / "plt"), "amr.plot_int=%d" % plot_int]
     announce("host " + " ".join(cmd[1:]))
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    return read_dump(plot)
+    return read_final_plotfile(tmp_path)
 
 
 @pytest.mark.skipif(not os.path.exists(HOST), reason="C++ host not built")
 @pytest.mark.parametrize("box_loop", [0, 1])
 def test_host_quadrant_matches_oracle(tmp_path, oracle_bxpow, box_loop):
     d = run_host(tmp_path, "quadrant-4096", "amr.n_cell=128 128", "amr.max_grid_size=32",
-                 "max_step=8", "adv.box_loop=%d" % box_loop)
-    lay = Layout(2, d["boxes"], d["ng"])
-    assert list(lay.offsets) == d["offsets"]
+                 "max_step=8", "adv.box_loop=%d" % box_loop, plot_int=4)
+    assert d["names"] == ["plt00000", "plt00004", "plt00008"]
+    lay = d["layout"]
     s = problems.quadrant_setup(ncell=128, max_grid_size=32)
     U = lay.new_array()
     problems.quadrant_2d(lay, U, (0.0, 0.0), s["dx"], problems.QUADRANT_CONFIG, 1.4)
@@ -71,7 +66,7 @@ def test_host_quadrant_matches_oracle(tmp_path, oracle_bxpow, box_loop):
 @pytest.mark.skipif(not os.path.exists(HOST), reason="C++ host not built")
 def test_host_sod_matches_oracle(tmp_path, oracle_bxpow):
     d = run_host(tmp_path, "sod-1024", "stop_time=0.05", "max_step=-1")
-    lay = Layout(1, d["boxes"], d["ng"])
+    lay = d["layout"]
     s = problems.sod_setup(ncell=1024)
     U = lay.new_array()
     problems.riemann_1d(lay, U, (0.0,), (1.0,), s["dx"], (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 1.4)

@@ -105,17 +105,27 @@ __device__ __forceinline__ void wave_item_range(long long nitems,
         last = nitems;
 }
 
-__device__ __forceinline__ void wave_max_to_global(unsigned long long v,
-                                                   unsigned long long *dst)
+// Block maximum, then one atomicMax per block (many waves hitting one
+// address serialise in L2).  Every thread of the block must call it.
+__device__ __forceinline__ void block_max_to_global(unsigned long long v,
+                                                    unsigned long long *dst)
 {
+    __shared__ unsigned long long red[16];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
+        v = umax64(v, __shfl_xor(v, off));
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
     {
-        const unsigned long long o = __shfl_xor(v, off);
-        v                          = umax64(v, o);
+        unsigned long long m = 0ULL;
+        for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i)
+            m = umax64(m, red[i]);
+        if (m != 0ULL)
+            atomicMax(dst, m);
     }
-    if ((threadIdx.x & 63) == 0 && v != 0ULL)
-        atomicMax(dst, v);
 }
 
 // Whole-wave lane shifts through DPP (wave_shr:1 / wave_shl:1, a GFX9
@@ -281,8 +291,8 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
             }
         }
     }
-    if (d_maxbits)
-        wave_max_to_global(vmax, d_maxbits);
+    if (d_maxbits) // uniform over the block; every thread gets here
+        block_max_to_global(vmax, d_maxbits);
 }
 
 // -------------------------------------------------------------- y/z sweep
@@ -424,8 +434,8 @@ __global__ __launch_bounds__(256) BX_T_OCC void k_sweep_t(
                     Up[n] = Unext[n];
         }
     }
-    if (d_maxbits)
-        wave_max_to_global(vmax, d_maxbits);
+    if (d_maxbits) // uniform over the block; every thread gets here
+        block_max_to_global(vmax, d_maxbits);
 }
 
 #ifndef BX_SWEEP_PART // kernels outside the per-dimension sweep instantiations
@@ -535,7 +545,7 @@ __global__ __launch_bounds__(256) void k_max_wave_speed(
             vmax = speed_max(vmax, cell_wave_speed<D>(V, pm));
         }
     }
-    wave_max_to_global(vmax, d_maxbits);
+    block_max_to_global(vmax, d_maxbits);
 }
 
 // ----------------------------------------------------------- dt control

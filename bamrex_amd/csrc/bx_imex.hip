@@ -250,14 +250,25 @@ __global__ __launch_bounds__(256) void k_peek_speed(const double *__restrict__ U
         const unsigned long long b = (unsigned long long)__double_as_longlong(s);
         m = b > m ? b : m;
     }
+    // block maximum first: one atomic per block (per-wave atomics on one
+    // address serialise in L2)
+    __shared__ unsigned long long red[4];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
     {
         const unsigned long long o = __shfl_xor(m, off);
         m                          = o > m ? o : m;
     }
-    if ((threadIdx.x & 63) == 0 && m != 0ULL)
-        atomicMax(out, m);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
+            m = red[i] > m ? red[i] : m;
+        if (m != 0ULL)
+            atomicMax(out, m);
+    }
 }
 
 __global__ void k_peek_zero_u64(unsigned long long *p) { *p = 0ULL; }
@@ -317,7 +328,7 @@ extern "C" int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, 
     }
     hipLaunchKernelGGL(k_peek_zero_u64, dim3(1), dim3(1), 0, s, d);
     const long long n    = (long long)nx * ny;
-    const int       grid = (int)std::min<long long>((n + 255) / 256, 2048);
+    const int       grid = (int)std::min<long long>((n + 255) / 256, 1024);
     hipLaunchKernelGGL(k_peek_speed, dim3(grid), dim3(256), 0, s, U, n, dx, dy, d);
     PEEK_CHECK(hipGetLastError());
     unsigned long long bits = 0;

@@ -557,6 +557,38 @@ static int amr_plan(bx_amr *a)
         a->cf.swap(merged);
     }
 
+    // split large rectangles along their longest direction so that a
+    // block (one segment) has at most CF_MAX_CELLS cells
+    {
+        const long long    CF_MAX_CELLS = 512;
+        std::vector<CFSeg> split;
+        for (const CFSeg &g : a->cf)
+        {
+            long long n = 1;
+            int       e_long = 0, len_long = 0;
+            for (int e = 0; e < 3; ++e)
+            {
+                const int len = g.hi[e] - g.lo[e] + 1;
+                n *= len;
+                if (len > len_long)
+                {
+                    len_long = len;
+                    e_long   = e;
+                }
+            }
+            const long long per_unit = n / len_long; // cells per unit of the long side
+            const int       step     = (int)std::max<long long>(1, CF_MAX_CELLS / per_unit);
+            for (int a0 = g.lo[e_long]; a0 <= g.hi[e_long]; a0 += step)
+            {
+                CFSeg h       = g;
+                h.lo[e_long]  = a0;
+                h.hi[e_long]  = std::min(g.hi[e_long], a0 + step - 1);
+                split.push_back(h);
+            }
+        }
+        a->cf.swap(split);
+    }
+
     // ---- flux registers: per direction, per fine box side, per coarse
     // face of that side whose outside neighbour is not fine
     a->cface.clear();
@@ -820,10 +852,10 @@ static int cf_fill(bx_amr *a, double *Uf, const double *Uold, const double *Unew
     const int nseg = (int)a->cf.size();
     if (nseg == 0)
         return BX_OK;
-    const int grid = std::min(nseg, 4096);
+    const int grid = std::min(nseg, 8192);
     bx_ctx   *C = a->crse, *F = a->fine;
 #define BX_CF(DD)                                                                          \
-    hipLaunchKernelGGL((k_cf_fill<DD>), dim3(grid), dim3(64), 0, s, Uf, F->d_boxes, Uold,  \
+    hipLaunchKernelGGL((k_cf_fill<DD>), dim3(grid), dim3(256), 0, s, Uf, F->d_boxes, Uold, \
                        Unew, C->d_boxes, a->d_cf, nseg, F->geom.ng, C->geom.ng, mode, alpha, \
                        beta)
     if (a->D == 1)
@@ -896,14 +928,28 @@ extern "C" int bx_amr_fill_fine(bx_amr *a, double *Uf, const double *Uc_old,
     return cf_fill(a, Uf, Uc_old, Uc_new, 1, alpha, beta, s);
 }
 
+static int amr_advance(bx_amr *a, double dt, const double dx_coarse[3], double time, double *Uc,
+                       double *Uc_tmp, double *Uc_old, double *Uf, double *Uf_tmp,
+                       hipStream_t s, bool fill_fine_after);
+
 extern "C" int bx_amr_advance(bx_amr *a, double dt, const double dx_coarse[3], double time,
                               double *Uc, double *Uc_tmp, double *Uc_old, double *Uf,
                               double *Uf_tmp, void *stream)
 {
+    return amr_advance(a, dt, dx_coarse, time, Uc, Uc_tmp, Uc_old, Uf, Uf_tmp,
+                       (hipStream_t)stream, true);
+}
+
+// fill_fine_after = false leaves the fine ghost cells as the last sweep left
+// them (the next step's fine fill rewrites all of them).
+static int amr_advance(bx_amr *a, double dt, const double dx_coarse[3], double time, double *Uc,
+                       double *Uc_tmp, double *Uc_old, double *Uf, double *Uf_tmp,
+                       hipStream_t s, bool fill_fine_after)
+{
+    void *stream = (void *)s;
     if (!a || !Uc || !Uc_tmp || !Uc_old || !Uf || !Uf_tmp || !amr_dx_ok(a, dx_coarse)
         || !(dt > 0))
         return set_error(BX_ERR_INVALID, "bx_amr_advance: bad argument");
-    hipStream_t s  = (hipStream_t)stream;
     bx_ctx     *C  = a->crse, *F = a->fine;
     const int   D  = a->D;
     double      dxf[3] = { 0, 0, 0 };
@@ -963,7 +1009,7 @@ extern "C" int bx_amr_advance(bx_amr *a, double dt, const double dx_coarse[3], d
         return rc;
     if ((rc = do_fill_all(C, Uc, s)) != BX_OK)
         return rc;
-    return do_fill_all(F, Uf, s);
+    return fill_fine_after ? do_fill_all(F, Uf, s) : BX_OK;
 }
 
 // estTimeStep of one level (AmrLevelAdv.cpp:395-477, explicit methods) from
@@ -1039,12 +1085,14 @@ extern "C" int bx_amr_run(bx_amr *a, const double dx_coarse[3], double cfl, doub
         double dt;
         if ((rc = bx_amr_est_dt(a, dx_coarse, cfl, t, stop_time, Uc, Uf, &dt, stream)) != BX_OK)
             return rc;
-        if ((rc = bx_amr_advance(a, dt, dx_coarse, t, Uc, Uc_tmp, Uc_old, Uf, Uf_tmp, stream))
+        if ((rc = amr_advance(a, dt, dx_coarse, t, Uc, Uc_tmp, Uc_old, Uf, Uf_tmp, s, false))
             != BX_OK)
             return rc;
         t = t + dt;
         ++n;
     }
+    if ((rc = do_fill_all(a->fine, Uf, s)) != BX_OK)
+        return rc;
     BX_HIP_CHECK(hipStreamSynchronize(s));
     *time = t;
     if (steps)

@@ -142,10 +142,11 @@ _SIGS = {
     "bx_imex_advective_dt": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.POINTER(ctypes.c_double), P]),
     "bx_imex_peek_assemble": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P, P, P, P, P,
-                                             P]),
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                             P, P, P, P, P, P, P]),
     "bx_imex_peek_finish": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, P, P]),
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           P, P]),
     "bx_cnn_create": (ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(P)]),
     "bx_cnn_destroy": (None, [P]),
     "bx_cnn_set_params": (ctypes.c_int, [P, P, ctypes.c_int]),
@@ -162,6 +163,7 @@ _SIGS = {
 }
 
 BX_SOLVE = {"mg": 0, "fgmres_mg": 1, "fgmres_cnn": 2, "mg_fused": 3}
+BX_IMEX_ADVECTION = {"rusanov": 0, "muscl_rusanov": 1}
 BX_CNN_MODE = {"smoother": 0, "corrector": 1}
 BX_CNN_NPARAMS = 4896
 

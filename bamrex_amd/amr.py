@@ -46,9 +46,13 @@ class AmrHierarchy:
                           [2 * x + 1 for x in domain_hi], bc_lo, bc_hi)
         self.Uc_old = DeviceBuffer(coarse_layout.total * 8)
         self.a = ctypes.c_void_p()
-        check(L.lib().bx_amr_create(self.coarse.ctx, self.fine.ctx, ctypes.byref(self.a)),
-              "bx_amr_create")
         self.stream = self.coarse.stream
+        try:
+            check(L.lib().bx_amr_create(self.coarse.ctx, self.fine.ctx, ctypes.byref(self.a)),
+                  "bx_amr_create")
+        except Exception:
+            self.close()
+            raise
 
     def info(self):
         n = [ctypes.c_int() for _ in range(3)]

@@ -450,6 +450,15 @@ static int amr_plan(bx_amr *a)
                                  "bx_amr_create: fine boxes must be ratio-2 aligned and inside the domain");
     if (gc.ng < 2 || gf.ng < 2)
         return set_error(BX_ERR_INVALID, "bx_amr_create: both levels need 2 ghost cells");
+    for (size_t i = 0; i < FB.size(); ++i)
+        for (size_t j = i + 1; j < FB.size(); ++j)
+        {
+            bool overlap = true;
+            for (int d = 0; d < D; ++d)
+                overlap = overlap && FB[i].lo[d] <= FB[j].hi[d] && FB[j].lo[d] <= FB[i].hi[d];
+            if (overlap)
+                return set_error(BX_ERR_INVALID, "bx_amr_create: fine boxes overlap");
+        }
 
     // ---- coarse-fine ghost cells: face ghost slabs of each fine box, cell
     // by cell, those whose (periodically wrapped) image is inside the domain

@@ -5,6 +5,8 @@
 // code around MLMG (CPU-only update kernels):
 //   compute_flux_imex_o1            Source/IMEX/IMEX_O1.cpp:18-175
 //   advance_rusanov_adv             Source/IMEX/IMEX_Fluxes.cpp:10-118
+//   advance_MUSCL_rusanov_adv (SP111 local step)
+//                                   Source/IMEX/IMEX_Fluxes.cpp:120-324
 //   adv_flux / kinetic_energy / pressure
 //                                   Source/IMEX_K/euler_K.H:8-65
 //   compute_enthalpy (reuse_pressure)
@@ -59,12 +61,8 @@ template <int DIR>
 struct AdvCell
 {
     double U[4], F[4], au;
-    __device__ __forceinline__ void load(const double *__restrict__ S, long long c,
-                                         long long n, double eps)
+    __device__ __forceinline__ void flux(double eps)
     {
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            U[k] = S[c + k * n];
         const double u = U[1 + DIR] / U[0];
         F[0]           = U[1 + DIR];
         F[1]           = U[1] * u;
@@ -72,7 +70,50 @@ struct AdvCell
         F[3]           = u * peek_ke(U[0], U[1], U[2], eps);
         au             = fabs(u);
     }
+    __device__ __forceinline__ void load(const double *__restrict__ S, long long c,
+                                         long long n, double eps)
+    {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            U[k] = S[c + k * n];
+        flux(eps);
+    }
 };
+
+// minmod (Source/Fluxes/Limiters.H:75-81)
+__device__ __forceinline__ double peek_minmod(double dl, double dr)
+{
+    return (dl > 0 && dr > 0) ? fmin(dl, dr) : ((dl < 0 && dr < 0) ? fmax(dl, dr) : 0.0);
+}
+
+// MUSCL states of cell c in direction DIR (advance_MUSCL_rusanov_adv,
+// IMEX_Fluxes.cpp:183-250): minmod half-slopes, then the SP111 local half
+// step with the states' own advective fluxes; L and R hold the updated
+// states with their fluxes.
+template <int DIR>
+__device__ __forceinline__ void muscl_states(const AdvCell<DIR> &m, const AdvCell<DIR> &c,
+                                             const AdvCell<DIR> &p, double hdtdx, double eps,
+                                             AdvCell<DIR> &L, AdvCell<DIR> &R)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const double h = 0.5 * peek_minmod(c.U[k] - m.U[k], p.U[k] - c.U[k]);
+        L.U[k]         = c.U[k] - h;
+        R.U[k]         = c.U[k] + h;
+    }
+    L.flux(eps);
+    R.flux(eps);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const double lu = hdtdx * (R.F[k] - L.F[k]);
+        L.U[k]          = L.U[k] - lu;
+        R.U[k]          = R.U[k] - lu;
+    }
+    L.flux(eps);
+    R.flux(eps);
+}
 
 // Rusanov face flux between cell m (lower) and c (upper)
 // (IMEX_Fluxes.cpp:70-90): 0.5 ((F_c + F_m) - max(|u_c|,|u_m|) (U_c - U_m))
@@ -87,30 +128,62 @@ __device__ __forceinline__ void rusanov(const AdvCell<DIR> &m, const AdvCell<DIR
         F[k] = 0.5 * ((c.F[k] + m.F[k]) - umax * (c.U[k] - m.U[k]));
 }
 
-// Rusanov fluxes at the four faces of cell (i, j).
+// Rusanov fluxes at the two faces of cell x along direction DIR, from the
+// cells at x-2 .. x+2 (offsets in elements of the periodic neighbours).
+// First order (advance_rusanov_adv) or MUSCL (advance_MUSCL_rusanov_adv):
+// the face between cells m and c takes R of m and L of c.
+template <int DIR, bool MUSCL>
+__device__ __forceinline__ void dir_faces(const double *__restrict__ U, long long n,
+                                          long long cm2, long long cm1, long long c0,
+                                          long long cp1, long long cp2, double eps,
+                                          double hdtdx, double (&Fl)[4], double (&Fr)[4])
+{
+    AdvCell<DIR> a1, a2, a3;
+    a1.load(U, cm1, n, eps);
+    a2.load(U, c0, n, eps);
+    a3.load(U, cp1, n, eps);
+    if (!MUSCL)
+    {
+        rusanov(a1, a2, Fl);
+        rusanov(a2, a3, Fr);
+        return;
+    }
+    AdvCell<DIR> a0, a4;
+    a0.load(U, cm2, n, eps);
+    a4.load(U, cp2, n, eps);
+    AdvCell<DIR> L1, R1, L2, R2, L3, R3;
+    muscl_states(a0, a1, a2, hdtdx, eps, L1, R1);
+    muscl_states(a1, a2, a3, hdtdx, eps, L2, R2);
+    muscl_states(a2, a3, a4, hdtdx, eps, L3, R3);
+    rusanov(R1, L2, Fl);
+    rusanov(R2, L3, Fr);
+}
+
+// Rusanov fluxes at the four faces of cell (i, j); hdtdx/hdtdy: half of
+// dt/dx, dt/dy (the MUSCL local step).
+template <bool MUSCL>
 __device__ __forceinline__ void cell_faces(const double *__restrict__ U, const PeekGeom &g,
-                                           int i, int j, double eps, double (&Fxl)[4],
-                                           double (&Fxr)[4], double (&Fyl)[4],
-                                           double (&Fyr)[4])
+                                           int i, int j, double eps, double hdtdx,
+                                           double hdtdy, double (&Fxl)[4], double (&Fxr)[4],
+                                           double (&Fyl)[4], double (&Fyr)[4])
 {
     const long long row = (long long)j * g.nx;
-    const long long c   = row + i;
-    AdvCell<0>      xl, xc, xr;
-    xl.load(U, row + wrap_lo(i, g.nx), g.n, eps);
-    xc.load(U, c, g.n, eps);
-    xr.load(U, row + wrap_hi(i, g.nx), g.n, eps);
-    rusanov(xl, xc, Fxl);
-    rusanov(xc, xr, Fxr);
-    AdvCell<1> yl, yc, yr;
-    yl.load(U, (long long)wrap_lo(j, g.ny) * g.nx + i, g.n, eps);
-    yc.load(U, c, g.n, eps);
-    yr.load(U, (long long)wrap_hi(j, g.ny) * g.nx + i, g.n, eps);
-    rusanov(yl, yc, Fyl);
-    rusanov(yc, yr, Fyr);
+    const int       im1 = wrap_lo(i, g.nx), ip1 = wrap_hi(i, g.nx);
+    const int       jm1 = wrap_lo(j, g.ny), jp1 = wrap_hi(j, g.ny);
+    const int       im2 = MUSCL ? wrap_lo(im1, g.nx) : im1;
+    const int       ip2 = MUSCL ? wrap_hi(ip1, g.nx) : ip1;
+    const int       jm2 = MUSCL ? wrap_lo(jm1, g.ny) : jm1;
+    const int       jp2 = MUSCL ? wrap_hi(jp1, g.ny) : jp1;
+    dir_faces<0, MUSCL>(U, g.n, row + im2, row + im1, row + i, row + ip1, row + ip2, eps, hdtdx,
+                        Fxl, Fxr);
+    const long long nx = g.nx;
+    dir_faces<1, MUSCL>(U, g.n, jm2 * nx + i, jm1 * nx + i, row + i, jp1 * nx + i, jp2 * nx + i,
+                        eps, hdtdy, Fyl, Fyr);
 }
 
 // Explicit advective update (IMEX_Fluxes.cpp:92-111):
 //   Uex = U - (dtdx (Fx(i+1) - Fx(i)) + dtdy (Fy(j+1) - Fy(j)))
+template <bool MUSCL>
 __global__ __launch_bounds__(256) void k_peek_advect(const double *__restrict__ U,
                                                      double *__restrict__ Uex, PeekGeom g,
                                                      double eps, double dtdx, double dtdy)
@@ -120,7 +193,7 @@ __global__ __launch_bounds__(256) void k_peek_advect(const double *__restrict__ 
     if (i >= g.nx || j >= g.ny)
         return;
     double Fxl[4], Fxr[4], Fyl[4], Fyr[4];
-    cell_faces(U, g, i, j, eps, Fxl, Fxr, Fyl, Fyr);
+    cell_faces<MUSCL>(U, g, i, j, eps, 0.5 * dtdx, 0.5 * dtdy, Fxl, Fxr, Fyl, Fyr);
     const long long c = (long long)j * g.nx + i;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -197,6 +270,7 @@ __device__ __forceinline__ void peek_correct(double (&F)[4], const double *__res
 
 // U_new = U - (dtdx (Fx(i+1) - Fx(i)) + dtdy (Fy(j+1) - Fy(j)))
 // (conservative_update with the SP111 weight b = 1, IMEX_RK.cpp:22-96)
+template <bool MUSCL>
 __global__ __launch_bounds__(256) void k_peek_finish(
     const double *__restrict__ U, const double *__restrict__ Uex,
     const double *__restrict__ enth, const double *__restrict__ p, double *__restrict__ Unew,
@@ -208,7 +282,7 @@ __global__ __launch_bounds__(256) void k_peek_finish(
     if (i >= g.nx || j >= g.ny)
         return;
     double Fxl[4], Fxr[4], Fyl[4], Fyr[4];
-    cell_faces(U, g, i, j, eps, Fxl, Fxr, Fyl, Fyr);
+    cell_faces<MUSCL>(U, g, i, j, eps, 0.5 * dtdx, 0.5 * dtdy, Fxl, Fxr, Fyl, Fyr);
     const long long row = (long long)j * g.nx;
     const long long c   = row + i;
     const long long cw  = row + wrap_lo(i, g.nx);
@@ -346,12 +420,24 @@ extern "C" int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, 
 
 extern "C" int bx_imex_peek_assemble(const double *U, const double *p, int nx, int ny,
                                      double dx, double dy, double dt, double gamma,
-                                     double eps, double *Uex, double *enth, double *rhs,
-                                     double *a, double *bX, double *bY, void *stream)
+                                     double eps, int advection_flux, double *Uex, double *enth,
+                                     double *rhs, double *a, double *bX, double *bY,
+                                     void *stream)
 {
     if (!U || !p || !Uex || !enth || !rhs || !a || !bX || !bY || !peek_sizes_ok(nx, ny)
-        || !(dx > 0) || !(dy > 0) || U == Uex)
+        || !(dx > 0) || !(dy > 0)
+        || (advection_flux != BX_IMEX_RUSANOV && advection_flux != BX_IMEX_MUSCL_RUSANOV))
         return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_assemble: bad argument");
+    // outputs are written while inputs and other outputs are read
+    const void *outs[6] = { Uex, enth, rhs, a, bX, bY };
+    for (int i = 0; i < 6; ++i)
+    {
+        if (outs[i] == (const void *)U || outs[i] == (const void *)p)
+            return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_assemble: output aliases an input");
+        for (int j = i + 1; j < 6; ++j)
+            if (outs[i] == outs[j])
+                return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_assemble: outputs alias");
+    }
     hipStream_t s = (hipStream_t)stream;
     PeekGeom    g{ nx, ny, (long long)nx * ny };
     // dtdx = dt / dx (IMEX_Fluxes.cpp:93); hdtdx = 0.5 dt idx (IMEX_O1.cpp:66)
@@ -359,8 +445,12 @@ extern "C" int bx_imex_peek_assemble(const double *U, const double *p, int nx, i
     const double dtdy  = dt / dy;
     const double hdtdx = 0.5 * dt * (1.0 / dx);
     const double hdtdy = 0.5 * dt * (1.0 / dy);
-    hipLaunchKernelGGL(k_peek_advect, peek_grid(nx, ny), dim3(64, 4), 0, s, U, Uex, g, eps,
-                       dtdx, dtdy);
+    if (advection_flux == BX_IMEX_MUSCL_RUSANOV)
+        hipLaunchKernelGGL(k_peek_advect<true>, peek_grid(nx, ny), dim3(64, 4), 0, s, U, Uex, g,
+                           eps, dtdx, dtdy);
+    else
+        hipLaunchKernelGGL(k_peek_advect<false>, peek_grid(nx, ny), dim3(64, 4), 0, s, U, Uex, g,
+                           eps, dtdx, dtdy);
     PEEK_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_peek_system, peek_grid(nx, ny), dim3(64, 4), 0, s, Uex, p, enth, rhs,
                        a, bX, bY, g, gamma, eps, hdtdx, hdtdy);
@@ -370,10 +460,14 @@ extern "C" int bx_imex_peek_assemble(const double *U, const double *p, int nx, i
 
 extern "C" int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
                                    const double *p_new, int nx, int ny, double dx, double dy,
-                                   double dt, double eps, double *U_new, void *stream)
+                                   double dt, double eps, int advection_flux, double *U_new,
+                                   void *stream)
 {
+    if (advection_flux != BX_IMEX_RUSANOV && advection_flux != BX_IMEX_MUSCL_RUSANOV)
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_finish: bad advection_flux");
     if (!U || !Uex || !enth || !p_new || !U_new || !peek_sizes_ok(nx, ny) || !(dx > 0)
-        || !(dy > 0) || U == U_new || Uex == U_new)
+        || !(dy > 0) || U == U_new || Uex == U_new || (const double *)U_new == enth
+        || (const double *)U_new == p_new)
         return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_finish: bad argument");
     PeekGeom     g{ nx, ny, (long long)nx * ny };
     const double dtdx   = dt / dx;
@@ -381,8 +475,14 @@ extern "C" int bx_imex_peek_finish(const double *U, const double *Uex, const dou
     const double heps   = 0.5 / eps;                   // IMEX_O1.cpp:499
     const double hdtdxe = 0.5 * dt * (1.0 / dx) / eps; // IMEX_O1.cpp:66-70
     const double hdtdye = 0.5 * dt * (1.0 / dy) / eps;
-    hipLaunchKernelGGL(k_peek_finish, peek_grid(nx, ny), dim3(64, 4), 0, (hipStream_t)stream,
-                       U, Uex, enth, p_new, U_new, g, eps, dtdx, dtdy, heps, hdtdxe, hdtdye);
+    if (advection_flux == BX_IMEX_MUSCL_RUSANOV)
+        hipLaunchKernelGGL(k_peek_finish<true>, peek_grid(nx, ny), dim3(64, 4), 0,
+                           (hipStream_t)stream, U, Uex, enth, p_new, U_new, g, eps, dtdx, dtdy,
+                           heps, hdtdxe, hdtdye);
+    else
+        hipLaunchKernelGGL(k_peek_finish<false>, peek_grid(nx, ny), dim3(64, 4), 0,
+                           (hipStream_t)stream, U, Uex, enth, p_new, U_new, g, eps, dtdx, dtdy,
+                           heps, hdtdxe, hdtdye);
     PEEK_CHECK(hipGetLastError());
     return BX_OK;
 }

@@ -119,11 +119,13 @@ class PeekStage:
     """Explicit work of one PEEK-SP111 step on an nx x ny periodic level
     (bx_imex_*): owns the stage's device arrays Uex, enth, rhs, a, bX, bY."""
 
-    def __init__(self, nx, ny, dx, dy=None, gamma=1.4, eps=1.0):
+    def __init__(self, nx, ny, dx, dy=None, gamma=1.4, eps=1.0, advection="rusanov"):
         self.nx, self.ny = nx, ny
         self.dx = dx
         self.dy = dx if dy is None else dy
         self.gamma, self.eps = gamma, eps
+        # imex.advection_flux (IMEXSettings.cpp:19-29)
+        self.adv = _lib.BX_IMEX_ADVECTION[advection]
         self.Uex = DeviceArray((4, ny, nx))
         self.enth = DeviceArray((ny, nx))
         self.rhs = DeviceArray((ny, nx))
@@ -147,7 +149,7 @@ class PeekStage:
         previous pressure p; returns the system (device arrays and the
         operator scalars alpha = 1, beta = dt^2)."""
         _chk(lib().bx_imex_peek_assemble(U.ptr, p.ptr, self.nx, self.ny, self.dx, self.dy, dt,
-                                         self.gamma, self.eps, self.Uex.ptr, self.enth.ptr,
+                                         self.gamma, self.eps, self.adv, self.Uex.ptr, self.enth.ptr,
                                          self.rhs.ptr, self.a.ptr, self.bX.ptr, self.bY.ptr, None),
              "bx_imex_peek_assemble")
         return dict(Uex=self.Uex, enth=self.enth, rhs=self.rhs, a=self.a, bX=self.bX, bY=self.bY,
@@ -157,7 +159,8 @@ class PeekStage:
         """Flux corrections with the solved pressure and the conservative
         update into U_out (not U)."""
         _chk(lib().bx_imex_peek_finish(U.ptr, self.Uex.ptr, self.enth.ptr, p_new.ptr, self.nx,
-                                       self.ny, self.dx, self.dy, dt, self.eps, U_out.ptr, None),
+                                       self.ny, self.dx, self.dy, dt, self.eps, self.adv, U_out.ptr,
+                                       None),
              "bx_imex_peek_finish")
 
     def free(self):
@@ -353,13 +356,13 @@ class GreshoWorkload:
     the pressure passed to advance() (the benchmark passes the plain
     multigrid solution, so every method solves the same sequence)."""
 
-    def __init__(self, n=1024, mach=0.01, gamma=1.4, eps=1.0, cfl=0.45):
+    def __init__(self, n=1024, mach=0.01, gamma=1.4, eps=1.0, cfl=0.45, advection="rusanov"):
         U, self.dx = gresho_state(n, mach, gamma, eps)
         self.n = n
         self.U = dev(U)
         self.U_next = DeviceArray(U.shape)
         self.p = DeviceArray((n, n))
-        self.stage = PeekStage(n, n, self.dx, gamma=gamma, eps=eps)
+        self.stage = PeekStage(n, n, self.dx, gamma=gamma, eps=eps, advection=advection)
         self.stage.pressure(self.U, self.p)
         self.cfl = cfl
 

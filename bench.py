@@ -50,6 +50,7 @@ def main():
     ap.add_argument("--no-imex", action="store_true", help="skip the IMEX pressure-solve leg")
     ap.add_argument("--imex-only", action="store_true", help="run only the IMEX pressure-solve leg")
     ap.add_argument("--imex-n", type=int, default=1024)
+    ap.add_argument("--imex-advection", default="muscl_rusanov", choices=["rusanov", "muscl_rusanov"])
     ap.add_argument("--imex-harvest-steps", type=int, default=8)
     ap.add_argument("--imex-steps", type=int, default=6)
     ap.add_argument("--imex-train-steps", type=int, default=640)
@@ -401,7 +402,7 @@ def imex_leg(args):
     from bamrex_amd import imex
     n = args.imex_n
     log("imex: Gresho %d^2, harvesting %d steps" % (n, args.imex_harvest_steps))
-    wl = imex.GreshoWorkload(n)
+    wl = imex.GreshoWorkload(n, advection=args.imex_advection)
     solver = imex.HelmholtzSolver(n, n, wl.dx)
     nf_max = 8 * args.imex_harvest_steps
     R = imex.DeviceArray((nf_max, n, n))
@@ -484,7 +485,8 @@ def imex_leg(args):
         log("imex step %d: " % k + "  ".join(
             "%s %.3f ms/%d it" % (m[0], res[m[0]]["seconds"][-1] * 1e3, res[m[0]]["iters"][-1])
             for m in methods))
-    out = {"workload": "gresho-peek-sp111-%dx%d-M0.01-periodic" % (n, n), "rtol": 1e-10,
+    out = {"workload": "gresho-peek-sp111-%s-%dx%d-M0.01-periodic" % (args.imex_advection, n, n),
+           "rtol": 1e-10,
            "harvest_fields": nf, "train_steps": args.imex_train_steps, "train_batch": args.imex_batch,
            "train_crop": args.imex_crop, "train_crops_total": args.imex_train_steps * args.imex_batch,
            "train_seconds": round(train_s, 3),

@@ -394,16 +394,21 @@ int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, double dy,
                          double cfl, double *dt, void *stream);
 /* First half of compute_flux_imex_o1 (Source/IMEX/IMEX_O1.cpp:18-131):
  * Rusanov advective update U -> Uex (advance_rusanov_adv,
- * Source/IMEX/IMEX_Fluxes.cpp:10-118), enthalpy p gamma/((gamma-1) rho_ex)
+ * Source/IMEX/IMEX_Fluxes.cpp:10-118, or its MUSCL form), enthalpy p gamma/((gamma-1) rho_ex)
  * from the previous pressure p (compute_enthalpy, IMEX_O1.cpp:177-228), the
  * pressure equation's source rhs (pressure_source_vector, :416-452) and its
  * coefficients a = eps/(gamma-1), b = face-averaged enthalpy
  * (setup_pressure_op, :570-614).  The operator's scalars are alpha = 1,
  * beta = dt*dt. */
+#define BX_IMEX_RUSANOV 0       /* imex.advection_flux = rusanov (1st order) */
+#define BX_IMEX_MUSCL_RUSANOV 1 /* = muscl_rusanov (advance_MUSCL_rusanov_adv,
+                                   Source/IMEX/IMEX_Fluxes.cpp:120-324, with
+                                   the SP111 local half step) */
 int bx_imex_peek_assemble(const double *U, const double *p, int nx, int ny,
                           double dx, double dy, double dt, double gamma,
-                          double eps, double *Uex, double *enth, double *rhs,
-                          double *a, double *bX, double *bY, void *stream);
+                          double eps, int advection_flux, double *Uex,
+                          double *enth, double *rhs, double *a, double *bX,
+                          double *bY, void *stream);
 /* Second half (IMEX_O1.cpp:132-175 with update_momentum_flux /
  * update_energy_flux, :480-566, and conservative_update, Source/IMEX/
  * IMEX_RK.cpp:22-96): the Rusanov fluxes of U corrected with the solved
@@ -411,7 +416,8 @@ int bx_imex_peek_assemble(const double *U, const double *p, int nx, int ny,
  * Uex. */
 int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
                         const double *p_new, int nx, int ny, double dx, double dy,
-                        double dt, double eps, double *U_new, void *stream);
+                        double dt, double eps, int advection_flux, double *U_new,
+                        void *stream);
 
 /* Learned smoother: bias-free 3x3 convolutions 1->16->16->16->1 with ReLU,
  * BF16 weights/activations, FP32 accumulation (MFMA for the 16->16 layers).

@@ -12,6 +12,7 @@ Settings restated: linearised PEEK, enthalpy from the previous pressure
   max_dx_scaled_speed                    Source/System/Euler/Euler.cpp:107-129
   estTimeStep (advective, cfl)           Source/AmrLevelAdv.cpp:431-436, 466
   advance_rusanov_adv                    Source/IMEX/IMEX_Fluxes.cpp:10-118
+  advance_MUSCL_rusanov_adv (SP111)      Source/IMEX/IMEX_Fluxes.cpp:120-324
   hdtdx = 0.5 dt idx, hdtdxe = hdtdx/eps Source/IMEX/IMEX_O1.cpp:66-70
   compute_enthalpy (reuse_pressure)      Source/IMEX/IMEX_O1.cpp:177-228
   pressure_source_vector                 Source/IMEX/IMEX_O1.cpp:416-452
@@ -27,7 +28,10 @@ in the order written, which is the order of the reference's expressions.
 Pinned by tests/test_imex_stage.py: the reference's Rusanov known answers
 (Tests/RusanovAdv01.cpp), the energy/pressure identity of
 Tests/IMEX_O1_01.cpp (ExplicitKEEnergyMatch), free-stream preservation and
-conservation, and brute-force per-cell evaluation of the advective dt.
+conservation, brute-force per-cell evaluation of the advective dt, and for
+the MUSCL form: equality with the first-order fluxes on piecewise-constant
+data (minmod vanishes at a single jump) and second-order convergence on a
+smooth advected density wave.
 """
 import numpy as np
 
@@ -76,20 +80,66 @@ def rusanov_faces(U, eps):
     return out
 
 
+def minmod(dl, dr):
+    """Source/Fluxes/Limiters.H:75-81."""
+    return np.where((dl > 0) & (dr > 0), np.fmin(dl, dr),
+                    np.where((dl < 0) & (dr < 0), np.fmax(dl, dr), 0.0))
+
+
+def adv_flux(S, d, eps):
+    """adv_flux<d> (euler_K.H:53-65) of the state S."""
+    u = S[1 + d] / S[0]
+    F = np.empty_like(S)
+    F[0] = S[1 + d]
+    F[1] = S[1] * u
+    F[2] = S[2] * u
+    F[3] = u * kinetic_energy(S, eps)
+    return F
+
+
+def muscl_rusanov_faces(U, eps, dt, dx, dy):
+    """advance_MUSCL_rusanov_adv (IMEX_Fluxes.cpp:120-324) with the SP111
+    local step: per direction, minmod half-slopes, a half step of the
+    states with their own fluxes (hdtdx = 0.5 * (dt/dx)), then Rusanov fluxes
+    between R of cell i-1 and L of cell i (face index i)."""
+    out = []
+    for d, h in ((0, dx), (1, dy)):
+        hd = 0.5 * (dt / h)
+        hdelta = 0.5 * minmod(U - _sh(U, d, -1), _sh(U, d, 1) - U)
+        L = U - hdelta
+        R = U + hdelta
+        lu = hd * (adv_flux(R, d, eps) - adv_flux(L, d, eps))
+        L = L - lu
+        R = R - lu
+        FL, FR = adv_flux(L, d, eps), adv_flux(R, d, eps)
+        Rm, FRm = _sh(R, d, -1), _sh(FR, d, -1)
+        a, b = np.abs(L[1 + d] / L[0]), np.abs(Rm[1 + d] / Rm[0])
+        umax = np.where(a < b, b, a)
+        out.append(0.5 * ((FL + FRm) - umax * (L - Rm)))
+    return out
+
+
 class PeekStage:
     """One PEEK-SP111 step: explicit advection, pressure system, and after
-    the solve the flux correction and conservative update."""
+    the solve the flux correction and conservative update.  advection:
+    "rusanov" (advance_rusanov_adv) or "muscl_rusanov"."""
 
-    def __init__(self, gamma=1.4, eps=1.0):
+    def __init__(self, gamma=1.4, eps=1.0, advection="rusanov"):
         self.gamma = gamma
         self.eps = eps
+        self.muscl = advection == "muscl_rusanov"
+
+    def _faces(self, U, dt, dx, dy):
+        if self.muscl:
+            return muscl_rusanov_faces(U, self.eps, dt, dx, dy)
+        return rusanov_faces(U, self.eps)
 
     def assemble(self, U, p, dt, dx, dy=None):
         """Returns Uex, enth, rhs, a, bX (ny, nx+1), bY (ny+1, nx), alpha,
         beta and the Rusanov fluxes Fx, Fy."""
         dy = dx if dy is None else dy
         g, eps = self.gamma, self.eps
-        Fx, Fy = rusanov_faces(U, eps)
+        Fx, Fy = self._faces(U, dt, dx, dy)
         dtdx, dtdy = dt / dx, dt / dy
         Uex = U - (dtdx * (_sh(Fx, 0, 1) - Fx) + dtdy * (_sh(Fy, 1, 1) - Fy))
         enth = p * g / ((g - 1) * Uex[0])

@@ -155,3 +155,17 @@ def test_amr_plotfile_after_run(tmp_path):
     for lay, U, lv in ((s["layout"], Uc, pf["levels"][0]), (s["fine_layout"], Uf, pf["levels"][1])):
         for b, fab in enumerate(lv["fabs"]):
             assert np.array_equal(fab["data"], lay.valid(U, b))
+
+
+def test_amr_create_rejects_bad_hierarchies():
+    """Overlapping fine boxes and boxes not aligned to the ratio are refused
+    at set-up (no kernel runs)."""
+    from bamrex_amd._lib import BamrexError
+    from bamrex_amd.level import Layout
+    s = uniform_setup(16, 8)
+    for boxes in ([((4, 4), (11, 11)), ((10, 4), (17, 11))],   # overlap
+                  [((3, 4), (10, 11))]):                       # odd lo
+        flay = Layout(2, boxes, 2)
+        with pytest.raises(BamrexError):
+            amr.AmrHierarchy(gparams(s), s["layout"], flay, s["domain_lo"], s["domain_hi"],
+                             s["bc"], s["bc"]).close()

@@ -29,20 +29,21 @@ def random_state(ny, nx, gamma, eps, seed):
     return U
 
 
-def check_stage(U, p, p_new, dt, dx, dy, gamma, eps, tag):
+def check_stage(U, p, p_new, dt, dx, dy, gamma, eps, tag, advection="rusanov"):
     ny, nx = p.shape
-    announce("peek stage %s %dx%d" % (tag, nx, ny))
-    st = PeekStage(nx, ny, dx, dy, gamma, eps)
+    announce("peek stage %s %s %dx%d" % (tag, advection, nx, ny))
+    st = PeekStage(nx, ny, dx, dy, gamma, eps, advection)
+    ost = io.PeekStage(gamma, eps, advection)
     dU, dp, dpn, dout = dev(U), dev(p), dev(p_new), DeviceArray(U.shape)
     try:
         sy = st.assemble(dU, dp, dt)
-        ref = io.PeekStage(gamma, eps).assemble(U, p, dt, dx, dy)
+        ref = ost.assemble(U, p, dt, dx, dy)
         for key in ("Uex", "enth", "rhs", "a", "bX", "bY"):
             got = sy[key].download()
             assert np.array_equal(got, ref[key]), (tag, key, np.argwhere(got != ref[key])[:3])
         assert sy["beta"] == ref["beta"] and sy["alpha"] == ref["alpha"]
         st.finish(dU, dpn, dt, dout)
-        exp = io.PeekStage(gamma, eps).finish(U, ref, p_new, dt, dx, dy)
+        exp = ost.finish(U, ref, p_new, dt, dx, dy)
         got = dout.download()
         assert np.array_equal(got, exp), (tag, "U_new", np.argwhere(got != exp)[:3])
         dpp = DeviceArray(p.shape)
@@ -56,8 +57,9 @@ def check_stage(U, p, p_new, dt, dx, dy, gamma, eps, tag):
             d.free()
 
 
-@pytest.mark.parametrize("shape", [(20, 48), (1, 1), (5, 1), (1, 7), (3, 65), (130, 256)])
-def test_peek_stage_random_states(shape):
+@pytest.mark.parametrize("advection", ["rusanov", "muscl_rusanov"])
+@pytest.mark.parametrize("shape", [(20, 48), (1, 1), (5, 1), (1, 7), (2, 3), (3, 65), (130, 256)])
+def test_peek_stage_random_states(shape, advection):
     """Random states, anisotropic cells, eps != 1, sizes across the
     64 x 4 thread-block edges and degenerate one-cell directions (where the
     periodic neighbour is the cell itself)."""
@@ -67,18 +69,19 @@ def test_peek_stage_random_states(shape):
     rng = np.random.default_rng(7)
     p = 2.0 + rng.random((ny, nx))
     p_new = p * (1 + 0.01 * rng.standard_normal((ny, nx)))
-    check_stage(U, p, p_new, 0.013, 1.0 / nx, 0.7 / ny, gamma, eps, "random")
+    check_stage(U, p, p_new, 0.013, 1.0 / nx, 0.7 / ny, gamma, eps, "random", advection)
 
 
-def test_peek_stage_gresho_steps_and_conservation():
+@pytest.mark.parametrize("advection", ["rusanov", "muscl_rusanov"])
+def test_peek_stage_gresho_steps_and_conservation(advection):
     """Five steps of the Gresho vortex at 256^2 (eps = 1) driven through the
     GPU stage and the oracle side by side with the same pressure: identical
     dt and state after every step, and the GPU totals conserved."""
     n = 256
     gamma, eps = 1.4, 1.0
     U, dx = imex.gresho_state(n)
-    st = PeekStage(n, n, dx, gamma=gamma, eps=eps)
-    ost = io.PeekStage(gamma, eps)
+    st = PeekStage(n, n, dx, gamma=gamma, eps=eps, advection=advection)
+    ost = io.PeekStage(gamma, eps, advection)
     dU, dUn, dp = dev(U), DeviceArray(U.shape), DeviceArray((n, n))
     st.pressure(dU, dp)
     p = io.pressure_from_state(U, gamma, eps)
@@ -131,3 +134,20 @@ def test_workload_full_size_step_matches_oracle():
         assert np.array_equal(wl.U.download(), ost.finish(U0, ref, p_new, dt, dx))
     finally:
         wl.free()
+
+
+def test_peek_assemble_rejects_aliased_outputs():
+    """An output that aliases an input is refused before any launch."""
+    from bamrex_amd import _lib
+    n = 8
+    st = PeekStage(n, n, 1.0 / n)
+    dU, dp = DeviceArray((4, n, n)), DeviceArray((n, n))
+    try:
+        rc = _lib.lib().bx_imex_peek_assemble(dU.ptr, dp.ptr, n, n, st.dx, st.dy, 0.01, 1.4, 1.0, 0,
+                                              st.Uex.ptr, st.enth.ptr, dp.ptr, st.a.ptr, st.bX.ptr,
+                                              st.bY.ptr, None)
+        assert rc != 0 and b"alias" in _lib.lib().bx_helm_last_error()
+    finally:
+        st.free()
+        dU.free()
+        dp.free()

@@ -4,6 +4,7 @@ import os
 import sys
 
 import numpy as np
+import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
@@ -111,14 +112,15 @@ def test_advective_dt_brute_force_and_anisotropic_cells():
     assert io.advective_dt(U, dx, dy, cfl=0.45) != io.advective_dt(U, dy, dx, cfl=0.45)
 
 
-def test_stage_anisotropic_free_stream_and_conservation():
+@pytest.mark.parametrize("advection", ["rusanov", "muscl_rusanov"])
+def test_stage_anisotropic_free_stream_and_conservation(advection):
     """Uniform flow stays uniform with dx != dy, and a non-uniform state's
     totals are conserved (periodic fluxes cancel) with dx != dy."""
     ny, nx = 12, 20
     gamma, eps = 1.4, 0.3
     U = state_from_primv(np.full((ny, nx), 0.8), np.full((ny, nx), 0.5),
                          np.full((ny, nx), -0.25), np.full((ny, nx), 3.0), gamma, eps)
-    st = io.PeekStage(gamma, eps)
+    st = io.PeekStage(gamma, eps, advection)
     dx, dy, dt = 1.0 / nx, 0.5 / ny, 0.01
     p = io.pressure_from_state(U, gamma, eps)
     sy = st.assemble(U, p, dt, dx, dy)
@@ -132,3 +134,46 @@ def test_stage_anisotropic_free_stream_and_conservation():
     Un2 = st.finish(U2, sy2, 1.01 * p2, dt, dx, dy)
     scale = np.abs(U2).sum(axis=(1, 2))
     assert np.all(np.abs(Un2.sum(axis=(1, 2)) - U2.sum(axis=(1, 2))) <= 1e-13 * scale)
+
+
+def test_muscl_equals_first_order_on_piecewise_constant_data():
+    """ToroTest2 data (constant states, two jumps): every minmod slope is
+    zero (one of the two differences vanishes at each jump), so the MUSCL
+    states are the cell values, the local step changes nothing, and the
+    MUSCL fluxes equal the first-order ones bit for bit."""
+    nx, ny = 8, 2
+    u = np.where(np.arange(nx) < 4, -2.0, 2.0)[None, :].repeat(ny, 0)
+    U = state_from_primv(np.ones((ny, nx)), u, np.zeros((ny, nx)), np.full((ny, nx), 0.4), 1.4, 1.0)
+    f1 = io.rusanov_faces(U, 1.0)
+    f2 = io.muscl_rusanov_faces(U, 1.0, 0.01, 1.0 / nx, 1.0 / ny)
+    for a, b in zip(f1, f2):
+        assert np.array_equal(a, b)
+
+
+def _density_wave_error(n, advection):
+    """L1 error of rho after advecting rho = 1 + 0.2 sin(2 pi x) with u = 1,
+    p = 1 (exact pressure, no solve) to t = 0.25 on an n x 4 periodic grid."""
+    gamma, eps = 1.4, 1.0
+    x = (np.arange(n) + 0.5) / n
+    rho0 = 1.0 + 0.2 * np.sin(2 * np.pi * x)[None, :].repeat(4, 0)
+    U = state_from_primv(rho0, np.ones((4, n)), np.zeros((4, n)), np.ones((4, n)), gamma, eps)
+    st = io.PeekStage(gamma, eps, advection)
+    dx, dy = 1.0 / n, 0.25
+    t, t_end = 0.0, 0.25
+    while t < t_end - 1e-14:
+        dt = min(0.4 * dx, t_end - t)
+        p = io.pressure_from_state(U, gamma, eps)
+        sy = st.assemble(U, p, dt, dx, dy)
+        U = st.finish(U, sy, np.ones_like(p), dt, dx, dy)
+        t += dt
+    exact = 1.0 + 0.2 * np.sin(2 * np.pi * (x - t_end))
+    return np.mean(np.abs(U[0, 0] - exact))
+
+
+def test_muscl_is_second_order_on_a_smooth_wave():
+    e = {a: [_density_wave_error(n, a) for n in (32, 64, 128)] for a in ("rusanov", "muscl_rusanov")}
+    r1 = e["rusanov"][1] / e["rusanov"][2]
+    r2 = e["muscl_rusanov"][1] / e["muscl_rusanov"][2]
+    assert 1.6 < r1 < 2.4, e          # first order
+    assert r2 > 3.0, e                 # second order (minmod clips the extrema)
+    assert e["muscl_rusanov"][2] < 0.2 * e["rusanov"][2], e

@@ -136,6 +136,12 @@ _SIGS = {
                                      ctypes.POINTER(ctypes.c_double), P]),
     "bx_amr_run": (ctypes.c_int, [P, DP, ctypes.c_double, ctypes.c_double, ctypes.c_int, P, P, P, P, P,
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), P]),
+    "bx_amr_create_levels": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(P)]),
+    "bx_amr_advance_levels": (ctypes.c_int, [P, ctypes.c_double, DP, ctypes.c_double, P, P, P, P]),
+    "bx_amr_est_dt_levels": (ctypes.c_int, [P, DP, ctypes.c_double, ctypes.c_double, ctypes.c_double, P,
+                                            ctypes.POINTER(ctypes.c_double), P]),
+    "bx_amr_run_levels": (ctypes.c_int, [P, DP, ctypes.c_double, ctypes.c_double, ctypes.c_int, P, P, P,
+                                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), P]),
     "bx_helm_last_error": (ctypes.c_char_p, []),
     "bx_imex_pressure": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, P,
                                         P]),

@@ -127,6 +127,97 @@ class AmrHierarchy:
         self.fine.close()
 
 
+class AmrLevels:
+    """Any number of levels (ratio 2 between neighbours) coupled by bx_amr:
+    layouts[l] in level-l index space; domain and dx0 are level 0's."""
+
+    def __init__(self, params, layouts, domain_lo, domain_hi, bc_lo=None, bc_hi=None):
+        self.dim = layouts[0].dim
+        self.levels = []
+        self.olds = []
+        self.a = ctypes.c_void_p()
+        try:
+            for l, lay in enumerate(layouts):
+                self.levels.append(Level(params, lay, [x << l for x in domain_lo],
+                                         [((x + 1) << l) - 1 for x in domain_hi], bc_lo, bc_hi))
+                self.olds.append(DeviceBuffer(lay.total * 8))
+            self.stream = self.levels[0].stream
+            ctxs = (ctypes.c_void_p * len(self.levels))(*[lv.ctx.value for lv in self.levels])
+            check(L.lib().bx_amr_create_levels(ctypes.cast(ctxs, ctypes.c_void_p), len(ctxs),
+                                               ctypes.byref(self.a)), "bx_amr_create_levels")
+        except Exception:
+            self.close()
+            raise
+
+    def _ptrs(self, which):
+        bufs = {"U": [lv.U for lv in self.levels], "T": [lv.U_tmp for lv in self.levels],
+                "O": self.olds}[which]
+        arr = (ctypes.c_void_p * len(bufs))(*[b.ptr.value for b in bufs])
+        return ctypes.cast(arr, ctypes.c_void_p), arr
+
+    def set_state(self, Us):
+        for lv, U in zip(self.levels, Us):
+            lv.set_state(U)
+
+    def get_state(self):
+        return [lv.get_state() for lv in self.levels]
+
+    def run(self, dx0, cfl, stop_time=-1.0, max_step=-1, start_time=0.0):
+        t = ctypes.c_double(start_time)
+        n = ctypes.c_int()
+        (pu, ku), (pt, kt), (po, ko) = self._ptrs("U"), self._ptrs("T"), self._ptrs("O")
+        check(L.lib().bx_amr_run_levels(self.a, L.dvec3(dx0), cfl, stop_time, max_step, pu, pt, po,
+                                        ctypes.byref(t), ctypes.byref(n), self.stream),
+              "bx_amr_run_levels")
+        return t.value, n.value
+
+    def write_plotfile(self, path, time, dx0, prob_lo, prob_hi, steps=0, derived=False):
+        """Every level in one AMReX plotfile (AmrLevel::writePlotFile)."""
+        from .plotfile import write_plotfile
+        dim = self.dim
+        levels = []
+        for l, (lv, U) in enumerate(zip(self.levels, self.get_state())):
+            g = lv.geom
+            levels.append(dict(layout=lv.layout, U=U, domain_lo=list(g.domain_lo)[:dim],
+                               domain_hi=list(g.domain_hi)[:dim],
+                               dx=tuple(x / 2 ** l for x in dx0), steps=steps << l))
+        p = self.levels[0].params
+        write_plotfile(path, levels, prob_lo, prob_hi, time, p.adiabatic, p.epsilon, derived=derived)
+
+    def close(self):
+        if self.a:
+            L.lib().bx_amr_destroy(self.a)
+            self.a = None
+        for b in self.olds:
+            b.free()
+        for lv in self.levels:
+            lv.close()
+        self.olds, self.levels = [], []
+
+
+def refine_levels(make, ncell, max_grid_size, regions, max_grid_sizes=None, **kw):
+    """Initial data of a hierarchy: make(...) on level 0 and, for each
+    region (lo, hi in level-l index space, inclusive) of levels 1.., the
+    same problem at 2^l times the resolution on boxes covering it."""
+    s = make(ncell=ncell, max_grid_size=max_grid_size, **kw)
+    dim = s["layout"].dim
+    layouts, Us = [s["layout"]], [s["U"]]
+    for l, (lo, hi) in enumerate(regions, start=1):
+        sf = make(ncell=ncell << l, max_grid_size=ncell << l, **kw)
+        mgs = (max_grid_sizes or [max_grid_size] * len(regions))[l - 1]
+        boxes = chop_domain(tuple(lo), tuple(hi), mgs)
+        lay = Layout(dim, boxes, s["layout"].ng)
+        U = lay.new_array()
+        full = sf["layout"].valid(sf["U"], 0)
+        for b, (blo, bhi) in enumerate(boxes):
+            sl = tuple(slice(blo[d], bhi[d] + 1) for d in reversed(range(dim)))
+            lay.valid(U, b)[...] = full[(slice(None),) + sl]
+        layouts.append(lay)
+        Us.append(U)
+    s.update(layouts=layouts, Us=Us)
+    return s
+
+
 def composite_totals(coarse_layout, Uc, fine_layout, Uf, dx_coarse):
     """Sum over the composite grid of U * cell volume (coarse cells not
     covered by the fine level, and fine cells), per component."""
@@ -148,4 +239,25 @@ def composite_totals(coarse_layout, Uc, fine_layout, Uf, dx_coarse):
     for b in range(len(fine_boxes)):
         v = fine_layout.valid(Uf, b)
         tot += volf * np.array([v[n].sum() for n in range(dim + 2)])
+    return tot
+
+
+def composite_totals_levels(layouts, Us, dx0):
+    """Sum over the composite grid of U * cell volume: each level's cells
+    not covered by the next finer level, per component."""
+    dim = layouts[0].dim
+    tot = np.zeros(dim + 2)
+    for l, (lay, U) in enumerate(zip(layouts, Us)):
+        vol = float(np.prod(dx0)) / 2 ** (dim * l)
+        finer = layouts[l + 1].boxes if l + 1 < len(layouts) else []
+        for b, (lo, hi) in enumerate(lay.boxes):
+            v = lay.valid(U, b)
+            mask = np.ones(v.shape[1:], dtype=bool)
+            for flo, fhi in finer:
+                clo = [max(x // 2, q) for x, q in zip(flo, lo)]
+                chi = [min(x // 2, q) for x, q in zip(fhi, hi)]
+                if all(a <= c for a, c in zip(clo, chi)):
+                    sl = tuple(slice(clo[d] - lo[d], chi[d] - lo[d] + 1) for d in reversed(range(dim)))
+                    mask[sl] = False
+            tot += vol * np.array([v[n][mask].sum() for n in range(dim + 2)])
     return tot

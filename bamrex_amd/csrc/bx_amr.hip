@@ -361,7 +361,8 @@ static hipError_t reg_scheme(int sch, int lim, int lv, const double *U, const De
 } // namespace bx
 
 // ================================================================ host side
-struct bx_amr
+// The coupling of two adjacent levels: its plan and flux registers.
+struct AmrPair
 {
     bx_ctx *crse = nullptr, *fine = nullptr;
     int     D = 0, NC = 0;
@@ -427,7 +428,7 @@ static bool wrap_into(const bx_geometry &g, int D, int x[3])
     return true;
 }
 
-static int amr_plan(bx_amr *a)
+static int amr_plan(AmrPair *a)
 {
     const bx_ctx      *C = a->crse, *F = a->fine;
     const int          D = a->D;
@@ -450,6 +451,28 @@ static int amr_plan(bx_amr *a)
                                  "bx_amr_create: fine boxes must be ratio-2 aligned and inside the domain");
     if (gc.ng < 2 || gf.ng < 2)
         return set_error(BX_ERR_INVALID, "bx_amr_create: both levels need 2 ghost cells");
+    // proper nesting: the coarse-fine interpolation reads the coarse cells
+    // within 2 of each fine box (coarsened), which must be coarse valid
+    // cells or coarse ghosts filled from other coarse boxes
+    for (const bx_box &b : FB)
+    {
+        int lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
+        for (int d = 0; d < D; ++d)
+        {
+            lo[d] = (b.lo[d] >> 1) - 2;
+            hi[d] = (b.hi[d] >> 1) + 2;
+        }
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x)
+                {
+                    int c[3] = { x, y, z };
+                    if (wrap_into(gc, D, c) && find_box(CB, c, D) < 0)
+                        return set_error(BX_ERR_INVALID,
+                                         "bx_amr_create: the fine level is not properly nested "
+                                         "(2 coarse cells around every fine box)");
+                }
+    }
     for (size_t i = 0; i < FB.size(); ++i)
         for (size_t j = i + 1; j < FB.size(); ++j)
         {
@@ -733,7 +756,7 @@ static int amr_plan(bx_amr *a)
     return BX_OK;
 }
 
-extern "C" void bx_amr_destroy(bx_amr *a)
+static void pair_destroy(AmrPair *a)
 {
     if (!a)
         return;
@@ -744,34 +767,80 @@ extern "C" void bx_amr_destroy(bx_amr *a)
     delete a;
 }
 
-extern "C" int bx_amr_create(bx_ctx *coarse, bx_ctx *fine, bx_amr **out)
+// A hierarchy of levels 0..nlev-1 (ratio 2 between neighbours) and the
+// couplings of adjacent levels.
+struct bx_amr
 {
-    if (!coarse || !fine || !out || coarse == fine)
-        return set_error(BX_ERR_INVALID, "bx_amr_create: bad argument");
-    if (!coarse->level_set || !fine->level_set)
-        return set_error(BX_ERR_INVALID, "bx_amr_create: both levels need bx_set_level first");
-    const bx_params &pc = coarse->p, &pf = fine->p;
+    int                    D = 0, NC = 0;
+    std::vector<bx_ctx *>  lev;
+    std::vector<AmrPair *> pairs; // pairs[l] couples lev[l] and lev[l+1]
+};
+
+extern "C" void bx_amr_destroy(bx_amr *a)
+{
+    if (!a)
+        return;
+    for (AmrPair *p : a->pairs)
+        pair_destroy(p);
+    delete a;
+}
+
+static bool same_params(const bx_params &pc, const bx_params &pf)
+{
     bool same = pc.dim == pf.dim && pc.num_method == pf.num_method
                 && pc.adiabatic == pf.adiabatic && pc.epsilon == pf.epsilon
                 && pc.limiter == pf.limiter;
     for (int n = 0; n < 5; ++n)
         same = same && pc.limiter_indices[n] == pf.limiter_indices[n]
                && pc.limit_on_primv[n] == pf.limit_on_primv[n];
-    if (!same)
-        return set_error(BX_ERR_INVALID, "bx_amr_create: the levels must share bx_params");
-    bx_amr *a = new bx_amr;
-    a->crse   = coarse;
-    a->fine   = fine;
-    a->D      = coarse->D;
-    a->NC     = coarse->NC;
-    int rc    = amr_plan(a);
-    if (rc != BX_OK)
+    return same;
+}
+
+extern "C" int bx_amr_create_levels(bx_ctx *const *levels, int nlev, bx_amr **out)
+{
+    if (!levels || nlev < 2 || nlev > 16 || !out)
+        return set_error(BX_ERR_INVALID, "bx_amr_create: bad argument");
+    for (int l = 0; l < nlev; ++l)
     {
-        bx_amr_destroy(a);
-        return rc;
+        if (!levels[l])
+            return set_error(BX_ERR_INVALID, "bx_amr_create: null level");
+        if (!levels[l]->level_set)
+            return set_error(BX_ERR_INVALID, "bx_amr_create: every level needs bx_set_level first");
+        for (int m = 0; m < l; ++m)
+            if (levels[m] == levels[l])
+                return set_error(BX_ERR_INVALID, "bx_amr_create: a level appears twice");
+        if (l && (levels[l]->D != levels[0]->D || !same_params(levels[0]->p, levels[l]->p)))
+            return set_error(BX_ERR_INVALID, "bx_amr_create: the levels must share bx_params");
+    }
+    bx_amr *a = new bx_amr;
+    a->D      = levels[0]->D;
+    a->NC     = levels[0]->NC;
+    a->lev.assign(levels, levels + nlev);
+    for (int l = 0; l + 1 < nlev; ++l)
+    {
+        AmrPair *p = new AmrPair;
+        p->crse    = levels[l];
+        p->fine    = levels[l + 1];
+        p->D       = a->D;
+        p->NC      = a->NC;
+        a->pairs.push_back(p);
+        const int rc = amr_plan(p);
+        if (rc != BX_OK)
+        {
+            bx_amr_destroy(a);
+            return rc;
+        }
     }
     *out = a;
     return BX_OK;
+}
+
+extern "C" int bx_amr_create(bx_ctx *coarse, bx_ctx *fine, bx_amr **out)
+{
+    bx_ctx *l[2] = { coarse, fine };
+    if (!coarse || !fine)
+        return set_error(BX_ERR_INVALID, "bx_amr_create: bad argument");
+    return bx_amr_create_levels(l, 2, out);
 }
 
 extern "C" int bx_amr_info(const bx_amr *a, int *n_cf_segs, int *n_register_faces,
@@ -779,17 +848,24 @@ extern "C" int bx_amr_info(const bx_amr *a, int *n_cf_segs, int *n_register_face
 {
     if (!a)
         return set_error(BX_ERR_INVALID, "bx_amr_info: null");
+    int nc = 0, nr = 0, nt = 0;
+    for (const AmrPair *p : a->pairs)
+    {
+        nc += (int)p->cf.size();
+        nr += p->nslot;
+        nt += (int)p->tg.size();
+    }
     if (n_cf_segs)
-        *n_cf_segs = (int)a->cf.size();
+        *n_cf_segs = nc;
     if (n_register_faces)
-        *n_register_faces = a->nslot;
+        *n_register_faces = nr;
     if (n_reflux_cells)
-        *n_reflux_cells = (int)a->tg.size();
+        *n_reflux_cells = nt;
     return BX_OK;
 }
 
 // Flux-register contribution of one level's sweep input in direction d.
-static int reg_fluxes(bx_amr *a, bool fine_level, int d, const double *U, double dt,
+static int reg_fluxes(AmrPair *a, bool fine_level, int d, const double *U, double dt,
                       const double dx[3], hipStream_t s)
 {
     const int nslot = a->slot0[d + 1] - a->slot0[d];
@@ -828,13 +904,14 @@ static int reg_fluxes(bx_amr *a, bool fine_level, int d, const double *U, double
     return BX_OK;
 }
 
-// One level's split step from A (valid + ghosts filled), result in A;
-// B is the ping-pong buffer.  Registers are fed from each sweep's input.
-static int level_step(bx_amr *a, bool fine_level, double dt, const double dx[3], double *A,
-                      double *B, hipStream_t s)
+// One split step of level l from A (valid + ghosts filled), result in A; B
+// is the ping-pong buffer.  Each sweep's input feeds the flux registers of
+// the pair below (as its fine side) and of the pair above (coarse side).
+static int level_step(bx_amr *h, int l, double dt, const double dx[3], double *A, double *B,
+                      hipStream_t s)
 {
-    bx_ctx   *L = fine_level ? a->fine : a->crse;
-    const int D = a->D;
+    bx_ctx   *L = h->lev[l];
+    const int D = h->D;
     int       rc;
     if ((rc = set_scalar(L->d_dt, dt, s)) != BX_OK)
         return rc;
@@ -844,7 +921,10 @@ static int level_step(bx_amr *a, bool fine_level, double dt, const double dx[3],
         double       *out = (d % 2 == 0) ? B : A;
         if ((rc = do_sweep(L, d, dx, in, out, nullptr, s)) != BX_OK)
             return rc;
-        if ((rc = reg_fluxes(a, fine_level, d, in, dt, dx, s)) != BX_OK)
+        if (l > 0 && (rc = reg_fluxes(h->pairs[l - 1], true, d, in, dt, dx, s)) != BX_OK)
+            return rc;
+        if (l + 1 < (int)h->lev.size()
+            && (rc = reg_fluxes(h->pairs[l], false, d, in, dt, dx, s)) != BX_OK)
             return rc;
         if ((rc = do_fill(L, out, (d + 1) % D, s)) != BX_OK)
             return rc;
@@ -855,7 +935,7 @@ static int level_step(bx_amr *a, bool fine_level, double dt, const double dx[3],
     return BX_OK;
 }
 
-static int cf_fill(bx_amr *a, double *Uf, const double *Uold, const double *Unew, int mode,
+static int cf_fill(AmrPair *a, double *Uf, const double *Uold, const double *Unew, int mode,
                    double alpha, double beta, hipStream_t s)
 {
     const int nseg = (int)a->cf.size();
@@ -878,7 +958,7 @@ static int cf_fill(bx_amr *a, double *Uf, const double *Uold, const double *Unew
     return BX_OK;
 }
 
-static int avg_down(bx_amr *a, const double *Uf, double *Uc, hipStream_t s)
+static int avg_down(AmrPair *a, const double *Uf, double *Uc, hipStream_t s)
 {
     const int nseg = (int)a->avg.size();
     if (nseg == 0)
@@ -908,15 +988,64 @@ static bool amr_dx_ok(const bx_amr *a, const double dxc[3])
     return true;
 }
 
+static int reflux(AmrPair *p, double *Uc, const double dx_crse[3], hipStream_t s)
+{
+    if (p->tg.empty())
+        return BX_OK;
+    double vol = dx_crse[0];
+    for (int d = 1; d < p->D; ++d)
+        vol *= dx_crse[d];
+    const int nt = (int)p->tg.size();
+    bx_ctx   *C  = p->crse;
+#define BX_RF(DD)                                                                          \
+    hipLaunchKernelGGL((k_reflux<DD>), dim3((nt + 127) / 128), dim3(128), 0, s, Uc,          \
+                       C->d_boxes, p->d_tg, nt, p->d_terms, p->d_reg, C->geom.ng, vol)
+    if (p->D == 1)
+        BX_RF(1);
+    else if (p->D == 2)
+        BX_RF(2);
+    else
+        BX_RF(3);
+#undef BX_RF
+    BX_HIP_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+// Ghost cells of level l (> 0) at `time`: fine-fine and physical, then the
+// coarse-fine ones from level l-1's states at t_old (Uold) and t_new (Unew).
+// Applied to both of the level's buffers when B is given (B's coarse-fine
+// ghosts only: the sweeps refill the rest).
+static int fill_level(bx_amr *h, int l, double *A, double *B, const double *Uold,
+                      const double *Unew, double t_old, double t_new, double time, hipStream_t s)
+{
+    int rc;
+    if ((rc = do_fill_all(h->lev[l], A, s)) != BX_OK)
+        return rc;
+    AmrPair *p = h->pairs[l - 1];
+    for (double *X : { A, B })
+    {
+        if (!X)
+            continue;
+        if (time == t_old || !(t_new != t_old))
+            rc = cf_fill(p, X, Uold, Unew, 0, 1.0, 0.0, s);
+        else // FillPatchTwoLevels time interpolation: alpha old + beta new
+            rc = cf_fill(p, X, Uold, Unew, 1, (t_new - time) / (t_new - t_old),
+                         (time - t_old) / (t_new - t_old), s);
+        if (rc != BX_OK)
+            return rc;
+    }
+    return BX_OK;
+}
+
 extern "C" int bx_amr_avg_down(bx_amr *a, const double *Uf, double *Uc, void *stream)
 {
     if (!a || !Uf || !Uc)
         return set_error(BX_ERR_INVALID, "bx_amr_avg_down: bad argument");
     hipStream_t s = (hipStream_t)stream;
     int         rc;
-    if ((rc = avg_down(a, Uf, Uc, s)) != BX_OK)
+    if ((rc = avg_down(a->pairs[0], Uf, Uc, s)) != BX_OK)
         return rc;
-    return do_fill_all(a->crse, Uc, s);
+    return do_fill_all(a->lev[0], Uc, s);
 }
 
 extern "C" int bx_amr_fill_fine(bx_amr *a, double *Uf, const double *Uc_old,
@@ -925,100 +1054,53 @@ extern "C" int bx_amr_fill_fine(bx_amr *a, double *Uf, const double *Uc_old,
 {
     if (!a || !Uf || !Uc_old || !Uc_new)
         return set_error(BX_ERR_INVALID, "bx_amr_fill_fine: bad argument");
-    hipStream_t s = (hipStream_t)stream;
-    int         rc;
-    if ((rc = do_fill_all(a->fine, Uf, s)) != BX_OK)
-        return rc;
-    if (time == t_old || !(t_new != t_old))
-        return cf_fill(a, Uf, Uc_old, Uc_new, 0, 1.0, 0.0, s);
-    // FillPatchTwoLevels time interpolation: alpha old + beta new
-    const double alpha = (t_new - time) / (t_new - t_old);
-    const double beta  = (time - t_old) / (t_new - t_old);
-    return cf_fill(a, Uf, Uc_old, Uc_new, 1, alpha, beta, s);
+    return fill_level(a, 1, Uf, nullptr, Uc_old, Uc_new, t_old, t_new, time,
+                      (hipStream_t)stream);
 }
 
-static int amr_advance(bx_amr *a, double dt, const double dx_coarse[3], double time, double *Uc,
-                       double *Uc_tmp, double *Uc_old, double *Uf, double *Uf_tmp,
-                       hipStream_t s, bool fill_fine_after);
-
-extern "C" int bx_amr_advance(bx_amr *a, double dt, const double dx_coarse[3], double time,
-                              double *Uc, double *Uc_tmp, double *Uc_old, double *Uf,
-                              double *Uf_tmp, void *stream)
+struct LevelBufs
 {
-    return amr_advance(a, dt, dx_coarse, time, Uc, Uc_tmp, Uc_old, Uf, Uf_tmp,
-                       (hipStream_t)stream, true);
-}
+    double *const *U, *const *T, *const *O;
+};
 
-// fill_fine_after = false leaves the fine ghost cells as the last sweep left
-// them (the next step's fine fill rewrites all of them).
-static int amr_advance(bx_amr *a, double dt, const double dx_coarse[3], double time, double *Uc,
-                       double *Uc_tmp, double *Uc_old, double *Uf, double *Uf_tmp,
-                       hipStream_t s, bool fill_fine_after)
+// Amr::timeStep(l) (AMReX): advance level l by dt from `time`; the finer
+// levels subcycle twice, then post_timestep(l) refluxes and averages down.
+// t_par / dt_par: the parent level's step (for the ghost interpolation).
+static int time_step(bx_amr *h, int l, double time, double dt, double t_par, double dt_par,
+                     const double dx0[3], const LevelBufs &b, hipStream_t s, bool fill_after)
 {
-    void *stream = (void *)s;
-    if (!a || !Uc || !Uc_tmp || !Uc_old || !Uf || !Uf_tmp || !amr_dx_ok(a, dx_coarse)
-        || !(dt > 0))
-        return set_error(BX_ERR_INVALID, "bx_amr_advance: bad argument");
-    bx_ctx     *C  = a->crse, *F = a->fine;
-    const int   D  = a->D;
-    double      dxf[3] = { 0, 0, 0 };
-    for (int d = 0; d < D; ++d)
-        dxf[d] = dx_coarse[d] / 2;
+    const int nlev = (int)h->lev.size();
+    double    dx[3] = { 0, 0, 0 };
+    for (int d = 0; d < h->D; ++d)
+        dx[d] = dx0[d] / (double)(1 << l);
     int rc;
-    // coarse level: keep the old state (ghosts filled) for the time interpolation
-    BX_HIP_CHECK(hipMemcpyAsync(Uc_old, Uc, sizeof(double) * C->level_extent,
-                                hipMemcpyDeviceToDevice, s));
-    if ((rc = level_step(a, false, dt, dx_coarse, Uc, Uc_tmp, s)) != BX_OK)
+    if (l > 0 && (rc = fill_level(h, l, b.U[l], b.T[l], b.O[l - 1], b.U[l - 1], t_par,
+                                  t_par + dt_par, time, s))
+                     != BX_OK)
         return rc;
-    if ((rc = do_fill_all(C, Uc, s)) != BX_OK)
+    if (l + 1 < nlev) // the old state, ghosts filled, for the finer level's fills
+        BX_HIP_CHECK(hipMemcpyAsync(b.O[l], b.U[l], sizeof(double) * h->lev[l]->level_extent,
+                                    hipMemcpyDeviceToDevice, s));
+    if ((rc = level_step(h, l, dt, dx, b.U[l], b.T[l], s)) != BX_OK)
         return rc;
-    // fine level: two steps of dt / 2 (dt_level[1] = dt_0 / 2)
-    const double dtf   = dt / 2;
-    const double t_new = time + dt;
-    for (int sub = 0; sub < 2; ++sub)
+    if (l + 1 < nlev)
     {
-        const double tf = sub == 0 ? time : time + dtf;
-        if ((rc = bx_amr_fill_fine(a, Uf, Uc_old, Uc, time, t_new, tf, stream)) != BX_OK)
+        if ((rc = do_fill_all(h->lev[l], b.U[l], s)) != BX_OK)
             return rc;
-        // The reference's second buffer starts as a copy of the filled state
-        // (Sborder2).  Of it the sweeps read the valid cells they write
-        // first and the ghost cells of the directions after the first: the
-        // fills after each sweep provide the fine-fine and physical ones,
-        // and the coarse-fine ones are interpolated here, to the same values.
-        if (tf == time)
-            rc = cf_fill(a, Uf_tmp, Uc_old, Uc, 0, 1.0, 0.0, s);
-        else
-            rc = cf_fill(a, Uf_tmp, Uc_old, Uc, 1, (t_new - tf) / (t_new - time),
-                         (tf - time) / (t_new - time), s);
-        if (rc != BX_OK)
-            return rc;
-        if ((rc = level_step(a, true, dtf, dxf, Uf, Uf_tmp, s)) != BX_OK)
+        const double dtf = dt / 2; // dt_level[l+1] = dt_level[l] / 2
+        for (int sub = 0; sub < 2; ++sub)
+            if ((rc = time_step(h, l + 1, sub == 0 ? time : time + dtf, dtf, time, dt, dx0, b, s,
+                                false))
+                != BX_OK)
+                return rc;
+        AmrPair *p = h->pairs[l];
+        if ((rc = reflux(p, b.U[l], dx, s)) != BX_OK || (rc = avg_down(p, b.U[l + 1], b.U[l], s))
+            || (rc = do_fill_all(h->lev[l], b.U[l], s)) != BX_OK)
             return rc;
     }
-    // post_timestep of the coarse level: reflux, then avgDown
-    if (!a->tg.empty())
-    {
-        double vol = dx_coarse[0];
-        for (int d = 1; d < D; ++d)
-            vol *= dx_coarse[d];
-        const int nt = (int)a->tg.size();
-#define BX_RF(DD)                                                                          \
-    hipLaunchKernelGGL((k_reflux<DD>), dim3((nt + 127) / 128), dim3(128), 0, s, Uc,          \
-                       C->d_boxes, a->d_tg, nt, a->d_terms, a->d_reg, C->geom.ng, vol)
-        if (D == 1)
-            BX_RF(1);
-        else if (D == 2)
-            BX_RF(2);
-        else
-            BX_RF(3);
-#undef BX_RF
-        BX_HIP_CHECK(hipGetLastError());
-    }
-    if ((rc = avg_down(a, Uf, Uc, s)) != BX_OK)
+    else if (fill_after && (rc = do_fill_all(h->lev[l], b.U[l], s)) != BX_OK)
         return rc;
-    if ((rc = do_fill_all(C, Uc, s)) != BX_OK)
-        return rc;
-    return fill_fine_after ? do_fill_all(F, Uf, s) : BX_OK;
+    return BX_OK;
 }
 
 // estTimeStep of one level (AmrLevelAdv.cpp:395-477, explicit methods) from
@@ -1037,35 +1119,32 @@ static void level_dt_est(const bx_ctx *L, unsigned long long bits, const double 
     *out = dt_est * cfl;
 }
 
-extern "C" int bx_amr_est_dt(bx_amr *a, const double dx_coarse[3], double cfl, double time,
-                             double stop_time, const double *Uc, const double *Uf, double *dt,
-                             void *stream)
+// computeNewDt (AmrLevelAdv.cpp:589-618): dt_0 = min over levels of
+// n_factor * dt_est(l), n_factor = 2^l, then the stop-time clamp.
+static int est_dt_levels(bx_amr *a, const double dx0[3], double cfl, double time,
+                         double stop_time, const double *const *U, double *dt, hipStream_t s)
 {
-    if (!a || !Uc || !Uf || !dt || !amr_dx_ok(a, dx_coarse))
-        return set_error(BX_ERR_INVALID, "bx_amr_est_dt: bad argument");
-    hipStream_t s      = (hipStream_t)stream;
-    double      dxf[3] = { 0, 0, 0 };
-    for (int d = 0; d < a->D; ++d)
-        dxf[d] = dx_coarse[d] / 2;
-    double dmin[2];
-    int    rc;
-    if ((rc = do_maxspeed(a->crse, Uc, s)) != BX_OK || (rc = do_maxspeed(a->fine, Uf, s)) != BX_OK)
-        return rc;
-    unsigned long long bits[2] = { 0, 0 };
-    BX_HIP_CHECK(hipMemcpyAsync(&bits[0], a->crse->d_maxbits, sizeof(bits[0]),
-                                hipMemcpyDeviceToHost, s));
-    BX_HIP_CHECK(hipMemcpyAsync(&bits[1], a->fine->d_maxbits, sizeof(bits[1]),
-                                hipMemcpyDeviceToHost, s));
+    const int nlev = (int)a->lev.size();
+    int       rc;
+    for (int l = 0; l < nlev; ++l)
+        if ((rc = do_maxspeed(a->lev[l], U[l], s)) != BX_OK)
+            return rc;
+    std::vector<unsigned long long> bits(nlev, 0ULL);
+    for (int l = 0; l < nlev; ++l)
+        BX_HIP_CHECK(hipMemcpyAsync(&bits[l], a->lev[l]->d_maxbits, sizeof(bits[l]),
+                                    hipMemcpyDeviceToHost, s));
     BX_HIP_CHECK(hipStreamSynchronize(s));
-    level_dt_est(a->crse, bits[0], dx_coarse, cfl, &dmin[0]);
-    level_dt_est(a->fine, bits[1], dxf, cfl, &dmin[1]);
-    // computeNewDt (AmrLevelAdv.cpp:589-618): n_cycle = {1, 2}
     double dt_0     = 1.0e+100;
     int    n_factor = 1;
-    for (int i = 0; i < 2; ++i)
+    for (int l = 0; l < nlev; ++l)
     {
-        n_factor *= (i == 0 ? 1 : 2);
-        dt_0 = std::min(dt_0, n_factor * dmin[i]);
+        double dxl[3] = { 0, 0, 0 };
+        for (int d = 0; d < a->D; ++d)
+            dxl[d] = dx0[d] / (double)(1 << l);
+        double dmin;
+        level_dt_est(a->lev[l], bits[l], dxl, cfl, &dmin);
+        n_factor *= (l == 0 ? 1 : 2);
+        dt_0 = std::min(dt_0, n_factor * dmin);
     }
     const double eps = 0.001 * dt_0;
     if (stop_time >= 0.0)
@@ -1075,36 +1154,114 @@ extern "C" int bx_amr_est_dt(bx_amr *a, const double dx_coarse[3], double cfl, d
     return BX_OK;
 }
 
-extern "C" int bx_amr_run(bx_amr *a, const double dx_coarse[3], double cfl, double stop_time,
-                          int max_step, double *Uc, double *Uc_tmp, double *Uc_old, double *Uf,
-                          double *Uf_tmp, double *time, int *steps, void *stream)
+static bool bufs_ok(const bx_amr *a, const LevelBufs &b)
 {
-    if (!a || !time || !amr_dx_ok(a, dx_coarse))
-        return set_error(BX_ERR_INVALID, "bx_amr_run: bad argument");
+    if (!b.U || !b.T || !b.O)
+        return false;
+    for (size_t l = 0; l < a->lev.size(); ++l)
+        if (!b.U[l] || !b.T[l] || (l + 1 < a->lev.size() && !b.O[l]))
+            return false;
+    return true;
+}
+
+extern "C" int bx_amr_advance_levels(bx_amr *a, double dt, const double dx0[3], double time,
+                                     double *const *U, double *const *U_tmp,
+                                     double *const *U_old, void *stream)
+{
+    LevelBufs b{ U, U_tmp, U_old };
+    if (!a || !amr_dx_ok(a, dx0) || !(dt > 0) || !bufs_ok(a, b))
+        return set_error(BX_ERR_INVALID, "bx_amr_advance: bad argument");
     hipStream_t s = (hipStream_t)stream;
     int         rc;
-    // post_init: average down, then every ghost cell of both levels
-    if ((rc = avg_down(a, Uf, Uc, s)) != BX_OK || (rc = do_fill_all(a->crse, Uc, s)) != BX_OK
-        || (rc = do_fill_all(a->fine, Uf, s)) != BX_OK)
+    if ((rc = time_step(a, 0, time, dt, time, dt, dx0, b, s, true)) != BX_OK)
         return rc;
+    for (size_t l = 1; l < a->lev.size(); ++l)
+        if ((rc = do_fill_all(a->lev[l], U[l], s)) != BX_OK)
+            return rc;
+    return BX_OK;
+}
+
+extern "C" int bx_amr_est_dt_levels(bx_amr *a, const double dx0[3], double cfl, double time,
+                                    double stop_time, const double *const *U, double *dt,
+                                    void *stream)
+{
+    if (!a || !U || !dt || !amr_dx_ok(a, dx0))
+        return set_error(BX_ERR_INVALID, "bx_amr_est_dt: bad argument");
+    for (size_t l = 0; l < a->lev.size(); ++l)
+        if (!U[l])
+            return set_error(BX_ERR_INVALID, "bx_amr_est_dt: bad argument");
+    return est_dt_levels(a, dx0, cfl, time, stop_time, U, dt, (hipStream_t)stream);
+}
+
+extern "C" int bx_amr_run_levels(bx_amr *a, const double dx0[3], double cfl, double stop_time,
+                                 int max_step, double *const *U, double *const *U_tmp,
+                                 double *const *U_old, double *time, int *steps, void *stream)
+{
+    LevelBufs b{ U, U_tmp, U_old };
+    if (!a || !time || !amr_dx_ok(a, dx0) || !bufs_ok(a, b))
+        return set_error(BX_ERR_INVALID, "bx_amr_run: bad argument");
+    hipStream_t s    = (hipStream_t)stream;
+    const int   nlev = (int)a->lev.size();
+    int         rc;
+    // post_init: average down from the finest level, then every ghost cell
+    for (int l = nlev - 2; l >= 0; --l)
+        if ((rc = avg_down(a->pairs[l], U[l + 1], U[l], s)) != BX_OK)
+            return rc;
+    for (int l = 0; l < nlev; ++l)
+        if ((rc = do_fill_all(a->lev[l], U[l], s)) != BX_OK)
+            return rc;
     double t = *time;
     int    n = 0;
     while ((n < max_step || max_step < 0) && (t < stop_time || stop_time < 0.0))
     {
         double dt;
-        if ((rc = bx_amr_est_dt(a, dx_coarse, cfl, t, stop_time, Uc, Uf, &dt, stream)) != BX_OK)
+        if ((rc = est_dt_levels(a, dx0, cfl, t, stop_time, U, &dt, s)) != BX_OK)
             return rc;
-        if ((rc = amr_advance(a, dt, dx_coarse, t, Uc, Uc_tmp, Uc_old, Uf, Uf_tmp, s, false))
-            != BX_OK)
+        if ((rc = time_step(a, 0, t, dt, t, dt, dx0, b, s, false)) != BX_OK)
             return rc;
         t = t + dt;
         ++n;
     }
-    if ((rc = do_fill_all(a->fine, Uf, s)) != BX_OK)
-        return rc;
+    for (int l = 1; l < nlev; ++l)
+        if ((rc = do_fill_all(a->lev[l], U[l], s)) != BX_OK)
+            return rc;
     BX_HIP_CHECK(hipStreamSynchronize(s));
     *time = t;
     if (steps)
         *steps = n;
     return BX_OK;
+}
+
+// ------------------------------------------------- two-level entry points
+static bool two_levels(const bx_amr *a) { return a && a->lev.size() == 2; }
+
+extern "C" int bx_amr_advance(bx_amr *a, double dt, const double dx_coarse[3], double time,
+                              double *Uc, double *Uc_tmp, double *Uc_old, double *Uf,
+                              double *Uf_tmp, void *stream)
+{
+    if (!two_levels(a))
+        return set_error(BX_ERR_INVALID, "bx_amr_advance: needs a two-level hierarchy");
+    double *U[2] = { Uc, Uf }, *T[2] = { Uc_tmp, Uf_tmp }, *O[2] = { Uc_old, nullptr };
+    return bx_amr_advance_levels(a, dt, dx_coarse, time, U, T, O, stream);
+}
+
+extern "C" int bx_amr_est_dt(bx_amr *a, const double dx_coarse[3], double cfl, double time,
+                             double stop_time, const double *Uc, const double *Uf, double *dt,
+                             void *stream)
+{
+    if (!two_levels(a))
+        return set_error(BX_ERR_INVALID, "bx_amr_est_dt: needs a two-level hierarchy");
+    const double *U[2] = { Uc, Uf };
+    return bx_amr_est_dt_levels(a, dx_coarse, cfl, time, stop_time, U, dt, stream);
+}
+
+extern "C" int bx_amr_run(bx_amr *a, const double dx_coarse[3], double cfl, double stop_time,
+                          int max_step, double *Uc, double *Uc_tmp, double *Uc_old, double *Uf,
+                          double *Uf_tmp, double *time, int *steps, void *stream)
+{
+    if (!two_levels(a))
+        return set_error(BX_ERR_INVALID, "bx_amr_run: needs a two-level hierarchy");
+    double *U[2] = { Uc, Uf }, *T[2] = { Uc_tmp, Uf_tmp }, *O[2] = { Uc_old, nullptr };
+    return bx_amr_run_levels(a, dx_coarse, cfl, stop_time, max_step, U, T, O, time, steps,
+                             stream);
 }

@@ -257,6 +257,7 @@ typedef struct bx_amr bx_amr;
 /* Plans the coarse-fine ghost cells, register faces and covered cells. */
 int  bx_amr_create(bx_ctx *coarse, bx_ctx *fine, bx_amr **out);
 void bx_amr_destroy(bx_amr *a);
+/* Totals over the level pairs. */
 int  bx_amr_info(const bx_amr *a, int *n_cf_segs, int *n_register_faces,
                  int *n_reflux_cells);
 /* AmrLevelAdv::avgDown (AmrLevelAdv.cpp:947-987): covered coarse cells
@@ -286,6 +287,26 @@ int bx_amr_run(bx_amr *a, const double dx_coarse[3], double cfl,
                double stop_time, int max_step, double *Uc, double *Uc_tmp,
                double *Uc_old, double *Uf, double *Uf_tmp, double *time,
                int *steps, void *stream);
+/* Any number of levels (2..16), refinement ratio 2 between neighbours
+ * (amr.max_level = nlev - 1, amr.ref_ratio = 2 ...): levels[l] is level l,
+ * its geometry the level-0 one refined by 2^l.  Level l advances by
+ * dt / 2^l, subcycled recursively as Amr::timeStep does; each level's
+ * post_timestep refluxes and averages down from the level above.  Per
+ * level l: U[l] (state, ghosts filled, in/out), U_tmp[l] (ping-pong),
+ * U_old[l] (scratch for the old state; not used for the finest level).
+ * dx0: level 0's cell sizes; dt: level 0's step.  The two-level calls above
+ * are these with nlev = 2. */
+int bx_amr_create_levels(bx_ctx *const *levels, int nlev, bx_amr **out);
+int bx_amr_advance_levels(bx_amr *a, double dt, const double dx0[3],
+                          double time, double *const *U, double *const *U_tmp,
+                          double *const *U_old, void *stream);
+int bx_amr_est_dt_levels(bx_amr *a, const double dx0[3], double cfl,
+                         double time, double stop_time, const double *const *U,
+                         double *dt, void *stream);
+int bx_amr_run_levels(bx_amr *a, const double dx0[3], double cfl,
+                      double stop_time, int max_step, double *const *U,
+                      double *const *U_tmp, double *const *U_old, double *time,
+                      int *steps, void *stream);
 
 /* ============================================================================
  * Pressure Helmholtz solve of the IMEX schemes (PEEK / PESK) on one 2D

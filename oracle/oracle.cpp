@@ -997,19 +997,19 @@ static double level_dt_est(const Level &L, const double dx[3], double cfl)
     return dt_est * cfl;
 }
 
-static double amr_dt(const Level &C, const Level &F, const double dxc[3], double cfl,
+// computeNewDt over levels: min of 2^l dt_est(l), then the stop clamp.
+static double amr_dt(const std::vector<Level> &L, const double dx0[3], double cfl,
                      double cur_time, double stop_time)
 {
-    double dxf[3];
-    for (int d = 0; d < 3; ++d)
-        dxf[d] = dxc[d] / 2;
-    const double dt_min[2] = { level_dt_est(C, dxc, cfl), level_dt_est(F, dxf, cfl) };
-    double       dt_0      = 1.0e+100;
-    int          n_factor  = 1;
-    for (int i = 0; i < 2; ++i)
+    double dt_0     = 1.0e+100;
+    int    n_factor = 1;
+    for (size_t l = 0; l < L.size(); ++l)
     {
-        n_factor *= (i == 0 ? 1 : 2);
-        dt_0 = std::min(dt_0, n_factor * dt_min[i]);
+        double dxl[3];
+        for (int d = 0; d < 3; ++d)
+            dxl[d] = dx0[d] / (double)(1 << l);
+        n_factor *= (l == 0 ? 1 : 2);
+        dt_0 = std::min(dt_0, n_factor * level_dt_est(L[l], dxl, cfl));
     }
     const double eps = 0.001 * dt_0;
     if (stop_time >= 0.0)
@@ -1018,81 +1018,107 @@ static double amr_dt(const Level &C, const Level &F, const double dxc[3], double
     return dt_0;
 }
 
-static void amr_advance(Level &C, Level &F, const std::vector<AmrSlot> &slots, double dt,
-                        const double dxc[3], double time, int num_method, bool do_reflux)
+struct AmrH
 {
-    const int           D  = C.D;
-    const int           NC = D + 2;
-    std::vector<double> reg(slots.size() * NC, 0.0);
-    double              dxf[3];
-    for (int d = 0; d < 3; ++d)
-        dxf[d] = dxc[d] / 2;
-    auto scale_of = [&](int d, double h, const double *dxl) {
-        double sc = h;
-        for (int e = 0; e < D; ++e)
-            if (e != d)
-                sc *= dxl[e];
-        return sc;
-    };
-    const Level Cold = C;
-    FluxHook crse_hook = [&](int box, int dir, const Fab &flux) {
-        const double sc = scale_of(dir, dt, dxc);
+    std::vector<Level>                L, old;
+    std::vector<std::vector<AmrSlot>> slots; // slots[l]: levels l and l+1
+    std::vector<std::vector<double>>  reg;
+    int                               num_method = 0;
+    bool                              do_reflux  = true;
+};
+
+static void reflux_level(Level &C, const std::vector<AmrSlot> &slots,
+                         const std::vector<double> &reg, const double dxc[3])
+{
+    const int NC  = C.D + 2;
+    double    vol = dxc[0];
+    for (int d = 1; d < C.D; ++d)
+        vol *= dxc[d];
+    for (int orient = 0; orient < 6; ++orient)
         for (size_t q = 0; q < slots.size(); ++q)
-            if (slots[q].d == dir && slots[q].cbox == box)
-                for (int m = 0; m < NC; ++m)
-                    reg[q * NC + m] = -1.0 * (flux(slots[q].cface[0], slots[q].cface[1],
-                                                   slots[q].cface[2], m)
-                                              * sc);
-    };
-    advance_explicit(C, dt, dxc, num_method, &crse_hook);
-    const double dtf   = dt / 2;
-    const double t_new = time + dt;
-    FluxHook fine_hook = [&](int box, int dir, const Fab &flux) {
-        const double sf   = scale_of(dir, dtf, dxf);
-        const int    nsub = 1 << (D - 1);
-        for (size_t q = 0; q < slots.size(); ++q)
-            if (slots[q].d == dir && slots[q].fbox == box)
-                for (int c = 0; c < nsub; ++c)
-                    for (int m = 0; m < NC; ++m)
-                        reg[q * NC + m]
-                            = reg[q * NC + m]
-                              + flux(slots[q].fface[c][0], slots[q].fface[c][1],
-                                     slots[q].fface[c][2], m)
-                                    * sf;
-    };
-    for (int sub = 0; sub < 2; ++sub)
-    {
-        const double tf = sub == 0 ? time : time + dtf;
-        fill_consv_boundary(F);
-        if (tf == time)
-            cf_fill(F, Cold, C, 0, 1.0, 0.0);
-        else
-            cf_fill(F, Cold, C, 1, (t_new - tf) / (t_new - time), (tf - time) / (t_new - time));
-        advance_explicit(F, dtf, dxf, num_method, &fine_hook);
-    }
-    if (do_reflux)
-    {
-        double vol = dxc[0];
-        for (int d = 1; d < D; ++d)
-            vol *= dxc[d];
-        for (int orient = 0; orient < 6; ++orient)
-            for (size_t q = 0; q < slots.size(); ++q)
+        {
+            const AmrSlot &s = slots[q];
+            if (s.orient != orient)
+                continue;
+            for (int m = 0; m < NC; ++m)
             {
-                const AmrSlot &s = slots[q];
-                if (s.orient != orient)
-                    continue;
-                for (int m = 0; m < NC; ++m)
-                {
-                    double &u = C.fabs[s.tbox](s.tcell[0], s.tcell[1], s.tcell[2], m);
-                    u         = u + (s.sign * reg[q * NC + m]) / vol;
-                }
+                double &u = C.fabs[s.tbox](s.tcell[0], s.tcell[1], s.tcell[2], m);
+                u         = u + (s.sign * reg[q * NC + m]) / vol;
             }
-    }
-    avg_down(C, F);
-    fill_consv_boundary(C);
-    fill_consv_boundary(F);
+        }
 }
 
+// Amr::timeStep(l): advance level l, subcycle the finer levels twice, then
+// reflux and avgDown (post_timestep).  t_par, dt_par: the parent's step.
+static void amr_time_step(AmrH &h, int l, double time, double dt, double t_par,
+                          double dt_par, const double dx0[3])
+{
+    const int nlev = (int)h.L.size();
+    const int D    = h.L[0].D;
+    const int NC   = D + 2;
+    double    dxl[3];
+    for (int d = 0; d < 3; ++d)
+        dxl[d] = dx0[d] / (double)(1 << l);
+    if (l > 0)
+    {
+        fill_consv_boundary(h.L[l]);
+        const double t_new = t_par + dt_par;
+        if (time == t_par)
+            cf_fill(h.L[l], h.old[l - 1], h.L[l - 1], 0, 1.0, 0.0);
+        else
+            cf_fill(h.L[l], h.old[l - 1], h.L[l - 1], 1, (t_new - time) / (t_new - t_par),
+                    (time - t_par) / (t_new - t_par));
+    }
+    if (l + 1 < nlev)
+        h.old[l] = h.L[l];
+    auto scale_of = [&](int d, double hstep, const double *dxx) {
+        double sc = hstep;
+        for (int e = 0; e < D; ++e)
+            if (e != d)
+                sc *= dxx[e];
+        return sc;
+    };
+    FluxHook hook = [&](int box, int dir, const Fab &flux) {
+        const double sc = scale_of(dir, dt, dxl);
+        if (l > 0) // fine side of the pair below: FineAdd(+1)
+        {
+            const std::vector<AmrSlot> &sl   = h.slots[l - 1];
+            std::vector<double>        &reg  = h.reg[l - 1];
+            const int                   nsub = 1 << (D - 1);
+            for (size_t q = 0; q < sl.size(); ++q)
+                if (sl[q].d == dir && sl[q].fbox == box)
+                    for (int c = 0; c < nsub; ++c)
+                        for (int m = 0; m < NC; ++m)
+                            reg[q * NC + m]
+                                = reg[q * NC + m]
+                                  + flux(sl[q].fface[c][0], sl[q].fface[c][1],
+                                         sl[q].fface[c][2], m)
+                                        * sc;
+        }
+        if (l + 1 < nlev) // coarse side of the pair above: CrseInit(-1)
+        {
+            const std::vector<AmrSlot> &sl  = h.slots[l];
+            std::vector<double>        &reg = h.reg[l];
+            for (size_t q = 0; q < sl.size(); ++q)
+                if (sl[q].d == dir && sl[q].cbox == box)
+                    for (int m = 0; m < NC; ++m)
+                        reg[q * NC + m]
+                            = -1.0
+                              * (flux(sl[q].cface[0], sl[q].cface[1], sl[q].cface[2], m) * sc);
+        }
+    };
+    advance_explicit(h.L[l], dt, dxl, h.num_method, &hook);
+    if (l + 1 < nlev)
+    {
+        const double dtf = dt / 2;
+        for (int sub = 0; sub < 2; ++sub)
+            amr_time_step(h, l + 1, sub == 0 ? time : time + dtf, dtf, time, dt, dx0);
+        if (h.do_reflux)
+            reflux_level(h.L[l], h.slots[l], h.reg[l], dxl);
+        avg_down(h.L[l], h.L[l + 1]);
+        fill_consv_boundary(h.L[l]);
+    }
+}
 } // namespace oracle
 
 // ===================================================================== C ABI
@@ -1385,9 +1411,81 @@ double or_pow_invz_fn(double x, double gamma)
     return or_pow_invz(x, 1 / ((gamma - 1) / (2 * gamma)), gamma);
 }
 
-// Two levels (ratio 2): the main.cpp loop with Amr::timeStep subcycling,
-// post_init avgDown first.  Coarse boxes/U as or_run; fine boxes in fine
-// index space, fine domain = coarse domain refined by 2, same BCs.
+// nlev levels (ratio 2): the main.cpp loop with Amr::timeStep subcycling,
+// post_init avgDown first.  Level l: boxes[l] (nbox[l] x 6 ints, level-l
+// index space), packed state U[l] with FAB offsets offsets[l]; level 0's
+// domain dlo..dhi, level l's refined by 2^l, same BCs.
+int or_amr_run_levels(const or_params *p, int nlev, const int *const *boxes,
+                      const int *nbox, int ng, const int *dlo, const int *dhi,
+                      const int *bclo, const int *bchi, const double *dx, double cfl,
+                      double start_time, double stop_time, int max_step, double *const *U,
+                      const int64_t *const *offsets, int do_reflux, double *time_out,
+                      int *steps_out)
+{
+    AmrH h;
+    h.L.resize(nlev);
+    h.old.resize(nlev);
+    h.num_method = p->num_method;
+    h.do_reflux  = do_reflux != 0;
+    for (int l = 0; l < nlev; ++l)
+    {
+        int ldlo[3], ldhi[3];
+        for (int d = 0; d < 3; ++d)
+        {
+            ldlo[d] = dlo[d] << l;
+            ldhi[d] = ((dhi[d] + 1) << l) - 1;
+        }
+        make_level(h.L[l], p, boxes[l], nbox[l], ng, ldlo, ldhi, bclo, bchi);
+        level_in(h.L[l], U[l], offsets[l]);
+    }
+    // proper nesting (every coarse cell within 2 of a coarsened fine box is
+    // a valid cell of the coarser level, periodic images included)
+    for (int l = 0; l + 1 < nlev; ++l)
+        for (const Box &v : h.L[l + 1].valid)
+        {
+            Box g;
+            for (int d = 0; d < 3; ++d)
+            {
+                g.lo[d] = d < p->dim ? (v.lo[d] >> 1) - 2 : 0;
+                g.hi[d] = d < p->dim ? (v.hi[d] >> 1) + 2 : 0;
+            }
+            bool ok = true;
+            for_box(g, [&](int i, int j, int k) {
+                int c[3] = { i, j, k };
+                if (wrap_level(h.L[l], c) && box_of(h.L[l], c) < 0)
+                    ok = false;
+            });
+            if (!ok)
+                return 1;
+        }
+    for (int l = 0; l + 1 < nlev; ++l)
+    {
+        h.slots.push_back(amr_slots(h.L[l], h.L[l + 1]));
+        h.reg.push_back(std::vector<double>(h.slots.back().size() * (p->dim + 2), 0.0));
+    }
+    for (int l = nlev - 2; l >= 0; --l)
+        avg_down(h.L[l], h.L[l + 1]);
+    for (int l = 0; l < nlev; ++l)
+        fill_consv_boundary(h.L[l]);
+    double time  = start_time;
+    int    steps = 0;
+    while ((steps < max_step || max_step < 0)
+           && (time < stop_time || stop_time < 0.0))
+    {
+        const double dt = amr_dt(h.L, dx, cfl, time, stop_time);
+        amr_time_step(h, 0, time, dt, time, dt, dx);
+        time = time + dt;
+        ++steps;
+    }
+    for (int l = 0; l < nlev; ++l)
+        level_out(h.L[l], U[l], offsets[l]);
+    if (time_out)
+        *time_out = time;
+    if (steps_out)
+        *steps_out = steps;
+    return 0;
+}
+
 int or_amr_run(const or_params *p, const int *cboxes, int ncbox, const int *fboxes,
                int nfbox, int ng, const int *dlo, const int *dhi, const int *bclo,
                const int *bchi, const double *dx, double cfl, double start_time,
@@ -1395,38 +1493,12 @@ int or_amr_run(const or_params *p, const int *cboxes, int ncbox, const int *fbox
                double *Uf, const int64_t *foffsets, int do_reflux, double *time_out,
                int *steps_out)
 {
-    Level C, F;
-    int   fdlo[3], fdhi[3];
-    for (int d = 0; d < 3; ++d)
-    {
-        fdlo[d] = 2 * dlo[d];
-        fdhi[d] = 2 * dhi[d] + 1;
-    }
-    make_level(C, p, cboxes, ncbox, ng, dlo, dhi, bclo, bchi);
-    make_level(F, p, fboxes, nfbox, ng, fdlo, fdhi, bclo, bchi);
-    level_in(C, Uc, coffsets);
-    level_in(F, Uf, foffsets);
-    const std::vector<AmrSlot> slots = amr_slots(C, F);
-    avg_down(C, F);
-    fill_consv_boundary(C);
-    fill_consv_boundary(F);
-    double time  = start_time;
-    int    steps = 0;
-    while ((steps < max_step || max_step < 0)
-           && (time < stop_time || stop_time < 0.0))
-    {
-        const double dt = amr_dt(C, F, dx, cfl, time, stop_time);
-        amr_advance(C, F, slots, dt, dx, time, p->num_method, do_reflux != 0);
-        time = time + dt;
-        ++steps;
-    }
-    level_out(C, Uc, coffsets);
-    level_out(F, Uf, foffsets);
-    if (time_out)
-        *time_out = time;
-    if (steps_out)
-        *steps_out = steps;
-    return 0;
+    const int     *b[2] = { cboxes, fboxes };
+    const int      n[2] = { ncbox, nfbox };
+    double        *U[2] = { Uc, Uf };
+    const int64_t *o[2] = { coffsets, foffsets };
+    return or_amr_run_levels(p, 2, b, n, ng, dlo, dhi, bclo, bchi, dx, cfl, start_time,
+                             stop_time, max_step, U, o, do_reflux, time_out, steps_out);
 }
 
 // The fine level's ghost cells at `time` between coarse states at t_old and

@@ -42,6 +42,7 @@ D = ctypes.POINTER(ctypes.c_double)
 I = ctypes.POINTER(ctypes.c_int)
 I64 = ctypes.POINTER(ctypes.c_int64)
 PP = ctypes.POINTER(or_params)
+P_ = ctypes.c_void_p
 _SIGS = {
     "or_run": [PP, I, ctypes.c_int, ctypes.c_int, I, I, I, I, D, ctypes.c_double, ctypes.c_double,
                ctypes.c_double, ctypes.c_int, D, I64, D, I],
@@ -58,6 +59,8 @@ _SIGS = {
     "or_hllc_face": [ctypes.c_int, ctypes.c_int, D, D, ctypes.c_double, ctypes.c_double, D],
     "or_amr_run": [PP, I, ctypes.c_int, I, ctypes.c_int, ctypes.c_int, I, I, I, I, D, ctypes.c_double,
                    ctypes.c_double, ctypes.c_double, ctypes.c_int, D, I64, D, I64, ctypes.c_int, D, I],
+    "or_amr_run_levels": [PP, ctypes.c_int, P_, I, ctypes.c_int, I, I, I, I, D, ctypes.c_double,
+                          ctypes.c_double, ctypes.c_double, ctypes.c_int, P_, P_, ctypes.c_int, D, I],
     "or_amr_fill_fine": [PP, I, ctypes.c_int, I, ctypes.c_int, ctypes.c_int, I, I, I, I, D, D, I64, D,
                          I64, ctypes.c_double, ctypes.c_double, ctypes.c_double],
 }
@@ -152,6 +155,31 @@ class Oracle:
                             foffs.ctypes.data_as(I64), int(bool(do_reflux)), ctypes.byref(t),
                             ctypes.byref(n))
         return t.value, n.value
+
+    def amr_run_levels(self, p, layouts, Us, domain_lo, domain_hi, bc_lo, bc_hi, dx, cfl,
+                       stop_time=-1.0, max_step=-1, start_time=0.0, do_reflux=True):
+        """Any number of levels, ratio 2 (or_amr_run_levels); each U updated
+        in place.  domain_lo/hi and dx are level 0's."""
+        n = len(layouts)
+        geo = [self._geom(lay, domain_lo, domain_hi, bc_lo, bc_hi) for lay in layouts]
+        barrs = (ctypes.c_void_p * n)(*[ctypes.cast(g[0], ctypes.c_void_p) for g in geo])
+        nbox = (ctypes.c_int * n)(*[len(lay.boxes) for lay in layouts])
+        Ups = (ctypes.c_void_p * n)(*[U.ctypes.data for U in Us])
+        offs = [g[5] for g in geo]
+        Ops = (ctypes.c_void_p * n)(*[o.ctypes.data for o in offs])
+        _, dlo, dhi, bl, bh, _ = geo[0]
+        t = ctypes.c_double()
+        k = ctypes.c_int()
+        dxa = np.array(list(dx) + [0.0] * (3 - len(dx)), dtype=np.float64)
+        rc = self.lib.or_amr_run_levels(ctypes.byref(p), n, ctypes.cast(barrs, ctypes.c_void_p),
+                                        nbox, layouts[0].ng, dlo, dhi, bl, bh, _dp(dxa), cfl,
+                                        start_time, stop_time, max_step,
+                                        ctypes.cast(Ups, ctypes.c_void_p),
+                                        ctypes.cast(Ops, ctypes.c_void_p), int(bool(do_reflux)),
+                                        ctypes.byref(t), ctypes.byref(k))
+        if rc != 0:
+            raise ValueError("or_amr_run_levels: levels not properly nested")
+        return t.value, k.value
 
     def amr_fill_fine(self, p, clayout, Uc_old, Uc_new, flayout, Uf, domain_lo, domain_hi, bc_lo,
                       bc_hi, t_old, t_new, time):

@@ -217,3 +217,61 @@ def test_cf_time_interpolation_weights(oracle_bxpow):
             assert abs(flay.fab(res[0.25], b)[(m,) + idx] - a0 - 0.75) <= 1e-14
             assert abs(flay.fab(res[1.0], b)[(m,) + idx] - a0 - 3.0) <= 1e-14
     assert len(cells) > 20
+
+
+THREE_LEVELS = {
+    # level-1 and level-2 regions in their own index spaces (nested by >= 2
+    # parent cells), level 0 = 32^2
+    "2d": dict(make=vortex2d, ncell=32, mgs=16,
+               regions=[((8, 12), (55, 51)), ((32, 40), (79, 71))], mgss=[16, 16]),
+    # level 1 spans the periodic x direction; level 2 touches the periodic
+    # edge, so its coarse-fine cells there wrap around
+    "2d_periodic_edge": dict(make=vortex2d, ncell=32, mgs=16,
+                             regions=[((0, 8), (63, 55)), ((0, 24), (39, 87))], mgss=[16, 16]),
+}
+
+
+@pytest.mark.parametrize("case", list(THREE_LEVELS))
+def test_three_levels_conserve_and_need_reflux(oracle_bxpow, case):
+    """Three levels (ratio 2): the composite totals stay constant to rounding
+    with refluxing at both coarse-fine interfaces, and drift without it."""
+    c = THREE_LEVELS[case]
+    s = amr.refine_levels(c["make"], c["ncell"], c["mgs"], c["regions"], c["mgss"])
+    op = oparams(s)
+    drift = {}
+    for rf in (True, False):
+        Us = [U.copy() for U in s["Us"]]
+        oracle_bxpow.amr_run_levels(op, s["layouts"], Us, s["domain_lo"], s["domain_hi"], s["bc"],
+                                    s["bc"], s["dx"], 0.9, max_step=0)
+        tot0 = amr.composite_totals_levels(s["layouts"], Us, s["dx"])
+        t, n = oracle_bxpow.amr_run_levels(op, s["layouts"], Us, s["domain_lo"], s["domain_hi"],
+                                           s["bc"], s["bc"], s["dx"], 0.9, max_step=6,
+                                           do_reflux=rf)
+        assert n == 6
+        tot1 = amr.composite_totals_levels(s["layouts"], Us, s["dx"])
+        drift[rf] = np.abs(tot1 - tot0) / np.abs(tot0).max()
+    assert np.all(drift[True] <= 2e-14), drift
+    assert np.max(drift[False]) > 1e-9, drift
+
+
+def test_two_level_entry_equals_levels_entry(oracle_bxpow):
+    """or_amr_run (two levels) is or_amr_run_levels with nlev = 2."""
+    s = amr.refine_setup(vortex2d, 32, 16, (6, 10), (21, 25), fine_max_grid_size=8)
+    op = oparams(s)
+    Uc, Uf = s["U"].copy(), s["Uf"].copy()
+    r1 = oracle_bxpow.amr_run(op, s["layout"], Uc, s["fine_layout"], Uf, s["domain_lo"],
+                              s["domain_hi"], s["bc"], s["bc"], s["dx"], 0.9, max_step=4)
+    Us = [s["U"].copy(), s["Uf"].copy()]
+    r2 = oracle_bxpow.amr_run_levels(op, [s["layout"], s["fine_layout"]], Us, s["domain_lo"],
+                                     s["domain_hi"], s["bc"], s["bc"], s["dx"], 0.9, max_step=4)
+    assert r1 == r2 and np.array_equal(Uc, Us[0]) and np.array_equal(Uf, Us[1])
+
+
+def test_oracle_refuses_improper_nesting(oracle_bxpow):
+    c = THREE_LEVELS["2d"]
+    s = amr.refine_levels(c["make"], c["ncell"], c["mgs"], [c["regions"][0], ((18, 40), (79, 71))],
+                          c["mgss"])
+    with pytest.raises(ValueError):
+        oracle_bxpow.amr_run_levels(oparams(s), s["layouts"], [U.copy() for U in s["Us"]],
+                                    s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], s["dx"], 0.9,
+                                    max_step=1)

@@ -169,3 +169,39 @@ def test_amr_create_rejects_bad_hierarchies():
         with pytest.raises(BamrexError):
             amr.AmrHierarchy(gparams(s), s["layout"], flay, s["domain_lo"], s["domain_hi"],
                              s["bc"], s["bc"]).close()
+
+
+
+@pytest.mark.parametrize("case", ["2d", "2d_periodic_edge"])
+def test_three_level_run_parity(oracle_bxpow, case):
+    """Three levels, recursively subcycled: bit-exact against the oracle."""
+    from test_amr_oracle import THREE_LEVELS
+    c = THREE_LEVELS[case]
+    s = amr.refine_levels(c["make"], c["ncell"], c["mgs"], c["regions"], c["mgss"])
+    announce("amr 3 levels %s" % case)
+    h = amr.AmrLevels(gparams(s), s["layouts"], s["domain_lo"], s["domain_hi"], s["bc"], s["bc"])
+    try:
+        h.set_state(s["Us"])
+        tg, ng = h.run(s["dx"], 0.9, max_step=5)
+        got = h.get_state()
+    finally:
+        h.close()
+    Us = [U.copy() for U in s["Us"]]
+    to, no = oracle_bxpow.amr_run_levels(oparams(s), s["layouts"], Us, s["domain_lo"],
+                                         s["domain_hi"], s["bc"], s["bc"], s["dx"], 0.9,
+                                         max_step=5)
+    assert (tg, ng) == (to, no)
+    for l, lay in enumerate(s["layouts"]):
+        assert_equal_valid(lay, got[l], Us[l], "%s level %d" % (case, l))
+
+
+def test_improperly_nested_level_is_refused():
+    """A level-2 box reaching within 2 level-1 cells of the level-1 region's
+    edge (inside the domain) is refused at set-up."""
+    from bamrex_amd._lib import BamrexError
+    from test_amr_oracle import THREE_LEVELS
+    c = dict(THREE_LEVELS["2d"])
+    c["regions"] = [((8, 12), (55, 51)), ((18, 40), (79, 71))]
+    s = amr.refine_levels(c["make"], c["ncell"], c["mgs"], c["regions"], c["mgss"])
+    with pytest.raises(BamrexError):
+        amr.AmrLevels(gparams(s), s["layouts"], s["domain_lo"], s["domain_hi"], s["bc"], s["bc"])

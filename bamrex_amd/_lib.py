@@ -153,6 +153,22 @@ _SIGS = {
     "bx_imex_peek_finish": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                            P, P]),
+    "bx_imex_stage_assemble": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, P, P, P, P, P, P,
+                                              P]),
+    "bx_imex_stage_fluxes": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_int, P, P, P]),
+    "bx_imex_rk_combine": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int, DP, DP,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, P, P,
+                                          P]),
+    "bx_imex_rk_pressure": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, P,
+                                           P]),
+    "bx_imex_rk_update": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int, DP,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, P, P]),
     "bx_cnn_create": (ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(P)]),
     "bx_cnn_destroy": (None, [P]),
     "bx_cnn_set_params": (ctypes.c_int, [P, P, ctypes.c_int]),
@@ -170,6 +186,7 @@ _SIGS = {
 
 BX_SOLVE = {"mg": 0, "fgmres_mg": 1, "fgmres_cnn": 2, "mg_fused": 3}
 BX_IMEX_ADVECTION = {"rusanov": 0, "muscl_rusanov": 1}
+BX_IMEX_MAX_STAGES = 4
 BX_CNN_MODE = {"smoother": 0, "corrector": 1}
 BX_CNN_NPARAMS = 4896
 

@@ -26,10 +26,13 @@
 //
 // Training loss on crops of (residual, error) pairs harvested from plain
 // multigrid solves (A e = r):
-//     L = sum_crops  ||mask (r - A e_cnn)||^2 / ||mask r||^2
+//     L = sum_crops  ||mask (r - A e_cnn)||^2 / max(||mask r||^2, |mask| sigma^2)
 // (mask = crop interior at distance >= 5 from the crop edge, where the 9x9
-// receptive field sees no padding), i.e. the A^2-norm of the remaining error
-// e - e_cnn.  Gradients: dL/de_cnn = -2 A^T (mask res) (A symmetric for PEEK),
+// receptive field sees no padding, sigma = rms over the crop), i.e. the
+// A^2-norm of the remaining error e - e_cnn.  The floor keeps a crop whose
+// residual sits in its border (seen by the network, outside the mask) from
+// dividing by a near-zero norm: such crops made single-batch spikes of the
+// loss of 1e9 and more.  Gradients: dL/de_cnn = -2 A^T (mask res) (A symmetric for PEEK),
 // then back through the convolutions; Adam on the FP32 master weights.
 #include <hip/hip_runtime.h>
 
@@ -344,7 +347,9 @@ __device__ inline long long gidx(const CropSet &c, int f, int x, int y)
     return ((long long)f * c.ny + y) * c.nx + x;
 }
 
-// per crop: sigma (rms over the crop) and ||mask r||^2
+// per crop: sigma (rms over the crop) and the loss normaliser
+// max(||mask r||^2, |mask| sigma^2): floored, since a crop whose residual
+// sits in its border (which the network sees) has a near-zero masked norm
 __global__ void k_crop_stats(CropSet c, double *__restrict__ sig,
                              double *__restrict__ rn2, int border)
 {
@@ -374,8 +379,10 @@ __global__ void k_crop_stats(CropSet c, double *__restrict__ sig,
     }
     if (threadIdx.x == 0)
     {
-        sig[b] = sqrt(s1[0] / (double)(c.cs * c.cs));
-        rn2[b] = s2[0];
+        const int    in    = c.cs - 2 * border;
+        const double floor = s1[0] * ((double)(in * in) / (double)(c.cs * c.cs));
+        sig[b]             = sqrt(s1[0] / (double)(c.cs * c.cs));
+        rn2[b]             = s2[0] < floor ? floor : s2[0];
     }
 }
 

@@ -87,10 +87,10 @@ __device__ __forceinline__ double peek_minmod(double dl, double dr)
 }
 
 // MUSCL states of cell c in direction DIR (advance_MUSCL_rusanov_adv,
-// IMEX_Fluxes.cpp:183-250): minmod half-slopes, then the SP111 local half
-// step with the states' own advective fluxes; L and R hold the updated
-// states with their fluxes.
-template <int DIR>
+// IMEX_Fluxes.cpp:183-250): minmod half-slopes, then with LOCAL (the SP111
+// local time step) a half step with the states' own advective fluxes; L
+// and R hold the states with their fluxes.
+template <int DIR, bool LOCAL>
 __device__ __forceinline__ void muscl_states(const AdvCell<DIR> &m, const AdvCell<DIR> &c,
                                              const AdvCell<DIR> &p, double hdtdx, double eps,
                                              AdvCell<DIR> &L, AdvCell<DIR> &R)
@@ -102,14 +102,17 @@ __device__ __forceinline__ void muscl_states(const AdvCell<DIR> &m, const AdvCel
         L.U[k]         = c.U[k] - h;
         R.U[k]         = c.U[k] + h;
     }
-    L.flux(eps);
-    R.flux(eps);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
+    if (LOCAL)
     {
-        const double lu = hdtdx * (R.F[k] - L.F[k]);
-        L.U[k]          = L.U[k] - lu;
-        R.U[k]          = R.U[k] - lu;
+        L.flux(eps);
+        R.flux(eps);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const double lu = hdtdx * (R.F[k] - L.F[k]);
+            L.U[k]          = L.U[k] - lu;
+            R.U[k]          = R.U[k] - lu;
+        }
     }
     L.flux(eps);
     R.flux(eps);
@@ -132,7 +135,7 @@ __device__ __forceinline__ void rusanov(const AdvCell<DIR> &m, const AdvCell<DIR
 // cells at x-2 .. x+2 (offsets in elements of the periodic neighbours).
 // First order (advance_rusanov_adv) or MUSCL (advance_MUSCL_rusanov_adv):
 // the face between cells m and c takes R of m and L of c.
-template <int DIR, bool MUSCL>
+template <int DIR, bool MUSCL, bool LOCAL>
 __device__ __forceinline__ void dir_faces(const double *__restrict__ U, long long n,
                                           long long cm2, long long cm1, long long c0,
                                           long long cp1, long long cp2, double eps,
@@ -152,16 +155,41 @@ __device__ __forceinline__ void dir_faces(const double *__restrict__ U, long lon
     a0.load(U, cm2, n, eps);
     a4.load(U, cp2, n, eps);
     AdvCell<DIR> L1, R1, L2, R2, L3, R3;
-    muscl_states(a0, a1, a2, hdtdx, eps, L1, R1);
-    muscl_states(a1, a2, a3, hdtdx, eps, L2, R2);
-    muscl_states(a2, a3, a4, hdtdx, eps, L3, R3);
+    muscl_states<DIR, LOCAL>(a0, a1, a2, hdtdx, eps, L1, R1);
+    muscl_states<DIR, LOCAL>(a1, a2, a3, hdtdx, eps, L2, R2);
+    muscl_states<DIR, LOCAL>(a2, a3, a4, hdtdx, eps, L3, R3);
     rusanov(R1, L2, Fl);
     rusanov(R2, L3, Fr);
 }
 
+// The Rusanov flux at the lower face of cell c0 along DIR (between cm1 and
+// c0), from the cells cm2 .. cp1; the same arithmetic as dir_faces' Fl.
+template <int DIR, bool MUSCL, bool LOCAL>
+__device__ __forceinline__ void low_face(const double *__restrict__ U, long long n,
+                                         long long cm2, long long cm1, long long c0,
+                                         long long cp1, double eps, double hdtdx,
+                                         double (&F)[4])
+{
+    AdvCell<DIR> a1, a2;
+    a1.load(U, cm1, n, eps);
+    a2.load(U, c0, n, eps);
+    if (!MUSCL)
+    {
+        rusanov(a1, a2, F);
+        return;
+    }
+    AdvCell<DIR> a0, a3;
+    a0.load(U, cm2, n, eps);
+    a3.load(U, cp1, n, eps);
+    AdvCell<DIR> L1, R1, L2, R2;
+    muscl_states<DIR, LOCAL>(a0, a1, a2, hdtdx, eps, L1, R1);
+    muscl_states<DIR, LOCAL>(a1, a2, a3, hdtdx, eps, L2, R2);
+    rusanov(R1, L2, F);
+}
+
 // Rusanov fluxes at the four faces of cell (i, j); hdtdx/hdtdy: half of
 // dt/dx, dt/dy (the MUSCL local step).
-template <bool MUSCL>
+template <bool MUSCL, bool LOCAL>
 __device__ __forceinline__ void cell_faces(const double *__restrict__ U, const PeekGeom &g,
                                            int i, int j, double eps, double hdtdx,
                                            double hdtdy, double (&Fxl)[4], double (&Fxr)[4],
@@ -174,17 +202,19 @@ __device__ __forceinline__ void cell_faces(const double *__restrict__ U, const P
     const int       ip2 = MUSCL ? wrap_hi(ip1, g.nx) : ip1;
     const int       jm2 = MUSCL ? wrap_lo(jm1, g.ny) : jm1;
     const int       jp2 = MUSCL ? wrap_hi(jp1, g.ny) : jp1;
-    dir_faces<0, MUSCL>(U, g.n, row + im2, row + im1, row + i, row + ip1, row + ip2, eps, hdtdx,
-                        Fxl, Fxr);
+    dir_faces<0, MUSCL, LOCAL>(U, g.n, row + im2, row + im1, row + i, row + ip1, row + ip2, eps,
+                               hdtdx, Fxl, Fxr);
     const long long nx = g.nx;
-    dir_faces<1, MUSCL>(U, g.n, jm2 * nx + i, jm1 * nx + i, row + i, jp1 * nx + i, jp2 * nx + i,
-                        eps, hdtdy, Fyl, Fyr);
+    dir_faces<1, MUSCL, LOCAL>(U, g.n, jm2 * nx + i, jm1 * nx + i, row + i, jp1 * nx + i,
+                               jp2 * nx + i, eps, hdtdy, Fyl, Fyr);
 }
 
-// Explicit advective update (IMEX_Fluxes.cpp:92-111):
-//   Uex = U - (dtdx (Fx(i+1) - Fx(i)) + dtdy (Fy(j+1) - Fy(j)))
-template <bool MUSCL>
-__global__ __launch_bounds__(256) void k_peek_advect(const double *__restrict__ U,
+// Explicit advective update (IMEX_Fluxes.cpp:92-111): the fluxes of the
+// explicit stage state Ue applied to the implicit stage state Ui,
+//   Uex = Ui - (dtdx (Fx(i+1) - Fx(i)) + dtdy (Fy(j+1) - Fy(j)))
+template <bool MUSCL, bool LOCAL>
+__global__ __launch_bounds__(256) void k_peek_advect(const double *__restrict__ Ue,
+                                                     const double *__restrict__ Ui,
                                                      double *__restrict__ Uex, PeekGeom g,
                                                      double eps, double dtdx, double dtdy)
 {
@@ -193,11 +223,11 @@ __global__ __launch_bounds__(256) void k_peek_advect(const double *__restrict__ 
     if (i >= g.nx || j >= g.ny)
         return;
     double Fxl[4], Fxr[4], Fyl[4], Fyr[4];
-    cell_faces<MUSCL>(U, g, i, j, eps, 0.5 * dtdx, 0.5 * dtdy, Fxl, Fxr, Fyl, Fyr);
+    cell_faces<MUSCL, LOCAL>(Ue, g, i, j, eps, 0.5 * dtdx, 0.5 * dtdy, Fxl, Fxr, Fyl, Fyr);
     const long long c = (long long)j * g.nx + i;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        Uex[c + k * g.n] = U[c + k * g.n] - (dtdx * (Fxr[k] - Fxl[k]) + dtdy * (Fyr[k] - Fyl[k]));
+        Uex[c + k * g.n] = Ui[c + k * g.n] - (dtdx * (Fxr[k] - Fxl[k]) + dtdy * (Fyr[k] - Fyl[k]));
 }
 
 // enth = p gamma / ((gamma - 1) rho_ex)   (compute_enthalpy, reuse_pressure)
@@ -268,21 +298,23 @@ __device__ __forceinline__ void peek_correct(double (&F)[4], const double *__res
             - hdtde * ((hc + hm) * (p[c] - p[m]));
 }
 
-// U_new = U - (dtdx (Fx(i+1) - Fx(i)) + dtdy (Fy(j+1) - Fy(j)))
-// (conservative_update with the SP111 weight b = 1, IMEX_RK.cpp:22-96)
+// SP111 step end: the corrected fluxes and, with the weight b = 1 of
+// sum_fluxes, the conservative update (Update.H:100-130)
+//   U_new = U - dtdx (Fx(i+1) - Fx(i)) - dtdy (Fy(j+1) - Fy(j)),
+// dtdx = dt (1/dx); the fluxes are recomputed per cell, not stored.
 template <bool MUSCL>
 __global__ __launch_bounds__(256) void k_peek_finish(
     const double *__restrict__ U, const double *__restrict__ Uex,
     const double *__restrict__ enth, const double *__restrict__ p, double *__restrict__ Unew,
-    PeekGeom g, double eps, double dtdx, double dtdy, double heps, double hdtdxe,
-    double hdtdye)
+    PeekGeom g, double eps, double hdtdx_adv, double hdtdy_adv, double dtdx, double dtdy,
+    double heps, double hdtdxe, double hdtdye)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= g.nx || j >= g.ny)
         return;
     double Fxl[4], Fxr[4], Fyl[4], Fyr[4];
-    cell_faces<MUSCL>(U, g, i, j, eps, 0.5 * dtdx, 0.5 * dtdy, Fxl, Fxr, Fyl, Fyr);
+    cell_faces<MUSCL, MUSCL>(U, g, i, j, eps, hdtdx_adv, hdtdy_adv, Fxl, Fxr, Fyl, Fyr);
     const long long row = (long long)j * g.nx;
     const long long c   = row + i;
     const long long cw  = row + wrap_lo(i, g.nx);
@@ -295,7 +327,150 @@ __global__ __launch_bounds__(256) void k_peek_finish(
     peek_correct<1>(Fyr, Uex, enth, p, c, cn, g.n, heps, hdtdye);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        Unew[c + k * g.n] = U[c + k * g.n] - (dtdx * (Fxr[k] - Fxl[k]) + dtdy * (Fyr[k] - Fyl[k]));
+        Unew[c + k * g.n] = U[c + k * g.n] - dtdx * (Fxr[k] - Fxl[k]) - dtdy * (Fyr[k] - Fyl[k]);
+}
+
+// Stage fluxes after the solve (compute_flux_imex_o1's flux output,
+// IMEX_O1.cpp:150-174): each cell writes its lower x face and lower y face,
+// Fx[k n + c] the face between (i-1, j) and (i, j), Fy[k n + c] the face
+// between (i, j-1) and (i, j).
+template <bool MUSCL, bool LOCAL>
+__global__ __launch_bounds__(256) void k_stage_fluxes(
+    const double *__restrict__ Ue, const double *__restrict__ Uex,
+    const double *__restrict__ enth, const double *__restrict__ p, double *__restrict__ Fx,
+    double *__restrict__ Fy, PeekGeom g, double eps, double hdtdx_adv, double hdtdy_adv,
+    double heps, double hdtdxe, double hdtdye)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    const long long nx  = g.nx;
+    const long long row = (long long)j * nx;
+    const long long c   = row + i;
+    const int       im1 = wrap_lo(i, g.nx), jm1 = wrap_lo(j, g.ny);
+    const int       im2 = MUSCL ? wrap_lo(im1, g.nx) : im1;
+    const int       jm2 = MUSCL ? wrap_lo(jm1, g.ny) : jm1;
+    const int       ip1 = MUSCL ? wrap_hi(i, g.nx) : i;
+    const int       jp1 = MUSCL ? wrap_hi(j, g.ny) : j;
+    double          F[4];
+    low_face<0, MUSCL, LOCAL>(Ue, g.n, row + im2, row + im1, c, row + ip1, eps, hdtdx_adv, F);
+    peek_correct<0>(F, Uex, enth, p, row + im1, c, g.n, heps, hdtdxe);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        Fx[c + k * g.n] = F[k];
+    low_face<1, MUSCL, LOCAL>(Ue, g.n, jm2 * nx + i, jm1 * nx + i, c, jp1 * nx + i, eps,
+                              hdtdy_adv, F);
+    peek_correct<1>(F, Uex, enth, p, jm1 * nx + i, c, g.n, heps, hdtdye);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        Fy[c + k * g.n] = F[k];
+}
+
+// Stored fluxes of the earlier stages (face layout of k_stage_fluxes) and
+// per-stage weights.
+struct RKFluxes
+{
+    const double *fx[BX_IMEX_MAX_STAGES];
+    const double *fy[BX_IMEX_MAX_STAGES];
+};
+
+struct RKWeights
+{
+    double x1[BX_IMEX_MAX_STAGES], y1[BX_IMEX_MAX_STAGES];
+    double x2[BX_IMEX_MAX_STAGES], y2[BX_IMEX_MAX_STAGES];
+};
+
+// The two stage states from the stored fluxes (conservative_update,
+// Update.H:218-302): out = U, then for each earlier stage s
+//   out -= dtdx_s (Fx_s(i+1) - Fx_s(i)) + dtdy_s (Fy_s(j+1) - Fy_s(j))
+// with dtdx_s = (A[stage][s] dt)(1/dx) of the implicit (1) and explicit (2)
+// rows.
+__global__ __launch_bounds__(256) void k_rk_combine(const double *__restrict__ U, RKFluxes f,
+                                                    int nst, RKWeights w,
+                                                    double *__restrict__ out1,
+                                                    double *__restrict__ out2, PeekGeom g)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    const long long row = (long long)j * g.nx;
+    const long long c   = row + i;
+    const long long ce  = row + wrap_hi(i, g.nx);
+    const long long cn  = (long long)wrap_hi(j, g.ny) * g.nx + i;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const long long o  = k * g.n;
+        double          o1 = U[c + o], o2 = o1;
+        for (int s = 0; s < nst; ++s)
+        {
+            const double dfx = f.fx[s][ce + o] - f.fx[s][c + o];
+            const double dfy = f.fy[s][cn + o] - f.fy[s][c + o];
+            o1 -= w.x1[s] * dfx + w.y1[s] * dfy;
+            o2 -= w.x2[s] * dfx + w.y2[s] * dfy;
+        }
+        out1[c + o] = o1;
+        out2[c + o] = o2;
+    }
+}
+
+// EK pressure of a later stage (compute_pressure, IMEX_O1.cpp:345-384): the
+// previous explicit state Uo advanced by its own advective fluxes over adt
+// (MUSCL without the local step), then
+//   p = (gamma - 1)(E_new - 0.5 eps (m_ex . m_ex) / rho_new).
+template <bool MUSCL>
+__global__ __launch_bounds__(256) void k_rk_pressure_ek(const double *__restrict__ Un,
+                                                        const double *__restrict__ Uo,
+                                                        double *__restrict__ p, PeekGeom g,
+                                                        double adia, double eps, double dtdx,
+                                                        double dtdy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    double Fxl[4], Fxr[4], Fyl[4], Fyr[4];
+    cell_faces<MUSCL, false>(Uo, g, i, j, eps, 0.5 * dtdx, 0.5 * dtdy, Fxl, Fxr, Fyl, Fyr);
+    const long long c  = (long long)j * g.nx + i;
+    const double    e1 = Uo[c + g.n] - (dtdx * (Fxr[1] - Fxl[1]) + dtdy * (Fyr[1] - Fyl[1]));
+    const double    e2 = Uo[c + 2 * g.n] - (dtdx * (Fxr[2] - Fxl[2]) + dtdy * (Fyr[2] - Fyl[2]));
+    p[c] = (adia - 1) * (Un[c + 3 * g.n] - 0.5 * eps * (e1 * e1 + e2 * e2) / Un[c]);
+}
+
+// Step end of a multi-stage tableau: sum_fluxes (IMEX_RK.cpp:196-241,
+// F = F_0 b_0, then F += F_s b_s) at the two faces of each direction and
+// the conservative update (Update.H:100-130)
+//   U_new = U - dtdx (F(i+1) - F(i)) - dtdy (F(j+1) - F(j)), dtdx = dt (1/dx).
+__global__ __launch_bounds__(256) void k_rk_update(const double *__restrict__ U, RKFluxes f,
+                                                   int nst, RKWeights b,
+                                                   double *__restrict__ Unew, PeekGeom g,
+                                                   double dtdx, double dtdy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    const long long row = (long long)j * g.nx;
+    const long long c   = row + i;
+    const long long ce  = row + wrap_hi(i, g.nx);
+    const long long cn  = (long long)wrap_hi(j, g.ny) * g.nx + i;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        const long long o   = k * g.n;
+        double          fxl = f.fx[0][c + o] * b.x1[0], fxr = f.fx[0][ce + o] * b.x1[0];
+        double          fyl = f.fy[0][c + o] * b.x1[0], fyr = f.fy[0][cn + o] * b.x1[0];
+        for (int s = 1; s < nst; ++s)
+        {
+            fxl += f.fx[s][c + o] * b.x1[s];
+            fxr += f.fx[s][ce + o] * b.x1[s];
+            fyl += f.fy[s][c + o] * b.x1[s];
+            fyr += f.fy[s][cn + o] * b.x1[s];
+        }
+        Unew[c + o] = U[c + o] - dtdx * (fxr - fxl) - dtdy * (fyr - fyl);
+    }
 }
 
 // p = (gamma - 1) (E - ke)   (pressure, euler_K.H:23-35)
@@ -418,25 +593,30 @@ extern "C" int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, 
     return BX_OK;
 }
 
-extern "C" int bx_imex_peek_assemble(const double *U, const double *p, int nx, int ny,
-                                     double dx, double dy, double dt, double gamma,
-                                     double eps, int advection_flux, double *Uex, double *enth,
-                                     double *rhs, double *a, double *bX, double *bY,
-                                     void *stream)
+static bool adv_ok(int advection_flux)
 {
-    if (!U || !p || !Uex || !enth || !rhs || !a || !bX || !bY || !peek_sizes_ok(nx, ny)
-        || !(dx > 0) || !(dy > 0)
-        || (advection_flux != BX_IMEX_RUSANOV && advection_flux != BX_IMEX_MUSCL_RUSANOV))
-        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_assemble: bad argument");
+    return advection_flux == BX_IMEX_RUSANOV || advection_flux == BX_IMEX_MUSCL_RUSANOV;
+}
+
+// fluxes of Ue applied to Ui (Uex), then the pressure system
+static int stage_assemble(const char *who, const double *Ui, const double *Ue, const double *p,
+                          int nx, int ny, double dx, double dy, double dt, double gamma,
+                          double eps, int advection_flux, int local, double *Uex, double *enth,
+                          double *rhs, double *a, double *bX, double *bY, void *stream)
+{
+    if (!Ui || !Ue || !p || !Uex || !enth || !rhs || !a || !bX || !bY
+        || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0) || !adv_ok(advection_flux))
+        return bx_helm_fail(BX_ERR_INVALID, std::string(who) + ": bad argument");
     // outputs are written while inputs and other outputs are read
     const void *outs[6] = { Uex, enth, rhs, a, bX, bY };
     for (int i = 0; i < 6; ++i)
     {
-        if (outs[i] == (const void *)U || outs[i] == (const void *)p)
-            return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_assemble: output aliases an input");
+        if (outs[i] == (const void *)Ui || outs[i] == (const void *)Ue
+            || outs[i] == (const void *)p)
+            return bx_helm_fail(BX_ERR_INVALID, std::string(who) + ": output aliases an input");
         for (int j = i + 1; j < 6; ++j)
             if (outs[i] == outs[j])
-                return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_assemble: outputs alias");
+                return bx_helm_fail(BX_ERR_INVALID, std::string(who) + ": outputs alias");
     }
     hipStream_t s = (hipStream_t)stream;
     PeekGeom    g{ nx, ny, (long long)nx * ny };
@@ -445,12 +625,15 @@ extern "C" int bx_imex_peek_assemble(const double *U, const double *p, int nx, i
     const double dtdy  = dt / dy;
     const double hdtdx = 0.5 * dt * (1.0 / dx);
     const double hdtdy = 0.5 * dt * (1.0 / dy);
-    if (advection_flux == BX_IMEX_MUSCL_RUSANOV)
-        hipLaunchKernelGGL(k_peek_advect<true>, peek_grid(nx, ny), dim3(64, 4), 0, s, U, Uex, g,
-                           eps, dtdx, dtdy);
+    if (advection_flux == BX_IMEX_MUSCL_RUSANOV && local)
+        hipLaunchKernelGGL((k_peek_advect<true, true>), peek_grid(nx, ny), dim3(64, 4), 0, s, Ue,
+                           Ui, Uex, g, eps, dtdx, dtdy);
+    else if (advection_flux == BX_IMEX_MUSCL_RUSANOV)
+        hipLaunchKernelGGL((k_peek_advect<true, false>), peek_grid(nx, ny), dim3(64, 4), 0, s, Ue,
+                           Ui, Uex, g, eps, dtdx, dtdy);
     else
-        hipLaunchKernelGGL(k_peek_advect<false>, peek_grid(nx, ny), dim3(64, 4), 0, s, U, Uex, g,
-                           eps, dtdx, dtdy);
+        hipLaunchKernelGGL((k_peek_advect<false, false>), peek_grid(nx, ny), dim3(64, 4), 0, s,
+                           Ue, Ui, Uex, g, eps, dtdx, dtdy);
     PEEK_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_peek_system, peek_grid(nx, ny), dim3(64, 4), 0, s, Uex, p, enth, rhs,
                        a, bX, bY, g, gamma, eps, hdtdx, hdtdy);
@@ -458,31 +641,178 @@ extern "C" int bx_imex_peek_assemble(const double *U, const double *p, int nx, i
     return BX_OK;
 }
 
+extern "C" int bx_imex_peek_assemble(const double *U, const double *p, int nx, int ny,
+                                     double dx, double dy, double dt, double gamma,
+                                     double eps, int advection_flux, double *Uex, double *enth,
+                                     double *rhs, double *a, double *bX, double *bY,
+                                     void *stream)
+{
+    return stage_assemble("bx_imex_peek_assemble", U, U, p, nx, ny, dx, dy, dt, gamma, eps,
+                          advection_flux, 1, Uex, enth, rhs, a, bX, bY, stream);
+}
+
+extern "C" int bx_imex_stage_assemble(const double *U_imp, const double *U_exp, const double *p,
+                                      int nx, int ny, double dx, double dy, double adt,
+                                      double gamma, double eps, int advection_flux,
+                                      int local_time_step, double *Uex, double *enth,
+                                      double *rhs, double *a, double *bX, double *bY,
+                                      void *stream)
+{
+    return stage_assemble("bx_imex_stage_assemble", U_imp, U_exp, p, nx, ny, dx, dy, adt, gamma,
+                          eps, advection_flux, local_time_step, Uex, enth, rhs, a, bX, bY,
+                          stream);
+}
+
 extern "C" int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
                                    const double *p_new, int nx, int ny, double dx, double dy,
                                    double dt, double eps, int advection_flux, double *U_new,
                                    void *stream)
 {
-    if (advection_flux != BX_IMEX_RUSANOV && advection_flux != BX_IMEX_MUSCL_RUSANOV)
+    if (!adv_ok(advection_flux))
         return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_finish: bad advection_flux");
     if (!U || !Uex || !enth || !p_new || !U_new || !peek_sizes_ok(nx, ny) || !(dx > 0)
         || !(dy > 0) || U == U_new || Uex == U_new || (const double *)U_new == enth
         || (const double *)U_new == p_new)
         return bx_helm_fail(BX_ERR_INVALID, "bx_imex_peek_finish: bad argument");
-    PeekGeom     g{ nx, ny, (long long)nx * ny };
-    const double dtdx   = dt / dx;
-    const double dtdy   = dt / dy;
-    const double heps   = 0.5 / eps;                   // IMEX_O1.cpp:499
-    const double hdtdxe = 0.5 * dt * (1.0 / dx) / eps; // IMEX_O1.cpp:66-70
-    const double hdtdye = 0.5 * dt * (1.0 / dy) / eps;
+    PeekGeom g{ nx, ny, (long long)nx * ny };
+    // MUSCL local step: 0.5 (dt/dx) (IMEX_Fluxes.cpp:177-180); update:
+    // dt (1/dx) (Update.H:113)
+    const double hdtdx_adv = 0.5 * (dt / dx);
+    const double hdtdy_adv = 0.5 * (dt / dy);
+    const double dtdx      = dt * (1.0 / dx);
+    const double dtdy      = dt * (1.0 / dy);
+    const double heps      = 0.5 / eps;                   // IMEX_O1.cpp:499
+    const double hdtdxe    = 0.5 * dt * (1.0 / dx) / eps; // IMEX_O1.cpp:66-70
+    const double hdtdye    = 0.5 * dt * (1.0 / dy) / eps;
     if (advection_flux == BX_IMEX_MUSCL_RUSANOV)
         hipLaunchKernelGGL(k_peek_finish<true>, peek_grid(nx, ny), dim3(64, 4), 0,
-                           (hipStream_t)stream, U, Uex, enth, p_new, U_new, g, eps, dtdx, dtdy,
-                           heps, hdtdxe, hdtdye);
+                           (hipStream_t)stream, U, Uex, enth, p_new, U_new, g, eps, hdtdx_adv,
+                           hdtdy_adv, dtdx, dtdy, heps, hdtdxe, hdtdye);
     else
         hipLaunchKernelGGL(k_peek_finish<false>, peek_grid(nx, ny), dim3(64, 4), 0,
-                           (hipStream_t)stream, U, Uex, enth, p_new, U_new, g, eps, dtdx, dtdy,
-                           heps, hdtdxe, hdtdye);
+                           (hipStream_t)stream, U, Uex, enth, p_new, U_new, g, eps, hdtdx_adv,
+                           hdtdy_adv, dtdx, dtdy, heps, hdtdxe, hdtdye);
+    PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_imex_stage_fluxes(const double *U_exp, const double *Uex, const double *enth,
+                                    const double *p_new, int nx, int ny, double dx, double dy,
+                                    double adt, double eps, int advection_flux,
+                                    int local_time_step, double *Fx, double *Fy, void *stream)
+{
+    if (!U_exp || !Uex || !enth || !p_new || !Fx || !Fy || !peek_sizes_ok(nx, ny) || !(dx > 0)
+        || !(dy > 0) || !adv_ok(advection_flux) || Fx == Fy)
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_stage_fluxes: bad argument");
+    const void *ins[4] = { U_exp, Uex, enth, p_new };
+    for (int i = 0; i < 4; ++i)
+        if (ins[i] == (const void *)Fx || ins[i] == (const void *)Fy)
+            return bx_helm_fail(BX_ERR_INVALID, "bx_imex_stage_fluxes: output aliases an input");
+    PeekGeom     g{ nx, ny, (long long)nx * ny };
+    hipStream_t  s         = (hipStream_t)stream;
+    const double hdtdx_adv = 0.5 * (adt / dx);
+    const double hdtdy_adv = 0.5 * (adt / dy);
+    const double heps      = 0.5 / eps;
+    const double hdtdxe    = 0.5 * adt * (1.0 / dx) / eps;
+    const double hdtdye    = 0.5 * adt * (1.0 / dy) / eps;
+    if (advection_flux == BX_IMEX_MUSCL_RUSANOV && local_time_step)
+        hipLaunchKernelGGL((k_stage_fluxes<true, true>), peek_grid(nx, ny), dim3(64, 4), 0, s,
+                           U_exp, Uex, enth, p_new, Fx, Fy, g, eps, hdtdx_adv, hdtdy_adv, heps,
+                           hdtdxe, hdtdye);
+    else if (advection_flux == BX_IMEX_MUSCL_RUSANOV)
+        hipLaunchKernelGGL((k_stage_fluxes<true, false>), peek_grid(nx, ny), dim3(64, 4), 0, s,
+                           U_exp, Uex, enth, p_new, Fx, Fy, g, eps, hdtdx_adv, hdtdy_adv, heps,
+                           hdtdxe, hdtdye);
+    else
+        hipLaunchKernelGGL((k_stage_fluxes<false, false>), peek_grid(nx, ny), dim3(64, 4), 0, s,
+                           U_exp, Uex, enth, p_new, Fx, Fy, g, eps, hdtdx_adv, hdtdy_adv, heps,
+                           hdtdxe, hdtdye);
+    PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+// the stored fluxes of nst stages: non-null, none an output
+static bool rk_fluxes(const double *const *Fx, const double *const *Fy, int nst,
+                      const void *out1, const void *out2, RKFluxes &f)
+{
+    if (!Fx || !Fy || nst < 0 || nst > BX_IMEX_MAX_STAGES)
+        return false;
+    for (int k = 0; k < BX_IMEX_MAX_STAGES; ++k)
+    {
+        f.fx[k] = k < nst ? Fx[k] : nullptr;
+        f.fy[k] = k < nst ? Fy[k] : nullptr;
+        if (k < nst
+            && (!f.fx[k] || !f.fy[k] || f.fx[k] == out1 || f.fy[k] == out1 || f.fx[k] == out2
+                || f.fy[k] == out2))
+            return false;
+    }
+    return true;
+}
+
+extern "C" int bx_imex_rk_combine(const double *U, const double *const *Fx,
+                                  const double *const *Fy, int nstage, const double *dt_imp,
+                                  const double *dt_exp, int nx, int ny, double dx, double dy,
+                                  double *U_imp, double *U_exp, void *stream)
+{
+    RKFluxes f;
+    if (!U || !U_imp || !U_exp || U_imp == U_exp || U == U_imp || U == U_exp
+        || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0)
+        || !rk_fluxes(Fx, Fy, nstage, U_imp, U_exp, f) || (nstage > 0 && (!dt_imp || !dt_exp)))
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_rk_combine: bad argument");
+    RKWeights w{};
+    // dtdx[iN] = dt[iN] * idx[0] (Update.H:250-255)
+    for (int k = 0; k < nstage; ++k)
+    {
+        w.x1[k] = dt_imp[k] * (1.0 / dx);
+        w.y1[k] = dt_imp[k] * (1.0 / dy);
+        w.x2[k] = dt_exp[k] * (1.0 / dx);
+        w.y2[k] = dt_exp[k] * (1.0 / dy);
+    }
+    PeekGeom g{ nx, ny, (long long)nx * ny };
+    hipLaunchKernelGGL(k_rk_combine, peek_grid(nx, ny), dim3(64, 4), 0, (hipStream_t)stream, U,
+                       f, nstage, w, U_imp, U_exp, g);
+    PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_imex_rk_pressure(const double *U_exp_new, const double *U_exp_old, int nx,
+                                   int ny, double dx, double dy, double adt, double gamma,
+                                   double eps, int advection_flux, double *p, void *stream)
+{
+    if (!U_exp_new || !U_exp_old || !p || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0)
+        || !adv_ok(advection_flux) || (const double *)p == U_exp_new
+        || (const double *)p == U_exp_old)
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_rk_pressure: bad argument");
+    PeekGeom     g{ nx, ny, (long long)nx * ny };
+    const double dtdx = adt / dx;
+    const double dtdy = adt / dy;
+    if (advection_flux == BX_IMEX_MUSCL_RUSANOV)
+        hipLaunchKernelGGL(k_rk_pressure_ek<true>, peek_grid(nx, ny), dim3(64, 4), 0,
+                           (hipStream_t)stream, U_exp_new, U_exp_old, p, g, gamma, eps, dtdx,
+                           dtdy);
+    else
+        hipLaunchKernelGGL(k_rk_pressure_ek<false>, peek_grid(nx, ny), dim3(64, 4), 0,
+                           (hipStream_t)stream, U_exp_new, U_exp_old, p, g, gamma, eps, dtdx,
+                           dtdy);
+    PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_imex_rk_update(const double *U, const double *const *Fx,
+                                 const double *const *Fy, int nstage, const double *b, int nx,
+                                 int ny, double dx, double dy, double dt, double *U_new,
+                                 void *stream)
+{
+    RKFluxes f;
+    if (!U || !U_new || U == U_new || !b || nstage < 1 || !peek_sizes_ok(nx, ny) || !(dx > 0)
+        || !(dy > 0) || !rk_fluxes(Fx, Fy, nstage, U_new, U_new, f))
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_rk_update: bad argument");
+    RKWeights w{};
+    for (int k = 0; k < nstage; ++k)
+        w.x1[k] = b[k];
+    PeekGeom g{ nx, ny, (long long)nx * ny };
+    hipLaunchKernelGGL(k_rk_update, peek_grid(nx, ny), dim3(64, 4), 0, (hipStream_t)stream, U, f,
+                       nstage, w, U_new, g, dt * (1.0 / dx), dt * (1.0 / dy));
     PEEK_CHECK(hipGetLastError());
     return BX_OK;
 }

@@ -168,6 +168,164 @@ class PeekStage:
             d.free()
 
 
+class ButcherTableau:
+    """IMEXButcherTableau (Source/IMEX/ButcherTableau.cpp:5-101): the
+    implicit and explicit matrices and weights of the reference's IMEX
+    Runge-Kutta schemes, by their imex.butcher_tableau names."""
+    NAMES = ("SP111", "SSP222", "SASSP322", "SASSP332", "SSP433")
+
+    def __init__(self, name):
+        third = 1.0 / 3
+        if name == "SP111":
+            self.n_imp, self.n_exp, self.order = 1, 1, 1
+            self.A_imp, self.A_exp, self.b = [[1.0]], [[0.0]], [1.0]
+        elif name == "SSP222":
+            g = 1 - 1 / math.sqrt(2)
+            self.n_imp, self.n_exp, self.order = 2, 2, 2
+            self.A_imp = [[g, 0.0], [1 - 2 * g, g]]
+            self.A_exp = [[0.0, 0.0], [1.0, 0.0]]
+            self.b = [0.5, 0.5]
+        elif name == "SASSP322":
+            self.n_imp, self.n_exp, self.order = 3, 2, 2
+            self.A_imp = [[0.5, 0.0, 0.0], [-0.5, 0.5, 0.0], [0.0, 0.5, 0.5]]
+            self.A_exp = [[0.0, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 1.0, 0.0]]
+            self.b = [0.0, 0.5, 0.5]
+        elif name == "SASSP332":
+            self.n_imp, self.n_exp, self.order = 3, 3, 2
+            self.A_imp = [[0.25, 0.0, 0.0], [0.0, 0.25, 0.0], [third, third, third]]
+            self.A_exp = [[0.0, 0.0, 0.0], [0.5, 0.0, 0.0], [0.5, 0.5, 0.0]]
+            self.b = [third, third, third]
+        elif name == "SSP433":
+            a, be, et = 0.24169426078821, 0.06042356519705, 0.12915286960590
+            self.n_imp, self.n_exp, self.order = 4, 3, 3
+            self.A_imp = [[a, 0.0, 0.0, 0.0], [-a, a, 0.0, 0.0], [0.0, 1 - a, a, 0.0],
+                          [be, et, 0.5 - be - et - a, a]]
+            self.A_exp = [[0.0, 0.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0], [0.0, 1.0, 0.0, 0.0],
+                          [0.0, 0.25, 0.25, 0.0]]
+            self.b = [0.0, 1.0 / 6, 1.0 / 6, 2.0 / 3]
+        else:
+            raise ValueError("imex.butcher_tableau: unknown tableau %r (one of %s)"
+                             % (name, ", ".join(self.NAMES)))
+        self.name = name
+
+    def imp_row(self, r, mult):
+        """get_A_imp_row (ButcherTableau.H:53-59)."""
+        return [x * mult for x in self.A_imp[r]]
+
+    def exp_row(self, r, mult):
+        """get_A_exp_row (ButcherTableau.H:61-67)."""
+        return [self.A_exp[r][i] * mult for i in range(self.n_imp)]
+
+
+def _ptrs(arrays):
+    return (ctypes.c_void_p * _lib.BX_IMEX_MAX_STAGES)(*([a.ptr.value for a in arrays]
+                                                          + [None] * (_lib.BX_IMEX_MAX_STAGES
+                                                                      - len(arrays))))
+
+
+def _dbls(vals):
+    return (ctypes.c_double * _lib.BX_IMEX_MAX_STAGES)(*(list(vals) + [0.0] * (_lib.BX_IMEX_MAX_STAGES
+                                                                              - len(vals))))
+
+
+class ImexRK:
+    """advance_imex_rk (Source/IMEX/IMEX_RK.cpp:22-96) on an nx x ny periodic
+    level, every stage's explicit work on the GPU (bx_imex_rk_*,
+    bx_imex_stage_*); the pressure solve of each stage is the caller's.
+
+    Per step: begin(U, p, dt); then per stage sy = system(), a solve of
+    sy's pressure equation from the initial guess sy["p_prev"], and
+    complete(p_new, U_out), which returns True after the last stage has
+    written U_out.  p (a device array) holds the stage pressures as the
+    reference's pressure MultiFab does: the EK pressure of a later stage,
+    then each solution; after the step, the last stage's.  SP111 ends with
+    the fused finish (bx_imex_peek_finish), later tableaux store the stage
+    fluxes and sum them (bx_imex_rk_update)."""
+
+    def __init__(self, nx, ny, dx, dy=None, gamma=1.4, eps=1.0, advection="rusanov",
+                 tableau="SP111"):
+        self.tab = ButcherTableau(tableau)
+        self.stage = PeekStage(nx, ny, dx, dy, gamma, eps, advection)
+        self.nx, self.ny, self.dx, self.dy = nx, ny, self.stage.dx, self.stage.dy
+        self.gamma, self.eps, self.adv = gamma, eps, self.stage.adv
+        ns = self.tab.n_imp
+        self.fused = self.tab.name == "SP111"
+        self.Fx = [] if self.fused else [DeviceArray((4, ny, nx)) for _ in range(ns)]
+        self.Fy = [] if self.fused else [DeviceArray((4, ny, nx)) for _ in range(ns)]
+        self.U_imp = DeviceArray((4, ny, nx)) if ns > 1 else None
+        self.U_exp = [DeviceArray((4, ny, nx)), DeviceArray((4, ny, nx))] if ns > 1 else []
+        self.s = None
+
+    def begin(self, U, p, dt):
+        self.U, self.p, self.dt, self.s = U, p, dt, 0
+        self._exp = []
+
+    def system(self):
+        """The current stage's pressure system (device arrays of the stage
+        and the operator scalars alpha = 1, beta = adt^2)."""
+        s, dt, tab = self.s, self.dt, self.tab
+        adt = tab.A_imp[s][s] * dt
+        nx, ny, dx, dy = self.nx, self.ny, self.dx, self.dy
+        if s == 0:
+            Ui = Ue = self.U
+        else:
+            Ui, Ue = self.U_imp, self.U_exp[s % 2]
+            _chk(lib().bx_imex_rk_combine(self.U.ptr, _ptrs(self.Fx[:s]), _ptrs(self.Fy[:s]), s,
+                                          _dbls(tab.imp_row(s, dt)[:s]), _dbls(tab.exp_row(s, dt)[:s]),
+                                          nx, ny, dx, dy, Ui.ptr, Ue.ptr, None), "bx_imex_rk_combine")
+            _chk(lib().bx_imex_rk_pressure(Ue.ptr, self._exp[s - 1].ptr, nx, ny, dx, dy, adt,
+                                           self.gamma, self.eps, self.adv, self.p.ptr, None),
+                 "bx_imex_rk_pressure")
+        self._exp.append(Ue)
+        self._Ue, self._adt = Ue, adt
+        st = self.stage
+        _chk(lib().bx_imex_stage_assemble(Ui.ptr, Ue.ptr, self.p.ptr, nx, ny, dx, dy, adt, self.gamma,
+                                          self.eps, self.adv, int(tab.name == "SP111"), st.Uex.ptr,
+                                          st.enth.ptr, st.rhs.ptr, st.a.ptr, st.bX.ptr, st.bY.ptr, None),
+             "bx_imex_stage_assemble")
+        return dict(Uex=st.Uex, enth=st.enth, rhs=st.rhs, a=st.a, bX=st.bX, bY=st.bY, alpha=1.0,
+                    beta=adt * adt, dt=dt, adt=adt, stage=s, p_prev=self.p)
+
+    def complete(self, p_new, U_out):
+        """The stage's fluxes with the solved pressure p_new; after the last
+        stage the step's update into U_out (not the step's U).  Returns
+        whether the step is done."""
+        s, st = self.s, self.stage
+        nx, ny, dx, dy = self.nx, self.ny, self.dx, self.dy
+        last = s + 1 == self.tab.n_imp
+        if self.fused:
+            _chk(lib().bx_imex_peek_finish(self.U.ptr, st.Uex.ptr, st.enth.ptr, p_new.ptr, nx, ny, dx,
+                                           dy, self.dt, self.eps, self.adv, U_out.ptr, None),
+                 "bx_imex_peek_finish")
+        else:
+            _chk(lib().bx_imex_stage_fluxes(self._Ue.ptr, st.Uex.ptr, st.enth.ptr, p_new.ptr, nx, ny,
+                                            dx, dy, self._adt, self.eps, self.adv, 0, self.Fx[s].ptr,
+                                            self.Fy[s].ptr, None), "bx_imex_stage_fluxes")
+            if last:
+                n = self.tab.n_imp
+                _chk(lib().bx_imex_rk_update(self.U.ptr, _ptrs(self.Fx[:n]), _ptrs(self.Fy[:n]), n,
+                                             _dbls(self.tab.b), nx, ny, dx, dy, self.dt, U_out.ptr,
+                                             None), "bx_imex_rk_update")
+        if p_new.ptr.value != self.p.ptr.value:
+            _lib.check(lib().bx_memcpy_dtod(self.p.ptr, p_new.ptr, self.p.nbytes), "dtod")
+        self.s = s + 1
+        return last
+
+    def advance(self, U, p, dt, U_out, solve):
+        """One whole step; solve(sy) returns the stage's solved pressure (a
+        device array)."""
+        self.begin(U, p, dt)
+        while True:
+            sy = self.system()
+            if self.complete(solve(sy), U_out):
+                return
+
+    def free(self):
+        for d in self.Fx + self.Fy + self.U_exp + ([self.U_imp] if self.U_imp else []):
+            d.free()
+        self.stage.free()
+
+
 class HelmholtzSolver:
     """Owns a bx_helm solver for one nx x ny periodic level."""
 
@@ -351,39 +509,50 @@ def harvest_residuals(solver, b_dev, x_dev, R_dev, first, count, rtol=1e-10):
 
 
 class GreshoWorkload:
-    """Successive PEEK-SP111 pressure systems of the low-Mach Gresho vortex
-    on an n x n periodic level, all on the GPU.  The state advances with
-    the pressure passed to advance() (the benchmark passes the plain
+    """Successive pressure systems of IMEX PEEK steps of the low-Mach Gresho
+    vortex on an n x n periodic level, all on the GPU: one system per stage
+    of the tableau (SP111: one per step, SSP222: two).  The state advances
+    with the pressure passed to advance() (the benchmark passes the plain
     multigrid solution, so every method solves the same sequence)."""
 
-    def __init__(self, n=1024, mach=0.01, gamma=1.4, eps=1.0, cfl=0.45, advection="rusanov"):
+    def __init__(self, n=1024, mach=0.01, gamma=1.4, eps=1.0, cfl=0.45, advection="rusanov",
+                 tableau="SP111"):
         U, self.dx = gresho_state(n, mach, gamma, eps)
         self.n = n
         self.U = dev(U)
         self.U_next = DeviceArray(U.shape)
         self.p = DeviceArray((n, n))
-        self.stage = PeekStage(n, n, self.dx, gamma=gamma, eps=eps, advection=advection)
+        self.rk = ImexRK(n, n, self.dx, gamma=gamma, eps=eps, advection=advection, tableau=tableau)
+        self.stage = self.rk.stage
         self.stage.pressure(self.U, self.p)
         self.cfl = cfl
+        self.in_step = False
 
-    def next_system(self):
-        """Assembles the next step's system; "p_prev" is the previous
-        pressure (the solve's initial guess, device array)."""
-        dt = self.stage.advective_dt(self.U, self.cfl)
-        sy = self.stage.assemble(self.U, self.p, dt)
-        sy["p_prev"] = self.p
-        return sy
+    def step_dt(self):
+        """The advective dt of the next step (estTimeStep)."""
+        return self.stage.advective_dt(self.U, self.cfl)
+
+    def next_system(self, dt=None):
+        """Assembles the next stage's system (starting a step with the
+        advective dt, or dt, when the last one is done); "p_prev" is the
+        solve's initial guess (device array)."""
+        if not self.in_step:
+            self.rk.begin(self.U, self.p, self.step_dt() if dt is None else dt)
+            self.in_step = True
+        return self.rk.system()
 
     def advance(self, sy, p_new):
-        """U <- stage update with the solved pressure p_new (DeviceArray);
-        p <- p_new."""
-        self.stage.finish(self.U, p_new, sy["dt"], self.U_next)
-        self.U, self.U_next = self.U_next, self.U
-        if p_new.ptr.value != self.p.ptr.value:
-            _lib.check(lib().bx_memcpy_dtod(self.p.ptr, p_new.ptr, self.p.nbytes), "dtod")
+        """Completes the stage with the solved pressure p_new (DeviceArray);
+        after a step's last stage, U <- the new state.  Returns whether a
+        step ended."""
+        done = self.rk.complete(p_new, self.U_next)
+        if done:
+            self.U, self.U_next = self.U_next, self.U
+            self.in_step = False
         lib().bx_stream_sync(None)
+        return done
 
     def free(self):
         for d in (self.U, self.U_next, self.p):
             d.free()
-        self.stage.free()
+        self.rk.free()

@@ -51,8 +51,10 @@ def main():
     ap.add_argument("--imex-only", action="store_true", help="run only the IMEX pressure-solve leg")
     ap.add_argument("--imex-n", type=int, default=1024)
     ap.add_argument("--imex-advection", default="muscl_rusanov", choices=["rusanov", "muscl_rusanov"])
-    ap.add_argument("--imex-harvest-steps", type=int, default=8)
-    ap.add_argument("--imex-steps", type=int, default=6)
+    ap.add_argument("--imex-tableau", default="SSP222",
+                    choices=["SP111", "SSP222", "SASSP322", "SASSP332", "SSP433"])
+    ap.add_argument("--imex-harvest-steps", type=int, default=8, help="pressure systems harvested")
+    ap.add_argument("--imex-steps", type=int, default=4, help="time steps (one solve per stage)")
     ap.add_argument("--imex-train-steps", type=int, default=640)
     ap.add_argument("--imex-batch", type=int, default=32)
     ap.add_argument("--imex-crop", type=int, default=64)
@@ -402,17 +404,25 @@ def imex_leg(args):
     from bamrex_amd import imex
     n = args.imex_n
     log("imex: Gresho %d^2, harvesting %d steps" % (n, args.imex_harvest_steps))
-    wl = imex.GreshoWorkload(n, advection=args.imex_advection)
+    wl = imex.GreshoWorkload(n, advection=args.imex_advection, tableau=args.imex_tableau)
+    nst = wl.rk.tab.n_imp
     solver = imex.HelmholtzSolver(n, n, wl.dx)
     nf_max = 8 * args.imex_harvest_steps
     R = imex.DeviceArray((nf_max, n, n))
     nf = 0
+    # one pressure system per stage: harvest_steps systems
     for k in range(args.imex_harvest_steps):
         sy = wl.next_system()
         solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
         db, dx_ = imex.dev(sy["rhs"]), imex.dev(sy["p_prev"])
         got, st = imex.harvest_residuals(solver, db, dx_, R, nf, nf_max - nf)
         nf += got
+        wl.advance(sy, dx_)
+    while wl.in_step:        # finish a step left open by the harvest
+        sy = wl.next_system()
+        solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+        db, dx_ = imex.dev(sy["rhs"]), imex.dev(sy["p_prev"])
+        solver.solve(db, dx_, "mg", rtol=1e-10, maxiter=200)
         wl.advance(sy, dx_)
     log("imex: %d residual fields harvested" % nf)
     net = imex.LearnedSmoother(seed=1)
@@ -432,60 +442,72 @@ def imex_leg(args):
                ("fgmres_cnn_smoother", "fgmres_cnn", "smoother", True)]
     res = {m[0]: {"seconds": [], "iters": [], "rel_resid": [], "fell_back": 0} for m in methods}
     r = imex.DeviceArray((n, n))
+    # explicit work per step (host clock around synchronised calls):
+    # advective dt; per stage the stage states, EK pressure, advection and
+    # pressure system ("assemble"), the coefficient upload and multigrid
+    # hierarchy build ("set_coeffs"), the stage fluxes and the step's update
+    # ("finish"), summed over the step's stages
     stage_ms = {"advective_dt": [], "assemble": [], "set_coeffs": [], "finish": []}
+    cold = {}
     for k in range(args.imex_steps):
         t0 = time.perf_counter()
-        dt = wl.stage.advective_dt(wl.U, wl.cfl)
-        t1 = time.perf_counter()
-        sy = wl.stage.assemble(wl.U, wl.p, dt)
-        sy["p_prev"] = wl.p
-        imex.lib().bx_stream_sync(None)
-        t2 = time.perf_counter()
-        solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
-        t3 = time.perf_counter()
-        for key, v in (("advective_dt", t1 - t0), ("assemble", t2 - t1), ("set_coeffs", t3 - t2)):
-            stage_ms[key].append(v * 1e3)
-        db = imex.dev(sy["rhs"])
-        bn = float(np.linalg.norm(sy["rhs"].download()))
-        p_mg = None
-        for label, meth, mode, graph in methods:
-            if mode is not None:
-                solver.attach(net, omega=1.0, mode=mode)
-            solver.set_graph(graph)
-            dx_ = imex.dev(sy["p_prev"])
-            # untimed solve first: graph capture after a configuration change
-            solver.solve(db, dx_, meth, rtol=1e-10, maxiter=200)
-            dx_.copy_from(sy["p_prev"])
-            st = solver.solve(db, dx_, meth, rtol=1e-10, maxiter=200)
-            solver.residual(db, dx_, r)
-            rel = float(np.linalg.norm(r.download())) / bn
-            res[label]["seconds"].append(st["seconds"])
-            res[label]["iters"].append(st["iters"])
-            res[label]["rel_resid"].append(rel)
-            res[label]["fell_back"] += int(st["fell_back"])
-            if label == "mg":
-                p_mg = dx_
-        if k == args.imex_steps - 1:
-            # cold start (zero initial guess) on the last system: where a
-            # better preconditioner saves iterations
-            cold = {}
+        dt = wl.step_dt()
+        stage_ms["advective_dt"].append((time.perf_counter() - t0) * 1e3)
+        acc = {"assemble": 0.0, "set_coeffs": 0.0, "finish": 0.0}
+        done = False
+        while not done:
+            t1 = time.perf_counter()
+            sy = wl.next_system(dt)
+            imex.lib().bx_stream_sync(None)
+            t2 = time.perf_counter()
+            solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+            t3 = time.perf_counter()
+            acc["assemble"] += t2 - t1
+            acc["set_coeffs"] += t3 - t2
+            db = imex.dev(sy["rhs"])
+            bn = float(np.linalg.norm(sy["rhs"].download()))
+            p_mg = None
             for label, meth, mode, graph in methods:
                 if mode is not None:
                     solver.attach(net, omega=1.0, mode=mode)
                 solver.set_graph(graph)
-                dx_ = imex.dev(np.zeros((n, n)))
+                dx_ = imex.dev(sy["p_prev"])
+                # untimed solve first: graph capture after a configuration change
                 solver.solve(db, dx_, meth, rtol=1e-10, maxiter=200)
-                dx_.upload(np.zeros((n, n)))
+                dx_.copy_from(sy["p_prev"])
                 st = solver.solve(db, dx_, meth, rtol=1e-10, maxiter=200)
-                cold[label] = {"s_per_solve": st["seconds"], "iters": st["iters"],
-                               "fell_back": int(st["fell_back"])}
-        t0 = time.perf_counter()
-        wl.advance(sy, p_mg)
-        stage_ms["finish"].append((time.perf_counter() - t0) * 1e3)
-        log("imex step %d: " % k + "  ".join(
-            "%s %.3f ms/%d it" % (m[0], res[m[0]]["seconds"][-1] * 1e3, res[m[0]]["iters"][-1])
-            for m in methods))
-    out = {"workload": "gresho-peek-sp111-%s-%dx%d-M0.01-periodic" % (args.imex_advection, n, n),
+                solver.residual(db, dx_, r)
+                rel = float(np.linalg.norm(r.download())) / bn
+                res[label]["seconds"].append(st["seconds"])
+                res[label]["iters"].append(st["iters"])
+                res[label]["rel_resid"].append(rel)
+                res[label]["fell_back"] += int(st["fell_back"])
+                if label == "mg":
+                    p_mg = dx_
+            if k == args.imex_steps - 1 and sy["stage"] == nst - 1:
+                # cold start (zero initial guess) on the last system: where a
+                # better preconditioner saves iterations
+                for label, meth, mode, graph in methods:
+                    if mode is not None:
+                        solver.attach(net, omega=1.0, mode=mode)
+                    solver.set_graph(graph)
+                    dx_ = imex.dev(np.zeros((n, n)))
+                    solver.solve(db, dx_, meth, rtol=1e-10, maxiter=200)
+                    dx_.upload(np.zeros((n, n)))
+                    st = solver.solve(db, dx_, meth, rtol=1e-10, maxiter=200)
+                    cold[label] = {"s_per_solve": st["seconds"], "iters": st["iters"],
+                                   "fell_back": int(st["fell_back"])}
+            t0 = time.perf_counter()
+            done = wl.advance(sy, p_mg)
+            acc["finish"] += time.perf_counter() - t0
+            log("imex step %d stage %d: " % (k, sy["stage"]) + "  ".join(
+                "%s %.3f ms/%d it" % (m[0], res[m[0]]["seconds"][-1] * 1e3, res[m[0]]["iters"][-1])
+                for m in methods))
+        for key, v in acc.items():
+            stage_ms[key].append(v * 1e3)
+    out = {"workload": "gresho-peek-%s-%s-%dx%d-M0.01-periodic" % (
+        args.imex_tableau.lower(), args.imex_advection, n, n),
+           "tableau": args.imex_tableau, "solves_per_step": nst,
            "rtol": 1e-10,
            "harvest_fields": nf, "train_steps": args.imex_train_steps, "train_batch": args.imex_batch,
            "train_crop": args.imex_crop, "train_crops_total": args.imex_train_steps * args.imex_batch,
@@ -498,14 +520,11 @@ def imex_leg(args):
                       "fell_back": res[label]["fell_back"], "network_mode": mode,
                       "graph_preconditioner": graph}
     out["cold_start"] = cold
-    # explicit work of a step on the GPU (bx_imex_*; host clock around
-    # synchronised calls): dt reduction, stage assembly, coefficient upload
-    # and multigrid hierarchy build, flux correction + update
     out["stage_ms_median"] = {k: round(float(np.median(v)), 4) for k, v in stage_ms.items()}
-    # a whole PEEK-SP111 step per solve method: the stage's explicit work
-    # plus the warm-started solve
+    # a whole PEEK step per solve method: the explicit work of its stages
+    # plus one warm-started solve per stage
     stage_total = sum(out["stage_ms_median"].values())
-    out["peek_step_ms"] = {m[0]: round(stage_total + out[m[0]]["s_per_solve"] * 1e3, 4)
+    out["peek_step_ms"] = {m[0]: round(stage_total + nst * out[m[0]]["s_per_solve"] * 1e3, 4)
                            for m in methods}
     out["speedup_cnn_vs_mg"] = out["mg"]["s_per_solve"] / out["fgmres_cnn"]["s_per_solve"]
     out["speedup_fgmres_mg_vs_mg"] = out["mg"]["s_per_solve"] / out["fgmres_mg"]["s_per_solve"]

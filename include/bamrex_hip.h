@@ -423,22 +423,76 @@ int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, double dy,
  * beta = dt*dt. */
 #define BX_IMEX_RUSANOV 0       /* imex.advection_flux = rusanov (1st order) */
 #define BX_IMEX_MUSCL_RUSANOV 1 /* = muscl_rusanov (advance_MUSCL_rusanov_adv,
-                                   Source/IMEX/IMEX_Fluxes.cpp:120-324, with
-                                   the SP111 local half step) */
+                                   Source/IMEX/IMEX_Fluxes.cpp:120-324; here
+                                   with the SP111 local half step) */
 int bx_imex_peek_assemble(const double *U, const double *p, int nx, int ny,
                           double dx, double dy, double dt, double gamma,
                           double eps, int advection_flux, double *Uex,
                           double *enth, double *rhs, double *a, double *bX,
                           double *bY, void *stream);
-/* Second half (IMEX_O1.cpp:132-175 with update_momentum_flux /
- * update_energy_flux, :480-566, and conservative_update, Source/IMEX/
- * IMEX_RK.cpp:22-96): the Rusanov fluxes of U corrected with the solved
- * pressure p_new, then U_new = U - dt div F.  U_new must not alias U or
- * Uex. */
+/* Second half of an SP111 step (IMEX_O1.cpp:132-175 with
+ * update_momentum_flux / update_energy_flux, :485-566, then sum_fluxes with
+ * b = 1 and conservative_update, Source/IMEX/IMEX_RK.cpp:93-95 and
+ * Source/Fluxes/Update.H:100-130): the advective fluxes of U corrected with
+ * the solved pressure p_new, then
+ * U_new = U - dt(1/dx) dFx - dt(1/dy) dFy.  U_new must not alias U or Uex. */
 int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
                         const double *p_new, int nx, int ny, double dx, double dy,
                         double dt, double eps, int advection_flux, double *U_new,
                         void *stream);
+
+/* Stages of an IMEX Runge-Kutta step (advance_imex_rk, Source/IMEX/
+ * IMEX_RK.cpp:22-96, any IMEXButcherTableau of ButcherTableau.cpp:5-101).
+ * Stage fluxes Fx, Fy hold 4 components of nx*ny faces each: Fx[k*n + c]
+ * is the face between cells (i-1, j) and (i, j), Fy[k*n + c] between
+ * (i, j-1) and (i, j), periodic.  The host loop per stage s:
+ *   bx_imex_rk_combine     (s > 0; stage 0 uses U for both states)
+ *   bx_imex_rk_pressure    (s > 0)
+ *   bx_imex_stage_assemble, the pressure solve, bx_imex_stage_fluxes
+ * and after the last stage bx_imex_rk_update. */
+#define BX_IMEX_MAX_STAGES 4
+/* The stage states (conservative_update into two outputs, Source/Fluxes/
+ * Update.H:218-302): U_imp = U - sum_{s<nstage} dt_imp[s] div F_s and
+ * U_exp likewise with dt_exp, dt_x[s] = A_x[stage][s] dt (host arrays of
+ * nstage values); Fx, Fy: host arrays of nstage device pointers. */
+int bx_imex_rk_combine(const double *U, const double *const *Fx,
+                       const double *const *Fy, int nstage, const double *dt_imp,
+                       const double *dt_exp, int nx, int ny, double dx, double dy,
+                       double *U_imp, double *U_exp, void *stream);
+/* The pressure of a later stage, EK form (compute_pressure, Source/IMEX/
+ * IMEX_O1.cpp:300-391, linearised with ke_method ex): the previous
+ * explicit state advanced over adt by its own advective fluxes (MUSCL
+ * without the local step) supplies the momentum, U_exp_new the density
+ * and energy: p = (gamma-1)(E - 0.5 eps |m_ex|^2 / rho). */
+int bx_imex_rk_pressure(const double *U_exp_new, const double *U_exp_old, int nx,
+                        int ny, double dx, double dy, double adt, double gamma,
+                        double eps, int advection_flux, double *p, void *stream);
+/* compute_flux_imex_o1 up to the solve with separate stage states: the
+ * advective fluxes of U_exp update U_imp; local_time_step selects the
+ * MUSCL local half step (the reference takes it for SP111 only,
+ * IMEX_O1.cpp:95-102).  Outputs as bx_imex_peek_assemble, whose call is
+ * this one with U_imp = U_exp = U, local_time_step = 1; operator scalars
+ * alpha = 1, beta = adt*adt. */
+int bx_imex_stage_assemble(const double *U_imp, const double *U_exp, const double *p,
+                           int nx, int ny, double dx, double dy, double adt,
+                           double gamma, double eps, int advection_flux,
+                           int local_time_step, double *Uex, double *enth,
+                           double *rhs, double *a, double *bX, double *bY,
+                           void *stream);
+/* The stage's fluxes after the solve (update_momentum_flux /
+ * update_energy_flux, IMEX_O1.cpp:485-566): the advective fluxes of U_exp
+ * corrected with the solved pressure p_new, into Fx, Fy. */
+int bx_imex_stage_fluxes(const double *U_exp, const double *Uex, const double *enth,
+                         const double *p_new, int nx, int ny, double dx, double dy,
+                         double adt, double eps, int advection_flux,
+                         int local_time_step, double *Fx, double *Fy, void *stream);
+/* Step end (sum_fluxes, IMEX_RK.cpp:196-241, and conservative_update,
+ * Update.H:100-130): F = sum_s b[s] F_s, U_new = U - dt(1/dx) dFx -
+ * dt(1/dy) dFy.  b: host array of nstage weights. */
+int bx_imex_rk_update(const double *U, const double *const *Fx,
+                      const double *const *Fy, int nstage, const double *b, int nx,
+                      int ny, double dx, double dy, double dt, double *U_new,
+                      void *stream);
 
 /* Learned smoother: bias-free 3x3 convolutions 1->16->16->16->1 with ReLU,
  * BF16 weights/activations, FP32 accumulation (MFMA for the 16->16 layers).

@@ -166,7 +166,10 @@ def _apply_local(u, a, bxl, bxr, byl, byr, alpha, dhx, dhy):
 def crop_loss_grad(params, R, crops, cs, a, bX, bY, alpha, beta, dx, dy, border=5):
     """Loss and gradient of the device's training objective (float64 except
     the emulated BF16 forward):
-      L = mean_b ||m (r_b - A e_b)||^2 / ||m r_b||^2,
+      L = mean_b ||m (r_b - A e_b)||^2 / n_b,
+      n_b = max(||m r_b||^2, |m| sigma_b^2)  (floored: a crop whose residual
+            sits in its border, which the network sees, has a near-zero
+            masked norm),
       e_b = sigma_b * N(r_b / sigma_b) / D,  sigma_b = rms of the crop."""
     ny, nx = a.shape
     D = op_diag(a, bX, bY, alpha, beta, dx, dy)
@@ -193,7 +196,7 @@ def crop_loss_grad(params, R, crops, cs, a, bX, bY, alpha, beta, dx, dy, border=
         c = [_gather(z[None], 0, x0, y0, cs) for z in (bxl_f, bxr_f, byl_f, byr_f)]
         e = sig * y[b].astype(np.float64) / Dl
         res = np.where(mask, r - _apply_local(e, al, *c, alpha, dhx, dhy), 0.0)
-        rn2 = np.sum(np.where(mask, r * r, 0.0))
+        rn2 = max(np.sum(np.where(mask, r * r, 0.0)), np.sum(r * r) * (mask.sum() / float(cs * cs)))
         rn2 = rn2 if rn2 > 0 else 1.0
         loss += np.sum(res * res) / rn2
         ar = _apply_local(res, al, *c, alpha, dhx, dhy)

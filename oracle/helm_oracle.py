@@ -136,7 +136,7 @@ def crop_loss_grad64(params, R, crops, cs, a, bX, bY, alpha, beta, dx, dy, borde
         c = [co._gather(z[None], 0, x0, y0, cs) for z in coef]
         e = sig * y[b] / Dl
         res = np.where(mask, r - co._apply_local(e, al, *c, alpha, dhx, dhy), 0.0)
-        rn2 = np.sum(np.where(mask, r * r, 0.0))
+        rn2 = max(np.sum(np.where(mask, r * r, 0.0)), np.sum(r * r) * (mask.sum() / float(cs * cs)))
         loss += np.sum(res * res) / rn2
         gy[b] = (-2.0 * co._apply_local(res, al, *c, alpha, dhx, dhy) / rn2) * sig / Dl / B
     return loss / B, co.backward(cache, gy, periodic=False)

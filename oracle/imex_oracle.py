@@ -1,4 +1,4 @@
-"""CPU restatement (numpy) of the explicit work of the IMEX PEEK-SP111 stage.
+"""CPU restatement (numpy) of the explicit work of the IMEX PEEK stages.
 
 TEST INFRASTRUCTURE ONLY: the checker for bx_imex_* (bamrex_amd/csrc/
 bx_imex.hip).  Only tests/, __graft_entry__.smoke() and bench.py's CPU
@@ -6,7 +6,9 @@ baseline may import it; the product path never does.
 
 Settings restated: linearised PEEK, enthalpy from the previous pressure
 (IMEXSettings::reuse_pressure), kinetic energy of the explicit state
-(IMEXSettings::ex), one implicit stage with adt = dt (SP111), 2D periodic.
+(IMEXSettings::ex), 2D periodic; one implicit stage with adt = dt (SP111,
+PeekStage) or the stage loop of any of the reference's IMEX Runge-Kutta
+tableaux (advance_imex_rk).
 
   kinetic_energy / pressure / adv_flux   Source/IMEX_K/euler_K.H:8-65
   max_dx_scaled_speed                    Source/System/Euler/Euler.cpp:107-129
@@ -19,7 +21,12 @@ Settings restated: linearised PEEK, enthalpy from the previous pressure
   setup_pressure_op                      Source/IMEX/IMEX_O1.cpp:570-614
   update_momentum_flux / update_energy_flux
                                          Source/IMEX/IMEX_O1.cpp:480-566
-  conservative_update (b = 1)            Source/IMEX/IMEX_RK.cpp:22-96
+  conservative_update (one flux set)     Source/Fluxes/Update.H:74-130
+  conservative_update (stage states)     Source/Fluxes/Update.H:218-302
+  sum_fluxes                             Source/IMEX/IMEX_RK.cpp:196-241
+  advance_imex_rk (stage loop)           Source/IMEX/IMEX_RK.cpp:22-96
+  compute_pressure (EK branch)           Source/IMEX/IMEX_O1.cpp:300-391
+  IMEXButcherTableau                     Source/IMEX/ButcherTableau.cpp:5-101
 
 Arrays are (ny, nx), x fastest; U has 4 components (rho, m_x, m_y, E).
 numpy evaluates each binary operation in IEEE double with no contraction,
@@ -31,8 +38,14 @@ Tests/IMEX_O1_01.cpp (ExplicitKEEnergyMatch), free-stream preservation and
 conservation, brute-force per-cell evaluation of the advective dt, and for
 the MUSCL form: equality with the first-order fluxes on piecewise-constant
 data (minmod vanishes at a single jump) and second-order convergence on a
-smooth advected density wave.
+smooth advected density wave.  The stage loop: the tableaux against the
+reference's Tests/IMEXButcherTableaux01.cpp values and their order
+conditions, SP111 through the loop equal to PeekStage, second order in
+time of SSP222 with MUSCL fluxes without the local step (first order with
+SP111), and conservation with a real pressure solve.
 """
+import math
+
 import numpy as np
 
 
@@ -97,20 +110,21 @@ def adv_flux(S, d, eps):
     return F
 
 
-def muscl_rusanov_faces(U, eps, dt, dx, dy):
-    """advance_MUSCL_rusanov_adv (IMEX_Fluxes.cpp:120-324) with the SP111
-    local step: per direction, minmod half-slopes, a half step of the
-    states with their own fluxes (hdtdx = 0.5 * (dt/dx)), then Rusanov fluxes
-    between R of cell i-1 and L of cell i (face index i)."""
+def muscl_rusanov_faces(U, eps, dt, dx, dy, local=True):
+    """advance_MUSCL_rusanov_adv (IMEX_Fluxes.cpp:120-324): per direction,
+    minmod half-slopes; with local (the SP111 local time step) a half step
+    of the states with their own fluxes (hdtdx = 0.5 * (dt/dx)); then
+    Rusanov fluxes between R of cell i-1 and L of cell i (face index i)."""
     out = []
     for d, h in ((0, dx), (1, dy)):
         hd = 0.5 * (dt / h)
         hdelta = 0.5 * minmod(U - _sh(U, d, -1), _sh(U, d, 1) - U)
         L = U - hdelta
         R = U + hdelta
-        lu = hd * (adv_flux(R, d, eps) - adv_flux(L, d, eps))
-        L = L - lu
-        R = R - lu
+        if local:
+            lu = hd * (adv_flux(R, d, eps) - adv_flux(L, d, eps))
+            L = L - lu
+            R = R - lu
         FL, FR = adv_flux(L, d, eps), adv_flux(R, d, eps)
         Rm, FRm = _sh(R, d, -1), _sh(FR, d, -1)
         a, b = np.abs(L[1 + d] / L[0]), np.abs(Rm[1 + d] / Rm[0])
@@ -119,29 +133,39 @@ def muscl_rusanov_faces(U, eps, dt, dx, dy):
     return out
 
 
+def advect(Uexp, Uimp, dt, dx, dy, eps, muscl, local):
+    """advance_rusanov_adv / advance_MUSCL_rusanov_adv: fluxes of the
+    explicit state Uexp applied to the state Uimp (IMEX_Fluxes.cpp:92-111,
+    290-310): Uimp - (dt/dx (Fx(i+1) - Fx(i)) + dt/dy (Fy(j+1) - Fy(j))).
+    Returns the updated state and the face fluxes."""
+    if muscl:
+        Fx, Fy = muscl_rusanov_faces(Uexp, eps, dt, dx, dy, local)
+    else:
+        Fx, Fy = rusanov_faces(Uexp, eps)
+    dtdx, dtdy = dt / dx, dt / dy
+    return Uimp - (dtdx * (_sh(Fx, 0, 1) - Fx) + dtdy * (_sh(Fy, 1, 1) - Fy)), Fx, Fy
+
+
 class PeekStage:
-    """One PEEK-SP111 step: explicit advection, pressure system, and after
-    the solve the flux correction and conservative update.  advection:
-    "rusanov" (advance_rusanov_adv) or "muscl_rusanov"."""
+    """One PEEK stage: explicit advection, pressure system, and after the
+    solve the flux correction; for SP111 also the conservative update.
+    advection: "rusanov" (advance_rusanov_adv) or "muscl_rusanov"."""
 
     def __init__(self, gamma=1.4, eps=1.0, advection="rusanov"):
         self.gamma = gamma
         self.eps = eps
         self.muscl = advection == "muscl_rusanov"
 
-    def _faces(self, U, dt, dx, dy):
-        if self.muscl:
-            return muscl_rusanov_faces(U, self.eps, dt, dx, dy)
-        return rusanov_faces(U, self.eps)
-
-    def assemble(self, U, p, dt, dx, dy=None):
-        """Returns Uex, enth, rhs, a, bX (ny, nx+1), bY (ny+1, nx), alpha,
-        beta and the Rusanov fluxes Fx, Fy."""
+    def assemble(self, U, p, dt, dx, dy=None, U_exp=None, local=True):
+        """compute_flux_imex_o1 up to the solve (IMEX_O1.cpp:64-125): the
+        advective fluxes of U_exp (default U) update U (the implicit stage
+        state); local: the MUSCL local half step (SP111 only).  Returns
+        Uex, enth, rhs, a, bX (ny, nx+1), bY (ny+1, nx), alpha, beta and the
+        advective fluxes Fx, Fy."""
         dy = dx if dy is None else dy
         g, eps = self.gamma, self.eps
-        Fx, Fy = self._faces(U, dt, dx, dy)
-        dtdx, dtdy = dt / dx, dt / dy
-        Uex = U - (dtdx * (_sh(Fx, 0, 1) - Fx) + dtdy * (_sh(Fy, 1, 1) - Fy))
+        Uexp = U if U_exp is None else U_exp
+        Uex, Fx, Fy = advect(Uexp, U, dt, dx, dy, eps, self.muscl, local)
         enth = p * g / ((g - 1) * Uex[0])
         ke = kinetic_energy(Uex, eps)
         hdtdx = 0.5 * dt * (1.0 / dx)
@@ -164,8 +188,9 @@ class PeekStage:
         return dict(Uex=Uex, enth=enth, rhs=rhs, a=a, bX=bX, bY=bY,
                     alpha=1.0, beta=dt * dt, Fx=Fx, Fy=Fy)
 
-    def finish(self, U, st, p_new, dt, dx, dy=None):
-        """Pressure and enthalpy flux corrections, then U_new = U - dt div F."""
+    def fluxes(self, st, p_new, dt, dx, dy=None):
+        """update_momentum_flux / update_energy_flux (IMEX_O1.cpp:485-566):
+        the stage's face fluxes after the solve (face i-1/2 at index i)."""
         dy = dx if dy is None else dy
         eps = self.eps
         heps = 0.5 / eps
@@ -180,5 +205,128 @@ class PeekStage:
             F[1 + d] += heps * (p_new + pm)
             F[3] += (0.5 * (h * ex[1 + d] + hm * _sh(ex[1 + d], d, -1))
                      - hd * ((h + hm) * (p_new - pm)))
-        dtdx, dtdy = dt / dx, dt / dy
-        return U - (dtdx * (_sh(Fx, 0, 1) - Fx) + dtdy * (_sh(Fy, 1, 1) - Fy))
+        return Fx, Fy
+
+    def finish(self, U, st, p_new, dt, dx, dy=None):
+        """SP111: the flux corrections, then the conservative update with the
+        weight b = 1 (sum_fluxes multiplies by 1 exactly)."""
+        dy = dx if dy is None else dy
+        Fx, Fy = self.fluxes(st, p_new, dt, dx, dy)
+        return conservative_update(U, [Fx], [Fy], [1.0], dt, dx, dy)
+
+
+def conservative_update(U, Fxs, Fys, b, dt, dx, dy):
+    """sum_fluxes (IMEX_RK.cpp:196-241: F = F_0 b_0, then F += F_s b_s) and
+    the one-flux conservative_update (Update.H:100-130):
+    U - dtdx (Fx(i+1) - Fx(i)) - dtdy (Fy(j+1) - Fy(j)), dtdx = dt (1/dx)."""
+    Fx = Fxs[0] * b[0]
+    Fy = Fys[0] * b[0]
+    for s in range(1, len(b)):
+        Fx = Fx + Fxs[s] * b[s]
+        Fy = Fy + Fys[s] * b[s]
+    dtdx = dt * (1.0 / dx)
+    dtdy = dt * (1.0 / dy)
+    return U - dtdx * (_sh(Fx, 0, 1) - Fx) - dtdy * (_sh(Fy, 1, 1) - Fy)
+
+
+def stage_states(U, Fxs, Fys, dt_imp, dt_exp, dx, dy):
+    """conservative_update into the two stage states (Update.H:218-302):
+    out = U, then for each earlier stage s
+    out -= dt_s (1/dx) (Fx_s(i+1) - Fx_s(i)) + dt_s (1/dy) (Fy_s(j+1) - Fy_s(j))
+    with dt_s = A[stage][s] dt for the implicit and the explicit row."""
+    idx, idy = 1.0 / dx, 1.0 / dy
+    out1, out2 = U.copy(), U.copy()
+    for s in range(len(Fxs)):
+        dFx = _sh(Fxs[s], 0, 1) - Fxs[s]
+        dFy = _sh(Fys[s], 1, 1) - Fys[s]
+        out1 = out1 - ((dt_imp[s] * idx) * dFx + (dt_imp[s] * idy) * dFy)
+        out2 = out2 - ((dt_exp[s] * idx) * dFx + (dt_exp[s] * idy) * dFy)
+    return out1, out2
+
+
+def pressure_ek(U_new, U_old, adt, dx, dy, gamma, eps, muscl):
+    """compute_pressure, EK branch (IMEX_O1.cpp:345-384): the previous
+    stage's explicit state advanced by its own advective fluxes over adt
+    (MUSCL without the local step), then
+    p = (gamma - 1)(E_new - 0.5 eps (m_ex . m_ex) / rho_new)."""
+    ex, _, _ = advect(U_old, U_old, adt, dx, dy, eps, muscl, False)
+    return (gamma - 1) * (U_new[3] - 0.5 * eps * (ex[1] * ex[1] + ex[2] * ex[2]) / U_new[0])
+
+
+class ButcherTableau:
+    """IMEXButcherTableau (ButcherTableau.cpp:5-101)."""
+    NAMES = ("SP111", "SSP222", "SASSP322", "SASSP332", "SSP433")
+
+    def __init__(self, name):
+        if name == "SP111":
+            self.n_imp, self.n_exp, self.order = 1, 1, 1
+            self.A_imp, self.A_exp = [[1.0]], [[0.0]]
+            self.b_imp = self.b_exp = [1.0]
+        elif name == "SSP222":
+            self.n_imp, self.n_exp, self.order = 2, 2, 2
+            al = 1 - 1 / math.sqrt(2)
+            self.A_imp = [[al, 0.0], [1 - 2 * al, al]]
+            self.A_exp = [[0.0, 0.0], [1.0, 0.0]]
+            self.b_imp = self.b_exp = [0.5, 0.5]
+        elif name == "SASSP322":
+            self.n_imp, self.n_exp, self.order = 3, 2, 2
+            self.A_imp = [[0.5, 0.0, 0.0], [-0.5, 0.5, 0.0], [0.0, 0.5, 0.5]]
+            self.A_exp = [[0.0, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 1.0, 0.0]]
+            self.b_imp = self.b_exp = [0.0, 0.5, 0.5]
+        elif name == "SASSP332":
+            self.n_imp, self.n_exp, self.order = 3, 3, 2
+            t = 1.0 / 3
+            self.A_imp = [[0.25, 0.0, 0.0], [0.0, 0.25, 0.0], [t, t, t]]
+            self.A_exp = [[0.0, 0.0, 0.0], [0.5, 0.0, 0.0], [0.5, 0.5, 0.0]]
+            self.b_imp = self.b_exp = [t, t, t]
+        elif name == "SSP433":
+            self.n_imp, self.n_exp, self.order = 4, 3, 3
+            al, be, et = 0.24169426078821, 0.06042356519705, 0.12915286960590
+            self.A_imp = [[al, 0.0, 0.0, 0.0], [-al, al, 0.0, 0.0], [0.0, 1 - al, al, 0.0],
+                          [be, et, 0.5 - be - et - al, al]]
+            self.A_exp = [[0.0, 0.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0], [0.0, 1.0, 0.0, 0.0],
+                          [0.0, 0.25, 0.25, 0.0]]
+            self.b_imp = self.b_exp = [0.0, 1.0 / 6, 1.0 / 6, 2.0 / 3]
+        else:
+            raise ValueError("unknown IMEX Butcher tableau %r" % (name,))
+        self.name = name
+
+    def imp_row(self, r, mult):
+        """get_A_imp_row (ButcherTableau.H:53-59)."""
+        return [a * mult for a in self.A_imp[r]]
+
+    def exp_row(self, r, mult):
+        """get_A_exp_row (ButcherTableau.H:61-67): n_imp entries."""
+        return [self.A_exp[r][i] * mult for i in range(self.n_imp)]
+
+
+def advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0, advection="rusanov",
+                    record=None):
+    """advance_imex_rk (IMEX_RK.cpp:22-96) on a 2D periodic level.  solve(sy,
+    p0) returns the stage's pressure (sy: the PeekStage.assemble system,
+    p0: the initial guess).  Returns (U_new, p) with p the last stage's
+    pressure.  record, a list, receives each stage's system, its states, the
+    initial and solved pressure and its corrected fluxes (Fx, Fy; the
+    advective ones as Fx_adv, Fy_adv)."""
+    tab = ButcherTableau(tableau) if isinstance(tableau, str) else tableau
+    st = PeekStage(gamma, eps, advection)
+    Fxs, Fys, exps = [], [], []
+    for s in range(tab.n_imp):
+        Uimp, Uexp = stage_states(U, Fxs, Fys, tab.imp_row(s, dt)[:s], tab.exp_row(s, dt)[:s],
+                                  dx, dy)
+        adt = tab.A_imp[s][s] * dt
+        if s > 0:
+            p = pressure_ek(Uexp, exps[-1], adt, dx, dy, gamma, eps, st.muscl)
+        sy = st.assemble(Uimp, p, adt, dx, dy, U_exp=Uexp, local=tab.name == "SP111")
+        p_in = p
+        p = solve(sy, p)
+        Fx, Fy = st.fluxes(sy, p, adt, dx, dy)
+        Fxs.append(Fx)
+        Fys.append(Fy)
+        exps.append(Uexp)
+        if record is not None:
+            rec = dict(sy)
+            rec.update(U_imp=Uimp, U_exp=Uexp, p_in=p_in, p=p, adt=adt, Fx=Fx, Fy=Fy,
+                       Fx_adv=sy["Fx"], Fy_adv=sy["Fy"])
+            record.append(rec)
+    return conservative_update(U, Fxs, Fys, tab.b_imp, dt, dx, dy), p

@@ -1,8 +1,11 @@
-"""GPU parity of the IMEX PEEK-SP111 stage kernels (bx_imex_*, through the
-C ABI) against the oracle's numpy restatement (oracle/imex_oracle.py).
+"""GPU parity of the IMEX PEEK stage kernels and of the IMEX Runge-Kutta
+step built on them (bx_imex_*, through the C ABI) against the oracle's numpy
+restatement (oracle/imex_oracle.py).
 
 Bar: bit-exact -- every output of the kernels is evaluated in the oracle's
 operation order with no contraction."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -151,3 +154,125 @@ def test_peek_assemble_rejects_aliased_outputs():
         st.free()
         dU.free()
         dp.free()
+
+
+def _rk_parity(U, p, dt, dx, dy, gamma, eps, tableau, advection, tag):
+    """One IMEX RK step on the GPU (ImexRK, through the C ABI) and in the
+    oracle (advance_imex_rk), with the stand-in solve p_new = pressure of
+    the stage's explicit state on both sides (bx_imex_pressure is
+    bit-exact with the oracle's pressure): every stage's system and the
+    step's state and pressure must be identical."""
+    ny, nx = p.shape
+    announce("imex rk %s %s %s %dx%d" % (tableau, advection, tag, nx, ny))
+    rec = []
+    U_ref, p_ref = io.advance_imex_rk(U, p, dt, dx, dy, tableau,
+                                      lambda sy, p0: io.pressure_from_state(sy["Uex"], gamma, eps),
+                                      gamma, eps, advection, record=rec)
+    rk = imex.ImexRK(nx, ny, dx, dy, gamma, eps, advection, tableau)
+    dU, dp, dout, dpn = dev(U), dev(p), DeviceArray(U.shape), DeviceArray(p.shape)
+    try:
+        seen = []
+
+        def solve(sy):
+            r = rec[sy["stage"]]
+            assert sy["adt"] == r["adt"] and sy["beta"] == r["beta"]
+            for key in ("Uex", "enth", "rhs", "bX", "bY"):
+                got = sy[key].download()
+                assert np.array_equal(got, r[key]), (tableau, sy["stage"], key,
+                                                      np.argwhere(got != r[key])[:3])
+            assert np.array_equal(sy["p_prev"].download(), r["p_in"]), (tableau, sy["stage"], "p_in")
+            rk.stage.pressure(sy["Uex"], dpn)
+            seen.append(sy["stage"])
+            return dpn
+
+        rk.advance(dU, dp, dt, dout, solve)
+        assert seen == list(range(rk.tab.n_imp))
+        got = dout.download()
+        assert np.array_equal(got, U_ref), (tableau, "U_new", np.argwhere(got != U_ref)[:3])
+        assert np.array_equal(dp.download(), p_ref)
+    finally:
+        rk.free()
+        for d in (dU, dp, dout, dpn):
+            d.free()
+
+
+@pytest.mark.parametrize("advection", ["rusanov", "muscl_rusanov"])
+@pytest.mark.parametrize("tableau", ["SP111", "SSP222", "SASSP322", "SASSP332", "SSP433"])
+def test_imex_rk_step_matches_oracle(tableau, advection):
+    """Every IMEX Butcher tableau of the reference, both advection fluxes,
+    random states on anisotropic cells, eps != 1, across the 64 x 4
+    thread-block edges and with a one-cell direction."""
+    gamma, eps = 1.4, 0.7
+    for ny, nx in ((20, 48), (3, 65), (1, 9)):
+        U = random_state(ny, nx, gamma, eps, seed=7 * nx + ny)
+        p = io.pressure_from_state(U, gamma, eps) * 1.01
+        _rk_parity(U, p, 0.009, 1.0 / nx, 0.6 / ny, gamma, eps, tableau, advection,
+                   "random")
+
+
+def test_imex_rk_ssp222_gresho_with_multigrid_solves():
+    """PEEK-MUSCL-SSP222 steps of the Gresho vortex at 256^2 with the
+    pressure of every stage solved on the GPU (plain multigrid, 1e-10): the
+    oracle, given the same solved pressures, reaches the identical state;
+    the GPU totals are conserved."""
+    n = 256
+    announce("imex rk ssp222 gresho 256")
+    wl = imex.GreshoWorkload(n, advection="muscl_rusanov", tableau="SSP222")
+    solver = imex.HelmholtzSolver(n, n, wl.dx)
+    ost_U = wl.U.download()
+    ost_p = wl.p.download()
+    try:
+        for k in range(3):
+            solved = []
+            U0 = wl.U.download()
+            dt = wl.step_dt()
+            assert dt == io.advective_dt(U0, wl.dx)
+            while True:
+                sy = wl.next_system(dt)
+                solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+                x = imex.dev(sy["p_prev"].download())
+                st = solver.solve(imex.dev(sy["rhs"].download()), x, "mg", rtol=1e-10, maxiter=100)
+                assert st["iters"] < 100
+                solved.append(x.download())
+                done = wl.advance(sy, x)
+                x.free()
+                if done:
+                    break
+            assert len(solved) == 2
+            it = iter(solved)
+            ost_U, ost_p = io.advance_imex_rk(ost_U, ost_p, dt, wl.dx, wl.dx, "SSP222",
+                                              lambda sy, p0: next(it), 1.4, 1.0, "muscl_rusanov")
+            got = wl.U.download()
+            assert np.array_equal(got, ost_U), (k, np.argwhere(got != ost_U)[:3])
+            assert np.array_equal(wl.p.download(), ost_p)
+            scale = np.abs(U0).sum(axis=(1, 2))
+            assert np.all(np.abs(got.sum(axis=(1, 2)) - U0.sum(axis=(1, 2))) <= 1e-12 * scale)
+    finally:
+        solver.close()
+        wl.free()
+
+
+def test_imex_rk_rejects_bad_arguments():
+    """Stage counts beyond the maximum, aliased outputs and unknown
+    tableaux are refused before any launch."""
+    from bamrex_amd import _lib
+    n = 8
+    L = _lib.lib()
+    dU, a1, a2 = DeviceArray((4, n, n)), DeviceArray((4, n, n)), DeviceArray((4, n, n))
+    try:
+        ptrs = (ctypes.c_void_p * 5)(*([a1.ptr.value] * 5))
+        w = (ctypes.c_double * 5)(*([0.1] * 5))
+        rc = L.bx_imex_rk_combine(dU.ptr, ptrs, ptrs, 5, w, w, n, n, 0.1, 0.1, a2.ptr, dU.ptr, None)
+        assert rc != 0 and b"bad argument" in L.bx_helm_last_error()
+        one = (ctypes.c_void_p * 4)(a1.ptr.value, None, None, None)
+        rc = L.bx_imex_rk_combine(dU.ptr, one, one, 1, w, w, n, n, 0.1, 0.1, a1.ptr, a2.ptr, None)
+        assert rc != 0    # a flux array is also an output
+        rc = L.bx_imex_rk_update(dU.ptr, one, one, 0, w, n, n, 0.1, 0.1, 0.01, a2.ptr, None)
+        assert rc != 0    # no stage
+        rc = L.bx_imex_rk_pressure(dU.ptr, a1.ptr, n, n, 0.1, 0.1, 0.01, 1.4, 1.0, 0, dU.ptr, None)
+        assert rc != 0    # p aliases the state
+    finally:
+        for d in (dU, a1, a2):
+            d.free()
+    with pytest.raises(ValueError):
+        imex.ImexRK(n, n, 0.1, tableau="RK4")

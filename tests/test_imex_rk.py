@@ -1,0 +1,279 @@
+"""CPU tests of the oracle's IMEX Runge-Kutta step (advance_imex_rk in
+oracle/imex_oracle.py), the checker of the bx_imex_rk_* kernels: the
+reference's tableau tests, the tableaux' order conditions, SP111 through
+the stage loop, the time order of SSP222, conservation and uniform-flow
+preservation with a real pressure solve."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
+
+import helm_oracle as ho  # noqa: E402
+import imex_oracle as io  # noqa: E402
+from bamrex_amd import imex  # noqa: E402
+
+
+def state_from_primv(rho, u, v, p, gamma, eps):
+    U = np.empty((4,) + rho.shape)
+    U[0] = rho
+    U[1] = rho * u
+    U[2] = rho * v
+    U[3] = p / (gamma - 1) + 0.5 * eps * rho * (u * u + v * v)
+    return U
+
+
+def direct_solve(dx, dy):
+    """A solve callback for advance_imex_rk: sparse LU of the periodic
+    5-point operator alpha a p - beta div(b grad p), assembled here and
+    checked against the oracle's operator (helm_oracle.apply) on a random
+    vector before use."""
+    def solve(sy, p0):
+        a, bX, bY, alpha, beta = sy["a"], sy["bX"], sy["bY"], sy["alpha"], sy["beta"]
+        ny, nx = a.shape
+        idx = np.arange(nx * ny).reshape(ny, nx)
+        rows, cols, vals = [], [], []
+        bw, be = bX[:, :nx], bX[:, 1:]
+        bs, bn = bY[:ny], bY[1:]
+        diag = alpha * a + beta * ((bw + be) / (dx * dx) + (bs + bn) / (dy * dy))
+        for nb, w in ((np.roll(idx, 1, 1), -beta * bw / (dx * dx)),
+                      (np.roll(idx, -1, 1), -beta * be / (dx * dx)),
+                      (np.roll(idx, 1, 0), -beta * bs / (dy * dy)),
+                      (np.roll(idx, -1, 0), -beta * bn / (dy * dy))):
+            rows.append(idx.ravel())
+            cols.append(nb.ravel())
+            vals.append(w.ravel())
+        rows.append(idx.ravel())
+        cols.append(idx.ravel())
+        vals.append(diag.ravel())
+        A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                          shape=(nx * ny, nx * ny))
+        x = np.random.default_rng(0).random((ny, nx))
+        ref = ho.apply(x, a, bX, bY, alpha, beta, dx, dy)
+        assert np.allclose((A @ x.ravel()).reshape(ny, nx), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+        return spla.spsolve(A.tocsc(), sy["rhs"].ravel()).reshape(ny, nx)
+    return solve
+
+
+def test_tableaux_match_the_reference_tests():
+    """Tests/IMEXButcherTableaux01.cpp: sizes, orders and entries."""
+    t = io.ButcherTableau("SP111")
+    assert (t.n_imp, t.n_exp, t.order) == (1, 1, 1)
+    assert t.A_imp == [[1]] and t.A_exp == [[0]] and t.b_imp == [1] and t.b_exp == [1]
+    t = io.ButcherTableau("SSP222")
+    assert (t.n_imp, t.n_exp, t.order) == (2, 2, 2)
+    g = 1 - 1.0 / math.sqrt(2.0)
+    assert t.A_exp == [[0, 0], [1, 0]]
+    assert np.allclose(t.A_imp, [[g, 0], [1 - 2 * g, g]], atol=1e-12, rtol=0)
+    assert t.b_imp == [0.5, 0.5]
+    t = io.ButcherTableau("SASSP322")
+    assert (t.n_imp, t.n_exp, t.order) == (3, 2, 2)
+    assert t.A_exp == [[0, 0, 0], [0, 0, 0], [0, 1, 0]]
+    assert t.A_imp == [[0.5, 0, 0], [-0.5, 0.5, 0], [0, 0.5, 0.5]]
+    assert t.b_imp == [0, 0.5, 0.5]
+    t = io.ButcherTableau("SASSP332")
+    assert (t.n_imp, t.n_exp, t.order) == (3, 3, 2)
+    assert t.A_exp == [[0, 0, 0], [0.5, 0, 0], [0.5, 0.5, 0]]
+    assert t.A_imp[0] == [0.25, 0, 0] and t.A_imp[1] == [0, 0.25, 0]
+    assert np.allclose(t.A_imp[2], [1 / 3.0] * 3, atol=1e-12, rtol=0)
+    assert np.allclose(t.b_imp, [1 / 3.0] * 3, atol=1e-12, rtol=0)
+    t = io.ButcherTableau("SSP433")
+    assert (t.n_imp, t.n_exp, t.order) == (4, 3, 3)
+    assert t.A_exp == [[0, 0, 0, 0], [0, 0, 0, 0], [0, 1, 0, 0], [0, 0.25, 0.25, 0]]
+    al, be, et = 0.24169426078821, 0.06042356519705, 0.12915286960590
+    assert np.allclose(t.A_imp, [[al, 0, 0, 0], [-al, al, 0, 0], [0, 1 - al, al, 0],
+                                 [be, et, 0.5 - be - et - al, al]], atol=1e-12, rtol=0)
+    for name in io.ButcherTableau.NAMES:
+        t = io.ButcherTableau(name)
+        assert t.b_imp == t.b_exp
+        for r in range(t.n_imp):
+            assert t.imp_row(r, 0.37) == [a * 0.37 for a in t.A_imp[r]]
+            assert len(t.exp_row(r, 0.37)) == t.n_imp
+    with pytest.raises(ValueError):
+        io.ButcherTableau("SSP999")
+
+
+@pytest.mark.parametrize("name", io.ButcherTableau.NAMES)
+def test_tableaux_satisfy_their_order_conditions(name):
+    """Additive Runge-Kutta order conditions (Pareschi & Russo 2005) up to
+    each tableau's stated order, for both parts and their coupling: a
+    mistyped entry fails one of them."""
+    t = io.ButcherTableau(name)
+    Ai, Ae = np.array(t.A_imp), np.array(t.A_exp)
+    b = np.array(t.b_imp)
+    ci, ce = Ai.sum(1), Ae.sum(1)
+    tol = 1e-10
+    assert abs(b.sum() - 1) < tol
+    # explicit part strictly lower triangular, implicit lower triangular
+    assert np.all(np.triu(Ae) == 0) and np.all(np.triu(Ai, 1) == 0)
+    if t.order >= 2:
+        for c in (ci, ce):
+            assert abs(b @ c - 0.5) < tol
+    if t.order >= 3:
+        for c1 in (ci, ce):
+            for c2 in (ci, ce):
+                assert abs(b @ (c1 * c2) - 1 / 3.0) < tol
+        for A in (Ai, Ae):
+            for c in (ci, ce):
+                assert abs(b @ (A @ c) - 1 / 6.0) < tol
+
+
+def test_sp111_stage_loop_equals_the_single_stage():
+    """SP111 through the stage loop (stage states equal to U, the MUSCL
+    local step, fluxes summed with weight 1) gives PeekStage's step bit for
+    bit, for both advection fluxes."""
+    gamma, eps = 1.4, 0.6
+    rng = np.random.default_rng(5)
+    ny, nx = 12, 20
+    U = state_from_primv(1 + 0.2 * rng.random((ny, nx)), rng.uniform(-1, 1, (ny, nx)),
+                         rng.uniform(-1, 1, (ny, nx)), 2 + rng.random((ny, nx)), gamma, eps)
+    p = 2 + rng.random((ny, nx))
+    dx, dy, dt = 1.0 / nx, 0.8 / ny, 0.011
+    for adv in ("rusanov", "muscl_rusanov"):
+        def stand_in(sy, p0):
+            return io.pressure_from_state(sy["Uex"], gamma, eps)
+        U1, p1 = io.advance_imex_rk(U, p, dt, dx, dy, "SP111", stand_in, gamma, eps, adv)
+        st = io.PeekStage(gamma, eps, adv)
+        sy = st.assemble(U, p, dt, dx, dy)
+        pn = stand_in(sy, p)
+        assert np.array_equal(p1, pn)
+        assert np.array_equal(U1, st.finish(U, sy, pn, dt, dx, dy))
+
+
+def test_stage_states_follow_the_tableau_rows():
+    """The SSP222 second stage: U_imp = U - (1 - 2 alpha) dt div F_0 and
+    U_exp = U - dt div F_0, and the EK pressure is formed from the first
+    explicit state (U) advanced over adt = alpha dt -- checked against a
+    per-cell evaluation in plain Python on a small grid."""
+    gamma, eps = 1.4, 1.0
+    rng = np.random.default_rng(8)
+    ny, nx = 5, 7
+    U = state_from_primv(1 + 0.2 * rng.random((ny, nx)), rng.uniform(-1, 1, (ny, nx)),
+                         rng.uniform(-1, 1, (ny, nx)), 2 + rng.random((ny, nx)), gamma, eps)
+    p = io.pressure_from_state(U, gamma, eps)
+    dx, dy, dt = 1.0 / nx, 1.0 / ny, 0.02
+    rec = []
+    io.advance_imex_rk(U, p, dt, dx, dy, "SSP222", lambda sy, p0: p0 * 1.001, gamma, eps,
+                       "muscl_rusanov", record=rec)
+    al = 1 - 1 / math.sqrt(2)
+    assert rec[0]["adt"] == al * dt and rec[1]["adt"] == al * dt
+    assert np.array_equal(rec[0]["U_imp"], U) and np.array_equal(rec[0]["U_exp"], U)
+    F0x, F0y = rec[0]["Fx"], rec[0]["Fy"]
+    for k in range(4):
+        for j in range(ny):
+            for i in range(nx):
+                dFx = F0x[k, j, (i + 1) % nx] - F0x[k, j, i]
+                dFy = F0y[k, (j + 1) % ny, i] - F0y[k, j, i]
+                wi = (1 - 2 * al) * dt
+                imp = U[k, j, i] - ((wi * (1 / dx)) * dFx + (wi * (1 / dy)) * dFy)
+                exp_ = U[k, j, i] - ((dt * (1 / dx)) * dFx + (dt * (1 / dy)) * dFy)
+                assert rec[1]["U_imp"][k, j, i] == imp
+                assert rec[1]["U_exp"][k, j, i] == exp_
+    # EK pressure of stage 1: momentum of U advanced by its own fluxes
+    # (MUSCL without the local step) over adt, density and energy of U_exp
+    ex, _, _ = io.advect(U, U, al * dt, dx, dy, eps, True, False)
+    Ue = rec[1]["U_exp"]
+    for j in range(ny):
+        for i in range(nx):
+            m2 = ex[1, j, i] * ex[1, j, i] + ex[2, j, i] * ex[2, j, i]
+            pe = (gamma - 1) * (Ue[3, j, i] - 0.5 * eps * m2 / Ue[0, j, i])
+            assert rec[1]["p_in"][j, i] == pe
+
+
+def _wave_errors(n, tableau, advection, solve_kind):
+    """L1 errors of rho and p after advecting rho = 1 + 0.2 sin(2 pi x) with
+    u = 1, p = 1 to t = 0.25 on an n x 4 periodic grid at CFL 0.4."""
+    gamma, eps = 1.4, 1.0
+    x = (np.arange(n) + 0.5) / n
+    rho0 = 1.0 + 0.2 * np.sin(2 * np.pi * x)[None, :].repeat(4, 0)
+    U = state_from_primv(rho0, np.ones((4, n)), np.zeros((4, n)), np.ones((4, n)), gamma, eps)
+    dx, dy = 1.0 / n, 0.25
+    solve = direct_solve(dx, dy) if solve_kind == "direct" else (lambda sy, p0: np.ones_like(p0))
+    p = np.ones((4, n))
+    t, t_end = 0.0, 0.25
+    while t < t_end - 1e-14:
+        dt = min(0.4 * dx, t_end - t)
+        U, p = io.advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma, eps, advection)
+        t += dt
+    exact = 1.0 + 0.2 * np.sin(2 * np.pi * (x - t_end))
+    e_rho = np.mean(np.abs(U[0, 0] - exact))
+    e_p = np.mean(np.abs(io.pressure_from_state(U, gamma, eps) - 1.0))
+    return e_rho, e_p
+
+
+def test_ssp222_is_second_order_in_time():
+    """MUSCL fluxes without the local step are second order in space only:
+    the SSP222 stages make the step second order (error ratio ~4 per
+    halving of dx and dt), where a first-order time combination -- the
+    planted mistake of an SSP222 whose explicit row takes half the first
+    stage's flux -- drops to ratio ~2."""
+    e = [_wave_errors(n, "SSP222", "muscl_rusanov", "stand_in")[0] for n in (32, 64, 128)]
+    assert e[1] / e[2] > 3.0, e
+    bad = io.ButcherTableau("SSP222")
+    bad.A_exp = [[0.0, 0.0], [0.5, 0.0]]
+    eb = [_wave_errors(n, bad, "muscl_rusanov", "stand_in")[0] for n in (32, 64, 128)]
+    assert eb[1] / eb[2] < 2.5, eb
+    assert eb[2] > 3 * e[2], (e, eb)
+
+
+def test_solved_pressure_of_an_advected_wave():
+    """With the pressure solved at every stage: SP111 keeps the uniform
+    pressure of the advected wave to rounding (its energy update and
+    pressure equation are one identity), and the multi-stage tableaux keep
+    it to first order in dx.  The first-order part comes from the EK
+    pressure of the later stages (compute_pressure, IMEX_O1.cpp:345-384),
+    which combines the momentum of the previous explicit state advanced
+    over A_ss dt with the density and energy of the current explicit state
+    (advanced over its whole row of A_exp); rho still converges faster than
+    first order."""
+    for adv in ("rusanov", "muscl_rusanov"):
+        r = [_wave_errors(n, "SP111", adv, "direct") for n in (32, 64)]
+        assert max(x[1] for x in r) < 1e-14, r
+    r = [_wave_errors(n, "SSP222", "muscl_rusanov", "direct") for n in (32, 64, 128)]
+    assert r[1][0] / r[2][0] > 2.4, r
+    assert 1.8 < r[1][1] / r[2][1] < 2.3 and r[2][1] < 1e-3, r
+
+
+@pytest.mark.parametrize("tableau", ["SSP222", "SASSP322", "SASSP332", "SSP433"])
+def test_multistage_conserves_and_keeps_uniform_flow(tableau):
+    """Every tableau: uniform flow unchanged, and the Gresho vortex's mass,
+    momentum and energy totals conserved with a real pressure solve."""
+    gamma, eps = 1.4, 1.0
+    n = 16
+    U = state_from_primv(np.full((n, n), 1.2), np.full((n, n), 0.3), np.full((n, n), -0.2),
+                         np.full((n, n), 2.0), gamma, eps)
+    dx = 1.0 / n
+    U1, p1 = io.advance_imex_rk(U, io.pressure_from_state(U, gamma, eps), 0.02, dx, dx, tableau,
+                                direct_solve(dx, dx), gamma, eps, "muscl_rusanov")
+    assert np.allclose(U1, U, rtol=1e-12, atol=0)
+    assert np.allclose(p1, 2.0, rtol=1e-12, atol=0)
+    Ug, dxg = imex.gresho_state(32)
+    pg = io.pressure_from_state(Ug, gamma, eps)
+    dtg = io.advective_dt(Ug, dxg)
+    Un, pn = io.advance_imex_rk(Ug, pg, dtg, dxg, dxg, tableau, direct_solve(dxg, dxg), gamma, eps,
+                                "muscl_rusanov")
+    scale = np.abs(Ug).sum(axis=(1, 2))
+    assert np.all(np.abs(Un.sum(axis=(1, 2)) - Ug.sum(axis=(1, 2))) <= 1e-13 * scale)
+    # the vortex is steady: one step at 32^2 moves rho by the discretisation
+    # error (about 1e-2, as SP111 does) and the pressure by ~1e-5 relative
+    assert np.max(np.abs(Un[0] - Ug[0])) < 2e-2
+    assert np.max(np.abs(pn - pg)) < 1e-4 * pg.max()
+
+
+def test_product_tableaux_equal_the_oracle_tableaux():
+    """The host's tableaux (bamrex_amd.imex.ButcherTableau, which drives the
+    GPU stages) are the oracle's, entry for entry and row for row."""
+    for name in io.ButcherTableau.NAMES:
+        a, b = imex.ButcherTableau(name), io.ButcherTableau(name)
+        assert (a.n_imp, a.n_exp, a.order) == (b.n_imp, b.n_exp, b.order)
+        assert a.A_imp == b.A_imp and a.A_exp == b.A_exp and a.b == b.b_imp
+        for r in range(a.n_imp):
+            assert a.imp_row(r, 0.013) == b.imp_row(r, 0.013)
+            assert a.exp_row(r, 0.013) == b.exp_row(r, 0.013)
+    assert imex.ButcherTableau.NAMES == io.ButcherTableau.NAMES

@@ -155,8 +155,8 @@ _SIGS = {
                                            P, P]),
     "bx_imex_stage_assemble": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, P, P, P, P, P, P,
-                                              P]),
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              P, P, P, P, P, P, P, P, P]),
     "bx_imex_stage_fluxes": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                             ctypes.c_int, P, P, P]),
@@ -164,8 +164,8 @@ _SIGS = {
                                           ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, P, P,
                                           P]),
     "bx_imex_rk_pressure": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, P,
-                                           P]),
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_int, P, P]),
     "bx_imex_rk_update": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int, DP,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, P, P]),
@@ -187,6 +187,7 @@ _SIGS = {
 BX_SOLVE = {"mg": 0, "fgmres_mg": 1, "fgmres_cnn": 2, "mg_fused": 3}
 BX_IMEX_ADVECTION = {"rusanov": 0, "muscl_rusanov": 1}
 BX_IMEX_MAX_STAGES = 4
+BX_IMEX_KE = {"ex": 0, "split": 1}
 BX_CNN_MODE = {"smoother": 0, "corrector": 1}
 BX_CNN_NPARAMS = 4896
 

@@ -1,33 +1,44 @@
-// Explicit work of the IMEX PEEK-SP111 stage on one 2D periodic level: the
-// Rusanov advective update, the enthalpy, the pressure-equation source and
-// coefficients, and after the pressure solve the momentum/energy flux
-// corrections with the conservative update.  In the reference this is host
-// code around MLMG (CPU-only update kernels):
+// Explicit work of the IMEX PEEK stages on one 2D periodic level, and the
+// IMEX Runge-Kutta step around them: the Rusanov / MUSCL-Rusanov advective
+// update, the enthalpy, the pressure-equation source and coefficients,
+// after the pressure solve the momentum/energy flux corrections, and for
+// the multi-stage tableaux the stage states from stored stage fluxes, the
+// EK pressure of later stages and the weighted flux sum with the update.
+// In the reference this is host code around MLMG (the multi-flux updates
+// are CPU-only there):
 //   compute_flux_imex_o1            Source/IMEX/IMEX_O1.cpp:18-175
 //   advance_rusanov_adv             Source/IMEX/IMEX_Fluxes.cpp:10-118
-//   advance_MUSCL_rusanov_adv (SP111 local step)
-//                                   Source/IMEX/IMEX_Fluxes.cpp:120-324
+//   advance_MUSCL_rusanov_adv       Source/IMEX/IMEX_Fluxes.cpp:120-324
 //   adv_flux / kinetic_energy / pressure
 //                                   Source/IMEX_K/euler_K.H:8-65
 //   compute_enthalpy (reuse_pressure)
 //                                   Source/IMEX/IMEX_O1.cpp:177-228
+//   compute_pressure (EK branch)    Source/IMEX/IMEX_O1.cpp:300-391
 //   pressure_source_vector          Source/IMEX/IMEX_O1.cpp:416-452
 //   setup_pressure_op (a, face b)   Source/IMEX/IMEX_O1.cpp:570-614
 //   update_momentum_flux / update_energy_flux
-//                                   Source/IMEX/IMEX_O1.cpp:480-566
-//   conservative_update (SP111)     Source/IMEX/IMEX_RK.cpp:22-96
+//                                   Source/IMEX/IMEX_O1.cpp:485-566
+//   advance_imex_rk, sum_fluxes     Source/IMEX/IMEX_RK.cpp:22-96, 196-241
+//   conservative_update             Source/Fluxes/Update.H:74-130, 218-302
 //   max_dx_scaled_speed             Source/System/Euler/Euler.cpp:107-129
 //
 // Layout: component-major (rho, m_x, m_y, E), x fastest, nx*ny cells per
 // component, no ghost cells (periodic neighbours are wrapped in the
-// kernels).  Face arrays: bX is ny rows of nx+1 faces, bY ny+1 rows of nx.
-// Every expression keeps the reference's operation order, so results are
-// bit-identical to a CPU evaluation without contraction.
+// kernels).  Face arrays: bX is ny rows of nx+1 faces, bY ny+1 rows of nx;
+// stage fluxes hold the lower face of every cell.  Every expression keeps
+// the reference's operation order, so results are bit-identical to a CPU
+// evaluation without contraction.
 //
 // Kernels and HBM bytes per cell (reads + writes, neighbours from cache):
-//   k_peek_advect   U -> Uex                       32 + 32
-//   k_peek_system   Uex, p -> enth, rhs, a, bX, bY 40 + 40
-//   k_peek_finish   U, Uex(m), enth, p -> U_new    64 + 32
+//   k_peek_advect    U_exp, U_imp -> Uex             64 + 32
+//   k_peek_system    Uex, p -> enth, rhs, a, bX, bY  40 + 40
+//                    (split: + U_exp 24, + cX, cY 16)
+//   k_rk_pressure_sk U_exp new, old -> p (split)     48 + 8
+//   k_peek_finish    U, Uex(m), enth, p -> U_new     64 + 32   (SP111)
+//   k_stage_fluxes   U_exp, Uex(m), enth, p -> F     64 + 64
+//   k_rk_combine     U, 2 s stage fluxes -> 2 states 32 + 64 s + 64
+//   k_rk_pressure_ek U_exp new (rho, E), old -> p    48 + 8
+//   k_rk_update      U, 2 n stage fluxes -> U_new    32 + 64 n + 32
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -242,11 +253,15 @@ __device__ __forceinline__ double peek_enth(const double *__restrict__ p,
 // (pressure_source_vector) and the operator coefficients a = eps/(gamma-1),
 // b = face average of the enthalpy (average_cellcenter_to_face:
 // 0.5 (h(i-1) + h(i))).  The periodic image faces (bX column nx, bY row ny)
-// repeat faces 0.
+// repeat faces 0.  SPLIT (ke_method split, compute_ke / compute_velocity,
+// IMEX_O1.cpp:232-298): ke = 0.5 eps (m_e . m_ex) / rho_e of the stage's
+// explicit state Ue, and the operator's velocity c = m_e / rho_e.
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_peek_system(
-    const double *__restrict__ Uex, const double *__restrict__ p, double *__restrict__ enth,
-    double *__restrict__ rhs, double *__restrict__ acoef, double *__restrict__ bX,
-    double *__restrict__ bY, PeekGeom g, double adia, double eps, double hdtdx, double hdtdy)
+    const double *__restrict__ Ue, const double *__restrict__ Uex, const double *__restrict__ p,
+    double *__restrict__ enth, double *__restrict__ rhs, double *__restrict__ acoef,
+    double *__restrict__ bX, double *__restrict__ bY, double *__restrict__ cX,
+    double *__restrict__ cY, PeekGeom g, double adia, double eps, double hdtdx, double hdtdy)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -264,7 +279,16 @@ __global__ __launch_bounds__(256) void k_peek_system(
     const double    he  = peek_enth(p, Uex, ce, adia);
     const double    hs  = peek_enth(p, Uex, cs, adia);
     const double    hn  = peek_enth(p, Uex, cn, adia);
-    const double    ke  = peek_ke(Uex[c], Uex[c + n], Uex[c + 2 * n], eps);
+    double          ke;
+    if (SPLIT)
+    {
+        const double r0 = Ue[c], m1 = Ue[c + n], m2 = Ue[c + 2 * n];
+        ke    = 0.5 * eps * (m1 * Uex[c + n] + m2 * Uex[c + 2 * n]) / r0;
+        cX[c] = m1 / r0;
+        cY[c] = m2 / r0;
+    }
+    else
+        ke = peek_ke(Uex[c], Uex[c + n], Uex[c + 2 * n], eps);
     rhs[c] = eps
              * (Uex[c + 3 * n] - ke
                 - (hdtdx * (he * Uex[ce + n] - hw * Uex[cw + n])
@@ -439,6 +463,22 @@ __global__ __launch_bounds__(256) void k_rk_pressure_ek(const double *__restrict
     p[c] = (adia - 1) * (Un[c + 3 * g.n] - 0.5 * eps * (e1 * e1 + e2 * e2) / Un[c]);
 }
 
+// SK pressure of a later stage (compute_pressure, IMEX_O1.cpp:326-344):
+//   p = (gamma - 1)(E_new - 0.5 eps (m_old . m_new) / rho_old).
+__global__ __launch_bounds__(256) void k_rk_pressure_sk(const double *__restrict__ Un,
+                                                        const double *__restrict__ Uo,
+                                                        double *__restrict__ p, PeekGeom g,
+                                                        double adia, double eps)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    const long long c = (long long)j * g.nx + i, n = g.n;
+    p[c] = (adia - 1)
+           * (Un[c + 3 * n] - 0.5 * eps * (Uo[c + n] * Un[c + n] + Uo[c + 2 * n] * Un[c + 2 * n]) / Uo[c]);
+}
+
 // Step end of a multi-stage tableau: sum_fluxes (IMEX_RK.cpp:196-241,
 // F = F_0 b_0, then F += F_s b_s) at the two faces of each direction and
 // the conservative update (Update.H:100-130)
@@ -601,20 +641,24 @@ static bool adv_ok(int advection_flux)
 // fluxes of Ue applied to Ui (Uex), then the pressure system
 static int stage_assemble(const char *who, const double *Ui, const double *Ue, const double *p,
                           int nx, int ny, double dx, double dy, double dt, double gamma,
-                          double eps, int advection_flux, int local, double *Uex, double *enth,
-                          double *rhs, double *a, double *bX, double *bY, void *stream)
+                          double eps, int advection_flux, int local, int ke_method, double *Uex,
+                          double *enth, double *rhs, double *a, double *bX, double *bY,
+                          double *cX, double *cY, void *stream)
 {
+    const bool split = ke_method == BX_IMEX_KE_SPLIT;
     if (!Ui || !Ue || !p || !Uex || !enth || !rhs || !a || !bX || !bY
-        || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0) || !adv_ok(advection_flux))
+        || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0) || !adv_ok(advection_flux)
+        || (ke_method != BX_IMEX_KE_EX && !split) || (split && (!cX || !cY)))
         return bx_helm_fail(BX_ERR_INVALID, std::string(who) + ": bad argument");
     // outputs are written while inputs and other outputs are read
-    const void *outs[6] = { Uex, enth, rhs, a, bX, bY };
-    for (int i = 0; i < 6; ++i)
+    const int   nout    = split ? 8 : 6;
+    const void *outs[8] = { Uex, enth, rhs, a, bX, bY, cX, cY };
+    for (int i = 0; i < nout; ++i)
     {
         if (outs[i] == (const void *)Ui || outs[i] == (const void *)Ue
             || outs[i] == (const void *)p)
             return bx_helm_fail(BX_ERR_INVALID, std::string(who) + ": output aliases an input");
-        for (int j = i + 1; j < 6; ++j)
+        for (int j = i + 1; j < nout; ++j)
             if (outs[i] == outs[j])
                 return bx_helm_fail(BX_ERR_INVALID, std::string(who) + ": outputs alias");
     }
@@ -635,8 +679,13 @@ static int stage_assemble(const char *who, const double *Ui, const double *Ue, c
         hipLaunchKernelGGL((k_peek_advect<false, false>), peek_grid(nx, ny), dim3(64, 4), 0, s,
                            Ue, Ui, Uex, g, eps, dtdx, dtdy);
     PEEK_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_peek_system, peek_grid(nx, ny), dim3(64, 4), 0, s, Uex, p, enth, rhs,
-                       a, bX, bY, g, gamma, eps, hdtdx, hdtdy);
+    if (split)
+        hipLaunchKernelGGL(k_peek_system<true>, peek_grid(nx, ny), dim3(64, 4), 0, s, Ue, Uex, p,
+                           enth, rhs, a, bX, bY, cX, cY, g, gamma, eps, hdtdx, hdtdy);
+    else
+        hipLaunchKernelGGL(k_peek_system<false>, peek_grid(nx, ny), dim3(64, 4), 0, s, Ue, Uex, p,
+                           enth, rhs, a, bX, bY, (double *)nullptr, (double *)nullptr, g, gamma,
+                           eps, hdtdx, hdtdy);
     PEEK_CHECK(hipGetLastError());
     return BX_OK;
 }
@@ -648,19 +697,20 @@ extern "C" int bx_imex_peek_assemble(const double *U, const double *p, int nx, i
                                      void *stream)
 {
     return stage_assemble("bx_imex_peek_assemble", U, U, p, nx, ny, dx, dy, dt, gamma, eps,
-                          advection_flux, 1, Uex, enth, rhs, a, bX, bY, stream);
+                          advection_flux, 1, BX_IMEX_KE_EX, Uex, enth, rhs, a, bX, bY, nullptr,
+                          nullptr, stream);
 }
 
 extern "C" int bx_imex_stage_assemble(const double *U_imp, const double *U_exp, const double *p,
                                       int nx, int ny, double dx, double dy, double adt,
                                       double gamma, double eps, int advection_flux,
-                                      int local_time_step, double *Uex, double *enth,
-                                      double *rhs, double *a, double *bX, double *bY,
-                                      void *stream)
+                                      int local_time_step, int ke_method, double *Uex,
+                                      double *enth, double *rhs, double *a, double *bX,
+                                      double *bY, double *cX, double *cY, void *stream)
 {
     return stage_assemble("bx_imex_stage_assemble", U_imp, U_exp, p, nx, ny, dx, dy, adt, gamma,
-                          eps, advection_flux, local_time_step, Uex, enth, rhs, a, bX, bY,
-                          stream);
+                          eps, advection_flux, local_time_step, ke_method, Uex, enth, rhs, a, bX,
+                          bY, cX, cY, stream);
 }
 
 extern "C" int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
@@ -777,13 +827,22 @@ extern "C" int bx_imex_rk_combine(const double *U, const double *const *Fx,
 
 extern "C" int bx_imex_rk_pressure(const double *U_exp_new, const double *U_exp_old, int nx,
                                    int ny, double dx, double dy, double adt, double gamma,
-                                   double eps, int advection_flux, double *p, void *stream)
+                                   double eps, int advection_flux, int ke_method, double *p,
+                                   void *stream)
 {
     if (!U_exp_new || !U_exp_old || !p || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0)
         || !adv_ok(advection_flux) || (const double *)p == U_exp_new
-        || (const double *)p == U_exp_old)
+        || (const double *)p == U_exp_old
+        || (ke_method != BX_IMEX_KE_EX && ke_method != BX_IMEX_KE_SPLIT))
         return bx_helm_fail(BX_ERR_INVALID, "bx_imex_rk_pressure: bad argument");
-    PeekGeom     g{ nx, ny, (long long)nx * ny };
+    PeekGeom g{ nx, ny, (long long)nx * ny };
+    if (ke_method == BX_IMEX_KE_SPLIT)
+    {
+        hipLaunchKernelGGL(k_rk_pressure_sk, peek_grid(nx, ny), dim3(64, 4), 0,
+                           (hipStream_t)stream, U_exp_new, U_exp_old, p, g, gamma, eps);
+        PEEK_CHECK(hipGetLastError());
+        return BX_OK;
+    }
     const double dtdx = adt / dx;
     const double dtdy = adt / dy;
     if (advection_flux == BX_IMEX_MUSCL_RUSANOV)

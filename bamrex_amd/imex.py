@@ -240,16 +240,28 @@ class ImexRK:
     reference's pressure MultiFab does: the EK pressure of a later stage,
     then each solution; after the step, the last stage's.  SP111 ends with
     the fused finish (bx_imex_peek_finish), later tableaux store the stage
-    fluxes and sum them (bx_imex_rk_update)."""
+    fluxes and sum them (bx_imex_rk_update).
+
+    ke_method (imex.ke_method): "ex" (PEEK) or "split" (PESK); with split the
+    system carries the operator's velocity term (gamma_c and the device
+    arrays cX, cY) for HelmholtzSolver.set_coeffs."""
 
     def __init__(self, nx, ny, dx, dy=None, gamma=1.4, eps=1.0, advection="rusanov",
-                 tableau="SP111"):
+                 tableau="SP111", ke_method="ex"):
         self.tab = ButcherTableau(tableau)
+        if ke_method not in _lib.BX_IMEX_KE:
+            raise ValueError("imex.ke_method: ex or split, not %r" % (ke_method,))
+        self.ke = _lib.BX_IMEX_KE[ke_method]
+        self.split = ke_method == "split"
         self.stage = PeekStage(nx, ny, dx, dy, gamma, eps, advection)
         self.nx, self.ny, self.dx, self.dy = nx, ny, self.stage.dx, self.stage.dy
         self.gamma, self.eps, self.adv = gamma, eps, self.stage.adv
         ns = self.tab.n_imp
+        # SP111 ends with the fused finish: it recomputes the stage fluxes per
+        # cell (their pressure corrections do not depend on ke_method)
         self.fused = self.tab.name == "SP111"
+        self.cX = DeviceArray((ny, nx)) if self.split else None
+        self.cY = DeviceArray((ny, nx)) if self.split else None
         self.Fx = [] if self.fused else [DeviceArray((4, ny, nx)) for _ in range(ns)]
         self.Fy = [] if self.fused else [DeviceArray((4, ny, nx)) for _ in range(ns)]
         self.U_imp = DeviceArray((4, ny, nx)) if ns > 1 else None
@@ -274,17 +286,20 @@ class ImexRK:
                                           _dbls(tab.imp_row(s, dt)[:s]), _dbls(tab.exp_row(s, dt)[:s]),
                                           nx, ny, dx, dy, Ui.ptr, Ue.ptr, None), "bx_imex_rk_combine")
             _chk(lib().bx_imex_rk_pressure(Ue.ptr, self._exp[s - 1].ptr, nx, ny, dx, dy, adt,
-                                           self.gamma, self.eps, self.adv, self.p.ptr, None),
+                                           self.gamma, self.eps, self.adv, self.ke, self.p.ptr, None),
                  "bx_imex_rk_pressure")
         self._exp.append(Ue)
         self._Ue, self._adt = Ue, adt
         st = self.stage
         _chk(lib().bx_imex_stage_assemble(Ui.ptr, Ue.ptr, self.p.ptr, nx, ny, dx, dy, adt, self.gamma,
-                                          self.eps, self.adv, int(tab.name == "SP111"), st.Uex.ptr,
-                                          st.enth.ptr, st.rhs.ptr, st.a.ptr, st.bX.ptr, st.bY.ptr, None),
+                                          self.eps, self.adv, int(tab.name == "SP111"), self.ke,
+                                          st.Uex.ptr, st.enth.ptr, st.rhs.ptr, st.a.ptr, st.bX.ptr,
+                                          st.bY.ptr, self.cX.ptr if self.split else None,
+                                          self.cY.ptr if self.split else None, None),
              "bx_imex_stage_assemble")
         return dict(Uex=st.Uex, enth=st.enth, rhs=st.rhs, a=st.a, bX=st.bX, bY=st.bY, alpha=1.0,
-                    beta=adt * adt, dt=dt, adt=adt, stage=s, p_prev=self.p)
+                    beta=adt * adt, gamma_c=(-0.5 * self.eps * adt) if self.split else 0.0,
+                    cX=self.cX, cY=self.cY, dt=dt, adt=adt, stage=s, p_prev=self.p)
 
     def complete(self, p_new, U_out):
         """The stage's fluxes with the solved pressure p_new; after the last
@@ -321,7 +336,8 @@ class ImexRK:
                 return
 
     def free(self):
-        for d in self.Fx + self.Fy + self.U_exp + ([self.U_imp] if self.U_imp else []):
+        for d in (self.Fx + self.Fy + self.U_exp + ([self.U_imp] if self.U_imp else [])
+                  + ([self.cX, self.cY] if self.split else [])):
             d.free()
         self.stage.free()
 

@@ -451,6 +451,13 @@ int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
  *   bx_imex_stage_assemble, the pressure solve, bx_imex_stage_fluxes
  * and after the last stage bx_imex_rk_update. */
 #define BX_IMEX_MAX_STAGES 4
+/* imex.ke_method (IMEXSettings, Source/IMEX/IMEX_O1.cpp:232-298): the
+ * kinetic energy of the explicit state (ex: the PEEK schemes) or split,
+ * 0.5 eps (m_e . m_ex)/rho_e with the explicit stage state e (the PESK
+ * schemes, whose operator has the velocity term gamma_c c . grad p with
+ * gamma_c = -eps adt / 2 and c = m_e / rho_e). */
+#define BX_IMEX_KE_EX 0
+#define BX_IMEX_KE_SPLIT 1
 /* The stage states (conservative_update into two outputs, Source/Fluxes/
  * Update.H:218-302): U_imp = U - sum_{s<nstage} dt_imp[s] div F_s and
  * U_exp likewise with dt_exp, dt_x[s] = A_x[stage][s] dt (host arrays of
@@ -459,26 +466,30 @@ int bx_imex_rk_combine(const double *U, const double *const *Fx,
                        const double *const *Fy, int nstage, const double *dt_imp,
                        const double *dt_exp, int nx, int ny, double dx, double dy,
                        double *U_imp, double *U_exp, void *stream);
-/* The pressure of a later stage, EK form (compute_pressure, Source/IMEX/
- * IMEX_O1.cpp:300-391, linearised with ke_method ex): the previous
+/* The pressure of a later stage (compute_pressure, Source/IMEX/
+ * IMEX_O1.cpp:300-391, linearised).  ke_method ex (EK): the previous
  * explicit state advanced over adt by its own advective fluxes (MUSCL
  * without the local step) supplies the momentum, U_exp_new the density
- * and energy: p = (gamma-1)(E - 0.5 eps |m_ex|^2 / rho). */
+ * and energy: p = (gamma-1)(E - 0.5 eps |m_ex|^2 / rho).  ke_method split
+ * (SK): p = (gamma-1)(E_new - 0.5 eps (m_old . m_new) / rho_old). */
 int bx_imex_rk_pressure(const double *U_exp_new, const double *U_exp_old, int nx,
                         int ny, double dx, double dy, double adt, double gamma,
-                        double eps, int advection_flux, double *p, void *stream);
+                        double eps, int advection_flux, int ke_method, double *p,
+                        void *stream);
 /* compute_flux_imex_o1 up to the solve with separate stage states: the
  * advective fluxes of U_exp update U_imp; local_time_step selects the
  * MUSCL local half step (the reference takes it for SP111 only,
  * IMEX_O1.cpp:95-102).  Outputs as bx_imex_peek_assemble, whose call is
- * this one with U_imp = U_exp = U, local_time_step = 1; operator scalars
- * alpha = 1, beta = adt*adt. */
+ * this one with U_imp = U_exp = U, local_time_step = 1, ke_method ex;
+ * operator scalars alpha = 1, beta = adt*adt, gamma_c = 0 (ex) or
+ * -0.5 eps adt (split).  cX, cY (nx*ny each, written for split, may be
+ * null for ex): the velocity coefficients of bx_helm_set_coeffs. */
 int bx_imex_stage_assemble(const double *U_imp, const double *U_exp, const double *p,
                            int nx, int ny, double dx, double dy, double adt,
                            double gamma, double eps, int advection_flux,
-                           int local_time_step, double *Uex, double *enth,
-                           double *rhs, double *a, double *bX, double *bY,
-                           void *stream);
+                           int local_time_step, int ke_method, double *Uex,
+                           double *enth, double *rhs, double *a, double *bX,
+                           double *bY, double *cX, double *cY, void *stream);
 /* The stage's fluxes after the solve (update_momentum_flux /
  * update_energy_flux, IMEX_O1.cpp:485-566): the advective fluxes of U_exp
  * corrected with the solved pressure p_new, into Fx, Fy. */

@@ -4,11 +4,12 @@ TEST INFRASTRUCTURE ONLY: the checker for bx_imex_* (bamrex_amd/csrc/
 bx_imex.hip).  Only tests/, __graft_entry__.smoke() and bench.py's CPU
 baseline may import it; the product path never does.
 
-Settings restated: linearised PEEK, enthalpy from the previous pressure
+Settings restated: linearised, enthalpy from the previous pressure
 (IMEXSettings::reuse_pressure), kinetic energy of the explicit state
-(IMEXSettings::ex), 2D periodic; one implicit stage with adt = dt (SP111,
-PeekStage) or the stage loop of any of the reference's IMEX Runge-Kutta
-tableaux (advance_imex_rk).
+(ke_method ex: PEEK) or split (ke_method split: PESK, with the velocity
+term of the pressure operator), 2D periodic; one implicit stage with
+adt = dt (SP111, PeekStage) or the stage loop of any of the reference's
+IMEX Runge-Kutta tableaux (advance_imex_rk).
 
   kinetic_energy / pressure / adv_flux   Source/IMEX_K/euler_K.H:8-65
   max_dx_scaled_speed                    Source/System/Euler/Euler.cpp:107-129
@@ -25,7 +26,9 @@ tableaux (advance_imex_rk).
   conservative_update (stage states)     Source/Fluxes/Update.H:218-302
   sum_fluxes                             Source/IMEX/IMEX_RK.cpp:196-241
   advance_imex_rk (stage loop)           Source/IMEX/IMEX_RK.cpp:22-96
-  compute_pressure (EK branch)           Source/IMEX/IMEX_O1.cpp:300-391
+  compute_pressure (EK and SK branches)  Source/IMEX/IMEX_O1.cpp:300-391
+  compute_ke (ex, split), compute_velocity
+                                         Source/IMEX/IMEX_O1.cpp:232-298
   IMEXButcherTableau                     Source/IMEX/ButcherTableau.cpp:5-101
 
 Arrays are (ny, nx), x fastest; U has 4 components (rho, m_x, m_y, E).
@@ -38,7 +41,9 @@ Tests/IMEX_O1_01.cpp (ExplicitKEEnergyMatch), free-stream preservation and
 conservation, brute-force per-cell evaluation of the advective dt, and for
 the MUSCL form: equality with the first-order fluxes on piecewise-constant
 data (minmod vanishes at a single jump) and second-order convergence on a
-smooth advected density wave.  The stage loop: the tableaux against the
+smooth advected density wave.  The split form: the identity of
+Tests/IMEX_O1_01.cpp (SplitKEEnergyMatch) with a real solve of the
+operator with its velocity term.  The stage loop: the tableaux against the
 reference's Tests/IMEXButcherTableaux01.cpp values and their order
 conditions, SP111 through the loop equal to PeekStage, second order in
 time of SSP222 with MUSCL fluxes without the local step (first order with
@@ -151,23 +156,36 @@ class PeekStage:
     solve the flux correction; for SP111 also the conservative update.
     advection: "rusanov" (advance_rusanov_adv) or "muscl_rusanov"."""
 
-    def __init__(self, gamma=1.4, eps=1.0, advection="rusanov"):
+    def __init__(self, gamma=1.4, eps=1.0, advection="rusanov", ke_method="ex"):
+        if ke_method not in ("ex", "split"):
+            raise ValueError("ke_method: ex or split")
         self.gamma = gamma
         self.eps = eps
         self.muscl = advection == "muscl_rusanov"
+        self.split = ke_method == "split"
 
     def assemble(self, U, p, dt, dx, dy=None, U_exp=None, local=True):
         """compute_flux_imex_o1 up to the solve (IMEX_O1.cpp:64-125): the
         advective fluxes of U_exp (default U) update U (the implicit stage
         state); local: the MUSCL local half step (SP111 only).  Returns
-        Uex, enth, rhs, a, bX (ny, nx+1), bY (ny+1, nx), alpha, beta and the
+        Uex, enth, rhs, a, bX (ny, nx+1), bY (ny+1, nx), alpha, beta, the
+        velocity term gamma_c, cX, cY (split: -eps dt / 2 and the velocity of
+        U_exp, compute_velocity and setup_pressure_op; ex: 0, None) and the
         advective fluxes Fx, Fy."""
         dy = dx if dy is None else dy
         g, eps = self.gamma, self.eps
         Uexp = U if U_exp is None else U_exp
         Uex, Fx, Fy = advect(Uexp, U, dt, dx, dy, eps, self.muscl, local)
         enth = p * g / ((g - 1) * Uex[0])
-        ke = kinetic_energy(Uex, eps)
+        if self.split:
+            # compute_ke, split: 0.5 eps (m_in . m_ex) / rho_in, in = U_exp
+            ke = 0.5 * eps * (Uexp[1] * Uex[1] + Uexp[2] * Uex[2]) / Uexp[0]
+            cX, cY = Uexp[1] / Uexp[0], Uexp[2] / Uexp[0]
+            gamma_c = -0.5 * eps * dt
+        else:
+            ke = kinetic_energy(Uex, eps)
+            cX = cY = None
+            gamma_c = 0.0
         hdtdx = 0.5 * dt * (1.0 / dx)
         hdtdy = 0.5 * dt * (1.0 / dy)
         rhs = eps * (Uex[3] - ke - (hdtdx * (_sh(enth, 0, 1) * _sh(Uex[1], 0, 1)
@@ -186,7 +204,7 @@ class PeekStage:
         bY[ny] = hy[0]
         a = np.full_like(p, eps / (g - 1))
         return dict(Uex=Uex, enth=enth, rhs=rhs, a=a, bX=bX, bY=bY,
-                    alpha=1.0, beta=dt * dt, Fx=Fx, Fy=Fy)
+                    alpha=1.0, beta=dt * dt, gamma_c=gamma_c, cX=cX, cY=cY, Fx=Fx, Fy=Fy)
 
     def fluxes(self, st, p_new, dt, dx, dy=None):
         """update_momentum_flux / update_energy_flux (IMEX_O1.cpp:485-566):
@@ -253,6 +271,13 @@ def pressure_ek(U_new, U_old, adt, dx, dy, gamma, eps, muscl):
     return (gamma - 1) * (U_new[3] - 0.5 * eps * (ex[1] * ex[1] + ex[2] * ex[2]) / U_new[0])
 
 
+def pressure_sk(U_new, U_old, gamma, eps):
+    """compute_pressure, SK branch (IMEX_O1.cpp:326-344):
+    p = (gamma - 1)(E_new - 0.5 eps (m_old . m_new) / rho_old)."""
+    return (gamma - 1) * (U_new[3] - 0.5 * eps * (U_old[1] * U_new[1] + U_old[2] * U_new[2])
+                          / U_old[0])
+
+
 class ButcherTableau:
     """IMEXButcherTableau (ButcherTableau.cpp:5-101)."""
     NAMES = ("SP111", "SSP222", "SASSP322", "SASSP332", "SSP433")
@@ -301,21 +326,24 @@ class ButcherTableau:
 
 
 def advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0, advection="rusanov",
-                    record=None):
+                    record=None, ke_method="ex"):
     """advance_imex_rk (IMEX_RK.cpp:22-96) on a 2D periodic level.  solve(sy,
     p0) returns the stage's pressure (sy: the PeekStage.assemble system,
     p0: the initial guess).  Returns (U_new, p) with p the last stage's
-    pressure.  record, a list, receives each stage's system, its states, the
+    pressure.  ke_method: "ex" (PEEK) or "split" (PESK: the system carries
+    the velocity term gamma_c, cX, cY).  record, a list, receives each stage's system, its states, the
     initial and solved pressure and its corrected fluxes (Fx, Fy; the
     advective ones as Fx_adv, Fy_adv)."""
     tab = ButcherTableau(tableau) if isinstance(tableau, str) else tableau
-    st = PeekStage(gamma, eps, advection)
+    st = PeekStage(gamma, eps, advection, ke_method)
     Fxs, Fys, exps = [], [], []
     for s in range(tab.n_imp):
         Uimp, Uexp = stage_states(U, Fxs, Fys, tab.imp_row(s, dt)[:s], tab.exp_row(s, dt)[:s],
                                   dx, dy)
         adt = tab.A_imp[s][s] * dt
-        if s > 0:
+        if s > 0 and st.split:
+            p = pressure_sk(Uexp, exps[-1], gamma, eps)
+        elif s > 0:
             p = pressure_ek(Uexp, exps[-1], adt, dx, dy, gamma, eps, st.muscl)
         sy = st.assemble(Uimp, p, adt, dx, dy, U_exp=Uexp, local=tab.name == "SP111")
         p_in = p

@@ -156,19 +156,19 @@ def test_peek_assemble_rejects_aliased_outputs():
         dp.free()
 
 
-def _rk_parity(U, p, dt, dx, dy, gamma, eps, tableau, advection, tag):
+def _rk_parity(U, p, dt, dx, dy, gamma, eps, tableau, advection, tag, ke_method="ex"):
     """One IMEX RK step on the GPU (ImexRK, through the C ABI) and in the
     oracle (advance_imex_rk), with the stand-in solve p_new = pressure of
     the stage's explicit state on both sides (bx_imex_pressure is
     bit-exact with the oracle's pressure): every stage's system and the
     step's state and pressure must be identical."""
     ny, nx = p.shape
-    announce("imex rk %s %s %s %dx%d" % (tableau, advection, tag, nx, ny))
+    announce("imex rk %s %s %s %s %dx%d" % (tableau, advection, ke_method, tag, nx, ny))
     rec = []
     U_ref, p_ref = io.advance_imex_rk(U, p, dt, dx, dy, tableau,
                                       lambda sy, p0: io.pressure_from_state(sy["Uex"], gamma, eps),
-                                      gamma, eps, advection, record=rec)
-    rk = imex.ImexRK(nx, ny, dx, dy, gamma, eps, advection, tableau)
+                                      gamma, eps, advection, record=rec, ke_method=ke_method)
+    rk = imex.ImexRK(nx, ny, dx, dy, gamma, eps, advection, tableau, ke_method)
     dU, dp, dout, dpn = dev(U), dev(p), DeviceArray(U.shape), DeviceArray(p.shape)
     try:
         seen = []
@@ -176,7 +176,9 @@ def _rk_parity(U, p, dt, dx, dy, gamma, eps, tableau, advection, tag):
         def solve(sy):
             r = rec[sy["stage"]]
             assert sy["adt"] == r["adt"] and sy["beta"] == r["beta"]
-            for key in ("Uex", "enth", "rhs", "bX", "bY"):
+            assert sy["gamma_c"] == r["gamma_c"]
+            keys = ("Uex", "enth", "rhs", "bX", "bY") + (("cX", "cY") if ke_method == "split" else ())
+            for key in keys:
                 got = sy[key].download()
                 assert np.array_equal(got, r[key]), (tableau, sy["stage"], key,
                                                       np.argwhere(got != r[key])[:3])
@@ -208,6 +210,65 @@ def test_imex_rk_step_matches_oracle(tableau, advection):
         p = io.pressure_from_state(U, gamma, eps) * 1.01
         _rk_parity(U, p, 0.009, 1.0 / nx, 0.6 / ny, gamma, eps, tableau, advection,
                    "random")
+
+
+@pytest.mark.parametrize("advection", ["rusanov", "muscl_rusanov"])
+@pytest.mark.parametrize("tableau", ["SP111", "SSP222", "SASSP322", "SSP433"])
+def test_imex_rk_split_ke_step_matches_oracle(tableau, advection):
+    """PESK (ke_method split): the split kinetic energy, the velocity
+    coefficients of the operator and the SK pressure of later stages,
+    bit-exact stage by stage."""
+    gamma, eps = 1.4, 0.7
+    for ny, nx in ((20, 48), (1, 9)):
+        U = random_state(ny, nx, gamma, eps, seed=5 * nx + ny)
+        p = io.pressure_from_state(U, gamma, eps) * 0.99
+        _rk_parity(U, p, 0.009, 1.0 / nx, 0.6 / ny, gamma, eps, tableau, advection, "random",
+                   ke_method="split")
+
+
+def test_imex_rk_pesk_ssp222_gresho_with_multigrid_solves():
+    """PESK-MUSCL-SSP222 steps of the Gresho vortex at 128^2, every stage's
+    non-symmetric operator (with the velocity term) solved by GPU
+    multigrid to 1e-10: the oracle, given the same solved pressures,
+    reaches the identical state; totals are conserved."""
+    n = 128
+    announce("imex rk pesk ssp222 gresho 128")
+    U0, dx = imex.gresho_state(n)
+    rk = imex.ImexRK(n, n, dx, advection="muscl_rusanov", tableau="SSP222", ke_method="split")
+    solver = imex.HelmholtzSolver(n, n, dx)
+    dU, dUn, dp = dev(U0), DeviceArray(U0.shape), DeviceArray((n, n))
+    rk.stage.pressure(dU, dp)
+    U, p = U0, dp.download()
+    try:
+        for k in range(2):
+            dt = rk.stage.advective_dt(dU)
+            solved = []
+
+            def solve(sy):
+                solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"],
+                                  gamma=sy["gamma_c"], cX=sy["cX"], cY=sy["cY"])
+                x = imex.dev(sy["p_prev"].download())
+                st = solver.solve(imex.dev(sy["rhs"].download()), x, "mg", rtol=1e-10, maxiter=100)
+                assert st["converged"] and st["iters"] < 100
+                solved.append(x.download())
+                return x
+
+            rk.advance(dU, dp, dt, dUn, solve)
+            it = iter(solved)
+            U_ref, p_ref = io.advance_imex_rk(U, p, dt, dx, dx, "SSP222", lambda sy, p0: next(it), 1.4,
+                                              1.0, "muscl_rusanov", ke_method="split")
+            got = dUn.download()
+            assert np.array_equal(got, U_ref), (k, np.argwhere(got != U_ref)[:3])
+            assert np.array_equal(dp.download(), p_ref)
+            scale = np.abs(U).sum(axis=(1, 2))
+            assert np.all(np.abs(got.sum(axis=(1, 2)) - U.sum(axis=(1, 2))) <= 1e-12 * scale)
+            U, p = U_ref, p_ref
+            dU, dUn = dUn, dU
+    finally:
+        solver.close()
+        rk.free()
+        for d in (dU, dUn, dp):
+            d.free()
 
 
 def test_imex_rk_ssp222_gresho_with_multigrid_solves():
@@ -269,10 +330,22 @@ def test_imex_rk_rejects_bad_arguments():
         assert rc != 0    # a flux array is also an output
         rc = L.bx_imex_rk_update(dU.ptr, one, one, 0, w, n, n, 0.1, 0.1, 0.01, a2.ptr, None)
         assert rc != 0    # no stage
-        rc = L.bx_imex_rk_pressure(dU.ptr, a1.ptr, n, n, 0.1, 0.1, 0.01, 1.4, 1.0, 0, dU.ptr, None)
+        rc = L.bx_imex_rk_pressure(dU.ptr, a1.ptr, n, n, 0.1, 0.1, 0.01, 1.4, 1.0, 0, 0, dU.ptr, None)
         assert rc != 0    # p aliases the state
+        rc = L.bx_imex_rk_pressure(dU.ptr, a1.ptr, n, n, 0.1, 0.1, 0.01, 1.4, 1.0, 0, 7, a2.ptr, None)
+        assert rc != 0    # unknown ke_method
+        sc = [DeviceArray((n, n)) for _ in range(6)]
+        try:
+            rc = L.bx_imex_stage_assemble(dU.ptr, dU.ptr, a1.ptr, n, n, 0.1, 0.1, 0.01, 1.4, 1.0, 0, 0,
+                                          1, a2.ptr, *[d.ptr for d in sc[:5]], None, None, None)
+            assert rc != 0    # split without velocity outputs
+        finally:
+            for d in sc:
+                d.free()
     finally:
         for d in (dU, a1, a2):
             d.free()
     with pytest.raises(ValueError):
         imex.ImexRK(n, n, 0.1, tableau="RK4")
+    with pytest.raises(ValueError):
+        imex.ImexRK(n, n, 0.1, ke_method="conservative")

@@ -31,9 +31,9 @@ def state_from_primv(rho, u, v, p, gamma, eps):
 
 def direct_solve(dx, dy):
     """A solve callback for advance_imex_rk: sparse LU of the periodic
-    5-point operator alpha a p - beta div(b grad p), assembled here and
-    checked against the oracle's operator (helm_oracle.apply) on a random
-    vector before use."""
+    5-point operator alpha a p - beta div(b grad p) + gamma_c c . grad p
+    (centred differences), assembled here and checked against the oracle's
+    operator (helm_oracle.apply) on a random vector before use."""
     def solve(sy, p0):
         a, bX, bY, alpha, beta = sy["a"], sy["bX"], sy["bY"], sy["alpha"], sy["beta"]
         ny, nx = a.shape
@@ -42,10 +42,14 @@ def direct_solve(dx, dy):
         bw, be = bX[:, :nx], bX[:, 1:]
         bs, bn = bY[:ny], bY[1:]
         diag = alpha * a + beta * ((bw + be) / (dx * dx) + (bs + bn) / (dy * dy))
-        for nb, w in ((np.roll(idx, 1, 1), -beta * bw / (dx * dx)),
-                      (np.roll(idx, -1, 1), -beta * be / (dx * dx)),
-                      (np.roll(idx, 1, 0), -beta * bs / (dy * dy)),
-                      (np.roll(idx, -1, 0), -beta * bn / (dy * dy))):
+        gc = sy.get("gamma_c", 0.0)
+        cX = sy.get("cX") if gc != 0.0 else None
+        cx = np.zeros((ny, nx)) if cX is None else gc * cX * (0.5 / dx)
+        cy = np.zeros((ny, nx)) if cX is None else gc * sy["cY"] * (0.5 / dy)
+        for nb, w in ((np.roll(idx, 1, 1), -beta * bw / (dx * dx) - cx),
+                      (np.roll(idx, -1, 1), -beta * be / (dx * dx) + cx),
+                      (np.roll(idx, 1, 0), -beta * bs / (dy * dy) - cy),
+                      (np.roll(idx, -1, 0), -beta * bn / (dy * dy) + cy)):
             rows.append(idx.ravel())
             cols.append(nb.ravel())
             vals.append(w.ravel())
@@ -55,7 +59,10 @@ def direct_solve(dx, dy):
         A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                           shape=(nx * ny, nx * ny))
         x = np.random.default_rng(0).random((ny, nx))
-        ref = ho.apply(x, a, bX, bY, alpha, beta, dx, dy)
+        if cX is None:
+            ref = ho.apply(x, a, bX, bY, alpha, beta, dx, dy)
+        else:
+            ref = ho.apply(x, a, bX, bY, alpha, beta, dx, dy, gc, cX, sy["cY"])
         assert np.allclose((A @ x.ravel()).reshape(ny, nx), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
         return spla.spsolve(A.tocsc(), sy["rhs"].ravel()).reshape(ny, nx)
     return solve
@@ -277,3 +284,89 @@ def test_product_tableaux_equal_the_oracle_tableaux():
             assert a.imp_row(r, 0.013) == b.imp_row(r, 0.013)
             assert a.exp_row(r, 0.013) == b.exp_row(r, 0.013)
     assert imex.ButcherTableau.NAMES == io.ButcherTableau.NAMES
+
+
+def test_split_ke_energy_matches_pressure():
+    """Tests/IMEX_O1_01.cpp:56-117 (SplitKEEnergyMatch, eps 0.5, gamma 1.6,
+    dt 0.01, Rusanov): with the split kinetic energy and the operator's
+    velocity term, the new energy equals
+    p/(gamma-1) + 0.5 eps (m_in . m_new) / rho_in -- a wrong sign of
+    gamma_c or a wrong split product breaks it."""
+    n = 32
+    gamma, eps, dt = 1.6, 0.5, 0.01
+    rng = np.random.default_rng(4)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) + 0.05 * rng.random((n, n))
+    u = 0.4 * np.sin(2 * np.pi * Y)
+    v = -0.3 * np.cos(2 * np.pi * X)
+    p = 1.0 + 0.2 * np.cos(2 * np.pi * (X + Y))
+    U = state_from_primv(rho, u, v, p, gamma, eps)
+    dx = 1.0 / n
+    for adv in ("rusanov", "muscl_rusanov"):
+        U1, pn = io.advance_imex_rk(U, p, dt, dx, dx, "SP111", direct_solve(dx, dx), gamma, eps, adv,
+                                    ke_method="split")
+        expected = pn / (gamma - 1) + 0.5 * eps / U[0] * (U[1] * U1[1] + U[2] * U1[2])
+        assert np.max(np.abs(U1[3] - expected) / np.abs(expected)) <= 1e-12, adv
+    # the identity fails with the explicit-state kinetic energy (PEEK)
+    U1, pn = io.advance_imex_rk(U, p, dt, dx, dx, "SP111", direct_solve(dx, dx), gamma, eps,
+                                "rusanov")
+    expected = pn / (gamma - 1) + 0.5 * eps / U[0] * (U[1] * U1[1] + U[2] * U1[2])
+    assert np.max(np.abs(U1[3] - expected) / np.abs(expected)) > 1e-8
+
+
+def test_split_system_and_sk_pressure_per_cell():
+    """The split system's kinetic energy, velocity term and the SK pressure
+    of SSP222's second stage against per-cell plain-Python evaluations."""
+    gamma, eps = 1.4, 0.8
+    rng = np.random.default_rng(9)
+    ny, nx = 4, 6
+    U = state_from_primv(1 + 0.2 * rng.random((ny, nx)), rng.uniform(-1, 1, (ny, nx)),
+                         rng.uniform(-1, 1, (ny, nx)), 2 + rng.random((ny, nx)), gamma, eps)
+    p = io.pressure_from_state(U, gamma, eps)
+    dx, dy, dt = 1.0 / nx, 1.0 / ny, 0.02
+    rec = []
+    io.advance_imex_rk(U, p, dt, dx, dy, "SSP222", lambda sy, p0: p0 * 0.999, gamma, eps,
+                       "muscl_rusanov", record=rec, ke_method="split")
+    al = 1 - 1 / math.sqrt(2)
+    r0, r1 = rec
+    assert r0["gamma_c"] == -0.5 * eps * (al * dt)
+    hx = 0.5 * (al * dt) * (1.0 / dx)
+    hy = 0.5 * (al * dt) * (1.0 / dy)
+    ex, h = r0["Uex"], r0["enth"]
+    for j in range(ny):
+        for i in range(nx):
+            assert r0["cX"][j, i] == U[1, j, i] / U[0, j, i]
+            assert r0["cY"][j, i] == U[2, j, i] / U[0, j, i]
+            ke = 0.5 * eps * (U[1, j, i] * ex[1, j, i] + U[2, j, i] * ex[2, j, i]) / U[0, j, i]
+            ip, im, jp, jm = (i + 1) % nx, (i - 1) % nx, (j + 1) % ny, (j - 1) % ny
+            rhs = eps * (ex[3, j, i] - ke
+                         - (hx * (h[j, ip] * ex[1, j, ip] - h[j, im] * ex[1, j, im])
+                            + hy * (h[jp, i] * ex[2, jp, i] - h[jm, i] * ex[2, jm, i])))
+            assert r0["rhs"][j, i] == rhs
+            Un, Uo = r1["U_exp"], U
+            m2 = Uo[1, j, i] * Un[1, j, i] + Uo[2, j, i] * Un[2, j, i]
+            assert r1["p_in"][j, i] == (gamma - 1) * (Un[3, j, i] - 0.5 * eps * m2 / Uo[0, j, i])
+
+
+@pytest.mark.parametrize("tableau", ["SP111", "SSP222", "SSP433"])
+def test_split_conserves_and_keeps_uniform_flow(tableau):
+    """PESK steps: uniform flow unchanged, and the Gresho vortex's totals
+    conserved with a real solve of the operator with its velocity term."""
+    gamma, eps = 1.4, 1.0
+    n = 16
+    U = state_from_primv(np.full((n, n), 1.2), np.full((n, n), 0.3), np.full((n, n), -0.2),
+                         np.full((n, n), 2.0), gamma, eps)
+    dx = 1.0 / n
+    U1, p1 = io.advance_imex_rk(U, io.pressure_from_state(U, gamma, eps), 0.02, dx, dx, tableau,
+                                direct_solve(dx, dx), gamma, eps, "muscl_rusanov", ke_method="split")
+    assert np.allclose(U1, U, rtol=1e-12, atol=0)
+    assert np.allclose(p1, 2.0, rtol=1e-12, atol=0)
+    Ug, dxg = imex.gresho_state(32)
+    pg = io.pressure_from_state(Ug, gamma, eps)
+    dtg = io.advective_dt(Ug, dxg)
+    Un, pn = io.advance_imex_rk(Ug, pg, dtg, dxg, dxg, tableau, direct_solve(dxg, dxg), gamma, eps,
+                                "muscl_rusanov", ke_method="split")
+    scale = np.abs(Ug).sum(axis=(1, 2))
+    assert np.all(np.abs(Un.sum(axis=(1, 2)) - Ug.sum(axis=(1, 2))) <= 1e-13 * scale)
+    assert np.max(np.abs(pn - pg)) < 1e-4 * pg.max()

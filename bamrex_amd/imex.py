@@ -532,13 +532,14 @@ class GreshoWorkload:
     multigrid solution, so every method solves the same sequence)."""
 
     def __init__(self, n=1024, mach=0.01, gamma=1.4, eps=1.0, cfl=0.45, advection="rusanov",
-                 tableau="SP111"):
+                 tableau="SP111", ke_method="ex"):
         U, self.dx = gresho_state(n, mach, gamma, eps)
         self.n = n
         self.U = dev(U)
         self.U_next = DeviceArray(U.shape)
         self.p = DeviceArray((n, n))
-        self.rk = ImexRK(n, n, self.dx, gamma=gamma, eps=eps, advection=advection, tableau=tableau)
+        self.rk = ImexRK(n, n, self.dx, gamma=gamma, eps=eps, advection=advection, tableau=tableau,
+                         ke_method=ke_method)
         self.stage = self.rk.stage
         self.stage.pressure(self.U, self.p)
         self.cfl = cfl

@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--imex-advection", default="muscl_rusanov", choices=["rusanov", "muscl_rusanov"])
     ap.add_argument("--imex-tableau", default="SSP222",
                     choices=["SP111", "SSP222", "SASSP322", "SASSP332", "SSP433"])
+    ap.add_argument("--imex-ke", default="ex", choices=["ex", "split"],
+                    help="imex.ke_method: ex (PEEK) or split (PESK)")
     ap.add_argument("--imex-harvest-steps", type=int, default=8, help="pressure systems harvested")
     ap.add_argument("--imex-steps", type=int, default=4, help="time steps (one solve per stage)")
     ap.add_argument("--imex-train-steps", type=int, default=640)
@@ -404,7 +406,12 @@ def imex_leg(args):
     from bamrex_amd import imex
     n = args.imex_n
     log("imex: Gresho %d^2, harvesting %d steps" % (n, args.imex_harvest_steps))
-    wl = imex.GreshoWorkload(n, advection=args.imex_advection, tableau=args.imex_tableau)
+    wl = imex.GreshoWorkload(n, advection=args.imex_advection, tableau=args.imex_tableau,
+                             ke_method=args.imex_ke)
+
+    def set_coeffs(sy):
+        solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"], gamma=sy["gamma_c"],
+                          cX=sy["cX"], cY=sy["cY"])
     nst = wl.rk.tab.n_imp
     solver = imex.HelmholtzSolver(n, n, wl.dx)
     nf_max = 8 * args.imex_harvest_steps
@@ -413,14 +420,14 @@ def imex_leg(args):
     # one pressure system per stage: harvest_steps systems
     for k in range(args.imex_harvest_steps):
         sy = wl.next_system()
-        solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+        set_coeffs(sy)
         db, dx_ = imex.dev(sy["rhs"]), imex.dev(sy["p_prev"])
         got, st = imex.harvest_residuals(solver, db, dx_, R, nf, nf_max - nf)
         nf += got
         wl.advance(sy, dx_)
     while wl.in_step:        # finish a step left open by the harvest
         sy = wl.next_system()
-        solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+        set_coeffs(sy)
         db, dx_ = imex.dev(sy["rhs"]), imex.dev(sy["p_prev"])
         solver.solve(db, dx_, "mg", rtol=1e-10, maxiter=200)
         wl.advance(sy, dx_)
@@ -460,7 +467,7 @@ def imex_leg(args):
             sy = wl.next_system(dt)
             imex.lib().bx_stream_sync(None)
             t2 = time.perf_counter()
-            solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+            set_coeffs(sy)
             t3 = time.perf_counter()
             acc["assemble"] += t2 - t1
             acc["set_coeffs"] += t3 - t2
@@ -505,8 +512,8 @@ def imex_leg(args):
                 for m in methods))
         for key, v in acc.items():
             stage_ms[key].append(v * 1e3)
-    out = {"workload": "gresho-peek-%s-%s-%dx%d-M0.01-periodic" % (
-        args.imex_tableau.lower(), args.imex_advection, n, n),
+    out = {"workload": "gresho-%s-%s-%s-%dx%d-M0.01-periodic" % (
+        "peek" if args.imex_ke == "ex" else "pesk", args.imex_tableau.lower(), args.imex_advection, n, n),
            "tableau": args.imex_tableau, "solves_per_step": nst,
            "rtol": 1e-10,
            "harvest_fields": nf, "train_steps": args.imex_train_steps, "train_batch": args.imex_batch,

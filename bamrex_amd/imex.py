@@ -270,12 +270,18 @@ class ImexRK:
 
     def begin(self, U, p, dt):
         self.U, self.p, self.dt, self.s = U, p, dt, 0
-        self._exp = []
+        self._exp = [None] * self.tab.n_imp   # each stage's explicit state
+        self._built = False
 
     def system(self):
         """The current stage's pressure system (device arrays of the stage
         and the operator scalars alpha = 1, beta = adt^2)."""
-        s, dt, tab = self.s, self.dt, self.tab
+        s, tab = self.s, self.tab
+        if s is None or s >= tab.n_imp:
+            raise RuntimeError("ImexRK.system: no step in progress (call begin)")
+        dt = self.dt
+        if self._built:
+            raise RuntimeError("ImexRK.system: stage %d already assembled (call complete)" % s)
         adt = tab.A_imp[s][s] * dt
         nx, ny, dx, dy = self.nx, self.ny, self.dx, self.dy
         if s == 0:
@@ -288,8 +294,8 @@ class ImexRK:
             _chk(lib().bx_imex_rk_pressure(Ue.ptr, self._exp[s - 1].ptr, nx, ny, dx, dy, adt,
                                            self.gamma, self.eps, self.adv, self.ke, self.p.ptr, None),
                  "bx_imex_rk_pressure")
-        self._exp.append(Ue)
-        self._Ue, self._adt = Ue, adt
+        self._exp[s] = Ue
+        self._Ue, self._adt, self._built = Ue, adt, True
         st = self.stage
         _chk(lib().bx_imex_stage_assemble(Ui.ptr, Ue.ptr, self.p.ptr, nx, ny, dx, dy, adt, self.gamma,
                                           self.eps, self.adv, int(tab.name == "SP111"), self.ke,
@@ -306,6 +312,8 @@ class ImexRK:
         stage the step's update into U_out (not the step's U).  Returns
         whether the step is done."""
         s, st = self.s, self.stage
+        if not getattr(self, "_built", False):
+            raise RuntimeError("ImexRK.complete: the stage's system was not assembled")
         nx, ny, dx, dy = self.nx, self.ny, self.dx, self.dy
         last = s + 1 == self.tab.n_imp
         if self.fused:
@@ -324,6 +332,7 @@ class ImexRK:
         if p_new.ptr.value != self.p.ptr.value:
             _lib.check(lib().bx_memcpy_dtod(self.p.ptr, p_new.ptr, self.p.nbytes), "dtod")
         self.s = s + 1
+        self._built = False
         return last
 
     def advance(self, U, p, dt, U_out, solve):

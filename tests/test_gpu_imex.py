@@ -349,3 +349,12 @@ def test_imex_rk_rejects_bad_arguments():
         imex.ImexRK(n, n, 0.1, tableau="RK4")
     with pytest.raises(ValueError):
         imex.ImexRK(n, n, 0.1, ke_method="conservative")
+    # stages out of order are refused on the host, before any launch
+    rk = imex.ImexRK(n, n, 0.1, tableau="SSP222")
+    try:
+        with pytest.raises(RuntimeError):
+            rk.system()
+        with pytest.raises(RuntimeError):
+            rk.complete(rk.stage.rhs, rk.stage.Uex)
+    finally:
+        rk.free()

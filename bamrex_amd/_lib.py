@@ -169,6 +169,10 @@ _SIGS = {
     "bx_imex_rk_update": (ctypes.c_int, [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int, DP,
                                          ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, P, P]),
+    "bx_imex_shock_indicator": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, P, P, ctypes.POINTER(ctypes.c_double), P]),
+    "bx_imex_stab_blend": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_int, P, P, P]),
     "bx_cnn_create": (ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(P)]),
     "bx_cnn_destroy": (None, [P]),
     "bx_cnn_set_params": (ctypes.c_int, [P, P, ctypes.c_int]),

@@ -562,6 +562,122 @@ __global__ __launch_bounds__(256) void k_peek_speed(const double *__restrict__ U
 
 __global__ void k_peek_zero_u64(unsigned long long *p) { *p = 0ULL; }
 
+// primitive x/y velocity and sound speed of cell c (primv_from_consv and
+// sound_speed, Source/System/Euler/Euler.H:167-198, 245-254)
+__device__ __forceinline__ void shock_prim(const double *__restrict__ U, long long c, long long n,
+                                           double adia, double eps, double &u, double &v,
+                                           double &cs)
+{
+    const double r = U[c];
+    u              = U[c + n] / r;
+    v              = U[c + 2 * n] / r;
+    const double usq = u * u + v * v;
+    const double p   = (adia - 1) * (U[c + 3 * n] - 0.5 * eps * r * usq);
+    cs               = sqrt(adia * p / (r * eps));
+}
+
+// compute_shock_indicator, first pass (IMEX_RK.cpp:277-311): per cell,
+// c_min over the cell and its four face neighbours, the centred velocity
+// divergence, premax = fmin(1, fmax(0, -(div_u + k1 c_min)/(k1 c_min))).
+__global__ __launch_bounds__(256) void k_shock_premax(const double *__restrict__ U,
+                                                      double *__restrict__ premax, PeekGeom g,
+                                                      double adia, double eps, double idx,
+                                                      double idy, double k1)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    const long long row = (long long)j * g.nx;
+    const long long c   = row + i;
+    // neighbours in the reference's order: i+1, i-1, j+1, j-1
+    const long long nb[4] = { row + wrap_hi(i, g.nx), row + wrap_lo(i, g.nx),
+                              (long long)wrap_hi(j, g.ny) * g.nx + i,
+                              (long long)wrap_lo(j, g.ny) * g.nx + i };
+    double          u, v, c_min;
+    shock_prim(U, c, g.n, adia, eps, u, v, c_min);
+    double un[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        double uk, vk, ck;
+        shock_prim(U, nb[k], g.n, adia, eps, uk, vk, ck);
+        c_min = (ck < c_min) ? ck : c_min; // std::min(c_min, ck)
+        un[k] = k < 2 ? uk : vk;
+    }
+    const double div_u = 0.5 * (idx * (un[0] - un[1]) + idy * (un[2] - un[3]));
+    premax[c]          = fmin(1.0, fmax(0.0, -(div_u + k1 * c_min) / (k1 * c_min)));
+}
+
+// second pass (IMEX_RK.cpp:313-339): the maximum of premax over the 3 x 3
+// neighbourhood, and the level maximum (values in [0, 1]: positive ones
+// order like their bit patterns; zeros of either sign contribute nothing).
+__global__ __launch_bounds__(256) void k_shock_max(const double *__restrict__ premax,
+                                                   double *__restrict__ ind, PeekGeom g,
+                                                   unsigned long long *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    unsigned long long m = 0ULL;
+    if (i < g.nx && j < g.ny)
+    {
+        const int xs[3] = { wrap_lo(i, g.nx), i, wrap_hi(i, g.nx) };
+        const int ys[3] = { wrap_lo(j, g.ny), j, wrap_hi(j, g.ny) };
+        double    mx    = premax[(long long)j * g.nx + i];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+            {
+                const double q = premax[(long long)ys[b] * g.nx + xs[a]];
+                mx             = (mx < q) ? q : mx; // std::max(mx, q)
+            }
+        ind[(long long)j * g.nx + i] = mx;
+        if (mx > 0.0)
+            m = (unsigned long long)__double_as_longlong(mx);
+    }
+    __shared__ unsigned long long red[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const unsigned long long o = __shfl_xor(m, off);
+        m                          = o > m ? o : m;
+    }
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((t & 63) == 0)
+        red[t >> 6] = m;
+    __syncthreads();
+    if (t == 0)
+    {
+        for (int k = 1; k < (int)((blockDim.x * blockDim.y) >> 6); ++k)
+            m = red[k] > m ? red[k] : m;
+        if (m != 0ULL)
+            atomicMax(out, m);
+    }
+}
+
+// the convex combination of advance_imex_rk_stab (IMEX_RK.cpp:157-193):
+// where ind > 0, U = ind U_o1 + (1 - ind) U and p likewise
+__global__ __launch_bounds__(256) void k_stab_blend(const double *__restrict__ ind,
+                                                   const double *__restrict__ Uo1,
+                                                   const double *__restrict__ po1,
+                                                   double *__restrict__ U, double *__restrict__ p,
+                                                   PeekGeom g)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    const long long c = (long long)j * g.nx + i;
+    const double    w = ind[c];
+    if (!(w > 0))
+        return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        U[c + k * g.n] = w * Uo1[c + k * g.n] + (1 - w) * U[c + k * g.n];
+    p[c] = w * po1[c] + (1 - w) * p[c];
+}
+
 } // namespace bx
 
 using namespace bx;
@@ -583,6 +699,28 @@ static dim3 peek_grid(int nx, int ny) { return dim3((nx + 63) / 64, (ny + 3) / 4
                                                 + hipGetErrorString(e_));        \
     } while (0)
 
+// One 8-byte accumulator per device and purpose, kept until the process
+// ends (calls of one purpose on one device are stream-ordered by the caller).
+enum
+{
+    ACC_SPEED = 0,
+    ACC_SHOCK = 1
+};
+static int peek_acc(const char *who, int slot, unsigned long long **out)
+{
+    static std::mutex          mu;
+    static unsigned long long *acc[64][2] = {};
+    int                        dev        = 0;
+    PEEK_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64)
+        return bx_helm_fail(BX_ERR_UNSUPPORTED, std::string(who) + ": device id");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!acc[dev][slot])
+        PEEK_CHECK(hipMalloc((void **)&acc[dev][slot], sizeof(unsigned long long)));
+    *out = acc[dev][slot];
+    return BX_OK;
+}
+
 extern "C" int bx_imex_pressure(const double *U, int nx, int ny, double gamma, double eps,
                                 double *p, void *stream)
 {
@@ -602,19 +740,9 @@ extern "C" int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, 
         return bx_helm_fail(BX_ERR_INVALID, "bx_imex_advective_dt: bad argument");
     hipStream_t         s = (hipStream_t)stream;
     unsigned long long *d = nullptr;
-    {
-        // one 8-byte accumulator per device, kept until the process ends
-        static std::mutex          mu;
-        static unsigned long long *acc[64] = {};
-        int                        dev     = 0;
-        PEEK_CHECK(hipGetDevice(&dev));
-        if (dev < 0 || dev >= 64)
-            return bx_helm_fail(BX_ERR_UNSUPPORTED, "bx_imex_advective_dt: device id");
-        std::lock_guard<std::mutex> lk(mu);
-        if (!acc[dev])
-            PEEK_CHECK(hipMalloc((void **)&acc[dev], sizeof(unsigned long long)));
-        d = acc[dev];
-    }
+    int                 rc;
+    if ((rc = peek_acc("bx_imex_advective_dt", ACC_SPEED, &d)) != BX_OK)
+        return rc;
     hipLaunchKernelGGL(k_peek_zero_u64, dim3(1), dim3(1), 0, s, d);
     const long long n    = (long long)nx * ny;
     const int       grid = (int)std::min<long long>((n + 255) / 256, 1024);
@@ -872,6 +1000,49 @@ extern "C" int bx_imex_rk_update(const double *U, const double *const *Fx,
     PeekGeom g{ nx, ny, (long long)nx * ny };
     hipLaunchKernelGGL(k_rk_update, peek_grid(nx, ny), dim3(64, 4), 0, (hipStream_t)stream, U, f,
                        nstage, w, U_new, g, dt * (1.0 / dx), dt * (1.0 / dy));
+    PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_imex_shock_indicator(const double *U, int nx, int ny, double dx, double dy,
+                                       double gamma, double eps, double k1, double *work,
+                                       double *ind, double *ind_max, void *stream)
+{
+    if (!U || !work || !ind || !ind_max || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0)
+        || !(k1 > 0) || work == ind || (const double *)work == U || (const double *)ind == U)
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_shock_indicator: bad argument");
+    hipStream_t         s = (hipStream_t)stream;
+    unsigned long long *d = nullptr;
+    int                 rc;
+    if ((rc = peek_acc("bx_imex_shock_indicator", ACC_SHOCK, &d)) != BX_OK)
+        return rc;
+    PeekGeom g{ nx, ny, (long long)nx * ny };
+    hipLaunchKernelGGL(k_peek_zero_u64, dim3(1), dim3(1), 0, s, d);
+    // idx = geom.InvCellSizeArray() (IMEX_RK.cpp:272)
+    hipLaunchKernelGGL(k_shock_premax, peek_grid(nx, ny), dim3(64, 4), 0, s, U, work, g, gamma,
+                       eps, 1.0 / dx, 1.0 / dy, k1);
+    PEEK_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_shock_max, peek_grid(nx, ny), dim3(64, 4), 0, s, work, ind, g, d);
+    PEEK_CHECK(hipGetLastError());
+    unsigned long long bits = 0;
+    PEEK_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, s));
+    PEEK_CHECK(hipStreamSynchronize(s));
+    double m;
+    memcpy(&m, &bits, sizeof(m));
+    *ind_max = m;
+    return BX_OK;
+}
+
+extern "C" int bx_imex_stab_blend(const double *ind, const double *U_o1, const double *p_o1,
+                                  int nx, int ny, double *U, double *p, void *stream)
+{
+    if (!ind || !U_o1 || !p_o1 || !U || !p || !peek_sizes_ok(nx, ny) || U == p
+        || (const double *)U == U_o1 || (const double *)p == p_o1 || (const double *)U == ind
+        || (const double *)p == ind)
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_stab_blend: bad argument");
+    PeekGeom g{ nx, ny, (long long)nx * ny };
+    hipLaunchKernelGGL(k_stab_blend, peek_grid(nx, ny), dim3(64, 4), 0, (hipStream_t)stream, ind,
+                       U_o1, p_o1, U, p, g);
     PEEK_CHECK(hipGetLastError());
     return BX_OK;
 }

@@ -351,6 +351,62 @@ class ImexRK:
         self.stage.free()
 
 
+class ImexRKStab:
+    """advance_imex_rk_stab (Source/IMEX/IMEX_RK.cpp:98-194; imex.stabilize):
+    the tableau's step (an ImexRK); for a tableau of order above 1, where
+    the shock indicator of the new state is positive (imex.k1), the convex
+    combination with the first-order step (SP111, Rusanov fluxes, from the
+    incoming pressure) of the state and the pressure."""
+
+    def __init__(self, nx, ny, dx, dy=None, gamma=1.4, eps=1.0, advection="rusanov",
+                 tableau="SSP222", ke_method="ex", k1=1e-3):
+        self.hi = ImexRK(nx, ny, dx, dy, gamma, eps, advection, tableau, ke_method)
+        self.k1 = k1
+        self.order = self.hi.tab.order
+        self.lo = (ImexRK(nx, ny, dx, dy, gamma, eps, "rusanov", "SP111", ke_method)
+                   if self.order > 1 else None)
+        self.nx, self.ny = nx, ny
+        self.p_o1 = DeviceArray((ny, nx))
+        self.U_o1 = DeviceArray((4, ny, nx))
+        self.ind = DeviceArray((ny, nx))
+        self.work = DeviceArray((ny, nx))
+        self.ind_max = None
+        self.blended = False
+
+    def indicator(self, U):
+        """The shock indicator of U into self.ind; returns its maximum."""
+        m = ctypes.c_double()
+        _chk(lib().bx_imex_shock_indicator(U.ptr, self.nx, self.ny, self.hi.dx, self.hi.dy,
+                                           self.hi.gamma, self.hi.eps, self.k1, self.work.ptr,
+                                           self.ind.ptr, ctypes.byref(m), None),
+             "bx_imex_shock_indicator")
+        return m.value
+
+    def advance(self, U, p, dt, U_out, solve):
+        """One stabilised step: U_out and p (in place) as advance_imex_rk_stab
+        leaves them; solve(sy) as for ImexRK.advance."""
+        self.blended = False
+        self.ind_max = None
+        self.p_o1.copy_from(p)
+        self.hi.advance(U, p, dt, U_out, solve)
+        if self.order <= 1:
+            return
+        self.ind_max = self.indicator(U_out)
+        if self.ind_max <= 0:
+            return
+        self.lo.advance(U, self.p_o1, dt, self.U_o1, solve)
+        _chk(lib().bx_imex_stab_blend(self.ind.ptr, self.U_o1.ptr, self.p_o1.ptr, self.nx, self.ny,
+                                      U_out.ptr, p.ptr, None), "bx_imex_stab_blend")
+        self.blended = True
+
+    def free(self):
+        for d in (self.p_o1, self.U_o1, self.ind, self.work):
+            d.free()
+        self.hi.free()
+        if self.lo is not None:
+            self.lo.free()
+
+
 class HelmholtzSolver:
     """Owns a bx_helm solver for one nx x ny periodic level."""
 

@@ -505,6 +505,22 @@ int bx_imex_rk_update(const double *U, const double *const *Fx,
                       int ny, double dx, double dy, double dt, double *U_new,
                       void *stream);
 
+/* High-Mach stabilisation of a step of order above 1
+ * (advance_imex_rk_stab, Source/IMEX/IMEX_RK.cpp:98-194; imex.stabilize,
+ * imex.k1).  The shock indicator of the step's new state U
+ * (compute_shock_indicator, :243-345): per cell c_min over the cell and
+ * its face neighbours and the centred velocity divergence,
+ * fmin(1, fmax(0, -(div u + k1 c_min)/(k1 c_min))), then its maximum over
+ * the 3 x 3 neighbourhood into ind; work: nx*ny doubles of scratch.
+ * Synchronous; writes the level maximum to the host double *ind_max. */
+int bx_imex_shock_indicator(const double *U, int nx, int ny, double dx, double dy,
+                            double gamma, double eps, double k1, double *work,
+                            double *ind, double *ind_max, void *stream);
+/* Where ind > 0: U = ind U_o1 + (1 - ind) U and p = ind p_o1 + (1 - ind) p,
+ * with the first-order step (SP111, Rusanov) U_o1, p_o1. */
+int bx_imex_stab_blend(const double *ind, const double *U_o1, const double *p_o1,
+                       int nx, int ny, double *U, double *p, void *stream);
+
 /* Learned smoother: bias-free 3x3 convolutions 1->16->16->16->1 with ReLU,
  * BF16 weights/activations, FP32 accumulation (MFMA for the 16->16 layers).
  * Correction e = omega (sigma / D) N(r / sigma), sigma = rms(r), D = diag(A).

@@ -30,6 +30,8 @@ IMEX Runge-Kutta tableaux (advance_imex_rk).
   compute_ke (ex, split), compute_velocity
                                          Source/IMEX/IMEX_O1.cpp:232-298
   IMEXButcherTableau                     Source/IMEX/ButcherTableau.cpp:5-101
+  advance_imex_rk_stab, compute_shock_indicator
+                                         Source/IMEX/IMEX_RK.cpp:98-194, 243-345
 
 Arrays are (ny, nx), x fastest; U has 4 components (rho, m_x, m_y, E).
 numpy evaluates each binary operation in IEEE double with no contraction,
@@ -358,3 +360,59 @@ def advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0, advect
                        Fx_adv=sy["Fx"], Fy_adv=sy["Fy"])
             record.append(rec)
     return conservative_update(U, Fxs, Fys, tab.b_imp, dt, dx, dy), p
+
+
+def shock_indicator(U, dx, dy, gamma, eps, k1):
+    """compute_shock_indicator (IMEX_RK.cpp:243-345) on a periodic level:
+    per cell, from the primitive states of the cell and its four face
+    neighbours (primv_from_consv, sound_speed: Euler.H:167-198, 245-254),
+    c_min = the smallest sound speed, div_u = 0.5 (idx (u_E - u_W) +
+    idy (v_N - v_S)), premax = fmin(1, fmax(0, -(div_u + k1 c_min) /
+    (k1 c_min))); the indicator is the maximum of premax over the 3 x 3
+    neighbourhood.  Returns (indicator, its maximum)."""
+    rho = U[0]
+    u = U[1] / U[0]
+    v = U[2] / U[0]
+    usq = u * u + v * v
+    p = (gamma - 1) * (U[3] - 0.5 * eps * U[0] * usq)
+    c = np.sqrt(gamma * p / (rho * eps))
+    c_min = c
+    for d, k in ((0, 1), (0, -1), (1, 1), (1, -1)):
+        cn = _sh(c, d, k)
+        c_min = np.where(cn < c_min, cn, c_min)
+    idx, idy = 1.0 / dx, 1.0 / dy
+    div_u = 0.5 * (idx * (_sh(u, 0, 1) - _sh(u, 0, -1)) + idy * (_sh(v, 1, 1) - _sh(v, 1, -1)))
+    premax = np.fmin(1.0, np.fmax(0.0, -(div_u + k1 * c_min) / (k1 * c_min)))
+    ind = premax.copy()
+    for dj in (-1, 0, 1):
+        for di in (-1, 0, 1):
+            nb = _sh(_sh(premax, 0, di), 1, dj)
+            ind = np.where(nb > ind, nb, ind)
+    return ind, float(ind.max())
+
+
+def advance_imex_rk_stab(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0,
+                         advection="rusanov", ke_method="ex", k1=1e-3, record=None):
+    """advance_imex_rk_stab (IMEX_RK.cpp:98-194): the tableau's step; for a
+    tableau of order above 1, where the shock indicator of the new state is
+    positive, the convex combination ind U_o1 + (1 - ind) U (and likewise
+    the pressure) with the first-order step U_o1 (SP111, Rusanov fluxes,
+    from the incoming pressure).  Returns (U_new, p)."""
+    tab = ButcherTableau(tableau) if isinstance(tableau, str) else tableau
+    U_hi, p_hi = advance_imex_rk(U, p, dt, dx, dy, tab, solve, gamma, eps, advection,
+                                 ke_method=ke_method)
+    if tab.order <= 1:
+        return U_hi, p_hi
+    ind, ind_max = shock_indicator(U_hi, dx, dy, gamma, eps, k1)
+    if record is not None:
+        record.update(ind=ind, ind_max=ind_max, U_hi=U_hi, p_hi=p_hi)
+    if ind_max <= 0:
+        return U_hi, p_hi
+    U_lo, p_lo = advance_imex_rk(U, p, dt, dx, dy, "SP111", solve, gamma, eps, "rusanov",
+                                 ke_method=ke_method)
+    if record is not None:
+        record.update(U_lo=U_lo, p_lo=p_lo)
+    on = ind > 0
+    U_new = np.where(on, ind * U_lo + (1 - ind) * U_hi, U_hi)
+    p_new = np.where(on, ind * p_lo + (1 - ind) * p_hi, p_hi)
+    return U_new, p_new

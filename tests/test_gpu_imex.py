@@ -358,3 +358,78 @@ def test_imex_rk_rejects_bad_arguments():
             rk.complete(rk.stage.rhs, rk.stage.Uex)
     finally:
         rk.free()
+
+
+def _compressive_state(ny, nx, gamma, eps, seed):
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    rho = 1.0 + 0.1 * rng.random((ny, nx))
+    u = -0.8 * np.sin(2 * np.pi * X) + 0.05 * rng.standard_normal((ny, nx))
+    v = -0.5 * np.sin(2 * np.pi * Y) + 0.05 * rng.standard_normal((ny, nx))
+    p = 1.0 + 0.1 * rng.random((ny, nx))
+    U = np.empty((4, ny, nx))
+    U[0], U[1], U[2] = rho, rho * u, rho * v
+    U[3] = p / (gamma - 1) + 0.5 * eps * rho * (u * u + v * v)
+    return U
+
+
+@pytest.mark.parametrize("shape", [(20, 48), (1, 9), (67, 130)])
+def test_shock_indicator_matches_oracle(shape):
+    """compute_shock_indicator on the GPU: the indicator field and its
+    maximum bit-exact against the oracle (dx != dy, eps != 1)."""
+    ny, nx = shape
+    announce("shock indicator %dx%d" % (nx, ny))
+    gamma, eps, k1 = 1.4, 0.7, 0.05
+    U = _compressive_state(ny, nx, gamma, eps, nx + ny)
+    dx, dy = 1.0 / nx, 0.8 / ny
+    ref, ref_max = io.shock_indicator(U, dx, dy, gamma, eps, k1)
+    st = imex.ImexRKStab(nx, ny, dx, dy, gamma, eps, k1=k1)
+    dU = dev(U)
+    try:
+        m = st.indicator(dU)
+        assert np.array_equal(st.ind.download(), ref)
+        assert m == ref_max
+    finally:
+        st.free()
+        dU.free()
+
+
+@pytest.mark.parametrize("ke_method", ["ex", "split"])
+@pytest.mark.parametrize("tableau", ["SSP222", "SSP433", "SP111"])
+def test_stabilised_step_matches_oracle(tableau, ke_method):
+    """advance_imex_rk_stab on the GPU (ImexRKStab) against the oracle with
+    the stand-in solve on both sides: a compressive state (blended, for the
+    second-order tableaux) and a uniform flow (not blended)."""
+    gamma, eps, k1 = 1.4, 1.0, 0.05
+    n = 32
+    dx = 1.0 / n
+    uni = np.empty((4, n, n))
+    uni[0], uni[1], uni[2], uni[3] = 1.2, 0.36, -0.24, 2.0 / 0.4 + 0.5 * 1.2 * (0.09 + 0.04)
+    for tag, U in (("compressive", _compressive_state(n, n, gamma, eps, 11)), ("uniform", uni)):
+        announce("stabilised step %s %s %s" % (tableau, ke_method, tag))
+        p = io.pressure_from_state(U, gamma, eps)
+        rec = {}
+        U_ref, p_ref = io.advance_imex_rk_stab(
+            U, p, 0.004, dx, dx, tableau,
+            lambda sy, p0: io.pressure_from_state(sy["Uex"], gamma, eps), gamma, eps,
+            "muscl_rusanov", ke_method=ke_method, k1=k1, record=rec)
+        st = imex.ImexRKStab(n, n, dx, dx, gamma, eps, "muscl_rusanov", tableau, ke_method, k1)
+        dU, dp, dout, dpn = dev(U), dev(p), DeviceArray(U.shape), DeviceArray((n, n))
+        try:
+            def stand_in(sy):
+                st.hi.stage.pressure(sy["Uex"], dpn)
+                return dpn
+            st.advance(dU, dp, 0.004, dout, stand_in)
+            blended = "U_lo" in rec
+            assert st.blended == blended and (blended or tag == "uniform" or tableau == "SP111")
+            if tableau != "SP111":
+                assert st.ind_max == rec["ind_max"]
+            got = dout.download()
+            assert np.array_equal(got, U_ref), (tag, np.argwhere(got != U_ref)[:3])
+            assert np.array_equal(dp.download(), p_ref)
+        finally:
+            st.free()
+            for d in (dU, dp, dout, dpn):
+                d.free()

@@ -370,3 +370,88 @@ def test_split_conserves_and_keeps_uniform_flow(tableau):
     scale = np.abs(Ug).sum(axis=(1, 2))
     assert np.all(np.abs(Un.sum(axis=(1, 2)) - Ug.sum(axis=(1, 2))) <= 1e-13 * scale)
     assert np.max(np.abs(pn - pg)) < 1e-4 * pg.max()
+
+
+def _compressive_state(ny, nx, gamma, eps, seed):
+    """A state with converging flow in the middle (negative div u)."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    rho = 1.0 + 0.1 * rng.random((ny, nx))
+    u = -0.8 * np.sin(2 * np.pi * X) + 0.05 * rng.standard_normal((ny, nx))
+    v = -0.5 * np.sin(2 * np.pi * Y) + 0.05 * rng.standard_normal((ny, nx))
+    p = 1.0 + 0.1 * rng.random((ny, nx))
+    return state_from_primv(rho, u, v, p, gamma, eps)
+
+
+def test_shock_indicator_per_cell():
+    """compute_shock_indicator (IMEX_RK.cpp:243-345) against a per-cell
+    plain-Python evaluation on a small periodic grid (dx != dy, eps != 1),
+    with values across 0 < ind < 1 present; uniform flow gives 0."""
+    gamma, eps, k1 = 1.4, 0.7, 0.05
+    ny, nx = 7, 9
+    dx, dy = 1.0 / nx, 0.8 / ny
+    U = _compressive_state(ny, nx, gamma, eps, 3)
+    ind, ind_max = io.shock_indicator(U, dx, dy, gamma, eps, k1)
+
+    def prim(j, i):
+        j, i = j % ny, i % nx
+        r = U[0, j, i]
+        u, v = U[1, j, i] / r, U[2, j, i] / r
+        pp = (gamma - 1) * (U[3, j, i] - 0.5 * eps * r * (u * u + v * v))
+        return u, v, math.sqrt(gamma * pp / (r * eps))
+
+    pre = np.empty((ny, nx))
+    for j in range(ny):
+        for i in range(nx):
+            cs = [prim(j, i)[2], prim(j, i + 1)[2], prim(j, i - 1)[2], prim(j + 1, i)[2],
+                  prim(j - 1, i)[2]]
+            cm = min(cs)
+            div = 0.5 * ((1.0 / dx) * (prim(j, i + 1)[0] - prim(j, i - 1)[0])
+                         + (1.0 / dy) * (prim(j + 1, i)[1] - prim(j - 1, i)[1]))
+            pre[j, i] = min(1.0, max(0.0, -(div + k1 * cm) / (k1 * cm)))
+    for j in range(ny):
+        for i in range(nx):
+            m = max(pre[(j + dj) % ny, (i + di) % nx] for dj in (-1, 0, 1) for di in (-1, 0, 1))
+            assert ind[j, i] == m, (j, i)
+    assert ind_max == ind.max() and 0 < ind_max
+    assert np.any((ind > 0) & (ind < 1)) and np.any(ind == 0)
+    Uu = state_from_primv(np.full((4, 5), 1.1), np.full((4, 5), 0.3), np.full((4, 5), -0.2),
+                          np.full((4, 5), 2.0), gamma, eps)
+    assert io.shock_indicator(Uu, 0.2, 0.25, gamma, eps, k1)[1] == 0.0
+
+
+def test_stabilised_step_blends_only_where_indicated():
+    """advance_imex_rk_stab: on a smooth uniform flow it is the plain step;
+    on a compressive flow the new state is ind U_o1 + (1 - ind) U_hi where
+    ind > 0 and U_hi elsewhere, U_o1 being the SP111 Rusanov step; SP111
+    itself is never blended."""
+    gamma, eps, k1 = 1.4, 1.0, 0.05
+    n = 16
+    dx = 1.0 / n
+    solve = direct_solve(dx, dx)
+    Uu = state_from_primv(np.full((n, n), 1.2), np.full((n, n), 0.3), np.full((n, n), -0.2),
+                          np.full((n, n), 2.0), gamma, eps)
+    pu = io.pressure_from_state(Uu, gamma, eps)
+    a = io.advance_imex_rk_stab(Uu, pu, 0.01, dx, dx, "SSP222", solve, gamma, eps, "muscl_rusanov",
+                                k1=k1)
+    b = io.advance_imex_rk(Uu, pu, 0.01, dx, dx, "SSP222", solve, gamma, eps, "muscl_rusanov")
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    U = _compressive_state(n, n, gamma, eps, 5)
+    p = io.pressure_from_state(U, gamma, eps)
+    rec = {}
+    Us, ps = io.advance_imex_rk_stab(U, p, 0.004, dx, dx, "SSP222", solve, gamma, eps,
+                                     "muscl_rusanov", k1=k1, record=rec)
+    ind = rec["ind"]
+    assert rec["ind_max"] > 0 and np.any(ind == 1) and np.any(ind == 0)
+    lo, hi = rec["U_lo"], rec["U_hi"]
+    assert np.array_equal(Us[:, ind == 0], hi[:, ind == 0])
+    assert np.array_equal(Us[:, ind == 1], lo[:, ind == 1])
+    mid = (ind > 0) & (ind < 1)
+    assert np.array_equal(Us[:, mid], (ind * lo + (1 - ind) * hi)[:, mid])
+    assert np.array_equal(ps[ind == 1], rec["p_lo"][ind == 1])
+    U1, _ = io.advance_imex_rk(U, p, 0.004, dx, dx, "SP111", solve, gamma, eps, "muscl_rusanov")
+    U2, _ = io.advance_imex_rk_stab(U, p, 0.004, dx, dx, "SP111", solve, gamma, eps, "muscl_rusanov",
+                                    k1=k1)
+    assert np.array_equal(U1, U2)

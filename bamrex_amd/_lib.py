@@ -173,6 +173,9 @@ _SIGS = {
                                                ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                ctypes.c_double, P, P, ctypes.POINTER(ctypes.c_double), P]),
     "bx_imex_stab_blend": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_int, P, P, P]),
+    "bx_imex_picard_update": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, P, P, P, P, P]),
     "bx_cnn_create": (ctypes.c_int, [ctypes.c_uint, ctypes.POINTER(P)]),
     "bx_cnn_destroy": (None, [P]),
     "bx_cnn_set_params": (ctypes.c_int, [P, P, ctypes.c_int]),
@@ -191,7 +194,7 @@ _SIGS = {
 BX_SOLVE = {"mg": 0, "fgmres_mg": 1, "fgmres_cnn": 2, "mg_fused": 3}
 BX_IMEX_ADVECTION = {"rusanov": 0, "muscl_rusanov": 1}
 BX_IMEX_MAX_STAGES = 4
-BX_IMEX_KE = {"ex": 0, "split": 1}
+BX_IMEX_KE = {"ex": 0, "split": 1, "fi": 2}
 BX_CNN_MODE = {"smoother": 0, "corrector": 1}
 BX_CNN_NPARAMS = 4896
 

@@ -255,10 +255,12 @@ __device__ __forceinline__ double peek_enth(const double *__restrict__ p,
 // 0.5 (h(i-1) + h(i))).  The periodic image faces (bX column nx, bY row ny)
 // repeat faces 0.  SPLIT (ke_method split, compute_ke / compute_velocity,
 // IMEX_O1.cpp:232-298): ke = 0.5 eps (m_e . m_ex) / rho_e of the stage's
-// explicit state Ue, and the operator's velocity c = m_e / rho_e.
-template <bool SPLIT>
+// explicit state Ue, and the operator's velocity c = m_e / rho_e.  FI (not
+// linearised): ke of the implicit stage state Ui (statenew = statein_imp).
+template <int KE>
 __global__ __launch_bounds__(256) void k_peek_system(
-    const double *__restrict__ Ue, const double *__restrict__ Uex, const double *__restrict__ p,
+    const double *__restrict__ Ue, const double *__restrict__ Ui,
+    const double *__restrict__ Uex, const double *__restrict__ p,
     double *__restrict__ enth, double *__restrict__ rhs, double *__restrict__ acoef,
     double *__restrict__ bX, double *__restrict__ bY, double *__restrict__ cX,
     double *__restrict__ cY, PeekGeom g, double adia, double eps, double hdtdx, double hdtdy)
@@ -280,13 +282,15 @@ __global__ __launch_bounds__(256) void k_peek_system(
     const double    hs  = peek_enth(p, Uex, cs, adia);
     const double    hn  = peek_enth(p, Uex, cn, adia);
     double          ke;
-    if (SPLIT)
+    if (KE == BX_IMEX_KE_SPLIT)
     {
         const double r0 = Ue[c], m1 = Ue[c + n], m2 = Ue[c + 2 * n];
         ke    = 0.5 * eps * (m1 * Uex[c + n] + m2 * Uex[c + 2 * n]) / r0;
         cX[c] = m1 / r0;
         cY[c] = m2 / r0;
     }
+    else if (KE == BX_IMEX_KE_FI)
+        ke = peek_ke(Ui[c], Ui[c + n], Ui[c + 2 * n], eps);
     else
         ke = peek_ke(Uex[c], Uex[c + n], Uex[c + 2 * n], eps);
     rhs[c] = eps
@@ -461,6 +465,50 @@ __global__ __launch_bounds__(256) void k_rk_pressure_ek(const double *__restrict
     const double    e1 = Uo[c + g.n] - (dtdx * (Fxr[1] - Fxl[1]) + dtdy * (Fyr[1] - Fyl[1]));
     const double    e2 = Uo[c + 2 * g.n] - (dtdx * (Fxr[2] - Fxl[2]) + dtdy * (Fyr[2] - Fyl[2]));
     p[c] = (adia - 1) * (Un[c + 3 * g.n] - 0.5 * eps * (e1 * e1 + e2 * e2) / Un[c]);
+}
+
+// The end of one Picard iteration of the FI stage (picard_iterate,
+// IMEX_O1.cpp:755-781) after the solve gave p: momentum m_ex - hdtde
+// (p(+) - p(-)) (update_momentum, :455-482), enthalpy from p, kinetic
+// energy of the updated state, the source and the face enthalpy.
+__global__ __launch_bounds__(256) void k_picard_update(
+    const double *__restrict__ Uex, const double *__restrict__ p, double *__restrict__ enth,
+    double *__restrict__ rhs, double *__restrict__ bX, double *__restrict__ bY, PeekGeom g,
+    double adia, double eps, double hdtdx, double hdtdy, double hdtdxe, double hdtdye)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.nx || j >= g.ny)
+        return;
+    const long long row = (long long)j * g.nx;
+    const long long c   = row + i;
+    const long long cw  = row + wrap_lo(i, g.nx);
+    const long long ce  = row + wrap_hi(i, g.nx);
+    const long long cs  = (long long)wrap_lo(j, g.ny) * g.nx + i;
+    const long long cn  = (long long)wrap_hi(j, g.ny) * g.nx + i;
+    const long long n   = g.n;
+    const double    mx  = Uex[c + n] - hdtdxe * (p[ce] - p[cw]);
+    const double    my  = Uex[c + 2 * n] - hdtdye * (p[cn] - p[cs]);
+    const double    h   = peek_enth(p, Uex, c, adia);
+    const double    hw  = peek_enth(p, Uex, cw, adia);
+    const double    he  = peek_enth(p, Uex, ce, adia);
+    const double    hs  = peek_enth(p, Uex, cs, adia);
+    const double    hn  = peek_enth(p, Uex, cn, adia);
+    const double    ke  = peek_ke(Uex[c], mx, my, eps);
+    rhs[c] = eps
+             * (Uex[c + 3 * n] - ke
+                - (hdtdx * (he * Uex[ce + n] - hw * Uex[cw + n])
+                   + hdtdy * (hn * Uex[cn + 2 * n] - hs * Uex[cs + 2 * n])));
+    enth[c]           = h;
+    const double bx_f = 0.5 * (hw + h);
+    const double by_f = 0.5 * (hs + h);
+    const int    bxw  = g.nx + 1;
+    bX[(long long)j * bxw + i] = bx_f;
+    if (i == 0)
+        bX[(long long)j * bxw + g.nx] = bx_f;
+    bY[c] = by_f;
+    if (j == 0)
+        bY[(long long)g.ny * g.nx + i] = by_f;
 }
 
 // SK pressure of a later stage (compute_pressure, IMEX_O1.cpp:326-344):
@@ -776,7 +824,8 @@ static int stage_assemble(const char *who, const double *Ui, const double *Ue, c
     const bool split = ke_method == BX_IMEX_KE_SPLIT;
     if (!Ui || !Ue || !p || !Uex || !enth || !rhs || !a || !bX || !bY
         || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0) || !adv_ok(advection_flux)
-        || (ke_method != BX_IMEX_KE_EX && !split) || (split && (!cX || !cY)))
+        || (ke_method != BX_IMEX_KE_EX && ke_method != BX_IMEX_KE_FI && !split)
+        || (split && (!cX || !cY)))
         return bx_helm_fail(BX_ERR_INVALID, std::string(who) + ": bad argument");
     // outputs are written while inputs and other outputs are read
     const int   nout    = split ? 8 : 6;
@@ -808,12 +857,17 @@ static int stage_assemble(const char *who, const double *Ui, const double *Ue, c
                            Ue, Ui, Uex, g, eps, dtdx, dtdy);
     PEEK_CHECK(hipGetLastError());
     if (split)
-        hipLaunchKernelGGL(k_peek_system<true>, peek_grid(nx, ny), dim3(64, 4), 0, s, Ue, Uex, p,
-                           enth, rhs, a, bX, bY, cX, cY, g, gamma, eps, hdtdx, hdtdy);
+        hipLaunchKernelGGL(k_peek_system<BX_IMEX_KE_SPLIT>, peek_grid(nx, ny), dim3(64, 4), 0, s,
+                           Ue, Ui, Uex, p, enth, rhs, a, bX, bY, cX, cY, g, gamma, eps, hdtdx,
+                           hdtdy);
+    else if (ke_method == BX_IMEX_KE_FI)
+        hipLaunchKernelGGL(k_peek_system<BX_IMEX_KE_FI>, peek_grid(nx, ny), dim3(64, 4), 0, s, Ue,
+                           Ui, Uex, p, enth, rhs, a, bX, bY, (double *)nullptr,
+                           (double *)nullptr, g, gamma, eps, hdtdx, hdtdy);
     else
-        hipLaunchKernelGGL(k_peek_system<false>, peek_grid(nx, ny), dim3(64, 4), 0, s, Ue, Uex, p,
-                           enth, rhs, a, bX, bY, (double *)nullptr, (double *)nullptr, g, gamma,
-                           eps, hdtdx, hdtdy);
+        hipLaunchKernelGGL(k_peek_system<BX_IMEX_KE_EX>, peek_grid(nx, ny), dim3(64, 4), 0, s, Ue,
+                           Ui, Uex, p, enth, rhs, a, bX, bY, (double *)nullptr,
+                           (double *)nullptr, g, gamma, eps, hdtdx, hdtdy);
     PEEK_CHECK(hipGetLastError());
     return BX_OK;
 }
@@ -961,9 +1015,18 @@ extern "C" int bx_imex_rk_pressure(const double *U_exp_new, const double *U_exp_
     if (!U_exp_new || !U_exp_old || !p || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0)
         || !adv_ok(advection_flux) || (const double *)p == U_exp_new
         || (const double *)p == U_exp_old
-        || (ke_method != BX_IMEX_KE_EX && ke_method != BX_IMEX_KE_SPLIT))
+        || (ke_method != BX_IMEX_KE_EX && ke_method != BX_IMEX_KE_SPLIT
+            && ke_method != BX_IMEX_KE_FI))
         return bx_helm_fail(BX_ERR_INVALID, "bx_imex_rk_pressure: bad argument");
     PeekGeom g{ nx, ny, (long long)nx * ny };
+    if (ke_method == BX_IMEX_KE_FI)
+    {
+        // FI: the new explicit state's own pressure (IMEX_O1.cpp:322-325)
+        hipLaunchKernelGGL(k_peek_pressure, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0,
+                           (hipStream_t)stream, U_exp_new, p, g.n, gamma, eps);
+        PEEK_CHECK(hipGetLastError());
+        return BX_OK;
+    }
     if (ke_method == BX_IMEX_KE_SPLIT)
     {
         hipLaunchKernelGGL(k_rk_pressure_sk, peek_grid(nx, ny), dim3(64, 4), 0,
@@ -1043,6 +1106,33 @@ extern "C" int bx_imex_stab_blend(const double *ind, const double *U_o1, const d
     PeekGeom g{ nx, ny, (long long)nx * ny };
     hipLaunchKernelGGL(k_stab_blend, peek_grid(nx, ny), dim3(64, 4), 0, (hipStream_t)stream, ind,
                        U_o1, p_o1, U, p, g);
+    PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_imex_picard_update(const double *Uex, const double *p, int nx, int ny,
+                                     double dx, double dy, double adt, double gamma, double eps,
+                                     double *enth, double *rhs, double *bX, double *bY,
+                                     void *stream)
+{
+    if (!Uex || !p || !enth || !rhs || !bX || !bY || !peek_sizes_ok(nx, ny) || !(dx > 0)
+        || !(dy > 0))
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_picard_update: bad argument");
+    const void *outs[4] = { enth, rhs, bX, bY };
+    for (int i = 0; i < 4; ++i)
+    {
+        if (outs[i] == (const void *)Uex || outs[i] == (const void *)p)
+            return bx_helm_fail(BX_ERR_INVALID, "bx_imex_picard_update: output aliases an input");
+        for (int j = i + 1; j < 4; ++j)
+            if (outs[i] == outs[j])
+                return bx_helm_fail(BX_ERR_INVALID, "bx_imex_picard_update: outputs alias");
+    }
+    PeekGeom     g{ nx, ny, (long long)nx * ny };
+    const double hdtdx = 0.5 * adt * (1.0 / dx); // IMEX_O1.cpp:66-70
+    const double hdtdy = 0.5 * adt * (1.0 / dy);
+    hipLaunchKernelGGL(k_picard_update, peek_grid(nx, ny), dim3(64, 4), 0, (hipStream_t)stream,
+                       Uex, p, enth, rhs, bX, bY, g, gamma, eps, hdtdx, hdtdy, hdtdx / eps,
+                       hdtdy / eps);
     PEEK_CHECK(hipGetLastError());
     return BX_OK;
 }

@@ -244,15 +244,22 @@ class ImexRK:
 
     ke_method (imex.ke_method): "ex" (PEEK) or "split" (PESK); with split the
     system carries the operator's velocity term (gamma_c and the device
-    arrays cX, cY) for HelmholtzSolver.set_coeffs."""
+    arrays cX, cY) for HelmholtzSolver.set_coeffs.  "fi" is the
+    non-linearised form (imex.linearise = false): each stage solves
+    picard_max_iter times (imex.picard_max_iter), calling picard(p) after
+    every solve, which updates the system in place."""
 
     def __init__(self, nx, ny, dx, dy=None, gamma=1.4, eps=1.0, advection="rusanov",
-                 tableau="SP111", ke_method="ex"):
+                 tableau="SP111", ke_method="ex", picard_max_iter=2):
         self.tab = ButcherTableau(tableau)
         if ke_method not in _lib.BX_IMEX_KE:
-            raise ValueError("imex.ke_method: ex or split, not %r" % (ke_method,))
+            raise ValueError("imex.ke_method: ex, split or fi, not %r" % (ke_method,))
         self.ke = _lib.BX_IMEX_KE[ke_method]
         self.split = ke_method == "split"
+        self.fi = ke_method == "fi"
+        if self.fi and not picard_max_iter >= 1:
+            raise ValueError("imex.picard_max_iter must be at least 1")
+        self.picard_max_iter = picard_max_iter
         self.stage = PeekStage(nx, ny, dx, dy, gamma, eps, advection)
         self.nx, self.ny, self.dx, self.dy = nx, ny, self.stage.dx, self.stage.dy
         self.gamma, self.eps, self.adv = gamma, eps, self.stage.adv
@@ -307,6 +314,17 @@ class ImexRK:
                     beta=adt * adt, gamma_c=(-0.5 * self.eps * adt) if self.split else 0.0,
                     cX=self.cX, cY=self.cY, dt=dt, adt=adt, stage=s, p_prev=self.p)
 
+    def picard(self, p_new):
+        """FI: the end of a Picard iteration after a solve gave p_new; the
+        stage's system (enth, rhs, bX, bY) is updated for the next solve,
+        or for the fluxes after the last one."""
+        if not (self.fi and getattr(self, "_built", False)):
+            raise RuntimeError("ImexRK.picard: only inside an FI stage")
+        st = self.stage
+        _chk(lib().bx_imex_picard_update(st.Uex.ptr, p_new.ptr, self.nx, self.ny, self.dx, self.dy,
+                                         self._adt, self.gamma, self.eps, st.enth.ptr, st.rhs.ptr,
+                                         st.bX.ptr, st.bY.ptr, None), "bx_imex_picard_update")
+
     def complete(self, p_new, U_out):
         """The stage's fluxes with the solved pressure p_new; after the last
         stage the step's update into U_out (not the step's U).  Returns
@@ -341,7 +359,15 @@ class ImexRK:
         self.begin(U, p, dt)
         while True:
             sy = self.system()
-            if self.complete(solve(sy), U_out):
+            if self.fi:
+                for _ in range(self.picard_max_iter):
+                    p_new = solve(sy)
+                    if p_new.ptr.value != self.p.ptr.value:
+                        self.p.copy_from(p_new)   # the next solve starts from it
+                    self.picard(self.p)
+            else:
+                p_new = solve(sy)
+            if self.complete(p_new, U_out):
                 return
 
     def free(self):

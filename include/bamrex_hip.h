@@ -458,6 +458,11 @@ int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
  * gamma_c = -eps adt / 2 and c = m_e / rho_e). */
 #define BX_IMEX_KE_EX 0
 #define BX_IMEX_KE_SPLIT 1
+/* imex.linearise = false (the FI schemes): the stage's first system takes
+ * the kinetic energy of the implicit stage state, each solve is followed
+ * by bx_imex_picard_update (imex.picard_max_iter solves per stage), and a
+ * later stage starts from the explicit state's own pressure. */
+#define BX_IMEX_KE_FI 2
 /* The stage states (conservative_update into two outputs, Source/Fluxes/
  * Update.H:218-302): U_imp = U - sum_{s<nstage} dt_imp[s] div F_s and
  * U_exp likewise with dt_exp, dt_x[s] = A_x[stage][s] dt (host arrays of
@@ -471,11 +476,22 @@ int bx_imex_rk_combine(const double *U, const double *const *Fx,
  * explicit state advanced over adt by its own advective fluxes (MUSCL
  * without the local step) supplies the momentum, U_exp_new the density
  * and energy: p = (gamma-1)(E - 0.5 eps |m_ex|^2 / rho).  ke_method split
- * (SK): p = (gamma-1)(E_new - 0.5 eps (m_old . m_new) / rho_old). */
+ * (SK): p = (gamma-1)(E_new - 0.5 eps (m_old . m_new) / rho_old).
+ * BX_IMEX_KE_FI: the pressure of U_exp_new, (gamma-1)(E - ke). */
 int bx_imex_rk_pressure(const double *U_exp_new, const double *U_exp_old, int nx,
                         int ny, double dx, double dy, double adt, double gamma,
                         double eps, int advection_flux, int ke_method, double *p,
                         void *stream);
+/* The end of one Picard iteration of an FI stage (picard_iterate,
+ * Source/IMEX/IMEX_O1.cpp:702-789) after a solve gave p: the momentum
+ * m_ex - (0.5 adt/dx/eps)(p(+) - p(-)) (update_momentum, :455-482), the
+ * enthalpy from p, the kinetic energy of that state, and from them the
+ * new source rhs and face coefficients bX, bY of the next solve (a is
+ * unchanged). */
+int bx_imex_picard_update(const double *Uex, const double *p, int nx, int ny,
+                          double dx, double dy, double adt, double gamma, double eps,
+                          double *enth, double *rhs, double *bX, double *bY,
+                          void *stream);
 /* compute_flux_imex_o1 up to the solve with separate stage states: the
  * advective fluxes of U_exp update U_imp; local_time_step selects the
  * MUSCL local half step (the reference takes it for SP111 only,

@@ -29,6 +29,7 @@ IMEX Runge-Kutta tableaux (advance_imex_rk).
   compute_pressure (EK and SK branches)  Source/IMEX/IMEX_O1.cpp:300-391
   compute_ke (ex, split), compute_velocity
                                          Source/IMEX/IMEX_O1.cpp:232-298
+  picard_iterate, update_momentum (FI)   Source/IMEX/IMEX_O1.cpp:455-482, 702-789
   IMEXButcherTableau                     Source/IMEX/ButcherTableau.cpp:5-101
   advance_imex_rk_stab, compute_shock_indicator
                                          Source/IMEX/IMEX_RK.cpp:98-194, 243-345
@@ -159,12 +160,13 @@ class PeekStage:
     advection: "rusanov" (advance_rusanov_adv) or "muscl_rusanov"."""
 
     def __init__(self, gamma=1.4, eps=1.0, advection="rusanov", ke_method="ex"):
-        if ke_method not in ("ex", "split"):
-            raise ValueError("ke_method: ex or split")
+        if ke_method not in ("ex", "split", "fi"):
+            raise ValueError("ke_method: ex, split or fi (not linearised)")
         self.gamma = gamma
         self.eps = eps
         self.muscl = advection == "muscl_rusanov"
         self.split = ke_method == "split"
+        self.fi = ke_method == "fi"
 
     def assemble(self, U, p, dt, dx, dy=None, U_exp=None, local=True):
         """compute_flux_imex_o1 up to the solve (IMEX_O1.cpp:64-125): the
@@ -184,6 +186,12 @@ class PeekStage:
             ke = 0.5 * eps * (Uexp[1] * Uex[1] + Uexp[2] * Uex[2]) / Uexp[0]
             cX, cY = Uexp[1] / Uexp[0], Uexp[2] / Uexp[0]
             gamma_c = -0.5 * eps * dt
+        elif self.fi:
+            # compute_ke, not linearised: the kinetic energy of the implicit
+            # stage state (statenew = statein_imp, IMEX_O1.cpp:108-109)
+            ke = kinetic_energy(U, eps)
+            cX = cY = None
+            gamma_c = 0.0
         else:
             ke = kinetic_energy(Uex, eps)
             cX = cY = None
@@ -207,6 +215,39 @@ class PeekStage:
         a = np.full_like(p, eps / (g - 1))
         return dict(Uex=Uex, enth=enth, rhs=rhs, a=a, bX=bX, bY=bY,
                     alpha=1.0, beta=dt * dt, gamma_c=gamma_c, cX=cX, cY=cY, Fx=Fx, Fy=Fy)
+
+    def picard_update(self, st, p, dt, dx, dy=None):
+        """The end of one Picard iteration (picard_iterate, IMEX_O1.cpp:
+        755-781) after the solve gave p: the momentum update_momentum
+        (m_ex - hdtde (p(+) - p(-))), the enthalpy from p, the kinetic energy
+        of the updated state, the source and the face coefficients.  Returns
+        the updated system (a new dict)."""
+        dy = dx if dy is None else dy
+        g, eps = self.gamma, self.eps
+        ex = st["Uex"]
+        hdtdx = 0.5 * dt * (1.0 / dx)
+        hdtdy = 0.5 * dt * (1.0 / dy)
+        hdtdxe, hdtdye = hdtdx / eps, hdtdy / eps
+        mx = ex[1] - hdtdxe * (_sh(p, 0, 1) - _sh(p, 0, -1))
+        my = ex[2] - hdtdye * (_sh(p, 1, 1) - _sh(p, 1, -1))
+        enth = p * g / ((g - 1) * ex[0])
+        ke = eps * (mx * mx + my * my) / (2 * ex[0])
+        rhs = eps * (ex[3] - ke - (hdtdx * (_sh(enth, 0, 1) * _sh(ex[1], 0, 1)
+                                            - _sh(enth, 0, -1) * _sh(ex[1], 0, -1))
+                                   + hdtdy * (_sh(enth, 1, 1) * _sh(ex[2], 1, 1)
+                                              - _sh(enth, 1, -1) * _sh(ex[2], 1, -1))))
+        hx = 0.5 * (_sh(enth, 0, -1) + enth)
+        hy = 0.5 * (_sh(enth, 1, -1) + enth)
+        ny, nx = p.shape
+        bX = np.empty((ny, nx + 1))
+        bX[:, :nx] = hx
+        bX[:, nx] = hx[:, 0]
+        bY = np.empty((ny + 1, nx))
+        bY[:ny] = hy
+        bY[ny] = hy[0]
+        out = dict(st)
+        out.update(enth=enth, rhs=rhs, bX=bX, bY=bY, m_new=(mx, my))
+        return out
 
     def fluxes(self, st, p_new, dt, dx, dy=None):
         """update_momentum_flux / update_energy_flux (IMEX_O1.cpp:485-566):
@@ -328,12 +369,14 @@ class ButcherTableau:
 
 
 def advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0, advection="rusanov",
-                    record=None, ke_method="ex"):
+                    record=None, ke_method="ex", picard_max_iter=2):
     """advance_imex_rk (IMEX_RK.cpp:22-96) on a 2D periodic level.  solve(sy,
     p0) returns the stage's pressure (sy: the PeekStage.assemble system,
     p0: the initial guess).  Returns (U_new, p) with p the last stage's
-    pressure.  ke_method: "ex" (PEEK) or "split" (PESK: the system carries
-    the velocity term gamma_c, cX, cY).  record, a list, receives each stage's system, its states, the
+    pressure.  ke_method: "ex" (PEEK), "split" (PESK: the system carries
+    the velocity term gamma_c, cX, cY) or "fi" (not linearised: each stage
+    solves picard_max_iter times, updating the system after every solve;
+    later stages start from the state's own pressure).  record, a list, receives each stage's system, its states, the
     initial and solved pressure and its corrected fluxes (Fx, Fy; the
     advective ones as Fx_adv, Fy_adv)."""
     tab = ButcherTableau(tableau) if isinstance(tableau, str) else tableau
@@ -343,13 +386,20 @@ def advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0, advect
         Uimp, Uexp = stage_states(U, Fxs, Fys, tab.imp_row(s, dt)[:s], tab.exp_row(s, dt)[:s],
                                   dx, dy)
         adt = tab.A_imp[s][s] * dt
-        if s > 0 and st.split:
+        if s > 0 and st.fi:
+            p = pressure_from_state(Uexp, gamma, eps)   # compute_pressure, FI
+        elif s > 0 and st.split:
             p = pressure_sk(Uexp, exps[-1], gamma, eps)
         elif s > 0:
             p = pressure_ek(Uexp, exps[-1], adt, dx, dy, gamma, eps, st.muscl)
         sy = st.assemble(Uimp, p, adt, dx, dy, U_exp=Uexp, local=tab.name == "SP111")
         p_in = p
-        p = solve(sy, p)
+        if st.fi:
+            for _ in range(picard_max_iter):
+                p = solve(sy, p)
+                sy = st.picard_update(sy, p, adt, dx, dy)
+        else:
+            p = solve(sy, p)
         Fx, Fy = st.fluxes(sy, p, adt, dx, dy)
         Fxs.append(Fx)
         Fys.append(Fy)

@@ -455,3 +455,97 @@ def test_stabilised_step_blends_only_where_indicated():
     U2, _ = io.advance_imex_rk_stab(U, p, 0.004, dx, dx, "SP111", solve, gamma, eps, "muscl_rusanov",
                                     k1=k1)
     assert np.array_equal(U1, U2)
+
+
+def test_fi_picard_fixed_point_satisfies_the_equation_of_state():
+    """Not linearised (FI): at a converged Picard fixed point the stage's
+    new energy equals p/(gamma-1) + eps |m_new|^2 / (2 rho_new) -- the
+    nonlinear system the iteration solves (picard_iterate, IMEX_O1.cpp:
+    702-789).  One iteration is not enough for it (negative control); the
+    iterates converge."""
+    n = 32
+    gamma, eps, dt = 1.6, 0.5, 0.01
+    rng = np.random.default_rng(4)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) + 0.05 * rng.random((n, n))
+    u = 0.4 * np.sin(2 * np.pi * Y)
+    v = -0.3 * np.cos(2 * np.pi * X)
+    p = 1.0 + 0.2 * np.cos(2 * np.pi * (X + Y))
+    U = state_from_primv(rho, u, v, p, gamma, eps)
+    dx = 1.0 / n
+
+    def eos_gap(U1, p1):
+        expected = p1 / (gamma - 1) + eps * (U1[1] * U1[1] + U1[2] * U1[2]) / (2 * U1[0])
+        return np.max(np.abs(U1[3] - expected) / np.abs(expected))
+
+    for adv in ("rusanov", "muscl_rusanov"):
+        U1, p1 = io.advance_imex_rk(U, p, dt, dx, dx, "SP111", direct_solve(dx, dx), gamma, eps, adv,
+                                    ke_method="fi", picard_max_iter=40)
+        assert eos_gap(U1, p1) <= 1e-11, adv
+    U1, p1 = io.advance_imex_rk(U, p, dt, dx, dx, "SP111", direct_solve(dx, dx), gamma, eps,
+                                "rusanov", ke_method="fi", picard_max_iter=1)
+    assert eos_gap(U1, p1) > 1e-9
+    gaps = []
+    for it in (2, 4, 8):
+        U1, p1 = io.advance_imex_rk(U, p, dt, dx, dx, "SP111", direct_solve(dx, dx), gamma, eps,
+                                    "rusanov", ke_method="fi", picard_max_iter=it)
+        gaps.append(eos_gap(U1, p1))
+    assert gaps[0] > gaps[1] > gaps[2], gaps
+
+
+def test_fi_picard_update_per_cell():
+    """One Picard update (update_momentum, compute_enthalpy, compute_ke of
+    the updated state, pressure_source_vector, face enthalpy) against a
+    per-cell plain-Python evaluation, dx != dy."""
+    gamma, eps = 1.4, 0.8
+    rng = np.random.default_rng(12)
+    ny, nx = 4, 5
+    U = state_from_primv(1 + 0.2 * rng.random((ny, nx)), rng.uniform(-1, 1, (ny, nx)),
+                         rng.uniform(-1, 1, (ny, nx)), 2 + rng.random((ny, nx)), gamma, eps)
+    dx, dy, dt = 1.0 / nx, 0.7 / ny, 0.02
+    st = io.PeekStage(gamma, eps, "rusanov", "fi")
+    sy = st.assemble(U, io.pressure_from_state(U, gamma, eps), dt, dx, dy)
+    assert np.array_equal(sy["rhs"], eps * (sy["Uex"][3] - io.kinetic_energy(U, eps) - (
+        (0.5 * dt * (1.0 / dx)) * (np.roll(sy["enth"] * sy["Uex"][1], -1, 1)
+                                   - np.roll(sy["enth"] * sy["Uex"][1], 1, 1))
+        + (0.5 * dt * (1.0 / dy)) * (np.roll(sy["enth"] * sy["Uex"][2], -1, 0)
+                                     - np.roll(sy["enth"] * sy["Uex"][2], 1, 0)))))
+    pn = 2 + rng.random((ny, nx))
+    up = st.picard_update(sy, pn, dt, dx, dy)
+    ex = sy["Uex"]
+    hx, hy = 0.5 * dt * (1.0 / dx), 0.5 * dt * (1.0 / dy)
+    for j in range(ny):
+        for i in range(nx):
+            ip, im, jp, jm = (i + 1) % nx, (i - 1) % nx, (j + 1) % ny, (j - 1) % ny
+            mx = ex[1, j, i] - (hx / eps) * (pn[j, ip] - pn[j, im])
+            my = ex[2, j, i] - (hy / eps) * (pn[jp, i] - pn[jm, i])
+            h = lambda jj, ii: pn[jj, ii] * gamma / ((gamma - 1) * ex[0, jj, ii])  # noqa: E731
+            ke = eps * (mx * mx + my * my) / (2 * ex[0, j, i])
+            rhs = eps * (ex[3, j, i] - ke - (hx * (h(j, ip) * ex[1, j, ip] - h(j, im) * ex[1, j, im])
+                                             + hy * (h(jp, i) * ex[2, jp, i] - h(jm, i) * ex[2, jm, i])))
+            assert up["enth"][j, i] == h(j, i)
+            assert up["rhs"][j, i] == rhs
+            assert up["bX"][j, i] == 0.5 * (h(j, im) + h(j, i))
+            assert up["bY"][j, i] == 0.5 * (h(jm, i) + h(j, i))
+
+
+@pytest.mark.parametrize("tableau", ["SP111", "SSP222"])
+def test_fi_conserves_and_keeps_uniform_flow(tableau):
+    gamma, eps = 1.4, 1.0
+    n = 16
+    U = state_from_primv(np.full((n, n), 1.2), np.full((n, n), 0.3), np.full((n, n), -0.2),
+                         np.full((n, n), 2.0), gamma, eps)
+    dx = 1.0 / n
+    U1, p1 = io.advance_imex_rk(U, io.pressure_from_state(U, gamma, eps), 0.02, dx, dx, tableau,
+                                direct_solve(dx, dx), gamma, eps, "muscl_rusanov", ke_method="fi")
+    assert np.allclose(U1, U, rtol=1e-12, atol=0)
+    assert np.allclose(p1, 2.0, rtol=1e-12, atol=0)
+    Ug, dxg = imex.gresho_state(32)
+    pg = io.pressure_from_state(Ug, gamma, eps)
+    dtg = io.advective_dt(Ug, dxg)
+    Un, pn = io.advance_imex_rk(Ug, pg, dtg, dxg, dxg, tableau, direct_solve(dxg, dxg), gamma, eps,
+                                "muscl_rusanov", ke_method="fi")
+    scale = np.abs(Ug).sum(axis=(1, 2))
+    assert np.all(np.abs(Un.sum(axis=(1, 2)) - Ug.sum(axis=(1, 2))) <= 1e-13 * scale)
+    assert np.max(np.abs(pn - pg)) < 1e-4 * pg.max()

@@ -156,7 +156,7 @@ _SIGS = {
     "bx_imex_stage_assemble": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                              P, P, P, P, P, P, P, P, P]),
+                                              ctypes.c_int, P, P, P, P, P, P, P, P, P]),
     "bx_imex_stage_fluxes": (ctypes.c_int, [P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                             ctypes.c_int, P, P, P]),
@@ -194,7 +194,8 @@ _SIGS = {
 BX_SOLVE = {"mg": 0, "fgmres_mg": 1, "fgmres_cnn": 2, "mg_fused": 3}
 BX_IMEX_ADVECTION = {"rusanov": 0, "muscl_rusanov": 1}
 BX_IMEX_MAX_STAGES = 4
-BX_IMEX_KE = {"ex": 0, "split": 1, "fi": 2}
+BX_IMEX_KE = {"ex": 0, "split": 1, "fi": 2, "conservative_kinetic": 3}
+BX_IMEX_ENTHALPY = {"reuse_pressure": 0, "conservative": 1, "conservative_ex": 2}
 BX_CNN_MODE = {"smoother": 0, "corrector": 1}
 BX_CNN_NPARAMS = 4896
 

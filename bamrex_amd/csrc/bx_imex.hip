@@ -256,11 +256,15 @@ __device__ __forceinline__ double peek_enth(const double *__restrict__ p,
 // repeat faces 0.  SPLIT (ke_method split, compute_ke / compute_velocity,
 // IMEX_O1.cpp:232-298): ke = 0.5 eps (m_e . m_ex) / rho_e of the stage's
 // explicit state Ue, and the operator's velocity c = m_e / rho_e.  FI (not
-// linearised): ke of the implicit stage state Ui (statenew = statein_imp).
-template <int KE>
+// linearised): ke of the implicit stage state Ui (statenew = statein_imp);
+// conservative_kinetic: ke of Ue.  ENTH (compute_enthalpy, IMEX_O1.cpp:
+// 177-229): reuse_pressure, h = p gamma / ((gamma - 1) rho_ex); conservative
+// (_ex), h = gamma (E - ke) / rho_ex of Ue (of Uex), and p is replaced by
+// the pressure of Uex (p is then only written).
+template <int KE, int ENTH>
 __global__ __launch_bounds__(256) void k_peek_system(
     const double *__restrict__ Ue, const double *__restrict__ Ui,
-    const double *__restrict__ Uex, const double *__restrict__ p,
+    const double *__restrict__ Uex, double *__restrict__ p,
     double *__restrict__ enth, double *__restrict__ rhs, double *__restrict__ acoef,
     double *__restrict__ bX, double *__restrict__ bY, double *__restrict__ cX,
     double *__restrict__ cY, PeekGeom g, double adia, double eps, double hdtdx, double hdtdy)
@@ -275,13 +279,20 @@ __global__ __launch_bounds__(256) void k_peek_system(
     const long long ce  = row + wrap_hi(i, g.nx);
     const long long cs  = (long long)wrap_lo(j, g.ny) * g.nx + i;
     const long long cn  = (long long)wrap_hi(j, g.ny) * g.nx + i;
-    const long long n   = g.n;
-    const double    h   = peek_enth(p, Uex, c, adia);
-    const double    hw  = peek_enth(p, Uex, cw, adia);
-    const double    he  = peek_enth(p, Uex, ce, adia);
-    const double    hs  = peek_enth(p, Uex, cs, adia);
-    const double    hn  = peek_enth(p, Uex, cn, adia);
-    double          ke;
+    const long long n     = g.n;
+    auto            enthf = [&](long long x) {
+        if (ENTH == BX_IMEX_ENTHALPY_REUSE_PRESSURE)
+            return peek_enth(p, Uex, x, adia);
+        const double *S = (ENTH == BX_IMEX_ENTHALPY_CONSERVATIVE) ? Ue : Uex;
+        // vol_enthalpy (euler_K.H:41-50) / rho_ex
+        return adia * (S[x + 3 * n] - peek_ke(S[x], S[x + n], S[x + 2 * n], eps)) / Uex[x];
+    };
+    const double h  = enthf(c);
+    const double hw = enthf(cw);
+    const double he = enthf(ce);
+    const double hs = enthf(cs);
+    const double hn = enthf(cn);
+    double       ke;
     if (KE == BX_IMEX_KE_SPLIT)
     {
         const double r0 = Ue[c], m1 = Ue[c + n], m2 = Ue[c + 2 * n];
@@ -291,8 +302,12 @@ __global__ __launch_bounds__(256) void k_peek_system(
     }
     else if (KE == BX_IMEX_KE_FI)
         ke = peek_ke(Ui[c], Ui[c + n], Ui[c + 2 * n], eps);
+    else if (KE == BX_IMEX_KE_CONSERVATIVE_KINETIC)
+        ke = peek_ke(Ue[c], Ue[c + n], Ue[c + 2 * n], eps);
     else
         ke = peek_ke(Uex[c], Uex[c + n], Uex[c + 2 * n], eps);
+    if (ENTH != BX_IMEX_ENTHALPY_REUSE_PRESSURE)
+        p[c] = (adia - 1) * (Uex[c + 3 * n] - peek_ke(Uex[c], Uex[c + n], Uex[c + 2 * n], eps));
     rhs[c] = eps
              * (Uex[c + 3 * n] - ke
                 - (hdtdx * (he * Uex[ce + n] - hw * Uex[cw + n])
@@ -815,17 +830,18 @@ static bool adv_ok(int advection_flux)
 }
 
 // fluxes of Ue applied to Ui (Uex), then the pressure system
-static int stage_assemble(const char *who, const double *Ui, const double *Ue, const double *p,
-                          int nx, int ny, double dx, double dy, double dt, double gamma,
-                          double eps, int advection_flux, int local, int ke_method, double *Uex,
-                          double *enth, double *rhs, double *a, double *bX, double *bY,
-                          double *cX, double *cY, void *stream)
+static int stage_assemble(const char *who, const double *Ui, const double *Ue, double *p, int nx,
+                          int ny, double dx, double dy, double dt, double gamma, double eps,
+                          int advection_flux, int local, int ke_method, int enthalpy_method,
+                          double *Uex, double *enth, double *rhs, double *a, double *bX,
+                          double *bY, double *cX, double *cY, void *stream)
 {
     const bool split = ke_method == BX_IMEX_KE_SPLIT;
     if (!Ui || !Ue || !p || !Uex || !enth || !rhs || !a || !bX || !bY
         || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0) || !adv_ok(advection_flux)
-        || (ke_method != BX_IMEX_KE_EX && ke_method != BX_IMEX_KE_FI && !split)
-        || (split && (!cX || !cY)))
+        || ke_method < BX_IMEX_KE_EX || ke_method > BX_IMEX_KE_CONSERVATIVE_KINETIC
+        || enthalpy_method < BX_IMEX_ENTHALPY_REUSE_PRESSURE
+        || enthalpy_method > BX_IMEX_ENTHALPY_CONSERVATIVE_EX || (split && (!cX || !cY)))
         return bx_helm_fail(BX_ERR_INVALID, std::string(who) + ": bad argument");
     // outputs are written while inputs and other outputs are read
     const int   nout    = split ? 8 : 6;
@@ -856,18 +872,31 @@ static int stage_assemble(const char *who, const double *Ui, const double *Ue, c
         hipLaunchKernelGGL((k_peek_advect<false, false>), peek_grid(nx, ny), dim3(64, 4), 0, s,
                            Ue, Ui, Uex, g, eps, dtdx, dtdy);
     PEEK_CHECK(hipGetLastError());
-    if (split)
-        hipLaunchKernelGGL(k_peek_system<BX_IMEX_KE_SPLIT>, peek_grid(nx, ny), dim3(64, 4), 0, s,
-                           Ue, Ui, Uex, p, enth, rhs, a, bX, bY, cX, cY, g, gamma, eps, hdtdx,
-                           hdtdy);
-    else if (ke_method == BX_IMEX_KE_FI)
-        hipLaunchKernelGGL(k_peek_system<BX_IMEX_KE_FI>, peek_grid(nx, ny), dim3(64, 4), 0, s, Ue,
-                           Ui, Uex, p, enth, rhs, a, bX, bY, (double *)nullptr,
-                           (double *)nullptr, g, gamma, eps, hdtdx, hdtdy);
-    else
-        hipLaunchKernelGGL(k_peek_system<BX_IMEX_KE_EX>, peek_grid(nx, ny), dim3(64, 4), 0, s, Ue,
-                           Ui, Uex, p, enth, rhs, a, bX, bY, (double *)nullptr,
-                           (double *)nullptr, g, gamma, eps, hdtdx, hdtdy);
+    // the FI stage (not linearised) always takes the enthalpy from p
+    const int em = ke_method == BX_IMEX_KE_FI ? BX_IMEX_ENTHALPY_REUSE_PRESSURE : enthalpy_method;
+    double   *vx = split ? cX : nullptr, *vy = split ? cY : nullptr;
+#define BX_SYSTEM(KE, EN)                                                                      \
+    hipLaunchKernelGGL((k_peek_system<KE, EN>), peek_grid(nx, ny), dim3(64, 4), 0, s, Ue, Ui, Uex, \
+                       p, enth, rhs, a, bX, bY, vx, vy, g, gamma, eps, hdtdx, hdtdy)
+#define BX_SYSTEM_KE(KE)                                                                       \
+    do                                                                                         \
+    {                                                                                          \
+        if (em == BX_IMEX_ENTHALPY_CONSERVATIVE)                                               \
+            BX_SYSTEM(KE, BX_IMEX_ENTHALPY_CONSERVATIVE);                                      \
+        else if (em == BX_IMEX_ENTHALPY_CONSERVATIVE_EX)                                       \
+            BX_SYSTEM(KE, BX_IMEX_ENTHALPY_CONSERVATIVE_EX);                                   \
+        else                                                                                   \
+            BX_SYSTEM(KE, BX_IMEX_ENTHALPY_REUSE_PRESSURE);                                    \
+    } while (0)
+    switch (ke_method)
+    {
+    case BX_IMEX_KE_SPLIT: BX_SYSTEM_KE(BX_IMEX_KE_SPLIT); break;
+    case BX_IMEX_KE_FI: BX_SYSTEM(BX_IMEX_KE_FI, BX_IMEX_ENTHALPY_REUSE_PRESSURE); break;
+    case BX_IMEX_KE_CONSERVATIVE_KINETIC: BX_SYSTEM_KE(BX_IMEX_KE_CONSERVATIVE_KINETIC); break;
+    default: BX_SYSTEM_KE(BX_IMEX_KE_EX); break;
+    }
+#undef BX_SYSTEM_KE
+#undef BX_SYSTEM
     PEEK_CHECK(hipGetLastError());
     return BX_OK;
 }
@@ -878,21 +907,24 @@ extern "C" int bx_imex_peek_assemble(const double *U, const double *p, int nx, i
                                      double *rhs, double *a, double *bX, double *bY,
                                      void *stream)
 {
-    return stage_assemble("bx_imex_peek_assemble", U, U, p, nx, ny, dx, dy, dt, gamma, eps,
-                          advection_flux, 1, BX_IMEX_KE_EX, Uex, enth, rhs, a, bX, bY, nullptr,
+    // reuse_pressure only reads p
+    return stage_assemble("bx_imex_peek_assemble", U, U, const_cast<double *>(p), nx, ny, dx, dy,
+                          dt, gamma, eps, advection_flux, 1, BX_IMEX_KE_EX,
+                          BX_IMEX_ENTHALPY_REUSE_PRESSURE, Uex, enth, rhs, a, bX, bY, nullptr,
                           nullptr, stream);
 }
 
-extern "C" int bx_imex_stage_assemble(const double *U_imp, const double *U_exp, const double *p,
+extern "C" int bx_imex_stage_assemble(const double *U_imp, const double *U_exp, double *p,
                                       int nx, int ny, double dx, double dy, double adt,
                                       double gamma, double eps, int advection_flux,
-                                      int local_time_step, int ke_method, double *Uex,
-                                      double *enth, double *rhs, double *a, double *bX,
-                                      double *bY, double *cX, double *cY, void *stream)
+                                      int local_time_step, int ke_method, int enthalpy_method,
+                                      double *Uex, double *enth, double *rhs, double *a,
+                                      double *bX, double *bY, double *cX, double *cY,
+                                      void *stream)
 {
     return stage_assemble("bx_imex_stage_assemble", U_imp, U_exp, p, nx, ny, dx, dy, adt, gamma,
-                          eps, advection_flux, local_time_step, ke_method, Uex, enth, rhs, a, bX,
-                          bY, cX, cY, stream);
+                          eps, advection_flux, local_time_step, ke_method, enthalpy_method, Uex,
+                          enth, rhs, a, bX, bY, cX, cY, stream);
 }
 
 extern "C" int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
@@ -1015,13 +1047,13 @@ extern "C" int bx_imex_rk_pressure(const double *U_exp_new, const double *U_exp_
     if (!U_exp_new || !U_exp_old || !p || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0)
         || !adv_ok(advection_flux) || (const double *)p == U_exp_new
         || (const double *)p == U_exp_old
-        || (ke_method != BX_IMEX_KE_EX && ke_method != BX_IMEX_KE_SPLIT
-            && ke_method != BX_IMEX_KE_FI))
+        || ke_method < BX_IMEX_KE_EX || ke_method > BX_IMEX_KE_CONSERVATIVE_KINETIC)
         return bx_helm_fail(BX_ERR_INVALID, "bx_imex_rk_pressure: bad argument");
     PeekGeom g{ nx, ny, (long long)nx * ny };
-    if (ke_method == BX_IMEX_KE_FI)
+    if (ke_method == BX_IMEX_KE_FI || ke_method == BX_IMEX_KE_CONSERVATIVE_KINETIC)
     {
-        // FI: the new explicit state's own pressure (IMEX_O1.cpp:322-325)
+        // FI and the other methods: the new explicit state's own pressure
+        // (IMEX_O1.cpp:322-325, 385-387)
         hipLaunchKernelGGL(k_peek_pressure, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0,
                            (hipStream_t)stream, U_exp_new, p, g.n, gamma, eps);
         PEEK_CHECK(hipGetLastError());

@@ -247,13 +247,23 @@ class ImexRK:
     arrays cX, cY) for HelmholtzSolver.set_coeffs.  "fi" is the
     non-linearised form (imex.linearise = false): each stage solves
     picard_max_iter times (imex.picard_max_iter), calling picard(p) after
-    every solve, which updates the system in place."""
+    every solve, which updates the system in place.  "conservative_kinetic"
+    takes the kinetic energy of the explicit stage state.  enthalpy_method
+    (imex.enthalpy_method): reuse_pressure, conservative or conservative_ex
+    (the last two replace the stage's pressure by that of the advected
+    state before the solve)."""
 
     def __init__(self, nx, ny, dx, dy=None, gamma=1.4, eps=1.0, advection="rusanov",
-                 tableau="SP111", ke_method="ex", picard_max_iter=2):
+                 tableau="SP111", ke_method="ex", picard_max_iter=2,
+                 enthalpy_method="reuse_pressure"):
         self.tab = ButcherTableau(tableau)
         if ke_method not in _lib.BX_IMEX_KE:
-            raise ValueError("imex.ke_method: ex, split or fi, not %r" % (ke_method,))
+            raise ValueError("imex.ke_method: ex, split, conservative_kinetic or fi, not %r"
+                             % (ke_method,))
+        if enthalpy_method not in _lib.BX_IMEX_ENTHALPY:
+            raise ValueError("imex.enthalpy_method: reuse_pressure, conservative or "
+                             "conservative_ex, not %r" % (enthalpy_method,))
+        self.enth_method = _lib.BX_IMEX_ENTHALPY[enthalpy_method]
         self.ke = _lib.BX_IMEX_KE[ke_method]
         self.split = ke_method == "split"
         self.fi = ke_method == "fi"
@@ -306,7 +316,7 @@ class ImexRK:
         st = self.stage
         _chk(lib().bx_imex_stage_assemble(Ui.ptr, Ue.ptr, self.p.ptr, nx, ny, dx, dy, adt, self.gamma,
                                           self.eps, self.adv, int(tab.name == "SP111"), self.ke,
-                                          st.Uex.ptr, st.enth.ptr, st.rhs.ptr, st.a.ptr, st.bX.ptr,
+                                          self.enth_method, st.Uex.ptr, st.enth.ptr, st.rhs.ptr, st.a.ptr, st.bX.ptr,
                                           st.bY.ptr, self.cX.ptr if self.split else None,
                                           self.cY.ptr if self.split else None, None),
              "bx_imex_stage_assemble")

@@ -463,6 +463,17 @@ int bx_imex_peek_finish(const double *U, const double *Uex, const double *enth,
  * by bx_imex_picard_update (imex.picard_max_iter solves per stage), and a
  * later stage starts from the explicit state's own pressure. */
 #define BX_IMEX_KE_FI 2
+/* imex.ke_method = conservative_kinetic: the kinetic energy of the explicit
+ * stage state; a later stage starts from that state's own pressure. */
+#define BX_IMEX_KE_CONSERVATIVE_KINETIC 3
+/* imex.enthalpy_method (compute_enthalpy, Source/IMEX/IMEX_O1.cpp:177-229)
+ * of the linearised stages: reuse_pressure, h = p gamma/((gamma-1) rho_ex);
+ * conservative, h = gamma (E - ke)/rho_ex of the explicit stage state;
+ * conservative_ex, the same of the advected state U_ex.  The conservative
+ * methods replace p by the pressure of U_ex (the solve's initial guess). */
+#define BX_IMEX_ENTHALPY_REUSE_PRESSURE 0
+#define BX_IMEX_ENTHALPY_CONSERVATIVE 1
+#define BX_IMEX_ENTHALPY_CONSERVATIVE_EX 2
 /* The stage states (conservative_update into two outputs, Source/Fluxes/
  * Update.H:218-302): U_imp = U - sum_{s<nstage} dt_imp[s] div F_s and
  * U_exp likewise with dt_exp, dt_x[s] = A_x[stage][s] dt (host arrays of
@@ -477,7 +488,8 @@ int bx_imex_rk_combine(const double *U, const double *const *Fx,
  * without the local step) supplies the momentum, U_exp_new the density
  * and energy: p = (gamma-1)(E - 0.5 eps |m_ex|^2 / rho).  ke_method split
  * (SK): p = (gamma-1)(E_new - 0.5 eps (m_old . m_new) / rho_old).
- * BX_IMEX_KE_FI: the pressure of U_exp_new, (gamma-1)(E - ke). */
+ * BX_IMEX_KE_FI and BX_IMEX_KE_CONSERVATIVE_KINETIC: the pressure of
+ * U_exp_new, (gamma-1)(E - ke). */
 int bx_imex_rk_pressure(const double *U_exp_new, const double *U_exp_old, int nx,
                         int ny, double dx, double dy, double adt, double gamma,
                         double eps, int advection_flux, int ke_method, double *p,
@@ -499,13 +511,15 @@ int bx_imex_picard_update(const double *Uex, const double *p, int nx, int ny,
  * this one with U_imp = U_exp = U, local_time_step = 1, ke_method ex;
  * operator scalars alpha = 1, beta = adt*adt, gamma_c = 0 (ex) or
  * -0.5 eps adt (split).  cX, cY (nx*ny each, written for split, may be
- * null for ex): the velocity coefficients of bx_helm_set_coeffs. */
-int bx_imex_stage_assemble(const double *U_imp, const double *U_exp, const double *p,
+ * null otherwise): the velocity coefficients of bx_helm_set_coeffs.  p is
+ * written by the conservative enthalpy methods (ignored for FI). */
+int bx_imex_stage_assemble(const double *U_imp, const double *U_exp, double *p,
                            int nx, int ny, double dx, double dy, double adt,
                            double gamma, double eps, int advection_flux,
-                           int local_time_step, int ke_method, double *Uex,
-                           double *enth, double *rhs, double *a, double *bX,
-                           double *bY, double *cX, double *cY, void *stream);
+                           int local_time_step, int ke_method, int enthalpy_method,
+                           double *Uex, double *enth, double *rhs, double *a,
+                           double *bX, double *bY, double *cX, double *cY,
+                           void *stream);
 /* The stage's fluxes after the solve (update_momentum_flux /
  * update_energy_flux, IMEX_O1.cpp:485-566): the advective fluxes of U_exp
  * corrected with the solved pressure p_new, into Fx, Fy. */

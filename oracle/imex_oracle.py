@@ -159,14 +159,20 @@ class PeekStage:
     solve the flux correction; for SP111 also the conservative update.
     advection: "rusanov" (advance_rusanov_adv) or "muscl_rusanov"."""
 
-    def __init__(self, gamma=1.4, eps=1.0, advection="rusanov", ke_method="ex"):
-        if ke_method not in ("ex", "split", "fi"):
-            raise ValueError("ke_method: ex, split or fi (not linearised)")
+    def __init__(self, gamma=1.4, eps=1.0, advection="rusanov", ke_method="ex",
+                 enthalpy_method="reuse_pressure"):
+        if ke_method not in ("ex", "split", "conservative_kinetic", "fi"):
+            raise ValueError("ke_method: ex, split, conservative_kinetic or fi (not linearised)")
+        if enthalpy_method not in ("reuse_pressure", "conservative", "conservative_ex"):
+            raise ValueError("enthalpy_method: reuse_pressure, conservative or conservative_ex")
         self.gamma = gamma
         self.eps = eps
         self.muscl = advection == "muscl_rusanov"
         self.split = ke_method == "split"
         self.fi = ke_method == "fi"
+        self.ke_method = ke_method
+        # compute_enthalpy (IMEX_O1.cpp:177-229): FI always from the pressure
+        self.enthalpy = "reuse_pressure" if self.fi else enthalpy_method
 
     def assemble(self, U, p, dt, dx, dy=None, U_exp=None, local=True):
         """compute_flux_imex_o1 up to the solve (IMEX_O1.cpp:64-125): the
@@ -180,7 +186,15 @@ class PeekStage:
         g, eps = self.gamma, self.eps
         Uexp = U if U_exp is None else U_exp
         Uex, Fx, Fy = advect(Uexp, U, dt, dx, dy, eps, self.muscl, local)
-        enth = p * g / ((g - 1) * Uex[0])
+        if self.enthalpy == "reuse_pressure":
+            enth = p * g / ((g - 1) * Uex[0])
+        else:
+            # conservative: vol_enthalpy(U_exp) / rho_ex; conservative_ex:
+            # vol_enthalpy(U_ex) / rho_ex (euler_K.H:41-50: gamma (E - ke));
+            # both replace the pressure by that of U_ex (the solve's guess)
+            src = Uexp if self.enthalpy == "conservative" else Uex
+            enth = g * (src[3] - kinetic_energy(src, eps)) / Uex[0]
+            p = pressure_from_state(Uex, g, eps)
         if self.split:
             # compute_ke, split: 0.5 eps (m_in . m_ex) / rho_in, in = U_exp
             ke = 0.5 * eps * (Uexp[1] * Uex[1] + Uexp[2] * Uex[2]) / Uexp[0]
@@ -190,6 +204,10 @@ class PeekStage:
             # compute_ke, not linearised: the kinetic energy of the implicit
             # stage state (statenew = statein_imp, IMEX_O1.cpp:108-109)
             ke = kinetic_energy(U, eps)
+            cX = cY = None
+            gamma_c = 0.0
+        elif self.ke_method == "conservative_kinetic":
+            ke = kinetic_energy(Uexp, eps)   # compute_ke: of statein = U_exp
             cX = cY = None
             gamma_c = 0.0
         else:
@@ -213,7 +231,7 @@ class PeekStage:
         bY[:ny] = hy
         bY[ny] = hy[0]
         a = np.full_like(p, eps / (g - 1))
-        return dict(Uex=Uex, enth=enth, rhs=rhs, a=a, bX=bX, bY=bY,
+        return dict(Uex=Uex, enth=enth, rhs=rhs, a=a, bX=bX, bY=bY, ke=ke, p=p,
                     alpha=1.0, beta=dt * dt, gamma_c=gamma_c, cX=cX, cY=cY, Fx=Fx, Fy=Fy)
 
     def picard_update(self, st, p, dt, dx, dy=None):
@@ -369,7 +387,8 @@ class ButcherTableau:
 
 
 def advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0, advection="rusanov",
-                    record=None, ke_method="ex", picard_max_iter=2):
+                    record=None, ke_method="ex", picard_max_iter=2,
+                    enthalpy_method="reuse_pressure"):
     """advance_imex_rk (IMEX_RK.cpp:22-96) on a 2D periodic level.  solve(sy,
     p0) returns the stage's pressure (sy: the PeekStage.assemble system,
     p0: the initial guess).  Returns (U_new, p) with p the last stage's
@@ -380,7 +399,7 @@ def advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0, advect
     initial and solved pressure and its corrected fluxes (Fx, Fy; the
     advective ones as Fx_adv, Fy_adv)."""
     tab = ButcherTableau(tableau) if isinstance(tableau, str) else tableau
-    st = PeekStage(gamma, eps, advection, ke_method)
+    st = PeekStage(gamma, eps, advection, ke_method, enthalpy_method)
     Fxs, Fys, exps = [], [], []
     for s in range(tab.n_imp):
         Uimp, Uexp = stage_states(U, Fxs, Fys, tab.imp_row(s, dt)[:s], tab.exp_row(s, dt)[:s],
@@ -390,9 +409,12 @@ def advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0, advect
             p = pressure_from_state(Uexp, gamma, eps)   # compute_pressure, FI
         elif s > 0 and st.split:
             p = pressure_sk(Uexp, exps[-1], gamma, eps)
-        elif s > 0:
+        elif s > 0 and ke_method == "ex":
             p = pressure_ek(Uexp, exps[-1], adt, dx, dy, gamma, eps, st.muscl)
+        elif s > 0:
+            p = pressure_from_state(Uexp, gamma, eps)    # any other method
         sy = st.assemble(Uimp, p, adt, dx, dy, U_exp=Uexp, local=tab.name == "SP111")
+        p = sy["p"]      # the conservative enthalpy methods replace it
         p_in = p
         if st.fi:
             for _ in range(picard_max_iter):

@@ -156,7 +156,8 @@ def test_peek_assemble_rejects_aliased_outputs():
         dp.free()
 
 
-def _rk_parity(U, p, dt, dx, dy, gamma, eps, tableau, advection, tag, ke_method="ex"):
+def _rk_parity(U, p, dt, dx, dy, gamma, eps, tableau, advection, tag, ke_method="ex",
+               enthalpy_method="reuse_pressure"):
     """One IMEX RK step on the GPU (ImexRK, through the C ABI) and in the
     oracle (advance_imex_rk), with the stand-in solve p_new = pressure of
     the stage's explicit state on both sides (bx_imex_pressure is
@@ -167,8 +168,10 @@ def _rk_parity(U, p, dt, dx, dy, gamma, eps, tableau, advection, tag, ke_method=
     rec = []
     U_ref, p_ref = io.advance_imex_rk(U, p, dt, dx, dy, tableau,
                                       lambda sy, p0: io.pressure_from_state(sy["Uex"], gamma, eps),
-                                      gamma, eps, advection, record=rec, ke_method=ke_method)
-    rk = imex.ImexRK(nx, ny, dx, dy, gamma, eps, advection, tableau, ke_method)
+                                      gamma, eps, advection, record=rec, ke_method=ke_method,
+                                      enthalpy_method=enthalpy_method)
+    rk = imex.ImexRK(nx, ny, dx, dy, gamma, eps, advection, tableau, ke_method,
+                     enthalpy_method=enthalpy_method)
     dU, dp, dout, dpn = dev(U), dev(p), DeviceArray(U.shape), DeviceArray(p.shape)
     try:
         seen = []
@@ -337,8 +340,11 @@ def test_imex_rk_rejects_bad_arguments():
         sc = [DeviceArray((n, n)) for _ in range(6)]
         try:
             rc = L.bx_imex_stage_assemble(dU.ptr, dU.ptr, a1.ptr, n, n, 0.1, 0.1, 0.01, 1.4, 1.0, 0, 0,
-                                          1, a2.ptr, *[d.ptr for d in sc[:5]], None, None, None)
+                                          1, 0, a2.ptr, *[d.ptr for d in sc[:5]], None, None, None)
             assert rc != 0    # split without velocity outputs
+            rc = L.bx_imex_stage_assemble(dU.ptr, dU.ptr, a1.ptr, n, n, 0.1, 0.1, 0.01, 1.4, 1.0, 0, 0,
+                                          0, 5, a2.ptr, *[d.ptr for d in sc[:5]], None, None, None)
+            assert rc != 0    # unknown enthalpy_method
         finally:
             for d in sc:
                 d.free()
@@ -481,3 +487,18 @@ def test_imex_rk_fi_step_matches_oracle(tableau, advection):
             rk.free()
             for d in (dU, dp, dout, dpn):
                 d.free()
+
+
+@pytest.mark.parametrize("enthalpy_method", ["conservative", "conservative_ex"])
+@pytest.mark.parametrize("ke_method", ["ex", "split", "conservative_kinetic"])
+def test_imex_rk_enthalpy_methods_match_oracle(ke_method, enthalpy_method):
+    """The conservative enthalpy methods (EEEK / EESK inputs) and the
+    conservative_kinetic energy, SSP222 and SASSP322 with MUSCL advection:
+    every stage's system, initial pressure and the step bit-exact."""
+    gamma, eps = 1.4, 0.7
+    for tableau in ("SSP222", "SASSP322"):
+        U = random_state(20, 48, gamma, eps, seed=77)
+        p = io.pressure_from_state(U, gamma, eps) * 1.01
+        _rk_parity(U, p, 0.009, 1.0 / 48, 0.6 / 20, gamma, eps, tableau, "muscl_rusanov",
+                   "%s/%s" % (ke_method, enthalpy_method), ke_method=ke_method,
+                   enthalpy_method=enthalpy_method)

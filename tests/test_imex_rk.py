@@ -549,3 +549,62 @@ def test_fi_conserves_and_keeps_uniform_flow(tableau):
     scale = np.abs(Ug).sum(axis=(1, 2))
     assert np.all(np.abs(Un.sum(axis=(1, 2)) - Ug.sum(axis=(1, 2))) <= 1e-13 * scale)
     assert np.max(np.abs(pn - pg)) < 1e-4 * pg.max()
+
+
+@pytest.mark.parametrize("enthalpy_method", ["reuse_pressure", "conservative", "conservative_ex"])
+@pytest.mark.parametrize("ke_method", ["ex", "split", "conservative_kinetic"])
+def test_linearised_energy_identity_for_every_method(ke_method, enthalpy_method):
+    """The identity behind Tests/IMEX_O1_01.cpp's EnergyMatch tests, for
+    every linearised kinetic-energy and enthalpy method: the new energy
+    equals p/(gamma-1) plus the kinetic energy the source used (the energy
+    flux correction and the pressure equation share the enthalpy), to the
+    solve's precision."""
+    n = 24
+    gamma, eps, dt = 1.6, 0.5, 0.01
+    rng = np.random.default_rng(6)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) + 0.05 * rng.random((n, n))
+    u = 0.4 * np.sin(2 * np.pi * Y)
+    v = -0.3 * np.cos(2 * np.pi * X)
+    p = 1.0 + 0.2 * np.cos(2 * np.pi * (X + Y))
+    U = state_from_primv(rho, u, v, p, gamma, eps)
+    dx = 1.0 / n
+    rec = []
+    U1, pn = io.advance_imex_rk(U, p, dt, dx, dx, "SP111", direct_solve(dx, dx), gamma, eps,
+                                "rusanov", record=rec, ke_method=ke_method,
+                                enthalpy_method=enthalpy_method)
+    r = rec[0]
+    if ke_method == "split":
+        ke_new = 0.5 * eps * (U[1] * U1[1] + U[2] * U1[2]) / U[0]
+    else:
+        ke_new = r["ke"]
+    expected = pn / (gamma - 1) + ke_new
+    assert np.max(np.abs(U1[3] - expected) / np.abs(expected)) <= 1e-12
+
+
+def test_conservative_enthalpy_per_cell():
+    """compute_enthalpy, conservative and conservative_ex (IMEX_O1.cpp:
+    211-228): gamma (E - ke)/rho_ex of the explicit stage state or of the
+    advected state, and the pressure replaced by that of the advected
+    state; conservative_kinetic takes the kinetic energy of the explicit
+    stage state -- per cell in plain Python."""
+    gamma, eps = 1.4, 0.8
+    rng = np.random.default_rng(14)
+    ny, nx = 3, 4
+    U = state_from_primv(1 + 0.2 * rng.random((ny, nx)), rng.uniform(-1, 1, (ny, nx)),
+                         rng.uniform(-1, 1, (ny, nx)), 2 + rng.random((ny, nx)), gamma, eps)
+    p = 2 + rng.random((ny, nx))
+    for meth in ("conservative", "conservative_ex"):
+        st = io.PeekStage(gamma, eps, "rusanov", "conservative_kinetic", meth)
+        sy = st.assemble(U, p, 0.01, 1.0 / nx)
+        ex = sy["Uex"]
+        for j in range(ny):
+            for i in range(nx):
+                src = U if meth == "conservative" else ex
+                ke_s = eps * (src[1, j, i] ** 2 + src[2, j, i] ** 2) / (2 * src[0, j, i])
+                assert sy["enth"][j, i] == gamma * (src[3, j, i] - ke_s) / ex[0, j, i]
+                ke_x = eps * (ex[1, j, i] ** 2 + ex[2, j, i] ** 2) / (2 * ex[0, j, i])
+                assert sy["p"][j, i] == (gamma - 1) * (ex[3, j, i] - ke_x)
+                ke_u = eps * (U[1, j, i] ** 2 + U[2, j, i] ** 2) / (2 * U[0, j, i])
+                assert sy["ke"][j, i] == ke_u

@@ -587,19 +587,36 @@ __global__ __launch_bounds__(256) void k_peek_pressure(const double *__restrict_
     p[c] = (adia - 1) * (U[c + 3 * n] - peek_ke(U[c], U[c + n], U[c + 2 * n], eps));
 }
 
-// max over cells of (|m_x|/dx + |m_y|/dy) / rho (max_dx_scaled_speed): the
-// values are non-negative, so their bit patterns order like the values; a
-// NaN (all exponent bits set) orders above every number and is propagated.
+// Maximum over cells of a speed, as ParReduce with ReduceOpMax leaves it
+// (max(d, s) = (d < s) ? s : d from the lowest value skips NaN): only
+// positive speeds enter, whose bit patterns order like their values.
+//   advective (max_dx_scaled_speed, Euler.cpp:107-129):
+//     (|m_x|/dx + |m_y|/dy) / rho
+//   ACOUSTIC (max_wave_speed, Euler.cpp:47-74):
+//     sqrt(gamma p / (rho eps)) + sqrt(u^2 + v^2)
+template <bool ACOUSTIC>
 __global__ __launch_bounds__(256) void k_peek_speed(const double *__restrict__ U, long long n,
-                                                    double dx, double dy,
+                                                    double dx, double dy, double adia,
+                                                    double eps,
                                                     unsigned long long *__restrict__ out)
 {
     unsigned long long m = 0ULL;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
          c += (long long)gridDim.x * blockDim.x)
     {
-        const double s = (fabs(U[c + n]) / dx + fabs(U[c + 2 * n]) / dy) / U[c];
-        const unsigned long long b = (unsigned long long)__double_as_longlong(s);
+        double s;
+        if (ACOUSTIC)
+        {
+            const double r   = U[c];
+            const double u   = U[c + n] / r;
+            const double v   = U[c + 2 * n] / r;
+            const double usq = u * u + v * v;
+            const double p   = (adia - 1) * (U[c + 3 * n] - 0.5 * eps * r * usq);
+            s                = sqrt(adia * p / (r * eps)) + sqrt(u * u + v * v);
+        }
+        else
+            s = (fabs(U[c + n]) / dx + fabs(U[c + 2 * n]) / dy) / U[c];
+        const unsigned long long b = s > 0 ? (unsigned long long)__double_as_longlong(s) : 0ULL;
         m = b > m ? b : m;
     }
     // block maximum first: one atomic per block (per-wave atomics on one
@@ -809,7 +826,8 @@ extern "C" int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, 
     hipLaunchKernelGGL(k_peek_zero_u64, dim3(1), dim3(1), 0, s, d);
     const long long n    = (long long)nx * ny;
     const int       grid = (int)std::min<long long>((n + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_peek_speed, dim3(grid), dim3(256), 0, s, U, n, dx, dy, d);
+    hipLaunchKernelGGL(k_peek_speed<false>, dim3(grid), dim3(256), 0, s, U, n, dx, dy, 0.0, 0.0,
+                       d);
     PEEK_CHECK(hipGetLastError());
     unsigned long long bits = 0;
     PEEK_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, s));
@@ -1166,5 +1184,37 @@ extern "C" int bx_imex_picard_update(const double *Uex, const double *p, int nx,
                        Uex, p, enth, rhs, bX, bY, g, gamma, eps, hdtdx, hdtdy, hdtdx / eps,
                        hdtdy / eps);
     PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+extern "C" int bx_imex_acoustic_dt(const double *U, int nx, int ny, double dx, double dy,
+                                   double gamma, double eps, double cfl, double *dt,
+                                   void *stream)
+{
+    if (!U || !dt || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0))
+        return bx_helm_fail(BX_ERR_INVALID, "bx_imex_acoustic_dt: bad argument");
+    hipStream_t         s = (hipStream_t)stream;
+    unsigned long long *d = nullptr;
+    int                 rc;
+    if ((rc = peek_acc("bx_imex_acoustic_dt", ACC_SPEED, &d)) != BX_OK)
+        return rc;
+    hipLaunchKernelGGL(k_peek_zero_u64, dim3(1), dim3(1), 0, s, d);
+    const long long n    = (long long)nx * ny;
+    const int       grid = (int)std::min<long long>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_peek_speed<true>, dim3(grid), dim3(256), 0, s, U, n, dx, dy, gamma, eps,
+                       d);
+    PEEK_CHECK(hipGetLastError());
+    unsigned long long bits = 0;
+    PEEK_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, s));
+    PEEK_CHECK(hipStreamSynchronize(s));
+    double smax;
+    memcpy(&smax, &bits, sizeof(smax));
+    // ExplicitTimeStepDimSplit (FluidSystem.cpp:200-205), then dt *= cfl
+    double dt_est = 1.0e20;
+    const double q0 = dx / smax, q1 = dy / smax;
+    dt_est = (q0 < dt_est) ? q0 : dt_est;
+    dt_est = (q1 < dt_est) ? q1 : dt_est;
+    dt_est *= cfl;
+    *dt = dt_est;
     return BX_OK;
 }

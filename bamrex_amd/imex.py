@@ -144,6 +144,13 @@ class PeekStage:
                                         ctypes.byref(out), None), "bx_imex_advective_dt")
         return out.value
 
+    def acoustic_dt(self, U, cfl=0.45):
+        """The acoustic time step (before imex.acoustic_timestep_end_time)."""
+        out = ctypes.c_double()
+        _chk(lib().bx_imex_acoustic_dt(U.ptr, self.nx, self.ny, self.dx, self.dy, self.gamma, self.eps,
+                                       cfl, ctypes.byref(out), None), "bx_imex_acoustic_dt")
+        return out.value
+
     def assemble(self, U, p, dt):
         """Explicit update and pressure system from the state U and the
         previous pressure p; returns the system (device arrays and the
@@ -633,7 +640,7 @@ class GreshoWorkload:
     multigrid solution, so every method solves the same sequence)."""
 
     def __init__(self, n=1024, mach=0.01, gamma=1.4, eps=1.0, cfl=0.45, advection="rusanov",
-                 tableau="SP111", ke_method="ex"):
+                 tableau="SP111", ke_method="ex", acoustic_timestep_end_time=-1.0):
         U, self.dx = gresho_state(n, mach, gamma, eps)
         self.n = n
         self.U = dev(U)
@@ -645,9 +652,14 @@ class GreshoWorkload:
         self.stage.pressure(self.U, self.p)
         self.cfl = cfl
         self.in_step = False
+        self.time = 0.0
+        self.acoustic_end = acoustic_timestep_end_time
 
     def step_dt(self):
-        """The advective dt of the next step (estTimeStep)."""
+        """The dt of the next step (estTimeStep, AmrLevelAdv.cpp:424-437):
+        acoustic before imex.acoustic_timestep_end_time, then advective."""
+        if self.time < self.acoustic_end:
+            return self.stage.acoustic_dt(self.U, self.cfl)
         return self.stage.advective_dt(self.U, self.cfl)
 
     def next_system(self, dt=None):
@@ -667,6 +679,7 @@ class GreshoWorkload:
         if done:
             self.U, self.U_next = self.U_next, self.U
             self.in_step = False
+            self.time += self.rk.dt
         lib().bx_stream_sync(None)
         return done
 

@@ -413,6 +413,15 @@ int bx_imex_pressure(const double *U, int nx, int ny, double gamma, double eps,
  * Synchronous; writes the host double *dt. */
 int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, double dy,
                          double cfl, double *dt, void *stream);
+/* Acoustic time step of the IMEX schemes before imex.acoustic_timestep_end_time
+ * (estTimeStep, AmrLevelAdv.cpp:424-430 and 466): ExplicitTimeStepDimSplit
+ * (Source/System/FluidSystem.cpp:195-206), min(dx, dy) / max over cells of
+ * sqrt(gamma p/(rho eps)) + |u| (max_wave_speed, Euler.cpp:47-74), times
+ * cfl.  Both reductions skip NaN speeds as AMReX's ReduceOpMax does.
+ * Synchronous; writes the host double *dt. */
+int bx_imex_acoustic_dt(const double *U, int nx, int ny, double dx, double dy,
+                        double gamma, double eps, double cfl, double *dt,
+                        void *stream);
 /* First half of compute_flux_imex_o1 (Source/IMEX/IMEX_O1.cpp:18-131):
  * Rusanov advective update U -> Uex (advance_rusanov_adv,
  * Source/IMEX/IMEX_Fluxes.cpp:10-118, or its MUSCL form), enthalpy p gamma/((gamma-1) rho_ex)

@@ -14,6 +14,7 @@ IMEX Runge-Kutta tableaux (advance_imex_rk).
   kinetic_energy / pressure / adv_flux   Source/IMEX_K/euler_K.H:8-65
   max_dx_scaled_speed                    Source/System/Euler/Euler.cpp:107-129
   estTimeStep (advective, cfl)           Source/AmrLevelAdv.cpp:431-436, 466
+  ExplicitTimeStepDimSplit (acoustic)    Source/System/FluidSystem.cpp:195-206
   advance_rusanov_adv                    Source/IMEX/IMEX_Fluxes.cpp:10-118
   advance_MUSCL_rusanov_adv (SP111)      Source/IMEX/IMEX_Fluxes.cpp:120-324
   hdtdx = 0.5 dt idx, hdtdxe = hdtdx/eps Source/IMEX/IMEX_O1.cpp:66-70
@@ -67,11 +68,39 @@ def pressure_from_state(U, gamma, eps):
     return (gamma - 1) * (U[3] - kinetic_energy(U, eps))
 
 
+def _reduce_max(s):
+    """ParReduce with ReduceOpMax from the lowest value: d = max(d, s) =
+    (d < s) ? s : d skips NaN; a positive maximum is what remains (0 if no
+    value is positive, which then gives an infinite dt as in C)."""
+    pos = s[s > 0]
+    return float(pos.max()) if pos.size else 0.0
+
+
 def advective_dt(U, dx, dy=None, cfl=0.45):
     """cfl * (1 / max((|m_x|/dx + |m_y|/dy) / rho))."""
     dy = dx if dy is None else dy
-    s = np.max((np.abs(U[1]) / dx + np.abs(U[2]) / dy) / U[0])
-    dt = 1 / float(s)
+    s = _reduce_max((np.abs(U[1]) / dx + np.abs(U[2]) / dy) / U[0])
+    dt = 1 / s if s > 0 else math.inf
+    dt *= cfl
+    return dt
+
+
+def acoustic_dt(U, dx, dy, gamma, eps, cfl=0.45):
+    """ExplicitTimeStepDimSplit (FluidSystem.cpp:195-206) with max_wave_speed
+    (Euler.cpp:47-74): s = max over cells of sqrt(gamma p / (rho eps)) +
+    sqrt(u^2 + v^2); dt = min(1e20, dx/s, dy/s) (std::min in that order),
+    then dt *= cfl (AmrLevelAdv.cpp:466)."""
+    u = U[1] / U[0]
+    v = U[2] / U[0]
+    usq = u * u + v * v
+    p = (gamma - 1) * (U[3] - 0.5 * eps * U[0] * usq)
+    with np.errstate(invalid="ignore"):
+        s = np.sqrt(gamma * p / (U[0] * eps)) + np.sqrt(u * u + v * v)
+    smax = _reduce_max(s)
+    dt = 1.0e20
+    for h in (dx, dy):
+        q = h / smax if smax > 0 else math.inf
+        dt = q if q < dt else dt
     dt *= cfl
     return dt
 

@@ -502,3 +502,28 @@ def test_imex_rk_enthalpy_methods_match_oracle(ke_method, enthalpy_method):
         _rk_parity(U, p, 0.009, 1.0 / 48, 0.6 / 20, gamma, eps, tableau, "muscl_rusanov",
                    "%s/%s" % (ke_method, enthalpy_method), ke_method=ke_method,
                    enthalpy_method=enthalpy_method)
+
+
+def test_time_step_estimates_match_oracle():
+    """The acoustic and advective dt on the GPU equal the oracle's, also
+    with a NaN speed (negative pressure) and a negative one (negative
+    density) that the reduction passes over."""
+    gamma, eps = 1.4, 0.6
+    for ny, nx in ((37, 70), (1, 5)):
+        announce("imex dt estimates %dx%d" % (nx, ny))
+        U = random_state(ny, nx, gamma, eps, seed=nx)
+        st = PeekStage(nx, ny, 1.0 / nx, 0.7 / ny, gamma, eps)
+        try:
+            for case in ("plain", "bad"):
+                V = U.copy()
+                if case == "bad":
+                    V[3, 0, 2] = 0.0
+                    V[0, ny - 1, 3] = -V[0, ny - 1, 3]
+                dV = dev(V)
+                try:
+                    assert st.acoustic_dt(dV, 0.45) == io.acoustic_dt(V, st.dx, st.dy, gamma, eps, 0.45)
+                    assert st.advective_dt(dV, 0.45) == io.advective_dt(V, st.dx, st.dy, 0.45)
+                finally:
+                    dV.free()
+        finally:
+            st.free()

@@ -608,3 +608,35 @@ def test_conservative_enthalpy_per_cell():
                 assert sy["p"][j, i] == (gamma - 1) * (ex[3, j, i] - ke_x)
                 ke_u = eps * (U[1, j, i] ** 2 + U[2, j, i] ** 2) / (2 * U[0, j, i])
                 assert sy["ke"][j, i] == ke_u
+
+
+def test_time_step_estimates_brute_force_and_nan_skipping():
+    """estTimeStep of the IMEX schemes: the acoustic estimate
+    (ExplicitTimeStepDimSplit with max_wave_speed) and the advective one,
+    cell by cell in plain Python with dx != dy; a NaN speed (negative
+    pressure) or a negative one (negative density) is passed over, as
+    ReduceOpMax's max(d, s) = (d < s) ? s : d passes over it."""
+    gamma, eps, cfl = 1.4, 0.6, 0.45
+    rng = np.random.default_rng(21)
+    ny, nx = 5, 6
+    U = state_from_primv(1 + 0.3 * rng.random((ny, nx)), rng.uniform(-1, 1, (ny, nx)),
+                         rng.uniform(-1, 1, (ny, nx)), 1 + rng.random((ny, nx)), gamma, eps)
+    dx, dy = 1.0 / nx, 0.7 / ny
+    sa, sv = [], []
+    for j in range(ny):
+        for i in range(nx):
+            r, mx, my, E = U[:, j, i]
+            u, v = mx / r, my / r
+            p = (gamma - 1) * (E - 0.5 * eps * r * (u * u + v * v))
+            sa.append(math.sqrt(gamma * p / (r * eps)) + math.sqrt(u * u + v * v))
+            sv.append((abs(mx) / dx + abs(my) / dy) / r)
+    dt_a = min(1e20, dx / max(sa), dy / max(sa)) * cfl
+    assert io.acoustic_dt(U, dx, dy, gamma, eps, cfl) == dt_a
+    assert io.advective_dt(U, dx, dy, cfl) == (1 / max(sv)) * cfl
+    bad = U.copy()
+    bad[3, 2, 3] = 0.0          # negative pressure: NaN sound speed
+    bad[0, 4, 1] = -bad[0, 4, 1]  # negative density: negative advective speed
+    sa2 = [x for k, x in enumerate(sa) if k != 2 * nx + 3]
+    assert io.acoustic_dt(bad, dx, dy, gamma, eps, cfl) >= min(1e20, dx / max(sa2), dy / max(sa2)) * cfl
+    sv2 = [x for k, x in enumerate(sv) if k != 4 * nx + 1]
+    assert io.advective_dt(bad, dx, dy, cfl) == (1 / max(sv2)) * cfl

@@ -370,14 +370,21 @@ class ImexRK:
         self._built = False
         return last
 
-    def advance(self, U, p, dt, U_out, solve):
+    def advance(self, U, p, dt, U_out, solve, residual=None, res_tol=1e-12):
         """One whole step; solve(sy) returns the stage's solved pressure (a
-        device array)."""
+        device array).  FI with residual (imex.compute_residual):
+        residual(sy) returns ||b - A p|| / ||b|| of the current pressure
+        sy["p_prev"] for the current system, and a stage's Picard loop ends
+        before a solve once it is below res_tol (imex.res_tol;
+        picard_iterate, IMEX_O1.cpp:736-751)."""
         self.begin(U, p, dt)
         while True:
             sy = self.system()
             if self.fi:
+                p_new = self.p
                 for _ in range(self.picard_max_iter):
+                    if residual is not None and abs(residual(sy)) < res_tol:
+                        break
                     p_new = solve(sy)
                     if p_new.ptr.value != self.p.ptr.value:
                         self.p.copy_from(p_new)   # the next solve starts from it

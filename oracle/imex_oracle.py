@@ -417,7 +417,7 @@ class ButcherTableau:
 
 def advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0, advection="rusanov",
                     record=None, ke_method="ex", picard_max_iter=2,
-                    enthalpy_method="reuse_pressure"):
+                    enthalpy_method="reuse_pressure", residual=None, res_tol=1e-12):
     """advance_imex_rk (IMEX_RK.cpp:22-96) on a 2D periodic level.  solve(sy,
     p0) returns the stage's pressure (sy: the PeekStage.assemble system,
     p0: the initial guess).  Returns (U_new, p) with p the last stage's
@@ -447,6 +447,10 @@ def advance_imex_rk(U, p, dt, dx, dy, tableau, solve, gamma=1.4, eps=1.0, advect
         p_in = p
         if st.fi:
             for _ in range(picard_max_iter):
+                # imex.compute_residual (IMEX_O1.cpp:736-751): residual(sy, p)
+                # is ||b - A p|| / ||b|| of the current system
+                if residual is not None and abs(residual(sy, p)) < res_tol:
+                    break
                 p = solve(sy, p)
                 sy = st.picard_update(sy, p, adt, dx, dy)
         else:

@@ -527,3 +527,37 @@ def test_time_step_estimates_match_oracle():
                     dV.free()
         finally:
             st.free()
+
+
+def test_imex_rk_fi_residual_check_ends_picard_loops():
+    """imex.compute_residual: a residual below res_tol before a solve ends
+    the stage's Picard loop (here after the first solve of each stage, and
+    before any solve in the second stage of a second run); the GPU step
+    follows the oracle's bit for bit."""
+    gamma, eps = 1.4, 0.7
+    ny, nx = 20, 48
+    U = random_state(ny, nx, gamma, eps, seed=91)
+    p = io.pressure_from_state(U, gamma, eps)
+    dx, dy, dt = 1.0 / nx, 0.6 / ny, 0.009
+    for pattern in ((1.0, 0.0, 1.0, 0.0), (1.0, 0.0, 0.0)):
+        announce("imex rk fi residual pattern %s" % (pattern,))
+        seq = iter(pattern)
+        U_ref, p_ref = io.advance_imex_rk(
+            U, p, dt, dx, dy, "SSP222", lambda sy, p0: 1.01 * io.pressure_from_state(sy["Uex"], gamma, eps),
+            gamma, eps, "muscl_rusanov", ke_method="fi", picard_max_iter=3,
+            residual=lambda sy, p0: next(seq))
+        rk = imex.ImexRK(nx, ny, dx, dy, gamma, eps, "muscl_rusanov", "SSP222", "fi", picard_max_iter=3)
+        dU, dp, dout, dpn = dev(U), dev(p), DeviceArray(U.shape), DeviceArray(p.shape)
+        seq2 = iter(pattern)
+        try:
+            def solve(sy):
+                rk.stage.pressure(sy["Uex"], dpn)
+                dpn.upload(1.01 * dpn.download())
+                return dpn
+            rk.advance(dU, dp, dt, dout, solve, residual=lambda sy: next(seq2))
+            assert np.array_equal(dout.download(), U_ref)
+            assert np.array_equal(dp.download(), p_ref)
+        finally:
+            rk.free()
+            for d in (dU, dp, dout, dpn):
+                d.free()

@@ -670,18 +670,34 @@ class GreshoWorkload:
         return self.stage.advective_dt(self.U, self.cfl)
 
     def next_system(self, dt=None):
-        """Assembles the next stage's system (starting a step with the
-        advective dt, or dt, when the last one is done); "p_prev" is the
-        solve's initial guess (device array)."""
+        """Assembles the next system to solve (starting a step with the
+        estimated dt, or dt, when the last one is done): a stage's, or for
+        FI the stage's system after the previous Picard update; "p_prev" is
+        the solve's initial guess (device array)."""
+        if getattr(self, "_picard_sy", None) is not None:
+            return self._picard_sy
         if not self.in_step:
             self.rk.begin(self.U, self.p, self.step_dt() if dt is None else dt)
             self.in_step = True
+        self._picard_it = 0
         return self.rk.system()
 
     def advance(self, sy, p_new):
-        """Completes the stage with the solved pressure p_new (DeviceArray);
-        after a step's last stage, U <- the new state.  Returns whether a
-        step ended."""
+        """Completes the system's solve with the solved pressure p_new
+        (DeviceArray): for FI with Picard iterations left, the Picard update
+        (the next next_system() returns the updated system); otherwise the
+        stage, and after a step's last stage U <- the new state.  Returns
+        whether a step ended."""
+        if self.rk.fi:
+            if p_new.ptr.value != self.p.ptr.value:
+                self.p.copy_from(p_new)
+            self.rk.picard(self.p)
+            self._picard_it += 1
+            if self._picard_it < self.rk.picard_max_iter:
+                self._picard_sy = sy
+                lib().bx_stream_sync(None)
+                return False
+            self._picard_sy = None
         done = self.rk.complete(p_new, self.U_next)
         if done:
             self.U, self.U_next = self.U_next, self.U

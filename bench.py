@@ -53,8 +53,9 @@ def main():
     ap.add_argument("--imex-advection", default="muscl_rusanov", choices=["rusanov", "muscl_rusanov"])
     ap.add_argument("--imex-tableau", default="SSP222",
                     choices=["SP111", "SSP222", "SASSP322", "SASSP332", "SSP433"])
-    ap.add_argument("--imex-ke", default="ex", choices=["ex", "split"],
-                    help="imex.ke_method: ex (PEEK) or split (PESK)")
+    ap.add_argument("--imex-ke", default="ex", choices=["ex", "split", "conservative_kinetic", "fi"],
+                    help="imex.ke_method: ex (PEEK), split (PESK), conservative_kinetic, or fi "
+                         "(not linearised, 2 Picard iterations per stage)")
     ap.add_argument("--imex-harvest-steps", type=int, default=8, help="pressure systems harvested")
     ap.add_argument("--imex-steps", type=int, default=4, help="time steps (one solve per stage)")
     ap.add_argument("--imex-train-steps", type=int, default=640)
@@ -412,7 +413,7 @@ def imex_leg(args):
     def set_coeffs(sy):
         solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"], gamma=sy["gamma_c"],
                           cX=sy["cX"], cY=sy["cY"])
-    nst = wl.rk.tab.n_imp
+    nst = wl.rk.tab.n_imp * (wl.rk.picard_max_iter if wl.rk.fi else 1)   # solves per step
     solver = imex.HelmholtzSolver(n, n, wl.dx)
     nf_max = 8 * args.imex_harvest_steps
     R = imex.DeviceArray((nf_max, n, n))
@@ -491,7 +492,7 @@ def imex_leg(args):
                 res[label]["fell_back"] += int(st["fell_back"])
                 if label == "mg":
                     p_mg = dx_
-            if k == args.imex_steps - 1 and sy["stage"] == nst - 1:
+            if k == args.imex_steps - 1 and sy["stage"] == wl.rk.tab.n_imp - 1:
                 # cold start (zero initial guess) on the last system: where a
                 # better preconditioner saves iterations
                 for label, meth, mode, graph in methods:
@@ -513,7 +514,8 @@ def imex_leg(args):
         for key, v in acc.items():
             stage_ms[key].append(v * 1e3)
     out = {"workload": "gresho-%s-%s-%s-%dx%d-M0.01-periodic" % (
-        "peek" if args.imex_ke == "ex" else "pesk", args.imex_tableau.lower(), args.imex_advection, n, n),
+        {"ex": "peek", "split": "pesk", "conservative_kinetic": "pe-ck", "fi": "fi"}[args.imex_ke],
+        args.imex_tableau.lower(), args.imex_advection, n, n),
            "tableau": args.imex_tableau, "solves_per_step": nst,
            "rtol": 1e-10,
            "harvest_fields": nf, "train_steps": args.imex_train_steps, "train_batch": args.imex_batch,

@@ -561,3 +561,39 @@ def test_imex_rk_fi_residual_check_ends_picard_loops():
             rk.free()
             for d in (dU, dp, dout, dpn):
                 d.free()
+
+
+def test_workload_fi_step_with_multigrid_solves():
+    """The benchmark's path for FI-MUSCL-SSP222 (GreshoWorkload stepping
+    through the Picard iterations, 128^2, plain multigrid to 1e-10): the
+    oracle, given the same four solved pressures, reaches the identical
+    state and pressure."""
+    n = 128
+    announce("workload fi ssp222 128")
+    wl = imex.GreshoWorkload(n, advection="muscl_rusanov", tableau="SSP222", ke_method="fi")
+    solver = imex.HelmholtzSolver(n, n, wl.dx)
+    U0, p0 = wl.U.download(), wl.p.download()
+    solved = []
+    try:
+        dt = wl.step_dt()
+        while True:
+            sy = wl.next_system(dt)
+            solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+            x = imex.dev(sy["p_prev"].download())
+            st = solver.solve(imex.dev(sy["rhs"].download()), x, "mg", rtol=1e-10, maxiter=100)
+            assert st["converged"]
+            solved.append(x.download())
+            done = wl.advance(sy, x)
+            x.free()
+            if done:
+                break
+        assert len(solved) == 4
+        it = iter(solved)
+        U_ref, p_ref = io.advance_imex_rk(U0, p0, dt, wl.dx, wl.dx, "SSP222", lambda sy, p: next(it),
+                                          1.4, 1.0, "muscl_rusanov", ke_method="fi", picard_max_iter=2)
+        assert np.array_equal(wl.U.download(), U_ref)
+        assert np.array_equal(wl.p.download(), p_ref)
+        assert wl.time == dt
+    finally:
+        solver.close()
+        wl.free()

@@ -471,11 +471,15 @@ class HelmholtzSolver:
         self._own = {}
         self.cnn = None
 
-    def set_coeffs(self, alpha, beta, a, bX, bY, gamma=0.0, cX=None, cY=None):
-        """Coefficients as host arrays or DeviceArrays (device arrays are
-        copied: the attached network keeps pointers to the solver's own)."""
+    def set_coeffs(self, alpha, beta, a, bX, bY, gamma=0.0, cX=None, cY=None, copy=True):
+        """Coefficients as host arrays or DeviceArrays.  Device arrays are
+        copied unless copy=False: then they are used in place and must keep
+        their values until the next set_coeffs (the attached network keeps
+        pointers to them)."""
         def own(name, x):
             cur = self._own.get(name)
+            if isinstance(x, DeviceArray) and not copy:
+                return x
             if isinstance(x, DeviceArray):
                 if cur is None or cur.nbytes != x.nbytes:
                     cur = self._own[name] = DeviceArray(x.shape)

@@ -411,8 +411,9 @@ def imex_leg(args):
                              ke_method=args.imex_ke)
 
     def set_coeffs(sy):
+        # the stage's arrays in place: they hold until the next system
         solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"], gamma=sy["gamma_c"],
-                          cX=sy["cX"], cY=sy["cY"])
+                          cX=sy["cX"], cY=sy["cY"], copy=False)
     nst = wl.rk.tab.n_imp * (wl.rk.picard_max_iter if wl.rk.fi else 1)   # solves per step
     solver = imex.HelmholtzSolver(n, n, wl.dx)
     nf_max = 8 * args.imex_harvest_steps

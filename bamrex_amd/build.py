@@ -73,7 +73,7 @@ def build(force=True, jobs=6, verbose=False, defines=(), lib_path=None, obj_dir=
           host=True):
     """Compile every HIP source for gfx950 and link libbamrex_hip.so.
 
-    force=True recompiles everything (the graft build check); force=False only
+    force=True recompiles everything (as __graft_entry__.build() asks); force=False only
     rebuilds stale objects."""
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found at %s" % HIPCC)

@@ -1106,7 +1106,7 @@ extern "C" int bx_set_tuning(bx_ctx *c, const bx_tuning *t)
         return set_error(BX_ERR_INVALID, "bx_set_tuning: null argument");
     auto blk_ok = [](int b) { return b >= 64 && b <= 256 && b % 64 == 0; };
     if (!blk_ok(t->block_x) || !blk_ok(t->block_t) || t->seg_k < 1
-        || t->seg_k > 4096 || t->grid_cap < 0)
+        || t->seg_k > BX_MAX_SEG_K || t->grid_cap < 0)
         return set_error(BX_ERR_INVALID, "bx_set_tuning: out of range");
     destroy_graphs(c);
     c->tune = *t;
@@ -1148,24 +1148,18 @@ static int time_sweep(bx_ctx *c, int dir, const double dx[3], double *U,
     return rc;
 }
 
-extern "C" int bx_autotune(bx_ctx *c, const double dx[3], double dt, double *U,
-                           double *U_tmp, void *stream, double *best_ms)
+// The search itself; on an error the caller restores the entry shape.
+static int autotune_search(bx_ctx *c, const double dx[3], double *U, double *U_tmp,
+                           hipStream_t s, double *best_ms)
 {
-    BX_REQUIRE_LEVEL(c);
-    if (!U || !U_tmp || U == U_tmp || !dx_ok(c, dx))
-        return set_error(BX_ERR_INVALID, "bx_autotune: bad argument");
-    hipStream_t s = (hipStream_t)stream;
-    int         rc;
-    if ((rc = do_fill_all(c, U, s)) != BX_OK)
-        return rc;
-    if ((rc = set_scalar(c->d_dt, dt, s)) != BX_OK)
-        return rc;
+    int       rc;
     bx_tuning best = c->tune;
     const int blocks[3] = { 64, 128, 256 };
     const int caps[4]   = { 0, 1, 2, 4 };
     // segment lengths: powers of two, and the median box length along the
     // transverse sweeps split into m equal parts (m = 1..8), which lets the
-    // item count divide evenly over 2, 3, ... resident waves per SIMD
+    // item count divide evenly over 2, 3, ... resident waves per SIMD;
+    // lengths above what bx_set_tuning accepts are skipped
     std::vector<int> ks = { 8, 16, 32, 64, 128 };
     if (c->D >= 2)
     {
@@ -1175,8 +1169,11 @@ extern "C" int bx_autotune(bx_ctx *c, const double dx[3], double dt, double *U,
         std::nth_element(len.begin(), len.begin() + len.size() / 2, len.end());
         const int L = len[len.size() / 2];
         for (int m = 1; m <= 8; ++m)
-            if ((L + m - 1) / m >= 4)
-                ks.push_back((L + m - 1) / m);
+        {
+            const int k = (L + m - 1) / m;
+            if (k >= 4 && k <= BX_MAX_SEG_K)
+                ks.push_back(k);
+        }
         std::sort(ks.begin(), ks.end());
         ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
     }
@@ -1195,15 +1192,14 @@ extern "C" int bx_autotune(bx_ctx *c, const double dx[3], double dt, double *U,
                 return rc;
             if (ms < bx_ms)
             {
-                bx_ms        = ms;
-                best.block_x = b;
+                bx_ms         = ms;
+                best.block_x  = b;
                 best.grid_cap = cap;
             }
         }
-    double t_ms = 0;
     if (c->D >= 2)
     {
-        t_ms = 1e30;
+        double t_ms = 1e30;
         for (int b : blocks)
             for (int k : ks)
             {
@@ -1235,7 +1231,29 @@ extern "C" int bx_autotune(bx_ctx *c, const double dx[3], double dt, double *U,
                 return rc;
             best_ms[d] = ms;
         }
-        (void)t_ms;
+    }
+    return BX_OK;
+}
+
+extern "C" int bx_autotune(bx_ctx *c, const double dx[3], double dt, double *U,
+                           double *U_tmp, void *stream, double *best_ms)
+{
+    BX_REQUIRE_LEVEL(c);
+    if (!U || !U_tmp || U == U_tmp || !dx_ok(c, dx))
+        return set_error(BX_ERR_INVALID, "bx_autotune: bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    int         rc;
+    if ((rc = do_fill_all(c, U, s)) != BX_OK)
+        return rc;
+    if ((rc = set_scalar(c->d_dt, dt, s)) != BX_OK)
+        return rc;
+    const bx_tuning entry = c->tune;
+    if ((rc = autotune_search(c, dx, U, U_tmp, s, best_ms)) != BX_OK)
+    {
+        // keep the error of the failed step, not that of the restore
+        const std::string msg = bx_last_error();
+        (void)bx_set_tuning(c, &entry);
+        return set_error(rc, msg);
     }
     return BX_OK;
 }

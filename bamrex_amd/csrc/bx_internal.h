@@ -4,6 +4,9 @@
 
 namespace bx {
 
+// Longest segment (cells per lane along y/z) a launch shape may use.
+constexpr int BX_MAX_SEG_K = 4096;
+
 // One box of a level as seen by the kernels.  The FAB of the box covers
 // grow(valid, ng) (ng only in the first D directions), component-major with
 // x fastest -- the amrex::Array4 layout:

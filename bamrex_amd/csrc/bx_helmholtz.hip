@@ -925,7 +925,10 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
         if (h->has_c && ((rc = halloc(&L.cX, nc)) || (rc = halloc(&L.cY, nc))))
             return rc;
         mg.lev.push_back(L);
-        if (nx % 2 || ny % 2 || nx / 2 < 8 || ny / 2 < 8)
+        // coarsen only to even extents: on a periodic level of odd extent
+        // the cells on either side of the seam have the same red/black
+        // colour, so one colour pass would update neighbours concurrently
+        if (nx % 4 || ny % 4 || nx / 2 < 8 || ny / 2 < 8)
             break;
         nx /= 2;
         ny /= 2;
@@ -1139,6 +1142,10 @@ extern "C" int bx_helm_create(int nx, int ny, double dx, double dy,
     if (!periodic)
         return herr(BX_ERR_UNSUPPORTED,
                     "bx_helm_create: only periodic levels are implemented");
+    // red/black Gauss-Seidel on a periodic level needs even extents (an odd
+    // extent makes the two cells across the seam the same colour)
+    if (nx % 2 || ny % 2)
+        return herr(BX_ERR_UNSUPPORTED, "bx_helm_create: nx and ny must be even");
     bx_helm *h = new bx_helm();
     h->nx = nx;
     h->ny = ny;

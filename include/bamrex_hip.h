@@ -343,7 +343,10 @@ typedef struct
 } bx_solve_stats;
 
 /* MLABecLaplacianGradT::define for one periodic level (MLALaplacianGrad.H:312-375).
- * nx, ny >= 8; non-periodic levels return BX_ERR_UNSUPPORTED. */
+ * nx, ny >= 8 and even (an odd periodic extent puts two cells of one red/black
+ * colour side by side across the seam); non-periodic levels and odd extents
+ * return BX_ERR_UNSUPPORTED.  Coarser multigrid levels halve both extents
+ * while the halves stay even and >= 8. */
 int  bx_helm_create(int nx, int ny, double dx, double dy, int periodic,
                     bx_helm **out);
 void bx_helm_destroy(bx_helm *h);

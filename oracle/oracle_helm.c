@@ -17,8 +17,9 @@
  *                    the AMReX documentation (AMReX is not part of the
  *                    reference tree): nu1 pre- and nu2 post- red/black sweeps,
  *                    residual restriction by cell averaging, piecewise-constant
- *                    prolongation, coarsening while both extents stay even and
- *                    >= 8 after halving, bottom = nu_bottom red/black sweep
+ *                    prolongation, coarsening while both halved extents are
+ *                    even and >= 8 (an odd periodic extent would give the two
+ *                    cells across the seam one colour), bottom = nu_bottom red/black sweep
  *                    pairs (16 by default).
  *                    Parity of these MLMG choices with AMReX itself is
  *                    unpinned (no AMReX build here); the pieces they use are
@@ -156,7 +157,7 @@ static void hier_build(Hier *H, int nx, int ny, double dx, double dy,
         L->x   = (double *)calloc(n, 8);
         L->rhs = (double *)calloc(n, 8);
         L->res = (double *)calloc(n, 8);
-        if (nx % 2 || ny % 2 || nx / 2 < 8 || ny / 2 < 8 || H->nlev == 24)
+        if (nx % 4 || ny % 4 || nx / 2 < 8 || ny / 2 < 8 || H->nlev == 24)
             break;
         nx /= 2;
         ny /= 2;

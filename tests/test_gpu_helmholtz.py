@@ -98,6 +98,37 @@ def test_mg_solve_bit_exact_iterate():
     s.close()
 
 
+@pytest.mark.parametrize("n", [100, 200])
+def test_vcycle_and_mg_solve_bit_exact_where_halving_reaches_odd(n):
+    """100 -> 50 and 200 -> 100 -> 50 stop before the odd 25 (the two cells
+    across a periodic seam of odd extent share a red/black colour, so one
+    colour pass would update neighbours concurrently); V-cycle and plain
+    multigrid iterate equal the sequential oracle bit for bit."""
+    announce("helm vcycle + mg solve %dx%d" % (n, n))
+    beta = 2e-4
+    s, (a, bX, bY), kw, rng = make_solver(n, n, 7, beta=beta)
+    s.set_options(precond_fp32=0)
+    r = rng.normal(size=(n, n))
+    dr, de = dev(r), DeviceArray((n, n))
+    s.precond(dr, de)
+    ref = ho.vcycle(r, a, bX, bY, 1.0, beta, 1 / n, 1 / n)
+    assert np.array_equal(de.download(), ref)
+    x0 = rng.normal(size=(n, n)) * 0.1
+    dx_ = dev(x0)
+    st = s.solve(dr, dx_, "mg", rtol=1e-10, maxiter=300)
+    xr, its, hist = ho.mg_solve(r, a, bX, bY, 1.0, beta, 1 / n, 1 / n, x0=x0, rtol=1e-10,
+                                maxiter=300)
+    assert st["converged"] and its == st["iters"], (st, its)
+    assert np.array_equal(dx_.download(), xr)
+    s.close()
+
+
+def test_odd_extent_rejected():
+    announce("helm create 26x25 (odd) must be refused")
+    with pytest.raises(Exception):
+        HelmholtzSolver(26, 25, 1.0 / 26, 1.0 / 25)
+
+
 @pytest.mark.parametrize("method", ["mg", "fgmres_mg"])
 def test_solve_reaches_tolerance_on_gresho_system(method):
     announce("helm %s gresho 256" % method)

@@ -152,6 +152,30 @@ def test_mg_solve_matches_direct_solve():
     assert np.all(rates < 0.3)
 
 
+def test_mg_solve_at_extent_halving_to_odd_matches_direct_solve():
+    """n = 100 halves to 50 and then to 25, an odd periodic extent on which
+    red/black colours meet across the seam; the hierarchy stops at 50 and
+    the solve still reaches the direct solution."""
+    scipy_sparse = pytest.importorskip("scipy.sparse")
+    from scipy.sparse.linalg import spsolve
+    rng = np.random.default_rng(14)
+    nx = ny = 100
+    a, bX, bY = rand_coeffs(rng, nx, ny)
+    alpha, beta, dx = 1.0, 2e-4, 1 / 100
+    b = rng.normal(size=(ny, nx))
+    x, its, hist = ho.mg_solve(b, a, bX, bY, alpha, beta, dx, dx, rtol=1e-10, maxiter=200)
+    assert 0 < its < 200
+    n = nx * ny
+    cols = []
+    for k in range(n):
+        e = np.zeros(n)
+        e[k] = 1
+        cols.append(ho.apply(e.reshape(ny, nx), a, bX, bY, alpha, beta, dx, dx).ravel())
+    A = scipy_sparse.csr_matrix(np.array(cols).T)
+    xd = spsolve(A, b.ravel()).reshape(ny, nx)
+    assert np.max(np.abs(x - xd)) <= 1e-8 * np.max(np.abs(xd))
+
+
 def test_vcycle_is_linear_and_symmetric():
     """One V-cycle from a zero guess with symmetric smoothing (red, black
     pre; red, black post) is a linear operator M; check linearity."""

@@ -1117,6 +1117,29 @@ extern "C" int bx_imex_rk_update(const double *U, const double *const *Fx,
     return BX_OK;
 }
 
+// The indicator launches and the read-back of its maximum; work holds the
+// per-cell indicator before the 3x3 maximum, acc the level maximum's bits.
+static int shock_indicator_run(const double *U, int nx, int ny, double dx, double dy,
+                               double gamma, double eps, double k1, double *work, double *ind,
+                               unsigned long long *acc, double *ind_max, hipStream_t s)
+{
+    PeekGeom g{ nx, ny, (long long)nx * ny };
+    hipLaunchKernelGGL(k_peek_zero_u64, dim3(1), dim3(1), 0, s, acc);
+    // idx = geom.InvCellSizeArray() (IMEX_RK.cpp:272)
+    hipLaunchKernelGGL(k_shock_premax, peek_grid(nx, ny), dim3(64, 4), 0, s, U, work, g, gamma,
+                       eps, 1.0 / dx, 1.0 / dy, k1);
+    PEEK_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_shock_max, peek_grid(nx, ny), dim3(64, 4), 0, s, work, ind, g, acc);
+    PEEK_CHECK(hipGetLastError());
+    unsigned long long bits = 0;
+    PEEK_CHECK(hipMemcpyAsync(&bits, acc, sizeof(bits), hipMemcpyDeviceToHost, s));
+    PEEK_CHECK(hipStreamSynchronize(s));
+    double m;
+    memcpy(&m, &bits, sizeof(m));
+    *ind_max = m;
+    return BX_OK;
+}
+
 extern "C" int bx_imex_shock_indicator(const double *U, int nx, int ny, double dx, double dy,
                                        double gamma, double eps, double k1, double *work,
                                        double *ind, double *ind_max, void *stream)
@@ -1124,26 +1147,12 @@ extern "C" int bx_imex_shock_indicator(const double *U, int nx, int ny, double d
     if (!U || !work || !ind || !ind_max || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0)
         || !(k1 > 0) || work == ind || (const double *)work == U || (const double *)ind == U)
         return bx_helm_fail(BX_ERR_INVALID, "bx_imex_shock_indicator: bad argument");
-    hipStream_t         s = (hipStream_t)stream;
     unsigned long long *d = nullptr;
     int                 rc;
     if ((rc = peek_acc("bx_imex_shock_indicator", ACC_SHOCK, &d)) != BX_OK)
         return rc;
-    PeekGeom g{ nx, ny, (long long)nx * ny };
-    hipLaunchKernelGGL(k_peek_zero_u64, dim3(1), dim3(1), 0, s, d);
-    // idx = geom.InvCellSizeArray() (IMEX_RK.cpp:272)
-    hipLaunchKernelGGL(k_shock_premax, peek_grid(nx, ny), dim3(64, 4), 0, s, U, work, g, gamma,
-                       eps, 1.0 / dx, 1.0 / dy, k1);
-    PEEK_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_shock_max, peek_grid(nx, ny), dim3(64, 4), 0, s, work, ind, g, d);
-    PEEK_CHECK(hipGetLastError());
-    unsigned long long bits = 0;
-    PEEK_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, s));
-    PEEK_CHECK(hipStreamSynchronize(s));
-    double m;
-    memcpy(&m, &bits, sizeof(m));
-    *ind_max = m;
-    return BX_OK;
+    return shock_indicator_run(U, nx, ny, dx, dy, gamma, eps, k1, work, ind, d, ind_max,
+                               (hipStream_t)stream);
 }
 
 extern "C" int bx_imex_stab_blend(const double *ind, const double *U_o1, const double *p_o1,

@@ -42,7 +42,7 @@ def main():
     ap.add_argument("--mgs", type=int, default=128)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--tuning", default="", help="block_x,block_t,seg_k,grid_cap (skips autotune)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -163,6 +163,15 @@ def main():
     value = nsteps * ncells / elapsed
     log("%d steps in %.3f ms (t=%.6g): %.3e cell-updates/s" % (nsteps, elapsed * 1e3, t_end, value))
 
+    # the timed run (tuned launch shapes, graph replay) against the same
+    # steps under the library's default shapes, launched from the host loop:
+    # the same bits in every valid cell, or the benchmark fails
+    check = state_check(s, gp, lev, nsteps, stream)
+    log("state check: %s" % check)
+    if not check["equal"]:
+        log("FAILED: the timed run's final state differs from the default-shape run")
+        sys.exit(3)
+
     dom = max(range(2), key=lambda d: kern_ms[d])
     algo_bytes = ncells * 2 * 4 * 8  # read + write NC doubles per valid cell
     achieved = algo_bytes / (kern_ms[dom] * 1e-3) / 1e9
@@ -191,7 +200,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.cpu_seconds)
+        cpu = cpu_baseline(args.cpu_repeats)
 
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
@@ -201,6 +210,7 @@ def main():
         "config": {"workload": "quadrant-2d-%dx%d-mgs%d-mhm-hllc-minbee-cfl0.8" % (
             args.ncell, args.ncell, args.mgs), "cells": ncells, "boxes": len(lay.boxes),
             "ghost_cells": lay.ng, "graph": True, "tuning": tuning},
+        "state_check": check,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "uneven": uneven,
@@ -209,6 +219,32 @@ def main():
         "imex": imex,
     }
     print(json.dumps(out), flush=True)
+
+
+def state_check(s, gp, lev, nsteps, stream):
+    """sha256 of the valid cells of the timed level's final state and of a
+    second level run for the same steps with the default launch shape and
+    no graph (host-driven steps)."""
+    import hashlib
+    import numpy as np
+    from bamrex_amd.level import Level
+    lay = s["layout"]
+    mask = np.zeros(lay.total, dtype=bool)
+    for b in range(len(lay.boxes)):
+        lay.valid(mask, b)[...] = True
+    got = lev.get_state()[mask]
+    ref_lev = Level(gp, lay, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], stream=stream)
+    try:
+        ref_lev.set_state(s["U"])
+        _, n_ref = ref_lev.run(s["dx"], s["cfl"], max_step=nsteps, use_graph=False)
+        ref = ref_lev.get_state()[mask]
+        default_tuning = ref_lev.get_tuning()
+    finally:
+        ref_lev.close()
+    h_got = hashlib.sha256(got.tobytes()).hexdigest()
+    h_ref = hashlib.sha256(ref.tobytes()).hexdigest()
+    return {"steps": nsteps, "sha256_timed": h_got, "sha256_default_shape_host_loop": h_ref,
+            "default_tuning": default_tuning, "equal": bool(h_got == h_ref and n_ref == nsteps)}
 
 
 def measured_traffic(kernel):
@@ -341,6 +377,54 @@ def vortex_leg(args):
            "l1_error": [float(x) for x in l1]}
     log("vortex: %d steps to t=%.4g, %.3e cell-updates/s, L1 rho %.3e" % (
         nsteps, t_end, out["cell_updates_per_s"], l1[0]))
+    if not args.no_cpu:
+        out["l1_cpu_vs_gpu"] = vortex_l1_cpu_gpu(t_end)
+    return out
+
+
+def vortex_l1_cpu_gpu(t_end, n=128, mgs=32):
+    """The isentropic vortex at n^3 on the GPU and in the CPU oracle (glibc
+    pow, as the reference), both to the stop time t_end: their L1 errors
+    against the analytic solution side by side (Exec/IsentropicVortex/
+    Prob.cpp:49-57, 67-135)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from bamrex_amd import problems
+    from bamrex_amd.level import Level, make_params
+    s = problems.vortex_setup(ncell=n, dim=3, max_grid_size=mgs)
+    lay = s["layout"]
+    kw = dict(limiter=s["limiter"], limiter_indices=s["limiter_indices"],
+              limit_on_primv=s["limit_on_primv"])
+    gp = make_params(3, s["adiabatic"], s["epsilon"], **kw)
+    op = pyoracle.params(3, s["adiabatic"], s["epsilon"], **kw)
+    lev = Level(gp, lay, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"])
+    try:
+        lev.set_state(s["U"])
+        t_gpu, n_gpu = lev.run(s["dx"], s["cfl"], stop_time=t_end, use_graph=False)
+        U_gpu = lev.get_state()
+    finally:
+        lev.close()
+    U_cpu = s["U"].copy()
+    t_cpu, n_cpu = pyoracle.Oracle(bxpow=False).run(op, lay, U_cpu, s["domain_lo"], s["domain_hi"],
+                                                     s["bc"], s["bc"], s["dx"], s["cfl"],
+                                                     stop_time=t_end)
+    ex = lay.new_array()
+    problems.isentropic_vortex(lay, ex, s["prob_lo"], s["prob_hi"], s["dx"], time=t_gpu)
+    l1g = np.zeros(lay.nc)
+    l1c = np.zeros(lay.nc)
+    dmax = 0.0
+    for b in range(len(lay.boxes)):
+        vg, vc, ve = lay.valid(U_gpu, b), lay.valid(U_cpu, b), lay.valid(ex, b)
+        l1g += np.abs(vg - ve).reshape(lay.nc, -1).sum(axis=1)
+        l1c += np.abs(vc - ve).reshape(lay.nc, -1).sum(axis=1)
+        dmax = max(dmax, float(np.max(np.abs(vg - vc))))
+    ncell = lay.ncells()
+    out = {"cells": ncell, "max_grid_size": mgs, "t_end": t_gpu, "steps_gpu": n_gpu,
+           "steps_cpu": n_cpu, "l1_gpu": [float(x) for x in l1g / ncell],
+           "l1_cpu": [float(x) for x in l1c / ncell], "max_abs_diff_gpu_cpu": dmax}
+    log("vortex %d^3 to t=%.4g: L1 rho GPU %.6e CPU %.6e (%d / %d steps)" % (
+        n, t_gpu, out["l1_gpu"][0], out["l1_cpu"][0], n_gpu, n_cpu))
     return out
 
 
@@ -544,33 +628,101 @@ def imex_leg(args):
     return out
 
 
-def cpu_baseline(budget_s):
-    """The oracle on a bounded sample of the same workload (a 1024^2 quadrant
-    level with the same boxes size and limiter), all host cores."""
+def host_cpu():
+    """CPU model and the cores this process may run on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, avail
+
+
+def _median_rate(fn, cells, repeats):
+    """Median over `repeats` runs of cells / seconds, fn() returning the
+    seconds of one run."""
+    rates = sorted(cells / fn() for _ in range(repeats))
+    return rates[len(rates) // 2]
+
+
+def cpu_baseline(repeats=3):
+    """The oracle (the same arithmetic as the kernels, glibc pow as the
+    reference calls it) on bounded samples of the C2, C3 and C4 workloads,
+    with all host threads and with one, median of `repeats` runs each (as
+    Exec/GreshoVortex/profiling/profile.sh repeats its runs).  The samples:
+    C2 one step of the 4096^2 quadrant level (one thread: of the 1024^2
+    level); C3 two steps of the isentropic vortex at 128^3 in boxes of 32^3
+    (one thread: 64^3);
+    C4 one step of the 3000-box uneven level (one thread: 600 of its boxes)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from bamrex_amd import problems
-    ncell = 1024
-    s = problems.quadrant_setup(ncell=ncell, max_grid_size=128)
-    p = pyoracle.params(2, s["adiabatic"], s["epsilon"], limiter=s["limiter"])
+    model, avail = host_cpu()
+    threads_all = int(os.environ.get("OMP_NUM_THREADS", avail))
     o = pyoracle.Oracle(bxpow=False)
-    # one step to size the sample
+
+    def run_rate(s, p, steps, threads):
+        o.threads(threads)
+        lay = s["layout"]
+
+        def go():
+            # the oracle times its own step loop (dt, sweeps, ghost fills),
+            # without building the level or copying the state in and out
+            U = s["U"].copy()
+            o.run(p, lay, U, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], s["dx"], s["cfl"],
+                  max_step=steps)
+            return o.last_run_seconds()
+        return _median_rate(go, lay.ncells() * steps, repeats)
+
+    def quad(n):
+        s = problems.quadrant_setup(ncell=n, max_grid_size=128)
+        return s, pyoracle.params(2, s["adiabatic"], s["epsilon"], limiter=s["limiter"])
+
+    def vortex(n):
+        # boxes of 32^3, so the oracle's box loop has work for every thread
+        s = problems.vortex_setup(ncell=n, dim=3, max_grid_size=32)
+        return s, pyoracle.params(3, s["adiabatic"], s["epsilon"], limiter=s["limiter"],
+                                  limiter_indices=s["limiter_indices"],
+                                  limit_on_primv=s["limit_on_primv"])
+
+    def uneven(nb):
+        s = problems.uneven_setup(nboxes=nb)
+        return s, pyoracle.params(2, s["adiabatic"], s["epsilon"], limiter=s["limiter"])
+
+    out = {}
     t0 = time.perf_counter()
-    U = s["U"].copy()
-    o.run(p, s["layout"], U, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], s["dx"], s["cfl"],
-          max_step=1)
-    one = time.perf_counter() - t0
-    nsteps = max(1, min(200, int(budget_s / max(one, 1e-3))))
-    U = s["U"].copy()
-    t0 = time.perf_counter()
-    o.run(p, s["layout"], U, s["domain_lo"], s["domain_hi"], s["bc"], s["bc"], s["dx"], s["cfl"],
-          max_step=nsteps)
-    el = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    log("cpu oracle: %d steps of %d^2 in %.2f s" % (nsteps, ncell, el))
-    return {"value": nsteps * ncell * ncell / el, "unit": "cell-updates/s", "cores": cores,
-            "kind": "port", "sample": "%d steps of the quadrant problem at %d^2, mgs 128" % (
-                nsteps, ncell)}
+    for key, (big, small, steps) in {"C2": (lambda: quad(4096), lambda: quad(1024), 1),
+                                      "C3": (lambda: vortex(128), lambda: vortex(64), 2),
+                                      "C4": (lambda: uneven(3000), lambda: uneven(600), 1)}.items():
+        s, p = big()
+        all_rate = run_rate(s, p, steps, threads_all)
+        cells_big = s["layout"].ncells()
+        del s
+        s, p = small()
+        one_rate = run_rate(s, p, steps, 1)
+        cells_small = s["layout"].ncells()
+        del s
+        out[key] = {"all_threads": all_rate, "one_thread": one_rate,
+                    "cells_all": cells_big, "cells_one": cells_small, "steps": steps}
+        log("cpu oracle %s: %.3e cell-updates/s on %d threads, %.3e on 1" % (
+            key, all_rate, threads_all, one_rate))
+    o.threads(threads_all)
+    log("cpu baseline: %.1f s" % (time.perf_counter() - t0))
+    return {"value": out["C2"]["all_threads"], "unit": "cell-updates/s", "cores": threads_all,
+            "kind": "port",
+            "sample": "1 step of the 4096^2 quadrant level (1024 boxes of 128^2), median of %d "
+                      "runs on %d threads" % (repeats, threads_all),
+            "one_thread": {"value": out["C2"]["one_thread"],
+                           "sample": "1 step of the 1024^2 quadrant level, median of %d runs" % repeats},
+            "cpu_model": model, "cpus_available": avail, "repeats": repeats,
+            "configs": out}
 
 
 if __name__ == "__main__":

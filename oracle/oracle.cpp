@@ -31,12 +31,15 @@
 // tests/test_oracle.py.
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <functional>
 #include <stdexcept>
 #include <vector>
+
+#include <omp.h>
 
 #ifdef ORACLE_BXPOW
 #include "../bamrex_amd/csrc/bx_pow.h"
@@ -614,10 +617,16 @@ static void fill_consv_boundary(Level &L)
     int       len[3];
     for (int d = 0; d < 3; ++d)
         len[d] = L.dhi[d] - L.dlo[d] + 1;
-    for (size_t b = 0; b < L.fabs.size(); ++b)
+    // boxes fill their own ghost cells from valid cells only, so the boxes
+    // are independent; the source box of the previous ghost cell is tried
+    // first (the boxes do not overlap, so the result does not depend on it)
+#pragma omp parallel for schedule(dynamic)
+    for (long bl = 0; bl < (long)L.fabs.size(); ++bl)
     {
-        Fab       &F  = L.fabs[b];
-        const Box &vb = L.valid[b];
+        const size_t b    = (size_t)bl;
+        Fab         &F    = L.fabs[b];
+        const Box   &vb   = L.valid[b];
+        size_t       last = 0;
         for_box(F.bx, [&](int i, int j, int k) {
             if (inside(vb, i, j, k))
                 return;
@@ -634,13 +643,17 @@ static void fill_consv_boundary(Level &L)
                 if (x[d] < L.dlo[d] || x[d] > L.dhi[d])
                     return; // physical boundary: PhysBCFunct below
             }
-            for (size_t s = 0; s < L.valid.size(); ++s)
+            for (size_t t = 0; t <= L.valid.size(); ++t)
+            {
+                const size_t s = t == 0 ? last : t - 1;
                 if (inside(L.valid[s], x[0], x[1], x[2]))
                 {
                     for (int n = 0; n < NC; ++n)
                         F(i, j, k, n) = L.fabs[s](x[0], x[1], x[2], n);
+                    last = s;
                     return;
                 }
+            }
         });
         for (int d = 0; d < D; ++d)
         {
@@ -715,10 +728,16 @@ static void sweep(Level &L, std::vector<Fab> &in, std::vector<Fab> &out,
 // max_wave_speed  Euler.cpp:46-74
 static double max_wave_speed(const Level &L)
 {
-    double       ws = 0; // identity of ReduceOpMax over non-negative values
-    const double a = L.parm.adiabatic, e = L.parm.epsilon;
-    const int    D = L.D;
-    for (size_t b = 0; b < L.valid.size(); ++b)
+    // identity of ReduceOpMax over non-negative values; a maximum does not
+    // depend on the order it is taken in, so boxes reduce independently
+    const double        a = L.parm.adiabatic, e = L.parm.epsilon;
+    const int           D = L.D;
+    std::vector<double> wb(L.valid.size(), 0.0);
+#pragma omp parallel for schedule(dynamic)
+    for (long bl = 0; bl < (long)L.valid.size(); ++bl)
+    {
+        const size_t b  = (size_t)bl;
+        double       ws = 0;
         for_box(L.valid[b], [&](int i, int j, int k) {
             const Vec    q   = primv_from_consv(L.fabs[b], a, e, i, j, k, D);
             double       vsq = q[1] * q[1];
@@ -728,6 +747,11 @@ static double max_wave_speed(const Level &L)
             const double s = std::sqrt(a * q[1 + D] / (q[0] * e)) + velocity;
             ws             = std::max(ws, s);
         });
+        wb[b] = ws;
+    }
+    double ws = 0;
+    for (double w : wb)
+        ws = std::max(ws, w);
     return ws;
 }
 
@@ -1202,6 +1226,11 @@ static void level_out(const Level &L, double *U, const int64_t *offsets)
 }
 
 // The main.cpp loop on a single level.  U: packed level (in/out).
+// Wall time of the step loop of the last or_run on this thread (the time
+// steps, without building the level and copying the state in and out).
+static thread_local double g_run_loop_seconds = 0.0;
+double or_last_run_seconds(void) { return g_run_loop_seconds; }
+
 int or_run(const or_params *p, const int *boxes, int nbox, int ng,
            const int *dlo, const int *dhi, const int *bclo, const int *bchi,
            const double *dx, double cfl, double start_time, double stop_time,
@@ -1212,8 +1241,9 @@ int or_run(const or_params *p, const int *boxes, int nbox, int ng,
     make_level(L, p, boxes, nbox, ng, dlo, dhi, bclo, bchi);
     level_in(L, U, offsets);
     fill_consv_boundary(L);
-    double time  = start_time;
-    int    steps = 0;
+    double     time  = start_time;
+    int        steps = 0;
+    const auto t0    = std::chrono::steady_clock::now();
     while ((steps < max_step || max_step < 0)
            && (time < stop_time || stop_time < 0.0))
     {
@@ -1222,6 +1252,8 @@ int or_run(const or_params *p, const int *boxes, int nbox, int ng,
         time = time + dt;
         ++steps;
     }
+    g_run_loop_seconds
+        = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     level_out(L, U, offsets);
     if (time_out)
         *time_out = time;
@@ -1529,6 +1561,15 @@ int or_amr_fill_fine(const or_params *p, const int *cboxes, int ncbox, const int
         cf_fill(F, Co, Cn, 1, (t_new - time) / (t_new - t_old), (time - t_old) / (t_new - t_old));
     level_out(F, Uf, foffsets);
     return 0;
+}
+
+// Threads of the box loop (omp_set_num_threads); n <= 0 leaves the count
+// unchanged.  Returns the count the next parallel region uses.
+int or_threads(int n)
+{
+    if (n > 0)
+        omp_set_num_threads(n);
+    return omp_get_max_threads();
 }
 
 } // extern "C"

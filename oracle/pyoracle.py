@@ -77,6 +77,10 @@ def _load(bxpow):
     for n, a in _SIGS.items():
         getattr(l, n).argtypes = a
         getattr(l, n).restype = ctypes.c_int
+    l.or_last_run_seconds.argtypes = []
+    l.or_last_run_seconds.restype = ctypes.c_double
+    l.or_threads.argtypes = [ctypes.c_int]
+    l.or_threads.restype = ctypes.c_int
     for n in ("or_limiter",):
         getattr(l, n).argtypes = [ctypes.c_int, ctypes.c_double]
         getattr(l, n).restype = ctypes.c_double
@@ -117,6 +121,15 @@ class Oracle:
 
     def __init__(self, bxpow=False):
         self.lib = _load(bxpow)
+
+    def last_run_seconds(self):
+        """Wall time of the step loop of the last run() (no level set-up or
+        state copies)."""
+        return self.lib.or_last_run_seconds()
+
+    def threads(self, n=0):
+        """Set the OpenMP threads of the box loop (n > 0); returns the count."""
+        return self.lib.or_threads(n)
 
     @staticmethod
     def _geom(layout, domain_lo, domain_hi, bc_lo, bc_hi):

@@ -196,13 +196,13 @@ extern "C" int bx_create(const bx_params *params, bx_ctx **out)
     c->D      = p.dim;
     c->NC     = NC;
     c->pm     = EulerParm{ p.adiabatic, p.epsilon, bx_pow_hllc_setup(p.adiabatic) };
+    // LV_SHARED: every component limited on the conserved energy
     bool shared = true, own = true;
     for (int n = 0; n < NC; ++n)
     {
         c->lt.idx[n]  = p.limiter_indices[n];
         c->lt.prim[n] = p.limit_on_primv[n] ? 1 : 0;
-        if (p.limiter_indices[n] != p.limiter_indices[0]
-            || (p.limit_on_primv[n] != 0) != (p.limit_on_primv[0] != 0))
+        if (p.limiter_indices[n] != NC - 1 || p.limit_on_primv[n])
             shared = false;
         if (p.limiter_indices[n] != n || p.limit_on_primv[n])
             own = false;

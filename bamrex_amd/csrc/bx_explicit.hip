@@ -193,7 +193,8 @@ __device__ __forceinline__ void x_item(const DevBox *__restrict__ boxes,
 }
 
 // PM (physics mode) bits: PM_NI -- the HLLC exponents take the near-integer
-// pow forms; PM_E1 -- epsilon == 1 (set only together with PM_NI).
+// pow forms with n = 7 (gamma = 1.4) fixed at compile time; PM_E1 --
+// epsilon == 1 (set only together with PM_NI).
 constexpr int PM_NI = 1;
 constexpr int PM_E1 = 2;
 
@@ -652,15 +653,18 @@ static hipError_t dispatch_dir_e(int dir, const SweepLaunch &L)
     return hipErrorInvalidValue;
 }
 
-// Physics mode.  The HLLC-based schemes take the near-integer pow forms when
-// gamma allows (PM_NI) and drop epsilon when it is 1 (PM_E1); FORCE, LF and
-// SLIC use no pow, so only epsilon == 1 (PM_NI | PM_E1) or not (0) matters.
+// Physics mode.  The HLLC-based schemes take the compile-time n = 7
+// near-integer pow forms when gamma gives n = 7 (PM_NI) and drop epsilon
+// when it is 1 (PM_E1); any other gamma runs the dispatching forms (0),
+// which choose the general or the run-time-n near-integer routine.  FORCE,
+// LF and SLIC use no pow, so only epsilon == 1 (PM_NI | PM_E1) or not (0)
+// matters.
 template <int D, int LIM, int LV, int SCH>
 static hipError_t dispatch_pm(int dir, const SweepLaunch &L)
 {
     if constexpr (scheme_uses_hllc(SCH))
     {
-        if (L.pm.ph.n == 0) // general pow (and any epsilon)
+        if (L.pm.ph.n != 7) // dispatching pow (and any epsilon)
             return dispatch_dir_e<D, LIM, LV, SCH, 0>(dir, L);
         if (L.pm.eps == 1.0)
             return dispatch_dir_e<D, LIM, LV, SCH, PM_NI | PM_E1>(dir, L);

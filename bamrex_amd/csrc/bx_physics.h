@@ -59,6 +59,37 @@ BX_DEV double div_by_recip(double a, double b, double y)
     return __builtin_fma(e, y, q0);
 }
 
+// sqrt(x), correctly rounded: the bits of IEEE sqrt.  For x in
+// [2^-767, 2^1024) -- every positive finite x except the tiny ones -- the
+// coupled Newton iteration from the hardware reciprocal square root
+// (g ~ sqrt(x), h ~ 1/(2 sqrt(x)), two final corrections g += (x - g^2) h;
+// Markstein, "Software division and square root using Goldschmidt's
+// algorithms", 2004) needs neither the rescaling of tiny arguments nor the
+// fix-up of zero and infinity that the library sqrt adds around the same
+// iteration; every other x takes the library sqrt.  Z0: x == 0 is common
+// (a fluid at rest), so it joins the fast path and returns x.
+template <bool Z0 = false>
+BX_DEV double sqrt_rn(double x)
+{
+    const unsigned hi = (unsigned)((unsigned long long)__double_as_longlong(x) >> 32);
+    const bool     in = hi - 0x10000000u < 0x6ff00000u;
+    if (__builtin_expect(Z0 ? (in || x == 0.0) : in, 1))
+    {
+        const double y = __builtin_amdgcn_rsq(x);
+        double       g = x * y;
+        double       h = y * 0.5;
+        const double r = __builtin_fma(-h, g, 0.5);
+        g              = __builtin_fma(g, r, g);
+        h              = __builtin_fma(h, r, h);
+        double d       = __builtin_fma(-g, g, x);
+        g              = __builtin_fma(d, h, g);
+        d              = __builtin_fma(-g, g, x);
+        g              = __builtin_fma(d, h, g);
+        return (Z0 && !in) ? x : g;
+    }
+    return sqrt(x);
+}
+
 // Also returns y = RN(1 / rho) for further quotients by rho.
 template <int D, bool E1 = false>
 BX_DEV void primv_from_consv(const double (&U)[D + 2], double (&Q)[D + 2],
@@ -129,9 +160,10 @@ BX_DEV double limiter_value(double vl, double vc, double vr)
 
 // Limiting-variable selection (CalcLimitingVal.H:9-22).  The host folds the
 // runtime (limiter_indices, limit_on_primv) table into one of three modes:
-//   LV_SHARED: every component limited on the same variable (default: energy)
+//   LV_SHARED: every component limited on the conserved energy (the default
+//              table, FluidSystem.cpp:67), fixed at compile time
 //   LV_OWN:    component n limited on conserved variable n
-//   LV_TABLE:  arbitrary table (evaluated with selects)
+//   LV_TABLE:  any other table (evaluated with selects)
 enum LimVarMode
 {
     LV_SHARED = 0,
@@ -172,24 +204,7 @@ BX_DEV void reconstruct_half_step(const double (&Um)[D + 2],
     double        phi[NC];
     if (LV == LV_SHARED)
     {
-        double vl, vc, vr;
-        if (lt.prim[0])
-        {
-            double Qm[NC], Q0[NC], Qp[NC];
-            primv_from_consv<D>(Um, Qm, pm);
-            primv_from_consv<D>(U0, Q0, pm);
-            primv_from_consv<D>(Up, Qp, pm);
-            vl = pick_var<D>(Um, Qm, lt.idx[0], 1);
-            vc = pick_var<D>(U0, Q0, lt.idx[0], 1);
-            vr = pick_var<D>(Up, Qp, lt.idx[0], 1);
-        }
-        else
-        {
-            vl = pick_var<D>(Um, Um, lt.idx[0], 0);
-            vc = pick_var<D>(U0, U0, lt.idx[0], 0);
-            vr = pick_var<D>(Up, Up, lt.idx[0], 0);
-        }
-        const double p = limiter_value<LIM>(vl, vc, vr);
+        const double p = limiter_value<LIM>(Um[NC - 1], U0[NC - 1], Up[NC - 1]);
 #pragma unroll
         for (int n = 0; n < NC; ++n)
             phi[n] = p;
@@ -239,9 +254,10 @@ BX_DEV double amrex_max(double a, double b) { return (a < b) ? b : a; }
 
 // HLLC flux at the face between left state UL and right state UR
 // (HLLC.cpp:90-172 with estimate_wave_speed_HLLC_Toro, HLLC.H:213-309).
-// NI: pm.ph.n != 0, so the two exponents take the near-integer forms with
-// no general pow inlined (same results as the dispatching forms: the floored
-// pressures are >= 1e-14 or NaN).
+// NI: pm.ph.n == 7 (gamma = 1.4), so the two exponents take the near-integer
+// forms for n = 7 fixed at compile time, with no general pow inlined (same
+// results as the dispatching forms: the floored pressures are >= 1e-14 or
+// NaN, and a compile-time n performs the operations of a run-time one).
 template <int D, int dir, bool E1 = false, bool NI = false>
 BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
                       double (&F)[D + 2], const EulerParm &pm)
@@ -269,21 +285,21 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
         constexpr double inv_min_rho = 1.0 / min_rho;
         const double     yLd = QL[0] < min_rho ? inv_min_rho : yL;
         const double     yRd = QR[0] < min_rho ? inv_min_rho : yR;
-        c_L                  = sqrt(div_by_recip(adiabatic * p_L, density_L, yLd));
-        c_R                  = sqrt(div_by_recip(adiabatic * p_R, density_R, yRd));
+        c_L                  = sqrt_rn(div_by_recip(adiabatic * p_L, density_L, yLd));
+        c_R                  = sqrt_rn(div_by_recip(adiabatic * p_R, density_R, yRd));
     }
     else
     {
-        c_L = sqrt(adiabatic * p_L / mul_eps<E1>(density_L, epsilon));
-        c_R = sqrt(adiabatic * p_R / mul_eps<E1>(density_R, epsilon));
+        c_L = sqrt_rn(adiabatic * p_L / mul_eps<E1>(density_L, epsilon));
+        c_R = sqrt_rn(adiabatic * p_R / mul_eps<E1>(density_R, epsilon));
     }
 
     const double z = (adiabatic - 1) / (2 * adiabatic);
     double       pzL, pzR, p_rr;
     if (NI)
     {
-        pzL = bx_pow_negz_ni(p_L, pm.ph);
-        pzR = bx_pow_negz_ni(p_R, pm.ph);
+        pzL = bx_pow_negz_ni<7>(p_L, pm.ph);
+        pzR = bx_pow_negz_ni<7>(p_R, pm.ph);
     }
     else
     {
@@ -293,16 +309,16 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
     const double ratio = (c_L + c_R - (adiabatic - 1) / 2 * (u_R - u_L))
                          / (c_L * pzL + c_R * pzR);
     if (NI)
-        p_rr = bx_pow_invz_ni(ratio, pm.ph);
+        p_rr = bx_pow_invz_ni<7>(ratio, pm.ph);
     else
         p_rr = bx_pow_invz(ratio, 1 / z, pm.ph);
     const double q_L
         = (p_rr > p_L)
-              ? sqrt(1 + (adiabatic + 1) / (2 * adiabatic) * (p_rr / p_L - 1))
+              ? sqrt_rn(1 + (adiabatic + 1) / (2 * adiabatic) * (p_rr / p_L - 1))
               : 1.;
     const double q_R
         = (p_rr > p_R)
-              ? sqrt(1 + (adiabatic + 1) / (2 * adiabatic) * (p_rr / p_R - 1))
+              ? sqrt_rn(1 + (adiabatic + 1) / (2 * adiabatic) * (p_rr / p_R - 1))
               : 1.;
     const double speed_L = u_L - c_L * q_L;
     const double speed_R = u_R + c_R * q_R;
@@ -348,6 +364,7 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
         const double u_K      = Qs[1 + dir];
         const double pressure = Qs[1 + D];
         const double energy   = Us[1 + D];
+        const double y_K      = left ? yL : yR; // RN(1 / density)
         double       st[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n)
@@ -356,7 +373,7 @@ BX_DEV void hllc_flux(const double (&UL)[D + 2], const double (&UR)[D + 2],
         st[1 + dir] = speed_interm;
         const double scale_factor
             = density * (speed_K - u_K) / (speed_K - speed_interm);
-        st[1 + D] = energy / density
+        st[1 + D] = div_by_recip(energy, density, y_K)
                     + (speed_interm - u_K)
                           * (mul_eps<E1>(speed_interm, epsilon)
                              + pressure / (density * (speed_K - u_K)));
@@ -463,10 +480,10 @@ BX_DEV double cell_wave_speed(const double (&U)[D + 2], const EulerParm &pm)
 #pragma unroll
     for (int d = 1; d < D; ++d)
         vsq = vsq + Q[1 + d] * Q[1 + d];
-    const double velocity = sqrt(vsq);
+    const double velocity = sqrt_rn<true>(vsq);
     if (E1)
-        return sqrt(div_by_recip(pm.gamma * Q[1 + D], Q[0], y)) + velocity;
-    return sqrt(pm.gamma * Q[1 + D] / mul_eps<E1>(Q[0], pm.eps)) + velocity;
+        return sqrt_rn(div_by_recip(pm.gamma * Q[1 + D], Q[0], y)) + velocity;
+    return sqrt_rn(pm.gamma * Q[1 + D] / mul_eps<E1>(Q[0], pm.eps)) + velocity;
 }
 
 } // namespace bx

@@ -256,40 +256,54 @@ BX_POW_FN double bx_pow(double x, double y)
 // estimate, so both values are known to ~1e-23 relative before the final
 // rounding: the result is the correctly rounded pow(x, y) except when the
 // exact value lies within ~1e-23 of a rounding boundary, the same answer the
-// general routine above gives, for about a third of its instruction count.
+// general routine above gives, for a fraction of its instruction count.
 // The double-double work runs on operands rescaled by exact powers of two
 // (x r^n is scale-free), so no input range needs the general routine: the
 // *_ni forms below have no branch into it and keep a kernel that calls them
 // free of its registers.
 namespace bxpow {
 
-BX_POW_FN void dd_sqr(double &h, double &l)
+// Double-double products left unnormalised: the pair (h, l) stands for
+// h + l with |l| at most a few ulps of h.  Each step forms the leading
+// product exactly (p + e = h x, by fma) and folds the cross terms into the
+// new low part; the dropped l*l terms and the roundings of the low parts
+// stay below ~2^-100 relative over the few steps of x^n (n <= 32), so no
+// renormalising TwoSum is needed between steps.
+BX_POW_FN void ddu_sqr(double &h, double &l)
 {
     const double p = h * h;
-    double       e = __builtin_fma(h, h, -p);
-    e              = __builtin_fma(2.0 * h, l, e);
-    fast_two_sum(p, e, h, l);
+    const double e = __builtin_fma(h, h, -p); // h^2 - p, exact
+    l              = __builtin_fma(h + h, l, e);
+    h              = p;
 }
 
-BX_POW_FN void dd_mul_d(double &h, double &l, double x)
+BX_POW_FN void ddu_mul_d(double &h, double &l, double x)
 {
     const double p = h * x;
-    double       e = __builtin_fma(h, x, -p);
-    e              = __builtin_fma(l, x, e);
-    fast_two_sum(p, e, h, l);
+    const double e = __builtin_fma(h, x, -p); // h x - p, exact
+    l              = __builtin_fma(l, x, e);
+    h              = p;
 }
 
-// x^n (n >= 1) as a double-double, left-to-right binary powering
-BX_POW_FN void dd_powi(double x, int n, double &h, double &l)
+// x^n (n >= 2) as an unnormalised double-double, left-to-right binary
+// powering; the first square is of the double x itself (exact: x^2 = h + l).
+// NF > 0 fixes n = NF at compile time (the loop unrolls); the operations,
+// and so the result, are the same as for a run-time n = NF.
+template <int NF>
+BX_POW_FN void ddu_powi(double x, int n, double &h, double &l)
 {
-    h             = x;
-    l             = 0.0;
+    if (NF > 0)
+        n = NF;
     const int top = 31 - __builtin_clz((unsigned)n);
-    for (int b = top - 1; b >= 0; --b)
+    h             = x * x;
+    l             = __builtin_fma(x, x, -h);
+    if ((n >> (top - 1)) & 1)
+        ddu_mul_d(h, l, x);
+    for (int b = top - 2; b >= 0; --b)
     {
-        dd_sqr(h, l);
+        ddu_sqr(h, l);
         if ((n >> b) & 1)
-            dd_mul_d(h, l, x);
+            ddu_mul_d(h, l, x);
     }
 }
 
@@ -376,17 +390,20 @@ BX_POW_FN int exponent_field(double x)
 // pressures, floored at 1e-14).  The Newton residual x r^n - 1 is formed on
 // x 2^(-n q) and r 2^q with q ~ k/n (k the exponent of x): exact rescalings
 // that keep the double-double intermediates normal for every such x.
+// NF > 0 fixes n = NF = s.n at compile time, with the same result.
+template <int NF = 0>
 BX_POW_FN double bx_pow_negz_ni(double x, const BxPowHllc &s)
 {
     using namespace bxpow;
+    const int    n  = NF > 0 ? NF : s.n;
     const double lx = log_approx(x);
     const double r  = exp_approx(-lx * s.inv_n); // x^(-1/n) to ~1e-12
     const int    q  = (exponent_field(x) * s.qmul) >> 16;
     double       h, l;
-    dd_powi(__builtin_ldexp(r, q), s.n, h, l);
-    dd_mul_d(h, l, __builtin_ldexp(x, -s.n * q)); // x r^n = h + l, close to 1
-    const double delta = (1.0 - h) - l;
-    const double c     = r * (delta * s.inv_n); // Newton correction
+    ddu_powi<NF>(__builtin_ldexp(r, q), n, h, l);
+    ddu_mul_d(h, l, __builtin_ldexp(x, -n * q)); // x r^n = h + l, close to 1
+    const double delta = (1.0 - h) - l;           // 1 - h exact (Sterbenz)
+    const double c     = r * (delta * s.inv_n);   // Newton correction
     const double res   = r + (c + r * (s.w * lx));
     return x == __builtin_inf() ? 0.0 : res;
 }
@@ -396,23 +413,26 @@ BX_POW_FN double bx_pow_negz(double x, double y, const BxPowHllc &s)
 {
     if (!(s.n != 0 && x >= 0x1p-1022))
         return bx_pow(x, y);
-    return bx_pow_negz_ni(x, s);
+    return bx_pow_negz_ni<0>(x, s);
 }
 
 // pow(x, 1/z) for s.n != 0 and any x.  x^n is formed on the mantissa
 // m = x 2^-k in [1, 2) and scaled back by 2^(n k) after the final rounding:
 // exact unless the result is below 2^-1022, where that second scaling
 // rounds again (results under 2^-1022 may be off by one subnormal ulp).
+// NF > 0 fixes n = NF = s.n at compile time, with the same result.
+template <int NF = 0>
 BX_POW_FN double bx_pow_invz_ni(double x, const BxPowHllc &s)
 {
     using namespace bxpow;
+    const int n = NF > 0 ? NF : s.n;
     const int k = exponent_field(x);
     double    h, l;
-    dd_powi(__builtin_ldexp(x, -k), s.n, h, l); // signed: (-m)^n
-    const double t   = s.d * log_approx(x);    // log |x|
-    double       res = __builtin_ldexp(h + __builtin_fma(h, t, l), s.n * k);
+    ddu_powi<NF>(__builtin_ldexp(x, -k), n, h, l); // signed: (-m)^n
+    const double t   = s.d * log_approx(x);        // log |x|
+    double       res = __builtin_ldexp(h + __builtin_fma(h, t, l), n * k);
     // 1/z is an integer only when d == 0; otherwise x < 0 gives NaN
-    const bool odd_int = s.d == 0.0 && (s.n & 1);
+    const bool odd_int = s.d == 0.0 && (n & 1);
     if (x == 0.0)
         res = odd_int ? x : 0.0;
     else if (__builtin_fabs(x) == __builtin_inf())
@@ -427,5 +447,5 @@ BX_POW_FN double bx_pow_invz(double x, double y, const BxPowHllc &s)
 {
     if (s.n == 0)
         return bx_pow(x, y);
-    return bx_pow_invz_ni(x, s);
+    return bx_pow_invz_ni<0>(x, s);
 }

@@ -294,7 +294,8 @@ static void build_sweep_tables(bx_ctx *c)
     for (int d = 0; d < 3; ++d)
         c->ib_t[d].assign(nb + 1, 0);
     c->cbeg.assign(nb + 1, 0);
-    const int K = c->tune.seg_k;
+    const int K    = c->tune.seg_k;
+    const int xstp = x_item_step(c->tune.block_x);
     for (int b = 0; b < nb; ++b)
     {
         const bx_box &bx  = c->hboxes[b];
@@ -302,7 +303,7 @@ static void build_sweep_tables(bx_ctx *c)
         const long long ny = D >= 2 ? box_n(bx, 1) : 1;
         const long long nz = D >= 3 ? box_n(bx, 2) : 1;
         const long long P  = ny * (nx + 2 * ng);
-        c->ib_x[b + 1]     = c->ib_x[b] + nz * ((P + 59) / 60);
+        c->ib_x[b + 1]     = c->ib_x[b] + nz * ((P + xstp - 1) / xstp);
         if (D >= 2)
         {
             const long long q = nx * nz * ((ny + K - 1) / K);
@@ -516,7 +517,8 @@ static int occupancy(bx_ctx *c, int dir, int block, int *occ)
 {
     int n = 0;
     BX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &n, sweep_kernel_ptr(c->D, dir), block, 0));
+        &n, sweep_kernel_ptr(c->D, dir), block,
+        dir == 0 ? x_lds_bytes(c->NC) : 0));
     *occ = std::max(1, n);
     return BX_OK;
 }
@@ -608,9 +610,12 @@ extern "C" int bx_set_level(bx_ctx *c, const bx_geometry *geom,
 }
 
 // ------------------------------------------------------------- launches
-static int persistent_grid(const bx_ctx *c, long long nitems, int block, int occ)
+// Occupancy-limited grid.  Items per block: one at a time per wave
+// (wave_items, the y/z sweeps) or per block (the x sweep).
+static int persistent_grid(const bx_ctx *c, long long nitems, int block, int occ,
+                           bool wave_items)
 {
-    const int       wpb   = block / 64;
+    const int       wpb   = wave_items ? block / 64 : 1;
     const long long need  = (nitems + wpb - 1) / wpb;
     const int       cap   = c->tune.grid_cap > 0 ? std::min(c->tune.grid_cap, occ) : occ;
     long long       limit = (long long)c->ncu * cap;
@@ -641,14 +646,14 @@ int do_sweep(bx_ctx *c, int dir, const double dx[3], const double *Uin,
         L.item_begin = c->d_ib_x;
         L.nitems     = c->ib_x.back();
         L.block      = c->tune.block_x;
-        L.grid       = persistent_grid(c, L.nitems, L.block, c->occ_x);
+        L.grid       = persistent_grid(c, L.nitems, L.block, c->occ_x, false);
     }
     else
     {
         L.item_begin = c->d_ib_t[dir];
         L.nitems     = c->ib_t[dir].back();
         L.block      = c->tune.block_t;
-        L.grid       = persistent_grid(c, L.nitems, L.block, c->occ_t);
+        L.grid       = persistent_grid(c, L.nitems, L.block, c->occ_t, true);
     }
     if (L.nitems == 0)
         return BX_OK;

@@ -13,10 +13,13 @@
 // items (a range stays inside one XCD's L2 because consecutive ranges are
 // handed to blocks of the same XCD).
 //
-//  * x sweep (unit stride): an item is 64 consecutive positions of one
-//    box plane, flattened over (valid rows) x (row incl. ghosts).  Lane t
-//    holds one position; the neighbour values the stencil needs move between
-//    lanes with __shfl_up/__shfl_down.  60 of 64 lanes produce an update.
+//  * x sweep (unit stride): an item is B consecutive positions of one box
+//    plane (B = the block size), flattened over (valid rows) x (row incl.
+//    ghosts); all waves of a block work on one item.  Thread t holds one
+//    position; the neighbour values the stencil needs (U at t-1 and t+1, the
+//    high-side half-stepped state at t-1, the face flux at t+1) pass through
+//    LDS, one buffer each, with a block barrier after each write.  B - 4 of
+//    B threads produce an update.
 //  * y/z sweeps (strided): an item is 64 (transverse cell, segment) pairs;
 //    lane t walks its column along the sweep direction through K cells,
 //    keeping the 3-cell window, the previous half-stepped state and the
@@ -81,6 +84,24 @@ __device__ __forceinline__ unsigned long long speed_max(unsigned long long acc,
     return s == s ? umax64(acc, (unsigned long long)__double_as_longlong(s)) : acc;
 }
 
+// Contiguous item range of this block (all its waves share each item),
+// XCD-aware: consecutive ranges go to blocks of the same XCD (blocks are
+// dealt to the 8 XCDs round-robin).
+__device__ __forceinline__ void block_item_range(long long nitems,
+                                                 long long &first,
+                                                 long long &last)
+{
+    const int nblk = gridDim.x;
+    int       blk  = blockIdx.x;
+    if ((nblk & 7) == 0)
+        blk = (blk & 7) * (nblk >> 3) + (blk >> 3);
+    const long long per = (nitems + nblk - 1) / nblk;
+    first               = (long long)blk * per;
+    last                = first + per;
+    if (last > nitems)
+        last = nitems;
+}
+
 // Contiguous item range of this wave, XCD-aware: blocks are dealt to the 8
 // XCDs round-robin, so block b runs on XCD b%8; renumber so that XCD x owns
 // a contiguous slice of the item space.
@@ -128,28 +149,9 @@ __device__ __forceinline__ void block_max_to_global(unsigned long long v,
     }
 }
 
-// Whole-wave lane shifts through DPP (wave_shr:1 / wave_shl:1, a GFX9
-// feature): lane t receives lane t-1 (shr) or lane t+1 (shl).  Lanes at the
-// wave ends keep their own value; those lanes never produce an update.
-__device__ __forceinline__ double wave_shr1(double v)
-{
-    const int2 w = __builtin_bit_cast(int2, v);
-    int2       r;
-    r.x = __builtin_amdgcn_update_dpp(w.x, w.x, 0x138, 0xf, 0xf, false);
-    r.y = __builtin_amdgcn_update_dpp(w.y, w.y, 0x138, 0xf, 0xf, false);
-    return __builtin_bit_cast(double, r);
-}
-__device__ __forceinline__ double wave_shl1(double v)
-{
-    const int2 w = __builtin_bit_cast(int2, v);
-    int2       r;
-    r.x = __builtin_amdgcn_update_dpp(w.x, w.x, 0x130, 0xf, 0xf, false);
-    r.y = __builtin_amdgcn_update_dpp(w.y, w.y, 0x130, 0xf, 0xf, false);
-    return __builtin_bit_cast(double, r);
-}
-
 // ---------------------------------------------------------------- x sweep
-// Geometry of one x-sweep item (64 consecutive positions of one box plane).
+// Geometry of one x-sweep item (B consecutive positions of one box plane,
+// B - 4 = step of them updated).
 struct XItem
 {
     long long pbase; // element offset of position 0 of the plane
@@ -163,10 +165,9 @@ template <int D>
 __device__ __forceinline__ void x_item(const DevBox *__restrict__ boxes,
                                        const long long *__restrict__ ibeg,
                                        int nbox, long long item, int ng,
-                                       int lane, int &b, long long &bend,
-                                       XItem &it)
+                                       int lane, int step, int &b,
+                                       long long &bend, XItem &it)
 {
-    constexpr int STEP = 60;
     if (item >= bend)
     {
         b    = find_box(ibeg, nbox, item);
@@ -177,7 +178,7 @@ __device__ __forceinline__ void x_item(const DevBox *__restrict__ boxes,
     const int     ny  = (D >= 2) ? bx.hi[1] - bx.lo[1] + 1 : 1;
     const int     nxd = nx + 2 * ng;
     const long long P = (long long)ny * nxd;
-    const int       cpp   = (int)((P + STEP - 1) / STEP);
+    const int       cpp   = (int)((P + step - 1) / step);
     const unsigned  local = (unsigned)(item - ibeg[b]);
     const int       kpl   = (int)(local / (unsigned)cpp);
     const int       chunk = (int)(local - (unsigned)kpl * (unsigned)cpp);
@@ -185,7 +186,7 @@ __device__ __forceinline__ void x_item(const DevBox *__restrict__ boxes,
     const int       ngz   = (D >= 3) ? ng : 0;
     it.pbase = bx.base + (long long)(kpl + ngz) * bx.kstride
                + (long long)ngy * bx.jstride;
-    it.p   = (long long)chunk * STEP - 2 + lane;
+    it.p   = (long long)chunk * step - 2 + lane;
     it.P   = P;
     it.nst = bx.nstride;
     it.nx  = nx;
@@ -208,16 +209,25 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     constexpr int  NC    = D + 2;
     constexpr bool E1    = (PM & PM_E1) != 0;
     constexpr bool NI    = (PM & PM_NI) != 0;
-    const int      lane  = threadIdx.x & 63;
+    const int      t     = threadIdx.x;
+    const int      B     = blockDim.x;
+    const int      step  = x_item_step(B);
+    const int      tm    = t > 0 ? t - 1 : 0;     // neighbours inside the item
+    const int      tp    = t < B - 1 ? t + 1 : t; // (clamped at its ends)
     const double   dt    = *d_dt;
     const double   hdtdx = 0.5 * dt / dx;
     const double   dtdx  = dt / dx;
     const double   fc    = 0.5 * dx / dt;   // FORCE / LF
     const double   rc    = dt / (2 * dx);   // FORCE
     constexpr bool RECON = scheme_reconstructs(SCH);
+    constexpr int  XS    = X_LDS_STRIDE;
+    extern __shared__ double x_lds[];
+    double *const  sU = x_lds;
+    double *const  sH = x_lds + NC * XS;
+    double *const  sF = x_lds + 2 * NC * XS;
 
     long long first, last;
-    wave_item_range(nitems, first, last);
+    block_item_range(nitems, first, last);
     unsigned long long vmax = 0ULL;
     int                b    = -1;
     long long          bend = -1;
@@ -231,9 +241,12 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     };
     if (first < last)
     {
-        x_item<D>(boxes, ibeg, nbox, first, ng, lane, b, bend, nxt);
+        x_item<D>(boxes, ibeg, nbox, first, ng, t, step, b, bend, nxt);
         issue(nxt, Un_);
     }
+    // Every thread of the block runs the same items, so the barriers below
+    // are reached uniformly.  Each buffer is written before one barrier and
+    // read after it; its next write comes after the following barrier.
     for (long long item = first; item < last; ++item)
     {
         cur = nxt;
@@ -243,13 +256,17 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
             U[n] = Un_[n];
         if (item + 1 < last)
         {
-            x_item<D>(boxes, ibeg, nbox, item + 1, ng, lane, b, bend, nxt);
+            x_item<D>(boxes, ibeg, nbox, item + 1, ng, t, step, b, bend, nxt);
             issue(nxt, Un_);
         }
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            sU[n * XS + t] = U[n];
+        __syncthreads();
         double Um[NC], F[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n)
-            Um[n] = wave_shr1(U[n]);
+            Um[n] = sU[n * XS + tm];
         if (!RECON)
         {
             face_flux<D, 0, E1, NI, SCH>(Um, U, F, pm, fc, rc);
@@ -259,22 +276,30 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
             double Up[NC], hL[NC], hR[NC], hRm[NC];
 #pragma unroll
             for (int n = 0; n < NC; ++n)
-                Up[n] = wave_shl1(U[n]);
+                Up[n] = sU[n * XS + tp];
             reconstruct_half_step<D, 0, LIM, LV, E1>(Um, U, Up, hL, hR, hdtdx,
                                                      pm, lt);
 #pragma unroll
             for (int n = 0; n < NC; ++n)
-                hRm[n] = wave_shr1(hR[n]);
+                sH[n * XS + t] = hR[n];
+            __syncthreads();
+#pragma unroll
+            for (int n = 0; n < NC; ++n)
+                hRm[n] = sH[n * XS + tm];
             face_flux<D, 0, E1, NI, SCH>(hRm, hL, F, pm, fc, rc);
         }
+#pragma unroll
+        for (int n = 0; n < NC; ++n)
+            sF[n * XS + t] = F[n];
+        __syncthreads();
         double Fp[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n)
-            Fp[n] = wave_shl1(F[n]);
+            Fp[n] = sF[n * XS + tp];
 
         const long long pc  = cur.p < 0 ? 0 : (cur.p >= cur.P ? cur.P - 1 : cur.p);
         const int       col = (int)((unsigned)pc % (unsigned)cur.nxd);
-        const bool upd = lane >= 2 && lane < 62 && cur.p < cur.P && col >= ng
+        const bool upd = t >= 2 && t < B - 2 && cur.p < cur.P && col >= ng
                          && col < ng + cur.nx;
         if (upd)
         {
@@ -623,7 +648,8 @@ template <int D, int LIM, int LV, int SCH, int PM>
 static hipError_t launch_x(const SweepLaunch &L)
 {
     hipLaunchKernelGGL((k_sweep_x<D, LIM, LV, SCH, PM>), dim3(L.grid),
-                       dim3(L.block), 0, L.stream, L.Uin, L.Uout, L.boxes,
+                       dim3(L.block), x_lds_bytes(D + 2), L.stream,
+                       L.Uin, L.Uout, L.boxes,
                        L.item_begin, L.nbox, L.nitems, L.ng, L.d_dt, L.dx,
                        L.pm, L.lt, L.d_maxbits);
     return hipGetLastError();

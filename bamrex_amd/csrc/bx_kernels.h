@@ -22,6 +22,16 @@ struct StepTail
     double              stop_time = -1;
 };
 
+// x sweep: an item is B consecutive positions of a box plane (B = block
+// size, at most X_LDS_STRIDE), of which B - 4 are updated; the block
+// exchanges neighbour values through three LDS buffers (U, high-side half
+// state, face flux) of NC rows of X_LDS_STRIDE doubles, allocated
+// dynamically.  The compile-time row stride keeps every LDS address an
+// immediate offset from the thread's own.
+constexpr int X_LDS_STRIDE = 256;
+constexpr int x_item_step(int B) { return B - 4; }
+constexpr int x_lds_bytes(int NC) { return 3 * NC * X_LDS_STRIDE * 8; }
+
 struct SweepLaunch
 {
     const double       *Uin;

@@ -390,21 +390,62 @@ BX_POW_FN int exponent_field(double x)
 // pressures, floored at 1e-14).  The Newton residual x r^n - 1 is formed on
 // x 2^(-n q) and r 2^q with q ~ k/n (k the exponent of x): exact rescalings
 // that keep the double-double intermediates normal for every such x.
-// NF > 0 fixes n = NF = s.n at compile time, with the same result.
+// NF > 0 fixes n = NF = s.n at compile time; for n = 7 the estimate comes
+// from tables (negz_est7), which changes the estimate but not its accuracy,
+// so NF = 7 is what any caller with s.n == 7 uses (bx_pow_negz dispatches).
+namespace bxpow {
+// x^(-1/7) 2^q to ~1e-13 relative (rs), x 2^(-7q) (xs), and log x to
+// ~1e-8 absolute (lx), for x positive and normal.  x = 2^k m with the
+// log_approx split (m in [0.75, 1.5)), k = 7q + i with 0 <= i < 7:
+//   x^(-1/7) = 2^-q 2^(-i/7) invc_j^(1/7) (1 + t)^(-1/7),  1 + t = m invc_j,
+// |t| < 2^-8, (1 + t)^(-1/7) by its binomial series to t^4.
+BX_POW_FN void negz_est7(double x, double &rs, double &xs, double &lx, int &q)
+{
+    const uint64_t ix = d2u(x);
+    const int      j  = (int)((ix >> 45) & 127);
+    const int      hi = j >> 6;
+    const int      k  = (int)((ix >> 52) & 0x7ff) - 1023 + hi;
+    const double   m
+        = u2d((ix & 0x000fffffffffffffULL) | ((uint64_t)(1023 - hi) << 52));
+    const double t = __builtin_fma(m, bx_log_invc[j], -1.0);
+    double       p = 110.0 / 2401.0;                // C(-1/7, 4)
+    p              = __builtin_fma(p, t, -20.0 / 343.0); // C(-1/7, 3)
+    p              = __builtin_fma(p, t, 4.0 / 49.0);    // C(-1/7, 2)
+    p              = __builtin_fma(p, t, -1.0 / 7.0);    // C(-1/7, 1)
+    p              = __builtin_fma(p, t, 1.0);
+    // q = floor(k / 7): k + 1050 in [28, 2075], where (v 9363) >> 16 == v / 7
+    q           = (((k + 1050) * 9363) >> 16) - 150;
+    const int i = k - 7 * q;
+    rs          = (bx_r7_invc[j] * bx_r7_two[i]) * p;
+    xs          = __builtin_ldexp(x, -7 * q);
+    lx          = ((double)k * BX_LN2_HI + bx_log_logc_hi[j]) + t * __builtin_fma(-0.5, t, 1.0);
+}
+} // namespace bxpow
+
 template <int NF = 0>
 BX_POW_FN double bx_pow_negz_ni(double x, const BxPowHllc &s)
 {
     using namespace bxpow;
-    const int    n  = NF > 0 ? NF : s.n;
-    const double lx = log_approx(x);
-    const double r  = exp_approx(-lx * s.inv_n); // x^(-1/n) to ~1e-12
-    const int    q  = (exponent_field(x) * s.qmul) >> 16;
-    double       h, l;
-    ddu_powi<NF>(__builtin_ldexp(r, q), n, h, l);
-    ddu_mul_d(h, l, __builtin_ldexp(x, -n * q)); // x r^n = h + l, close to 1
-    const double delta = (1.0 - h) - l;           // 1 - h exact (Sterbenz)
-    const double c     = r * (delta * s.inv_n);   // Newton correction
-    const double res   = r + (c + r * (s.w * lx));
+    const int n = NF > 0 ? NF : s.n;
+    double    r, xs, lx;
+    int       q;
+    if constexpr (NF == 7)
+    {
+        negz_est7(x, r, xs, lx, q); // r = x^(-1/7) 2^q
+    }
+    else
+    {
+        lx = log_approx(x);
+        q  = (exponent_field(x) * s.qmul) >> 16;
+        r  = __builtin_ldexp(exp_approx(-lx * s.inv_n), q); // x^(-1/n) 2^q to ~1e-12
+        xs = __builtin_ldexp(x, -n * q);
+    }
+    double h, l;
+    ddu_powi<NF>(r, n, h, l);
+    ddu_mul_d(h, l, xs);                        // x r^n = h + l, close to 1
+    const double delta = (1.0 - h) - l;         // 1 - h exact (Sterbenz)
+    const double c     = r * (delta * s.inv_n); // Newton correction
+    const double res   = __builtin_ldexp(r + (c + r * (s.w * lx)), -q);
     return x == __builtin_inf() ? 0.0 : res;
 }
 
@@ -413,6 +454,8 @@ BX_POW_FN double bx_pow_negz(double x, double y, const BxPowHllc &s)
 {
     if (!(s.n != 0 && x >= 0x1p-1022))
         return bx_pow(x, y);
+    if (s.n == 7)
+        return bx_pow_negz_ni<7>(x, s);
     return bx_pow_negz_ni<0>(x, s);
 }
 

@@ -594,11 +594,12 @@ __global__ __launch_bounds__(256) void k_peek_pressure(const double *__restrict_
 //     (|m_x|/dx + |m_y|/dy) / rho
 //   ACOUSTIC (max_wave_speed, Euler.cpp:47-74):
 //     sqrt(gamma p / (rho eps)) + sqrt(u^2 + v^2)
+// Block maximum of the speeds as ordered bits (valid in thread 0); speeds
+// that are NaN or not positive are skipped.
 template <bool ACOUSTIC>
-__global__ __launch_bounds__(256) void k_peek_speed(const double *__restrict__ U, long long n,
-                                                    double dx, double dy, double adia,
-                                                    double eps,
-                                                    unsigned long long *__restrict__ out)
+__device__ __forceinline__ unsigned long long block_speed_max(const double *__restrict__ U,
+                                                              long long n, double dx, double dy,
+                                                              double adia, double eps)
 {
     unsigned long long m = 0ULL;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
@@ -632,11 +633,66 @@ __global__ __launch_bounds__(256) void k_peek_speed(const double *__restrict__ U
         red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0)
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
+            m = red[i] > m ? red[i] : m;
+    return m;
+}
+
+template <bool ACOUSTIC>
+__global__ __launch_bounds__(256) void k_peek_speed(const double *__restrict__ U, long long n,
+                                                    double dx, double dy, double adia,
+                                                    double eps,
+                                                    unsigned long long *__restrict__ out)
+{
+    const unsigned long long m = block_speed_max<ACOUSTIC>(U, n, dx, dy, adia, eps);
+    if (threadIdx.x == 0 && m != 0ULL)
+        atomicMax(out, m);
+}
+
+// The same block maxima, one partial per block with no atomics: every
+// block writes its slot (0 when it saw no positive speed).
+template <bool ACOUSTIC>
+__global__ __launch_bounds__(256) void k_peek_speed_part(const double *__restrict__ U,
+                                                         long long n, double dx, double dy,
+                                                         double adia, double eps,
+                                                         unsigned long long *__restrict__ part)
+{
+    const unsigned long long m = block_speed_max<ACOUSTIC>(U, n, dx, dy, adia, eps);
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = m;
+}
+
+// One block: the maximum of the first nparts partials, then the acoustic
+// time step from it with the host path's arithmetic (ExplicitTimeStepDimSplit,
+// FluidSystem.cpp:200-205, then dt *= cfl), written to device memory.
+__global__ __launch_bounds__(256) void k_acoustic_dt_from_parts(
+    const unsigned long long *__restrict__ part, int nparts, double dx, double dy, double cfl,
+    double *__restrict__ dt_out)
+{
+    unsigned long long m = 0ULL;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+        m = part[i] > m ? part[i] : m;
+    __shared__ unsigned long long red[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const unsigned long long o = __shfl_xor(m, off);
+        m                          = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
     {
         for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
             m = red[i] > m ? red[i] : m;
-        if (m != 0ULL)
-            atomicMax(out, m);
+        const double smax   = __longlong_as_double((long long)m);
+        double       dt_est = 1.0e20;
+        const double q0 = dx / smax, q1 = dy / smax;
+        dt_est = (q0 < dt_est) ? q0 : dt_est;
+        dt_est = (q1 < dt_est) ? q1 : dt_est;
+        dt_est *= cfl;
+        *dt_out = dt_est;
     }
 }
 
@@ -1192,6 +1248,65 @@ extern "C" int bx_imex_picard_update(const double *Uex, const double *p, int nx,
     hipLaunchKernelGGL(k_picard_update, peek_grid(nx, ny), dim3(64, 4), 0, (hipStream_t)stream,
                        Uex, p, enth, rhs, bX, bY, g, gamma, eps, hdtdx, hdtdy, hdtdx / eps,
                        hdtdy / eps);
+    PEEK_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of bx_imex_acoustic_dt_noalloc: one partial maximum per block of
+// its speed kernel, at most NOALLOC_PARTS blocks (8 KiB), in a file-scope
+// device array that each GPU holds until the process ends.  Host code uses
+// only the address hipGetSymbolAddress returns for the current device
+// (noalloc_scratch), never the array's name.  Every call on one GPU shares
+// that GPU's array: calls that overlap on one GPU (different streams, or
+// host threads) race.
+constexpr int NOALLOC_PARTS = 1024;
+static __device__ __align__(256) unsigned long long scratch[NOALLOC_PARTS];
+
+static int noalloc_scratch(const char *who, unsigned long long **out)
+{
+    static std::mutex          mu;
+    static unsigned long long *ptr[64] = {};
+    int                        dev     = 0;
+    PEEK_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64)
+        return bx_helm_fail(BX_ERR_UNSUPPORTED, std::string(who) + ": device id");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev])
+    {
+        void            *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return bx_helm_fail(BX_ERR_HIP, std::string(who) + ": hipGetSymbolAddress(scratch): "
+                                                + hipGetErrorString(e));
+        ptr[dev] = (unsigned long long *)p;
+    }
+    *out = ptr[dev];
+    return BX_OK;
+}
+
+extern "C" int bx_imex_acoustic_dt_noalloc(const double *U, int nx, int ny, double dx,
+                                           double dy, double gamma, double eps, double cfl,
+                                           double *dt_dev, void *stream)
+{
+    const char *who = "bx_imex_acoustic_dt_noalloc";
+    if (!U || !dt_dev || !peek_sizes_ok(nx, ny) || !(dx > 0) || !(dy > 0))
+        return bx_helm_fail(BX_ERR_INVALID, std::string(who) + ": bad argument");
+    const long long n    = (long long)nx * ny;
+    const long long grid = std::min<long long>((n + 255) / 256, NOALLOC_PARTS);
+    if (grid < 1 || grid > NOALLOC_PARTS) // the grid-stride loop covers any n
+        return bx_helm_fail(BX_ERR_INVALID,
+                            std::string(who) + ": more partial maxima than the scratch holds");
+    unsigned long long *part = nullptr;
+    int                 rc;
+    if ((rc = noalloc_scratch(who, &part)) != BX_OK)
+        return rc;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_peek_speed_part<true>, dim3((unsigned)grid), dim3(256), 0, s, U, n, dx,
+                       dy, gamma, eps, part);
+    PEEK_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_acoustic_dt_from_parts, dim3(1), dim3(256), 0, s, part, (int)grid, dx,
+                       dy, cfl, dt_dev);
     PEEK_CHECK(hipGetLastError());
     return BX_OK;
 }

@@ -425,6 +425,16 @@ int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, double dy,
 int bx_imex_acoustic_dt(const double *U, int nx, int ny, double dx, double dy,
                         double gamma, double eps, double cfl, double *dt,
                         void *stream);
+/* No counterpart in the reference: bx_imex_acoustic_dt for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  Same time step, bit for bit, but written to the device
+ * double *dt_dev and asynchronous on `stream`; it calls no allocator.  Its
+ * scratch (one partial maximum per block, at most 1024 blocks: 8 KiB) is a
+ * file-scope device array that each GPU holds until the process ends, shared
+ * by every call on that GPU: calls that overlap on one GPU race. */
+int bx_imex_acoustic_dt_noalloc(const double *U, int nx, int ny, double dx,
+                                double dy, double gamma, double eps, double cfl,
+                                double *dt_dev, void *stream);
 /* First half of compute_flux_imex_o1 (Source/IMEX/IMEX_O1.cpp:18-131):
  * Rusanov advective update U -> Uex (advance_rusanov_adv,
  * Source/IMEX/IMEX_Fluxes.cpp:10-118, or its MUSCL form), enthalpy p gamma/((gamma-1) rho_ex)

@@ -43,6 +43,25 @@ class bx_plot_level(ctypes.Structure):
                 ("dx", ctypes.c_double * 3), ("level_steps", ctypes.c_int)]
 
 
+class bx_chk_level(ctypes.Structure):
+    _fields_ = [("boxes", ctypes.c_void_p), ("nbox", ctypes.c_int), ("ng", ctypes.c_int),
+                ("U", ctypes.c_void_p), ("domain_lo", ctypes.c_int * 3), ("domain_hi", ctypes.c_int * 3),
+                ("dx", ctypes.c_double * 3), ("level_steps", ctypes.c_int), ("dt", ctypes.c_double)]
+
+
+class bx_chk_info(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("nlev", ctypes.c_int), ("ref_ratio", ctypes.c_int),
+                ("time", ctypes.c_double), ("prob_lo", ctypes.c_double * 3),
+                ("prob_hi", ctypes.c_double * 3)]
+
+
+class bx_chk_level_info(ctypes.Structure):
+    _fields_ = [("nbox", ctypes.c_int), ("ng", ctypes.c_int), ("level_steps", ctypes.c_int),
+                ("dt", ctypes.c_double), ("domain_lo", ctypes.c_int * 3),
+                ("domain_hi", ctypes.c_int * 3), ("dx", ctypes.c_double * 3),
+                ("ndoubles", ctypes.c_int64), ("checksum", ctypes.c_uint64)]
+
+
 class bx_geometry(ctypes.Structure):
     _fields_ = [
         ("domain_lo", ctypes.c_int * 3),
@@ -122,6 +141,12 @@ _SIGS = {
     "bx_helm_set_graph": (ctypes.c_int, [P, ctypes.c_int]),
     "bx_helm_set_fused": (ctypes.c_int, [P, ctypes.c_int]),
     "bx_helm_solve": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, P, P]),
+    "bx_write_checkpoint": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P, ctypes.c_int,
+                                           DP, DP, ctypes.c_double, P]),
+    "bx_read_checkpoint_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(bx_chk_info), P,
+                                               ctypes.c_int]),
+    "bx_read_checkpoint_boxes": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, P, ctypes.c_int]),
+    "bx_read_checkpoint_level": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, P, ctypes.c_int64, P]),
     "bx_write_plotfile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P, ctypes.c_int, DP, DP,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "bx_amr_create": (ctypes.c_int, [P, P, ctypes.POINTER(P)]),

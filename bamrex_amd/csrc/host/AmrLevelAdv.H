@@ -44,6 +44,7 @@ class AmrLevelAdv
 
     void read_params(const ParmParse &pp_root);
     void define(int max_grid_size, int ng = 2);
+    void define(const std::vector<bx_box> &boxes, int ng);
     void initData(const ParmParse &pp_root);
 
     double estTimeStep(double cur_time);
@@ -57,6 +58,8 @@ class AmrLevelAdv
     void download(std::vector<double> &U) const;
     void upload(const std::vector<double> &U);
     void writePlotFile(const std::string &path, double time, int step) const;
+    void checkPoint(const std::string &path, double time, int step, double dt_next) const;
+    void restart(const std::string &path, double &time, int &step);
 
     const BoxLayout &layout() const { return m_layout; }
     const Geometry  &geom() const { return m_geom; }

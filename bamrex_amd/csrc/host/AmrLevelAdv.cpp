@@ -167,10 +167,7 @@ void AmrLevelAdv::read_params(const ParmParse &root)
 void AmrLevelAdv::define(int max_grid_size, int ng)
 {
     const int dim = m_geom.dim;
-    m_layout.dim  = dim;
-    m_layout.ng   = ng;
-    m_layout.nc   = dim + 2;
-    m_layout.boxes.clear();
+    std::vector<bx_box> boxes;
     std::vector<std::vector<std::pair<int, int>>> r(3);
     for (int d = 0; d < 3; ++d)
     {
@@ -190,8 +187,19 @@ void AmrLevelAdv::define(int max_grid_size, int ng)
                 b.lo[0] = x.first;  b.hi[0] = x.second;
                 b.lo[1] = y.first;  b.hi[1] = y.second;
                 b.lo[2] = z.first;  b.hi[2] = z.second;
-                m_layout.boxes.push_back(b);
+                boxes.push_back(b);
             }
+    define(boxes, ng);
+}
+
+// The level on a given box list (a restart takes the checkpoint's)
+void AmrLevelAdv::define(const std::vector<bx_box> &boxes, int ng)
+{
+    const int dim = m_geom.dim;
+    m_layout.dim  = dim;
+    m_layout.ng   = ng;
+    m_layout.nc   = dim + 2;
+    m_layout.boxes = boxes;
     std::vector<int64_t> offs(m_layout.boxes.size());
     check(bx_pack_offsets(dim, ng, m_layout.boxes.data(), (int)offs.size(),
                           offs.data(), &m_layout.total),
@@ -335,8 +343,57 @@ int AmrLevelAdv::run(double stop_time, int max_step, double &time)
     check(bx_run(m_ctx, m_geom.dx, cfl, time, stop_time, max_step, m_S1, m_S2,
                  1, m_stream, &t_out, &steps, &in_tmp),
           "bx_run");
+    if (in_tmp) // an odd number of sweeps in graph mode ends in the second buffer
+        std::swap(m_S1, m_S2);
     time = t_out;
     return steps;
+}
+
+// AmrLevel::checkPoint (AmrLevelAdv.cpp:108-113): time, step, the dt of the
+// next step, boxes and the state with its ghost cells, written by the library
+void AmrLevelAdv::checkPoint(const std::string &path, double time, int step,
+                             double dt_next) const
+{
+    check(bx_stream_sync(m_stream), "sync");
+    bx_chk_level lv{};
+    lv.boxes = m_layout.boxes.data();
+    lv.nbox  = (int)m_layout.boxes.size();
+    lv.ng    = m_layout.ng;
+    lv.U     = m_S1;
+    for (int d = 0; d < 3; ++d)
+    {
+        lv.domain_lo[d] = 0;
+        lv.domain_hi[d] = d < m_geom.dim ? m_geom.n_cell[d] - 1 : 0;
+        lv.dx[d]        = d < m_geom.dim ? m_geom.dx[d] : 0.0;
+    }
+    lv.level_steps = step;
+    lv.dt          = dt_next;
+    check(bx_write_checkpoint(path.c_str(), m_geom.dim, 1, &lv, 2, m_geom.prob_lo,
+                              m_geom.prob_hi, time, m_stream),
+          "bx_write_checkpoint");
+}
+
+// AmrLevelAdv::restart (AmrLevelAdv.cpp:95-106): the level on the
+// checkpoint's boxes with its state; returns the checkpoint's time and step
+void AmrLevelAdv::restart(const std::string &path, double &time, int &step)
+{
+    bx_chk_info       info{};
+    bx_chk_level_info li{};
+    check(bx_read_checkpoint_info(path.c_str(), &info, &li, 1), "bx_read_checkpoint_info");
+    if (info.dim != m_geom.dim)
+        throw std::runtime_error("restart: the checkpoint is " + std::to_string(info.dim)
+                                 + "D, the inputs " + std::to_string(m_geom.dim) + "D");
+    for (int d = 0; d < m_geom.dim; ++d)
+        if (li.domain_hi[d] - li.domain_lo[d] + 1 != m_geom.n_cell[d])
+            throw std::runtime_error("restart: the checkpoint's domain differs from amr.n_cell");
+    std::vector<bx_box> boxes(li.nbox);
+    check(bx_read_checkpoint_boxes(path.c_str(), 0, boxes.data(), li.nbox),
+          "bx_read_checkpoint_boxes");
+    define(boxes, li.ng);
+    check(bx_read_checkpoint_level(path.c_str(), 0, m_S1, m_layout.total, m_stream),
+          "bx_read_checkpoint_level");
+    time = info.time;
+    step = li.level_steps;
 }
 
 // AmrLevel::writePlotFile (AmrLevelAdv.cpp:116-136): an AMReX plotfile

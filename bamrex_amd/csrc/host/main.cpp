@@ -6,6 +6,12 @@
 // Plotfiles as the reference's Amr writes them: amr.plot_file (root name,
 // default "plt") + the step in 5 digits, at the start, every amr.plot_int
 // steps and at the end; amr.plot_files_output = 0 turns them off.
+// Checkpoints likewise: amr.check_file (default "chk") + the step, every
+// amr.check_int steps (and at the start of a new run when check_int > 0), and
+// the final one of Source/main.cpp:97-100 unless
+// amr.checkpoint_files_output = 0.  amr.restart = <checkpoint directory>
+// continues a run from a checkpoint (AmrLevelAdv::restart,
+// Source/AmrLevelAdv.cpp:95-106) instead of from the initial data.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -44,49 +50,81 @@ int main(int argc, char **argv)
         ParmParse pa("amr");
         int       mgs = level.geom().dim == 3 ? 32 : 128;
         pa.query("max_grid_size", mgs);
-        level.define(mgs);
-        level.initData(pp);
 
-        std::string plot_root = "plt";
-        int         plot_int = -1, plot_on = 1;
+        std::string plot_root = "plt", check_root = "chk", restart_dir;
+        int         plot_int = -1, plot_on = 1, check_int = -1, check_on = 1;
         pa.query("plot_file", plot_root);
         pa.query("plot_int", plot_int);
         pa.query("plot_files_output", plot_on);
-        const bool plots = plot_on != 0 && plot_int > 0;
-        auto       plot_name = [&](int step) {
+        pa.query("check_file", check_root);
+        pa.query("check_int", check_int);
+        pa.query("checkpoint_files_output", check_on);
+        pa.query("restart", restart_dir);
+        const bool plots  = plot_on != 0 && plot_int > 0;
+        const bool checks = check_on != 0 && check_int > 0;
+        auto       name   = [](const std::string &root, int step) {
             char b[16];
             std::snprintf(b, sizeof b, "%05d", step);
-            return plot_root + b;
+            return root + b;
         };
 
         double time  = strt_time;
-        int    steps = 0, last_plot = -1;
-        if (plots)
+        int    steps = 0, last_plot = -1, last_check = -1;
+        if (!restart_dir.empty())
         {
-            level.writePlotFile(plot_name(0), time, 0);
-            last_plot = 0;
+            level.restart(restart_dir, time, steps);
+            last_plot = last_check = steps;
+            std::printf("Restarted from %s at step %d, time %.17g\n", restart_dir.c_str(),
+                        steps, time);
+        }
+        else
+        {
+            level.define(mgs);
+            level.initData(pp);
+            if (plots)
+            {
+                level.writePlotFile(name(plot_root, 0), time, 0);
+                last_plot = 0;
+            }
+            if (checks)
+            {
+                level.checkPoint(name(check_root, 0), time, 0,
+                                 level.computeNewDt(time, stop_time));
+                last_check = 0;
+            }
         }
         while ((steps < max_step || max_step < 0) && (time < stop_time || stop_time < 0.0))
         {
-            // run up to the next plot step (or to the end)
+            // run up to the next plot or checkpoint step (or to the end)
             int chunk = max_step < 0 ? -1 : max_step - steps;
-            if (plots)
-            {
-                const int to_plot = plot_int - steps % plot_int;
-                chunk             = chunk < 0 ? to_plot : std::min(chunk, to_plot);
-            }
+            for (int every : { plots ? plot_int : 0, checks ? check_int : 0 })
+                if (every > 0)
+                {
+                    const int to_next = every - steps % every;
+                    chunk             = chunk < 0 ? to_next : std::min(chunk, to_next);
+                }
             const int done = level.run(stop_time, chunk, time);
             steps += done;
             if (plots && steps % plot_int == 0 && steps != last_plot)
             {
-                level.writePlotFile(plot_name(steps), time, steps);
+                level.writePlotFile(name(plot_root, steps), time, steps);
                 last_plot = steps;
+            }
+            if (checks && steps % check_int == 0 && steps != last_check)
+            {
+                level.checkPoint(name(check_root, steps), time, steps,
+                                 level.computeNewDt(time, stop_time));
+                last_check = steps;
             }
             if (done == 0)
                 break;
         }
+        // the final checkpoint and plotfile (Source/main.cpp:97-105)
+        if (check_on != 0 && last_check != steps)
+            level.checkPoint(name(check_root, steps), time, steps,
+                             level.computeNewDt(time, stop_time));
         if (plots && last_plot != steps)
-            level.writePlotFile(plot_name(steps), time, steps);
+            level.writePlotFile(name(plot_root, steps), time, steps);
         const double run_time
             = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
                   .count();

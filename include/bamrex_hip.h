@@ -241,6 +241,65 @@ int bx_write_plotfile(const char *dir, int dim, int nlev,
                       double time, double gamma, double eps, int derived);
 
 /* ==========================================================================
+ * Checkpoint / restart
+ * ==========================================================================
+ * AmrLevelAdv::checkPoint / restart (Source/AmrLevelAdv.cpp:95-113, through
+ * AMReX's AmrLevel::checkPoint / restart) and the final checkpoint of
+ * Source/main.cpp:97-100: what a restart needs to continue bit for bit --
+ * the time, each level's step count, dt, box list and state.  A directory
+ * holds a text Header and per level Level_<l>/State, the level's packed
+ * state (bx_pack_offsets layout, ghost cells included) as little-endian
+ * doubles with an FNV-1a checksum in the header.  Host code; the state
+ * pointers are device pointers; synchronous on `stream`. */
+typedef struct
+{
+    const bx_box *boxes;
+    int           nbox;
+    int           ng;
+    const double *U;            /* device: the packed level state */
+    int           domain_lo[3];
+    int           domain_hi[3];
+    double        dx[3];
+    int           level_steps;
+    double        dt;
+} bx_chk_level;
+typedef struct
+{
+    int    dim;
+    int    nlev;
+    int    ref_ratio;
+    double time;
+    double prob_lo[3];
+    double prob_hi[3];
+} bx_chk_info;
+typedef struct
+{
+    int      nbox;
+    int      ng;
+    int      level_steps;
+    double   dt;
+    int      domain_lo[3];
+    int      domain_hi[3];
+    double   dx[3];
+    int64_t  ndoubles;
+    uint64_t checksum;
+} bx_chk_level_info;
+int bx_write_checkpoint(const char *dir, int dim, int nlev,
+                        const bx_chk_level *levels, int ref_ratio,
+                        const double prob_lo[3], const double prob_hi[3],
+                        double time, void *stream);
+/* The header: run data and up to max_levels levels' data. */
+int bx_read_checkpoint_info(const char *dir, bx_chk_info *info,
+                            bx_chk_level_info *levels, int max_levels);
+/* Level `level`'s box list (nbox must be the level's box count). */
+int bx_read_checkpoint_boxes(const char *dir, int level, bx_box *boxes,
+                             int nbox);
+/* Level `level`'s state into the device buffer U of ndoubles doubles (the
+ * level's packed size), checked against the header's checksum. */
+int bx_read_checkpoint_level(const char *dir, int level, double *U,
+                             int64_t ndoubles, void *stream);
+
+/* ==========================================================================
  * Two AMR levels (refinement ratio 2, fine level subcycled)
  * ==========================================================================
  * What AMReX's Amr / FluxRegister / FillPatch machinery does around

@@ -38,7 +38,8 @@ def read_final_plotfile(tmp_path):
 
 def run_host(tmp_path, inputs, *overrides, plot_int=1000):
     cmd = [HOST, os.path.join(INPUTS, inputs)] + list(overrides) + [
-        "amr.plot_file=" + str(tmp_path / "plt"), "amr.plot_int=%d" % plot_int]
+        "amr.plot_file=" + str(tmp_path / "plt"), "amr.plot_int=%d" % plot_int,
+        "amr.check_file=" + str(tmp_path / "chk")]
     announce("host " + " ".join(cmd[1:]))
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr

@@ -23,7 +23,7 @@ OBJ = os.path.join(HERE, "_obj")
 HIP_SOURCES = ["bx_explicit.hip", "bx_sweep2d.hip", "bx_sweep3d.hip", "bx_api.hip",
                "bx_helmholtz.hip", "bx_cnn.hip", "bx_imex.hip", "bx_amr.hip", "bx_plotfile.hip",
                "bx_checkpoint.hip"]
-HOST_SOURCES = ["ParmParse.cpp", "AmrLevelAdv.cpp", "Prob.cpp", "main.cpp"]
+HOST_SOURCES = ["ParmParse.cpp", "AmrLevelAdv.cpp", "ImexLevel.cpp", "Prob.cpp", "main.cpp"]
 HOST_BIN = os.path.join(HERE, "bamrex_run")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -145,6 +145,7 @@ template bool ParmParse::query<std::string>(const std::string &, std::string &) 
 template void ParmParse::get<int>(const std::string &, int &) const;
 template void ParmParse::get<double>(const std::string &, double &) const;
 template void ParmParse::get<std::string>(const std::string &, std::string &) const;
+template void ParmParse::get<bool>(const std::string &, bool &) const;
 template bool ParmParse::queryarr<int>(const std::string &, std::vector<int> &) const;
 template bool ParmParse::queryarr<double>(const std::string &, std::vector<double> &) const;
 template bool ParmParse::queryarr<std::string>(const std::string &,

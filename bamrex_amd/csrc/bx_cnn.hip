@@ -55,6 +55,11 @@ constexpr int NW1 = 9 * C, NW2 = 9 * C * C, NW3 = 9 * C * C, NW4 = 9 * C;
 constexpr int OFF1 = 0, OFF2 = NW1, OFF3 = OFF2 + NW2, OFF4 = OFF3 + NW3;
 constexpr int NPAR = OFF4 + NW4;
 static_assert(NPAR == BX_CNN_NPARAMS, "parameter count");
+// training reductions: pixels per weight-gradient block, the row of one
+// block's partials (W3 | W2 | W4 | W1), blocks of the loss sum
+constexpr int WG_CH       = 128;
+constexpr int WG_ROW      = 2 * NW2 + NW4 + NW1;
+constexpr int LOSS_BLOCKS = 2048;
 
 __host__ __device__ inline unsigned short f2bf(float f)
 {
@@ -456,12 +461,13 @@ __global__ __launch_bounds__(256) void k_crop_e(CropSet c, OpRef o,
     }
 }
 
-// res = mask (r - A e); loss contribution res^2 / ||mask r||^2
+// res = mask (r - A e); loss contribution res^2 / ||mask r||^2, one partial
+// sum per block (summed in a fixed order by k_sum_partials_f64)
 __global__ __launch_bounds__(256) void k_crop_res(CropSet c, OpRef o,
                                                   const double *__restrict__ e,
                                                   const double *__restrict__ rn2,
                                                   double *__restrict__ res,
-                                                  double *__restrict__ loss,
+                                                  double *__restrict__ loss_part,
                                                   int batch, int border)
 {
     const long long n = (long long)batch * c.cs * c.cs;
@@ -483,10 +489,27 @@ __global__ __launch_bounds__(256) void k_crop_res(CropSet c, OpRef o,
         }
         res[i] = v;
     }
+    __shared__ double red[4];
     for (int off = 32; off > 0; off >>= 1)
         acc += __shfl_xor(acc, off);
-    if ((threadIdx.x & 63) == 0 && acc != 0)
-        atomicAdd(loss, acc);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        loss_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// out = sum of n partials, one thread, in index order
+__global__ void k_sum_partials_f64(const double *__restrict__ part, int n,
+                                   double *__restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+    {
+        double s = 0;
+        for (int i = 0; i < n; ++i)
+            s += part[i];
+        *out = s;
+    }
 }
 
 // g_y = dL/dy = -2 A^T(res / ||mask r||^2) * sigma / D   (A symmetric)
@@ -583,12 +606,16 @@ __global__ __launch_bounds__(256) void k_bwd16_data(const float *__restrict__ go
         gin[p * C + k] = bf2f(hin[p * C + k]) > 0.f ? acc[k] : 0.f;
 }
 
-// weight gradient of a 16->16 layer: dW[t][n][k] += sum_p gout[p][n] hin[p+off_t][k]
-// block: 256 threads, thread = (n, k), 9 taps each; chunk of CH pixels
+// weight gradient of a 16->16 layer: dW[t][n][k] = sum_p gout[p][n] hin[p+off_t][k]
+// block: 256 threads, thread = (n, k), 9 taps each; chunk of CH pixels, in
+// pixel order; the block's sums go to its own slot of `part` (layout
+// [block][t][n][k]) and k_sum_partials_f32 adds the blocks in block order,
+// so the gradient is the same on every run
 template <int CH>
 __global__ __launch_bounds__(256) void k_bwd16_weight(const float *__restrict__ gout,
                                                       const unsigned short *__restrict__ hin,
-                                                      float *__restrict__ dW, Geo g)
+                                                      float *__restrict__ part, int stride,
+                                                      Geo g)
 {
     const long long npx = (long long)g.w * g.h * g.nimg;
     const int       n = threadIdx.x >> 4, k = threadIdx.x & 15;
@@ -614,21 +641,37 @@ __global__ __launch_bounds__(256) void k_bwd16_weight(const float *__restrict__ 
                 acc[t] += gv * bf2f(hin[(long long)q * C + k]);
         }
     }
+    float *dst = part + (long long)blockIdx.x * stride;
 #pragma unroll
     for (int t = 0; t < 9; ++t)
-        if (acc[t] != 0.f)
-            atomicAdd(&dW[(t * C + n) * C + k], acc[t]);
+        dst[(t * C + n) * C + k] = acc[t];
+}
+
+// out[i] = sum over the nblk partial rows of part[b * stride + i], i < nval,
+// in block order
+__global__ __launch_bounds__(256) void k_sum_partials_f32(const float *__restrict__ part,
+                                                          int nblk, int stride, int nval,
+                                                          float *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nval)
+        return;
+    float s = 0.f;
+    for (int b = 0; b < nblk; ++b)
+        s += part[(long long)b * stride + i];
+    out[i] = s;
 }
 
 // weight gradient of layer 4 (dW4[t][k] += sum_p g_y[p] h3[p+off_t][k]) and
 // layer 1 (dW1[t][n] += sum_p g_h1[p][n] x[p+off_t]); thread = (t, c)
+// (per-block partials, layout [block][W4 144 | W1 144], as for the 16->16 layers)
 template <int CH>
 __global__ __launch_bounds__(320) void k_bwd_weight_small(const float *__restrict__ gy,
                                                           const unsigned short *__restrict__ h3,
                                                           const float *__restrict__ gh1,
                                                           const unsigned short *__restrict__ x,
-                                                          float *__restrict__ dW4,
-                                                          float *__restrict__ dW1, Geo g)
+                                                          float *__restrict__ part, int stride,
+                                                          Geo g)
 {
     const long long npx = (long long)g.w * g.h * g.nimg;
     const int       tid = threadIdx.x; // 0..143 for W4, 144 threads for W1 (second block half)
@@ -653,8 +696,7 @@ __global__ __launch_bounds__(320) void k_bwd_weight_small(const float *__restric
         else
             acc += gh1[p * C + c] * bf2f(x[q]);
     }
-    if (acc != 0.f)
-        atomicAdd(w4 ? &dW4[idx] : &dW1[idx], acc);
+    part[(long long)blockIdx.x * stride + tid] = acc;
 }
 
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ gr,
@@ -940,7 +982,9 @@ struct bx_cnn
     long long tcap = 0;
     float    *gy = nullptr, *g3 = nullptr, *g2 = nullptr, *g1 = nullptr;
     double   *te = nullptr, *tres = nullptr, *tsig = nullptr, *trn2 = nullptr,
-           *tloss = nullptr;
+           *tloss = nullptr; // [0]: the loss, [1 ..]: per-block partials
+    float    *wpart = nullptr; // weight-gradient partials, one row per block
+    long long wpcap = 0;
     int      *txy = nullptr;
     int       txycap = 0;
 };
@@ -978,7 +1022,7 @@ extern "C" void bx_cnn_destroy(bx_cnn *c)
                      (void *)c->y, (void *)c->P, (void *)c->sig, (void *)c->part, (void *)c->gy,
                      (void *)c->g3, (void *)c->g2, (void *)c->g1, (void *)c->te,
                      (void *)c->tres, (void *)c->tsig, (void *)c->trn2,
-                     (void *)c->tloss, (void *)c->txy })
+                     (void *)c->tloss, (void *)c->txy, (void *)c->wpart })
         cfree(p);
     delete c;
 }
@@ -1236,6 +1280,14 @@ static int ensure_train(bx_cnn *c, long long npx, int batch)
         C_CHECK(hipMalloc((void **)&c->tres, npx * 8));
         c->tcap = npx;
     }
+    const long long nblk = (npx + WG_CH - 1) / WG_CH;
+    if (nblk > c->wpcap)
+    {
+        cfree(c->wpart);
+        c->wpart = nullptr;
+        C_CHECK(hipMalloc((void **)&c->wpart, nblk * WG_ROW * 4));
+        c->wpcap = nblk;
+    }
     if (batch > c->txycap)
     {
         for (void *p : { (void *)c->tsig, (void *)c->trn2, (void *)c->txy, (void *)c->tloss })
@@ -1243,13 +1295,15 @@ static int ensure_train(bx_cnn *c, long long npx, int batch)
         C_CHECK(hipMalloc((void **)&c->tsig, batch * 8));
         C_CHECK(hipMalloc((void **)&c->trn2, batch * 8));
         C_CHECK(hipMalloc((void **)&c->txy, batch * 12));
-        C_CHECK(hipMalloc((void **)&c->tloss, 64));
+        C_CHECK(hipMalloc((void **)&c->tloss, (1 + LOSS_BLOCKS) * 8));
         c->txycap = batch;
     }
     return ensure_infer(c, npx);
 }
 
 // One loss + gradient evaluation on the crops `xy` (device, batch x 3 ints).
+// Every sum is formed in a fixed order (per-block partials, then the blocks
+// in block order), so the loss and the gradient are the same on every run.
 static int loss_grad(bx_cnn *c, const double *R, int nfields, int nx, int ny,
                      int batch, int cs, double *loss_host, hipStream_t s)
 {
@@ -1266,30 +1320,40 @@ static int loss_grad(bx_cnn *c, const double *R, int nfields, int nx, int ny,
     net_forward<false>(c, g, s);
     hipLaunchKernelGGL(k_crop_e, dim3(2048), dim3(256), 0, s, cset, c->opr, c->tsig,
                        c->y, c->te, batch);
-    C_CHECK(hipMemsetAsync(c->tloss, 0, 8, s));
-    hipLaunchKernelGGL(k_crop_res, dim3(2048), dim3(256), 0, s, cset, c->opr, c->te,
-                       c->trn2, c->tres, c->tloss, batch, border);
+    hipLaunchKernelGGL(k_crop_res, dim3(LOSS_BLOCKS), dim3(256), 0, s, cset, c->opr, c->te,
+                       c->trn2, c->tres, c->tloss + 1, batch, border);
+    hipLaunchKernelGGL(k_sum_partials_f64, dim3(1), dim3(64), 0, s, c->tloss + 1, LOSS_BLOCKS,
+                       c->tloss);
     const float inv_b = 1.0f / (float)batch;
     hipLaunchKernelGGL(k_crop_gy, dim3(2048), dim3(256), 0, s, cset, c->opr, c->tres,
                        c->trn2, c->tsig, c->gy, batch, inv_b);
-    C_CHECK(hipMemsetAsync(c->grad, 0, NPAR * 4, s));
+    const unsigned nblk = (unsigned)((npx + WG_CH - 1) / WG_CH);
+    float *const   p3 = c->wpart, *const p2 = c->wpart + NW2, *const p14 = c->wpart + 2 * NW2;
     // layer 4
     hipLaunchKernelGGL(k_bwd4_data, dim3(nb), dim3(256), 0, s, c->gy, c->h3,
                        c->wr + OFF4, c->g3, g);
     // layer 3: g3 is dL/d(h3 pre-activation); weights W3 map h2 -> h3
     hipLaunchKernelGGL(k_bwd16_data, dim3(nb), dim3(256), 0, s, c->g3, c->h2,
                        c->wr + OFF3, c->g2, g);
-    hipLaunchKernelGGL((k_bwd16_weight<1024>), dim3((unsigned)((npx + 1023) / 1024)),
-                       dim3(256), 0, s, c->g3, c->h2, c->grad + OFF3, g);
+    hipLaunchKernelGGL((k_bwd16_weight<WG_CH>), dim3(nblk), dim3(256), 0, s, c->g3, c->h2,
+                       p3, WG_ROW, g);
     // layer 2: W2 maps h1 -> h2
     hipLaunchKernelGGL(k_bwd16_data, dim3(nb), dim3(256), 0, s, c->g2, c->h1,
                        c->wr + OFF2, c->g1, g);
-    hipLaunchKernelGGL((k_bwd16_weight<1024>), dim3((unsigned)((npx + 1023) / 1024)),
-                       dim3(256), 0, s, c->g2, c->h1, c->grad + OFF2, g);
+    hipLaunchKernelGGL((k_bwd16_weight<WG_CH>), dim3(nblk), dim3(256), 0, s, c->g2, c->h1,
+                       p2, WG_ROW, g);
     // layers 4 and 1 weights
-    hipLaunchKernelGGL((k_bwd_weight_small<1024>), dim3((unsigned)((npx + 1023) / 1024)),
-                       dim3(288), 0, s, c->gy, c->h3, c->g1, c->x, c->grad + OFF4,
-                       c->grad + OFF1, g);
+    hipLaunchKernelGGL((k_bwd_weight_small<WG_CH>), dim3(nblk), dim3(288), 0, s, c->gy, c->h3,
+                       c->g1, c->x, p14, WG_ROW, g);
+    // the blocks' partials in block order
+    hipLaunchKernelGGL(k_sum_partials_f32, dim3((NW2 + 255) / 256), dim3(256), 0, s, p3,
+                       (int)nblk, WG_ROW, NW2, c->grad + OFF3);
+    hipLaunchKernelGGL(k_sum_partials_f32, dim3((NW2 + 255) / 256), dim3(256), 0, s, p2,
+                       (int)nblk, WG_ROW, NW2, c->grad + OFF2);
+    hipLaunchKernelGGL(k_sum_partials_f32, dim3(1), dim3(256), 0, s, p14, (int)nblk, WG_ROW,
+                       NW4, c->grad + OFF4);
+    hipLaunchKernelGGL(k_sum_partials_f32, dim3(1), dim3(256), 0, s, p14 + NW4, (int)nblk,
+                       WG_ROW, NW1, c->grad + OFF1);
     C_CHECK(hipGetLastError());
     double l = 0;
     C_CHECK(hipMemcpyAsync(&l, c->tloss, 8, hipMemcpyDeviceToHost, s));

@@ -295,6 +295,34 @@ def test_training_reduces_loss_and_fallback_on_garbage():
     net.close()
 
 
+def test_training_is_reproducible():
+    """Every training sum (loss, weight gradients) is formed in a fixed
+    order, with no floating-point atomics: two trainings from the same seed
+    on the same fields give the same loss history and weights, bit for
+    bit."""
+    announce("cnn training reproducibility 96")
+    n = 96
+    rng = np.random.default_rng(12)
+    a, bX, bY = rand_coeffs(rng, n, n)
+    s = HelmholtzSolver(n, n, 1.0 / n)
+    s.set_coeffs(1.0, 2e-3, a, bX, bY)
+    dR = dev(rng.normal(size=(3, n, n)))
+    runs = []
+    for _ in range(2):
+        net = LearnedSmoother(seed=6)
+        s.attach(net)
+        hist = net.train(dR, 3, steps=25, crop=32, batch=8, lr=2e-3, seed=13)
+        runs.append((np.array(hist), net.get_params()))
+        s.attach(None)
+        net.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    fresh = LearnedSmoother(seed=6)
+    assert not np.array_equal(runs[0][1], fresh.get_params())
+    fresh.close()
+    s.close()
+
+
 def test_graph_replay_matches_direct_launches():
     announce("helm fgmres graph vs direct 128")
     n = 128

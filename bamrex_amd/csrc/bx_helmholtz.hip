@@ -773,6 +773,99 @@ __global__ __launch_bounds__(256) void k_add_combo(double *__restrict__ x,
     }
 }
 
+// FGMRES least-squares state on the device: the Hessenberg matrix H
+// ((m+1) x m, row stride m), the Givens rotations (cs, sn), the rotated
+// right-hand side g, the solution y of the triangular system, and a status
+// word the host reads (est = |g[j+1]|, the new column's norm, and whether y
+// came out non-finite).
+struct Arnoldi
+{
+    double *H, *cs, *sn, *g, *y, *status;
+    int     m;
+};
+
+// g = beta e_0
+__global__ void k_arnoldi_init(Arnoldi a, double beta)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        for (int i = 0; i <= a.m; ++i)
+            a.g[i] = i == 0 ? beta : 0.0;
+}
+
+// Column j of H from the two Gram-Schmidt passes (d1 + d2) and the new
+// vector's squared norm d3, its previous rotations, a new rotation that
+// zeroes H[j+1][j], and g rotated with it: one thread, in the order of a
+// textbook Givens update (Saad, Iterative Methods, Alg. 6.9).
+__global__ void k_arnoldi_step(Arnoldi a, const double *__restrict__ d1,
+                               const double *__restrict__ d2,
+                               const double *__restrict__ d3, int j)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const int m  = a.m;
+    double   *H  = a.H;
+    for (int i = 0; i <= j; ++i)
+        H[i * m + j] = d1[i] + d2[i];
+    const double hn    = sqrt(d3[0]);
+    H[(j + 1) * m + j] = hn;
+    for (int i = 0; i < j; ++i)
+    {
+        const double t0    = a.cs[i] * H[i * m + j] + a.sn[i] * H[(i + 1) * m + j];
+        const double t1    = -a.sn[i] * H[i * m + j] + a.cs[i] * H[(i + 1) * m + j];
+        H[i * m + j]       = t0;
+        H[(i + 1) * m + j] = t1;
+    }
+    const double den   = hypot(H[j * m + j], H[(j + 1) * m + j]);
+    a.cs[j]            = den > 0 ? H[j * m + j] / den : 1.0;
+    a.sn[j]            = den > 0 ? H[(j + 1) * m + j] / den : 0.0;
+    H[j * m + j]       = den;
+    H[(j + 1) * m + j] = 0;
+    a.g[j + 1]         = -a.sn[j] * a.g[j];
+    a.g[j]             = a.cs[j] * a.g[j];
+    a.status[0]        = fabs(a.g[j + 1]);
+    a.status[1]        = hn;
+}
+
+// y = R^-1 g over the first j columns (back substitution); a non-finite y
+// is replaced by zeros (the iterate is kept) and flagged in status[2]
+__global__ void k_arnoldi_solve(Arnoldi a, int j)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const int m      = a.m;
+    bool      finite = true;
+    for (int i = j - 1; i >= 0; --i)
+    {
+        double t = a.g[i];
+        for (int q = i + 1; q < j; ++q)
+            t -= a.H[i * m + q] * a.y[q];
+        a.y[i] = t / a.H[i * m + i];
+        finite = finite && isfinite(a.y[i]);
+    }
+    if (!finite)
+        for (int i = 0; i < j; ++i)
+            a.y[i] = 0.0;
+    a.status[2] = finite ? 0.0 : 1.0;
+}
+
+// doubles of the least-squares state for restart length up to m_max
+static long long arnoldi_doubles(int m_max)
+{
+    return (long long)(m_max + 1) * m_max + 3LL * m_max + (m_max + 1) + 4;
+}
+static Arnoldi arnoldi_view(double *buf, int m, int m_max)
+{
+    Arnoldi a;
+    a.m      = m;
+    a.H      = buf;
+    a.cs     = a.H + (long long)(m_max + 1) * m_max;
+    a.sn     = a.cs + m_max;
+    a.g      = a.sn + m_max;
+    a.y      = a.g + (m_max + 1);
+    a.status = a.y + m_max;
+    return a;
+}
+
 // ------------------------------------------------------------ host class
 static dim3 grid2(int nx, int ny, int bs = 256)
 {
@@ -839,6 +932,9 @@ struct bx_helm
     double *V = nullptr, *Z = nullptr, *w = nullptr, *r = nullptr,
            *partial = nullptr, *dots = nullptr, *hdev = nullptr;
     int     nb_dot = 512;
+    // the least-squares state (Arnoldi) lives in arn; the host reads the
+    // status words through hstat (pinned)
+    double *arn = nullptr, *hstat = nullptr;
     // learned smoother
     bx_cnn *cnn       = nullptr;
     double  cnn_omega = 1.0;
@@ -1126,6 +1222,24 @@ static int norm2(bx_helm *h, const double *v, hipStream_t s, double *out)
     return rc;
 }
 
+// ||u||_2 and ||v||_2 with one host synchronisation
+static int norm2_pair(bx_helm *h, const double *u, const double *v, hipStream_t s,
+                      double *nu, double *nv)
+{
+    hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, u, u, h->n, 1,
+                       h->partial);
+    hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, v, v, h->n, 1,
+                       h->partial + h->nb_dot);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(2), dim3(256), 0, s, h->partial, h->nb_dot,
+                       2, h->dots);
+    H_CHECK(hipGetLastError());
+    H_CHECK(hipMemcpyAsync(h->hstat, h->dots, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    H_CHECK(hipStreamSynchronize(s));
+    *nu = std::sqrt(h->hstat[0]);
+    *nv = std::sqrt(h->hstat[1]);
+    return BX_OK;
+}
+
 // r = b - A x at FP64
 static void residual64(bx_helm *h, const double *b, const double *x, double *r,
                        hipStream_t s)
@@ -1161,10 +1275,17 @@ extern "C" int bx_helm_create(int nx, int ny, double dx, double dy,
         || (rc = halloc(&h->fr, h->n)) || (rc = halloc(&h->fe, h->n))
         || (rc = halloc(&h->partial, (long long)h->nb_dot * (h->m_max + 1)))
         || (rc = halloc(&h->dots, 2 * (h->m_max + 1) + 2)) || (rc = halloc(&h->hdev, h->m_max + 1))
-        || (rc = halloc(&h->pin, h->n)) || (rc = halloc(&h->pout, h->n)))
+        || (rc = halloc(&h->pin, h->n)) || (rc = halloc(&h->pout, h->n))
+        || (rc = halloc(&h->arn, arnoldi_doubles(h->m_max))))
     {
         bx_helm_destroy(h);
         return rc;
+    }
+    if (hipHostMalloc((void **)&h->hstat, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    {
+        h->hstat = nullptr;
+        bx_helm_destroy(h);
+        return herr(BX_ERR_HIP, "bx_helm_create: hipHostMalloc failed");
     }
     *out = h;
     return BX_OK;
@@ -1185,9 +1306,12 @@ extern "C" void bx_helm_destroy(bx_helm *h)
                      (void *)h->cY, (void *)h->V, (void *)h->Z, (void *)h->w,
                      (void *)h->r, (void *)h->partial, (void *)h->dots,
                      (void *)h->hdev, (void *)h->tmp1, (void *)h->tmp2,
-                     (void *)h->fr, (void *)h->fe, (void *)h->pin, (void *)h->pout })
+                     (void *)h->fr, (void *)h->fe, (void *)h->pin, (void *)h->pout,
+                     (void *)h->arn })
         if (p)
             (void)hipFree(p);
+    if (h->hstat)
+        (void)hipHostFree(h->hstat);
     delete h;
 }
 
@@ -1477,18 +1601,22 @@ static int precond_replay(bx_helm *h, const double *rv, double *e, hipStream_t s
 
 // Plain multigrid solve (MLMG::solve semantics, FP64): x <- x + V(b - A x)
 // until ||b - A x||_2 <= rtol ||b||_2.
+// rn0 >= 0: h->r already holds b - A x, of norm rn0.
 static int solve_mg(bx_helm *h, const double *b, double *x, double rtol,
                     int maxiter, double bnorm, bx_solve_stats *st,
-                    hipStream_t s, bool fused = false)
+                    hipStream_t s, bool fused = false, double rn0 = -1)
 {
     MGHierarchy<double> &mg = h->mg64;
     MGLevel<double>     &L0 = mg.lev[0];
     const long long      n  = h->n;
-    double               rn = 0;
+    double               rn = rn0;
     int                  rc;
-    residual64(h, b, x, h->r, s);
-    if ((rc = norm2(h, h->r, s, &rn)))
-        return rc;
+    if (rn0 < 0)
+    {
+        residual64(h, b, x, h->r, s);
+        if ((rc = norm2(h, h->r, s, &rn)))
+            return rc;
+    }
     int it = 0;
     while (rn > rtol * bnorm && it < maxiter)
     {
@@ -1511,43 +1639,59 @@ static int solve_mg(bx_helm *h, const double *b, double *x, double rtol,
 }
 
 // Flexible GMRES(m), right preconditioned, FP64 vectors and dot products,
-// classical Gram-Schmidt applied twice.  Stall detection: if the residual
-// has not dropped by 2x over `stall_window` iterations, or the iteration cap
-// is reached, the caller falls back to plain multigrid.
+// classical Gram-Schmidt applied twice.  The Hessenberg matrix, the Givens
+// rotations and the triangular solve stay on the device; the host reads two
+// status words per iteration (the residual estimate and the new column's
+// norm), the one synchronisation an iteration needs to decide whether to
+// go on.  Stall detection: if the residual has not dropped by 2x over 8
+// iterations, or the iteration cap is reached, the caller falls back to
+// plain multigrid.  rn0 >= 0: h->r already holds b - A x, of norm rn0.
 static int solve_fgmres(bx_helm *h, const double *b, double *x, double rtol,
                         int maxiter, double bnorm, bx_solve_stats *st,
-                        hipStream_t s, bool *stalled)
+                        hipStream_t s, bool *stalled, double rn0 = -1)
 {
     const long long n = h->n;
     const int       m = h->m;
-    std::vector<double> H((m + 1) * m, 0.0), cs(m), sn(m), g(m + 1),
-        hdots(2 * (m + 1) + 1), y(m);
-    *stalled      = false;
-    int    total  = 0;
-    double best   = 1e300;
-    int    best_at = 0;
+    const Arnoldi   A = arnoldi_view(h->arn, m, h->m_max);
+    *stalled          = false;
+    int    total      = 0;
+    double best       = 1e300;
+    int    best_at    = 0;
     int    rc;
+    bool   stop       = false; // a stall found inside the last cycle
+    int    pending_j  = 0;     // directions added by the last cycle
+    double beta       = rn0;
     while (true)
     {
-        residual64(h, b, x, h->r, s);
-        double beta = 0;
-        if ((rc = norm2(h, h->r, s, &beta)))
-            return rc;
+        if (beta < 0)
+        {
+            residual64(h, b, x, h->r, s);
+            if ((rc = norm2(h, h->r, s, &beta))) // also completes the status copy
+                return rc;
+        }
+        if (pending_j > 0)
+        {
+            if (h->hstat[2] != 0) // non-finite y: the iterate was kept
+                stop = true;
+            else
+                st->iters += pending_j;
+            pending_j = 0;
+        }
         st->rel_resid = beta / bnorm;
         if (beta <= rtol * bnorm)
         {
             st->converged = 1;
             break;
         }
-        if (total >= maxiter)
+        if (stop || total >= maxiter)
         {
             *stalled = true;
             break;
         }
         hipLaunchKernelGGL(k_scale, dim3(2048), dim3(256), 0, s, h->V, h->r, n,
                            1.0 / beta);
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0]  = beta;
+        hipLaunchKernelGGL(k_arnoldi_init, dim3(1), dim3(64), 0, s, A, beta);
+        beta  = -1;
         int j = 0;
         for (; j < m && total < maxiter; ++j)
         {
@@ -1559,7 +1703,7 @@ static int solve_fgmres(bx_helm *h, const double *b, double *x, double rtol,
             launch_apply(h->mg64.lev[0], h->mg64, zj, h->w,
                          (const double *)nullptr, s);
             // CGS2 chained on the device (the projections are read from
-            // device memory); one host synchronisation per iteration
+            // device memory)
             double *d1 = h->dots, *d2 = h->dots + (m + 1), *d3 = h->dots + 2 * (m + 1);
             hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->V, h->w, n,
                                j + 1, h->partial);
@@ -1579,35 +1723,17 @@ static int solve_fgmres(bx_helm *h, const double *b, double *x, double rtol,
                                h->nb_dot, 1, d3);
             hipLaunchKernelGGL(k_scale_inv_norm, dim3(2048), dim3(256), 0, s,
                                h->V + (long long)(j + 1) * n, h->w, n, d3);
+            hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, s, A, d1, d2, d3, j);
             H_CHECK(hipGetLastError());
-            H_CHECK(hipMemcpyAsync(hdots.data(), h->dots, sizeof(double) * (2 * (m + 1) + 1),
+            H_CHECK(hipMemcpyAsync(h->hstat, A.status, 2 * sizeof(double),
                                    hipMemcpyDeviceToHost, s));
             H_CHECK(hipStreamSynchronize(s));
-            for (int i = 0; i <= j; ++i)
-                H[i * m + j] = hdots[i] + hdots[(m + 1) + i];
-            const double hn    = std::sqrt(hdots[2 * (m + 1)]);
-            H[(j + 1) * m + j] = hn;
-            // Givens rotations
-            for (int i = 0; i < j; ++i)
-            {
-                const double t0 = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
-                const double t1 = -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
-                H[i * m + j]       = t0;
-                H[(i + 1) * m + j] = t1;
-            }
-            const double den = std::hypot(H[j * m + j], H[(j + 1) * m + j]);
-            cs[j]            = den > 0 ? H[j * m + j] / den : 1.0;
-            sn[j]            = den > 0 ? H[(j + 1) * m + j] / den : 0.0;
-            H[j * m + j]       = den;
-            H[(j + 1) * m + j] = 0;
-            g[j + 1]           = -sn[j] * g[j];
-            g[j]               = cs[j] * g[j];
+            const double est = h->hstat[0], hn = h->hstat[1];
             ++total;
-            const double est = std::fabs(g[j + 1]);
             if (!std::isfinite(est) || !std::isfinite(hn))
             {
                 // a non-finite preconditioner output: drop this direction
-                *stalled = true;
+                stop = true;
                 break;
             }
             if (est < best * 0.5)
@@ -1622,45 +1748,20 @@ static int solve_fgmres(bx_helm *h, const double *b, double *x, double rtol,
             }
             if (total - best_at >= 8)
             {
-                *stalled = true;
+                stop = true;
                 ++j;
                 break;
             }
         }
-        // solve the j x j triangular system, x += Z y
-        bool finite = true;
-        for (int i = j - 1; i >= 0; --i)
-        {
-            double t = g[i];
-            for (int q = i + 1; q < j; ++q)
-                t -= H[i * m + q] * y[q];
-            y[i]   = t / H[i * m + i];
-            finite = finite && std::isfinite(y[i]);
-        }
-        if (!finite)
-        {
-            *stalled = true;
-            j        = 0; // keep the current iterate
-        }
+        // x += Z y, y from the triangular system of the first j columns
         if (j > 0)
         {
-            H_CHECK(hipMemcpyAsync(h->hdev, y.data(), sizeof(double) * j,
-                                   hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_add_combo, dim3(2048), dim3(256), 0, s, x, h->Z,
-                               h->hdev, j, n);
-        }
-        st->iters += j;
-        if (*stalled)
-        {
-            residual64(h, b, x, h->r, s);
-            double rn = 0;
-            if ((rc = norm2(h, h->r, s, &rn)))
-                return rc;
-            st->rel_resid = rn / bnorm;
-            st->converged = rn <= rtol * bnorm;
-            if (st->converged)
-                *stalled = false;
-            break;
+            hipLaunchKernelGGL(k_arnoldi_solve, dim3(1), dim3(64), 0, s, A, j);
+            H_CHECK(hipMemcpyAsync(h->hstat + 2, A.status + 2, sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+            hipLaunchKernelGGL(k_add_combo, dim3(2048), dim3(256), 0, s, x, h->Z, A.y, j, n);
+            H_CHECK(hipGetLastError());
+            pending_j = j;
         }
     }
     return BX_OK;
@@ -1678,9 +1779,11 @@ extern "C" int bx_helm_solve(bx_helm *h, const double *b, double *x, int method,
     hipStream_t s = (hipStream_t)stream;
     std::memset(st, 0, sizeof(*st));
     const auto t0 = std::chrono::steady_clock::now();
-    double     bnorm = 0;
+    double     bnorm = 0, rn0 = 0;
     int        rc;
-    if ((rc = norm2(h, b, s, &bnorm)))
+    // ||b|| and the initial residual's norm with one synchronisation
+    residual64(h, b, x, h->r, s);
+    if ((rc = norm2_pair(h, b, h->r, s, &bnorm, &rn0)))
         return rc;
     if (bnorm == 0)
     {
@@ -1693,7 +1796,7 @@ extern "C" int bx_helm_solve(bx_helm *h, const double *b, double *x, int method,
     if (method == BX_SOLVE_MG || method == BX_SOLVE_MG_FUSED)
     {
         if ((rc = solve_mg(h, b, x, rtol, maxiter, bnorm, st, s,
-                           method == BX_SOLVE_MG_FUSED)))
+                           method == BX_SOLVE_MG_FUSED, rn0)))
             return rc;
     }
     else
@@ -1702,7 +1805,7 @@ extern "C" int bx_helm_solve(bx_helm *h, const double *b, double *x, int method,
         if (method == BX_SOLVE_FGMRES_MG)
             h->cnn = nullptr;
         bool stalled = false;
-        rc           = solve_fgmres(h, b, x, rtol, maxiter, bnorm, st, s, &stalled);
+        rc           = solve_fgmres(h, b, x, rtol, maxiter, bnorm, st, s, &stalled, rn0);
         h->cnn       = saved;
         if (rc)
             return rc;

@@ -56,9 +56,10 @@ def main():
     ap.add_argument("--imex-ke", default="ex", choices=["ex", "split", "conservative_kinetic", "fi"],
                     help="imex.ke_method: ex (PEEK), split (PESK), conservative_kinetic, or fi "
                          "(not linearised, 2 Picard iterations per stage)")
-    ap.add_argument("--imex-harvest-steps", type=int, default=8, help="pressure systems harvested")
+    ap.add_argument("--imex-harvest-systems", type=int, default=104,
+                    help="pressure systems harvested for training (4 more are held out)")
     ap.add_argument("--imex-steps", type=int, default=4, help="time steps (one solve per stage)")
-    ap.add_argument("--imex-train-steps", type=int, default=640)
+    ap.add_argument("--imex-train-steps", type=int, default=2000)
     ap.add_argument("--imex-batch", type=int, default=32)
     ap.add_argument("--imex-crop", type=int, default=64)
     ap.add_argument("--no-vortex", action="store_true", help="skip the 3D isentropic vortex leg")
@@ -490,7 +491,7 @@ def imex_leg(args):
     import numpy as np
     from bamrex_amd import imex
     n = args.imex_n
-    log("imex: Gresho %d^2, harvesting %d steps" % (n, args.imex_harvest_steps))
+    log("imex: Gresho %d^2, harvesting %d systems (+4 held out)" % (n, args.imex_harvest_systems))
     wl = imex.GreshoWorkload(n, advection=args.imex_advection, tableau=args.imex_tableau,
                              ke_method=args.imex_ke)
 
@@ -500,33 +501,74 @@ def imex_leg(args):
                           cX=sy["cX"], cY=sy["cY"], copy=False)
     nst = wl.rk.tab.n_imp * (wl.rk.picard_max_iter if wl.rk.fi else 1)   # solves per step
     solver = imex.HelmholtzSolver(n, n, wl.dx)
-    nf_max = 8 * args.imex_harvest_steps
+    nsys, n_hold = args.imex_harvest_systems, 4
+    nf_max = 4 * nsys
     R = imex.DeviceArray((nf_max, n, n))
-    nf = 0
-    # one pressure system per stage: harvest_steps systems
-    for k in range(args.imex_harvest_steps):
+    Rh = imex.DeviceArray((4 * n_hold, n, n))
+    nf = nh = 0
+    held = []   # per held-out system: host coefficients and its field range in Rh
+    # one pressure system per stage; each plain-multigrid solve, one V-cycle
+    # at a time, gives its residual fields (A e = r, e the remaining error)
+    for k in range(nsys + n_hold):
         sy = wl.next_system()
         set_coeffs(sy)
         db, dx_ = imex.dev(sy["rhs"]), imex.dev(sy["p_prev"])
-        got, st = imex.harvest_residuals(solver, db, dx_, R, nf, nf_max - nf)
-        nf += got
+        if k < nsys:
+            got, st = imex.harvest_residuals(solver, db, dx_, R, nf, nf_max - nf)
+            nf += got
+        else:
+            got, st = imex.harvest_residuals(solver, db, dx_, Rh, nh, 4 * n_hold - nh)
+            held.append(dict(alpha=sy["alpha"], beta=sy["beta"], gamma=sy["gamma_c"],
+                             a=sy["a"].download(), bX=sy["bX"].download(),
+                             bY=sy["bY"].download(),
+                             cX=sy["cX"].download() if sy["cX"] is not None else None,
+                             cY=sy["cY"].download() if sy["cY"] is not None else None,
+                             fields=(nh, nh + got)))
+            nh += got
         wl.advance(sy, dx_)
+        db.free()
+        dx_.free()
     while wl.in_step:        # finish a step left open by the harvest
         sy = wl.next_system()
         set_coeffs(sy)
         db, dx_ = imex.dev(sy["rhs"]), imex.dev(sy["p_prev"])
         solver.solve(db, dx_, "mg", rtol=1e-10, maxiter=200)
         wl.advance(sy, dx_)
-    log("imex: %d residual fields harvested" % nf)
+    log("imex: %d residual fields harvested from %d systems, %d held out from %d" % (
+        nf, nsys, nh, n_hold))
+
+    def held_out_reduction():
+        """mean ||r - A M r|| / ||r|| of one preconditioner application on
+        the held-out systems' fields, each with its own system's operator"""
+        r, e, q = (imex.DeviceArray((n, n)) for _ in range(3))
+        out = []
+        rh = Rh.download()
+        for h in held:
+            solver.set_coeffs(h["alpha"], h["beta"], h["a"], h["bX"], h["bY"], gamma=h["gamma"],
+                              cX=h["cX"], cY=h["cY"])
+            for i in range(*h["fields"]):
+                r.upload(rh[i])
+                solver.precond(r, e)
+                solver.residual(r, e, q)
+                out.append(float(np.linalg.norm(q.download()) / np.linalg.norm(rh[i])))
+        for d in (r, e, q):
+            d.free()
+        return float(np.mean(out))
+    rho = {"V": held_out_reduction()}
     net = imex.LearnedSmoother(seed=1)
-    solver.attach(net, omega=1.0)
+    solver.attach(net, omega=1.0, mode="corrector")
+    rho["V+untrained"] = held_out_reduction()
     t0 = time.perf_counter()
     hist = net.train(R, nf, steps=args.imex_train_steps, crop=args.imex_crop, batch=args.imex_batch,
                      lr=1e-3, seed=11)
     train_s = time.perf_counter() - t0
-    log("imex: trained %d steps x %d crops in %.2f s, loss %.4g -> %.4g" % (
-        args.imex_train_steps, args.imex_batch, train_s, float(np.mean(hist[:10])),
-        float(np.mean(hist[-10:]))))
+    rho["V+trained"] = held_out_reduction()
+    log("imex: trained %d steps x %d crops in %.2f s, loss %.4g -> %.4g; held-out one-application "
+        "residual reduction V %.4g, V+untrained %.4g, V+trained %.4g" % (
+            args.imex_train_steps, args.imex_batch, train_s, float(np.mean(hist[:10])),
+            float(np.mean(hist[-10:])), rho["V"], rho["V+untrained"], rho["V+trained"]))
+    R.free()
+    Rh.free()
     # (label, solve method, network mode, graph-replayed preconditioner)
     methods = [("mg", "mg", None, False),
                ("mg_fused", "mg_fused", None, False),
@@ -603,7 +645,9 @@ def imex_leg(args):
         args.imex_tableau.lower(), args.imex_advection, n, n),
            "tableau": args.imex_tableau, "solves_per_step": nst,
            "rtol": 1e-10,
-           "harvest_fields": nf, "train_steps": args.imex_train_steps, "train_batch": args.imex_batch,
+           "harvest_systems": nsys, "harvest_fields": nf, "held_out_systems": n_hold,
+           "held_out_fields": nh, "held_out_residual_reduction": rho,
+           "train_steps": args.imex_train_steps, "train_batch": args.imex_batch,
            "train_crop": args.imex_crop, "train_crops_total": args.imex_train_steps * args.imex_batch,
            "train_seconds": round(train_s, 3),
            "train_loss_first_last": [float(np.mean(hist[:10])), float(np.mean(hist[-10:]))]}
@@ -620,8 +664,21 @@ def imex_leg(args):
     stage_total = sum(out["stage_ms_median"].values())
     out["peek_step_ms"] = {m[0]: round(stage_total + nst * out[m[0]]["s_per_solve"] * 1e3, 4)
                            for m in methods}
+    # against plain multigrid (kernels launched one by one, as MLMG), against
+    # the faster of the two plain multigrid variants, and against the same
+    # FGMRES without the network; warm starts (the previous pressure) and
+    # the cold start (zero guess) on the last system
+    best_mg = min(out["mg"]["s_per_solve"], out["mg_fused"]["s_per_solve"])
     out["speedup_cnn_vs_mg"] = out["mg"]["s_per_solve"] / out["fgmres_cnn"]["s_per_solve"]
+    out["speedup_cnn_vs_best_mg"] = best_mg / out["fgmres_cnn"]["s_per_solve"]
+    out["speedup_cnn_vs_fgmres_mg"] = (out["fgmres_mg"]["s_per_solve"]
+                                       / out["fgmres_cnn"]["s_per_solve"])
     out["speedup_fgmres_mg_vs_mg"] = out["mg"]["s_per_solve"] / out["fgmres_mg"]["s_per_solve"]
+    if cold:
+        cbest = min(cold["mg"]["s_per_solve"], cold["mg_fused"]["s_per_solve"])
+        out["cold_speedup_cnn_vs_best_mg"] = cbest / cold["fgmres_cnn"]["s_per_solve"]
+        out["cold_speedup_cnn_vs_fgmres_mg"] = (cold["fgmres_mg"]["s_per_solve"]
+                                                / cold["fgmres_cnn"]["s_per_solve"])
     solver.close()
     net.close()
     wl.free()

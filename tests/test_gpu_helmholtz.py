@@ -414,6 +414,78 @@ def test_trained_network_gain():
     net.close()
 
 
+def test_trained_network_gain_held_out_1024():
+    """The same gain at the benchmark's size, on held-out systems: the
+    network is trained on residual fields of 24 warm-started 1024^2 Gresho
+    pressure systems and measured on the fields of the next 2 systems, which
+    it never saw.  Asserted: one V-cycle + network application reduces the
+    held-out residuals at least 1.5x more than the V-cycle alone, and FGMRES
+    with the network needs no more iterations from the previous pressure
+    than FGMRES with the V-cycle alone.  The iteration counts from the
+    previous pressure and from zero are printed."""
+    announce("cnn gain 1024 held out")
+    n, n_train_sys, n_hold = 1024, 24, 2
+    wl = imex.GreshoWorkload(n, advection="muscl_rusanov", tableau="SSP222")
+    dx = wl.dx
+    s = HelmholtzSolver(n, n, dx)
+    cap = 4 * (n_train_sys + n_hold)
+    R = DeviceArray((cap, n, n))
+    nf = 0
+    held = []
+    for k in range(n_train_sys + n_hold):
+        syd = wl.next_system()
+        s.set_coeffs(syd["alpha"], syd["beta"], syd["a"], syd["bX"], syd["bY"])
+        db, dxv = dev(syd["rhs"]), dev(syd["p_prev"])
+        p_prev = syd["p_prev"].download()
+        got, _ = imex.harvest_residuals(s, db, dxv, R, nf, cap - nf)
+        if k == n_train_sys:
+            n_train = nf
+        if k >= n_train_sys:
+            sy = {k_: (v.download() if isinstance(v, DeviceArray) else v) for k_, v in syd.items()}
+            sy["p_prev"] = p_prev
+            held.append((sy, (nf, nf + got)))
+        nf += got
+        wl.advance(syd, dxv)
+    Rh = R.download()[n_train:nf]
+
+    def rho():
+        out = []
+        for sy, (a, b) in held:
+            s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+            out.append(_reduction(s, sy, dx, list(Rh[a - n_train:b - n_train])))
+        return float(np.mean(out))
+
+    def iterations():
+        its = {}
+        for sy, _ in held:
+            s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+            db = dev(sy["rhs"])
+            for start, x0 in (("warm", sy["p_prev"]), ("cold", np.zeros((n, n)))):
+                for meth in ("fgmres_mg", "fgmres_cnn"):
+                    dxv = dev(x0)
+                    st = s.solve(db, dxv, meth, rtol=1e-10, maxiter=100)
+                    assert st["converged"] and not st["fell_back"]
+                    its.setdefault((start, meth), []).append(st["iters"])
+        return its
+
+    rho_v = rho()
+    net = LearnedSmoother(seed=9)
+    s.attach(net, omega=1.0, mode="corrector")
+    hist = net.train(R, n_train, steps=800, crop=64, batch=32, lr=1e-3, seed=5)
+    rho_trained = rho()
+    its = iterations()
+    print("[gain 1024] train fields %d from %d systems, held-out fields %d from %d systems; loss "
+          "%.4f -> %.4f; one application reduces held-out residuals by V %.4g, V+trained %.4g; "
+          "iterations %s" % (n_train, n_train_sys, nf - n_train, n_hold, np.mean(hist[:20]),
+                             np.mean(hist[-20:]), rho_v, rho_trained, its), flush=True)
+    assert np.mean(hist[-20:]) < np.mean(hist[:20])
+    assert rho_trained < rho_v / 1.5
+    assert sum(its[("warm", "fgmres_cnn")]) <= sum(its[("warm", "fgmres_mg")])
+    R.free()
+    s.close()
+    net.close()
+
+
 def test_blocked_smoother_vcycle_matches_per_colour_launches():
     """The V-cycle with blocked smoothing and fused residual-restriction is
     bit-identical to the per-colour-launch V-cycle, in FP64 (and equal to

@@ -19,10 +19,11 @@
 //   W4[9][16]        tap t, in k
 // tap t = (dy+1)*3 + (dx+1), neighbour (x+dx, y+dy).
 //
-// All four layers run on v_mfma_f32_16x16x16bf16_1k (see the forward
-// kernels): one wave takes 16 pixels (M); the 16 -> 16 layers accumulate 9
-// MFMAs (one per tap, K = 16 input channels), the first layer takes the 9
-// taps as K, the last one forms 9 per-tap partials per pixel as N.
+// All layers run on v_mfma_f32_16x16x16bf16_1k with the output channels as
+// rows and 16 pixels as columns: the 16 -> 16 layers accumulate 9 MFMAs
+// (one per tap, K = 16 input channels), the first layer takes the 9 taps as
+// K, the last one forms 9 per-tap partials per pixel.  The weight gradients
+// are FP32 GEMMs over pixels on v_mfma_f32_16x16x4f32.
 //
 // Training loss on crops of (residual, error) pairs harvested from plain
 // multigrid solves (A e = r):
@@ -36,6 +37,7 @@
 // then back through the convolutions; Adam on the FP32 master weights.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -49,6 +51,8 @@ namespace {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 
 constexpr int C = 16;
 constexpr int NW1 = 9 * C, NW2 = 9 * C * C, NW3 = 9 * C * C, NW4 = 9 * C;
@@ -57,7 +61,7 @@ constexpr int NPAR = OFF4 + NW4;
 static_assert(NPAR == BX_CNN_NPARAMS, "parameter count");
 // training reductions: pixels per weight-gradient block, the row of one
 // block's partials (W3 | W2 | W4 | W1), blocks of the loss sum
-constexpr int WG_CH       = 128;
+constexpr int WG_CH       = 256;
 constexpr int WG_ROW      = 2 * NW2 + NW4 + NW1;
 constexpr int LOSS_BLOCKS = 2048;
 
@@ -94,19 +98,37 @@ __device__ inline int nbr(int x, int y, int dx, int dy, const Geo &g, int img)
 }
 
 // ---- inference / training forward --------------------------------------
-// Every layer runs on v_mfma_f32_16x16x16bf16_1k with one wave per 16 pixels:
-//   A[i][k] (lane l: i = l%16, k = 4(l/16)+v), B[k][j] (lane l: j = l%16),
-//   D[i][j] (lane l: i = 4(l/16)+v, j = l%16).
+// Every layer puts output channels (or, in layer 4, taps) in the MFMA's
+// rows and 16 pixels in its columns, so a lane's result is 4 consecutive
+// channels of one pixel (one 8-byte store).  v_mfma_f32_16x16x16bf16_1k:
+// A[i][k] (lane l: i = l%16, k = 4(l/16)+v), B[k][j] (lane l: j = l%16,
+// k = 4(l/16)+v), D[i][j] (lane l: i = 4(l/16)+v, j = l%16).
 // Each D[i][j] is a dot product over k of row i of A with column j of B, so a
-// pixel's result does not depend on which pixels share its tile: the fused
-// inference kernel below and these layer-by-layer kernels agree bit for bit.
-//   layer 1: A = the 9 taps of x at a pixel (k = tap, 7 zero), B = W1 (tap x out)
-//   layers 2, 3: A = 16 channels of a neighbour, B = W (in x out), 9 MFMAs
-//   layer 4: A = 16 channels of h3 at a pixel, B = W4 (in x tap), giving the
-//            per-tap partials P[pixel][tap]; y = sum over taps of P at the
-//            neighbours, in tap order.
+// pixel's result does not depend on which pixels share its tile, and each
+// 16 -> 16 layer adds its 9 taps' products in tap order: the fused inference
+// kernel below and these layer-by-layer kernels agree bit for bit.
+//   layer 1: A = W1 (out channel x tap), B = the 9 taps of x (k = tap, 7 zero)
+//   layers 2, 3: A = W[tap] (out x in channel), B = the neighbour's channels
+//   layer 4: A = W4 (tap x in channel), B = h3, giving the per-tap partials
+//            P[pixel][tap]; y = sum over taps of P at the neighbours, in tap
+//            order.
 
-// B operand of layer 1: B[k = tap][j = out] = W1[tap][out] (tap < 9)
+// ReLU on the bits (a negative float, -0 included, becomes +0; the integer
+// maximum is one instruction) and round-to-nearest-even to bf16, two values
+// per hardware conversion
+__device__ inline unsigned relu_pack2(float lo, float hi)
+{
+    const int il = __builtin_bit_cast(int, lo), ih = __builtin_bit_cast(int, hi);
+    v2f v = { __builtin_bit_cast(float, il > 0 ? il : 0), __builtin_bit_cast(float, ih > 0 ? ih : 0) };
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, v2bf));
+}
+// four ReLU'd channels as bf16, packed for one 8-byte store
+__device__ inline uint2 relu_pack4(const v4f &acc)
+{
+    return make_uint2(relu_pack2(acc[0], acc[1]), relu_pack2(acc[2], acc[3]));
+}
+
+// A operand of layer 1: A[i = out][k = tap] = W1[tap][out] (tap < 9)
 __device__ inline v4s w1_operand(const unsigned short *__restrict__ Wb1, int m, int kg)
 {
     v4s b;
@@ -118,7 +140,7 @@ __device__ inline v4s w1_operand(const unsigned short *__restrict__ Wb1, int m, 
     }
     return b;
 }
-// B operand of layer 4: B[k = in channel][j = tap] = W4[tap][in] (tap < 9)
+// A operand of layer 4: A[i = tap][k = in channel] = W4[tap][in] (tap < 9)
 __device__ inline v4s w4_operand(const unsigned short *__restrict__ Wb4, int m, int kg)
 {
     v4s b;
@@ -128,7 +150,8 @@ __device__ inline v4s w4_operand(const unsigned short *__restrict__ Wb4, int m, 
     return b;
 }
 
-// layer 1: x (bf16, 1 channel) -> h1 (bf16, 16, ReLU)
+// layer 1: x (bf16, 1 channel) -> h1 (bf16, 16, ReLU); B = the taps of
+// pixel p0 + m (lane: taps 4kg .. 4kg+3)
 template <bool PERIODIC>
 __global__ __launch_bounds__(256) void k_conv1(const unsigned short *__restrict__ xin,
                                                unsigned short *__restrict__ hout,
@@ -161,17 +184,21 @@ __global__ __launch_bounds__(256) void k_conv1(const unsigned short *__restrict_
         a[v] = xv;
     }
     v4f acc = { 0.f, 0.f, 0.f, 0.f };
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w1_operand(Wb1, m, kg), acc, 0, 0, 0);
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-    {
-        const float s = acc[v];
-        hout[(p0 + 4 * kg + v) * C + m] = f2bf(s > 0.f ? s : 0.f);
-    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w1_operand(Wb1, m, kg), a, acc, 0, 0, 0);
+    // lane: channels 4kg .. 4kg+3 of pixel p0 + m
+    *reinterpret_cast<uint2 *>(hout + (p0 + m) * C + 4 * kg) = relu_pack4(acc);
 }
 
-// layers 2, 3: h (bf16, 16) -> h' (bf16, 16, ReLU) on MFMA.
+// 16 -> 16 layers, D[i = out channel][j = pixel]: 9 MFMAs in tap order,
+// A = W[tap] (lane l: out channel l%16, input channels 4(l/16) .. +3),
+// B = the neighbour's channels (lane l: pixel l%16, the same 4 channels).
 // Wb: bf16 weights [9][16 out][16 in].
+__device__ inline v4s w16_tap(const unsigned short *__restrict__ Wb, int t, int i, int kg)
+{
+    return *reinterpret_cast<const v4s *>(Wb + (t * C + i) * C + 4 * kg);
+}
+
+// layers 2, 3: h (bf16, 16) -> h' (bf16, 16, ReLU), one wave per 16 pixels
 template <bool PERIODIC>
 __global__ __launch_bounds__(256) void k_conv16(const unsigned short *__restrict__ hin,
                                                 unsigned short *__restrict__ hout,
@@ -194,20 +221,13 @@ __global__ __launch_bounds__(256) void k_conv16(const unsigned short *__restrict
     for (int t = 0; t < 9; ++t)
     {
         const int q = nbr<PERIODIC>(x0 + m, y, t % 3 - 1, t / 3 - 1, g, img);
-        v4s       a = { 0, 0, 0, 0 };
+        v4s       b = { 0, 0, 0, 0 };
         if (q >= 0)
-            a = *reinterpret_cast<const v4s *>(hin + (long long)q * C + 4 * kg);
-        const v4s b
-            = *reinterpret_cast<const v4s *>(Wb + (t * C + m) * C + 4 * kg);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            b = *reinterpret_cast<const v4s *>(hin + (long long)q * C + 4 * kg);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w16_tap(Wb, t, m, kg), b, acc, 0, 0, 0);
     }
-    // lane holds D[4*kg + v][m]: pixel p0 + 4*kg + v, channel m
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-    {
-        const float s = acc[v];
-        hout[(p0 + 4 * kg + v) * C + m] = f2bf(s > 0.f ? s : 0.f);
-    }
+    // lane: channels 4kg .. 4kg+3 of pixel p0 + m
+    *reinterpret_cast<uint2 *>(hout + (p0 + m) * C + 4 * kg) = relu_pack4(acc);
 }
 
 // layer 4, part 1: per-tap partials P[p][t] = sum_k W4[t][k] h3[p][k]
@@ -222,12 +242,11 @@ __global__ __launch_bounds__(256) void k_conv4_taps(const unsigned short *__rest
     if (p0 >= npx)
         return;
     const int m = lane & 15, kg = lane >> 4;
-    const v4s a = *reinterpret_cast<const v4s *>(hin + (p0 + m) * C + 4 * kg);
+    const v4s b = *reinterpret_cast<const v4s *>(hin + (p0 + m) * C + 4 * kg);
     v4f       acc = { 0.f, 0.f, 0.f, 0.f };
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w4_operand(Wb4, m, kg), acc, 0, 0, 0);
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-        P[(p0 + 4 * kg + v) * C + m] = acc[v];
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w4_operand(Wb4, m, kg), b, acc, 0, 0, 0);
+    // lane: taps 4kg .. 4kg+3 of pixel p0 + m
+    *reinterpret_cast<v4f *>(P + (p0 + m) * C + 4 * kg) = acc;
 }
 
 // layer 4, part 2: y[p] = sum over taps t (in order) of P[p + off_t][t]
@@ -255,32 +274,11 @@ __global__ __launch_bounds__(256) void k_conv4_sum(const float *__restrict__ P,
     yout[p] = s;
 }
 
-// x = bf16(r * inv_sigma)
-__global__ __launch_bounds__(256) void k_prep_full(const double *__restrict__ r,
-                                                   unsigned short *__restrict__ x,
-                                                   long long n,
-                                                   const double *__restrict__ ss)
-{
-    const double inv = ss[0] > 0 ? 1.0 / ss[0] : 0.0;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        x[i] = f2bf((float)(r[i] * inv));
-}
 
-// e = omega * sigma * y / D
-__global__ __launch_bounds__(256) void k_finish_full(const float *__restrict__ y,
-                                                     const double *__restrict__ Dg,
-                                                     double *__restrict__ e,
-                                                     long long n, double omega,
-                                                     const double *__restrict__ ss)
-{
-    const double sg = ss[0];
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        e[i] = omega * sg * (double)y[i] / Dg[i];
-}
 
-// rms of r (two-stage)
+// sum of squares of r, one partial per block (the fused kernel adds the
+// partials in a fixed order)
+constexpr int SS_BLOCKS = 2048;
 __global__ __launch_bounds__(256) void k_sumsq(const double *__restrict__ r,
                                                long long n,
                                                double *__restrict__ partial)
@@ -300,24 +298,6 @@ __global__ __launch_bounds__(256) void k_sumsq(const double *__restrict__ r,
     }
     if (threadIdx.x == 0)
         partial[blockIdx.x] = sh[0];
-}
-__global__ void k_rms_final(const double *__restrict__ partial, int nb,
-                            long long n, double *__restrict__ out)
-{
-    __shared__ double sh[256];
-    double            s = 0;
-    for (int i = threadIdx.x; i < nb; i += blockDim.x)
-        s += partial[i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1)
-    {
-        if ((int)threadIdx.x < off)
-            sh[threadIdx.x] += sh[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        out[0] = sqrt(sh[0] / (double)n);
 }
 
 // operator diagonal: alpha a + dhx (bXl + bXr) + dhy (bYl + bYr)
@@ -606,45 +586,69 @@ __global__ __launch_bounds__(256) void k_bwd16_data(const float *__restrict__ go
         gin[p * C + k] = bf2f(hin[p * C + k]) > 0.f ? acc[k] : 0.f;
 }
 
-// weight gradient of a 16->16 layer: dW[t][n][k] = sum_p gout[p][n] hin[p+off_t][k]
-// block: 256 threads, thread = (n, k), 9 taps each; chunk of CH pixels, in
-// pixel order; the block's sums go to its own slot of `part` (layout
-// [block][t][n][k]) and k_sum_partials_f32 adds the blocks in block order,
-// so the gradient is the same on every run
+// Weight gradients as FP32 GEMMs over pixels on v_mfma_f32_16x16x4f32
+// (A[i][kk]: lane l holds i = l%16, kk = l/16; B[kk][j]: lane l holds
+// kk = l/16, j = l%16; D[i][j]: lane l holds i = 4(l/16)+v, j = l%16), one
+// MFMA per 4 pixels (kk).  A block of 4 waves takes CH pixels, a quarter
+// per wave in pixel order; the waves' sums are added in wave order and the
+// block's row goes to its own slot of `part` (k_sum_partials_f32 adds the
+// rows in block order), so the gradient is the same on every run.
+//
+// 16 -> 16 layer: dW[t][n][k] = sum_p gout[p][n] hin[p+off_t][k], one GEMM
+// per tap t: A[n][p] = gout[p][n], B[p][k] = hin[p+off_t][k].  Row layout
+// [t][n][k].
 template <int CH>
 __global__ __launch_bounds__(256) void k_bwd16_weight(const float *__restrict__ gout,
                                                       const unsigned short *__restrict__ hin,
                                                       float *__restrict__ part, int stride,
                                                       Geo g)
 {
-    const long long npx = (long long)g.w * g.h * g.nimg;
-    const int       n = threadIdx.x >> 4, k = threadIdx.x & 15;
-    float           acc[9];
+    __shared__ float red[4][9 * C * C];
+    const long long npx  = (long long)g.w * g.h * g.nimg;
+    const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int       c = lane & 15, pk = lane >> 4;
+    v4f             acc[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t)
-        acc[t] = 0.f;
-    const long long pbeg = (long long)blockIdx.x * CH;
-    for (long long p = pbeg; p < pbeg + CH && p < npx; ++p)
+        acc[t] = v4f{ 0.f, 0.f, 0.f, 0.f };
+    const long long pb = (long long)blockIdx.x * CH + w * (CH / 4);
+    const unsigned  wh_ = (unsigned)g.w * (unsigned)g.h;
+    for (int i0 = 0; i0 < CH / 4; i0 += 4)
     {
-        const float gv = gout[p * C + n];
-        if (gv == 0.f)
-            continue;
-        const unsigned wh_ = (unsigned)g.w * (unsigned)g.h;
-        const int img = (int)((unsigned)p / wh_);
-        const int rem = (int)((unsigned)p - (unsigned)img * wh_);
-        const int y = (int)((unsigned)rem / (unsigned)g.w), x = rem - y * g.w;
+        const long long p  = pb + i0 + pk;
+        const bool      pv = p < npx;
+        float           av = 0.f;
+        int             x = 0, y = 0, img = 0;
+        if (pv)
+        {
+            av = gout[p * C + c];
+            img = (int)((unsigned)p / wh_);
+            const int rem = (int)((unsigned)p - (unsigned)img * wh_);
+            y = (int)((unsigned)rem / (unsigned)g.w);
+            x = rem - y * g.w;
+        }
 #pragma unroll
         for (int t = 0; t < 9; ++t)
         {
-            const int q = nbr<false>(x, y, t % 3 - 1, t / 3 - 1, g, img);
-            if (q >= 0)
-                acc[t] += gv * bf2f(hin[(long long)q * C + k]);
+            float bv = 0.f;
+            if (pv)
+            {
+                const int q = nbr<false>(x, y, t % 3 - 1, t / 3 - 1, g, img);
+                if (q >= 0)
+                    bv = bf2f(hin[(long long)q * C + c]);
+            }
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t], 0, 0, 0);
         }
     }
-    float *dst = part + (long long)blockIdx.x * stride;
 #pragma unroll
     for (int t = 0; t < 9; ++t)
-        dst[(t * C + n) * C + k] = acc[t];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            red[w][(t * C + 4 * pk + v) * C + c] = acc[t][v];
+    __syncthreads();
+    float *dst = part + (long long)blockIdx.x * stride;
+    for (int i = threadIdx.x; i < 9 * C * C; i += blockDim.x)
+        dst[i] = (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
 }
 
 // out[i] = sum over the nblk partial rows of part[b * stride + i], i < nval,
@@ -662,41 +666,66 @@ __global__ __launch_bounds__(256) void k_sum_partials_f32(const float *__restric
     out[i] = s;
 }
 
-// weight gradient of layer 4 (dW4[t][k] += sum_p g_y[p] h3[p+off_t][k]) and
-// layer 1 (dW1[t][n] += sum_p g_h1[p][n] x[p+off_t]); thread = (t, c)
-// (per-block partials, layout [block][W4 144 | W1 144], as for the 16->16 layers)
+// Layers 4 and 1, one GEMM each (tap t = row or column 0..8, 9..15 zero):
+//   dW4[t][k] = sum_p g_y[p - off_t] h3[p][k]:  A[t][p] = g_y[p - off_t], B[p][k] = h3[p][k]
+//   dW1[t][n] = sum_p g_h1[p][n] x[p + off_t]:  A[n][p] = g_h1[p][n],    B[p][t] = x[p + off_t]
+// (the first is sum_p g_y[p] h3[p+off_t][k] with the pixel index shifted).
+// Row layout [W4 144 | W1 144].
 template <int CH>
-__global__ __launch_bounds__(320) void k_bwd_weight_small(const float *__restrict__ gy,
+__global__ __launch_bounds__(256) void k_bwd_weight_small(const float *__restrict__ gy,
                                                           const unsigned short *__restrict__ h3,
                                                           const float *__restrict__ gh1,
                                                           const unsigned short *__restrict__ x,
                                                           float *__restrict__ part, int stride,
                                                           Geo g)
 {
-    const long long npx = (long long)g.w * g.h * g.nimg;
-    const int       tid = threadIdx.x; // 0..143 for W4, 144 threads for W1 (second block half)
-    const bool      w4  = tid < 144;
-    const int       idx = w4 ? tid : tid - 144;
-    if (idx >= 144)
-        return;
-    const int t = idx / C, c = idx % C;
-    float     acc = 0.f;
-    const long long pbeg = (long long)blockIdx.x * CH;
-    for (long long p = pbeg; p < pbeg + CH && p < npx; ++p)
+    __shared__ float red[4][2 * C * C];
+    const long long npx  = (long long)g.w * g.h * g.nimg;
+    const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int       c = lane & 15, pk = lane >> 4;
+    const int       tdx = c % 3 - 1, tdy = c / 3 - 1; // tap c (when c < 9)
+    v4f             a4 = { 0.f, 0.f, 0.f, 0.f }, a1 = { 0.f, 0.f, 0.f, 0.f };
+    const long long pb  = (long long)blockIdx.x * CH + w * (CH / 4);
+    const unsigned  wh_ = (unsigned)g.w * (unsigned)g.h;
+    for (int i0 = 0; i0 < CH / 4; i0 += 4)
     {
-        const unsigned wh_ = (unsigned)g.w * (unsigned)g.h;
-        const int img = (int)((unsigned)p / wh_);
-        const int rem = (int)((unsigned)p - (unsigned)img * wh_);
-        const int y = (int)((unsigned)rem / (unsigned)g.w), xx = rem - y * g.w;
-        const int q = nbr<false>(xx, y, t % 3 - 1, t / 3 - 1, g, img);
-        if (q < 0)
-            continue;
-        if (w4)
-            acc += gy[p] * bf2f(h3[(long long)q * C + c]);
-        else
-            acc += gh1[p * C + c] * bf2f(x[q]);
+        const long long p  = pb + i0 + pk;
+        float           A4 = 0.f, B4 = 0.f, A1 = 0.f, B1 = 0.f;
+        if (p < npx)
+        {
+            const int img = (int)((unsigned)p / wh_);
+            const int rem = (int)((unsigned)p - (unsigned)img * wh_);
+            const int yy = (int)((unsigned)rem / (unsigned)g.w), xx = rem - yy * g.w;
+            if (c < 9)
+            {
+                const int q4 = nbr<false>(xx, yy, -tdx, -tdy, g, img);
+                if (q4 >= 0)
+                    A4 = gy[q4];
+                const int q1 = nbr<false>(xx, yy, tdx, tdy, g, img);
+                if (q1 >= 0)
+                    B1 = bf2f(x[q1]);
+            }
+            B4 = bf2f(h3[p * C + c]);
+            A1 = gh1[p * C + c];
+        }
+        a4 = __builtin_amdgcn_mfma_f32_16x16x4f32(A4, B4, a4, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(A1, B1, a1, 0, 0, 0);
     }
-    part[(long long)blockIdx.x * stride + tid] = acc;
+    // lane: D[i = 4 pk + v][j = c]; W4 (t = i, k = c), W1 (t = c, n = i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+        const int i = 4 * pk + v;
+        red[w][i * C + c]         = a4[v];
+        red[w][C * C + c * C + i] = a1[v];
+    }
+    __syncthreads();
+    float *dst = part + (long long)blockIdx.x * stride;
+    for (int i = threadIdx.x; i < NW4 + NW1; i += blockDim.x)
+    {
+        const int src = i < NW4 ? i : C * C + (i - NW4);
+        dst[i]        = (red[0][src] + red[1][src]) + (red[2][src] + red[3][src]);
+    }
 }
 
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ gr,
@@ -730,34 +759,41 @@ __global__ void k_round_bf16(const float *__restrict__ w, float *__restrict__ wr
 }
 
 // ---- fused inference: all four layers of one output tile in LDS --------
-// Output tile FT x FT.  Each layer is evaluated on the region its successor
-// needs (the halo shrinks by one per layer): x on (FT+8)^2, h1 on (FT+6)^2,
-// h2 on (FT+4)^2, h3 on (FT+2)^2, the layer-4 partials on (FT+2)^2.  Each
-// MFMA tile takes 16 consecutive pixels of the flattened region.  Activations
-// sit in LDS as four planes of 4 channels (8 bytes per pixel and plane), with
-// plane strides of 16 (mod 32) pixels, so the 8-byte A-operand reads of the
-// 64 lanes fall in distinct banks.  Zero-padded fields (not PERIODIC) store 0
-// for every activation outside the image, as the layered kernels read their
-// padding.
-constexpr int FT   = 28;
-constexpr int FX   = FT + 8; // 36
-constexpr int FR1  = FT + 6; // 34
-constexpr int FR2  = FT + 4; // 32
-constexpr int FR3  = FT + 2; // 30
-constexpr int NP1  = 1168;   // >= FR1^2, = 16 (mod 32)
-constexpr int NP2  = 1040;   // >= FR2^2
-constexpr int NP3  = 912;    // >= FR3^2
-static_assert(NP1 >= FR1 * FR1 && NP2 >= FR2 * FR2 && NP3 >= FR3 * FR3, "planes");
-constexpr int F_LDS = FX * FX * 2 + 4 * NP1 * 8 + 4 * NP2 * 8;
+// Output tile FT x FT, 8 waves.  Each layer is evaluated on the region its
+// successor needs (the halo shrinks by one per layer): x on (FT+8)^2, h1 on
+// (FT+6)^2, h2 on (FT+4)^2, h3 on (FT+2)^2, the layer-4 partials on
+// (FT+2)^2.  Each MFMA tile takes 16 consecutive pixels of the flattened
+// region.  Activations sit in LDS as four planes of 4 channels (8 bytes per
+// pixel and plane): a lane's B operand (4 channels of one pixel) is one
+// 8-byte access, and the 16 lanes that share a channel group touch 128
+// consecutive bytes.
+// Every global load of a thread (the input halo, the operator diagonal of
+// its output pixels) is issued before the first use of any of them.
+// Zero-padded fields (not PERIODIC) store 0 for every activation outside the
+// image, as the layered kernels read their padding.
+constexpr int FT     = 28;
+constexpr int FX     = FT + 8; // 36
+constexpr int FR1    = FT + 6; // 34
+constexpr int FR2    = FT + 4; // 32
+constexpr int FR3    = FT + 2; // 30
+constexpr int NP1    = FR1 * FR1;
+constexpr int NP2    = FR2 * FR2;
+constexpr int NP3    = FR3 * FR3;
+constexpr int F_THR  = 512;
+constexpr int F_WAVE = F_THR / 64;
+constexpr int F_LDS  = FX * FX * 2 + 4 * NP1 * 8 + 4 * NP2 * 8;
+constexpr int F_XL   = (FX * FX + F_THR - 1) / F_THR; // input loads per thread
+constexpr int F_OL   = (FT * FT + F_THR - 1) / F_THR; // output pixels per thread
 static_assert(4 * NP3 * 8 <= 4 * NP1 * 8, "h3 aliases h1");
 static_assert(9 * NP3 * 4 <= 4 * NP2 * 8, "layer-4 partials alias h2");
 
 struct FusedIO
 {
-    const double *r;   // INFER: residual (FP64)
-    const float  *xin; // raw mode: network input (rounded to bf16 on load)
-    const double *ss;  // INFER: sigma (rms of r)
-    const double *D;   // INFER: operator diagonal
+    const double *r;    // INFER: residual (FP64)
+    const float  *xin;  // raw mode: network input (rounded to bf16 on load)
+    const double *part; // INFER: SS_BLOCKS partial sums of r^2 (sigma = rms of r)
+    double        n;    // INFER: number of cells of r
+    const double *D;    // INFER: operator diagonal
     double       *e;   // INFER: correction
     float        *y;   // raw mode: network output
     double        omega;
@@ -769,73 +805,158 @@ __device__ inline int wrapi(int v, int n)
     return v < 0 ? v + n : v;
 }
 
-// D of a 16 -> 16 or 1 -> 16 layer (lane: pixels 4kg+v of the tile,
-// channel m) with ReLU, into the planes of the WO x WO region at origin org
+// the pixel of the image behind tile-region coordinate (x, y) (already
+// offset to image coordinates); -1 outside a zero-padded image
+template <bool PERIODIC>
+__device__ inline long long region_pixel(int x, int y, const Geo &g, bool big, long long ibase)
+{
+    if (!PERIODIC)
+        return (x < 0 || y < 0 || x >= g.w || y >= g.h) ? -1 : ibase + (long long)y * g.w + x;
+    if (big) // the region wraps at most once
+    {
+        x = x < 0 ? x + g.w : (x >= g.w ? x - g.w : x);
+        y = y < 0 ? y + g.h : (y >= g.h ? y - g.h : y);
+    }
+    else
+    {
+        x = wrapi(x, g.w);
+        y = wrapi(y, g.h);
+    }
+    return ibase + (long long)y * g.w + x;
+}
+
+// One wave's walk over the 16-pixel tiles of a WO x WO region, tiles wave,
+// wave + stride, ...: lane m's pixel as region coordinates (i, j), advanced
+// without a division per tile.
+template <int WO, int STEP>
+struct TileWalk
+{
+    int p, i, j;
+    __device__ TileWalk(int first) : p(first), i(first % WO), j(first / WO) {}
+    __device__ void next()
+    {
+        p += STEP;
+        i += STEP % WO;
+        j += STEP / WO;
+        if (i >= WO)
+        {
+            i -= WO;
+            ++j;
+        }
+    }
+};
+
+// a layer's D (lane: channels 4kq..4kq+3 of the tile's pixel at region
+// coordinates (i, j), flattened p) with ReLU, one 8-byte store into the four
+// planes of the WO x WO region at origin org
 template <bool PERIODIC, int WO, int NP>
 __device__ inline void store_act(unsigned short *__restrict__ hout, int ox, int oy, int org,
-                                 const Geo &g, int tile, const v4f &acc, int m, int kg)
+                                 const Geo &g, int p, int i, int j, const v4f &acc, int kq)
 {
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
+    if (p >= WO * WO)
+        return;
+    uint2 h = relu_pack4(acc);
+    if (!PERIODIC)
     {
-        const int po = tile * 16 + 4 * kg + v;
-        if (po >= WO * WO)
-            continue;
-        const float    sv = acc[v];
-        unsigned short h  = f2bf(sv > 0.f ? sv : 0.f);
-        if (!PERIODIC)
-        {
-            const int x = ox + org + po % WO, y = oy + org + po / WO;
-            if (x < 0 || y < 0 || x >= g.w || y >= g.h)
-                h = 0;
-        }
-        hout[((m >> 2) * NP + po) * 4 + (m & 3)] = h;
+        const int x = ox + org + i, y = oy + org + j;
+        if (x < 0 || y < 0 || x >= g.w || y >= g.h)
+            h = make_uint2(0u, 0u);
     }
+    *reinterpret_cast<uint2 *>(hout + (kq * NP + p) * 4) = h;
 }
 
-// A 16 -> 16 layer on the WO x WO region (input (WO+2)^2): 9 MFMAs (one per
-// tap) per 16 output pixels, two tiles interleaved per wave.
-template <bool PERIODIC, int WO, int NPI, int NPO>
-__device__ inline void conv16_tiles(const unsigned short *__restrict__ hin,
-                                    unsigned short *__restrict__ hout, int ox, int oy,
-                                    int org, const Geo &g,
-                                    const unsigned short *__restrict__ W, int wave, int m,
-                                    int kg)
+// A 16 -> 16 layer on the WO x WO region (input (WO+2)^2, row stride WI =
+// WO + 2; output row stride WO), one strip of 16 columns x R rows per wave
+// and work item.  The wave reads each input row's three column shifts once
+// (one 8-byte read per lane: 4 channels of one pixel) and feeds every output
+// row that uses it, so an activation is read 3 times instead of 9; each
+// output row still receives its 9 taps in tap order.  Columns past WO and
+// rows past WO are computed on clamped (in-bounds) reads and not stored.
+template <bool PERIODIC, int WO, int NPI, int NPO, int R>
+__device__ inline void conv16_rows(const unsigned short *__restrict__ hin,
+                                   unsigned short *__restrict__ hout, int ox, int oy,
+                                   int org, const Geo &g,
+                                   const unsigned short *__restrict__ W, int wave, int m,
+                                   int kg)
 {
-    constexpr int WI = WO + 2, NPX = WO * WO, NT = (NPX + 15) / 16;
-    v4s           b[9];
+    constexpr int WI = WO + 2, NS = (WO + 15) / 16, NRB = (WO + R - 1) / R;
+    v4s           a[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t)
-        b[t] = *reinterpret_cast<const v4s *>(W + (t * C + m) * C + 4 * kg);
-    for (int tile = wave; tile < NT; tile += 8)
+        a[t] = w16_tap(W, t, m, kg);
+    const unsigned short *src = hin + kg * NPI * 4;
+    for (int item = wave; item < NS * NRB; item += F_WAVE)
     {
-        const int  t2  = tile + 4;
-        const int  p1  = tile * 16 + m, p2 = t2 * 16 + m;
-        const bool a1v = p1 < NPX, a2v = p2 < NPX;
-        const int  o1  = a1v ? (p1 / WO) * WI + p1 % WO : 0;
-        const int  o2  = a2v ? (p2 / WO) * WI + p2 % WO : 0;
-        v4f        c1 = { 0.f, 0.f, 0.f, 0.f }, c2 = { 0.f, 0.f, 0.f, 0.f };
+        const int c  = (item % NS) * 16 + m; // this lane's output column
+        const int cc = c < WO ? c : WO - 1;
+        const int y0 = (item / NS) * R;
+        v4f       acc[R];
 #pragma unroll
-        for (int t = 0; t < 9; ++t)
+        for (int y = 0; y < R; ++y)
+            acc[y] = v4f{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+        for (int r = 0; r < R + 2; ++r)
         {
-            const int d  = (t / 3) * WI + t % 3;
-            v4s       a1 = { 0, 0, 0, 0 }, a2 = { 0, 0, 0, 0 };
-            if (a1v)
-                a1 = *reinterpret_cast<const v4s *>(hin + (kg * NPI + o1 + d) * 4);
-            if (a2v)
-                a2 = *reinterpret_cast<const v4s *>(hin + (kg * NPI + o2 + d) * 4);
-            c1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b[t], c1, 0, 0, 0);
-            c2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b[t], c2, 0, 0, 0);
+            const int ri = y0 + r < WI ? y0 + r : WI - 1; // input row
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx)
+            {
+                const v4s b = *reinterpret_cast<const v4s *>(src + (ri * WI + cc + dx) * 4);
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy)
+                    if (r - dy >= 0 && r - dy < R)
+                        acc[r - dy] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
+                            a[dy * 3 + dx], b, acc[r - dy], 0, 0, 0);
+            }
         }
-        store_act<PERIODIC, WO, NPO>(hout, ox, oy, org, g, tile, c1, m, kg);
-        if (t2 < NT)
-            store_act<PERIODIC, WO, NPO>(hout, ox, oy, org, g, t2, c2, m, kg);
+#pragma unroll
+        for (int y = 0; y < R; ++y)
+            if (c < WO && y0 + y < WO)
+                store_act<PERIODIC, WO, NPO>(hout, ox, oy, org, g, (y0 + y) * WO + c, c, y0 + y,
+                                             acc[y], kg);
     }
 }
 
+// one thread's input loads for the tile at (ox, oy) of image ibase: the
+// input on [-4, FT+4)^2
 template <bool PERIODIC, bool INFER>
-__global__ __launch_bounds__(256) void k_net_fused(FusedIO io,
-                                                   const unsigned short *__restrict__ Wb, Geo g)
+__device__ inline void fused_loads(const FusedIO &io, const Geo &g, int ox, int oy,
+                                   long long ibase, bool big, int tid, double *xv)
+{
+#pragma unroll
+    for (int u = 0; u < F_XL; ++u)
+    {
+        const int       q = tid + u * F_THR;
+        const long long p = q < FX * FX
+                                ? region_pixel<PERIODIC>(ox - 4 + q % FX, oy - 4 + q / FX, g, big, ibase)
+                                : -1;
+        xv[u] = 0.0;
+        if (p >= 0)
+            xv[u] = INFER ? io.r[p] : (double)io.xin[p];
+    }
+}
+
+// a thread's output pixel u of the tile at (ox, oy): -1 if outside the image
+__device__ inline long long out_pixel(const Geo &g, int ox, int oy, long long ibase, int tid,
+                                      int u)
+{
+    const int q = tid + u * F_THR;
+    const int x = ox + q % FT, y = oy + q / FT;
+    return (q < FT * FT && x < g.w && y < g.h) ? ibase + (long long)y * g.w + x : -1;
+}
+
+// Persistent, two blocks per compute unit (4 waves per SIMD, at most 128
+// VGPRs): block b takes tiles b, b + gridDim.x, ... (tile T: column
+// T % ntx, row (T / ntx) % nty, image T / (ntx nty)).  The global loads of a
+// block's next tile's input are issued right after its current input is in
+// LDS, so they are in flight while the four layers of the current tile run.
+// LDS reuse across tiles: the next tile writes x and h1 only after every
+// wave has passed the current tile's last barrier, and its h2 (which the
+// current tile's output phase reads as the partials) only after two more.
+template <bool PERIODIC, bool INFER>
+__global__ __launch_bounds__(F_THR, 4) void k_net_fused(FusedIO io,
+                                                     const unsigned short *__restrict__ Wb,
+                                                     Geo g, int ntx, int nty, int ntiles)
 {
     __shared__ __align__(16) unsigned char smem[F_LDS];
     unsigned short *sx  = reinterpret_cast<unsigned short *>(smem);
@@ -843,117 +964,169 @@ __global__ __launch_bounds__(256) void k_net_fused(FusedIO io,
     unsigned short *sh2 = sh1 + 4 * NP1 * 4;
     unsigned short *sh3 = sh1;                          // h1 is dead once h2 is formed
     float          *sP  = reinterpret_cast<float *>(sh2); // h2 is dead once h3 is formed
-    const int ox = blockIdx.x * FT, oy = blockIdx.y * FT, img = blockIdx.z;
-    const int tid = threadIdx.x;
-    const long long ibase = (long long)img * g.h * g.w;
-    auto inside = [&](int x, int y) { return x >= 0 && y >= 0 && x < g.w && y < g.h; };
-
-    // x = bf16(r / sigma) on [-4, FT+4)^2
-    double inv = 1.0;
+    const int       tid = threadIdx.x;
+    const bool      big = g.w >= FX && g.h >= FX;
+    int             T   = blockIdx.x;
+    if (T >= ntiles) // the whole block
+        return;
+    // sigma from the partial sums: every block adds them in the same order
+    double sg = 0.0, inv = 1.0;
     if (INFER)
-        inv = io.ss[0] > 0 ? 1.0 / io.ss[0] : 0.0;
-    const bool big = g.w >= FX && g.h >= FX;
-    for (int q = tid; q < FX * FX; q += blockDim.x)
     {
-        const int i = q % FX, j = q / FX;
-        int       x = ox - 4 + i, y = oy - 4 + j;
-        unsigned short v = 0;
-        if (PERIODIC || inside(x, y))
+        double *red = reinterpret_cast<double *>(sh1); // free until the first tile
+        double  acc = 0.0;
+        for (int i = tid; i < SS_BLOCKS; i += F_THR)
+            acc += io.part[i];
+        red[tid] = acc;
+        __syncthreads();
+        for (int off = F_THR / 2; off > 0; off >>= 1)
         {
-            if (PERIODIC)
-            {
-                if (big) // the region wraps at most once
-                {
-                    x = x < 0 ? x + g.w : (x >= g.w ? x - g.w : x);
-                    y = y < 0 ? y + g.h : (y >= g.h ? y - g.h : y);
-                }
-                else
-                {
-                    x = wrapi(x, g.w);
-                    y = wrapi(y, g.h);
-                }
-            }
-            const long long p = ibase + (long long)y * g.w + x;
-            v = INFER ? f2bf((float)(io.r[p] * inv)) : f2bf(io.xin[p]);
+            if (tid < off)
+                red[tid] += red[tid + off];
+            __syncthreads();
         }
-        sx[q] = v;
+        sg  = sqrt(red[0] / io.n);
+        inv = sg > 0 ? 1.0 / sg : 0.0;
+        __syncthreads(); // red[0] is read before the first tile overwrites it
     }
-    __syncthreads();
-
-    const int lane = tid & 63, wave = tid >> 6, m = lane & 15, kg = lane >> 4;
-    // layer 1 on [-3, FT+3)^2: A = the 9 taps of x at the pixel
+    double xv[F_XL];
     {
-        const v4s b = w1_operand(Wb + OFF1, m, kg);
-        for (int tile = wave; tile * 16 < FR1 * FR1; tile += 4)
+        const int img = T / (ntx * nty), rem = T - img * ntx * nty;
+        fused_loads<PERIODIC, INFER>(io, g, (rem % ntx) * FT, (rem / ntx) * FT,
+                                     (long long)img * g.h * g.w, big, tid, xv);
+    }
+    for (; T < ntiles; T += gridDim.x)
+    {
+        const int       img   = T / (ntx * nty), rem = T - img * ntx * nty;
+        const int       ox    = (rem % ntx) * FT, oy = (rem / ntx) * FT;
+        const long long ibase = (long long)img * g.h * g.w;
+#pragma unroll
+        for (int u = 0; u < F_XL; ++u)
         {
-            const int pm = tile * 16 + m;
-            v4s       a  = { 0, 0, 0, 0 };
-            if (pm < FR1 * FR1)
+            const int q = tid + u * F_THR;
+            if (q < FX * FX)
+                sx[q] = INFER ? f2bf((float)(xv[u] * inv)) : f2bf((float)xv[u]);
+        }
+        __syncthreads();
+        const int Tn = T + (int)gridDim.x;
+        if (Tn < ntiles)
+        {
+            const int imn = Tn / (ntx * nty), rmn = Tn - imn * ntx * nty;
+            fused_loads<PERIODIC, INFER>(io, g, (rmn % ntx) * FT, (rmn / ntx) * FT,
+                                         (long long)imn * g.h * g.w, big, tid, xv);
+        }
+
+        const int lane = tid & 63, wave = tid >> 6, m = lane & 15, kg = lane >> 4;
+        // layer 1 on [-3, FT+3)^2: B = the 9 taps of x at the pixel (lane: taps
+        // 4kg .. 4kg+3 of pixel m, offsets fixed for the layer)
+        {
+            const v4s a = w1_operand(Wb + OFF1, m, kg);
+            // taps 9..15 (zero weights) read tap 0's neighbour, a pixel past the
+            // region's end reads pixel 0: no masked reads (as in the 16 -> 16
+            // layers)
+            int tof[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
             {
-                const int im = pm % FR1, jm = pm / FR1;
+                const int t = 4 * kg + v;
+                tof[v]      = t < 9 ? (t / 3) * FX + t % 3 : 0;
+            }
+            // two tiles per iteration (independent chains)
+            TileWalk<FR1, 32 * F_WAVE> w1(wave * 16 + m), w2((wave + F_WAVE) * 16 + m);
+            for (int tile = wave; tile * 16 < FR1 * FR1; tile += 2 * F_WAVE)
+            {
+                const int base1 = w1.p < FR1 * FR1 ? w1.j * FX + w1.i : 0;
+                const int base2 = w2.p < FR1 * FR1 ? w2.j * FX + w2.i : 0;
+                v4s       b1, b2;
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
                 {
-                    const int t = 4 * kg + v;
-                    a[v] = t < 9 ? (short)sx[(jm + t / 3) * FX + im + t % 3] : (short)0;
+                    b1[v] = (short)sx[base1 + tof[v]];
+                    b2[v] = (short)sx[base2 + tof[v]];
                 }
+                v4f c1 = { 0.f, 0.f, 0.f, 0.f }, c2 = c1;
+                c1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b1, c1, 0, 0, 0);
+                c2 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b2, c2, 0, 0, 0);
+                store_act<PERIODIC, FR1, NP1>(sh1, ox, oy, -3, g, w1.p, w1.i, w1.j, c1, kg);
+                store_act<PERIODIC, FR1, NP1>(sh1, ox, oy, -3, g, w2.p, w2.i, w2.j, c2, kg);
+                w1.next();
+                w2.next();
             }
-            v4f acc = { 0.f, 0.f, 0.f, 0.f };
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-            store_act<PERIODIC, FR1, NP1>(sh1, ox, oy, -3, g, tile, acc, m, kg);
         }
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // layers 2 and 3
-    conv16_tiles<PERIODIC, FR2, NP1, NP2>(sh1, sh2, ox, oy, -2, g, Wb + OFF2, wave, m, kg);
-    __syncthreads();
-    conv16_tiles<PERIODIC, FR3, NP2, NP3>(sh2, sh3, ox, oy, -1, g, Wb + OFF3, wave, m, kg);
-    __syncthreads();
+        // layers 2 and 3
+        conv16_rows<PERIODIC, FR2, NP1, NP2, 4>(sh1, sh2, ox, oy, -2, g, Wb + OFF2, wave, m, kg);
+        __syncthreads();
+        conv16_rows<PERIODIC, FR3, NP2, NP3, 4>(sh2, sh3, ox, oy, -1, g, Wb + OFF3, wave, m, kg);
+        __syncthreads();
 
-    // layer 4, part 1: per-tap partials on [-1, FT+1)^2
-    {
-        const v4s b = w4_operand(Wb + OFF4, m, kg);
-        for (int tile = wave; tile * 16 < FR3 * FR3; tile += 4)
-        {
-            const int pm = tile * 16 + m;
-            v4s       a  = { 0, 0, 0, 0 };
-            if (pm < FR3 * FR3)
-                a = *reinterpret_cast<const v4s *>(sh3 + (kg * NP3 + pm) * 4);
-            v4f acc = { 0.f, 0.f, 0.f, 0.f };
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-            if (m < 9)
+        // the operator diagonal at this thread's output pixels, in flight
+        // during layer 4's first part
+        double dg[F_OL];
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
+        for (int u = 0; u < F_OL; ++u)
+        {
+            const long long op = out_pixel(g, ox, oy, ibase, tid, u);
+            dg[u]              = INFER && op >= 0 ? io.D[op] : 1.0;
+        }
+        // layer 4, part 1: per-tap partials on [-1, FT+1)^2 (lane: taps 4kg ..
+        // 4kg+3 of pixel m; rows 9..15 of W4 are zero)
+        {
+            const v4s a = w4_operand(Wb + OFF4, m, kg);
+            for (int tile = wave; tile * 16 < FR3 * FR3; tile += 2 * F_WAVE)
+            {
+                int pm[2];
+                v4s b[2];
+                v4f acc[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
                 {
-                    const int po = tile * 16 + 4 * kg + v;
-                    if (po < FR3 * FR3)
-                        sP[m * NP3 + po] = acc[v];
+                    pm[h] = (tile + h * F_WAVE) * 16 + m;
+                    b[h]  = *reinterpret_cast<const v4s *>(
+                        sh3 + (kg * NP3 + (pm[h] < FR3 * FR3 ? pm[h] : 0)) * 4);
                 }
-        }
-    }
-    __syncthreads();
-
-    // layer 4, part 2: y = sum over taps of the partials at the neighbours
-    for (int q = tid; q < FT * FT; q += blockDim.x)
-    {
-        const int i = q % FT, j = q / FT;
-        const int x = ox + i, y = oy + j;
-        if (x >= g.w || y >= g.h)
-            continue;
-        float s = 0.f;
 #pragma unroll
-        for (int t = 0; t < 9; ++t)
+                for (int h = 0; h < 2; ++h)
+                {
+                    acc[h] = v4f{ 0.f, 0.f, 0.f, 0.f };
+                    acc[h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[h], acc[h], 0, 0, 0);
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    if (pm[h] < FR3 * FR3 && kg < 3)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            if (4 * kg + v < 9)
+                                sP[(4 * kg + v) * NP3 + pm[h]] = acc[h][v];
+            }
+        }
+        __syncthreads();
+
+        // layer 4, part 2: y = sum over taps of the partials at the neighbours
+#pragma unroll
+        for (int u = 0; u < F_OL; ++u)
         {
-            if (!PERIODIC && !inside(x + t % 3 - 1, y + t / 3 - 1))
+            const long long op = out_pixel(g, ox, oy, ibase, tid, u);
+            if (op < 0)
                 continue;
-            s += sP[t * NP3 + (j + t / 3) * FR3 + i + t % 3];
+            const int q = tid + u * F_THR;
+            const int i = q % FT, j = q / FT;
+            const int x = ox + i, y = oy + j;
+            float     s = 0.f;
+#pragma unroll
+            for (int t = 0; t < 9; ++t)
+            {
+                const int xx = x + t % 3 - 1, yy = y + t / 3 - 1;
+                if (!PERIODIC && (xx < 0 || yy < 0 || xx >= g.w || yy >= g.h))
+                    continue;
+                s += sP[t * NP3 + (j + t / 3) * FR3 + i + t % 3];
+            }
+            if (INFER)
+                io.e[op] = io.omega * sg * (double)s / dg[u];
+            else
+                io.y[op] = s;
         }
-        const long long p = ibase + (long long)y * g.w + x;
-        if (INFER)
-            io.e[p] = io.omega * io.ss[0] * (double)s / io.D[p];
-        else
-            io.y[p] = s;
     }
 }
 
@@ -977,7 +1150,7 @@ struct bx_cnn
     long long       cap = 0, epoch = 0;
     unsigned short *x = nullptr, *h1 = nullptr, *h2 = nullptr, *h3 = nullptr;
     float          *y = nullptr, *P = nullptr; // P: layer-4 partials, 16 per pixel
-    double         *sig = nullptr, *part = nullptr;
+    double         *part = nullptr; // partial sums of r^2
     // training buffers
     long long tcap = 0;
     float    *gy = nullptr, *g3 = nullptr, *g2 = nullptr, *g1 = nullptr;
@@ -1019,7 +1192,7 @@ extern "C" void bx_cnn_destroy(bx_cnn *c)
     for (void *p : { (void *)c->w, (void *)c->wr, (void *)c->wb, (void *)c->grad,
                      (void *)c->adam_m, (void *)c->adam_v, (void *)c->D,
                      (void *)c->x, (void *)c->h1, (void *)c->h2, (void *)c->h3,
-                     (void *)c->y, (void *)c->P, (void *)c->sig, (void *)c->part, (void *)c->gy,
+                     (void *)c->y, (void *)c->P, (void *)c->part, (void *)c->gy,
                      (void *)c->g3, (void *)c->g2, (void *)c->g1, (void *)c->te,
                      (void *)c->tres, (void *)c->tsig, (void *)c->trn2,
                      (void *)c->tloss, (void *)c->txy, (void *)c->wpart })
@@ -1046,7 +1219,7 @@ extern "C" int bx_cnn_create(unsigned seed, bx_cnn **out)
         || hipMalloc((void **)&c->wb, NPAR * 2) || hipMalloc((void **)&c->grad, NPAR * 4)
         || hipMalloc((void **)&c->adam_m, NPAR * 4)
         || hipMalloc((void **)&c->adam_v, NPAR * 4)
-        || hipMalloc((void **)&c->sig, 64) || hipMalloc((void **)&c->part, 1024 * 8))
+        || hipMalloc((void **)&c->part, SS_BLOCKS * 8))
     {
         bx_cnn_destroy(c);
         return cerr(BX_ERR_HIP, "bx_cnn_create: hipMalloc failed");
@@ -1175,10 +1348,24 @@ extern "C" int bx_cnn_set_operator(bx_cnn *c, int nx, int ny, double alpha,
 
 
 
-static dim3 fused_grid(const Geo &g)
+// The fused network over images g: FT x FT tiles, a persistent grid of two
+// blocks per compute unit (the LDS of two blocks fits a CU).
+template <bool PERIODIC, bool INFER>
+static int launch_fused(const FusedIO &io, const unsigned short *wb, const Geo &g,
+                        hipStream_t s)
 {
-    return dim3((unsigned)((g.w + FT - 1) / FT), (unsigned)((g.h + FT - 1) / FT),
-                (unsigned)g.nimg);
+    const int ntx = (g.w + FT - 1) / FT, nty = (g.h + FT - 1) / FT;
+    const long long nt = (long long)ntx * nty * g.nimg;
+    if (nt > 0x7fffffffLL)
+        return cerr(BX_ERR_INVALID, "network: too many tiles in one launch");
+    int dev = 0, cus = 0;
+    C_CHECK(hipGetDevice(&dev));
+    C_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int nb = (int)std::min<long long>(nt, 2LL * std::max(cus, 1));
+    hipLaunchKernelGGL((k_net_fused<PERIODIC, INFER>), dim3((unsigned)nb), dim3(F_THR), 0, s, io,
+                       wb, g, ntx, nty, (int)nt);
+    C_CHECK(hipGetLastError());
+    return BX_OK;
 }
 
 template <bool PER>
@@ -1207,13 +1394,9 @@ int bx_cnn_forward(bx_cnn *c, const double *r, double *e, int nx, int ny,
     int             rc;
     if ((rc = ensure_infer(c, n)))
         return rc;
-    hipLaunchKernelGGL(k_sumsq, dim3(512), dim3(256), 0, s, r, n, c->part);
-    hipLaunchKernelGGL(k_rms_final, dim3(1), dim3(256), 0, s, c->part, 512, n, c->sig);
-    FusedIO io{ r, nullptr, c->sig, c->D, e, nullptr, omega };
-    hipLaunchKernelGGL((k_net_fused<true, true>), fused_grid(Geo{ nx, ny, 1 }), dim3(256),
-                       0, s, io, c->wb, Geo{ nx, ny, 1 });
-    C_CHECK(hipGetLastError());
-    return BX_OK;
+    hipLaunchKernelGGL(k_sumsq, dim3(SS_BLOCKS), dim3(256), 0, s, r, n, c->part);
+    FusedIO io{ r, nullptr, c->part, (double)n, c->D, e, nullptr, omega };
+    return launch_fused<true, true>(io, c->wb, Geo{ nx, ny, 1 }, s);
 }
 
 extern "C" int bx_cnn_infer(bx_cnn *c, const double *r, double *e, int nx, int ny,
@@ -1239,14 +1422,10 @@ extern "C" int bx_cnn_net_forward(bx_cnn *c, const float *x_dev, float *y_dev, i
     {
         if (n > 0x7fffffffLL)
             return cerr(BX_ERR_INVALID, "network: more than 2^31 pixels in one launch");
-        FusedIO io{ nullptr, x_dev, nullptr, nullptr, nullptr, y_dev, 0.0 };
-        if (periodic)
-            hipLaunchKernelGGL((k_net_fused<true, false>), fused_grid(g), dim3(256), 0, s,
-                               io, c->wb, g);
-        else
-            hipLaunchKernelGGL((k_net_fused<false, false>), fused_grid(g), dim3(256), 0, s,
-                               io, c->wb, g);
-        C_CHECK(hipGetLastError());
+        FusedIO io{ nullptr, x_dev, nullptr, 0.0, nullptr, nullptr, y_dev, 0.0 };
+        if ((rc = periodic ? launch_fused<true, false>(io, c->wb, g, s)
+                           : launch_fused<false, false>(io, c->wb, g, s)))
+            return rc;
         C_CHECK(hipStreamSynchronize(s));
         return BX_OK;
     }
@@ -1304,13 +1483,15 @@ static int ensure_train(bx_cnn *c, long long npx, int batch)
 // One loss + gradient evaluation on the crops `xy` (device, batch x 3 ints).
 // Every sum is formed in a fixed order (per-block partials, then the blocks
 // in block order), so the loss and the gradient are the same on every run.
-static int loss_grad(bx_cnn *c, const double *R, int nfields, int nx, int ny,
-                     int batch, int cs, double *loss_host, hipStream_t s)
+// Loss (summed over the batch, into *loss_dev) and gradient (c->grad) of one
+// batch of crops (xy: device, per crop field, x0, y0); no host
+// synchronisation.
+static int loss_grad(bx_cnn *c, const double *R, int nx, int ny, int batch, int cs,
+                     const int *xy, double *loss_dev, hipStream_t s)
 {
-    (void)nfields;
     const int       border = 5;
     const long long npx    = (long long)batch * cs * cs;
-    CropSet         cset{ R, c->txy, nx, ny, cs };
+    CropSet         cset{ R, xy, nx, ny, cs };
     Geo             g{ cs, cs, batch };
     const unsigned  nb = (unsigned)((npx + 255) / 256);
     hipLaunchKernelGGL(k_crop_stats, dim3(batch), dim3(256), 0, s, cset, c->tsig,
@@ -1323,7 +1504,7 @@ static int loss_grad(bx_cnn *c, const double *R, int nfields, int nx, int ny,
     hipLaunchKernelGGL(k_crop_res, dim3(LOSS_BLOCKS), dim3(256), 0, s, cset, c->opr, c->te,
                        c->trn2, c->tres, c->tloss + 1, batch, border);
     hipLaunchKernelGGL(k_sum_partials_f64, dim3(1), dim3(64), 0, s, c->tloss + 1, LOSS_BLOCKS,
-                       c->tloss);
+                       loss_dev);
     const float inv_b = 1.0f / (float)batch;
     hipLaunchKernelGGL(k_crop_gy, dim3(2048), dim3(256), 0, s, cset, c->opr, c->tres,
                        c->trn2, c->tsig, c->gy, batch, inv_b);
@@ -1343,7 +1524,7 @@ static int loss_grad(bx_cnn *c, const double *R, int nfields, int nx, int ny,
     hipLaunchKernelGGL((k_bwd16_weight<WG_CH>), dim3(nblk), dim3(256), 0, s, c->g2, c->h1,
                        p2, WG_ROW, g);
     // layers 4 and 1 weights
-    hipLaunchKernelGGL((k_bwd_weight_small<WG_CH>), dim3(nblk), dim3(288), 0, s, c->gy, c->h3,
+    hipLaunchKernelGGL((k_bwd_weight_small<WG_CH>), dim3(nblk), dim3(256), 0, s, c->gy, c->h3,
                        c->g1, c->x, p14, WG_ROW, g);
     // the blocks' partials in block order
     hipLaunchKernelGGL(k_sum_partials_f32, dim3((NW2 + 255) / 256), dim3(256), 0, s, p3,
@@ -1355,10 +1536,6 @@ static int loss_grad(bx_cnn *c, const double *R, int nfields, int nx, int ny,
     hipLaunchKernelGGL(k_sum_partials_f32, dim3(1), dim3(256), 0, s, p14 + NW4, (int)nblk,
                        WG_ROW, NW1, c->grad + OFF1);
     C_CHECK(hipGetLastError());
-    double l = 0;
-    C_CHECK(hipMemcpyAsync(&l, c->tloss, 8, hipMemcpyDeviceToHost, s));
-    C_CHECK(hipStreamSynchronize(s));
-    *loss_host = l / batch;
     return BX_OK;
 }
 
@@ -1379,14 +1556,22 @@ extern "C" int bx_cnn_loss_grad(bx_cnn *c, const double *R, int nfields, int nx,
     if ((rc = ensure_train(c, (long long)batch * crop * crop, batch)))
         return rc;
     C_CHECK(hipMemcpyAsync(c->txy, crops_host, batch * 12, hipMemcpyHostToDevice, s));
-    if ((rc = loss_grad(c, R, nfields, nx, ny, batch, crop, loss, s)))
+    if ((rc = loss_grad(c, R, nx, ny, batch, crop, c->txy, c->tloss, s)))
         return rc;
+    double l = 0;
+    C_CHECK(hipMemcpyAsync(&l, c->tloss, 8, hipMemcpyDeviceToHost, s));
+    C_CHECK(hipStreamSynchronize(s));
+    *loss = l / batch;
     if (grad_host)
     {
         C_CHECK(hipMemcpy(grad_host, c->grad, NPAR * 4, hipMemcpyDeviceToHost));
     }
     return BX_OK;
 }
+
+static int train_steps(bx_cnn *c, const double *R, int nx, int ny, int crop, int batch,
+                       int steps, float lr, const std::vector<int> &xy, int *dxy,
+                       double *dloss, double *loss_history, hipStream_t s);
 
 extern "C" int bx_cnn_train(bx_cnn *c, const double *R, int nfields, int nx, int ny,
                             int crop, int batch, int steps, float lr, unsigned seed,
@@ -1401,23 +1586,41 @@ extern "C" int bx_cnn_train(bx_cnn *c, const double *R, int nfields, int nx, int
     int         rc;
     if ((rc = ensure_train(c, (long long)batch * crop * crop, batch)))
         return rc;
+    // every step's crops drawn up front and uploaded once, the losses kept
+    // on the device: the steps run without host synchronisation
     std::mt19937     rng(seed);
-    std::vector<int> xy(3 * batch);
-    const float      b1 = 0.9f, b2 = 0.999f;
+    std::vector<int> xy(3LL * batch * steps);
+    for (long long i = 0; i < (long long)batch * steps; ++i)
+    {
+        xy[3 * i]     = (int)(rng() % (unsigned)nfields);
+        xy[3 * i + 1] = (int)(rng() % (unsigned)nx);
+        xy[3 * i + 2] = (int)(rng() % (unsigned)ny);
+    }
+    int    *dxy   = nullptr;
+    double *dloss = nullptr;
+    if (hipMalloc((void **)&dxy, xy.size() * sizeof(int)) != hipSuccess
+        || hipMalloc((void **)&dloss, (size_t)steps * sizeof(double)) != hipSuccess)
+    {
+        cfree(dxy);
+        return cerr(BX_ERR_HIP, "bx_cnn_train: hipMalloc failed");
+    }
+    rc = train_steps(c, R, nx, ny, crop, batch, steps, lr, xy, dxy, dloss, loss_history, s);
+    cfree(dxy);
+    cfree(dloss);
+    return rc;
+}
+
+static int train_steps(bx_cnn *c, const double *R, int nx, int ny, int crop, int batch,
+                       int steps, float lr, const std::vector<int> &xy, int *dxy,
+                       double *dloss, double *loss_history, hipStream_t s)
+{
+    int         rc;
+    const float b1 = 0.9f, b2 = 0.999f;
+    C_CHECK(hipMemcpyAsync(dxy, xy.data(), xy.size() * sizeof(int), hipMemcpyHostToDevice, s));
     for (int st = 0; st < steps; ++st)
     {
-        for (int b = 0; b < batch; ++b)
-        {
-            xy[3 * b]     = (int)(rng() % (unsigned)nfields);
-            xy[3 * b + 1] = (int)(rng() % (unsigned)nx);
-            xy[3 * b + 2] = (int)(rng() % (unsigned)ny);
-        }
-        C_CHECK(hipMemcpyAsync(c->txy, xy.data(), batch * 12, hipMemcpyHostToDevice, s));
-        double l = 0;
-        if ((rc = loss_grad(c, R, nfields, nx, ny, batch, crop, &l, s)))
+        if ((rc = loss_grad(c, R, nx, ny, batch, crop, dxy + 3LL * batch * st, dloss + st, s)))
             return rc;
-        if (loss_history)
-            loss_history[st] = l;
         ++c->adam_t;
         const float c1 = 1.f - std::pow(b1, (float)c->adam_t);
         const float c2 = 1.f - std::pow(b2, (float)c->adam_t);
@@ -1427,6 +1630,12 @@ extern "C" int bx_cnn_train(bx_cnn *c, const double *R, int nfields, int nx, int
         if ((rc = refresh_copies(c, s)))
             return rc;
     }
+    std::vector<double> l(steps);
+    C_CHECK(hipMemcpyAsync(l.data(), dloss, (size_t)steps * sizeof(double), hipMemcpyDeviceToHost,
+                           s));
     C_CHECK(hipStreamSynchronize(s));
+    if (loss_history)
+        for (int st = 0; st < steps; ++st)
+            loss_history[st] = l[st] / batch;
     return BX_OK;
 }

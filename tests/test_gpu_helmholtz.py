@@ -216,6 +216,37 @@ def test_cnn_forward_random_weights():
     net.close()
 
 
+def test_cnn_infer_matches_oracle():
+    """The correction the solver applies, e = omega sigma N(r / sigma) / D
+    (sigma = rms of r, D = the operator diagonal), from the fused inference
+    kernel on a periodic 96 x 112 level against the oracle's network on the
+    same bf16-rounded input, at the tolerance of the MFMA layers' FP32
+    sums."""
+    announce("cnn infer 96x112 periodic")
+    nx, ny = 112, 96
+    rng = np.random.default_rng(21)
+    a, bX, bY = rand_coeffs(rng, nx, ny)
+    s = HelmholtzSolver(nx, ny, 1.0 / nx, 1.0 / ny)
+    s.set_coeffs(1.0, 2e-3, a, bX, bY)
+    net = LearnedSmoother(seed=3)
+    p = (rng.normal(size=co.NPARAMS) * 0.3).astype(np.float32)
+    net.set_params(p)
+    s.attach(net)
+    r = rng.normal(size=(ny, nx)) * 1e-3
+    dr, de = dev(r), DeviceArray((ny, nx))
+    net.infer(dr, de, omega=0.8)
+    e = de.download()
+    sigma = np.sqrt(np.mean(r * r))
+    y, _ = co.forward(p, (r / sigma)[None], periodic=True, quantize=True)
+    D = co.op_diag(a, bX, bY, 1.0, 2e-3, 1.0 / nx, 1.0 / ny)
+    ref = 0.8 * sigma * y[0].astype(np.float64) / D
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(e - ref)) <= 3e-2 * scale
+    assert np.mean(np.abs(e - ref)) <= 2e-3 * scale
+    s.close()
+    net.close()
+
+
 def test_cnn_loss_and_gradient_match_oracle():
     announce("cnn loss/grad 96x96 batch 6 crop 32")
     n = 96

@@ -479,17 +479,27 @@ __global__ __launch_bounds__(256) void k_crop_res(CropSet c, OpRef o,
         loss_part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// out = sum of n partials, one thread, in index order
-__global__ void k_sum_partials_f64(const double *__restrict__ part, int n,
-                                   double *__restrict__ out)
+// *out = sum of n partials in a fixed order: thread t of one 256-thread
+// block adds a contiguous run of them in index order, then a fixed tree
+// adds the 256 run sums
+__global__ __launch_bounds__(256) void k_sum_partials_f64(const double *__restrict__ part, int n,
+                                                          double *__restrict__ out)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0)
+    __shared__ double sh[256];
+    const int per = (n + 255) / 256, b0 = threadIdx.x * per, b1 = min(n, b0 + per);
+    double    s = 0;
+    for (int b = b0; b < b1; ++b)
+        s += part[b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1)
     {
-        double s = 0;
-        for (int i = 0; i < n; ++i)
-            s += part[i];
-        *out = s;
+        if ((int)threadIdx.x < off)
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
     }
+    if (threadIdx.x == 0)
+        *out = sh[0];
 }
 
 // g_y = dL/dy = -2 A^T(res / ||mask r||^2) * sigma / D   (A symmetric)
@@ -652,18 +662,30 @@ __global__ __launch_bounds__(256) void k_bwd16_weight(const float *__restrict__ 
 }
 
 // out[i] = sum over the nblk partial rows of part[b * stride + i], i < nval,
-// in block order
+// in a fixed order: the rows are split into 16 consecutive groups, each
+// added in row order by its own thread, and the group sums are added in
+// group order (16 values per block)
 __global__ __launch_bounds__(256) void k_sum_partials_f32(const float *__restrict__ part,
                                                           int nblk, int stride, int nval,
                                                           float *__restrict__ out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nval)
-        return;
-    float s = 0.f;
-    for (int b = 0; b < nblk; ++b)
-        s += part[(long long)b * stride + i];
-    out[i] = s;
+    __shared__ float sh[16][17];
+    const int li = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int i   = blockIdx.x * 16 + li;
+    const int per = (nblk + 15) / 16, b0 = grp * per, b1 = min(nblk, b0 + per);
+    float     s   = 0.f;
+    if (i < nval)
+        for (int b = b0; b < b1; ++b)
+            s += part[(long long)b * stride + i];
+    sh[grp][li] = s;
+    __syncthreads();
+    if (grp == 0 && i < nval)
+    {
+        float t = 0.f;
+        for (int q = 0; q < 16; ++q)
+            t += sh[q][li];
+        out[i] = t;
+    }
 }
 
 // Layers 4 and 1, one GEMM each (tap t = row or column 0..8, 9..15 zero):
@@ -1503,7 +1525,7 @@ static int loss_grad(bx_cnn *c, const double *R, int nx, int ny, int batch, int 
                        c->y, c->te, batch);
     hipLaunchKernelGGL(k_crop_res, dim3(LOSS_BLOCKS), dim3(256), 0, s, cset, c->opr, c->te,
                        c->trn2, c->tres, c->tloss + 1, batch, border);
-    hipLaunchKernelGGL(k_sum_partials_f64, dim3(1), dim3(64), 0, s, c->tloss + 1, LOSS_BLOCKS,
+    hipLaunchKernelGGL(k_sum_partials_f64, dim3(1), dim3(256), 0, s, c->tloss + 1, LOSS_BLOCKS,
                        loss_dev);
     const float inv_b = 1.0f / (float)batch;
     hipLaunchKernelGGL(k_crop_gy, dim3(2048), dim3(256), 0, s, cset, c->opr, c->tres,
@@ -1527,13 +1549,13 @@ static int loss_grad(bx_cnn *c, const double *R, int nx, int ny, int batch, int 
     hipLaunchKernelGGL((k_bwd_weight_small<WG_CH>), dim3(nblk), dim3(256), 0, s, c->gy, c->h3,
                        c->g1, c->x, p14, WG_ROW, g);
     // the blocks' partials in block order
-    hipLaunchKernelGGL(k_sum_partials_f32, dim3((NW2 + 255) / 256), dim3(256), 0, s, p3,
+    hipLaunchKernelGGL(k_sum_partials_f32, dim3((NW2 + 15) / 16), dim3(256), 0, s, p3,
                        (int)nblk, WG_ROW, NW2, c->grad + OFF3);
-    hipLaunchKernelGGL(k_sum_partials_f32, dim3((NW2 + 255) / 256), dim3(256), 0, s, p2,
+    hipLaunchKernelGGL(k_sum_partials_f32, dim3((NW2 + 15) / 16), dim3(256), 0, s, p2,
                        (int)nblk, WG_ROW, NW2, c->grad + OFF2);
-    hipLaunchKernelGGL(k_sum_partials_f32, dim3(1), dim3(256), 0, s, p14, (int)nblk, WG_ROW,
+    hipLaunchKernelGGL(k_sum_partials_f32, dim3((NW4 + 15) / 16), dim3(256), 0, s, p14, (int)nblk, WG_ROW,
                        NW4, c->grad + OFF4);
-    hipLaunchKernelGGL(k_sum_partials_f32, dim3(1), dim3(256), 0, s, p14 + NW4, (int)nblk,
+    hipLaunchKernelGGL(k_sum_partials_f32, dim3((NW1 + 15) / 16), dim3(256), 0, s, p14 + NW4, (int)nblk,
                        WG_ROW, NW1, c->grad + OFF1);
     C_CHECK(hipGetLastError());
     return BX_OK;

@@ -489,7 +489,7 @@ def imex_leg(args):
     of the preceding steps.  Every solve starts from the previous pressure
     and is checked by an FP64 residual computed independently."""
     import numpy as np
-    from bamrex_amd import imex
+    from bamrex_amd import _lib, imex
     n = args.imex_n
     log("imex: Gresho %d^2, harvesting %d systems (+4 held out)" % (n, args.imex_harvest_systems))
     wl = imex.GreshoWorkload(n, advection=args.imex_advection, tableau=args.imex_tableau,
@@ -567,6 +567,29 @@ def imex_leg(args):
         "residual reduction V %.4g, V+untrained %.4g, V+trained %.4g" % (
             args.imex_train_steps, args.imex_batch, train_s, float(np.mean(hist[:10])),
             float(np.mean(hist[-10:])), rho["V"], rho["V+untrained"], rho["V+trained"]))
+    # one network application (bx_cnn_infer: sigma partial sums + the fused
+    # kernel) on a held-out field, 30 back-to-back calls between HIP events
+    # on the default stream the calls use
+    r_in, e_out = imex.DeviceArray((n, n)), imex.DeviceArray((n, n))
+    r_in.upload(Rh.download()[0])
+    L = imex.lib()
+    for _ in range(5):
+        _lib.check(L.bx_cnn_infer(net.c, r_in.ptr, e_out.ptr, n, n, 1.0, None), "bx_cnn_infer")
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    _lib.check(L.bx_event_create(ctypes.byref(ev0)), "event")
+    _lib.check(L.bx_event_create(ctypes.byref(ev1)), "event")
+    _lib.check(L.bx_event_record(ev0, None), "record")
+    for _ in range(30):
+        _lib.check(L.bx_cnn_infer(net.c, r_in.ptr, e_out.ptr, n, n, 1.0, None), "bx_cnn_infer")
+    _lib.check(L.bx_event_record(ev1, None), "record")
+    ms = ctypes.c_float()
+    _lib.check(L.bx_event_elapsed_ms(ev0, ev1, ctypes.byref(ms)), "elapsed")
+    L.bx_event_destroy(ev0)
+    L.bx_event_destroy(ev1)
+    infer_us = ms.value * 1e3 / 30
+    log("imex: network application %.1f us at %d^2" % (infer_us, n))
+    r_in.free()
+    e_out.free()
     R.free()
     Rh.free()
     # (label, solve method, network mode, graph-replayed preconditioner)
@@ -649,7 +672,7 @@ def imex_leg(args):
            "held_out_fields": nh, "held_out_residual_reduction": rho,
            "train_steps": args.imex_train_steps, "train_batch": args.imex_batch,
            "train_crop": args.imex_crop, "train_crops_total": args.imex_train_steps * args.imex_batch,
-           "train_seconds": round(train_s, 3),
+           "train_seconds": round(train_s, 3), "network_application_us": round(infer_us, 2),
            "train_loss_first_last": [float(np.mean(hist[:10])), float(np.mean(hist[-10:]))]}
     for label, meth, mode, graph in methods:
         out[label] = {"s_per_solve": float(np.mean(res[label]["seconds"])),

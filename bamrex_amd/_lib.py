@@ -167,6 +167,7 @@ _SIGS = {
                                             ctypes.POINTER(ctypes.c_double), P]),
     "bx_amr_run_levels": (ctypes.c_int, [P, DP, ctypes.c_double, ctypes.c_double, ctypes.c_int, P, P, P,
                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), P]),
+    "bx_amr_set_graph": (ctypes.c_int, [P, ctypes.c_int]),
     "bx_helm_last_error": (ctypes.c_char_p, []),
     "bx_imex_pressure": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, P,
                                         P]),

@@ -95,6 +95,11 @@ class AmrHierarchy:
                                      self.fine.U_tmp.ptr, self.stream), "bx_amr_advance")
         L.lib().bx_stream_sync(self.stream)
 
+    def set_graph(self, use_graph):
+        """Runs of a fixed number of steps replay one captured coarse step
+        (default) or launch every kernel from the host (same bits)."""
+        check(L.lib().bx_amr_set_graph(self.a, int(bool(use_graph))), "bx_amr_set_graph")
+
     def run(self, dx_coarse, cfl, stop_time=-1.0, max_step=-1, start_time=0.0):
         t = ctypes.c_double(start_time)
         n = ctypes.c_int()
@@ -161,6 +166,11 @@ class AmrLevels:
 
     def get_state(self):
         return [lv.get_state() for lv in self.levels]
+
+    def set_graph(self, use_graph):
+        """Runs of a fixed number of steps replay one captured coarse step
+        (default) or launch every kernel from the host (same bits)."""
+        check(L.lib().bx_amr_set_graph(self.a, int(bool(use_graph))), "bx_amr_set_graph")
 
     def run(self, dx0, cfl, stop_time=-1.0, max_step=-1, start_time=0.0):
         t = ctypes.c_double(start_time)

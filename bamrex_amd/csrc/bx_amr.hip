@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -94,7 +95,8 @@ __device__ __forceinline__ double cf_mc_slope(double um, double uc, double up)
 }
 
 // Coarse-fine ghost fill.  One block per segment; threads over its cells.
-// Time weights: mode 0 = the coarse old state, 1 = alpha old + beta new.
+// Time weights: mode 0 = the coarse old state, 1 = alpha old + beta new
+// (from w[0..2] when w is given).
 // Per component: MC slopes in each direction, scaled by the largest
 // alpha <= 1 that keeps the four (2D) / eight (3D) children of the coarse
 // cell within the minimum and maximum of its 3^D neighbourhood; the child
@@ -104,9 +106,15 @@ __global__ __launch_bounds__(256) void k_cf_fill(
     double *__restrict__ Uf, const DevBox *__restrict__ fboxes,
     const double *__restrict__ Uold, const double *__restrict__ Unew,
     const DevBox *__restrict__ cboxes, const CFSeg *__restrict__ segs, int nseg, int ngf,
-    int ngc, int mode, double alpha, double beta)
+    int ngc, int mode, double alpha, double beta, const double *__restrict__ w)
 {
     constexpr int NC = D + 2;
+    if (w) // mode, alpha, beta from device memory (a replayed step)
+    {
+        mode  = (int)w[0];
+        alpha = w[1];
+        beta  = w[2];
+    }
     for (int si = blockIdx.x; si < nseg; si += gridDim.x)
     {
         const CFSeg  &g  = segs[si];
@@ -172,17 +180,25 @@ __global__ __launch_bounds__(256) void k_cf_fill(
 // area), each evaluated by the same face computation the sweeps use.
 // accumulate = 0: reg = -F (CrseInit with -1, nsub = 1);
 // accumulate = 1: reg += F for each face in order (FineAdd with +1).
+// coef: when given, the coefficients come from device memory.
 template <int D, int DIR, int LIM, int LV, int SCH>
 __global__ __launch_bounds__(256) void k_reg_flux(
     const double *__restrict__ U, const DevBox *__restrict__ boxes,
     const RegFace *__restrict__ faces, int nslot, int nsub, double *__restrict__ reg,
     int accumulate, int ng, double hdtdx, double fc, double rc, double scale,
-    EulerParm pm, LimTable lt)
+    const double *__restrict__ coef, EulerParm pm, LimTable lt)
 {
     constexpr int NC = D + 2;
     const int     sl = blockIdx.x * blockDim.x + threadIdx.x;
     if (sl >= nslot)
         return;
+    if (coef) // hdtdx, fc, rc, scale from device memory (a replayed step)
+    {
+        hdtdx = coef[0];
+        fc    = coef[1];
+        rc    = coef[2];
+        scale = coef[3];
+    }
     for (int q = 0; q < nsub; ++q)
     {
         const RegFace  &f  = faces[(long long)sl * nsub + q];
@@ -293,16 +309,142 @@ __global__ __launch_bounds__(256) void k_avg_down(double *__restrict__ Uc,
     }
 }
 
+// y = x, n doubles (the level copies of a step: kernels, so a captured step
+// holds no copy nodes)
+__global__ __launch_bounds__(256) void k_copy_f64(double *__restrict__ y,
+                                                  const double *__restrict__ x, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        y[i] = x[i];
+}
+
+// ---- device-side step schedule (a captured coarse step, replayed) ------
+// What the host computes before and during a coarse step from the time t
+// and the levels' maximum wave speeds, computed on the device with the same
+// expressions in the same order (so the same bits): dt of every level, the
+// flux-register coefficients of every level and direction, and the time
+// weights of every coarse-fine fill in call order.
+constexpr int AMR_MAXLEV = 8;
+constexpr int AMR_MAXCF  = 256;
+struct AmrSched
+{
+    double t, dt0;
+    double coef[AMR_MAXLEV][3][4]; // hdtdx, fc, rc, scale
+    double cfw[AMR_MAXCF][3];      // mode, alpha, beta
+};
+struct SchedArgs
+{
+    const unsigned long long *maxbits[AMR_MAXLEV];
+    double                   *dt[AMR_MAXLEV];
+    double                    dx0[3];
+    double                    cfl;
+    int                       nlev, D;
+};
+
+// Amr::timeStep's recursion, in the same order (depth first, substep 0
+// before substep 1) with an explicit stack: the fine levels' ghost-fill
+// weights (two fills per call: the level's state and its ping-pong
+// buffer), each level's dt and register coefficients
+__device__ void sched_walk(AmrSched *S, const SchedArgs &a, double t, double dt_0)
+{
+    struct Call
+    {
+        int    l;
+        double time, dt, t_par, dt_par;
+    };
+    Call st[2 * AMR_MAXLEV];
+    int  sp = 0, slot = 0;
+    st[sp++] = { 0, t, dt_0, t, dt_0 };
+    while (sp > 0)
+    {
+        const Call c = st[--sp];
+        if (c.l > 0)
+        {
+            const double t_old = c.t_par, t_new = c.t_par + c.dt_par;
+            double       w[3];
+            if (c.time == t_old || !(t_new != t_old))
+            {
+                w[0] = 0;
+                w[1] = 1.0;
+                w[2] = 0.0;
+            }
+            else
+            {
+                w[0] = 1;
+                w[1] = (t_new - c.time) / (t_new - t_old);
+                w[2] = (c.time - t_old) / (t_new - t_old);
+            }
+            for (int k = 0; k < 2; ++k, ++slot)
+                for (int q = 0; q < 3; ++q)
+                    S->cfw[slot][q] = w[q];
+        }
+        *a.dt[c.l] = c.dt;
+        double dx[3] = { 0, 0, 0 };
+        for (int d = 0; d < a.D; ++d)
+            dx[d] = a.dx0[d] / (double)(1 << c.l);
+        for (int d = 0; d < a.D; ++d)
+        {
+            double scale = c.dt;
+            for (int e = 0; e < a.D; ++e)
+                if (e != d)
+                    scale *= dx[e];
+            S->coef[c.l][d][0] = 0.5 * c.dt / dx[d];
+            S->coef[c.l][d][1] = 0.5 * dx[d] / c.dt;
+            S->coef[c.l][d][2] = c.dt / (2 * dx[d]);
+            S->coef[c.l][d][3] = scale;
+        }
+        if (c.l + 1 < a.nlev)
+        {
+            const double dtf = c.dt / 2;
+            st[sp++]         = { c.l + 1, c.time + dtf, dtf, c.time, c.dt }; // substep 1
+            st[sp++]         = { c.l + 1, c.time, dtf, c.time, c.dt };       // substep 0
+        }
+    }
+}
+
+// computeNewDt without a stop time (dt_0 = min over levels of 2^l times the
+// level's CFL estimate from its maximum wave speed), the step's schedule
+// from S->t, then S->t += dt_0
+__global__ void k_amr_schedule(AmrSched *S, SchedArgs a)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    double dt_0     = 1.0e+100;
+    int    n_factor = 1;
+    for (int l = 0; l < a.nlev; ++l)
+    {
+        double dxl[3] = { 0, 0, 0 };
+        for (int d = 0; d < a.D; ++d)
+            dxl[d] = a.dx0[d] / (double)(1 << l);
+        const double max_speed = __longlong_as_double((long long)*a.maxbits[l]);
+        double       dt_est    = 1.0e20;
+        for (int d = 0; d < a.D; ++d)
+        {
+            const double c = dxl[d] / max_speed;
+            dt_est         = (c < dt_est) ? c : dt_est;
+        }
+        const double dmin = dt_est * a.cfl;
+        n_factor *= (l == 0 ? 1 : 2);
+        const double cand = n_factor * dmin;
+        dt_0              = (cand < dt_0) ? cand : dt_0;
+    }
+    const double t = S->t;
+    sched_walk(S, a, t, dt_0);
+    S->dt0 = dt_0;
+    S->t   = t + dt_0;
+}
+
 // ------------------------------------------------------ launch dispatch
 template <int D, int DIR, int LIM, int LV, int SCH>
 static hipError_t launch_reg(const double *U, const DevBox *boxes, const RegFace *faces,
                              int nslot, int nsub, double *reg, int acc, int ng,
                              double hdtdx, double fc, double rc, double scale,
-                             EulerParm pm, LimTable lt, hipStream_t s)
+                             const double *coef, EulerParm pm, LimTable lt, hipStream_t s)
 {
     hipLaunchKernelGGL((k_reg_flux<D, DIR, LIM, LV, SCH>), dim3((nslot + 127) / 128),
                        dim3(128), 0, s, U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx,
-                       fc, rc, scale, pm, lt);
+                       fc, rc, scale, coef, pm, lt);
     return hipGetLastError();
 }
 
@@ -310,23 +452,23 @@ template <int D, int DIR, int SCH>
 static hipError_t reg_lim(int lim, int lv, const double *U, const DevBox *boxes,
                           const RegFace *faces, int nslot, int nsub, double *reg, int acc,
                           int ng, double hdtdx, double fc, double rc, double scale,
-                          EulerParm pm, LimTable lt, hipStream_t s)
+                          const double *coef, EulerParm pm, LimTable lt, hipStream_t s)
 {
 #define BX_RL(LIMV)                                                                 \
     do                                                                              \
     {                                                                               \
         if (lv == LV_SHARED)                                                        \
             return launch_reg<D, DIR, LIMV, LV_SHARED, SCH>(                        \
-                U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, pm, lt, s); \
+                U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, coef, pm, lt, s); \
         if (lv == LV_OWN)                                                           \
             return launch_reg<D, DIR, LIMV, LV_OWN, SCH>(                           \
-                U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, pm, lt, s); \
+                U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, coef, pm, lt, s); \
         return launch_reg<D, DIR, LIMV, LV_TABLE, SCH>(                             \
-            U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, pm, lt, s); \
+            U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, coef, pm, lt, s); \
     } while (0)
     if constexpr (!scheme_reconstructs(SCH))
         return launch_reg<D, DIR, LIM_MINBEE, LV_SHARED, SCH>(
-            U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, pm, lt, s);
+            U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, fc, rc, scale, coef, pm, lt, s);
     else
         switch (lim)
         {
@@ -342,11 +484,11 @@ template <int D, int DIR>
 static hipError_t reg_scheme(int sch, int lim, int lv, const double *U, const DevBox *boxes,
                              const RegFace *faces, int nslot, int nsub, double *reg, int acc,
                              int ng, double hdtdx, double fc, double rc, double scale,
-                             EulerParm pm, LimTable lt, hipStream_t s)
+                             const double *coef, EulerParm pm, LimTable lt, hipStream_t s)
 {
 #define BX_RS(S)                                                                        \
     return reg_lim<D, DIR, S>(lim, lv, U, boxes, faces, nslot, nsub, reg, acc, ng, hdtdx, \
-                              fc, rc, scale, pm, lt, s)
+                              fc, rc, scale, coef, pm, lt, s)
     switch (sch)
     {
     case SCH_MHM: BX_RS(SCH_MHM);
@@ -774,6 +916,14 @@ struct bx_amr
     int                    D = 0, NC = 0;
     std::vector<bx_ctx *>  lev;
     std::vector<AmrPair *> pairs; // pairs[l] couples lev[l] and lev[l+1]
+    // the coarse step captured as a hipGraph (runs of a fixed number of
+    // steps): its device schedule, capture stream, executable and what it
+    // bakes in (buffers, cell sizes, CFL, each level's boxes and tuning)
+    int                 use_graph = 1;
+    AmrSched           *d_sched   = nullptr;
+    hipStream_t         cap       = nullptr;
+    hipGraphExec_t      gx        = nullptr;
+    std::vector<uint64_t> gkey;
 };
 
 extern "C" void bx_amr_destroy(bx_amr *a)
@@ -782,7 +932,21 @@ extern "C" void bx_amr_destroy(bx_amr *a)
         return;
     for (AmrPair *p : a->pairs)
         pair_destroy(p);
+    if (a->gx)
+        (void)hipGraphExecDestroy(a->gx);
+    if (a->cap)
+        (void)hipStreamDestroy(a->cap);
+    if (a->d_sched)
+        (void)hipFree(a->d_sched);
     delete a;
+}
+
+extern "C" int bx_amr_set_graph(bx_amr *a, int use_graph)
+{
+    if (!a)
+        return set_error(BX_ERR_INVALID, "bx_amr_set_graph: null hierarchy");
+    a->use_graph = use_graph ? 1 : 0;
+    return BX_OK;
 }
 
 static bool same_params(const bx_params &pc, const bx_params &pf)
@@ -865,8 +1029,9 @@ extern "C" int bx_amr_info(const bx_amr *a, int *n_cf_segs, int *n_register_face
 }
 
 // Flux-register contribution of one level's sweep input in direction d.
+// coef: the coefficients in device memory (a captured step), else from dt.
 static int reg_fluxes(AmrPair *a, bool fine_level, int d, const double *U, double dt,
-                      const double dx[3], hipStream_t s)
+                      const double dx[3], hipStream_t s, const double *coef)
 {
     const int nslot = a->slot0[d + 1] - a->slot0[d];
     if (nslot == 0)
@@ -888,7 +1053,7 @@ static int reg_fluxes(AmrPair *a, bool fine_level, int d, const double *U, doubl
     hipError_t     e   = hipErrorInvalidValue;
 #define BX_R(DD, DIR)                                                                    \
     reg_scheme<DD, DIR>(sch, lim, lv, U, L->d_boxes, faces, nslot, nsub, reg, fine_level, \
-                        ng, hdtdx, fc, rc, scale, L->pm, L->lt, s)
+                        ng, hdtdx, fc, rc, scale, coef, L->pm, L->lt, s)
     switch (a->D * 4 + d)
     {
     case 4: e = BX_R(1, 0); break;
@@ -904,39 +1069,56 @@ static int reg_fluxes(AmrPair *a, bool fine_level, int d, const double *U, doubl
     return BX_OK;
 }
 
+// A captured coarse step's device schedule and the slot of the next
+// coarse-fine fill in it; nullptr for a host-driven step.
+struct CapturedStep
+{
+    AmrSched *S;
+    int       slot;
+};
+
+static int copy_level(double *dst, const double *src, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_f64, dim3(2048), dim3(256), 0, s, dst, src, n);
+    BX_HIP_CHECK(hipGetLastError());
+    return BX_OK;
+}
+
 // One split step of level l from A (valid + ghosts filled), result in A; B
 // is the ping-pong buffer.  Each sweep's input feeds the flux registers of
 // the pair below (as its fine side) and of the pair above (coarse side).
+// In a captured step the schedule kernel has written the level's dt and
+// register coefficients.
 static int level_step(bx_amr *h, int l, double dt, const double dx[3], double *A, double *B,
-                      hipStream_t s)
+                      hipStream_t s, CapturedStep *g)
 {
     bx_ctx   *L = h->lev[l];
     const int D = h->D;
     int       rc;
-    if ((rc = set_scalar(L->d_dt, dt, s)) != BX_OK)
+    if (!g && (rc = set_scalar(L->d_dt, dt, s)) != BX_OK)
         return rc;
     for (int d = 0; d < D; ++d)
     {
-        const double *in  = (d % 2 == 0) ? A : B;
-        double       *out = (d % 2 == 0) ? B : A;
+        const double *in   = (d % 2 == 0) ? A : B;
+        double       *out  = (d % 2 == 0) ? B : A;
+        const double *coef = g ? g->S->coef[l][d] : nullptr;
         if ((rc = do_sweep(L, d, dx, in, out, nullptr, s)) != BX_OK)
             return rc;
-        if (l > 0 && (rc = reg_fluxes(h->pairs[l - 1], true, d, in, dt, dx, s)) != BX_OK)
+        if (l > 0 && (rc = reg_fluxes(h->pairs[l - 1], true, d, in, dt, dx, s, coef)) != BX_OK)
             return rc;
         if (l + 1 < (int)h->lev.size()
-            && (rc = reg_fluxes(h->pairs[l], false, d, in, dt, dx, s)) != BX_OK)
+            && (rc = reg_fluxes(h->pairs[l], false, d, in, dt, dx, s, coef)) != BX_OK)
             return rc;
         if ((rc = do_fill(L, out, (d + 1) % D, s)) != BX_OK)
             return rc;
     }
     if (D % 2 == 1)
-        BX_HIP_CHECK(hipMemcpyAsync(A, B, sizeof(double) * L->level_extent,
-                                    hipMemcpyDeviceToDevice, s));
+        return copy_level(A, B, L->level_extent, s);
     return BX_OK;
 }
 
 static int cf_fill(AmrPair *a, double *Uf, const double *Uold, const double *Unew, int mode,
-                   double alpha, double beta, hipStream_t s)
+                   double alpha, double beta, hipStream_t s, const double *w = nullptr)
 {
     const int nseg = (int)a->cf.size();
     if (nseg == 0)
@@ -946,7 +1128,7 @@ static int cf_fill(AmrPair *a, double *Uf, const double *Uold, const double *Une
 #define BX_CF(DD)                                                                          \
     hipLaunchKernelGGL((k_cf_fill<DD>), dim3(grid), dim3(256), 0, s, Uf, F->d_boxes, Uold, \
                        Unew, C->d_boxes, a->d_cf, nseg, F->geom.ng, C->geom.ng, mode, alpha, \
-                       beta)
+                       beta, w)
     if (a->D == 1)
         BX_CF(1);
     else if (a->D == 2)
@@ -1015,8 +1197,10 @@ static int reflux(AmrPair *p, double *Uc, const double dx_crse[3], hipStream_t s
 // coarse-fine ones from level l-1's states at t_old (Uold) and t_new (Unew).
 // Applied to both of the level's buffers when B is given (B's coarse-fine
 // ghosts only: the sweeps refill the rest).
+// In a captured step the weights come from the schedule's next slot.
 static int fill_level(bx_amr *h, int l, double *A, double *B, const double *Uold,
-                      const double *Unew, double t_old, double t_new, double time, hipStream_t s)
+                      const double *Unew, double t_old, double t_new, double time, hipStream_t s,
+                      CapturedStep *g = nullptr)
 {
     int rc;
     if ((rc = do_fill_all(h->lev[l], A, s)) != BX_OK)
@@ -1026,7 +1210,9 @@ static int fill_level(bx_amr *h, int l, double *A, double *B, const double *Uold
     {
         if (!X)
             continue;
-        if (time == t_old || !(t_new != t_old))
+        if (g)
+            rc = cf_fill(p, X, Uold, Unew, 0, 1.0, 0.0, s, g->S->cfw[g->slot++]);
+        else if (time == t_old || !(t_new != t_old))
             rc = cf_fill(p, X, Uold, Unew, 0, 1.0, 0.0, s);
         else // FillPatchTwoLevels time interpolation: alpha old + beta new
             rc = cf_fill(p, X, Uold, Unew, 1, (t_new - time) / (t_new - t_old),
@@ -1066,8 +1252,11 @@ struct LevelBufs
 // Amr::timeStep(l) (AMReX): advance level l by dt from `time`; the finer
 // levels subcycle twice, then post_timestep(l) refluxes and averages down.
 // t_par / dt_par: the parent level's step (for the ghost interpolation).
+// g: a step being captured (time, dt, t_par, dt_par then only shape the
+// recursion; the values come from the device schedule).
 static int time_step(bx_amr *h, int l, double time, double dt, double t_par, double dt_par,
-                     const double dx0[3], const LevelBufs &b, hipStream_t s, bool fill_after)
+                     const double dx0[3], const LevelBufs &b, hipStream_t s, bool fill_after,
+                     CapturedStep *g = nullptr)
 {
     const int nlev = (int)h->lev.size();
     double    dx[3] = { 0, 0, 0 };
@@ -1075,13 +1264,13 @@ static int time_step(bx_amr *h, int l, double time, double dt, double t_par, dou
         dx[d] = dx0[d] / (double)(1 << l);
     int rc;
     if (l > 0 && (rc = fill_level(h, l, b.U[l], b.T[l], b.O[l - 1], b.U[l - 1], t_par,
-                                  t_par + dt_par, time, s))
+                                  t_par + dt_par, time, s, g))
                      != BX_OK)
         return rc;
-    if (l + 1 < nlev) // the old state, ghosts filled, for the finer level's fills
-        BX_HIP_CHECK(hipMemcpyAsync(b.O[l], b.U[l], sizeof(double) * h->lev[l]->level_extent,
-                                    hipMemcpyDeviceToDevice, s));
-    if ((rc = level_step(h, l, dt, dx, b.U[l], b.T[l], s)) != BX_OK)
+    if (l + 1 < nlev // the old state, ghosts filled, for the finer level's fills
+        && (rc = copy_level(b.O[l], b.U[l], h->lev[l]->level_extent, s)) != BX_OK)
+        return rc;
+    if ((rc = level_step(h, l, dt, dx, b.U[l], b.T[l], s, g)) != BX_OK)
         return rc;
     if (l + 1 < nlev)
     {
@@ -1090,7 +1279,7 @@ static int time_step(bx_amr *h, int l, double time, double dt, double t_par, dou
         const double dtf = dt / 2; // dt_level[l+1] = dt_level[l] / 2
         for (int sub = 0; sub < 2; ++sub)
             if ((rc = time_step(h, l + 1, sub == 0 ? time : time + dtf, dtf, time, dt, dx0, b, s,
-                                false))
+                                false, g))
                 != BX_OK)
                 return rc;
         AmrPair *p = h->pairs[l];
@@ -1193,6 +1382,106 @@ extern "C" int bx_amr_est_dt_levels(bx_amr *a, const double dx0[3], double cfl, 
     return est_dt_levels(a, dx0, cfl, time, stop_time, U, dt, (hipStream_t)stream);
 }
 
+// the coarse step fits the device schedule's slots
+static bool graph_fits(const bx_amr *a)
+{
+    const int nlev = (int)a->lev.size();
+    if (nlev > AMR_MAXLEV)
+        return false;
+    long long ncf = 0; // two fills per fine-level step, 2^l steps of level l
+    for (int l = 1; l < nlev; ++l)
+        ncf += 2LL << l;
+    return ncf <= AMR_MAXCF;
+}
+
+// Captures one coarse step (the levels' maximum wave speeds, the schedule,
+// Amr::timeStep(0)) into a->gx, unless the graph already holds this
+// configuration.
+static int capture_step(bx_amr *a, const double dx0[3], double cfl, const LevelBufs &b)
+{
+    const int             nlev = (int)a->lev.size();
+    std::vector<uint64_t> key;
+    for (double v : { dx0[0], dx0[1], dx0[2], cfl })
+    {
+        uint64_t u;
+        std::memcpy(&u, &v, sizeof(u));
+        key.push_back(u);
+    }
+    for (int l = 0; l < nlev; ++l)
+    {
+        const bx_ctx *L = a->lev[l];
+        // every device table a captured launch reads (a re-upload that moves
+        // one forces a new capture)
+        for (const void *p : { (const void *)b.U[l], (const void *)b.T[l],
+                               (const void *)(l + 1 < nlev ? b.O[l] : nullptr),
+                               (const void *)L->d_boxes, (const void *)L->d_ib_x,
+                               (const void *)L->d_ib_t[0], (const void *)L->d_ib_t[1],
+                               (const void *)L->d_ib_t[2], (const void *)L->d_cbeg,
+                               (const void *)L->d_segs[0], (const void *)L->d_segs[1],
+                               (const void *)L->d_segs[2], (const void *)L->d_segs[3],
+                               (const void *)L->d_segs[4], (const void *)L->d_maxbits,
+                               (const void *)L->d_dt })
+            key.push_back((uint64_t)(uintptr_t)p);
+        for (int v : { L->tune.block_x, L->tune.block_t, L->tune.seg_k, L->tune.grid_cap,
+                       L->occ_x, L->occ_t })
+            key.push_back((uint64_t)(int64_t)v);
+    }
+    if (a->gx && key == a->gkey)
+        return BX_OK;
+    if (a->gx)
+    {
+        BX_HIP_CHECK(hipGraphExecDestroy(a->gx));
+        a->gx = nullptr;
+    }
+    if (!a->d_sched)
+        BX_HIP_CHECK(hipMalloc((void **)&a->d_sched, sizeof(AmrSched)));
+    if (!a->cap)
+        BX_HIP_CHECK(hipStreamCreateWithFlags(&a->cap, hipStreamNonBlocking));
+    SchedArgs sa{};
+    for (int l = 0; l < nlev; ++l)
+    {
+        sa.maxbits[l] = a->lev[l]->d_maxbits;
+        sa.dt[l]      = a->lev[l]->d_dt;
+    }
+    for (int d = 0; d < 3; ++d)
+        sa.dx0[d] = dx0[d];
+    sa.cfl  = cfl;
+    sa.nlev = nlev;
+    sa.D    = a->D;
+    hipStream_t s = a->cap;
+    BX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = BX_OK;
+    for (int l = 0; l < nlev && rc == BX_OK; ++l)
+        rc = do_maxspeed(a->lev[l], b.U[l], s);
+    if (rc == BX_OK)
+    {
+        hipLaunchKernelGGL(k_amr_schedule, dim3(1), dim3(64), 0, s, a->d_sched, sa);
+        CapturedStep g{ a->d_sched, 0 };
+        // the host's time values only shape the recursion here
+        rc = time_step(a, 0, 0.0, 1.0, 0.0, 1.0, dx0, b, s, false, &g);
+    }
+    hipGraph_t      graph = nullptr;
+    const hipError_t ec   = hipStreamEndCapture(s, &graph);
+    if (rc != BX_OK || ec != hipSuccess)
+    {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        return rc != BX_OK ? rc
+                           : set_error(BX_ERR_HIP, std::string("amr step capture: ")
+                                                       + hipGetErrorString(ec));
+    }
+    const hipError_t ei = hipGraphInstantiate(&a->gx, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess)
+    {
+        a->gx = nullptr;
+        return set_error(BX_ERR_HIP, std::string("amr step instantiate: ")
+                                         + hipGetErrorString(ei));
+    }
+    a->gkey = key;
+    return BX_OK;
+}
+
 extern "C" int bx_amr_run_levels(bx_amr *a, const double dx0[3], double cfl, double stop_time,
                                  int max_step, double *const *U, double *const *U_tmp,
                                  double *const *U_old, double *time, int *steps, void *stream)
@@ -1212,6 +1501,20 @@ extern "C" int bx_amr_run_levels(bx_amr *a, const double dx0[3], double cfl, dou
             return rc;
     double t = *time;
     int    n = 0;
+    if (a->use_graph && stop_time < 0.0 && max_step > 0 && graph_fits(a))
+    {
+        // a fixed number of steps: the step is captured once and replayed,
+        // its dt, time and interpolation weights computed on the device
+        if ((rc = capture_step(a, dx0, cfl, b)) != BX_OK)
+            return rc;
+        if ((rc = set_scalar(&a->d_sched->t, t, s)) != BX_OK)
+            return rc;
+        for (; n < max_step; ++n)
+            BX_HIP_CHECK(hipGraphLaunch(a->gx, s));
+        BX_HIP_CHECK(hipMemcpyAsync(&t, &a->d_sched->t, sizeof(double), hipMemcpyDeviceToHost,
+                                    s));
+        BX_HIP_CHECK(hipStreamSynchronize(s));
+    }
     while ((n < max_step || max_step < 0) && (t < stop_time || stop_time < 0.0))
     {
         double dt;

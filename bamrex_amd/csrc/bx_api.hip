@@ -501,6 +501,8 @@ static void build_ghost_segments(bx_ctx *c)
         }
     }
     for (int l = 0; l < 4; ++l)
+        c->segs[4].insert(c->segs[4].end(), c->segs[l].begin(), c->segs[l].end());
+    for (int l = 0; l < 5; ++l)
     {
         long long acc = 0;
         for (auto &s : c->segs[l])
@@ -598,7 +600,7 @@ extern "C" int bx_set_level(bx_ctx *c, const bx_geometry *geom,
     if ((rc = upload_sweep_tables(c)) != BX_OK)
         return rc;
     build_ghost_segments(c);
-    for (int l = 0; l < 4; ++l)
+    for (int l = 0; l < 5; ++l)
         if ((rc = upload(&c->d_segs[l], c->segs[l])) != BX_OK)
             return rc;
     if ((rc = occupancy(c, 0, c->tune.block_x, &c->occ_x)) != BX_OK)
@@ -675,11 +677,7 @@ int do_fill(bx_ctx *c, double *U, int list, hipStream_t s, const StepTail &tail)
 
 int do_fill_all(bx_ctx *c, double *U, hipStream_t s)
 {
-    int rc;
-    for (int l = 0; l < c->D; ++l)
-        if ((rc = do_fill(c, U, l, s)) != BX_OK)
-            return rc;
-    return do_fill(c, U, 3, s);
+    return do_fill(c, U, 4, s);
 }
 
 int do_maxspeed(bx_ctx *c, const double *U, hipStream_t s)

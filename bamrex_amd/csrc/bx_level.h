@@ -58,9 +58,11 @@ struct bx_ctx
               *d_cbeg = nullptr;
     long long  ncells = 0;
 
-    std::vector<GhostSeg> segs[4]; // [0..D-1]: ghosts in dir d only; [3]: rest
-    GhostSeg             *d_segs[4] = { nullptr, nullptr, nullptr, nullptr };
-    long long             gcells[4] = { 0, 0, 0, 0 };
+    // [0..D-1]: ghosts in dir d only; [3]: rest; [4]: all of them (each
+    // segment copies valid cells only, so one launch fills every ghost)
+    std::vector<GhostSeg> segs[5];
+    GhostSeg             *d_segs[5] = { nullptr, nullptr, nullptr, nullptr, nullptr };
+    long long             gcells[5] = { 0, 0, 0, 0, 0 };
 
     unsigned long long *d_maxbits = nullptr;
     double             *d_dt = nullptr, *d_time = nullptr;

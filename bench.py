@@ -435,7 +435,9 @@ def amr_leg(args):
     each direction (an n^2 fine region, boxes of 128), MHM-HLLC, minbee,
     CFL 0.8.  One coarse step = coarse advance + two fine advances of dt/2
     with coarse-fine interpolation, flux registers, reflux, avgDown and the
-    two-level dt (host-driven launches, one host read of the dt per step)."""
+    two-level dt.  The run replays one captured coarse step (hipGraph): dt,
+    the time and the coarse-fine time weights are computed on the device, so
+    a step needs no host synchronisation."""
     from bamrex_amd import _lib, amr, problems
     from bamrex_amd.level import make_params
     lib = _lib.lib()
@@ -470,7 +472,8 @@ def amr_leg(args):
            "coarse_cells": nc, "fine_cells": nf, "coarse_steps": nsteps,
            "ms_per_coarse_step": round(el * 1e3 / nsteps, 4),
            "cell_updates_per_s": (nc + 2 * nf) * nsteps / el,
-           "cf_segments": info["cf_segments"], "register_faces": info["register_faces"]}
+           "cf_segments": info["cf_segments"], "register_faces": info["register_faces"],
+           "graph_replayed": True}
     log("amr: %d coarse steps, %.3f ms/step, %.3e cell-updates/s" % (
         nsteps, out["ms_per_coarse_step"], out["cell_updates_per_s"]))
     return out

@@ -366,6 +366,17 @@ int bx_amr_run_levels(bx_amr *a, const double dx0[3], double cfl,
                       double stop_time, int max_step, double *const *U,
                       double *const *U_tmp, double *const *U_old, double *time,
                       int *steps, void *stream);
+/* How bx_amr_run / bx_amr_run_levels launch a run of a fixed number of steps
+ * (stop_time < 0, max_step > 0, at most 7 levels).  use_graph = 1 (the
+ * default): one coarse step (the levels' maximum wave speeds, computeNewDt,
+ * Amr::timeStep) is captured once as a hipGraph and replayed max_step times;
+ * dt, the time, the flux-register coefficients and the coarse-fine time
+ * weights are computed on the device with the host's expressions, so the
+ * states and the final time are the same bits as with use_graph = 0, which
+ * reads dt back and launches every kernel from the host each step.  Runs
+ * with a stop time are always host-driven.  No counterpart in the reference
+ * (its steps are host-driven); this only chooses the launch mode. */
+int bx_amr_set_graph(bx_amr *a, int use_graph);
 
 /* ============================================================================
  * Pressure Helmholtz solve of the IMEX schemes (PEEK / PESK) on one 2D

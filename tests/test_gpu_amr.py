@@ -195,6 +195,56 @@ def test_three_level_run_parity(oracle_bxpow, case):
         assert_equal_valid(lay, got[l], Us[l], "%s level %d" % (case, l))
 
 
+@pytest.mark.parametrize("name", ["quadrant_inner", "vortex_periodic_edge", "vortex3d"])
+def test_amr_graph_replay_equals_host_steps(name):
+    """Runs of a fixed number of steps replay one captured coarse step whose
+    dt, time, register coefficients and coarse-fine weights come from the
+    device schedule: the same bits and times as host-driven steps, also for
+    a second run that continues from the first's time (the graph reused)
+    and after a launch-shape change (the graph recaptured)."""
+    make, n, mgs, rlo, rhi, fmgs, bc, steps, method = CASES[name]
+    s = amr.refine_setup(make, n, mgs, rlo, rhi, fine_max_grid_size=fmgs)
+    cfl = s.get("cfl", 0.8)
+    out = {}
+    for graph in (False, True):
+        announce("amr %s graph=%s" % (name, graph))
+        h = amr.AmrHierarchy(gparams(s, method), s["layout"], s["fine_layout"], s["domain_lo"],
+                             s["domain_hi"], s["bc"], s["bc"])
+        try:
+            h.set_graph(graph)
+            h.set_state(s["U"], s["Uf"])
+            t1, n1 = h.run(s["dx"], cfl, max_step=3)
+            t2, n2 = h.run(s["dx"], cfl, max_step=steps - 3, start_time=t1)
+            h.coarse.set_tuning(block_x=128, block_t=64, seg_k=8)
+            t3, n3 = h.run(s["dx"], cfl, max_step=2, start_time=t2)
+            out[graph] = ((t1, n1, t2, n2, t3, n3), h.get_state())
+        finally:
+            h.close()
+    assert out[True][0] == out[False][0]
+    assert_equal_valid(s["layout"], out[True][1][0], out[False][1][0], name + " coarse")
+    assert_equal_valid(s["fine_layout"], out[True][1][1], out[False][1][1], name + " fine")
+
+
+def test_three_level_graph_replay_equals_host_steps():
+    from test_amr_oracle import THREE_LEVELS
+    c = THREE_LEVELS["2d"]
+    s = amr.refine_levels(c["make"], c["ncell"], c["mgs"], c["regions"], c["mgss"])
+    out = {}
+    for graph in (False, True):
+        announce("amr 3 levels graph=%s" % graph)
+        h = amr.AmrLevels(gparams(s), s["layouts"], s["domain_lo"], s["domain_hi"], s["bc"],
+                          s["bc"])
+        try:
+            h.set_graph(graph)
+            h.set_state(s["Us"])
+            out[graph] = (h.run(s["dx"], 0.9, max_step=5), h.get_state())
+        finally:
+            h.close()
+    assert out[True][0] == out[False][0]
+    for l, lay in enumerate(s["layouts"]):
+        assert_equal_valid(lay, out[True][1][l], out[False][1][l], "level %d" % l)
+
+
 def test_improperly_nested_level_is_refused():
     """A level-2 box reaching within 2 level-1 cells of the level-1 region's
     edge (inside the domain) is refused at set-up."""

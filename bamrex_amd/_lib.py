@@ -212,6 +212,8 @@ _SIGS = {
     "bx_cnn_destroy": (None, [P]),
     "bx_cnn_set_params": (ctypes.c_int, [P, P, ctypes.c_int]),
     "bx_cnn_get_params": (ctypes.c_int, [P, P, ctypes.c_int]),
+    "bx_cnn_save": (ctypes.c_int, [P, ctypes.c_char_p]),
+    "bx_cnn_load": (ctypes.c_int, [P, ctypes.c_char_p]),
     "bx_cnn_set_operator": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, P, P, P, P]),
     "bx_cnn_infer": (ctypes.c_int, [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_double, P]),

@@ -39,6 +39,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <random>
 #include <string>
@@ -1309,6 +1311,61 @@ extern "C" int bx_cnn_get_params(bx_cnn *c, float *p, int n)
         return cerr(BX_ERR_INVALID, "bx_cnn_get_params: expected BX_CNN_NPARAMS values");
     C_CHECK(hipMemcpy(p, c->w, NPAR * 4, hipMemcpyDeviceToHost));
     return BX_OK;
+}
+
+// Weight file: 8-byte magic "BXCNN001", uint32 parameter count, uint32
+// FNV-1a checksum of the parameter bytes, then the FP32 master parameters
+// (little-endian, the order of bx_cnn_get_params).
+static uint32_t fnv1a32(const unsigned char *p, size_t n)
+{
+    uint32_t h = 2166136261u;
+    for (size_t i = 0; i < n; ++i)
+        h = (h ^ p[i]) * 16777619u;
+    return h;
+}
+
+extern "C" int bx_cnn_save(bx_cnn *c, const char *path)
+{
+    if (!c || !path)
+        return cerr(BX_ERR_INVALID, "bx_cnn_save: null argument");
+    std::vector<float> w(NPAR);
+    int                rc;
+    if ((rc = bx_cnn_get_params(c, w.data(), NPAR)) != BX_OK)
+        return rc;
+    const uint32_t n = NPAR;
+    const uint32_t h = fnv1a32(reinterpret_cast<const unsigned char *>(w.data()), NPAR * 4);
+    FILE          *f = std::fopen(path, "wb");
+    if (!f)
+        return cerr(BX_ERR_INVALID, std::string("bx_cnn_save: cannot open ") + path);
+    const bool ok = std::fwrite("BXCNN001", 1, 8, f) == 8 && std::fwrite(&n, 4, 1, f) == 1
+                    && std::fwrite(&h, 4, 1, f) == 1
+                    && std::fwrite(w.data(), 4, NPAR, f) == (size_t)NPAR;
+    if (std::fclose(f) != 0 || !ok)
+        return cerr(BX_ERR_INVALID, std::string("bx_cnn_save: write failed: ") + path);
+    return BX_OK;
+}
+
+extern "C" int bx_cnn_load(bx_cnn *c, const char *path)
+{
+    if (!c || !path)
+        return cerr(BX_ERR_INVALID, "bx_cnn_load: null argument");
+    FILE *f = std::fopen(path, "rb");
+    if (!f)
+        return cerr(BX_ERR_INVALID, std::string("bx_cnn_load: cannot open ") + path);
+    char               magic[8];
+    uint32_t           n = 0, h = 0;
+    std::vector<float> w(NPAR);
+    const bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "BXCNN001", 8) == 0
+                    && std::fread(&n, 4, 1, f) == 1 && n == (uint32_t)NPAR
+                    && std::fread(&h, 4, 1, f) == 1
+                    && std::fread(w.data(), 4, NPAR, f) == (size_t)NPAR && std::fgetc(f) == EOF;
+    std::fclose(f);
+    if (!ok)
+        return cerr(BX_ERR_INVALID, std::string("bx_cnn_load: not a network weight file of ")
+                                       + std::to_string(NPAR) + " parameters: " + path);
+    if (fnv1a32(reinterpret_cast<const unsigned char *>(w.data()), NPAR * 4) != h)
+        return cerr(BX_ERR_INVALID, std::string("bx_cnn_load: checksum mismatch: ") + path);
+    return bx_cnn_set_params(c, w.data(), NPAR);
 }
 
 static int ensure_infer(bx_cnn *c, long long npx)

@@ -579,6 +579,12 @@ class LearnedSmoother:
         _chk(lib().bx_cnn_set_params(self.c, w.ctypes.data, self.NPARAMS), "bx_cnn_set_params",
              "bx_cnn_last_error")
 
+    def save(self, path):
+        _chk(lib().bx_cnn_save(self.c, str(path).encode()), "bx_cnn_save", "bx_cnn_last_error")
+
+    def load(self, path):
+        _chk(lib().bx_cnn_load(self.c, str(path).encode()), "bx_cnn_load", "bx_cnn_last_error")
+
     def net_forward(self, x, periodic=False, layered=False):
         """y = N(x) for x of shape (nimg, h, w) float32 (rounded to bf16):
         the fused inference kernel, or the layer-by-layer training kernels."""

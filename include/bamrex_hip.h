@@ -644,7 +644,7 @@ int bx_imex_stab_blend(const double *ind, const double *U_o1, const double *p_o1
                        int nx, int ny, double *U, double *p, void *stream);
 
 /* Learned smoother: bias-free 3x3 convolutions 1->16->16->16->1 with ReLU,
- * BF16 weights/activations, FP32 accumulation (MFMA for the 16->16 layers).
+ * BF16 weights/activations, FP32 accumulation (MFMA in every layer).
  * Correction e = omega (sigma / D) N(r / sigma), sigma = rms(r), D = diag(A).
  * Parameters: W1[9][16], W2[9][16][16], W3[9][16][16], W4[9][16] in that
  * order ([tap][out][in]; tap = (dy+1)*3 + dx+1). */
@@ -654,6 +654,12 @@ void bx_cnn_destroy(bx_cnn *c);
 /* FP32 host arrays of BX_CNN_NPARAMS values.  set resets the Adam state. */
 int bx_cnn_set_params(bx_cnn *c, const float *params, int n);
 int bx_cnn_get_params(bx_cnn *c, float *params, int n);
+/* The trained network to and from a file (bx_cnn_load/save of the proposed
+ * boundary): magic "BXCNN001", the parameter count, an FNV-1a checksum, then
+ * the FP32 parameters in the order above.  load checks all three
+ * (BX_ERR_INVALID otherwise) and resets the Adam state. */
+int bx_cnn_save(bx_cnn *c, const char *path);
+int bx_cnn_load(bx_cnn *c, const char *path);
 /* Registers the operator the network is trained for and applied with (its
  * diagonal and, for training, its coefficients; device pointers must stay
  * valid).  nx must be a multiple of 16. */

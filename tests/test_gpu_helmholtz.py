@@ -247,6 +247,47 @@ def test_cnn_infer_matches_oracle():
     net.close()
 
 
+def test_cnn_save_load_round_trip(tmp_path):
+    """bx_cnn_save / bx_cnn_load: a loaded network has the saved parameters
+    bit for bit and gives the same correction; a corrupted, truncated or
+    foreign file is refused and leaves the network unchanged."""
+    announce("cnn save/load 64")
+    n = 64
+    rng = np.random.default_rng(31)
+    a, bX, bY = rand_coeffs(rng, n, n)
+    s = HelmholtzSolver(n, n, 1.0 / n)
+    s.set_coeffs(1.0, 2e-3, a, bX, bY)
+    net = LearnedSmoother(seed=3)
+    net.set_params((rng.normal(size=co.NPARAMS) * 0.3).astype(np.float32))
+    path = tmp_path / "net.bxcnn"
+    net.save(path)
+    other = LearnedSmoother(seed=8)
+    s.attach(other)
+    r = dev(rng.normal(size=(n, n)))
+    e1, e2 = DeviceArray((n, n)), DeviceArray((n, n))
+    other.load(path)
+    assert np.array_equal(other.get_params(), net.get_params())
+    other.infer(r, e1)
+    s.attach(net)
+    net.infer(r, e2)
+    assert np.array_equal(e1.download(), e2.download())
+    raw = bytearray(path.read_bytes())
+    assert len(raw) == 16 + 4 * co.NPARAMS and raw[:8] == b"BXCNN001"
+    keep = other.get_params()
+    bad = tmp_path / "bad"
+    for blob in (raw[:100], raw[:16] + bytes([raw[16] ^ 1]) + raw[17:], b"BXCNN002" + raw[8:],
+                 raw + b"x"):
+        bad.write_bytes(bytes(blob))
+        with pytest.raises(Exception):
+            other.load(bad)
+        assert np.array_equal(other.get_params(), keep)
+    with pytest.raises(Exception):
+        other.load(tmp_path / "missing")
+    s.close()
+    net.close()
+    other.close()
+
+
 def test_cnn_loss_and_gradient_match_oracle():
     announce("cnn loss/grad 96x96 batch 6 crop 32")
     n = 96

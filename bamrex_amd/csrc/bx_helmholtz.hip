@@ -296,12 +296,15 @@ __global__ __launch_bounds__(256) void k_resid_restrict(
 }
 
 // The coarse end of a V-cycle (the coarsest levels, as many as fit in
-// TAIL_LDS_BYTES of LDS) in one workgroup: the coefficients and the work
-// arrays of those levels are copied into LDS, the same phases as the
-// per-level launches run there separated by workgroup barriers, and the
-// correction of the first tail level is written back.
+// the device's LDS per workgroup, at most TAIL_LDS_BYTES) in one workgroup of
+// TAIL_THREADS threads: the coefficients and the work arrays of those
+// levels are copied into LDS, the same phases as the per-level launches run
+// there separated by workgroup barriers, and the correction of the first
+// tail level is written back.  (gfx950: 160 KB of LDS, so an FP32 V-cycle's
+// tail starts at 64 x 64, an FP64 one at 32 x 32.)
 constexpr int TAIL_MAX_LEVELS = 8;
-constexpr int TAIL_LDS_BYTES  = 60 * 1024;
+constexpr int TAIL_LDS_BYTES  = 160 * 1024;
+constexpr int TAIL_THREADS    = 1024;
 
 template <class T> struct TailLevel
 {
@@ -392,7 +395,7 @@ __device__ void copy_in(T *dst, const T *__restrict__ src, int n)
 
 // A: level table in device memory (the levels from the first tail level)
 template <class T>
-__global__ __launch_bounds__(256) void k_vcycle_tail(const TailArgs<T> *__restrict__ A,
+__global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(const TailArgs<T> *__restrict__ A,
                                                       int nlev, int nu1, int nu2,
                                                       int nu_bottom,
                                                       const T *__restrict__ sc)
@@ -773,6 +776,100 @@ __global__ __launch_bounds__(256) void k_add_combo(double *__restrict__ x,
     }
 }
 
+// ---- one FGMRES iteration's vector work in fewer passes ----------------
+// Block partial sums of k dot products (k <= 16), one row of gridDim.x
+// partials per vector (layout as k_multidot's), from per-thread sums.
+__device__ inline void block_partials16(const double (&acc)[16], int k,
+                                        double *__restrict__ partial)
+{
+    __shared__ double sh[4][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+        if (q >= k)
+            break;
+        double v = acc[q];
+        for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_xor(v, off);
+        if (lane == 0)
+            sh[wv][q] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < k)
+        partial[(long long)threadIdx.x * gridDim.x + blockIdx.x]
+            = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+}
+
+// w = A z (FP64 operator, the k_apply expression) and the partial dot
+// products of w with V[0..k) (the first Gram-Schmidt projection); 256
+// threads per block, grid-stride over cells
+__global__ __launch_bounds__(256) void k_apply_multidot(
+    const double *__restrict__ z, double *__restrict__ w, const double *__restrict__ a,
+    const double *__restrict__ bX, const double *__restrict__ bY,
+    const double *__restrict__ cX, const double *__restrict__ cY, int nx, int ny,
+    const double *__restrict__ sc, double dxinv, double dyinv,
+    const double *__restrict__ V, int k, double *__restrict__ partial)
+{
+    const long long n = (long long)nx * ny;
+    double          acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        acc[q] = 0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x)
+    {
+        const int    j = (int)(c / nx), i = (int)(c - (long long)j * nx);
+        const double v = apply_at(z, a, bX, bY, cX, cY, nx, ny, sc[0], sc[1], sc[2], dxinv,
+                                  dyinv, i, j);
+        w[c] = v;
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if (q < k)
+                acc[q] += V[(long long)q * n + c] * v;
+    }
+    block_partials16(acc, k, partial);
+}
+
+// w -= sum_q h[q] V[q] (q < k, the previous projection's coefficients), then
+// the partial dot products of the new w with V[0..k) (second = true) or
+// with itself (second = false: its squared norm, one row)
+__global__ __launch_bounds__(256) void k_subtract_dots(double *__restrict__ w,
+                                                       const double *__restrict__ V,
+                                                       const double *__restrict__ h, int k,
+                                                       long long n, int second,
+                                                       double *__restrict__ partial)
+{
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        acc[q] = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+    {
+        double vq[16];
+        double s = w[i];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if (q < k)
+            {
+                vq[q] = V[(long long)q * n + i];
+                s -= h[q] * vq[q];
+            }
+        w[i] = s;
+        if (second)
+        {
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                if (q < k)
+                    acc[q] += vq[q] * s;
+        }
+        else
+            acc[0] += s * s;
+    }
+    block_partials16(acc, second ? k : 1, partial);
+}
+
 // FGMRES least-squares state on the device: the Hessenberg matrix H
 // ((m+1) x m, row stride m), the Givens rotations (cs, sn), the rotated
 // right-hand side g, the solution y of the triangular system, and a status
@@ -796,12 +893,10 @@ __global__ void k_arnoldi_init(Arnoldi a, double beta)
 // vector's squared norm d3, its previous rotations, a new rotation that
 // zeroes H[j+1][j], and g rotated with it: one thread, in the order of a
 // textbook Givens update (Saad, Iterative Methods, Alg. 6.9).
-__global__ void k_arnoldi_step(Arnoldi a, const double *__restrict__ d1,
-                               const double *__restrict__ d2,
-                               const double *__restrict__ d3, int j)
+__device__ void arnoldi_step(Arnoldi a, const double *__restrict__ d1,
+                             const double *__restrict__ d2, const double *__restrict__ d3,
+                             int j)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0)
-        return;
     const int m  = a.m;
     double   *H  = a.H;
     for (int i = 0; i <= j; ++i)
@@ -824,6 +919,39 @@ __global__ void k_arnoldi_step(Arnoldi a, const double *__restrict__ d1,
     a.g[j]             = a.cs[j] * a.g[j];
     a.status[0]        = fabs(a.g[j + 1]);
     a.status[1]        = hn;
+}
+
+__global__ void k_arnoldi_step(Arnoldi a, const double *__restrict__ d1,
+                               const double *__restrict__ d2,
+                               const double *__restrict__ d3, int j)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        arnoldi_step(a, d1, d2, d3, j);
+}
+
+// the norm's partials summed by one block, then (thread 0) the Arnoldi step
+__global__ __launch_bounds__(256) void k_norm_arnoldi(const double *__restrict__ partial, int nb,
+                                                      double *__restrict__ d3, Arnoldi ar,
+                                                      const double *__restrict__ d1,
+                                                      const double *__restrict__ d2, int j)
+{
+    __shared__ double sh[256];
+    double            s = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        s += partial[b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1)
+    {
+        if ((int)threadIdx.x < off)
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        d3[0] = sh[0];
+        arnoldi_step(ar, d1, d2, d3, j);
+    }
 }
 
 // y = R^-1 g over the first j columns (back substitution); a non-finite y
@@ -1037,6 +1165,10 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
         const int nl   = (int)mg.lev.size();
         int       l0   = nl;
         long long used = 0;
+        int       dev = 0, lds_max = 0;
+        H_CHECK(hipGetDevice(&dev));
+        H_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const long long tail_bytes = std::min<long long>(lds_max, TAIL_LDS_BYTES);
         auto      need = [&](const MGLevel<T> &L) {
             const long long n = (long long)L.nx * L.ny;
             return (n + (L.nx + 1) * (long long)L.ny + L.nx * (long long)(L.ny + 1)
@@ -1044,7 +1176,7 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
                    * (long long)sizeof(T);
         };
         while (l0 > 0 && nl - (l0 - 1) <= TAIL_MAX_LEVELS
-               && used + need(mg.lev[l0 - 1]) <= TAIL_LDS_BYTES)
+               && used + need(mg.lev[l0 - 1]) <= tail_bytes)
             used += need(mg.lev[--l0]);
         if (l0 < nl)
         {
@@ -1116,7 +1248,7 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
     MGLevel<T> &L = mg.lev[l];
     if (mg.tail_l0 >= 0 && l == mg.tail_l0)
     {
-        hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(256), mg.tail_lds, s,
+        hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(TAIL_THREADS), mg.tail_lds, s,
                            (const TailArgs<T> *)mg.d_tail, (int)mg.lev.size() - l, mg.nu1,
                            mg.nu2, mg.nu_bottom, mg.d_sc);
         return;
@@ -1700,30 +1832,55 @@ static int solve_fgmres(bx_helm *h, const double *b, double *x, double rtol,
             if ((rc = precond_replay(h, vj, zj, s)))
                 return rc;
             ++st->precond_applies;
-            launch_apply(h->mg64.lev[0], h->mg64, zj, h->w,
-                         (const double *)nullptr, s);
             // CGS2 chained on the device (the projections are read from
             // device memory)
             double *d1 = h->dots, *d2 = h->dots + (m + 1), *d3 = h->dots + 2 * (m + 1);
-            hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->V, h->w, n,
-                               j + 1, h->partial);
-            hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(256), 0, s, h->partial,
-                               h->nb_dot, j + 1, d1);
-            hipLaunchKernelGGL(k_subtract_combo, dim3(2048), dim3(256), 0, s, h->w,
-                               h->V, d1, j + 1, n);
-            hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->V, h->w, n,
-                               j + 1, h->partial);
-            hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(256), 0, s, h->partial,
-                               h->nb_dot, j + 1, d2);
-            hipLaunchKernelGGL(k_subtract_combo, dim3(2048), dim3(256), 0, s, h->w,
-                               h->V, d2, j + 1, n);
-            hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->w, h->w, n,
-                               1, h->partial);
-            hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, h->partial,
-                               h->nb_dot, 1, d3);
-            hipLaunchKernelGGL(k_scale_inv_norm, dim3(2048), dim3(256), 0, s,
-                               h->V + (long long)(j + 1) * n, h->w, n, d3);
-            hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, s, A, d1, d2, d3, j);
+            if (j + 1 <= 16)
+            {
+                // w = A z with the first projection's partial sums, then
+                // each subtraction with the next pass's partial sums, the
+                // norm's reduction with the Arnoldi step: 7 launches, not 11
+                const MGLevel<double> &L0 = h->mg64.lev[0];
+                hipLaunchKernelGGL(k_apply_multidot, dim3(h->nb_dot), dim3(256), 0, s, zj, h->w,
+                                   L0.a, L0.bX, L0.bY, L0.cX, L0.cY, L0.nx, L0.ny, h->mg64.d_sc,
+                                   L0.dxinv, L0.dyinv, h->V, j + 1, h->partial);
+                hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(256), 0, s, h->partial,
+                                   h->nb_dot, j + 1, d1);
+                hipLaunchKernelGGL(k_subtract_dots, dim3(h->nb_dot), dim3(256), 0, s, h->w, h->V,
+                                   d1, j + 1, n, 1, h->partial);
+                hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(256), 0, s, h->partial,
+                                   h->nb_dot, j + 1, d2);
+                hipLaunchKernelGGL(k_subtract_dots, dim3(h->nb_dot), dim3(256), 0, s, h->w, h->V,
+                                   d2, j + 1, n, 0, h->partial);
+                hipLaunchKernelGGL(k_norm_arnoldi, dim3(1), dim3(256), 0, s, h->partial,
+                                   h->nb_dot, d3, A, d1, d2, j);
+                hipLaunchKernelGGL(k_scale_inv_norm, dim3(2048), dim3(256), 0, s,
+                                   h->V + (long long)(j + 1) * n, h->w, n, d3);
+            }
+            else
+            {
+                launch_apply(h->mg64.lev[0], h->mg64, zj, h->w,
+                             (const double *)nullptr, s);
+                hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->V, h->w, n,
+                                   j + 1, h->partial);
+                hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(256), 0, s, h->partial,
+                                   h->nb_dot, j + 1, d1);
+                hipLaunchKernelGGL(k_subtract_combo, dim3(2048), dim3(256), 0, s, h->w,
+                                   h->V, d1, j + 1, n);
+                hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->V, h->w, n,
+                                   j + 1, h->partial);
+                hipLaunchKernelGGL(k_reduce_partials, dim3(j + 1), dim3(256), 0, s, h->partial,
+                                   h->nb_dot, j + 1, d2);
+                hipLaunchKernelGGL(k_subtract_combo, dim3(2048), dim3(256), 0, s, h->w,
+                                   h->V, d2, j + 1, n);
+                hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->w, h->w, n,
+                                   1, h->partial);
+                hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, h->partial,
+                                   h->nb_dot, 1, d3);
+                hipLaunchKernelGGL(k_scale_inv_norm, dim3(2048), dim3(256), 0, s,
+                                   h->V + (long long)(j + 1) * n, h->w, n, d3);
+                hipLaunchKernelGGL(k_arnoldi_step, dim3(1), dim3(64), 0, s, A, d1, d2, d3, j);
+            }
             H_CHECK(hipGetLastError());
             H_CHECK(hipMemcpyAsync(h->hstat, A.status, 2 * sizeof(double),
                                    hipMemcpyDeviceToHost, s));

@@ -1075,6 +1075,9 @@ struct CapturedStep
 {
     AmrSched *S;
     int       slot;
+    // the finest level's steps still to come in this coarse step: its last
+    // one computes the level's maximum wave speed for the next step's dt
+    int fine_left;
 };
 
 static int copy_level(double *dst, const double *src, long long n, hipStream_t s)
@@ -1102,7 +1105,14 @@ static int level_step(bx_amr *h, int l, double dt, const double dx[3], double *A
         const double *in   = (d % 2 == 0) ? A : B;
         double       *out  = (d % 2 == 0) ? B : A;
         const double *coef = g ? g->S->coef[l][d] : nullptr;
-        if ((rc = do_sweep(L, d, dx, in, out, nullptr, s)) != BX_OK)
+        // the finest level's last sweep of the coarse step reduces the wave
+        // speed of its output, the level's final state (avgDown does not
+        // change the finest level)
+        unsigned long long *mb
+            = (g && l + 1 == (int)h->lev.size() && g->fine_left == 1 && d == D - 1)
+                  ? L->d_maxbits
+                  : nullptr;
+        if ((rc = do_sweep(L, d, dx, in, out, mb, s)) != BX_OK)
             return rc;
         if (l > 0 && (rc = reg_fluxes(h->pairs[l - 1], true, d, in, dt, dx, s, coef)) != BX_OK)
             return rc;
@@ -1112,6 +1122,8 @@ static int level_step(bx_amr *h, int l, double dt, const double dx[3], double *A
         if ((rc = do_fill(L, out, (d + 1) % D, s)) != BX_OK)
             return rc;
     }
+    if (g && l + 1 == (int)h->lev.size())
+        --g->fine_left;
     if (D % 2 == 1)
         return copy_level(A, B, L->level_extent, s);
     return BX_OK;
@@ -1451,12 +1463,19 @@ static int capture_step(bx_amr *a, const double dx0[3], double cfl, const LevelB
     hipStream_t s = a->cap;
     BX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = BX_OK;
-    for (int l = 0; l < nlev && rc == BX_OK; ++l)
+    // the finest level's maximum comes from its last sweep of the previous
+    // step (or from the run's start)
+    for (int l = 0; l + 1 < nlev && rc == BX_OK; ++l)
         rc = do_maxspeed(a->lev[l], b.U[l], s);
     if (rc == BX_OK)
     {
         hipLaunchKernelGGL(k_amr_schedule, dim3(1), dim3(64), 0, s, a->d_sched, sa);
-        CapturedStep g{ a->d_sched, 0 };
+        rc = launch_set_u64(a->lev[nlev - 1]->d_maxbits, 0ULL, s) == hipSuccess ? BX_OK
+                                                                                 : BX_ERR_HIP;
+    }
+    if (rc == BX_OK)
+    {
+        CapturedStep g{ a->d_sched, 0, 1 << (nlev - 1) };
         // the host's time values only shape the recursion here
         rc = time_step(a, 0, 0.0, 1.0, 0.0, 1.0, dx0, b, s, false, &g);
     }
@@ -1507,7 +1526,8 @@ extern "C" int bx_amr_run_levels(bx_amr *a, const double dx0[3], double cfl, dou
         // its dt, time and interpolation weights computed on the device
         if ((rc = capture_step(a, dx0, cfl, b)) != BX_OK)
             return rc;
-        if ((rc = set_scalar(&a->d_sched->t, t, s)) != BX_OK)
+        if ((rc = set_scalar(&a->d_sched->t, t, s)) != BX_OK
+            || (rc = do_maxspeed(a->lev[nlev - 1], U[nlev - 1], s)) != BX_OK)
             return rc;
         for (; n < max_step; ++n)
             BX_HIP_CHECK(hipGraphLaunch(a->gx, s));

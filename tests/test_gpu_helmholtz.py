@@ -185,11 +185,14 @@ def test_cnn_forward_exact_case_bit_exact(periodic):
 
 @pytest.mark.parametrize("shape,periodic", [((2, 37, 48), False), ((2, 37, 48), True),
                                             ((1, 80, 96), False), ((1, 5, 16), True),
-                                            ((3, 30, 32), False)])
+                                            ((3, 30, 32), False), ((8, 256, 256), True),
+                                            ((6, 250, 256), False)])
 def test_cnn_fused_forward_equals_layered(shape, periodic):
     """The fused inference kernel (all layers of a tile in LDS) gives the
     same bits as the layer-by-layer kernels, for zero padding and wrap-around,
-    partial tiles, several images, and images smaller than a tile's halo."""
+    partial tiles, several images, images smaller than a tile's halo, and
+    more tiles than the persistent grid has blocks (800 and 540 tiles: a
+    block runs several tiles, reusing its LDS)."""
     announce("cnn fused vs layered %s periodic=%s" % (shape, periodic))
     rng = np.random.default_rng(sum(shape) + periodic)
     net = LearnedSmoother(seed=3)
@@ -240,6 +243,34 @@ def test_cnn_infer_matches_oracle():
     y, _ = co.forward(p, (r / sigma)[None], periodic=True, quantize=True)
     D = co.op_diag(a, bX, bY, 1.0, 2e-3, 1.0 / nx, 1.0 / ny)
     ref = 0.8 * sigma * y[0].astype(np.float64) / D
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(e - ref)) <= 3e-2 * scale
+    assert np.mean(np.abs(e - ref)) <= 2e-3 * scale
+    s.close()
+    net.close()
+
+
+def test_cnn_infer_many_tiles_matches_oracle():
+    """bx_cnn_infer on a 1024 x 640 periodic level: 37 x 23 = 851 tiles, more
+    than the persistent grid's blocks, against the oracle's network."""
+    announce("cnn infer 1024x640 periodic")
+    nx, ny = 1024, 640
+    rng = np.random.default_rng(23)
+    a, bX, bY = rand_coeffs(rng, nx, ny)
+    s = HelmholtzSolver(nx, ny, 1.0 / nx, 1.0 / ny)
+    s.set_coeffs(1.0, 2e-3, a, bX, bY)
+    net = LearnedSmoother(seed=3)
+    p = (rng.normal(size=co.NPARAMS) * 0.3).astype(np.float32)
+    net.set_params(p)
+    s.attach(net)
+    r = rng.normal(size=(ny, nx))
+    dr, de = dev(r), DeviceArray((ny, nx))
+    net.infer(dr, de, omega=1.0)
+    e = de.download()
+    sigma = np.sqrt(np.mean(r * r))
+    y, _ = co.forward(p, (r / sigma)[None], periodic=True, quantize=True)
+    D = co.op_diag(a, bX, bY, 1.0, 2e-3, 1.0 / nx, 1.0 / ny)
+    ref = sigma * y[0].astype(np.float64) / D
     scale = np.max(np.abs(ref))
     assert np.max(np.abs(e - ref)) <= 3e-2 * scale
     assert np.mean(np.abs(e - ref)) <= 2e-3 * scale

@@ -922,7 +922,7 @@ struct bx_amr
     int                 use_graph = 1;
     AmrSched           *d_sched   = nullptr;
     hipStream_t         cap       = nullptr;
-    hipGraphExec_t      gx        = nullptr;
+    hipGraphExec_t      gx[2]     = { nullptr, nullptr }; // even / odd steps
     std::vector<uint64_t> gkey;
 };
 
@@ -932,8 +932,9 @@ extern "C" void bx_amr_destroy(bx_amr *a)
         return;
     for (AmrPair *p : a->pairs)
         pair_destroy(p);
-    if (a->gx)
-        (void)hipGraphExecDestroy(a->gx);
+    for (hipGraphExec_t g : a->gx)
+        if (g)
+            (void)hipGraphExecDestroy(g);
     if (a->cap)
         (void)hipStreamDestroy(a->cap);
     if (a->d_sched)
@@ -1078,6 +1079,11 @@ struct CapturedStep
     // the finest level's steps still to come in this coarse step: its last
     // one computes the level's maximum wave speed for the next step's dt
     int fine_left;
+    // each level's current state and scratch buffers: a coarser level's
+    // step leaves its result in the scratch buffer and the two exchange
+    // roles (level l > 0 does so 2^l times per coarse step, so only level
+    // 0's roles differ between consecutive coarse steps)
+    std::vector<double *> U, O;
 };
 
 static int copy_level(double *dst, const double *src, long long n, hipStream_t s)
@@ -1087,23 +1093,25 @@ static int copy_level(double *dst, const double *src, long long n, hipStream_t s
     return BX_OK;
 }
 
-// One split step of level l from A (valid + ghosts filled), result in A; B
-// is the ping-pong buffer.  Each sweep's input feeds the flux registers of
-// the pair below (as its fine side) and of the pair above (coarse side).
-// In a captured step the schedule kernel has written the level's dt and
-// register coefficients.
+// One split step of level l from A (valid + ghosts filled), result in A, or
+// in Y when Y is given (A then keeps the old state); B is the ping-pong
+// buffer.  Each sweep's input feeds the flux registers of the pair below (as
+// its fine side) and of the pair above (coarse side).  In a captured step
+// the schedule kernel has written the level's dt and register coefficients.
 static int level_step(bx_amr *h, int l, double dt, const double dx[3], double *A, double *B,
-                      hipStream_t s, CapturedStep *g)
+                      hipStream_t s, CapturedStep *g, double *Y = nullptr)
 {
-    bx_ctx   *L = h->lev[l];
-    const int D = h->D;
-    int       rc;
+    bx_ctx       *L = h->lev[l];
+    const int     D = h->D;
+    int           rc;
+    const double *in = A;
     if (!g && (rc = set_scalar(L->d_dt, dt, s)) != BX_OK)
         return rc;
     for (int d = 0; d < D; ++d)
     {
-        const double *in   = (d % 2 == 0) ? A : B;
-        double       *out  = (d % 2 == 0) ? B : A;
+        // in place: A -> B -> A ...; into Y: the chain alternates Y and B so
+        // that the last sweep writes Y
+        double *out = !Y ? ((d % 2 == 0) ? B : A) : (((D - 1 - d) % 2 == 0) ? Y : B);
         const double *coef = g ? g->S->coef[l][d] : nullptr;
         // the finest level's last sweep of the coarse step reduces the wave
         // speed of its output, the level's final state (avgDown does not
@@ -1121,10 +1129,11 @@ static int level_step(bx_amr *h, int l, double dt, const double dx[3], double *A
             return rc;
         if ((rc = do_fill(L, out, (d + 1) % D, s)) != BX_OK)
             return rc;
+        in = out;
     }
     if (g && l + 1 == (int)h->lev.size())
         --g->fine_left;
-    if (D % 2 == 1)
+    if (!Y && D % 2 == 1)
         return copy_level(A, B, L->level_extent, s);
     return BX_OK;
 }
@@ -1275,28 +1284,46 @@ static int time_step(bx_amr *h, int l, double time, double dt, double t_par, dou
     for (int d = 0; d < h->D; ++d)
         dx[d] = dx0[d] / (double)(1 << l);
     int rc;
-    if (l > 0 && (rc = fill_level(h, l, b.U[l], b.T[l], b.O[l - 1], b.U[l - 1], t_par,
+    // the buffers in their current roles (captured: views of g->U, g->O, so
+    // they follow the exchanges below)
+    const LevelBufs bb = g ? LevelBufs{ g->U.data(), b.T, g->O.data() } : b;
+    if (l > 0 && (rc = fill_level(h, l, bb.U[l], bb.T[l], bb.O[l - 1], bb.U[l - 1], t_par,
                                   t_par + dt_par, time, s, g))
                      != BX_OK)
         return rc;
-    if (l + 1 < nlev // the old state, ghosts filled, for the finer level's fills
-        && (rc = copy_level(b.O[l], b.U[l], h->lev[l]->level_extent, s)) != BX_OK)
-        return rc;
-    if ((rc = level_step(h, l, dt, dx, b.U[l], b.T[l], s, g)) != BX_OK)
-        return rc;
+    // The finer levels' fills need this level's old state (ghosts filled).
+    // Host-driven: copied to O[l] before the step.  Captured: the step writes
+    // its result into O[l] and U[l] keeps the old state, then the two
+    // exchange roles (the run alternates two graphs for level 0 and copies
+    // back once after an odd number of steps).
+    if (l + 1 < nlev && g)
+    {
+        if ((rc = level_step(h, l, dt, dx, bb.U[l], bb.T[l], s, g, bb.O[l])) != BX_OK)
+            return rc;
+        std::swap(g->U[l], g->O[l]);
+    }
+    else
+    {
+        if (l + 1 < nlev
+            && (rc = copy_level(b.O[l], b.U[l], h->lev[l]->level_extent, s)) != BX_OK)
+            return rc;
+        if ((rc = level_step(h, l, dt, dx, b.U[l], b.T[l], s, g)) != BX_OK)
+            return rc;
+    }
     if (l + 1 < nlev)
     {
-        if ((rc = do_fill_all(h->lev[l], b.U[l], s)) != BX_OK)
+        if ((rc = do_fill_all(h->lev[l], bb.U[l], s)) != BX_OK)
             return rc;
         const double dtf = dt / 2; // dt_level[l+1] = dt_level[l] / 2
         for (int sub = 0; sub < 2; ++sub)
-            if ((rc = time_step(h, l + 1, sub == 0 ? time : time + dtf, dtf, time, dt, dx0, b, s,
-                                false, g))
+            if ((rc = time_step(h, l + 1, sub == 0 ? time : time + dtf, dtf, time, dt, dx0, bb,
+                                s, false, g))
                 != BX_OK)
                 return rc;
         AmrPair *p = h->pairs[l];
-        if ((rc = reflux(p, b.U[l], dx, s)) != BX_OK || (rc = avg_down(p, b.U[l + 1], b.U[l], s))
-            || (rc = do_fill_all(h->lev[l], b.U[l], s)) != BX_OK)
+        if ((rc = reflux(p, bb.U[l], dx, s)) != BX_OK
+            || (rc = avg_down(p, bb.U[l + 1], bb.U[l], s))
+            || (rc = do_fill_all(h->lev[l], bb.U[l], s)) != BX_OK)
             return rc;
     }
     else if (fill_after && (rc = do_fill_all(h->lev[l], b.U[l], s)) != BX_OK)
@@ -1406,9 +1433,12 @@ static bool graph_fits(const bx_amr *a)
     return ncf <= AMR_MAXCF;
 }
 
+static int capture_one(bx_amr *a, const SchedArgs &sa, const LevelBufs &b, const double dx0[3],
+                       hipGraphExec_t *gx);
+
 // Captures one coarse step (the levels' maximum wave speeds, the schedule,
-// Amr::timeStep(0)) into a->gx, unless the graph already holds this
-// configuration.
+// Amr::timeStep(0)) into a->gx, for even and for odd steps, unless the
+// graphs already hold this configuration.
 static int capture_step(bx_amr *a, const double dx0[3], double cfl, const LevelBufs &b)
 {
     const int             nlev = (int)a->lev.size();
@@ -1438,13 +1468,14 @@ static int capture_step(bx_amr *a, const double dx0[3], double cfl, const LevelB
                        L->occ_x, L->occ_t })
             key.push_back((uint64_t)(int64_t)v);
     }
-    if (a->gx && key == a->gkey)
+    if (a->gx[0] && a->gx[1] && key == a->gkey)
         return BX_OK;
-    if (a->gx)
-    {
-        BX_HIP_CHECK(hipGraphExecDestroy(a->gx));
-        a->gx = nullptr;
-    }
+    for (hipGraphExec_t &gx : a->gx)
+        if (gx)
+        {
+            BX_HIP_CHECK(hipGraphExecDestroy(gx));
+            gx = nullptr;
+        }
     if (!a->d_sched)
         BX_HIP_CHECK(hipMalloc((void **)&a->d_sched, sizeof(AmrSched)));
     if (!a->cap)
@@ -1460,7 +1491,27 @@ static int capture_step(bx_amr *a, const double dx0[3], double cfl, const LevelB
     sa.cfl  = cfl;
     sa.nlev = nlev;
     sa.D    = a->D;
-    hipStream_t s = a->cap;
+    // odd steps start with level 0's state and scratch buffers exchanged
+    for (int q = 0; q < 2; ++q)
+    {
+        std::vector<double *> U0(b.U, b.U + nlev), O0(b.O, b.O + nlev);
+        if (q == 1)
+            std::swap(U0[0], O0[0]);
+        const int rc = capture_one(a, sa, LevelBufs{ U0.data(), b.T, O0.data() }, dx0,
+                                   &a->gx[q]);
+        if (rc != BX_OK)
+            return rc;
+    }
+    a->gkey = key;
+    return BX_OK;
+}
+
+// One coarse step of the buffers b captured into *gx.
+static int capture_one(bx_amr *a, const SchedArgs &sa, const LevelBufs &b, const double dx0[3],
+                       hipGraphExec_t *gx)
+{
+    const int   nlev = (int)a->lev.size();
+    hipStream_t s    = a->cap;
     BX_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = BX_OK;
     // the finest level's maximum comes from its last sweep of the previous
@@ -1475,7 +1526,9 @@ static int capture_step(bx_amr *a, const double dx0[3], double cfl, const LevelB
     }
     if (rc == BX_OK)
     {
-        CapturedStep g{ a->d_sched, 0, 1 << (nlev - 1) };
+        CapturedStep g{ a->d_sched, 0, 1 << (nlev - 1),
+                        std::vector<double *>(b.U, b.U + nlev),
+                        std::vector<double *>(b.O, b.O + nlev) };
         // the host's time values only shape the recursion here
         rc = time_step(a, 0, 0.0, 1.0, 0.0, 1.0, dx0, b, s, false, &g);
     }
@@ -1489,15 +1542,14 @@ static int capture_step(bx_amr *a, const double dx0[3], double cfl, const LevelB
                            : set_error(BX_ERR_HIP, std::string("amr step capture: ")
                                                        + hipGetErrorString(ec));
     }
-    const hipError_t ei = hipGraphInstantiate(&a->gx, graph, nullptr, nullptr, 0);
+    const hipError_t ei = hipGraphInstantiate(gx, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (ei != hipSuccess)
     {
-        a->gx = nullptr;
+        *gx = nullptr;
         return set_error(BX_ERR_HIP, std::string("amr step instantiate: ")
                                          + hipGetErrorString(ei));
     }
-    a->gkey = key;
     return BX_OK;
 }
 
@@ -1530,7 +1582,10 @@ extern "C" int bx_amr_run_levels(bx_amr *a, const double dx0[3], double cfl, dou
             || (rc = do_maxspeed(a->lev[nlev - 1], U[nlev - 1], s)) != BX_OK)
             return rc;
         for (; n < max_step; ++n)
-            BX_HIP_CHECK(hipGraphLaunch(a->gx, s));
+            BX_HIP_CHECK(hipGraphLaunch(a->gx[n & 1], s));
+        if ((n & 1) // level 0's state is in its scratch buffer
+            && (rc = copy_level(U[0], U_old[0], a->lev[0]->level_extent, s)) != BX_OK)
+            return rc;
         BX_HIP_CHECK(hipMemcpyAsync(&t, &a->d_sched->t, sizeof(double), hipMemcpyDeviceToHost,
                                     s));
         BX_HIP_CHECK(hipStreamSynchronize(s));

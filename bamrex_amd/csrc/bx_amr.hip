@@ -1127,7 +1127,10 @@ static int level_step(bx_amr *h, int l, double dt, const double dx[3], double *A
         if (l + 1 < (int)h->lev.size()
             && (rc = reg_fluxes(h->pairs[l], false, d, in, dt, dx, s, coef)) != BX_OK)
             return rc;
-        if ((rc = do_fill(L, out, (d + 1) % D, s)) != BX_OK)
+        // the next sweep's ghosts; after the last sweep nothing reads the
+        // level's ghosts before a full fill (time_step's fill after a coarser
+        // level's step, fill_level before a finer level's, the run's last)
+        if (d + 1 < D && (rc = do_fill(L, out, d + 1, s)) != BX_OK)
             return rc;
         in = out;
     }

@@ -791,8 +791,9 @@ __global__ void k_round_bf16(const float *__restrict__ w, float *__restrict__ wr
 // pixel and plane): a lane's B operand (4 channels of one pixel) is one
 // 8-byte access, and the 16 lanes that share a channel group touch 128
 // consecutive bytes.
-// Every global load of a thread (the input halo, the operator diagonal of
-// its output pixels) is issued before the first use of any of them.
+// A block's global loads run ahead of their use: the next tile's input halo
+// while the current tile's four layers run, the operator diagonal of the
+// output pixels during layer 4.
 // Zero-padded fields (not PERIODIC) store 0 for every activation outside the
 // image, as the layered kernels read their padding.
 constexpr int FT     = 28;
@@ -818,8 +819,8 @@ struct FusedIO
     const double *part; // INFER: SS_BLOCKS partial sums of r^2 (sigma = rms of r)
     double        n;    // INFER: number of cells of r
     const double *D;    // INFER: operator diagonal
-    double       *e;   // INFER: correction
-    float        *y;   // raw mode: network output
+    double       *e;    // INFER: correction
+    float        *y;    // raw mode: network output
     double        omega;
 };
 

@@ -589,6 +589,58 @@ def test_trained_network_gain_held_out_1024():
     net.close()
 
 
+def test_learned_solver_drives_the_imex_trajectory():
+    """PEEK-MUSCL-SSP222 Gresho steps at 256^2 whose state advances with the
+    pressure of the learned solver (FGMRES + V-cycle + trained corrector, to
+    1e-10) stay within 1e-6 of the trajectory advanced with plain multigrid
+    solves to the same tolerance, step after step: the solver the network
+    serves is usable as the stage solve, not only measured beside it.  (A
+    relative residual of 1e-10 bounds each pressure's relative error by
+    ~1e-10 times the operator's condition number, ~700 here.)"""
+    announce("learned solver trajectory 256")
+    n, steps = 256, 4
+    train_wl = imex.GreshoWorkload(n, advection="muscl_rusanov", tableau="SSP222")
+    s = HelmholtzSolver(n, n, train_wl.dx)
+    R = DeviceArray((24, n, n))
+    nf = 0
+    for _ in range(8):
+        sy = train_wl.next_system()
+        s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+        db, dxv = dev(sy["rhs"]), dev(sy["p_prev"])
+        got, _ = imex.harvest_residuals(s, db, dxv, R, nf, 24 - nf)
+        nf += got
+        train_wl.advance(sy, dxv)
+    train_wl.free()
+    net = LearnedSmoother(seed=4)
+    s.attach(net, omega=1.0, mode="corrector")
+    net.train(R, nf, steps=300, crop=64, batch=16, lr=1e-3, seed=2)
+    R.free()
+    states = {}
+    for meth in ("mg", "fgmres_cnn"):
+        wl = imex.GreshoWorkload(n, advection="muscl_rusanov", tableau="SSP222")
+        traj = []
+        while len(traj) < steps:
+            sy = wl.next_system()
+            s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
+            db, dxv = dev(sy["rhs"]), dev(sy["p_prev"])
+            st = s.solve(db, dxv, meth, rtol=1e-10, maxiter=100)
+            assert st["converged"] and not st["fell_back"], (meth, st)
+            if wl.advance(sy, dxv):
+                traj.append(wl.U.download())
+        states[meth] = traj
+        wl.free()
+    diffs = []
+    for k in range(steps):
+        a, b = states["mg"][k], states["fgmres_cnn"][k]
+        scale = np.max(np.abs(a), axis=(1, 2), keepdims=True)
+        diffs.append(float(np.max(np.abs(a - b) / scale)))
+    print("[trajectory] max relative state difference per step %s" % diffs, flush=True)
+    assert max(diffs) <= 1e-6
+    assert not np.array_equal(states["mg"][-1], states["fgmres_cnn"][-1])  # two solvers ran
+    s.close()
+    net.close()
+
+
 def test_blocked_smoother_vcycle_matches_per_colour_launches():
     """The V-cycle with blocked smoothing and fused residual-restriction is
     bit-identical to the per-colour-launch V-cycle, in FP64 (and equal to

@@ -13,6 +13,7 @@ from bamrex_amd import problems
 from bamrex_amd.level import Layout, Level, chop_domain, make_params
 
 from conftest import announce
+from ghosts import GHOST_BCS, expected_fab, padded
 
 pytestmark = pytest.mark.gpu
 
@@ -152,28 +153,31 @@ def test_sweep_limiting_modes(oracle_bxpow, mode):
         lev.close()
 
 
-@pytest.mark.parametrize("bc", ["transmissive", "reflective", "periodic"])
-def test_fill_boundary_matches_oracle(oracle_bxpow, bc):
-    dim, n = 2, 20
-    lay, U = random_level(dim, n, 7, seed=3)
-    dom_lo, dom_hi = (0, 0), (n - 1, n - 1)
+@pytest.mark.parametrize("dim,bc_lo,bc_hi", GHOST_BCS)
+def test_fill_boundary_matches_oracle(oracle_bxpow, dim, bc_lo, bc_hi):
+    """Every ghost cell, corners and edges included, against the oracle's
+    fill and against numpy's padding of the whole domain."""
+    n = 20 if dim == 2 else 12
+    lay, U = random_level(dim, n, 7 if dim == 2 else 5, seed=3)
+    dom_lo, dom_hi = (0,) * dim, (n - 1,) * dim
     gp = make_params(dim, 1.4)
     op = pyoracle.params(dim, 1.4)
-    lev = Level(gp, lay, dom_lo, dom_hi, [bc] * 2, [bc] * 2)
+    field = np.zeros((dim + 2,) + (n,) * dim)
+    for b, (lo, hi) in enumerate(lay.boxes):
+        sl = (slice(None),) + tuple(slice(lo[d], hi[d] + 1) for d in reversed(range(dim)))
+        field[sl] = lay.valid(U, b)
+    full = padded(field, lay.ng, bc_lo, bc_hi)
+    lev = Level(gp, lay, dom_lo, dom_hi, bc_lo, bc_hi)
     try:
         lev.set_state(U)
-        announce("fill_boundary bc=%s" % bc)
+        announce("fill_boundary dim=%d bc_lo=%s bc_hi=%s" % (dim, bc_lo, bc_hi))
         lev.fill_boundary()
         got = lev.get_state()
         ref = U.copy()
-        oracle_bxpow.fill_boundary(op, lay, ref, dom_lo, dom_hi, [bc] * 2, [bc] * 2)
-        g = lay.ng
+        oracle_bxpow.fill_boundary(op, lay, ref, dom_lo, dom_hi, bc_lo, bc_hi)
         for b in range(len(lay.boxes)):
-            fa, fr = lay.fab(got, b), lay.fab(ref, b)
-            ny, nx = fa.shape[1] - 2 * g, fa.shape[2] - 2 * g
-            # face ghosts (the ones the sweeps read); corners are not read
-            np.testing.assert_array_equal(fa[:, g:g + ny, :], fr[:, g:g + ny, :])
-            np.testing.assert_array_equal(fa[:, :, g:g + nx], fr[:, :, g:g + nx])
+            np.testing.assert_array_equal(lay.fab(got, b), lay.fab(ref, b), "box %d" % b)
+            np.testing.assert_array_equal(lay.fab(got, b), expected_fab(full, lay, b), "box %d" % b)
     finally:
         lev.close()
 

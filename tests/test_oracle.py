@@ -35,6 +35,7 @@ import pyoracle
 from bamrex_amd.level import Layout, chop_domain
 from bamrex_amd import problems
 from exact_riemann import star_state, sample
+from ghosts import GHOST_BCS, expected_fab, padded
 
 # ------------------------------------------------------- Euler01 / BoxTest
 BOX2D = dict(primv_L=[2, 1, -2, 3.3], primv_R=[1, -2, -1, 3.6],
@@ -447,6 +448,28 @@ def test_fill_boundary_face_ghosts(oracle_glibc, bc):
             np.testing.assert_array_equal(f[:, j + 2, gi], exp)
     # hi-x ghosts of box 0 come from box 1 (interior)
     np.testing.assert_array_equal(f[:, 2:6, 6], field[:, 0:4, 4])
+
+
+@pytest.mark.parametrize("dim,bc_lo,bc_hi", GHOST_BCS)
+def test_fill_boundary_all_ghosts_match_numpy_pad(oracle_glibc, dim, bc_lo, bc_hi):
+    """Every ghost cell -- faces, edges and corners, interior and physical --
+    of a level chopped into uneven boxes equals the cell numpy's padding
+    gives it; a mixed-up axis, a wrong reflection index, a missing sign or
+    a corner taken from the unfilled neighbour fails."""
+    n, ng = (10, 3) if dim == 2 else (7, 2)
+    boxes = chop_domain((0,) * dim, (n - 1,) * dim, 4)
+    lay = Layout(dim, boxes, ng)
+    U = lay.new_array()
+    rng = np.random.default_rng(5)
+    field = rng.uniform(-1, 1, (dim + 2,) + (n,) * dim)
+    for b, (lo, hi) in enumerate(boxes):
+        sl = (slice(None),) + tuple(slice(lo[d], hi[d] + 1) for d in reversed(range(dim)))
+        lay.valid(U, b)[...] = field[sl]
+    p = pyoracle.params(dim, 1.4)
+    oracle_glibc.fill_boundary(p, lay, U, (0,) * dim, (n - 1,) * dim, bc_lo, bc_hi)
+    full = padded(field, ng, bc_lo, bc_hi)
+    for b in range(len(boxes)):
+        np.testing.assert_array_equal(lay.fab(U, b), expected_fab(full, lay, b), "box %d" % b)
 
 
 def test_scheme_amplifies_one_ulp_perturbations(oracle_glibc, oracle_bxpow):

@@ -198,7 +198,10 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
     int pairs, const T *__restrict__ coarse)
 {
     // each thread owns the region cells tid + 256 m; their coefficients stay
-    // in registers across the half-sweeps, phi lives in LDS
+    // in registers across the half-sweeps, phi lives in LDS.  Every global
+    // load of the region is issued before the first LDS write (phi and the
+    // coarse correction land in registers first), so a thread waits for
+    // memory once instead of once per region cell.
     constexpr int MAXC = ((SM_TY + 2 * SM_MAXH) * (SM_TX + 2 * SM_MAXH) + 255) / 256;
     __shared__ T ph[(SM_TY + 2 * SM_MAXH) * (SM_TX + 2 * SM_MAXH)];
     const int NH = 2 * pairs;
@@ -209,8 +212,12 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
     const T cdhx = sc[2] / (2 * hx), cdhy = sc[2] / (2 * hy);
     auto wrapx = [&](int g) { g %= nx; return g < 0 ? g + nx : g; };
     auto wrapy = [&](int g) { g %= ny; return g < 0 ? g + ny : g; };
-    T   r_rhs[MAXC], r_a[MAXC], r_bxl[MAXC], r_bxr[MAXC], r_byl[MAXC], r_byr[MAXC];
-    T   r_cx[MAXC], r_cy[MAXC];
+    // the two x faces of a cell are adjacent in memory: one paired load
+    // straight into a register pair
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    T   r_rhs[MAXC], r_a[MAXC], r_byl[MAXC], r_byr[MAXC];
+    T   r_cx[MAXC], r_cy[MAXC], r_ph[MAXC], r_co[MAXC];
+    T2  r_bx[MAXC];
     int r_dist[MAXC], r_par[MAXC];
 #pragma unroll
     for (int m = 0; m < MAXC; ++m)
@@ -222,10 +229,9 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
         const int lj = c / RW, li = c - lj * RW;
         const int gi = wrapx(x0 + li), gj = wrapy(y0 + lj);
         const long long g = (long long)gj * nx + gi;
-        T v = phi[g];
+        r_ph[m] = phi[g];
         if (coarse)
-            v += coarse[(long long)(gj >> 1) * (nx >> 1) + (gi >> 1)];
-        ph[c] = v;
+            r_co[m] = coarse[(long long)(gj >> 1) * (nx >> 1) + (gi >> 1)];
         // distance from the region edge; only interior cells are updated
         const int d = min(min(li, RW - 1 - li), min(lj, RH - 1 - lj));
         r_dist[m]   = d;
@@ -234,13 +240,16 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
             continue;
         r_rhs[m] = rhs[g];
         r_a[m]   = a[g];
-        r_bxl[m] = bX[(long long)gj * (nx + 1) + gi];
-        r_bxr[m] = bX[(long long)gj * (nx + 1) + gi + 1];
+        __builtin_memcpy(&r_bx[m], bX + (long long)gj * (nx + 1) + gi, sizeof(T2));
         r_byl[m] = bY[(long long)gj * nx + gi];
         r_byr[m] = bY[(long long)(gj + 1) * nx + gi];
         r_cx[m]  = cX ? cX[g] : T(0);
         r_cy[m]  = cY ? cY[g] : T(0);
     }
+#pragma unroll
+    for (int m = 0; m < MAXC; ++m)
+        if (r_dist[m] >= 0)
+            ph[threadIdx.x + 256 * m] = coarse ? r_ph[m] + r_co[m] : r_ph[m];
     __syncthreads();
     for (int k = 0; k < NH; ++k)
     {
@@ -252,7 +261,7 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
             if (r_dist[m] <= k || ((r_par[m] + rb) & 1))
                 continue;
             const int c = threadIdx.x + 256 * m;
-            ph[c] = gsrb_value(r_rhs[m], r_a[m], r_bxl[m], r_bxr[m], r_byl[m], r_byr[m],
+            ph[c] = gsrb_value(r_rhs[m], r_a[m], r_bx[m].x, r_bx[m].y, r_byl[m], r_byr[m],
                                r_cx[m], r_cy[m], cX != nullptr, ph[c - 1], ph[c + 1],
                                ph[c - RW], ph[c + RW], ph[c], alpha, dhx, dhy, cdhx, cdhy);
         }
@@ -386,12 +395,31 @@ __device__ void tail_smooth(const LdsLevel<T> &L, T alpha, T beta, T gam, int pa
         }
 }
 
+// Workgroup copy of n elements from global memory into LDS, one dword per
+// lane per iteration, each going from memory straight into LDS
+// (global_load_lds_dword: lane i of a wave writes the dword at M0 + 4 i, so
+// the wave passes the LDS address of its lane 0, which is the same for every
+// lane because blockDim.x is a multiple of 64).  No thread waits for a load
+// before issuing the next one; lds_copy_wait() completes them all.
+typedef __attribute__((address_space(1))) void global_void;
+typedef __attribute__((address_space(3))) void lds_void;
+
 template <class T>
 __device__ void copy_in(T *dst, const T *__restrict__ src, int n)
 {
-    for (int c = threadIdx.x; c < n; c += blockDim.x)
-        dst[c] = src[c];
+    static_assert(sizeof(T) % 4 == 0, "dword copies");
+    const int       nw   = n * (int)(sizeof(T) / 4);
+    const unsigned *s32  = reinterpret_cast<const unsigned *>(src);
+    unsigned       *d32  = reinterpret_cast<unsigned *>(dst);
+    const int       lane = threadIdx.x & 63;
+    for (int c = threadIdx.x; c < nw; c += blockDim.x)
+        __builtin_amdgcn_global_load_lds((global_void *)(s32 + c), (lds_void *)(d32 + (c - lane)),
+                                         4, 0, 0);
 }
+
+// This wave's copy_in loads have reached LDS (vmcnt 0); a workgroup barrier
+// after it makes every wave's copies visible.
+__device__ __forceinline__ void lds_copy_wait() { __builtin_amdgcn_s_waitcnt(0); }
 
 // A: level table in device memory (the levels from the first tail level)
 template <class T>
@@ -421,6 +449,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(const TailArgs<T> 
             copy_in(sm + G.o_rhs, G.rhs, n);
         }
     }
+    lds_copy_wait();
     __syncthreads();
     const int last = nlev - 1;
     for (int l = 0; l < last; ++l)

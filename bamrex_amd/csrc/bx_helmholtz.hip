@@ -267,7 +267,7 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
         }
         __syncthreads();
     }
-    for (int c = threadIdx.x; c < SM_TX * SM_TY; c += blockDim.x)
+    for (int c = threadIdx.x; c < SM_TX * SM_TY; c += 256)
     {
         const int tj = c / SM_TX, ti = c - tj * SM_TX;
         const int gi = blockIdx.x * SM_TX + ti, gj = blockIdx.y * SM_TY + tj;
@@ -321,7 +321,7 @@ template <class T> struct TailLevel
     T       *x, *rhs;                // global work arrays (read/written for level 0 only)
     int      nx, ny;
     T        hx, hy, dxinv, dyinv;
-    int      o_a, o_bX, o_bY, o_cX, o_cY, o_x, o_rhs, o_res; // LDS offsets (elements)
+    int      o_a, o_bX, o_bY, o_cX, o_cY, o_x, o_rhs; // LDS offsets (elements)
 };
 
 template <class T> struct TailArgs
@@ -332,7 +332,7 @@ template <class T> struct TailArgs
 // LDS view of one tail level
 template <class T> struct LdsLevel
 {
-    T  *a, *bX, *bY, *cX, *cY, *x, *rhs, *res;
+    T  *a, *bX, *bY, *cX, *cY, *x, *rhs;
     int nx, ny;
     T   hx, hy, dxinv, dyinv;
 };
@@ -348,7 +348,6 @@ __device__ LdsLevel<T> lds_level(const TailLevel<T> &G, T *sm)
     L.cY  = G.cY ? sm + G.o_cY : nullptr;
     L.x   = sm + G.o_x;
     L.rhs = sm + G.o_rhs;
-    L.res = sm + G.o_res;
     L.nx = G.nx; L.ny = G.ny; L.hx = G.hx; L.hy = G.hy; L.dxinv = G.dxinv; L.dyinv = G.dyinv;
     return L;
 }
@@ -380,14 +379,16 @@ __device__ void tail_smooth(const LdsLevel<T> &L, T alpha, T beta, T gam, int pa
         __syncthreads();
         return;
     }
+    // the n / 2 cells of colour rb only (nx is even): cell k of the colour
+    // is row j = k / (nx / 2), column 2 (k mod nx / 2) + ((j + rb) & 1), so
+    // every lane of a wave has a cell to update
+    const int hx2 = L.nx >> 1, h = n >> 1;
     for (int it = 0; it < pairs; ++it)
         for (int rb = 0; rb < 2; ++rb)
         {
-            for (int c = threadIdx.x; c < n; c += blockDim.x)
+            for (int k = threadIdx.x; k < h; k += TAIL_THREADS)
             {
-                const int j = c / L.nx, i = c - j * L.nx;
-                if ((i + j + rb) & 1)
-                    continue;
+                const int j = k / hx2, i = 2 * (k - j * hx2) + ((j + rb) & 1);
                 gsrb_at(L.x, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, alpha, dhx, dhy,
                         cdhx, cdhy, i, j);
             }
@@ -399,7 +400,7 @@ __device__ void tail_smooth(const LdsLevel<T> &L, T alpha, T beta, T gam, int pa
 // lane per iteration, each going from memory straight into LDS
 // (global_load_lds_dword: lane i of a wave writes the dword at M0 + 4 i, so
 // the wave passes the LDS address of its lane 0, which is the same for every
-// lane because blockDim.x is a multiple of 64).  No thread waits for a load
+// lane because TAIL_THREADS is a multiple of 64).  No thread waits for a load
 // before issuing the next one; lds_copy_wait() completes them all.
 typedef __attribute__((address_space(1))) void global_void;
 typedef __attribute__((address_space(3))) void lds_void;
@@ -412,7 +413,7 @@ __device__ void copy_in(T *dst, const T *__restrict__ src, int n)
     const unsigned *s32  = reinterpret_cast<const unsigned *>(src);
     unsigned       *d32  = reinterpret_cast<unsigned *>(dst);
     const int       lane = threadIdx.x & 63;
-    for (int c = threadIdx.x; c < nw; c += blockDim.x)
+    for (int c = threadIdx.x; c < nw; c += TAIL_THREADS)
         __builtin_amdgcn_global_load_lds((global_void *)(s32 + c), (lds_void *)(d32 + (c - lane)),
                                          4, 0, 0);
 }
@@ -457,20 +458,23 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(const TailArgs<T> 
         const LdsLevel<T> L = lds_level(A->lev[l], sm);
         const LdsLevel<T> C = lds_level(A->lev[l + 1], sm);
         tail_smooth(L, alpha, beta, gam, nu1);
-        const int n = L.nx * L.ny;
-        for (int c = threadIdx.x; c < n; c += blockDim.x)
-        {
-            const int j = c / L.nx, i = c - j * L.nx;
-            L.res[c] = L.rhs[c]
-                       - apply_at(L.x, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, alpha, beta,
-                                  gam, L.dxinv, L.dyinv, i, j);
-        }
-        __syncthreads();
+        // coarse rhs = R (rhs - A x): the four fine residuals of a coarse
+        // cell computed where they are averaged (the values and the
+        // averaging order of restrict_at over a stored residual)
         const int nc = C.nx * C.ny;
-        for (int c = threadIdx.x; c < nc; c += blockDim.x)
+        for (int c = threadIdx.x; c < nc; c += TAIL_THREADS)
         {
             const int J = c / C.nx, I = c - J * C.nx;
-            C.rhs[c] = restrict_at(L.res, C.nx, I, J);
+            T         r[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const int i = 2 * I + (q & 1), j = 2 * J + (q >> 1);
+                r[q] = L.rhs[j * L.nx + i]
+                       - apply_at(L.x, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, alpha, beta,
+                                  gam, L.dxinv, L.dyinv, i, j);
+            }
+            C.rhs[c] = T(0.25) * (r[0] + r[1] + r[2] + r[3]);
             C.x[c]   = T(0);
         }
         __syncthreads();
@@ -481,7 +485,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(const TailArgs<T> 
         const LdsLevel<T> L = lds_level(A->lev[l], sm);
         const LdsLevel<T> C = lds_level(A->lev[l + 1], sm);
         const int n = L.nx * L.ny;
-        for (int c = threadIdx.x; c < n; c += blockDim.x)
+        for (int c = threadIdx.x; c < n; c += TAIL_THREADS)
         {
             const int j = c / L.nx, i = c - j * L.nx;
             L.x[c] += C.x[(j >> 1) * C.nx + (i >> 1)];
@@ -490,7 +494,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(const TailArgs<T> 
         tail_smooth(L, alpha, beta, gam, nu2);
     }
     const TailLevel<T> &G0 = A->lev[0];
-    for (int c = threadIdx.x; c < G0.nx * G0.ny; c += blockDim.x)
+    for (int c = threadIdx.x; c < G0.nx * G0.ny; c += TAIL_THREADS)
         G0.x[c] = sm[G0.o_x + c];
 }
 
@@ -562,7 +566,8 @@ __global__ void k_avg_faces_y(const T *__restrict__ f, T *__restrict__ c,
 // Bottom solve of the V-cycle in one launch: `iters` red/black sweeps of a
 // level of at most BOTTOM_MAX cells, held in LDS by a single workgroup.  The
 // cell update is the one of k_gsrb, so the result is the same bits as
-// 2 * iters separate colour launches.
+// 2 * iters separate colour launches.  Launched with 256 threads (its loops
+// step by that constant, so no phase reads the block size from memory).
 constexpr int BOTTOM_MAX = 1024;
 
 template <class T>
@@ -580,7 +585,7 @@ __global__ __launch_bounds__(256) void k_bottom_gsrb(
     __shared__ T cX[BOTTOM_MAX], cY[BOTTOM_MAX];
     const int n   = nx * ny;
     const int tid = threadIdx.x;
-    for (int c = tid; c < n; c += blockDim.x)
+    for (int c = tid; c < n; c += 256)
     {
         phi[c] = phi_g[c];
         rhs[c] = rhs_g[c];
@@ -591,15 +596,15 @@ __global__ __launch_bounds__(256) void k_bottom_gsrb(
             cY[c] = cY_g[c];
         }
     }
-    for (int c = tid; c < (nx + 1) * ny; c += blockDim.x)
+    for (int c = tid; c < (nx + 1) * ny; c += 256)
         bX[c] = bX_g[c];
-    for (int c = tid; c < nx * (ny + 1); c += blockDim.x)
+    for (int c = tid; c < nx * (ny + 1); c += 256)
         bY[c] = bY_g[c];
     __syncthreads();
     for (int it = 0; it < iters; ++it)
         for (int redblack = 0; redblack < 2; ++redblack)
         {
-            for (int c = tid; c < n; c += blockDim.x)
+            for (int c = tid; c < n; c += 256)
             {
                 const int j = c / nx, i = c - j * nx;
                 if ((i + j + redblack) & 1)
@@ -620,7 +625,7 @@ __global__ __launch_bounds__(256) void k_bottom_gsrb(
             }
             __syncthreads();
         }
-    for (int c = tid; c < n; c += blockDim.x)
+    for (int c = tid; c < n; c += 256)
         phi_g[c] = phi[c];
 }
 
@@ -1108,6 +1113,11 @@ struct bx_helm
     hipGraphExec_t  pgraph[2]  = { nullptr, nullptr };
     long long       gen = 0, pgraph_gen[2] = { -1, -1 }, pgraph_epoch[2] = { -1, -1 };
     double         *pin = nullptr, *pout = nullptr;
+    // one multigrid iteration replayed as a captured graph (slot 0: V-cycle
+    // of per-colour launches, slot 1: blocked smoother and LDS tail), on b in
+    // pin and x in pout
+    hipGraphExec_t  mgraph[2]     = { nullptr, nullptr };
+    long long       mgraph_gen[2] = { -1, -1 };
 };
 
 static thread_local std::string g_helm_err;
@@ -1201,7 +1211,7 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
         auto      need = [&](const MGLevel<T> &L) {
             const long long n = (long long)L.nx * L.ny;
             return (n + (L.nx + 1) * (long long)L.ny + L.nx * (long long)(L.ny + 1)
-                    + (L.cX ? 2 * n : 0) + 3 * n)
+                    + (L.cX ? 2 * n : 0) + 2 * n)
                    * (long long)sizeof(T);
         };
         while (l0 > 0 && nl - (l0 - 1) <= TAIL_MAX_LEVELS
@@ -1227,7 +1237,6 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
                 tl.o_cY  = off; off += L.cY ? n : 0;
                 tl.o_x   = off; off += n;
                 tl.o_rhs = off; off += n;
-                tl.o_res = off; off += n;
             }
             H_CHECK(hipMalloc((void **)&mg.d_tail, sizeof(TailArgs<T>)));
             H_CHECK(hipMemcpy(mg.d_tail, &ta, sizeof(ta), hipMemcpyHostToDevice));
@@ -1459,6 +1468,9 @@ extern "C" void bx_helm_destroy(bx_helm *h)
     h->mg64.free_all();
     h->mg32.free_all();
     for (hipGraphExec_t g : h->pgraph)
+        if (g)
+            (void)hipGraphExecDestroy(g);
+    for (hipGraphExec_t g : h->mgraph)
         if (g)
             (void)hipGraphExecDestroy(g);
     if (h->cap_stream)
@@ -1760,39 +1772,105 @@ static int precond_replay(bx_helm *h, const double *rv, double *e, hipStream_t s
     return BX_OK;
 }
 
+// One multigrid iteration on the device: x += V(r) with r = h->r, then
+// r = b - A x and the partial sums of ||r||^2 reduced into dots[0] (the
+// kernels of norm2 without its read-back).
+static void mg_iteration(bx_helm *h, const double *b, double *x, bool fused, hipStream_t s)
+{
+    MGHierarchy<double> &mg = h->mg64;
+    MGLevel<double>     &L0 = mg.lev[0];
+    const long long      n  = h->n;
+    hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, L0.rhs, h->r, n, 1.0);
+    hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s, L0.x, n, 0.0);
+    vcycle(mg, 0, s, fused);
+    hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, s, x, L0.x, n, 1.0);
+    residual64(h, b, x, h->r, s);
+    hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->r, h->r, n, 1,
+                       h->partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, h->partial, h->nb_dot, 1,
+                       h->dots);
+}
+
+// The captured iteration on b = pin, x = pout, recaptured when the
+// hierarchy or the cycle options change (h->gen).
+static int mg_iteration_graph(bx_helm *h, bool fused, hipStream_t s, hipGraphExec_t *out)
+{
+    const int       slot = fused ? 1 : 0;
+    hipGraphExec_t &gx   = h->mgraph[slot];
+    if (!gx || h->mgraph_gen[slot] != h->gen)
+    {
+        if (gx)
+        {
+            H_CHECK(hipGraphExecDestroy(gx));
+            gx = nullptr;
+        }
+        if (!h->cap_stream)
+            H_CHECK(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+        H_CHECK(hipStreamSynchronize(s));
+        hipGraph_t g = nullptr;
+        H_CHECK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+        mg_iteration(h, h->pin, h->pout, fused, h->cap_stream);
+        hipError_t ec = hipStreamEndCapture(h->cap_stream, &g);
+        if (ec != hipSuccess)
+            return herr(BX_ERR_HIP, std::string("mg iteration capture: ") + hipGetErrorString(ec));
+        ec = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ec != hipSuccess)
+        {
+            gx = nullptr;
+            return herr(BX_ERR_HIP,
+                        std::string("mg iteration instantiate: ") + hipGetErrorString(ec));
+        }
+        h->mgraph_gen[slot] = h->gen;
+    }
+    *out = gx;
+    return BX_OK;
+}
+
 // Plain multigrid solve (MLMG::solve semantics, FP64): x <- x + V(b - A x)
-// until ||b - A x||_2 <= rtol ||b||_2.
+// until ||b - A x||_2 <= rtol ||b||_2.  With graphs on, each iteration is one
+// replay of the captured iteration (the same kernels in the same order) on
+// copies of b and x.
 // rn0 >= 0: h->r already holds b - A x, of norm rn0.
 static int solve_mg(bx_helm *h, const double *b, double *x, double rtol,
                     int maxiter, double bnorm, bx_solve_stats *st,
                     hipStream_t s, bool fused = false, double rn0 = -1)
 {
-    MGHierarchy<double> &mg = h->mg64;
-    MGLevel<double>     &L0 = mg.lev[0];
-    const long long      n  = h->n;
-    double               rn = rn0;
-    int                  rc;
+    const long long bytes = sizeof(double) * h->n;
+    double          rn    = rn0;
+    int             rc;
     if (rn0 < 0)
     {
         residual64(h, b, x, h->r, s);
         if ((rc = norm2(h, h->r, s, &rn)))
             return rc;
     }
-    int it = 0;
+    hipGraphExec_t gx = nullptr;
+    int            it = 0;
     while (rn > rtol * bnorm && it < maxiter)
     {
-        hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, L0.rhs, h->r, n,
-                           1.0);
-        hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s, L0.x, n,
-                           0.0);
-        vcycle(mg, 0, s, fused);
-        hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, s, x, L0.x, n, 1.0);
-        residual64(h, b, x, h->r, s);
-        if ((rc = norm2(h, h->r, s, &rn)))
-            return rc;
+        if (it == 0 && h->use_graph)
+        {
+            if ((rc = mg_iteration_graph(h, fused, s, &gx)))
+                return rc;
+            H_CHECK(hipMemcpyAsync(h->pin, b, bytes, hipMemcpyDeviceToDevice, s));
+            H_CHECK(hipMemcpyAsync(h->pout, x, bytes, hipMemcpyDeviceToDevice, s));
+        }
+        if (gx)
+            H_CHECK(hipGraphLaunch(gx, s));
+        else
+        {
+            mg_iteration(h, b, x, fused, s);
+            H_CHECK(hipGetLastError());
+        }
+        H_CHECK(hipMemcpyAsync(h->hstat, h->dots, sizeof(double), hipMemcpyDeviceToHost, s));
+        H_CHECK(hipStreamSynchronize(s));
+        rn = std::sqrt(h->hstat[0]);
         ++it;
         ++st->precond_applies;
     }
+    if (gx)
+        H_CHECK(hipMemcpyAsync(x, h->pout, bytes, hipMemcpyDeviceToDevice, s));
     st->iters += it;
     st->rel_resid = rn / bnorm;
     st->converged = rn <= rtol * bnorm;

@@ -187,26 +187,29 @@ __global__ __launch_bounds__(256) void k_prolong_add(const T *__restrict__ c,
 // same launch still read the halo of a tile after it has been written.  With `coarse` the prolongation phi += P e_c is applied at load
 // (the post-smoothing of a V-cycle).  Coefficients are read from global
 // memory (read-only, cache-resident across the half-sweeps).
+// Tiles: 64 x 16 on large levels; 32 x 8 where 64 x 16 tiles would leave
+// most CUs idle (a small level is latency-bound: more, shorter workgroups).
 constexpr int SM_TX = 64, SM_TY = 16, SM_MAXH = 8;
+constexpr int SM_TX_SMALL = 32, SM_TY_SMALL = 8;
 
-template <class T>
-__global__ __launch_bounds__(256) void k_gsrb_tiled(
+template <class T, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_gsrb_tiled(
     const T *__restrict__ phi, T *__restrict__ phi_out, const T *__restrict__ rhs,
     const T *__restrict__ a,
     const T *__restrict__ bX, const T *__restrict__ bY, const T *__restrict__ cX,
     const T *__restrict__ cY, int nx, int ny, const T *__restrict__ sc, T hx, T hy,
     int pairs, const T *__restrict__ coarse)
 {
-    // each thread owns the region cells tid + 256 m; their coefficients stay
+    // each thread owns the region cells tid + NT m; their coefficients stay
     // in registers across the half-sweeps, phi lives in LDS.  Every global
     // load of the region is issued before the first LDS write (phi and the
     // coarse correction land in registers first), so a thread waits for
     // memory once instead of once per region cell.
-    constexpr int MAXC = ((SM_TY + 2 * SM_MAXH) * (SM_TX + 2 * SM_MAXH) + 255) / 256;
-    __shared__ T ph[(SM_TY + 2 * SM_MAXH) * (SM_TX + 2 * SM_MAXH)];
+    constexpr int MAXC = ((TY + 2 * SM_MAXH) * (TX + 2 * SM_MAXH) + NT - 1) / NT;
+    __shared__ T ph[(TY + 2 * SM_MAXH) * (TX + 2 * SM_MAXH)];
     const int NH = 2 * pairs;
-    const int RW = SM_TX + 2 * NH, RH = SM_TY + 2 * NH;
-    const int x0 = blockIdx.x * SM_TX - NH, y0 = blockIdx.y * SM_TY - NH;
+    const int RW = TX + 2 * NH, RH = TY + 2 * NH;
+    const int x0 = blockIdx.x * TX - NH, y0 = blockIdx.y * TY - NH;
     const T alpha = sc[0];
     const T dhx = sc[1] / (hx * hx), dhy = sc[1] / (hy * hy);
     const T cdhx = sc[2] / (2 * hx), cdhy = sc[2] / (2 * hy);
@@ -222,7 +225,7 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
 #pragma unroll
     for (int m = 0; m < MAXC; ++m)
     {
-        const int c = threadIdx.x + 256 * m;
+        const int c = threadIdx.x + NT * m;
         r_dist[m]   = -1;
         if (c >= RW * RH)
             continue;
@@ -249,7 +252,7 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
 #pragma unroll
     for (int m = 0; m < MAXC; ++m)
         if (r_dist[m] >= 0)
-            ph[threadIdx.x + 256 * m] = coarse ? r_ph[m] + r_co[m] : r_ph[m];
+            ph[threadIdx.x + NT * m] = coarse ? r_ph[m] + r_co[m] : r_ph[m];
     __syncthreads();
     for (int k = 0; k < NH; ++k)
     {
@@ -260,17 +263,17 @@ __global__ __launch_bounds__(256) void k_gsrb_tiled(
             // updated at half-sweep k: distance > k and the colour of k
             if (r_dist[m] <= k || ((r_par[m] + rb) & 1))
                 continue;
-            const int c = threadIdx.x + 256 * m;
+            const int c = threadIdx.x + NT * m;
             ph[c] = gsrb_value(r_rhs[m], r_a[m], r_bx[m].x, r_bx[m].y, r_byl[m], r_byr[m],
                                r_cx[m], r_cy[m], cX != nullptr, ph[c - 1], ph[c + 1],
                                ph[c - RW], ph[c + RW], ph[c], alpha, dhx, dhy, cdhx, cdhy);
         }
         __syncthreads();
     }
-    for (int c = threadIdx.x; c < SM_TX * SM_TY; c += 256)
+    for (int c = threadIdx.x; c < TX * TY; c += NT)
     {
-        const int tj = c / SM_TX, ti = c - tj * SM_TX;
-        const int gi = blockIdx.x * SM_TX + ti, gj = blockIdx.y * SM_TY + tj;
+        const int tj = c / TX, ti = c - tj * TX;
+        const int gi = blockIdx.x * TX + ti, gj = blockIdx.y * TY + tj;
         if (gi < nx && gj < ny)
             phi_out[(long long)gj * nx + gi] = ph[(tj + NH) * RW + ti + NH];
     }
@@ -1280,6 +1283,28 @@ static void launch_smooth(const MGLevel<T> &L, const MGHierarchy<T> &mg,
                        mg.d_sc, L.hx, L.hy, redblack);
 }
 
+// `pairs` blocked red/black pairs on level L, in -> out (+ P e_c with
+// coarse), in 64 x 16 tiles of 512 threads, or 32 x 8 tiles of 256 threads
+// when the level has fewer than one 64 x 16 tile per CU of an MI355X (256).
+template <class T>
+static void launch_tiled(const MGLevel<T> &L, const MGHierarchy<T> &mg, const T *in, T *out,
+                         int pairs, const T *coarse, hipStream_t s)
+{
+    const long long big = (long long)((L.nx + SM_TX - 1) / SM_TX) * ((L.ny + SM_TY - 1) / SM_TY);
+    if (big >= 256)
+        hipLaunchKernelGGL((k_gsrb_tiled<T, SM_TX, SM_TY, 512>),
+                           dim3((unsigned)((L.nx + SM_TX - 1) / SM_TX),
+                                (unsigned)((L.ny + SM_TY - 1) / SM_TY)),
+                           dim3(512), 0, s, in, out, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx,
+                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse);
+    else
+        hipLaunchKernelGGL((k_gsrb_tiled<T, SM_TX_SMALL, SM_TY_SMALL, 256>),
+                           dim3((unsigned)((L.nx + SM_TX_SMALL - 1) / SM_TX_SMALL),
+                                (unsigned)((L.ny + SM_TY_SMALL - 1) / SM_TY_SMALL)),
+                           dim3(256), 0, s, in, out, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx,
+                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse);
+}
+
 template <class T>
 static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
 {
@@ -1314,25 +1339,19 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
     {
         // three launches per level: blocked pre-smoothing, residual +
         // restriction + zeroing, prolongation + blocked post-smoothing
-        const dim3 tg((unsigned)((L.nx + SM_TX - 1) / SM_TX),
-                      (unsigned)((L.ny + SM_TY - 1) / SM_TY));
         // pre-smoothing x -> res, post-smoothing res + P e_c -> x (the res
         // array is free in this path: the residual goes straight to C.rhs)
         const T *xs = L.x;
         if (mg.nu1 > 0)
         {
-            hipLaunchKernelGGL((k_gsrb_tiled<T>), tg, dim3(256), 0, s, L.x, L.res, L.rhs,
-                               L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.hx, L.hy,
-                               mg.nu1, (const T *)nullptr);
+            launch_tiled(L, mg, L.x, L.res, mg.nu1, (const T *)nullptr, s);
             xs = L.res;
         }
         hipLaunchKernelGGL((k_resid_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s, xs,
                            L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.dxinv,
                            L.dyinv, C.rhs, C.x);
         vcycle(mg, l + 1, s, fused);
-        hipLaunchKernelGGL((k_gsrb_tiled<T>), tg, dim3(256), 0, s, xs, L.x, L.rhs, L.a,
-                           L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.hx, L.hy, mg.nu2,
-                           (const T *)C.x);
+        launch_tiled(L, mg, xs, L.x, mg.nu2, (const T *)C.x, s);
         return;
     }
     for (int it = 0; it < mg.nu1; ++it)

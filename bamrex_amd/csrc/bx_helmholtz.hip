@@ -1116,11 +1116,6 @@ struct bx_helm
     hipGraphExec_t  pgraph[2]  = { nullptr, nullptr };
     long long       gen = 0, pgraph_gen[2] = { -1, -1 }, pgraph_epoch[2] = { -1, -1 };
     double         *pin = nullptr, *pout = nullptr;
-    // one multigrid iteration replayed as a captured graph (slot 0: V-cycle
-    // of per-colour launches, slot 1: blocked smoother and LDS tail), on b in
-    // pin and x in pout
-    hipGraphExec_t  mgraph[2]     = { nullptr, nullptr };
-    long long       mgraph_gen[2] = { -1, -1 };
 };
 
 static thread_local std::string g_helm_err;
@@ -1489,9 +1484,6 @@ extern "C" void bx_helm_destroy(bx_helm *h)
     for (hipGraphExec_t g : h->pgraph)
         if (g)
             (void)hipGraphExecDestroy(g);
-    for (hipGraphExec_t g : h->mgraph)
-        if (g)
-            (void)hipGraphExecDestroy(g);
     if (h->cap_stream)
         (void)hipStreamDestroy(h->cap_stream);
     for (void *p : { (void *)h->a, (void *)h->bX, (void *)h->bY, (void *)h->cX,
@@ -1810,86 +1802,34 @@ static void mg_iteration(bx_helm *h, const double *b, double *x, bool fused, hip
                        h->dots);
 }
 
-// The captured iteration on b = pin, x = pout, recaptured when the
-// hierarchy or the cycle options change (h->gen).
-static int mg_iteration_graph(bx_helm *h, bool fused, hipStream_t s, hipGraphExec_t *out)
-{
-    const int       slot = fused ? 1 : 0;
-    hipGraphExec_t &gx   = h->mgraph[slot];
-    if (!gx || h->mgraph_gen[slot] != h->gen)
-    {
-        if (gx)
-        {
-            H_CHECK(hipGraphExecDestroy(gx));
-            gx = nullptr;
-        }
-        if (!h->cap_stream)
-            H_CHECK(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
-        H_CHECK(hipStreamSynchronize(s));
-        hipGraph_t g = nullptr;
-        H_CHECK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
-        mg_iteration(h, h->pin, h->pout, fused, h->cap_stream);
-        hipError_t ec = hipStreamEndCapture(h->cap_stream, &g);
-        if (ec != hipSuccess)
-            return herr(BX_ERR_HIP, std::string("mg iteration capture: ") + hipGetErrorString(ec));
-        ec = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ec != hipSuccess)
-        {
-            gx = nullptr;
-            return herr(BX_ERR_HIP,
-                        std::string("mg iteration instantiate: ") + hipGetErrorString(ec));
-        }
-        h->mgraph_gen[slot] = h->gen;
-    }
-    *out = gx;
-    return BX_OK;
-}
-
 // Plain multigrid solve (MLMG::solve semantics, FP64): x <- x + V(b - A x)
-// until ||b - A x||_2 <= rtol ||b||_2.  With graphs on, each iteration is one
-// replay of the captured iteration (the same kernels in the same order) on
-// copies of b and x.
+// until ||b - A x||_2 <= rtol ||b||_2.  The iterations are launched kernel by
+// kernel: at 1024^2 the GPU is the bottleneck, and a captured iteration
+// (which needs copies of b and x into fixed buffers) measured slower.
 // rn0 >= 0: h->r already holds b - A x, of norm rn0.
 static int solve_mg(bx_helm *h, const double *b, double *x, double rtol,
                     int maxiter, double bnorm, bx_solve_stats *st,
                     hipStream_t s, bool fused = false, double rn0 = -1)
 {
-    const long long bytes = sizeof(double) * h->n;
-    double          rn    = rn0;
-    int             rc;
+    double rn = rn0;
+    int    rc;
     if (rn0 < 0)
     {
         residual64(h, b, x, h->r, s);
         if ((rc = norm2(h, h->r, s, &rn)))
             return rc;
     }
-    hipGraphExec_t gx = nullptr;
-    int            it = 0;
+    int it = 0;
     while (rn > rtol * bnorm && it < maxiter)
     {
-        if (it == 0 && h->use_graph)
-        {
-            if ((rc = mg_iteration_graph(h, fused, s, &gx)))
-                return rc;
-            H_CHECK(hipMemcpyAsync(h->pin, b, bytes, hipMemcpyDeviceToDevice, s));
-            H_CHECK(hipMemcpyAsync(h->pout, x, bytes, hipMemcpyDeviceToDevice, s));
-        }
-        if (gx)
-            H_CHECK(hipGraphLaunch(gx, s));
-        else
-        {
-            mg_iteration(h, b, x, fused, s);
-            H_CHECK(hipGetLastError());
-        }
+        mg_iteration(h, b, x, fused, s);
+        H_CHECK(hipGetLastError());
         H_CHECK(hipMemcpyAsync(h->hstat, h->dots, sizeof(double), hipMemcpyDeviceToHost, s));
         H_CHECK(hipStreamSynchronize(s));
         rn = std::sqrt(h->hstat[0]);
         ++it;
         ++st->precond_applies;
     }
-    if (gx)
-        H_CHECK(hipMemcpyAsync(x, h->pout, bytes, hipMemcpyDeviceToDevice, s));
     st->iters += it;
     st->rel_resid = rn / bnorm;
     st->converged = rn <= rtol * bnorm;

@@ -601,24 +601,43 @@ __device__ __forceinline__ unsigned long long block_speed_max(const double *__re
                                                               long long n, double dx, double dy,
                                                               double adia, double eps)
 {
+    // four cells per thread per pass, spaced by the grid's width, with all
+    // their loads issued before any arithmetic; a cell past the end repeats
+    // the pass's first cell (the maximum does not change)
+    constexpr int      K = 4;
+    const long long    T = (long long)gridDim.x * blockDim.x;
     unsigned long long m = 0ULL;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-         c += (long long)gridDim.x * blockDim.x)
+    for (long long c0 = (long long)blockIdx.x * blockDim.x + threadIdx.x; c0 < n; c0 += K * T)
     {
-        double s;
-        if (ACOUSTIC)
+        double q[K][4];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
         {
-            const double r   = U[c];
-            const double u   = U[c + n] / r;
-            const double v   = U[c + 2 * n] / r;
-            const double usq = u * u + v * v;
-            const double p   = (adia - 1) * (U[c + 3 * n] - 0.5 * eps * r * usq);
-            s                = sqrt(adia * p / (r * eps)) + sqrt(u * u + v * v);
+            const long long c = c0 + k * T < n ? c0 + k * T : c0;
+            q[k][0]           = U[c];
+            q[k][1]           = U[c + n];
+            q[k][2]           = U[c + 2 * n];
+            q[k][3]           = ACOUSTIC ? U[c + 3 * n] : 0.0;
         }
-        else
-            s = (fabs(U[c + n]) / dx + fabs(U[c + 2 * n]) / dy) / U[c];
-        const unsigned long long b = s > 0 ? (unsigned long long)__double_as_longlong(s) : 0ULL;
-        m = b > m ? b : m;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            double s;
+            if (ACOUSTIC)
+            {
+                const double r   = q[k][0];
+                const double u   = q[k][1] / r;
+                const double v   = q[k][2] / r;
+                const double usq = u * u + v * v;
+                const double p   = (adia - 1) * (q[k][3] - 0.5 * eps * r * usq);
+                s                = sqrt(adia * p / (r * eps)) + sqrt(u * u + v * v);
+            }
+            else
+                s = (fabs(q[k][1]) / dx + fabs(q[k][2]) / dy) / q[k][0];
+            const unsigned long long bits
+                = s > 0 ? (unsigned long long)__double_as_longlong(s) : 0ULL;
+            m = bits > m ? bits : m;
+        }
     }
     // block maximum first: one atomic per block (per-wave atomics on one
     // address serialise in L2)

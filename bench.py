@@ -49,6 +49,9 @@ def main():
                     help="also run a plain copy kernel of this size (counter calibration)")
     ap.add_argument("--no-imex", action="store_true", help="skip the IMEX pressure-solve leg")
     ap.add_argument("--imex-only", action="store_true", help="run only the IMEX pressure-solve leg")
+    ap.add_argument("--imex-profile-only", action="store_true",
+                    help="short IMEX workload for kernel profiling: --imex-steps steps, each "
+                    "stage solved by every method (untrained network), no harvest or training")
     ap.add_argument("--imex-n", type=int, default=1024)
     ap.add_argument("--imex-advection", default="muscl_rusanov", choices=["rusanov", "muscl_rusanov"])
     ap.add_argument("--imex-tableau", default="SSP222",
@@ -87,6 +90,9 @@ def main():
     log("device: " + name.value.decode())
     if args.imex_only:
         print(json.dumps({"imex": imex_leg(args)}), flush=True)
+        return
+    if args.imex_profile_only:
+        print(json.dumps({"imex_profile": imex_profile_leg(args)}), flush=True)
         return
     if args.amr_only:
         print(json.dumps({"amr": amr_leg(args)}), flush=True)
@@ -483,8 +489,7 @@ def imex_leg(args):
     """IMEX PEEK low-Mach Gresho vortex (M = 0.01, gamma 1.4, n^2 periodic):
     the pressure Helmholtz solve of successive steps to 1e-10 relative
     residual, plain multigrid (FP64 V-cycles, the reference's MLMG::solve
-    semantics; each iteration replayed as a captured graph, and "mg_eager"
-    with its kernels launched one by one) against FP64 flexible GMRES
+    semantics, kernels launched one by one) against FP64 flexible GMRES
     whose preconditioner (FP32 V-cycle, replayed as a captured graph) is
     followed by a learned correction of the V-cycle's residual ("corrector")
     or has the learned network as its fine-level smoother ("smoother"), and
@@ -596,11 +601,9 @@ def imex_leg(args):
     e_out.free()
     R.free()
     Rh.free()
-    # (label, solve method, network mode, graph replay: of each multigrid
-    # iteration for mg / mg_fused, of the preconditioner for FGMRES)
-    methods = [("mg_eager", "mg", None, False),
-               ("mg", "mg", None, True),
-               ("mg_fused", "mg_fused", None, True),
+    # (label, solve method, network mode, graph-replayed preconditioner)
+    methods = [("mg", "mg", None, False),
+               ("mg_fused", "mg_fused", None, False),
                ("fgmres_mg", "fgmres_mg", None, True),
                ("fgmres_cnn", "fgmres_cnn", "corrector", True),
                ("fgmres_cnn_smoother", "fgmres_cnn", "smoother", True)]
@@ -685,7 +688,7 @@ def imex_leg(args):
                       "iters_mean": float(np.mean(res[label]["iters"])),
                       "max_rel_resid": float(np.max(res[label]["rel_resid"])),
                       "fell_back": res[label]["fell_back"], "network_mode": mode,
-                      "graph_replay": graph}
+                      "graph_preconditioner": graph}
     out["cold_start"] = cold
     out["stage_ms_median"] = {k: round(float(np.median(v)), 4) for k, v in stage_ms.items()}
     # a whole PEEK step per solve method: the explicit work of its stages
@@ -693,7 +696,7 @@ def imex_leg(args):
     stage_total = sum(out["stage_ms_median"].values())
     out["peek_step_ms"] = {m[0]: round(stage_total + nst * out[m[0]]["s_per_solve"] * 1e3, 4)
                            for m in methods}
-    # against plain multigrid (graph-replayed iterations), against
+    # against plain multigrid (kernels launched one by one, as MLMG), against
     # the faster of the two plain multigrid variants, and against the same
     # FGMRES without the network; warm starts (the previous pressure) and
     # the cold start (zero guess) on the last system
@@ -712,6 +715,47 @@ def imex_leg(args):
     net.close()
     wl.free()
     return out
+
+
+def imex_profile_leg(args):
+    """A short IMEX workload whose kernels a profiler can attribute: the
+    Gresho vortex at --imex-n, --imex-steps steps of --imex-tableau with the
+    explicit stage work on the GPU, and every stage's pressure system solved
+    to 1e-10 by each method (mg, mg_fused, fgmres_mg, fgmres_cnn with an
+    untrained network; FGMRES preconditioners graph-replayed), warm-started
+    from the previous pressure; the state advances with the mg solution."""
+    from bamrex_amd import imex
+    n = args.imex_n
+    wl = imex.GreshoWorkload(n, advection=args.imex_advection, tableau=args.imex_tableau,
+                             ke_method=args.imex_ke)
+    solver = imex.HelmholtzSolver(n, n, wl.dx)
+    net = imex.LearnedSmoother(seed=1)
+    methods = ["mg", "mg_fused", "fgmres_mg", "fgmres_cnn"]
+    iters = {m: [] for m in methods}
+    for k in range(args.imex_steps):
+        dt = wl.step_dt()
+        done = False
+        while not done:
+            sy = wl.next_system(dt)
+            solver.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"],
+                              gamma=sy["gamma_c"], cX=sy["cX"], cY=sy["cY"], copy=False)
+            db = imex.dev(sy["rhs"])
+            p_mg = None
+            for meth in methods:
+                if meth == "fgmres_cnn":
+                    solver.attach(net, omega=1.0, mode="corrector")
+                dx_ = imex.dev(sy["p_prev"])
+                st = solver.solve(db, dx_, meth, rtol=1e-10, maxiter=200)
+                iters[meth].append(st["iters"])
+                if meth == "mg":
+                    p_mg = dx_
+            done = wl.advance(sy, p_mg)
+        log("imex profile: step %d done" % k)
+    solver.close()
+    net.close()
+    wl.free()
+    return {"workload": "gresho %d^2 %s, %d steps" % (n, args.imex_tableau, args.imex_steps),
+            "iters": iters}
 
 
 def host_cpu():

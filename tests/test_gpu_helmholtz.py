@@ -455,38 +455,6 @@ def test_graph_replay_matches_direct_launches():
     net.close()
 
 
-def test_mg_iteration_graph_matches_direct_launches():
-    """Multigrid solves whose iterations are replayed as captured graphs give
-    the iterate, iteration count and residual of the same iterations launched
-    kernel by kernel, for both V-cycle forms; a second solve from another
-    initial guess reuses the captured graph; the plain form equals the
-    oracle's multigrid."""
-    announce("helm mg iteration graph vs direct 128")
-    n = 128
-    systems, dx = gresho_systems(n, 1)
-    sy, p0 = systems[0]
-    s = HelmholtzSolver(n, n, dx)
-    s.set_coeffs(sy["alpha"], sy["beta"], sy["a"], sy["bX"], sy["bY"])
-    rng = np.random.default_rng(7)
-    starts = [p0, p0 + 1e-3 * rng.normal(size=p0.shape)]
-    for meth in ("mg", "mg_fused"):
-        out = {}
-        for graph in (False, True):
-            s.set_graph(graph)
-            for k, x0 in enumerate(starts):
-                db, dxv = dev(sy["rhs"]), dev(x0)
-                st = s.solve(db, dxv, meth, rtol=1e-10, maxiter=60)
-                out[(graph, k)] = (dxv.download(), st["iters"], st["rel_resid"])
-        for k in range(len(starts)):
-            assert np.array_equal(out[(False, k)][0], out[(True, k)][0]), (meth, k)
-            assert out[(False, k)][1:] == out[(True, k)][1:], (meth, k)
-        if meth == "mg":
-            xr, its, _ = ho.mg_solve(sy["rhs"], sy["a"], sy["bX"], sy["bY"], sy["alpha"],
-                                     sy["beta"], dx, dx, x0=p0, rtol=1e-10, maxiter=60)
-            assert its == out[(True, 0)][1] and np.array_equal(out[(True, 0)][0], xr)
-    s.close()
-
-
 def _reduction(s, sy, dx, fields):
     """mean ||r - A M r|| / ||r|| of one preconditioner application, FP64
     residual from the oracle."""

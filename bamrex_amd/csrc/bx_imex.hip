@@ -639,8 +639,7 @@ __device__ __forceinline__ unsigned long long block_speed_max(const double *__re
             m = bits > m ? bits : m;
         }
     }
-    // block maximum first: one atomic per block (per-wave atomics on one
-    // address serialise in L2)
+    // wave maxima, then the block's in thread 0
     __shared__ unsigned long long red[4];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
@@ -657,19 +656,8 @@ __device__ __forceinline__ unsigned long long block_speed_max(const double *__re
     return m;
 }
 
-template <bool ACOUSTIC>
-__global__ __launch_bounds__(256) void k_peek_speed(const double *__restrict__ U, long long n,
-                                                    double dx, double dy, double adia,
-                                                    double eps,
-                                                    unsigned long long *__restrict__ out)
-{
-    const unsigned long long m = block_speed_max<ACOUSTIC>(U, n, dx, dy, adia, eps);
-    if (threadIdx.x == 0 && m != 0ULL)
-        atomicMax(out, m);
-}
-
-// The same block maxima, one partial per block with no atomics: every
-// block writes its slot (0 when it saw no positive speed).
+// Block maxima, one partial per block with no atomics: every block writes
+// its slot (0 when it saw no positive speed).
 template <bool ACOUSTIC>
 __global__ __launch_bounds__(256) void k_peek_speed_part(const double *__restrict__ U,
                                                          long long n, double dx, double dy,
@@ -712,6 +700,32 @@ __global__ __launch_bounds__(256) void k_acoustic_dt_from_parts(
         dt_est = (q1 < dt_est) ? q1 : dt_est;
         dt_est *= cfl;
         *dt_out = dt_est;
+    }
+}
+
+// One block: out[0] = the maximum of the first nparts partials.
+__global__ __launch_bounds__(256) void k_max_of_parts(const unsigned long long *__restrict__ part,
+                                                      int nparts,
+                                                      unsigned long long *__restrict__ out)
+{
+    unsigned long long m = 0ULL;
+    for (int i = threadIdx.x; i < nparts; i += 256)
+        m = part[i] > m ? part[i] : m;
+    __shared__ unsigned long long red[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const unsigned long long o = __shfl_xor(m, off);
+        m                          = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (int i = 1; i < 4; ++i)
+            m = red[i] > m ? red[i] : m;
+        out[0] = m;
     }
 }
 
@@ -861,6 +875,12 @@ enum
     ACC_SPEED = 0,
     ACC_SHOCK = 1
 };
+// Accumulators of the speed and shock reductions, one set per device: word
+// 0 the result, words 1 .. SPEED_PARTS the per-block partial maxima (the
+// partials are reduced by one block instead of by atomics on one address,
+// which serialise when a thousand blocks finish together).
+constexpr int SPEED_PARTS = 1024;
+
 static int peek_acc(const char *who, int slot, unsigned long long **out)
 {
     static std::mutex          mu;
@@ -871,7 +891,8 @@ static int peek_acc(const char *who, int slot, unsigned long long **out)
         return bx_helm_fail(BX_ERR_UNSUPPORTED, std::string(who) + ": device id");
     std::lock_guard<std::mutex> lk(mu);
     if (!acc[dev][slot])
-        PEEK_CHECK(hipMalloc((void **)&acc[dev][slot], sizeof(unsigned long long)));
+        PEEK_CHECK(hipMalloc((void **)&acc[dev][slot],
+                             sizeof(unsigned long long) * (1 + SPEED_PARTS)));
     *out = acc[dev][slot];
     return BX_OK;
 }
@@ -898,11 +919,11 @@ extern "C" int bx_imex_advective_dt(const double *U, int nx, int ny, double dx, 
     int                 rc;
     if ((rc = peek_acc("bx_imex_advective_dt", ACC_SPEED, &d)) != BX_OK)
         return rc;
-    hipLaunchKernelGGL(k_peek_zero_u64, dim3(1), dim3(1), 0, s, d);
     const long long n    = (long long)nx * ny;
-    const int       grid = (int)std::min<long long>((n + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_peek_speed<false>, dim3(grid), dim3(256), 0, s, U, n, dx, dy, 0.0, 0.0,
-                       d);
+    const int       grid = (int)std::min<long long>((n + 255) / 256, SPEED_PARTS);
+    hipLaunchKernelGGL(k_peek_speed_part<false>, dim3(grid), dim3(256), 0, s, U, n, dx, dy, 0.0, 0.0,
+                       d + 1);
+    hipLaunchKernelGGL(k_max_of_parts, dim3(1), dim3(256), 0, s, d + 1, grid, d);
     PEEK_CHECK(hipGetLastError());
     unsigned long long bits = 0;
     PEEK_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, s));
@@ -1341,11 +1362,11 @@ extern "C" int bx_imex_acoustic_dt(const double *U, int nx, int ny, double dx, d
     int                 rc;
     if ((rc = peek_acc("bx_imex_acoustic_dt", ACC_SPEED, &d)) != BX_OK)
         return rc;
-    hipLaunchKernelGGL(k_peek_zero_u64, dim3(1), dim3(1), 0, s, d);
     const long long n    = (long long)nx * ny;
-    const int       grid = (int)std::min<long long>((n + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_peek_speed<true>, dim3(grid), dim3(256), 0, s, U, n, dx, dy, gamma, eps,
-                       d);
+    const int       grid = (int)std::min<long long>((n + 255) / 256, SPEED_PARTS);
+    hipLaunchKernelGGL(k_peek_speed_part<true>, dim3(grid), dim3(256), 0, s, U, n, dx, dy, gamma, eps,
+                       d + 1);
+    hipLaunchKernelGGL(k_max_of_parts, dim3(1), dim3(256), 0, s, d + 1, grid, d);
     PEEK_CHECK(hipGetLastError());
     unsigned long long bits = 0;
     PEEK_CHECK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, s));

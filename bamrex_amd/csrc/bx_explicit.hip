@@ -161,36 +161,71 @@ struct XItem
     int       nx, nxd;
 };
 
-template <int D>
-__device__ __forceinline__ void x_item(const DevBox *__restrict__ boxes,
-                                       const long long *__restrict__ ibeg,
-                                       int nbox, long long item, int ng,
-                                       int lane, int step, int &b,
-                                       long long &bend, XItem &it)
+// Walk over a block's consecutive x-sweep items.  The box geometry and the
+// item's (plane, chunk) position are carried from one item to the next, so
+// the divisions that locate an item run only when the walk enters a box
+// (the first item of the block's range, or a box boundary); within a box
+// the next item is the next chunk of the plane, or the first chunk of the
+// next plane.  Every value is uniform over the block.
+struct XCursor
 {
-    if (item >= bend)
+    long long bend;                   // first item past the current box
+    long long base, kstride, jstride; // box geometry
+    long long nst, P;
+    int       nx, nxd, cpp, kpl, chunk;
+};
+
+template <int D>
+__device__ __forceinline__ void x_enter(const DevBox *__restrict__ boxes,
+                                        const long long *__restrict__ ibeg, int nbox,
+                                        long long item, int ng, int step, XCursor &q)
+{
+    const int     b  = find_box(ibeg, nbox, item);
+    const DevBox &bx = boxes[b];
+    q.bend           = ibeg[b + 1];
+    q.nx             = bx.hi[0] - bx.lo[0] + 1;
+    const int ny     = (D >= 2) ? bx.hi[1] - bx.lo[1] + 1 : 1;
+    q.nxd            = q.nx + 2 * ng;
+    q.P              = (long long)ny * q.nxd;
+    q.cpp            = (int)((q.P + step - 1) / step);
+    q.base           = bx.base;
+    q.kstride        = bx.kstride;
+    q.jstride        = bx.jstride;
+    q.nst            = bx.nstride;
+    const unsigned local = (unsigned)(item - ibeg[b]);
+    q.kpl                = (int)(local / (unsigned)q.cpp);
+    q.chunk              = (int)(local - (unsigned)q.kpl * (unsigned)q.cpp);
+}
+
+// item: the item the cursor moves to (the one after its current item)
+template <int D>
+__device__ __forceinline__ void x_advance(const DevBox *__restrict__ boxes,
+                                          const long long *__restrict__ ibeg, int nbox,
+                                          long long item, int ng, int step, XCursor &q)
+{
+    if (item >= q.bend)
     {
-        b    = find_box(ibeg, nbox, item);
-        bend = ibeg[b + 1];
+        x_enter<D>(boxes, ibeg, nbox, item, ng, step, q);
+        return;
     }
-    const DevBox &bx  = boxes[b];
-    const int     nx  = bx.hi[0] - bx.lo[0] + 1;
-    const int     ny  = (D >= 2) ? bx.hi[1] - bx.lo[1] + 1 : 1;
-    const int     nxd = nx + 2 * ng;
-    const long long P = (long long)ny * nxd;
-    const int       cpp   = (int)((P + step - 1) / step);
-    const unsigned  local = (unsigned)(item - ibeg[b]);
-    const int       kpl   = (int)(local / (unsigned)cpp);
-    const int       chunk = (int)(local - (unsigned)kpl * (unsigned)cpp);
-    const int       ngy   = (D >= 2) ? ng : 0;
-    const int       ngz   = (D >= 3) ? ng : 0;
-    it.pbase = bx.base + (long long)(kpl + ngz) * bx.kstride
-               + (long long)ngy * bx.jstride;
-    it.p   = (long long)chunk * step - 2 + lane;
-    it.P   = P;
-    it.nst = bx.nstride;
-    it.nx  = nx;
-    it.nxd = nxd;
+    if (++q.chunk == q.cpp)
+    {
+        q.chunk = 0;
+        ++q.kpl;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void x_item(const XCursor &q, int ng, int lane, int step, XItem &it)
+{
+    const int ngy = (D >= 2) ? ng : 0;
+    const int ngz = (D >= 3) ? ng : 0;
+    it.pbase      = q.base + (long long)(q.kpl + ngz) * q.kstride + (long long)ngy * q.jstride;
+    it.p          = (long long)q.chunk * step - 2 + lane;
+    it.P          = q.P;
+    it.nst        = q.nst;
+    it.nx         = q.nx;
+    it.nxd        = q.nxd;
 }
 
 // PM (physics mode) bits: PM_NI -- the HLLC exponents take the near-integer
@@ -229,8 +264,7 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     long long first, last;
     block_item_range(nitems, first, last);
     unsigned long long vmax = 0ULL;
-    int                b    = -1;
-    long long          bend = -1;
+    XCursor            q;
     XItem              cur, nxt;
     double             Un_[NC]; // prefetched state of the next item
     auto issue = [&](const XItem &it, double (&V)[NC]) {
@@ -241,7 +275,8 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     };
     if (first < last)
     {
-        x_item<D>(boxes, ibeg, nbox, first, ng, t, step, b, bend, nxt);
+        x_enter<D>(boxes, ibeg, nbox, first, ng, step, q);
+        x_item<D>(q, ng, t, step, nxt);
         issue(nxt, Un_);
     }
     // Every thread of the block runs the same items, so the barriers below
@@ -256,7 +291,8 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
             U[n] = Un_[n];
         if (item + 1 < last)
         {
-            x_item<D>(boxes, ibeg, nbox, item + 1, ng, t, step, b, bend, nxt);
+            x_advance<D>(boxes, ibeg, nbox, item + 1, ng, step, q);
+            x_item<D>(q, ng, t, step, nxt);
             issue(nxt, Un_);
         }
 #pragma unroll

@@ -317,6 +317,9 @@ __global__ __launch_bounds__(256) void k_resid_restrict(
 constexpr int TAIL_MAX_LEVELS = 8;
 constexpr int TAIL_LDS_BYTES  = 160 * 1024;
 constexpr int TAIL_THREADS    = 1024;
+// cells of one colour per thread on a tail level (FP64: fewer, so that the
+// unrolled half-sweep stays within 128 VGPRs)
+template <class T> constexpr int tail_maxk() { return sizeof(T) <= 4 ? 4 : 2; }
 
 template <class T> struct TailLevel
 {
@@ -384,16 +387,42 @@ __device__ void tail_smooth(const LdsLevel<T> &L, T alpha, T beta, T gam, int pa
     }
     // the n / 2 cells of colour rb only (nx is even): cell k of the colour
     // is row j = k / (nx / 2), column 2 (k mod nx / 2) + ((j + rb) & 1), so
-    // every lane of a wave has a cell to update
-    const int hx2 = L.nx >> 1, h = n >> 1;
+    // every lane of a wave has a cell to update.  A thread's cells
+    // k = tid + TAIL_THREADS m (at most tail_maxk, checked when the tail is
+    // planned) are located once per level; the half-sweeps only pick the
+    // column of their colour.  The tail runs on one CU, so its time is the
+    // instruction count per cell: the update takes 32-bit LDS indices.
+    constexpr int MK  = tail_maxk<T>();
+    const int     hx2 = L.nx >> 1, h = n >> 1;
+    int           jr[MK], ie[MK];
+#pragma unroll
+    for (int m = 0; m < MK; ++m)
+    {
+        const int k = threadIdx.x + TAIL_THREADS * m;
+        jr[m]       = k < h ? k / hx2 : -1;
+        ie[m]       = 2 * (k - jr[m] * hx2);
+    }
+    const int nx = L.nx, ny = L.ny;
     for (int it = 0; it < pairs; ++it)
         for (int rb = 0; rb < 2; ++rb)
         {
-            for (int k = threadIdx.x; k < h; k += TAIL_THREADS)
+#pragma unroll
+            for (int m = 0; m < MK; ++m)
             {
-                const int j = k / hx2, i = 2 * (k - j * hx2) + ((j + rb) & 1);
-                gsrb_at(L.x, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, alpha, dhx, dhy,
-                        cdhx, cdhy, i, j);
+                const int j = jr[m];
+                if (j < 0)
+                    break;
+                const int i  = ie[m] + ((j + rb) & 1);
+                const int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+                const int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+                const int c  = j * nx + i;
+                const T   bxl = L.bX[j * (nx + 1) + i], bxr = L.bX[j * (nx + 1) + i + 1];
+                const T   byl = L.bY[c], byr = L.bY[c + nx];
+                L.x[c] = gsrb_value(L.rhs[c], L.a[c], bxl, bxr, byl, byr,
+                                    L.cX ? L.cX[c] : T(0), L.cY ? L.cY[c] : T(0),
+                                    L.cX != nullptr, L.x[j * nx + im], L.x[j * nx + ip],
+                                    L.x[jm * nx + i], L.x[jp * nx + i], L.x[c], alpha, dhx,
+                                    dhy, cdhx, cdhy);
             }
             __syncthreads();
         }
@@ -1213,7 +1242,9 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
                    * (long long)sizeof(T);
         };
         while (l0 > 0 && nl - (l0 - 1) <= TAIL_MAX_LEVELS
-               && used + need(mg.lev[l0 - 1]) <= tail_bytes)
+               && used + need(mg.lev[l0 - 1]) <= tail_bytes
+               && (long long)mg.lev[l0 - 1].nx * mg.lev[l0 - 1].ny
+                      <= 2LL * tail_maxk<T>() * TAIL_THREADS)
             used += need(mg.lev[--l0]);
         if (l0 < nl)
         {

@@ -185,7 +185,8 @@ __global__ __launch_bounds__(256) void k_prolong_add(const T *__restrict__ c,
 // same neighbour values, so the result is identical to 2*pairs launches of
 // k_gsrb.  Input and output are different arrays: other workgroups of the
 // same launch still read the halo of a tile after it has been written.  With `coarse` the prolongation phi += P e_c is applied at load
-// (the post-smoothing of a V-cycle).  Coefficients are read from global
+// (the post-smoothing of a V-cycle); with `accumulate` the result is added
+// to phi_out instead of stored (a multigrid iteration's x += e).  Coefficients are read from global
 // memory (read-only, cache-resident across the half-sweeps).
 // Tiles: 64 x 16 on large levels; 32 x 8 where 64 x 16 tiles would leave
 // most CUs idle (a small level is latency-bound: more, shorter workgroups).
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
     const T *__restrict__ a,
     const T *__restrict__ bX, const T *__restrict__ bY, const T *__restrict__ cX,
     const T *__restrict__ cY, int nx, int ny, const T *__restrict__ sc, T hx, T hy,
-    int pairs, const T *__restrict__ coarse)
+    int pairs, const T *__restrict__ coarse, int accumulate)
 {
     // each thread owns the region cells tid + NT m; their coefficients stay
     // in registers across the half-sweeps, phi lives in LDS.  Every global
@@ -275,7 +276,14 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
         const int tj = c / TX, ti = c - tj * TX;
         const int gi = blockIdx.x * TX + ti, gj = blockIdx.y * TY + tj;
         if (gi < nx && gj < ny)
-            phi_out[(long long)gj * nx + gi] = ph[(tj + NH) * RW + ti + NH];
+        {
+            const long long g = (long long)gj * nx + gi;
+            const T         v = ph[(tj + NH) * RW + ti + NH];
+            // accumulate: phi_out += 1 * v, the bits of k_axpy adding the
+            // correction to an iterate (each cell is read and written by
+            // its own thread only)
+            phi_out[g] = accumulate ? phi_out[g] + T(1) * v : v;
+        }
     }
 }
 
@@ -772,6 +780,46 @@ __global__ __launch_bounds__(256) void k_multidot(
             partial[(long long)(v0 + threadIdx.x) * gridDim.x + blockIdx.x] = s;
         }
         __syncthreads();
+    }
+}
+
+// r = rhs - A x (k_apply's residual) and the partial sums of r^2 with the
+// grid, the per-thread order and the reduction tree of k_multidot(r, r, k =
+// 1), so k_reduce_partials gives the bits of norm2's sum: the residual and
+// its norm in one pass.  Launched with nb_dot blocks of 256 threads; the
+// level has at most 2^31 - 1 cells.
+__global__ __launch_bounds__(256) void k_residual_sumsq(
+    const double *__restrict__ x, double *__restrict__ r, const double *__restrict__ rhs,
+    const double *__restrict__ a, const double *__restrict__ bX,
+    const double *__restrict__ bY, const double *__restrict__ cX,
+    const double *__restrict__ cY, int nx, int ny, const double *__restrict__ sc,
+    double dxinv, double dyinv, double *__restrict__ partial)
+{
+    __shared__ double sh[4];
+    const int    lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int    n    = nx * ny;
+    const double al = sc[0], be = sc[1], ga = sc[2];
+    double       acc = 0;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256)
+    {
+        const int    j  = c / nx, i = c - j * nx;
+        const double ri = rhs[c] - apply_at(x, a, bX, bY, cX, cY, nx, ny, al, be, ga, dxinv,
+                                            dyinv, i, j);
+        r[c] = ri;
+        acc += ri * ri;
+    }
+    double s = acc;
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off);
+    if (lane == 0)
+        sh[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double t = 0;
+        for (int q = 0; q < 4; ++q)
+            t += sh[q];
+        partial[blockIdx.x] = t;
     }
 }
 
@@ -1314,7 +1362,7 @@ static void launch_smooth(const MGLevel<T> &L, const MGHierarchy<T> &mg,
 // when the level has fewer than one 64 x 16 tile per CU of an MI355X (256).
 template <class T>
 static void launch_tiled(const MGLevel<T> &L, const MGHierarchy<T> &mg, const T *in, T *out,
-                         int pairs, const T *coarse, hipStream_t s)
+                         int pairs, const T *coarse, hipStream_t s, int accumulate = 0)
 {
     const long long big = (long long)((L.nx + SM_TX - 1) / SM_TX) * ((L.ny + SM_TY - 1) / SM_TY);
     if (big >= 256)
@@ -1322,17 +1370,22 @@ static void launch_tiled(const MGLevel<T> &L, const MGHierarchy<T> &mg, const T 
                            dim3((unsigned)((L.nx + SM_TX - 1) / SM_TX),
                                 (unsigned)((L.ny + SM_TY - 1) / SM_TY)),
                            dim3(512), 0, s, in, out, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx,
-                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse);
+                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate);
     else
         hipLaunchKernelGGL((k_gsrb_tiled<T, SM_TX_SMALL, SM_TY_SMALL, 256>),
                            dim3((unsigned)((L.nx + SM_TX_SMALL - 1) / SM_TX_SMALL),
                                 (unsigned)((L.ny + SM_TY_SMALL - 1) / SM_TY_SMALL)),
                            dim3(256), 0, s, in, out, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx,
-                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse);
+                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate);
 }
 
+// One V-cycle from level l.  acc (level 0 only): when the blocked path
+// runs, its last smoothing adds the correction to acc (acc += e, as k_axpy)
+// instead of storing it, and the call returns true; otherwise the
+// correction is in mg.lev[l].x and the call returns false.
 template <class T>
-static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
+static bool vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false,
+                   T *acc = nullptr)
 {
     MGLevel<T> &L = mg.lev[l];
     if (mg.tail_l0 >= 0 && l == mg.tail_l0)
@@ -1340,7 +1393,7 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
         hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(TAIL_THREADS), mg.tail_lds, s,
                            (const TailArgs<T> *)mg.d_tail, (int)mg.lev.size() - l, mg.nu1,
                            mg.nu2, mg.nu_bottom, mg.d_sc);
-        return;
+        return false;
     }
     if (l + 1 == (int)mg.lev.size())
     {
@@ -1350,14 +1403,14 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
             hipLaunchKernelGGL((k_bottom_gsrb<T>), dim3(1), dim3(256), 0, s, L.x, L.rhs,
                                L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.hx,
                                L.hy, mg.nu_bottom);
-            return;
+            return false;
         }
         for (int it = 0; it < mg.nu_bottom; ++it)
         {
             launch_smooth(L, mg, L.x, L.rhs, 0, s);
             launch_smooth(L, mg, L.x, L.rhs, 1, s);
         }
-        return;
+        return false;
     }
     MGLevel<T> &C = mg.lev[l + 1];
     if (fused && mg.nu2 > 0 && 2 * mg.nu1 <= SM_MAXH && 2 * mg.nu2 <= SM_MAXH
@@ -1377,8 +1430,13 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
                            L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.dxinv,
                            L.dyinv, C.rhs, C.x);
         vcycle(mg, l + 1, s, fused);
+        if (acc)
+        {
+            launch_tiled(L, mg, xs, acc, mg.nu2, (const T *)C.x, s, 1);
+            return true;
+        }
         launch_tiled(L, mg, xs, L.x, mg.nu2, (const T *)C.x, s);
-        return;
+        return false;
     }
     for (int it = 0; it < mg.nu1; ++it)
     {
@@ -1398,6 +1456,7 @@ static void vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false)
         launch_smooth(L, mg, L.x, L.rhs, 0, s);
         launch_smooth(L, mg, L.x, L.rhs, 1, s);
     }
+    return false;
 }
 
 // coarse-grid part of the V-cycle only (levels >= 1), from L.res on level 0
@@ -1475,6 +1534,8 @@ extern "C" int bx_helm_create(int nx, int ny, double dx, double dy,
     // extent makes the two cells across the seam the same colour)
     if (nx % 2 || ny % 2)
         return herr(BX_ERR_UNSUPPORTED, "bx_helm_create: nx and ny must be even");
+    if ((long long)nx * ny > 2147483647LL)
+        return herr(BX_ERR_UNSUPPORTED, "bx_helm_create: more than 2^31 - 1 cells");
     bx_helm *h = new bx_helm();
     h->nx = nx;
     h->ny = ny;
@@ -1814,50 +1875,70 @@ static int precond_replay(bx_helm *h, const double *rv, double *e, hipStream_t s
     return BX_OK;
 }
 
-// One multigrid iteration on the device: x += V(r) with r = h->r, then
-// r = b - A x and the partial sums of ||r||^2 reduced into dots[0] (the
-// kernels of norm2 without its read-back).
-static void mg_iteration(bx_helm *h, const double *b, double *x, bool fused, hipStream_t s)
+// b - A x into the finest level's right-hand side (the V-cycle only reads
+// it), with the partial sums of its squares reduced into dots[0] (the bits
+// of residual64 + norm2's kernels).
+static void mg_residual(bx_helm *h, const double *b, const double *x, hipStream_t s)
 {
     MGHierarchy<double> &mg = h->mg64;
     MGLevel<double>     &L0 = mg.lev[0];
-    const long long      n  = h->n;
-    hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, L0.rhs, h->r, n, 1.0);
-    hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s, L0.x, n, 0.0);
-    vcycle(mg, 0, s, fused);
-    hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, s, x, L0.x, n, 1.0);
-    residual64(h, b, x, h->r, s);
-    hipLaunchKernelGGL(k_multidot, dim3(h->nb_dot), dim3(256), 0, s, h->r, h->r, n, 1,
+    hipLaunchKernelGGL(k_residual_sumsq, dim3(h->nb_dot), dim3(256), 0, s, x, L0.rhs, b, L0.a,
+                       L0.bX, L0.bY, L0.cX, L0.cY, L0.nx, L0.ny, mg.d_sc, L0.dxinv, L0.dyinv,
                        h->partial);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, h->partial, h->nb_dot, 1,
                        h->dots);
 }
 
+// One multigrid iteration on the device: x += V(r) with r in the finest
+// level's right-hand side, then the new residual there and its squared
+// norm in dots[0].
+static void mg_iteration(bx_helm *h, const double *b, double *x, bool fused, hipStream_t s)
+{
+    MGHierarchy<double> &mg = h->mg64;
+    MGLevel<double>     &L0 = mg.lev[0];
+    const long long      n  = h->n;
+    hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s, L0.x, n, 0.0);
+    if (!vcycle(mg, 0, s, fused, x))
+        hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, s, x, L0.x, n, 1.0);
+    mg_residual(h, b, x, s);
+}
+
+static int read_rnorm(bx_helm *h, hipStream_t s, double *rn)
+{
+    H_CHECK(hipGetLastError());
+    H_CHECK(hipMemcpyAsync(h->hstat, h->dots, sizeof(double), hipMemcpyDeviceToHost, s));
+    H_CHECK(hipStreamSynchronize(s));
+    *rn = std::sqrt(h->hstat[0]);
+    return BX_OK;
+}
+
 // Plain multigrid solve (MLMG::solve semantics, FP64): x <- x + V(b - A x)
 // until ||b - A x||_2 <= rtol ||b||_2.  The iterations are launched kernel by
 // kernel: at 1024^2 the GPU is the bottleneck, and a captured iteration
-// (which needs copies of b and x into fixed buffers) measured slower.
+// (which needs copies of b and x into fixed buffers) measured slower.  The
+// residual lives in the finest level's right-hand side.
 // rn0 >= 0: h->r already holds b - A x, of norm rn0.
 static int solve_mg(bx_helm *h, const double *b, double *x, double rtol,
                     int maxiter, double bnorm, bx_solve_stats *st,
                     hipStream_t s, bool fused = false, double rn0 = -1)
 {
-    double rn = rn0;
-    int    rc;
+    MGLevel<double> &L0 = h->mg64.lev[0];
+    double           rn = rn0;
+    int              rc;
     if (rn0 < 0)
     {
-        residual64(h, b, x, h->r, s);
-        if ((rc = norm2(h, h->r, s, &rn)))
+        mg_residual(h, b, x, s);
+        if ((rc = read_rnorm(h, s, &rn)))
             return rc;
     }
+    else if (rn > rtol * bnorm && maxiter > 0)
+        hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, L0.rhs, h->r, h->n, 1.0);
     int it = 0;
     while (rn > rtol * bnorm && it < maxiter)
     {
         mg_iteration(h, b, x, fused, s);
-        H_CHECK(hipGetLastError());
-        H_CHECK(hipMemcpyAsync(h->hstat, h->dots, sizeof(double), hipMemcpyDeviceToHost, s));
-        H_CHECK(hipStreamSynchronize(s));
-        rn = std::sqrt(h->hstat[0]);
+        if ((rc = read_rnorm(h, s, &rn)))
+            return rc;
         ++it;
         ++st->precond_applies;
     }

@@ -186,7 +186,9 @@ __global__ __launch_bounds__(256) void k_prolong_add(const T *__restrict__ c,
 // k_gsrb.  Input and output are different arrays: other workgroups of the
 // same launch still read the halo of a tile after it has been written.  With `coarse` the prolongation phi += P e_c is applied at load
 // (the post-smoothing of a V-cycle); with `accumulate` the result is added
-// to phi_out instead of stored (a multigrid iteration's x += e).  Coefficients are read from global
+// to phi_out instead of stored (a multigrid iteration's x += e); with
+// `zero_in` phi is taken as zero without being read (a V-cycle's initial
+// guess).  Coefficients are read from global
 // memory (read-only, cache-resident across the half-sweeps).
 // Tiles: 64 x 16 on large levels; 32 x 8 where 64 x 16 tiles would leave
 // most CUs idle (a small level is latency-bound: more, shorter workgroups).
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
     const T *__restrict__ a,
     const T *__restrict__ bX, const T *__restrict__ bY, const T *__restrict__ cX,
     const T *__restrict__ cY, int nx, int ny, const T *__restrict__ sc, T hx, T hy,
-    int pairs, const T *__restrict__ coarse, int accumulate)
+    int pairs, const T *__restrict__ coarse, int accumulate, int zero_in)
 {
     // each thread owns the region cells tid + NT m; their coefficients stay
     // in registers across the half-sweeps, phi lives in LDS.  Every global
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
         const int lj = c / RW, li = c - lj * RW;
         const int gi = wrapx(x0 + li), gj = wrapy(y0 + lj);
         const long long g = (long long)gj * nx + gi;
-        r_ph[m] = phi[g];
+        r_ph[m] = zero_in ? T(0) : phi[g];
         if (coarse)
             r_co[m] = coarse[(long long)(gj >> 1) * (nx >> 1) + (gi >> 1)];
         // distance from the region edge; only interior cells are updated
@@ -1362,7 +1364,8 @@ static void launch_smooth(const MGLevel<T> &L, const MGHierarchy<T> &mg,
 // when the level has fewer than one 64 x 16 tile per CU of an MI355X (256).
 template <class T>
 static void launch_tiled(const MGLevel<T> &L, const MGHierarchy<T> &mg, const T *in, T *out,
-                         int pairs, const T *coarse, hipStream_t s, int accumulate = 0)
+                         int pairs, const T *coarse, hipStream_t s, int accumulate = 0,
+                         int zero_in = 0)
 {
     const long long big = (long long)((L.nx + SM_TX - 1) / SM_TX) * ((L.ny + SM_TY - 1) / SM_TY);
     if (big >= 256)
@@ -1370,32 +1373,42 @@ static void launch_tiled(const MGLevel<T> &L, const MGHierarchy<T> &mg, const T 
                            dim3((unsigned)((L.nx + SM_TX - 1) / SM_TX),
                                 (unsigned)((L.ny + SM_TY - 1) / SM_TY)),
                            dim3(512), 0, s, in, out, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx,
-                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate);
+                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate, zero_in);
     else
         hipLaunchKernelGGL((k_gsrb_tiled<T, SM_TX_SMALL, SM_TY_SMALL, 256>),
                            dim3((unsigned)((L.nx + SM_TX_SMALL - 1) / SM_TX_SMALL),
                                 (unsigned)((L.ny + SM_TY_SMALL - 1) / SM_TY_SMALL)),
                            dim3(256), 0, s, in, out, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx,
-                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate);
+                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate, zero_in);
 }
 
 // One V-cycle from level l.  acc (level 0 only): when the blocked path
 // runs, its last smoothing adds the correction to acc (acc += e, as k_axpy)
 // instead of storing it, and the call returns true; otherwise the
-// correction is in mg.lev[l].x and the call returns false.
+// correction is in mg.lev[l].x and the call returns false.  x_zero: the
+// cycle starts from a zero guess on level l (the blocked pre-smoothing
+// takes it as zero without reading it; any other path zeroes the array
+// first).
 template <class T>
 static bool vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false,
-                   T *acc = nullptr)
+                   T *acc = nullptr, bool x_zero = false)
 {
-    MGLevel<T> &L = mg.lev[l];
-    if (mg.tail_l0 >= 0 && l == mg.tail_l0)
+    MGLevel<T> &L       = mg.lev[l];
+    const bool  tail    = mg.tail_l0 >= 0 && l == mg.tail_l0;
+    const bool  bottom  = l + 1 == (int)mg.lev.size();
+    const bool  blocked = !tail && !bottom && fused && mg.nu2 > 0 && 2 * mg.nu1 <= SM_MAXH
+                         && 2 * mg.nu2 <= SM_MAXH && L.nx % 2 == 0 && L.ny % 2 == 0;
+    if (x_zero && !(blocked && mg.nu1 > 0))
+        hipLaunchKernelGGL((k_fill<T>), dim3(2048), dim3(256), 0, s, L.x,
+                           (long long)L.nx * L.ny, (T)0);
+    if (tail)
     {
         hipLaunchKernelGGL((k_vcycle_tail<T>), dim3(1), dim3(TAIL_THREADS), mg.tail_lds, s,
                            (const TailArgs<T> *)mg.d_tail, (int)mg.lev.size() - l, mg.nu1,
                            mg.nu2, mg.nu_bottom, mg.d_sc);
         return false;
     }
-    if (l + 1 == (int)mg.lev.size())
+    if (bottom)
     {
         if (L.nx * L.ny <= BOTTOM_MAX && (L.nx + 1) * L.ny <= BOTTOM_MAX + 64
             && L.nx * (L.ny + 1) <= BOTTOM_MAX + 64)
@@ -1413,8 +1426,7 @@ static bool vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false,
         return false;
     }
     MGLevel<T> &C = mg.lev[l + 1];
-    if (fused && mg.nu2 > 0 && 2 * mg.nu1 <= SM_MAXH && 2 * mg.nu2 <= SM_MAXH
-        && L.nx % 2 == 0 && L.ny % 2 == 0)
+    if (blocked)
     {
         // three launches per level: blocked pre-smoothing, residual +
         // restriction + zeroing, prolongation + blocked post-smoothing
@@ -1423,7 +1435,7 @@ static bool vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false,
         const T *xs = L.x;
         if (mg.nu1 > 0)
         {
-            launch_tiled(L, mg, L.x, L.res, mg.nu1, (const T *)nullptr, s);
+            launch_tiled(L, mg, L.x, L.res, mg.nu1, (const T *)nullptr, s, 0, x_zero ? 1 : 0);
             xs = L.res;
         }
         hipLaunchKernelGGL((k_resid_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s, xs,
@@ -1782,9 +1794,7 @@ static int precond_vcycle(bx_helm *h, const double *rv, double *e, hipStream_t s
         MGHierarchy<float> &mg = h->mg32;
         hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0, s,
                            rv, mg.lev[0].rhs, n, 1.0);
-        hipLaunchKernelGGL((k_fill<float>), dim3(2048), dim3(256), 0, s,
-                           mg.lev[0].x, n, 0.f);
-        vcycle(mg, 0, s, h->fused != 0);
+        vcycle(mg, 0, s, h->fused != 0, (float *)nullptr, true);
         hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0, s,
                            mg.lev[0].x, e, n, 1.0f);
     }
@@ -1793,9 +1803,7 @@ static int precond_vcycle(bx_helm *h, const double *rv, double *e, hipStream_t s
         MGHierarchy<double> &mg = h->mg64;
         hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, mg.lev[0].rhs,
                            rv, n, 1.0);
-        hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s,
-                           mg.lev[0].x, n, 0.0);
-        vcycle(mg, 0, s, h->fused != 0);
+        vcycle(mg, 0, s, h->fused != 0, (double *)nullptr, true);
         hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, e,
                            mg.lev[0].x, n, 1.0);
     }
@@ -1897,8 +1905,7 @@ static void mg_iteration(bx_helm *h, const double *b, double *x, bool fused, hip
     MGHierarchy<double> &mg = h->mg64;
     MGLevel<double>     &L0 = mg.lev[0];
     const long long      n  = h->n;
-    hipLaunchKernelGGL((k_fill<double>), dim3(2048), dim3(256), 0, s, L0.x, n, 0.0);
-    if (!vcycle(mg, 0, s, fused, x))
+    if (!vcycle(mg, 0, s, fused, x, true))
         hipLaunchKernelGGL(k_axpy, dim3(2048), dim3(256), 0, s, x, L0.x, n, 1.0);
     mg_residual(h, b, x, s);
 }

@@ -38,6 +38,24 @@ namespace bx {
 // Cell operations shared by the grid kernels and the single-workgroup tail
 // of the V-cycle, so both give the same bits.
 
+// (A x) of one cell from its value, its four neighbours' and its
+// coefficients (mlalapgrad_adotx); dhx = beta dxinv dxinv, dhy likewise
+template <class T>
+__device__ __forceinline__ T apply_vals(T xc, T xl, T xr, T xd, T xu, T a_c, T bxl, T bxr,
+                                        T byl, T byr, T cx, T cy, bool with_c, T alpha,
+                                        T dhx, T dhy, T gamma, T dxinv, T dyinv)
+{
+    T v = alpha * a_c * xc - dhx * (bxr * (xr - xc) - bxl * (xc - xl))
+          - dhy * (byr * (xu - xc) - byl * (xc - xd));
+    if (with_c)
+    {
+        const T chdx = gamma * cx * T(0.5) * dxinv;
+        const T chdy = gamma * cy * T(0.5) * dyinv;
+        v = v + chdx * (xr - xl) + chdy * (xu - xd);
+    }
+    return v;
+}
+
 // (A x)_ij on a periodic level (mlalapgrad_adotx)
 template <class T>
 __device__ __forceinline__ T apply_at(const T *__restrict__ x, const T *__restrict__ a,
@@ -59,15 +77,8 @@ __device__ __forceinline__ T apply_at(const T *__restrict__ x, const T *__restri
     const T byr = bY[(long long)(j + 1) * nx + i];
     const T xl = x[(long long)j * nx + im], xr = x[(long long)j * nx + ip];
     const T xd = x[(long long)jm * nx + i], xu = x[(long long)jp * nx + i];
-    T v = alpha * a[c] * xc - dhx * (bxr * (xr - xc) - bxl * (xc - xl))
-          - dhy * (byr * (xu - xc) - byl * (xc - xd));
-    if (cX)
-    {
-        const T chdx = gamma * cX[c] * T(0.5) * dxinv;
-        const T chdy = gamma * cY[c] * T(0.5) * dyinv;
-        v = v + chdx * (xr - xl) + chdy * (xu - xd);
-    }
-    return v;
+    return apply_vals(xc, xl, xr, xd, xu, a[c], bxl, bxr, byl, byr, cX ? cX[c] : T(0),
+                      cY ? cY[c] : T(0), cX != nullptr, alpha, dhx, dhy, gamma, dxinv, dyinv);
 }
 
 // red/black Gauss-Seidel update of cell (i, j) (alapgrad_gsrb; periodic
@@ -201,18 +212,25 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
     const T *__restrict__ a,
     const T *__restrict__ bX, const T *__restrict__ bY, const T *__restrict__ cX,
     const T *__restrict__ cY, int nx, int ny, const T *__restrict__ sc, T hx, T hy,
-    int pairs, const T *__restrict__ coarse, int accumulate, int zero_in)
+    int pairs, const T *__restrict__ coarse, int accumulate, int zero_in,
+    T *__restrict__ crhs, T *__restrict__ cxz, T dxinv, T dyinv)
 {
     // each thread owns the region cells tid + NT m; their coefficients stay
     // in registers across the half-sweeps, phi lives in LDS.  Every global
     // load of the region is issued before the first LDS write (phi and the
     // coarse correction land in registers first), so a thread waits for
-    // memory once instead of once per region cell.
+    // memory once instead of once per region cell.  With crhs the region
+    // has one more halo cell, so that the smoothed values are exact on the
+    // tile's neighbours too, and the block also writes the coarse
+    // right-hand side R (rhs - A x) of its tile and zeroes the coarse
+    // iterate (the residuals and averaging order of k_resid_restrict).
     constexpr int MAXC = ((TY + 2 * SM_MAXH) * (TX + 2 * SM_MAXH) + NT - 1) / NT;
     __shared__ T ph[(TY + 2 * SM_MAXH) * (TX + 2 * SM_MAXH)];
+    __shared__ T rt[TX * TY];
     const int NH = 2 * pairs;
-    const int RW = TX + 2 * NH, RH = TY + 2 * NH;
-    const int x0 = blockIdx.x * TX - NH, y0 = blockIdx.y * TY - NH;
+    const int HR = crhs ? NH + 1 : NH; // region halo
+    const int RW = TX + 2 * HR, RH = TY + 2 * HR;
+    const int x0 = blockIdx.x * TX - HR, y0 = blockIdx.y * TY - HR;
     const T alpha = sc[0];
     const T dhx = sc[1] / (hx * hx), dhy = sc[1] / (hy * hy);
     const T cdhx = sc[2] / (2 * hx), cdhy = sc[2] / (2 * hy);
@@ -273,6 +291,38 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
         }
         __syncthreads();
     }
+    if (crhs)
+    {
+        // residuals of the tile's cells; the coefficients are read again
+        // (from cache) rather than kept live past the half-sweeps
+        const T beta = sc[1], gamma = sc[2];
+        const T ahx = beta * dxinv * dxinv, ahy = beta * dyinv * dyinv;
+        for (int c = threadIdx.x; c < TX * TY; c += NT)
+        {
+            const int tj = c / TX, ti = c - tj * TX;
+            const int gi = wrapx(blockIdx.x * TX + ti), gj = wrapy(blockIdx.y * TY + tj);
+            const long long g  = (long long)gj * nx + gi;
+            const int       lc = (tj + HR) * RW + ti + HR;
+            const long long fx = (long long)gj * (nx + 1) + gi;
+            rt[c] = rhs[g]
+                    - apply_vals(ph[lc], ph[lc - 1], ph[lc + 1], ph[lc - RW], ph[lc + RW], a[g],
+                                 bX[fx], bX[fx + 1], bY[g], bY[g + nx], cX ? cX[g] : T(0),
+                                 cY ? cY[g] : T(0), cX != nullptr, alpha, ahx, ahy, gamma,
+                                 dxinv, dyinv);
+        }
+        __syncthreads();
+        const int ncx = nx >> 1;
+        for (int c = threadIdx.x; c < (TX / 2) * (TY / 2); c += NT)
+        {
+            const int cj = c / (TX / 2), ci = c - cj * (TX / 2);
+            const int I = blockIdx.x * (TX / 2) + ci, J = blockIdx.y * (TY / 2) + cj;
+            if (2 * I >= nx || 2 * J >= ny)
+                continue;
+            const T *q = rt + (2 * cj) * TX + 2 * ci;
+            crhs[(long long)J * ncx + I] = T(0.25) * (q[0] + q[1] + q[TX] + q[TX + 1]);
+            cxz[(long long)J * ncx + I]  = T(0);
+        }
+    }
     for (int c = threadIdx.x; c < TX * TY; c += NT)
     {
         const int tj = c / TX, ti = c - tj * TX;
@@ -280,7 +330,7 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
         if (gi < nx && gj < ny)
         {
             const long long g = (long long)gj * nx + gi;
-            const T         v = ph[(tj + NH) * RW + ti + NH];
+            const T         v = ph[(tj + HR) * RW + ti + HR];
             // accumulate: phi_out += 1 * v, the bits of k_axpy adding the
             // correction to an iterate (each cell is read and written by
             // its own thread only)
@@ -1365,7 +1415,7 @@ static void launch_smooth(const MGLevel<T> &L, const MGHierarchy<T> &mg,
 template <class T>
 static void launch_tiled(const MGLevel<T> &L, const MGHierarchy<T> &mg, const T *in, T *out,
                          int pairs, const T *coarse, hipStream_t s, int accumulate = 0,
-                         int zero_in = 0)
+                         int zero_in = 0, T *crhs = nullptr, T *cxz = nullptr)
 {
     const long long big = (long long)((L.nx + SM_TX - 1) / SM_TX) * ((L.ny + SM_TY - 1) / SM_TY);
     if (big >= 256)
@@ -1373,13 +1423,15 @@ static void launch_tiled(const MGLevel<T> &L, const MGHierarchy<T> &mg, const T 
                            dim3((unsigned)((L.nx + SM_TX - 1) / SM_TX),
                                 (unsigned)((L.ny + SM_TY - 1) / SM_TY)),
                            dim3(512), 0, s, in, out, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx,
-                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate, zero_in);
+                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate, zero_in, crhs, cxz, L.dxinv,
+                           L.dyinv);
     else
         hipLaunchKernelGGL((k_gsrb_tiled<T, SM_TX_SMALL, SM_TY_SMALL, 256>),
                            dim3((unsigned)((L.nx + SM_TX_SMALL - 1) / SM_TX_SMALL),
                                 (unsigned)((L.ny + SM_TY_SMALL - 1) / SM_TY_SMALL)),
                            dim3(256), 0, s, in, out, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx,
-                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate, zero_in);
+                           L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate, zero_in, crhs, cxz, L.dxinv,
+                           L.dyinv);
 }
 
 // One V-cycle from level l.  acc (level 0 only): when the blocked path
@@ -1428,19 +1480,31 @@ static bool vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false,
     MGLevel<T> &C = mg.lev[l + 1];
     if (blocked)
     {
-        // three launches per level: blocked pre-smoothing, residual +
-        // restriction + zeroing, prolongation + blocked post-smoothing
+        // two launches per level: blocked pre-smoothing with the residual's
+        // restriction and the coarse zeroing, prolongation + blocked
+        // post-smoothing
         // pre-smoothing x -> res, post-smoothing res + P e_c -> x (the res
         // array is free in this path: the residual goes straight to C.rhs)
         const T *xs = L.x;
-        if (mg.nu1 > 0)
+        if (mg.nu1 > 0 && 2 * mg.nu1 + 1 <= SM_MAXH)
         {
-            launch_tiled(L, mg, L.x, L.res, mg.nu1, (const T *)nullptr, s, 0, x_zero ? 1 : 0);
+            // pre-smoothing with the residual's restriction in the same launch
+            launch_tiled(L, mg, L.x, L.res, mg.nu1, (const T *)nullptr, s, 0, x_zero ? 1 : 0,
+                         C.rhs, C.x);
             xs = L.res;
         }
-        hipLaunchKernelGGL((k_resid_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s, xs,
-                           L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc, L.dxinv,
-                           L.dyinv, C.rhs, C.x);
+        else
+        {
+            if (mg.nu1 > 0)
+            {
+                launch_tiled(L, mg, L.x, L.res, mg.nu1, (const T *)nullptr, s, 0,
+                             x_zero ? 1 : 0);
+                xs = L.res;
+            }
+            hipLaunchKernelGGL((k_resid_restrict<T>), grid2(C.nx, C.ny), dim3(256), 0, s, xs,
+                               L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx, L.ny, mg.d_sc,
+                               L.dxinv, L.dyinv, C.rhs, C.x);
+        }
         vcycle(mg, l + 1, s, fused);
         if (acc)
         {

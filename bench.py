@@ -706,6 +706,7 @@ def imex_leg(args):
     out["speedup_cnn_vs_fgmres_mg"] = (out["fgmres_mg"]["s_per_solve"]
                                        / out["fgmres_cnn"]["s_per_solve"])
     out["speedup_fgmres_mg_vs_mg"] = out["mg"]["s_per_solve"] / out["fgmres_mg"]["s_per_solve"]
+    out["speedup_mg_fused_vs_mg"] = out["mg"]["s_per_solve"] / out["mg_fused"]["s_per_solve"]
     if cold:
         cbest = min(cold["mg"]["s_per_solve"], cold["mg_fused"]["s_per_solve"])
         out["cold_speedup_cnn_vs_best_mg"] = cbest / cold["fgmres_cnn"]["s_per_solve"]

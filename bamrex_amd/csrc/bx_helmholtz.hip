@@ -1236,15 +1236,24 @@ struct bx_helm
     double *tmp1 = nullptr, *tmp2 = nullptr;
     float  *fr = nullptr, *fe = nullptr;
     bool    coeffs_set = false;
-    // preconditioner replayed as a captured graph (FGMRES): input pin,
-    // output pout; recaptured when anything it bakes in changes
-    // (slot 0: plain V-cycle, slot 1: with the network)
-    int             use_graph  = 1;
-    int             fused      = 1; // blocked-smoother V-cycle in the FGMRES preconditioner
-    hipStream_t     cap_stream = nullptr;
-    hipGraphExec_t  pgraph[2]  = { nullptr, nullptr };
-    long long       gen = 0, pgraph_gen[2] = { -1, -1 }, pgraph_epoch[2] = { -1, -1 };
-    double         *pin = nullptr, *pout = nullptr;
+    // preconditioner replayed as captured graphs (FGMRES), one per
+    // (network attached or not, input vector, output vector): FGMRES applies
+    // it to its basis vectors V_j into Z_j, a bounded set, so the graphs
+    // read and write those vectors directly; an entry is recaptured when
+    // anything it bakes in changes (gen, the network's weights epoch)
+    struct PGraph
+    {
+        int            slot;
+        const double  *in;
+        double        *out;
+        hipGraphExec_t exec;
+        long long      gen, epoch;
+    };
+    int                 use_graph  = 1;
+    int                 fused      = 1; // blocked-smoother V-cycle in the FGMRES preconditioner
+    hipStream_t         cap_stream = nullptr;
+    std::vector<PGraph> pgraphs;
+    long long           gen = 0;
 };
 
 static thread_local std::string g_helm_err;
@@ -1627,7 +1636,6 @@ extern "C" int bx_helm_create(int nx, int ny, double dx, double dy,
         || (rc = halloc(&h->fr, h->n)) || (rc = halloc(&h->fe, h->n))
         || (rc = halloc(&h->partial, (long long)h->nb_dot * (h->m_max + 1)))
         || (rc = halloc(&h->dots, 2 * (h->m_max + 1) + 2)) || (rc = halloc(&h->hdev, h->m_max + 1))
-        || (rc = halloc(&h->pin, h->n)) || (rc = halloc(&h->pout, h->n))
         || (rc = halloc(&h->arn, arnoldi_doubles(h->m_max))))
     {
         bx_helm_destroy(h);
@@ -1649,16 +1657,16 @@ extern "C" void bx_helm_destroy(bx_helm *h)
         return;
     h->mg64.free_all();
     h->mg32.free_all();
-    for (hipGraphExec_t g : h->pgraph)
-        if (g)
-            (void)hipGraphExecDestroy(g);
+    for (const auto &pg : h->pgraphs)
+        if (pg.exec)
+            (void)hipGraphExecDestroy(pg.exec);
     if (h->cap_stream)
         (void)hipStreamDestroy(h->cap_stream);
     for (void *p : { (void *)h->a, (void *)h->bX, (void *)h->bY, (void *)h->cX,
                      (void *)h->cY, (void *)h->V, (void *)h->Z, (void *)h->w,
                      (void *)h->r, (void *)h->partial, (void *)h->dots,
                      (void *)h->hdev, (void *)h->tmp1, (void *)h->tmp2,
-                     (void *)h->fr, (void *)h->fe, (void *)h->pin, (void *)h->pout,
+                     (void *)h->fr, (void *)h->fe,
                      (void *)h->arn })
         if (p)
             (void)hipFree(p);
@@ -1894,20 +1902,28 @@ extern "C" int bx_helm_attach_cnn(bx_helm *h, bx_cnn *cnn, double omega, int mod
     return BX_OK;
 }
 
-// e = M r through the captured graph: r -> pin, replay, pout -> e
+// e = M r through the captured graph for (r, e)
 static int precond_replay(bx_helm *h, const double *rv, double *e, hipStream_t s)
 {
     if (!h->use_graph)
         return precond(h, rv, e, s);
     const int       slot  = h->cnn ? 1 : 0;
     const long long epoch = h->cnn ? bx_cnn_epoch(h->cnn) : 0;
-    hipGraphExec_t &gx    = h->pgraph[slot];
-    if (!gx || h->pgraph_gen[slot] != h->gen || h->pgraph_epoch[slot] != epoch)
+    bx_helm::PGraph *pg   = nullptr;
+    for (auto &q : h->pgraphs)
+        if (q.slot == slot && q.in == rv && q.out == e)
+            pg = &q;
+    if (!pg)
     {
-        if (gx)
+        h->pgraphs.push_back({ slot, rv, e, nullptr, -1, -1 });
+        pg = &h->pgraphs.back();
+    }
+    if (!pg->exec || pg->gen != h->gen || pg->epoch != epoch)
+    {
+        if (pg->exec)
         {
-            H_CHECK(hipGraphExecDestroy(gx));
-            gx = nullptr;
+            H_CHECK(hipGraphExecDestroy(pg->exec));
+            pg->exec = nullptr;
         }
         if (!h->cap_stream)
             H_CHECK(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
@@ -1921,7 +1937,7 @@ static int precond_replay(bx_helm *h, const double *rv, double *e, hipStream_t s
         H_CHECK(hipStreamSynchronize(s));
         hipGraph_t g = nullptr;
         H_CHECK(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
-        int rc = precond(h, h->pin, h->pout, h->cap_stream);
+        int rc = precond(h, rv, e, h->cap_stream);
         hipError_t ec = hipStreamEndCapture(h->cap_stream, &g);
         if (rc)
         {
@@ -1931,19 +1947,17 @@ static int precond_replay(bx_helm *h, const double *rv, double *e, hipStream_t s
         }
         if (ec != hipSuccess)
             return herr(BX_ERR_HIP, std::string("precond capture: ") + hipGetErrorString(ec));
-        ec = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+        ec = hipGraphInstantiate(&pg->exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ec != hipSuccess)
         {
-            gx = nullptr;
+            pg->exec = nullptr;
             return herr(BX_ERR_HIP, std::string("precond instantiate: ") + hipGetErrorString(ec));
         }
-        h->pgraph_gen[slot]   = h->gen;
-        h->pgraph_epoch[slot] = epoch;
+        pg->gen   = h->gen;
+        pg->epoch = epoch;
     }
-    H_CHECK(hipMemcpyAsync(h->pin, rv, sizeof(double) * h->n, hipMemcpyDeviceToDevice, s));
-    H_CHECK(hipGraphLaunch(gx, s));
-    H_CHECK(hipMemcpyAsync(e, h->pout, sizeof(double) * h->n, hipMemcpyDeviceToDevice, s));
+    H_CHECK(hipGraphLaunch(pg->exec, s));
     return BX_OK;
 }
 

@@ -1219,7 +1219,11 @@ struct bx_helm
     double *a = nullptr, *bX = nullptr, *bY = nullptr, *cX = nullptr,
            *cY = nullptr;
     MGHierarchy<double> mg64;
+    // the FP32 hierarchy of the FGMRES preconditioner, built from the
+    // current coefficients when a preconditioner first needs it after
+    // bx_helm_set_coeffs (multigrid solves never do)
     MGHierarchy<float>  mg32;
+    bool                mg32_stale   = true;
     int                 precond_fp32 = 1;
     // FGMRES workspace (restart length m <= m_max, the allocated basis)
     int     m = 20, m_max = 20;
@@ -1705,13 +1709,29 @@ extern "C" int bx_helm_set_coeffs(bx_helm *h, double alpha, double beta,
     }
     int  rc;
     bool realloc = false;
-    if ((rc = build_hierarchy(h, h->mg64, s, &realloc))
-        || (rc = build_hierarchy(h, h->mg32, s, &realloc)))
+    if ((rc = build_hierarchy(h, h->mg64, s, &realloc)))
         return rc;
     if (realloc)
         ++h->gen;
+    h->mg32_stale = true;
     H_CHECK(hipStreamSynchronize(s));
     h->coeffs_set = true;
+    return BX_OK;
+}
+
+// Builds the FP32 hierarchy if a preconditioner will read it and it is
+// older than the coefficients.  Never called inside a graph capture.
+static int ensure_mg32(bx_helm *h, hipStream_t s)
+{
+    if (!h->mg32_stale || !h->precond_fp32)
+        return BX_OK;
+    bool realloc = false;
+    int  rc;
+    if ((rc = build_hierarchy(h, h->mg32, s, &realloc)))
+        return rc;
+    if (realloc)
+        ++h->gen;
+    h->mg32_stale = false;
     return BX_OK;
 }
 
@@ -1888,6 +1908,9 @@ extern "C" int bx_helm_precond(bx_helm *h, const double *r, double *e,
 {
     if (!h || !h->coeffs_set || !r || !e)
         return herr(BX_ERR_STATE, "bx_helm_precond: coefficients not set");
+    int rc;
+    if ((rc = ensure_mg32(h, (hipStream_t)stream)))
+        return rc;
     return precond(h, r, e, (hipStream_t)stream);
 }
 
@@ -2221,6 +2244,8 @@ extern "C" int bx_helm_solve(bx_helm *h, const double *b, double *x, int method,
     }
     else
     {
+        if ((rc = ensure_mg32(h, s)))
+            return rc;
         bx_cnn *saved = h->cnn;
         if (method == BX_SOLVE_FGMRES_MG)
             h->cnn = nullptr;

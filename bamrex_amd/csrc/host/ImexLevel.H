@@ -26,7 +26,7 @@ struct IMEXSettings
     int         ke_method       = BX_IMEX_KE_EX;
     std::string butcher_tableau;
     double      acoustic_timestep_end_time = 0;
-    int         solver   = BX_SOLVE_MG;
+    int         solver   = BX_SOLVE_MG_FUSED;
     double      rtol     = 1e-12;
     int         maxiter  = 1000;
 };

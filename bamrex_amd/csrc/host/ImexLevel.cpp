@@ -67,7 +67,9 @@ IMEXSettings get_imex_settings(double stop_time)
                                  "bx_imex_shock_indicator / bx_imex_stab_blend provide it");
     s.acoustic_timestep_end_time = 0.1 * stop_time;
     pp.query("acoustic_timestep_end_time", s.acoustic_timestep_end_time);
-    std::string solver = "mg";
+    // default: the blocked-smoother V-cycle, whose iterates equal the
+    // per-colour one's bit for bit
+    std::string solver = "mg_fused";
     pp.query("pressure_solver", solver);
     if (solver == "mg")
         s.solver = BX_SOLVE_MG;

@@ -199,7 +199,10 @@ __global__ __launch_bounds__(256) void k_prolong_add(const T *__restrict__ c,
 // (the post-smoothing of a V-cycle); with `accumulate` the result is added
 // to phi_out instead of stored (a multigrid iteration's x += e); with
 // `zero_in` phi is taken as zero without being read (a V-cycle's initial
-// guess).  Coefficients are read from global
+// guess); with rhs_in the right-hand side is read from that FP64 array and
+// with out64 the result is written there in FP64 instead of to phi_out (a
+// preconditioner's input residual and output correction, the conversions
+// of k_convert / k_scale).  Coefficients are read from global
 // memory (read-only, cache-resident across the half-sweeps).
 // Tiles: 64 x 16 on large levels; 32 x 8 where 64 x 16 tiles would leave
 // most CUs idle (a small level is latency-bound: more, shorter workgroups).
@@ -213,7 +216,8 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
     const T *__restrict__ bX, const T *__restrict__ bY, const T *__restrict__ cX,
     const T *__restrict__ cY, int nx, int ny, const T *__restrict__ sc, T hx, T hy,
     int pairs, const T *__restrict__ coarse, int accumulate, int zero_in,
-    T *__restrict__ crhs, T *__restrict__ cxz, T dxinv, T dyinv)
+    T *__restrict__ crhs, T *__restrict__ cxz, T dxinv, T dyinv,
+    const double *__restrict__ rhs_in, double *__restrict__ out64)
 {
     // each thread owns the region cells tid + NT m; their coefficients stay
     // in registers across the half-sweeps, phi lives in LDS.  Every global
@@ -262,7 +266,7 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
         r_par[m]    = (gi + gj) & 1;
         if (d == 0)
             continue;
-        r_rhs[m] = rhs[g];
+        r_rhs[m] = rhs_in ? (T)rhs_in[g] : rhs[g];
         r_a[m]   = a[g];
         __builtin_memcpy(&r_bx[m], bX + (long long)gj * (nx + 1) + gi, sizeof(T2));
         r_byl[m] = bY[(long long)gj * nx + gi];
@@ -304,7 +308,7 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
             const long long g  = (long long)gj * nx + gi;
             const int       lc = (tj + HR) * RW + ti + HR;
             const long long fx = (long long)gj * (nx + 1) + gi;
-            rt[c] = rhs[g]
+            rt[c] = (rhs_in ? (T)rhs_in[g] : rhs[g])
                     - apply_vals(ph[lc], ph[lc - 1], ph[lc + 1], ph[lc - RW], ph[lc + RW], a[g],
                                  bX[fx], bX[fx + 1], bY[g], bY[g + nx], cX ? cX[g] : T(0),
                                  cY ? cY[g] : T(0), cX != nullptr, alpha, ahx, ahy, gamma,
@@ -334,7 +338,10 @@ __global__ __launch_bounds__(NT) void k_gsrb_tiled(
             // accumulate: phi_out += 1 * v, the bits of k_axpy adding the
             // correction to an iterate (each cell is read and written by
             // its own thread only)
-            phi_out[g] = accumulate ? phi_out[g] + T(1) * v : v;
+            if (out64)
+                out64[g] = (double)v;
+            else
+                phi_out[g] = accumulate ? phi_out[g] + T(1) * v : v;
         }
     }
 }
@@ -1428,7 +1435,8 @@ static void launch_smooth(const MGLevel<T> &L, const MGHierarchy<T> &mg,
 template <class T>
 static void launch_tiled(const MGLevel<T> &L, const MGHierarchy<T> &mg, const T *in, T *out,
                          int pairs, const T *coarse, hipStream_t s, int accumulate = 0,
-                         int zero_in = 0, T *crhs = nullptr, T *cxz = nullptr)
+                         int zero_in = 0, T *crhs = nullptr, T *cxz = nullptr,
+                         const double *rhs_in = nullptr, double *out64 = nullptr)
 {
     const long long big = (long long)((L.nx + SM_TX - 1) / SM_TX) * ((L.ny + SM_TY - 1) / SM_TY);
     if (big >= 256)
@@ -1437,14 +1445,14 @@ static void launch_tiled(const MGLevel<T> &L, const MGHierarchy<T> &mg, const T 
                                 (unsigned)((L.ny + SM_TY - 1) / SM_TY)),
                            dim3(512), 0, s, in, out, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx,
                            L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate, zero_in, crhs, cxz, L.dxinv,
-                           L.dyinv);
+                           L.dyinv, rhs_in, out64);
     else
         hipLaunchKernelGGL((k_gsrb_tiled<T, SM_TX_SMALL, SM_TY_SMALL, 256>),
                            dim3((unsigned)((L.nx + SM_TX_SMALL - 1) / SM_TX_SMALL),
                                 (unsigned)((L.ny + SM_TY_SMALL - 1) / SM_TY_SMALL)),
                            dim3(256), 0, s, in, out, L.rhs, L.a, L.bX, L.bY, L.cX, L.cY, L.nx,
                            L.ny, mg.d_sc, L.hx, L.hy, pairs, coarse, accumulate, zero_in, crhs, cxz, L.dxinv,
-                           L.dyinv);
+                           L.dyinv, rhs_in, out64);
 }
 
 // One V-cycle from level l.  acc (level 0 only): when the blocked path
@@ -1454,15 +1462,37 @@ static void launch_tiled(const MGLevel<T> &L, const MGHierarchy<T> &mg, const T 
 // cycle starts from a zero guess on level l (the blocked pre-smoothing
 // takes it as zero without reading it; any other path zeroes the array
 // first).
+// Level l takes the blocked path of vcycle().
+template <class T>
+static bool blocked_level(const MGHierarchy<T> &mg, int l, bool fused)
+{
+    const MGLevel<T> &L      = mg.lev[l];
+    const bool        tail   = mg.tail_l0 >= 0 && l == mg.tail_l0;
+    const bool        bottom = l + 1 == (int)mg.lev.size();
+    return !tail && !bottom && fused && mg.nu2 > 0 && 2 * mg.nu1 <= SM_MAXH
+           && 2 * mg.nu2 <= SM_MAXH && L.nx % 2 == 0 && L.ny % 2 == 0;
+}
+
+// Level l's blocked path also restricts the residual in its pre-smoothing
+// launch, so no other launch reads the level's right-hand side array.
+template <class T>
+static bool direct_io_level(const MGHierarchy<T> &mg, int l, bool fused)
+{
+    return blocked_level(mg, l, fused) && mg.nu1 > 0 && 2 * mg.nu1 + 1 <= SM_MAXH;
+}
+
+// rhs_in / out64 (only where direct_io_level holds): level l's right-hand
+// side is read from rhs_in (FP64) and the correction written to out64
+// (FP64) by the smoothing launches themselves.
 template <class T>
 static bool vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false,
-                   T *acc = nullptr, bool x_zero = false)
+                   T *acc = nullptr, bool x_zero = false, const double *rhs_in = nullptr,
+                   double *out64 = nullptr)
 {
     MGLevel<T> &L       = mg.lev[l];
     const bool  tail    = mg.tail_l0 >= 0 && l == mg.tail_l0;
     const bool  bottom  = l + 1 == (int)mg.lev.size();
-    const bool  blocked = !tail && !bottom && fused && mg.nu2 > 0 && 2 * mg.nu1 <= SM_MAXH
-                         && 2 * mg.nu2 <= SM_MAXH && L.nx % 2 == 0 && L.ny % 2 == 0;
+    const bool  blocked = blocked_level(mg, l, fused);
     if (x_zero && !(blocked && mg.nu1 > 0))
         hipLaunchKernelGGL((k_fill<T>), dim3(2048), dim3(256), 0, s, L.x,
                            (long long)L.nx * L.ny, (T)0);
@@ -1503,7 +1533,7 @@ static bool vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false,
         {
             // pre-smoothing with the residual's restriction in the same launch
             launch_tiled(L, mg, L.x, L.res, mg.nu1, (const T *)nullptr, s, 0, x_zero ? 1 : 0,
-                         C.rhs, C.x);
+                         C.rhs, C.x, rhs_in);
             xs = L.res;
         }
         else
@@ -1521,10 +1551,12 @@ static bool vcycle(MGHierarchy<T> &mg, int l, hipStream_t s, bool fused = false,
         vcycle(mg, l + 1, s, fused);
         if (acc)
         {
-            launch_tiled(L, mg, xs, acc, mg.nu2, (const T *)C.x, s, 1);
+            launch_tiled(L, mg, xs, acc, mg.nu2, (const T *)C.x, s, 1, 0, (T *)nullptr,
+                         (T *)nullptr, rhs_in);
             return true;
         }
-        launch_tiled(L, mg, xs, L.x, mg.nu2, (const T *)C.x, s);
+        launch_tiled(L, mg, xs, L.x, mg.nu2, (const T *)C.x, s, 0, 0, (T *)nullptr,
+                     (T *)nullptr, rhs_in, out64);
         return false;
     }
     for (int it = 0; it < mg.nu1; ++it)
@@ -1881,23 +1913,35 @@ static int precond(bx_helm *h, const double *rv, double *e, hipStream_t s)
 static int precond_vcycle(bx_helm *h, const double *rv, double *e, hipStream_t s)
 {
     const long long n = h->n;
+    // where the finest level's smoothing launches can read r and write e
+    // themselves, no conversion launches
     if (h->precond_fp32)
     {
         MGHierarchy<float> &mg = h->mg32;
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0, s,
-                           rv, mg.lev[0].rhs, n, 1.0);
-        vcycle(mg, 0, s, h->fused != 0, (float *)nullptr, true);
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0, s,
-                           mg.lev[0].x, e, n, 1.0f);
+        if (direct_io_level(mg, 0, h->fused != 0))
+            vcycle(mg, 0, s, true, (float *)nullptr, true, rv, e);
+        else
+        {
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(2048), dim3(256), 0, s,
+                               rv, mg.lev[0].rhs, n, 1.0);
+            vcycle(mg, 0, s, h->fused != 0, (float *)nullptr, true);
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(2048), dim3(256), 0, s,
+                               mg.lev[0].x, e, n, 1.0f);
+        }
     }
     else
     {
         MGHierarchy<double> &mg = h->mg64;
-        hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, mg.lev[0].rhs,
-                           rv, n, 1.0);
-        vcycle(mg, 0, s, h->fused != 0, (double *)nullptr, true);
-        hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, e,
-                           mg.lev[0].x, n, 1.0);
+        if (direct_io_level(mg, 0, h->fused != 0))
+            vcycle(mg, 0, s, true, (double *)nullptr, true, rv, e);
+        else
+        {
+            hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, mg.lev[0].rhs,
+                               rv, n, 1.0);
+            vcycle(mg, 0, s, h->fused != 0, (double *)nullptr, true);
+            hipLaunchKernelGGL((k_scale), dim3(2048), dim3(256), 0, s, e,
+                               mg.lev[0].x, n, 1.0);
+        }
     }
     H_CHECK(hipGetLastError());
     return BX_OK;

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bamrex_hip.h"
@@ -616,16 +617,12 @@ __global__ void k_avg_faces_x(const T *__restrict__ f, T *__restrict__ c,
 // average (k_restrict), bX / bY by face average (k_avg_faces_x / _y); the
 // same expressions, so the same bits.  Grid (ncx + 1) x (ncy + 1).
 template <class T>
-__global__ __launch_bounds__(256) void k_coarsen_coeffs(
+__device__ __forceinline__ void coarsen_at(
     const T *__restrict__ fa, const T *__restrict__ fbX, const T *__restrict__ fbY,
     const T *__restrict__ fcX, const T *__restrict__ fcY, T *__restrict__ ca,
     T *__restrict__ cbX, T *__restrict__ cbY, T *__restrict__ ccX, T *__restrict__ ccY,
-    int ncx, int ncy)
+    int ncx, int ncy, int I, int J)
 {
-    const int I = blockIdx.x * blockDim.x + threadIdx.x; // 0..ncx
-    const int J = blockIdx.y;                            // 0..ncy
-    if (I > ncx)
-        return;
     const int nfx = 2 * ncx;
     if (J < ncy)
     {
@@ -646,6 +643,53 @@ __global__ __launch_bounds__(256) void k_coarsen_coeffs(
         cbY[(long long)J * ncx + I] = T(0.5)
                                       * (fbY[(long long)(2 * J) * nfx + 2 * I]
                                          + fbY[(long long)(2 * J) * nfx + 2 * I + 1]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_coarsen_coeffs(
+    const T *__restrict__ fa, const T *__restrict__ fbX, const T *__restrict__ fbY,
+    const T *__restrict__ fcX, const T *__restrict__ fcY, T *__restrict__ ca,
+    T *__restrict__ cbX, T *__restrict__ cbY, T *__restrict__ ccX, T *__restrict__ ccY,
+    int ncx, int ncy)
+{
+    const int I = blockIdx.x * blockDim.x + threadIdx.x; // 0..ncx
+    const int J = blockIdx.y;                            // 0..ncy
+    if (I <= ncx)
+        coarsen_at(fa, fbX, fbY, fcX, fcY, ca, cbX, cbY, ccX, ccY, ncx, ncy, I, J);
+}
+
+// The small coarse levels of a hierarchy, finest first, in one workgroup
+// (a barrier between levels): the same expressions as k_coarsen_coeffs.
+constexpr int COARSEN_MAX_LEVELS = 12;
+constexpr int COARSEN_SMALL      = 64 * 64; // coarse levels up to this many cells
+template <class T> struct CoarsenLevel
+{
+    const T *fa, *fbX, *fbY, *fcX, *fcY;
+    T       *ca, *cbX, *cbY, *ccX, *ccY;
+    int      ncx, ncy;
+};
+template <class T> struct CoarsenList
+{
+    int             n;
+    CoarsenLevel<T> lev[COARSEN_MAX_LEVELS];
+};
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_coarsen_small(CoarsenList<T> cl)
+{
+    for (int l = 0; l < cl.n; ++l)
+    {
+        const CoarsenLevel<T> &c  = cl.lev[l];
+        const int              w  = c.ncx + 1;
+        const int              nq = w * (c.ncy + 1);
+        for (int q = threadIdx.x; q < nq; q += 1024)
+        {
+            const int J = q / w, I = q - J * w;
+            coarsen_at(c.fa, c.fbX, c.fbY, c.fcX, c.fcY, c.ca, c.cbX, c.cbY, c.ccX, c.ccY,
+                       c.ncx, c.ncy, I, J);
+        }
+        __syncthreads();
+    }
 }
 
 template <class T>
@@ -738,15 +782,23 @@ __global__ __launch_bounds__(256) void k_convert(const TI *__restrict__ in,
         out[i] = (TO)(in[i] * scale);
 }
 
-// Level-0 coefficients of a hierarchy from the FP64 fine arrays, one launch:
+// Level-0 coefficients of a hierarchy from FP64 arrays, and its scalars
+// (alpha, beta, gamma) into device memory, one launch:
 // a, c over nx*ny cells, bX over (nx+1)*ny faces, bY over nx*(ny+1).
 template <class T>
 __global__ __launch_bounds__(256) void k_convert_coeffs(
     const double *__restrict__ a, const double *__restrict__ bX,
     const double *__restrict__ bY, const double *__restrict__ cX,
     const double *__restrict__ cY, T *__restrict__ oa, T *__restrict__ obX,
-    T *__restrict__ obY, T *__restrict__ ocX, T *__restrict__ ocY, int nx, int ny)
+    T *__restrict__ obY, T *__restrict__ ocX, T *__restrict__ ocY, int nx, int ny,
+    T *__restrict__ sc, T alpha, T beta, T gamma)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        sc[0] = alpha;
+        sc[1] = beta;
+        sc[2] = gamma;
+    }
     const long long n   = (long long)nx * ny;
     const long long nbx = (long long)(nx + 1) * ny;
     const long long nby = (long long)nx * (ny + 1);
@@ -1193,13 +1245,24 @@ template <class T> struct MGHierarchy
     int                     tail_lds   = 0;       // its LDS bytes
     TailArgs<T>            *d_tail     = nullptr;
     int                     nu1 = 2, nu2 = 2, nu_bottom = 16;
+    // the finest level's coefficient arrays are the solver's own FP64
+    // copies (FP64 hierarchy): not freed here
+    bool                    alias0 = false;
 
     void free_all()
     {
-        for (auto &l : lev)
-            for (T *p : { l.a, l.bX, l.bY, l.cX, l.cY, l.x, l.rhs, l.res })
+        for (size_t l = 0; l < lev.size(); ++l)
+        {
+            const MGLevel<T> &L = lev[l];
+            if (!(l == 0 && alias0))
+                for (T *p : { L.a, L.bX, L.bY, L.cX, L.cY })
+                    if (p)
+                        (void)hipFree(p);
+            for (T *p : { L.x, L.rhs, L.res })
                 if (p)
                     (void)hipFree(p);
+        }
+        alias0 = false;
         lev.clear();
         if (d_sc)
             (void)hipFree(d_sc);
@@ -1292,9 +1355,13 @@ template <class T> static int halloc(T **p, long long n)
     return BX_OK;
 }
 
+// src*: the FP64 finest-level coefficients the hierarchy is built from
+// (copied into, or converted to, its level 0)
 template <class T>
 static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
-                           bool *reallocated)
+                           bool *reallocated, const double *src_a, const double *src_bX,
+                           const double *src_bY, const double *src_cX,
+                           const double *src_cY)
 {
     mg.alpha      = (T)h->alpha;
     mg.beta       = (T)h->beta;
@@ -1315,7 +1382,9 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
             return rc;
         mg.with_c = h->has_c;
     }
-    H_CHECK(hipMemcpyAsync(mg.d_sc, mg.host_sc, sizeof(T) * 3, hipMemcpyHostToDevice, s));
+    // the FP64 hierarchy's finest level is the solver's own FP64 copy of the
+    // coefficients
+    constexpr bool alias = std::is_same<T, double>::value;
     int nx = h->nx, ny = h->ny;
     double dx = h->dx, dy = h->dy;
     while (!keep)
@@ -1328,11 +1397,24 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
         L.dxinv = (T)(1.0 / dx);
         L.dyinv = (T)(1.0 / dy);
         const long long nc = (long long)nx * ny;
-        if ((rc = halloc(&L.a, nc)) || (rc = halloc(&L.bX, (long long)(nx + 1) * ny))
-            || (rc = halloc(&L.bY, (long long)nx * (ny + 1))) || (rc = halloc(&L.x, nc))
-            || (rc = halloc(&L.rhs, nc)) || (rc = halloc(&L.res, nc)))
-            return rc;
-        if (h->has_c && ((rc = halloc(&L.cX, nc)) || (rc = halloc(&L.cY, nc))))
+        if (alias && mg.lev.empty())
+        {
+            L.a  = (T *)h->a;
+            L.bX = (T *)h->bX;
+            L.bY = (T *)h->bY;
+            L.cX = h->has_c ? (T *)h->cX : nullptr;
+            L.cY = h->has_c ? (T *)h->cY : nullptr;
+            mg.alias0 = true;
+        }
+        else
+        {
+            if ((rc = halloc(&L.a, nc)) || (rc = halloc(&L.bX, (long long)(nx + 1) * ny))
+                || (rc = halloc(&L.bY, (long long)nx * (ny + 1))))
+                return rc;
+            if (h->has_c && ((rc = halloc(&L.cX, nc)) || (rc = halloc(&L.cY, nc))))
+                return rc;
+        }
+        if ((rc = halloc(&L.x, nc)) || (rc = halloc(&L.rhs, nc)) || (rc = halloc(&L.res, nc)))
             return rc;
         mg.lev.push_back(L);
         // coarsen only to even extents: on a periodic level of odd extent
@@ -1393,20 +1475,35 @@ static int build_hierarchy(bx_helm *h, MGHierarchy<T> &mg, hipStream_t s,
             mg.tail_lds = (int)used;
         }
     }
-    // level 0 coefficients from the FP64 fine arrays, then one averaging
-    // launch per coarse level
+    // level 0 and the scalars in one launch, then one averaging launch per
+    // large coarse level and one for all the small ones (the set-up is
+    // bound by launches, not bytes)
     MGLevel<T> &L0 = mg.lev[0];
-    hipLaunchKernelGGL((k_convert_coeffs<T>), dim3(1024), dim3(256), 0, s, h->a, h->bX,
-                       h->bY, h->has_c ? h->cX : nullptr, h->has_c ? h->cY : nullptr, L0.a,
-                       L0.bX, L0.bY, L0.cX, L0.cY, h->nx, h->ny);
+    const bool  c  = h->has_c;
+    hipLaunchKernelGGL((k_convert_coeffs<T>), dim3(1024), dim3(256), 0, s, src_a, src_bX,
+                       src_bY, c ? src_cX : nullptr, c ? src_cY : nullptr, L0.a, L0.bX, L0.bY,
+                       L0.cX, L0.cY, h->nx, h->ny, mg.d_sc, mg.alpha, mg.beta, mg.gamma);
+    CoarsenList<T> small{};
     for (size_t l = 1; l < mg.lev.size(); ++l)
     {
         MGLevel<T> &F = mg.lev[l - 1], &C = mg.lev[l];
-        const bool  c = h->has_c;
+        if ((long long)C.nx * C.ny <= COARSEN_SMALL && small.n < COARSEN_MAX_LEVELS)
+        {
+            small.lev[small.n++] = { F.a,  F.bX, F.bY, c ? F.cX : nullptr, c ? F.cY : nullptr,
+                                     C.a,  C.bX, C.bY, C.cX, C.cY, C.nx, C.ny };
+            continue;
+        }
+        if (small.n) // levels coarsen in order: flush before a large one
+        {
+            hipLaunchKernelGGL((k_coarsen_small<T>), dim3(1), dim3(1024), 0, s, small);
+            small.n = 0;
+        }
         hipLaunchKernelGGL((k_coarsen_coeffs<T>), grid2(C.nx + 1, C.ny + 1), dim3(256), 0, s,
                            F.a, F.bX, F.bY, c ? F.cX : nullptr, c ? F.cY : nullptr, C.a, C.bX,
                            C.bY, C.cX, C.cY, C.nx, C.ny);
     }
+    if (small.n)
+        hipLaunchKernelGGL((k_coarsen_small<T>), dim3(1), dim3(1024), 0, s, small);
     H_CHECK(hipGetLastError());
     return BX_OK;
 }
@@ -1724,24 +1821,13 @@ extern "C" int bx_helm_set_coeffs(bx_helm *h, double alpha, double beta,
     h->beta  = beta;
     h->gamma = gamma;
     h->has_c = cX != nullptr && gamma != 0.0;
-    H_CHECK(hipMemcpyAsync(h->a, a, sizeof(double) * h->n, hipMemcpyDeviceToDevice, s));
-    H_CHECK(hipMemcpyAsync(h->bX, bXf, sizeof(double) * (h->nx + 1) * h->ny,
-                           hipMemcpyDeviceToDevice, s));
-    H_CHECK(hipMemcpyAsync(h->bY, bYf, sizeof(double) * h->nx * (h->ny + 1),
-                           hipMemcpyDeviceToDevice, s));
-    if (h->has_c)
-    {
-        int rc;
-        if (!h->cX && ((rc = halloc(&h->cX, h->n)) || (rc = halloc(&h->cY, h->n))))
-            return rc;
-        H_CHECK(hipMemcpyAsync(h->cX, cX, sizeof(double) * h->n,
-                               hipMemcpyDeviceToDevice, s));
-        H_CHECK(hipMemcpyAsync(h->cY, cY, sizeof(double) * h->n,
-                               hipMemcpyDeviceToDevice, s));
-    }
-    int  rc;
+    int rc;
+    if (h->has_c && !h->cX && ((rc = halloc(&h->cX, h->n)) || (rc = halloc(&h->cY, h->n))))
+        return rc;
+    // the FP64 hierarchy's level 0 is the solver's copy (h->a, ...): its
+    // level-0 launch copies the caller's arrays there
     bool realloc = false;
-    if ((rc = build_hierarchy(h, h->mg64, s, &realloc)))
+    if ((rc = build_hierarchy(h, h->mg64, s, &realloc, a, bXf, bYf, cX, cY)))
         return rc;
     if (realloc)
         ++h->gen;
@@ -1759,7 +1845,7 @@ static int ensure_mg32(bx_helm *h, hipStream_t s)
         return BX_OK;
     bool realloc = false;
     int  rc;
-    if ((rc = build_hierarchy(h, h->mg32, s, &realloc)))
+    if ((rc = build_hierarchy(h, h->mg32, s, &realloc, h->a, h->bX, h->bY, h->cX, h->cY)))
         return rc;
     if (realloc)
         ++h->gen;

@@ -250,6 +250,7 @@ extern "C" void bx_destroy(bx_ctx *c)
     destroy_graphs(c);
     free_dev(c->d_boxes);
     free_dev(c->d_ib_x);
+    free_dev(c->d_xbox);
     for (auto &p : c->d_ib_t)
         free_dev(p);
     free_dev(c->d_cbeg);
@@ -317,12 +318,22 @@ static void build_sweep_tables(bx_ctx *c)
         c->cbeg[b + 1] = c->cbeg[b] + nx * ny * nz;
     }
     c->ncells = c->cbeg[nb];
+    const long long nch = (c->ib_x[nb] + X_CHUNK - 1) / X_CHUNK;
+    c->xbox.assign(nch, 0);
+    for (long long ch = 0, b = 0; ch < nch; ++ch)
+    {
+        while (c->ib_x[b + 1] <= ch * X_CHUNK)
+            ++b;
+        c->xbox[ch] = (int)b;
+    }
 }
 
 static int upload_sweep_tables(bx_ctx *c)
 {
     int rc;
     if ((rc = upload(&c->d_ib_x, c->ib_x)) != BX_OK)
+        return rc;
+    if ((rc = upload(&c->d_xbox, c->xbox)) != BX_OK)
         return rc;
     for (int d = 1; d < c->D; ++d)
         if ((rc = upload(&c->d_ib_t[d], c->ib_t[d])) != BX_OK)
@@ -646,6 +657,7 @@ int do_sweep(bx_ctx *c, int dir, const double dx[3], const double *Uin,
     if (dir == 0)
     {
         L.item_begin = c->d_ib_x;
+        L.xbox       = c->d_xbox;
         L.nitems     = c->ib_x.back();
         L.block      = c->tune.block_x;
         L.grid       = persistent_grid(c, L.nitems, L.block, c->occ_x, false);

@@ -9,19 +9,19 @@
 //                                  Source/AmrLevelAdv.cpp:395-477, 537-619
 //
 // Each sweep is ONE persistent launch over a device-resident box list.  The
-// level's work is cut into items; every wave owns a contiguous range of
-// items (a range stays inside one XCD's L2 because consecutive ranges are
-// handed to blocks of the same XCD).
+// level's work is cut into items.
 //
 //  * x sweep (unit stride): an item is B consecutive positions of one box
 //    plane (B = the block size), flattened over (valid rows) x (row incl.
-//    ghosts); all waves of a block work on one item.  Thread t holds one
+//    ghosts); all waves of a block work on one item, and the blocks take
+//    chunks of X_CHUNK consecutive items in turn.  Thread t holds one
 //    position; the neighbour values the stencil needs (U at t-1 and t+1, the
 //    high-side half-stepped state at t-1, the face flux at t+1) pass through
 //    LDS, one buffer each, with a block barrier after each write.  B - 4 of
 //    B threads produce an update.
-//  * y/z sweeps (strided): an item is 64 (transverse cell, segment) pairs;
-//    lane t walks its column along the sweep direction through K cells,
+//  * y/z sweeps (strided): an item is 64 (transverse cell, segment) pairs,
+//    and every wave owns a contiguous range of items (consecutive ranges go
+//    to the waves of one XCD, so a range stays in one L2); lane t walks its column along the sweep direction through K cells,
 //    keeping the 3-cell window, the previous half-stepped state and the
 //    previous face flux in registers (K+2 reconstructions per K updates).
 // No temporaries leave registers: per cell the sweep reads U once and
@@ -82,24 +82,6 @@ __device__ __forceinline__ unsigned long long speed_max(unsigned long long acc,
                                                         double s)
 {
     return s == s ? umax64(acc, (unsigned long long)__double_as_longlong(s)) : acc;
-}
-
-// Contiguous item range of this block (all its waves share each item),
-// XCD-aware: consecutive ranges go to blocks of the same XCD (blocks are
-// dealt to the 8 XCDs round-robin).
-__device__ __forceinline__ void block_item_range(long long nitems,
-                                                 long long &first,
-                                                 long long &last)
-{
-    const int nblk = gridDim.x;
-    int       blk  = blockIdx.x;
-    if ((nblk & 7) == 0)
-        blk = (blk & 7) * (nblk >> 3) + (blk >> 3);
-    const long long per = (nitems + nblk - 1) / nblk;
-    first               = (long long)blk * per;
-    last                = first + per;
-    if (last > nitems)
-        last = nitems;
 }
 
 // Contiguous item range of this wave, XCD-aware: blocks are dealt to the 8
@@ -164,24 +146,27 @@ struct XItem
 // Walk over a block's consecutive x-sweep items.  The box geometry and the
 // item's (plane, chunk) position are carried from one item to the next, so
 // the divisions that locate an item run only when the walk enters a box
-// (the first item of the block's range, or a box boundary); within a box
-// the next item is the next chunk of the plane, or the first chunk of the
-// next plane.  Every value is uniform over the block.
+// (the first item of a chunk of items, with its box from a per-chunk
+// table, or the next box at a box boundary); within a box the next item is
+// the next chunk of the plane, or the first chunk of the next plane.  Every
+// value is uniform over the block.
 struct XCursor
 {
+    int       b;                      // current box
     long long bend;                   // first item past the current box
     long long base, kstride, jstride; // box geometry
     long long nst, P;
     int       nx, nxd, cpp, kpl, chunk;
 };
 
+// b: the box that holds item
 template <int D>
 __device__ __forceinline__ void x_enter(const DevBox *__restrict__ boxes,
-                                        const long long *__restrict__ ibeg, int nbox,
+                                        const long long *__restrict__ ibeg, int b,
                                         long long item, int ng, int step, XCursor &q)
 {
-    const int     b  = find_box(ibeg, nbox, item);
     const DevBox &bx = boxes[b];
+    q.b              = b;
     q.bend           = ibeg[b + 1];
     q.nx             = bx.hi[0] - bx.lo[0] + 1;
     const int ny     = (D >= 2) ? bx.hi[1] - bx.lo[1] + 1 : 1;
@@ -200,12 +185,12 @@ __device__ __forceinline__ void x_enter(const DevBox *__restrict__ boxes,
 // item: the item the cursor moves to (the one after its current item)
 template <int D>
 __device__ __forceinline__ void x_advance(const DevBox *__restrict__ boxes,
-                                          const long long *__restrict__ ibeg, int nbox,
+                                          const long long *__restrict__ ibeg,
                                           long long item, int ng, int step, XCursor &q)
 {
-    if (item >= q.bend)
+    if (item >= q.bend) // every box has at least one item
     {
-        x_enter<D>(boxes, ibeg, nbox, item, ng, step, q);
+        x_enter<D>(boxes, ibeg, q.b + 1, item, ng, step, q);
         return;
     }
     if (++q.chunk == q.cpp)
@@ -238,8 +223,9 @@ template <int D, int LIM, int LV, int SCH, int PM>
 __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     const double *__restrict__ Uin, double *__restrict__ Uout,
     const DevBox *__restrict__ boxes, const long long *__restrict__ ibeg,
-    int nbox, long long nitems, int ng, const double *__restrict__ d_dt,
-    double dx, EulerParm pm, LimTable lt, unsigned long long *d_maxbits)
+    const int *__restrict__ xbox, long long nitems, int ng,
+    const double *__restrict__ d_dt, double dx, EulerParm pm, LimTable lt,
+    unsigned long long *d_maxbits)
 {
     constexpr int  NC    = D + 2;
     constexpr bool E1    = (PM & PM_E1) != 0;
@@ -261,10 +247,22 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
     double *const  sH = x_lds + NC * XS;
     double *const  sF = x_lds + 2 * NC * XS;
 
-    long long first, last;
-    block_item_range(nitems, first, last);
+    // Chunks of X_CHUNK consecutive items are dealt to the blocks in turn,
+    // so every block (and every XCD) samples the whole level: items whose
+    // HLLC branches diverge (along shocks) or take the costlier star-state
+    // branch are spread over all blocks instead of lengthening the blocks
+    // that own one region.  Within a chunk the cursor advances without
+    // divisions.
+    const long long nblk = gridDim.x;
+    long long       item = (long long)blockIdx.x * X_CHUNK;
+    const long long end  = nitems;
+    auto next_of = [&](long long it) -> long long {
+        return ((it + 1) % X_CHUNK) ? it + 1 : it + 1 + (nblk - 1) * X_CHUNK;
+    };
+    long long          nxt_item = next_of(item);
     unsigned long long vmax = 0ULL;
     XCursor            q;
+    long long          qitem = -1; // the item the cursor is on
     XItem              cur, nxt;
     double             Un_[NC]; // prefetched state of the next item
     auto issue = [&](const XItem &it, double (&V)[NC]) {
@@ -273,25 +271,30 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
         for (int n = 0; n < NC; ++n)
             V[n] = Uin[it.pbase + pc + n * it.nst];
     };
-    if (first < last)
+    if (item < end)
     {
-        x_enter<D>(boxes, ibeg, nbox, first, ng, step, q);
+        x_enter<D>(boxes, ibeg, xbox[item / X_CHUNK], item, ng, step, q);
+        qitem = item;
         x_item<D>(q, ng, t, step, nxt);
         issue(nxt, Un_);
     }
     // Every thread of the block runs the same items, so the barriers below
     // are reached uniformly.  Each buffer is written before one barrier and
     // read after it; its next write comes after the following barrier.
-    for (long long item = first; item < last; ++item)
+    while (item < end)
     {
         cur = nxt;
         double U[NC];
 #pragma unroll
         for (int n = 0; n < NC; ++n)
             U[n] = Un_[n];
-        if (item + 1 < last)
+        if (nxt_item < end)
         {
-            x_advance<D>(boxes, ibeg, nbox, item + 1, ng, step, q);
+            if (nxt_item == qitem + 1)
+                x_advance<D>(boxes, ibeg, nxt_item, ng, step, q);
+            else // the first item of a chunk
+                x_enter<D>(boxes, ibeg, xbox[nxt_item / X_CHUNK], nxt_item, ng, step, q);
+            qitem = nxt_item;
             x_item<D>(q, ng, t, step, nxt);
             issue(nxt, Un_);
         }
@@ -352,6 +355,8 @@ __global__ __launch_bounds__(256) BX_X_OCC void k_sweep_x(
                 vmax = speed_max(vmax, sp);
             }
         }
+        item     = nxt_item;
+        nxt_item = next_of(nxt_item);
     }
     if (d_maxbits) // uniform over the block; every thread gets here
         block_max_to_global(vmax, d_maxbits);
@@ -686,7 +691,7 @@ static hipError_t launch_x(const SweepLaunch &L)
     hipLaunchKernelGGL((k_sweep_x<D, LIM, LV, SCH, PM>), dim3(L.grid),
                        dim3(L.block), x_lds_bytes(D + 2), L.stream,
                        L.Uin, L.Uout, L.boxes,
-                       L.item_begin, L.nbox, L.nitems, L.ng, L.d_dt, L.dx,
+                       L.item_begin, L.xbox, L.nitems, L.ng, L.d_dt, L.dx,
                        L.pm, L.lt, L.d_maxbits);
     return hipGetLastError();
 }

@@ -31,6 +31,8 @@ struct StepTail
 constexpr int X_LDS_STRIDE = 256;
 constexpr int x_item_step(int B) { return B - 4; }
 constexpr int x_lds_bytes(int NC) { return 3 * NC * X_LDS_STRIDE * 8; }
+// consecutive x-sweep items a block takes before the next block's turn
+constexpr int X_CHUNK = 8;
 
 struct SweepLaunch
 {
@@ -38,6 +40,7 @@ struct SweepLaunch
     double             *Uout;
     const DevBox       *boxes;
     const long long    *item_begin;
+    const int          *xbox = nullptr; // x sweep: box of each chunk's first item
     int                 nbox;
     long long           nitems;
     int                 ng;

@@ -54,6 +54,8 @@ struct bx_ctx
     int64_t              level_extent = 0; // doubles spanned by all FABs
 
     std::vector<long long> ib_x, ib_t[3], cbeg;
+    std::vector<int>       xbox; // box of the first item of each x-sweep chunk
+    int                   *d_xbox = nullptr;
     long long *d_ib_x = nullptr, *d_ib_t[3] = { nullptr, nullptr, nullptr },
               *d_cbeg = nullptr;
     long long  ncells = 0;

@@ -672,3 +672,46 @@ def test_blocked_smoother_vcycle_matches_per_colour_launches():
     st2 = s.solve(db, x2, "mg_fused", rtol=1e-10, maxiter=60)
     assert st1["iters"] == st2["iters"] and np.array_equal(x1.download(), x2.download())
     s.close()
+
+
+def test_gradient_term_solves_and_coefficient_refresh():
+    """With the gradient term (gamma, cX, cY): plain and blocked multigrid
+    give the same iterate, which meets the tolerance under the oracle's
+    operator.  A solver whose hierarchies were built for other coefficients
+    of the same sizes (first without the gradient term, then with it, then
+    the target values: re-allocated, then refreshed in place) gives the
+    bits of a fresh solver for every method, FGMRES's lazily built FP32
+    hierarchy included."""
+    announce("helm gradient term + coefficient refresh 256x128")
+    nx, ny, beta = 256, 128, 5e-3
+    rng = np.random.default_rng(21)
+    a, bX, bY = rand_coeffs(rng, nx, ny)
+    kw = dict(gamma=-0.02, cX=rng.normal(size=(ny, nx)), cY=rng.normal(size=(ny, nx)))
+    b = rng.normal(size=(ny, nx))
+    fresh = HelmholtzSolver(nx, ny, 1.0 / nx, 1.0 / ny)
+    fresh.set_coeffs(1.0, beta, a, bX, bY, **kw)
+    reused = HelmholtzSolver(nx, ny, 1.0 / nx, 1.0 / ny)
+    a2, bX2, bY2 = rand_coeffs(rng, nx, ny)
+    reused.set_coeffs(1.0, 2 * beta, a2, bX2, bY2)
+    db = dev(b)
+    warm = dev(np.zeros((ny, nx)))
+    reused.solve(db, warm, "fgmres_mg", rtol=1e-8, maxiter=50)  # builds and captures
+    reused.set_coeffs(1.0, 3 * beta, a2, bX2, bY2, gamma=0.01, cX=kw["cY"], cY=kw["cX"])
+    reused.solve(db, warm, "fgmres_mg", rtol=1e-8, maxiter=50)  # builds and captures
+    reused.set_coeffs(1.0, beta, a, bX, bY, **kw)
+    out = {}
+    for meth in ("mg", "mg_fused", "fgmres_mg"):
+        for name, s in (("fresh", fresh), ("reused", reused)):
+            x = dev(np.zeros((ny, nx)))
+            st = s.solve(db, x, meth, rtol=1e-10, maxiter=100)
+            assert st["converged"], (meth, name, st)
+            out[(meth, name)] = (st["iters"], x.download())
+        assert out[(meth, "fresh")][0] == out[(meth, "reused")][0], meth
+        assert np.array_equal(out[(meth, "fresh")][1], out[(meth, "reused")][1]), meth
+        x = out[(meth, "fresh")][1]
+        r = ho.apply(x, a, bX, bY, 1.0, beta, 1.0 / nx, 1.0 / ny, rhs=b, **kw)
+        assert np.linalg.norm(r) <= 1.0001e-10 * np.linalg.norm(b), meth
+    assert out[("mg", "fresh")][0] == out[("mg_fused", "fresh")][0]
+    assert np.array_equal(out[("mg", "fresh")][1], out[("mg_fused", "fresh")][1])
+    fresh.close()
+    reused.close()

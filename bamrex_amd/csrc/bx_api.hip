@@ -1146,7 +1146,11 @@ static int time_sweep(bx_ctx *c, int dir, const double dx[3], double *U,
     hipEvent_t e0, e1;
     BX_HIP_CHECK(hipEventCreate(&e0));
     BX_HIP_CHECK(hipEventCreate(&e1));
-    int rc = do_sweep(c, dir, dx, U, U_tmp, nullptr, s); // warm-up
+    // untimed launches first: a new shape follows a host synchronisation
+    // (table upload), and the GPU clock dips at the start of a busy period
+    int rc = BX_OK;
+    for (int r = 0; r < 8 && rc == BX_OK; ++r)
+        rc = do_sweep(c, dir, dx, U, U_tmp, nullptr, s);
     if (rc == BX_OK)
     {
         (void)hipEventRecord(e0, s);
@@ -1169,6 +1173,13 @@ static int autotune_search(bx_ctx *c, const double dx[3], double *U, double *U_t
 {
     int       rc;
     bx_tuning best = c->tune;
+    // untimed sweeps at the entry shape first: after an idle period the GPU
+    // clock dips over the first ~10 ms of back-to-back work, which would
+    // bias the first candidates
+    for (int r = 0; r < 16; ++r)
+        for (int d = 0; d < c->D; ++d)
+            if ((rc = do_sweep(c, d, dx, U, U_tmp, nullptr, s)) != BX_OK)
+                return rc;
     const int blocks[3] = { 64, 128, 256 };
     const int caps[4]   = { 0, 1, 2, 4 };
     // segment lengths: powers of two, and the median box length along the
